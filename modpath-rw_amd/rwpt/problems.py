@@ -1,0 +1,233 @@
+"""Synthetic MODPATH-RW problems: structured grids, face flows, particle clouds.
+
+The reference reads MODFLOW outputs (FlowModelData.f90:217-622); here the
+same arrays are synthesised so that the hot path can run without MODFLOW:
+
+* face flows follow the MODFLOW-2005 conventions that
+  ModpathCellData.f90:1315-1351 consumes: ``frf`` = flow from column j to
+  j+1, ``fff`` = flow from row i to i+1, ``flf`` = flow from layer k to k+1,
+  and ``bflow`` holds 6 boundary flows per cell (positive into the cell);
+* cell numbers are 1-based, (k-1)*nrow*ncol + (i-1)*ncol + j, arrays are
+  ordered the same way (layer slowest, column fastest);
+* the heterogeneous field is the discrete curl of a vector potential built
+  from seeded lognormal fields, so every cell balances to rounding.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Problem:
+    ncol: int
+    nrow: int
+    nlay: int
+    delr: np.ndarray
+    delc: np.ndarray
+    top: np.ndarray
+    bot: np.ndarray
+    frf: np.ndarray
+    fff: np.ndarray
+    flf: np.ndarray
+    porosity: np.ndarray
+    bflow: Optional[np.ndarray] = None
+    retardation: Optional[np.ndarray] = None
+    ibound: Optional[np.ndarray] = None
+    icbound: Optional[np.ndarray] = None
+    default_icbound: int = 0
+    disp_model: int = 0
+    aL: np.ndarray = dataclasses.field(default_factory=lambda: np.array([0.1]))
+    aT: np.ndarray = dataclasses.field(default_factory=lambda: np.array([0.01]))
+    aLV: Optional[np.ndarray] = None
+    aTV: Optional[np.ndarray] = None
+    dmeff: np.ndarray = dataclasses.field(default_factory=lambda: np.array([1e-4]))
+    advection_kind: int = 2      # 1 exponential, 2 eulerian
+    time_step_kind: int = 4      # 1 ADV, 2 DISP, 3 MIN_ADV_DISP, 4 FIXED
+    ts_params: tuple = (0.25, 0.0)
+    dim_mask: tuple = (1, 1, 1)
+    backward: int = 0
+
+    @property
+    def ncell(self) -> int:
+        return self.ncol * self.nrow * self.nlay
+
+    @property
+    def disp_uniform(self) -> int:
+        return int(self.aL.size == 1)
+
+    @property
+    def uniform_porosity(self) -> int:
+        """isUniformPorosity, ModpathBasicData.f90:208."""
+        return int(bool(np.all(self.porosity == self.porosity[0])))
+
+    def cell_number(self, k, i, j):
+        """1-based MODFLOW cell number from 1-based layer, row, column."""
+        return (np.asarray(k) - 1) * self.nrow * self.ncol + \
+            (np.asarray(i) - 1) * self.ncol + np.asarray(j)
+
+
+def _uniform_grid(ncol, nrow, nlay, d=1.0):
+    delr = np.full(ncol, d)
+    delc = np.full(nrow, d)
+    ncell = ncol * nrow * nlay
+    k = np.repeat(np.arange(nlay), nrow * ncol)
+    top = (nlay - k) * d
+    bot = top - d
+    return delr, delc, top.astype(np.float64), bot.astype(np.float64)
+
+
+def uniform_flow(ncol=100, nrow=50, nlay=1, qx=1.0, porosity=0.3, d=1.0):
+    """Uniform flow q_x on every x-face, including the two boundary faces."""
+    delr, delc, top, bot = _uniform_grid(ncol, nrow, nlay, d)
+    ncell = ncol * nrow * nlay
+    area = d * d
+    frf = np.zeros(ncell)
+    col = np.tile(np.arange(1, ncol + 1), nrow * nlay)
+    frf[col < ncol] = qx * area
+    fff = np.zeros(ncell)
+    flf = np.zeros(ncell)
+    bflow = np.zeros(6 * ncell)
+    first = np.nonzero(col == 1)[0]
+    last = np.nonzero(col == ncol)[0]
+    bflow[6 * first + 0] = qx * area       # into the cell through face 1
+    bflow[6 * last + 1] = -qx * area       # out of the cell through face 2
+    return Problem(ncol=ncol, nrow=nrow, nlay=nlay, delr=delr, delc=delc,
+                   top=top, bot=bot, frf=frf, fff=fff, flf=flf,
+                   porosity=np.full(ncell, porosity), bflow=bflow)
+
+
+def config0(**kw):
+    """100x50x1, q_x=1, phi=0.3, aL=0.1, aT=0.01, Dm=1e-4, 2-D, FIXED 0.25 d."""
+    P = uniform_flow(100, 50, 1, qx=1.0, porosity=0.3)
+    P.aL = np.array([0.1])
+    P.aT = np.array([0.01])
+    P.dmeff = np.array([1e-4])
+    P.dim_mask = (1, 1, 0)
+    P.ts_params = (0.25, 0.0)
+    P.time_step_kind = 4
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+def gaussian_field(shape, corr_len, sigma, rng):
+    """Stationary Gaussian field with exponential covariance
+    sigma^2 exp(-r/corr_len) by circulant embedding on a padded periodic grid
+    (FFT); the clipped negative eigenvalues make it approximate at the edges
+    of the padding only."""
+    pad = [int(2 ** np.ceil(np.log2(n + 3 * corr_len))) if n > 1 else 1
+           for n in shape]
+    grids = np.meshgrid(*[np.minimum(np.arange(p), p - np.arange(p)).astype(float)
+                          for p in pad], indexing="ij")
+    r = np.sqrt(sum(g * g for g in grids))
+    cov = sigma ** 2 * np.exp(-r / corr_len)
+    lam = np.real(np.fft.fftn(cov))
+    lam[lam < 0] = 0.0
+    noise = rng.standard_normal(pad) + 1j * rng.standard_normal(pad)
+    f = np.fft.ifftn(np.sqrt(lam) * noise) * np.sqrt(np.prod(pad))
+    f = np.real(f)[tuple(slice(0, n) for n in shape)]
+    return f
+
+
+def heterogeneous_flow(ncol=200, nrow=200, nlay=20, qmean=0.5, sigma_lnk=1.0,
+                       corr_len=10.0, seed=20240601, pert=0.5,
+                       porosity_range=(0.20, 0.30), d=1.0):
+    """Divergence-free face flows = mean q_x plus the discrete curl of a
+    seeded vector potential A whose three components are lognormal fields
+    K_c/mean(K_c) - 1 (sigma_lnK, exponential covariance, corr_len cells).
+    Tangential A is zero on the domain boundary, so only the mean flow
+    crosses the two x-boundaries.  The curl is rescaled so that the standard
+    deviation of the perturbation x-flux equals ``pert * qmean``."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = ncol, nrow, nlay
+    # edge-centred potential components (local axes: x=column, y=-row, z=up)
+    def lognormal(shape):
+        y = gaussian_field(shape, corr_len, sigma_lnk, rng)
+        k = np.exp(y)
+        return k / k.mean() - 1.0
+    Ax = lognormal((nx, ny + 1, nz + 1))
+    Ay = lognormal((nx + 1, ny, nz + 1))
+    Az = lognormal((nx + 1, ny + 1, nz))
+    Ax[:, 0, :] = Ax[:, -1, :] = 0.0
+    Ax[:, :, 0] = Ax[:, :, -1] = 0.0
+    Ay[0, :, :] = Ay[-1, :, :] = 0.0
+    Ay[:, :, 0] = Ay[:, :, -1] = 0.0
+    Az[0, :, :] = Az[-1, :, :] = 0.0
+    Az[:, 0, :] = Az[:, -1, :] = 0.0
+    # face fluxes (volumetric, d = 1 m cells) of curl A
+    Qx = (Az[:, 1:, :] - Az[:, :-1, :]) * d - (Ay[:, :, 1:] - Ay[:, :, :-1]) * d
+    Qy = (Ax[:, :, 1:] - Ax[:, :, :-1]) * d - (Az[1:, :, :] - Az[:-1, :, :]) * d
+    Qz = (Ay[1:, :, :] - Ay[:-1, :, :]) * d - (Ax[:, 1:, :] - Ax[:, :-1, :]) * d
+    s = Qx[1:-1].std()
+    scale = pert * qmean * d * d / s if s > 0 else 0.0
+    Qx *= scale
+    Qy *= scale
+    Qz *= scale
+    Qx += qmean * d * d
+    delr, delc, top, bot = _uniform_grid(ncol, nrow, nlay, d)
+    ncell = ncol * nrow * nlay
+    # MODFLOW arrays in (layer, row, column) order
+    frf = np.zeros((nlay, nrow, ncol))
+    fff = np.zeros((nlay, nrow, ncol))
+    flf = np.zeros((nlay, nrow, ncol))
+    # local y index iy = nrow-1-i0, z index iz = nlay-1-k0 (0-based row/layer)
+    qx_cell_right = Qx[1:, :, :]                   # (nx, ny, nz) right faces
+    frf[:] = np.transpose(qx_cell_right[:, ::-1, ::-1], (2, 1, 0))
+    frf[:, :, -1] = 0.0
+    # front face of row i0 is the lower-y face of local iy: node iy
+    qy_front = Qy[:, :-1, :]                        # lower-y face of each cell
+    fff[:] = -np.transpose(qy_front[:, ::-1, ::-1], (2, 1, 0))
+    fff[:, -1, :] = 0.0
+    qz_bottom = Qz[:, :, :-1]                       # lower-z face of each cell
+    flf[:] = -np.transpose(qz_bottom[:, ::-1, ::-1], (2, 1, 0))
+    flf[-1, :, :] = 0.0
+    bflow = np.zeros((nlay, nrow, ncol, 6))
+    bflow[:, :, 0, 0] = qmean * d * d
+    bflow[:, :, -1, 1] = -qmean * d * d
+    lo, hi = porosity_range
+    phi = rng.uniform(lo, hi, size=ncell)
+    return Problem(ncol=ncol, nrow=nrow, nlay=nlay, delr=delr, delc=delc,
+                   top=top, bot=bot, frf=frf.ravel(), fff=fff.ravel(),
+                   flf=flf.ravel(), porosity=phi, bflow=bflow.ravel())
+
+
+def config1(ncol=200, nrow=200, nlay=20, seed=20240601, **kw):
+    """Heterogeneous 200x200x20 field, aL=0.5, aT=0.05, Dm=1e-4, FIXED 0.5 d."""
+    P = heterogeneous_flow(ncol, nrow, nlay, seed=seed)
+    P.aL = np.array([0.5])
+    P.aT = np.array([0.05])
+    P.dmeff = np.array([1e-4])
+    P.dim_mask = (1, 1, 1)
+    P.ts_params = (0.5, 0.0)
+    P.time_step_kind = 4
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+def release_in_cell(P: Problem, n, k, i, j, seed=1, lz=None):
+    """n particles uniformly inside cell (k, i, j) (1-based)."""
+    rng = np.random.default_rng(seed)
+    cell = np.full(n, int(P.cell_number(k, i, j)), dtype=np.int32)
+    lx = rng.uniform(0.0, 1.0, n)
+    ly = rng.uniform(0.0, 1.0, n)
+    lzv = rng.uniform(0.0, 1.0, n) if lz is None else np.full(n, float(lz))
+    return cell, lx, ly, lzv
+
+
+def release_in_block(P: Problem, n, cols, rows, lays, seed=1):
+    """n particles uniformly inside the block of cells with 1-based inclusive
+    column, row and layer ranges (equal-size cells)."""
+    rng = np.random.default_rng(seed)
+    j = rng.integers(cols[0], cols[1] + 1, n)
+    i = rng.integers(rows[0], rows[1] + 1, n)
+    k = rng.integers(lays[0], lays[1] + 1, n)
+    cell = P.cell_number(k, i, j).astype(np.int32)
+    return cell, rng.uniform(0, 1, n), rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+
+
+BASE_SEED = (0x1DADBEEFBAADD0D0, 0x5BADD0D0DEADBEEF)
+"""baseseed of pr_InitializeRandomWalk, TrackSubCell.F90:553-555."""

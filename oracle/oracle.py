@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and bench.py's cpu_baseline leg import this.
+The oracle restates the reference algorithm on the CPU; see rwpt_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("ncol", C.c_int), ("nrow", C.c_int), ("nlay", C.c_int),
+        ("delr", _dp), ("delc", _dp), ("top", _dp), ("bot", _dp),
+        ("ibound", _ip), ("frf", _dp), ("fff", _dp), ("flf", _dp),
+        ("bflow", _dp), ("porosity", _dp), ("retardation", _dp),
+        ("head", _dp), ("layer_type", _ip), ("source_flow", _dp),
+        ("sink_flow", _dp), ("zone", _ip), ("icbound", _ip),
+        ("default_icbound", C.c_int), ("disp_model", C.c_int),
+        ("disp_uniform", C.c_int),
+        ("aL", _dp), ("aT", _dp), ("aLV", _dp), ("aTV", _dp), ("dmeff", _dp),
+        ("advection_kind", C.c_int), ("time_step_kind", C.c_int),
+        ("ts_params", C.c_double * 2), ("dim_mask", C.c_int * 3),
+        ("uniform_porosity", C.c_int), ("backward", C.c_int),
+        ("stop_weak_sinks", C.c_int), ("stop_weak_sources", C.c_int),
+        ("stop_zone", C.c_int), ("steady_state", C.c_int),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run build())")
+        L = C.CDLL(path)
+        L.orc_advance.argtypes = [C.POINTER(OrcProblem), C.c_long, _ip, _dp, _dp,
+                                  _dp, _dp, _ip, _u64p, C.c_double, C.c_double,
+                                  C.POINTER(C.c_longlong), C.c_int]
+        L.orc_advance.restype = C.c_int
+        L.orc_cell_tables.argtypes = [C.POINTER(OrcProblem), _dp]
+        L.orc_rng_streams.argtypes = [_u64p, C.c_long, C.c_long, _u64p]
+        L.orc_xoroshiro_next.argtypes = [_u64p]
+        L.orc_xoroshiro_next.restype = C.c_uint64
+        L.orc_uniform.argtypes = [_u64p]
+        L.orc_uniform.restype = C.c_double
+        L.orc_normal12.argtypes = [_u64p]
+        L.orc_normal12.restype = C.c_double
+        L.orc_rng_jump.argtypes = [_u64p]
+        L.orc_global_xyz.argtypes = [C.POINTER(OrcProblem), C.c_long, _ip, _dp,
+                                     _dp, _dp, _dp, _dp, _dp]
+        L.orc_histogram.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    _ip, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class OracleProblem:
+    """Keeps the numpy arrays alive while the C struct points at them."""
+
+    def __init__(self, P):
+        self._keep = {}
+        s = OrcProblem()
+        s.ncol, s.nrow, s.nlay = P.ncol, P.nrow, P.nlay
+
+        def dd(name, arr):
+            a = _d(arr)
+            self._keep[name] = a
+            return _ptr(a, _dp)
+
+        def ii(name, arr):
+            a = _i(arr)
+            self._keep[name] = a
+            return _ptr(a, _ip)
+
+        s.delr = dd("delr", P.delr)
+        s.delc = dd("delc", P.delc)
+        s.top = dd("top", P.top)
+        s.bot = dd("bot", P.bot)
+        s.ibound = ii("ibound", P.ibound)
+        s.frf = dd("frf", P.frf)
+        s.fff = dd("fff", P.fff)
+        s.flf = dd("flf", P.flf)
+        s.bflow = dd("bflow", P.bflow)
+        s.porosity = dd("porosity", P.porosity)
+        s.retardation = dd("retardation", P.retardation)
+        s.head = None
+        s.layer_type = None
+        s.source_flow = None
+        s.sink_flow = None
+        s.zone = None
+        s.icbound = ii("icbound", P.icbound)
+        s.default_icbound = P.default_icbound
+        s.disp_model = P.disp_model
+        s.disp_uniform = P.disp_uniform
+        s.aL = dd("aL", P.aL)
+        s.aT = dd("aT", P.aT)
+        s.aLV = dd("aLV", P.aLV)
+        s.aTV = dd("aTV", P.aTV)
+        s.dmeff = dd("dmeff", P.dmeff)
+        s.advection_kind = P.advection_kind
+        s.time_step_kind = P.time_step_kind
+        s.ts_params[0], s.ts_params[1] = P.ts_params
+        for d in range(3):
+            s.dim_mask[d] = P.dim_mask[d]
+        s.uniform_porosity = P.uniform_porosity
+        s.backward = P.backward
+        s.stop_weak_sinks = 0
+        s.stop_weak_sources = 0
+        s.stop_zone = 0
+        s.steady_state = 1
+        self.s = s
+
+
+def rng_streams(n, first=0, base=None):
+    from_base = np.array(base if base is not None else
+                         (0x1DADBEEFBAADD0D0, 0x5BADD0D0DEADBEEF), dtype=np.uint64)
+    out = np.zeros(2 * n, dtype=np.uint64)
+    lib().orc_rng_streams(from_base.ctypes.data_as(_u64p), first, n,
+                          out.ctypes.data_as(_u64p))
+    return out
+
+
+def advance(P, state, t_engine, t_stop, nthreads=0):
+    """state: dict with cell, lx, ly, lz, t, status, rng, substeps (modified
+    in place).  Returns the oracle's return code."""
+    OP = P if isinstance(P, OracleProblem) else OracleProblem(P)
+    n = state["cell"].size
+    rc = lib().orc_advance(C.byref(OP.s), n,
+                           _ptr(state["cell"], _ip), _ptr(state["lx"], _dp),
+                           _ptr(state["ly"], _dp), _ptr(state["lz"], _dp),
+                           _ptr(state["t"], _dp), _ptr(state["status"], _ip),
+                           _ptr(state["rng"], _u64p), t_engine, t_stop,
+                           state["substeps"].ctypes.data_as(C.POINTER(C.c_longlong)),
+                           nthreads)
+    return rc
+
+
+def cell_tables(P):
+    OP = P if isinstance(P, OracleProblem) else OracleProblem(P)
+    out = np.zeros((P.ncell if not isinstance(P, OracleProblem) else
+                    OP.s.ncol * OP.s.nrow * OP.s.nlay, 104))
+    lib().orc_cell_tables(C.byref(OP.s), out.ctypes.data_as(_dp))
+    return out
+
+
+def global_xyz(P, cell, lx, ly, lz):
+    OP = P if isinstance(P, OracleProblem) else OracleProblem(P)
+    n = cell.size
+    gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_global_xyz(C.byref(OP.s), n, _ptr(_i(cell), _ip), _ptr(_d(lx), _dp),
+                         _ptr(_d(ly), _dp), _ptr(_d(lz), _dp), _ptr(gx, _dp),
+                         _ptr(gy, _dp), _ptr(gz, _dp))
+    return gx, gy, gz
+
+
+def histogram(x, y, z, origin, binsize, nbins, w=None):
+    n = x.size
+    counts = np.zeros(int(np.prod(nbins)))
+    o = _d(origin)
+    b = _d(binsize)
+    nb = _i(nbins)
+    ww = _d(w)
+    lib().orc_histogram(n, _ptr(_d(x), _dp), _ptr(_d(y), _dp), _ptr(_d(z), _dp),
+                        _ptr(ww, _dp), _ptr(o, _dp), _ptr(b, _dp), _ptr(nb, _ip),
+                        _ptr(counts, _dp))
+    return counts
+
+
+def new_state(cell, lx, ly, lz, t0=0.0, rng=None):
+    n = cell.size
+    return {
+        "cell": _i(cell).copy(), "lx": _d(lx).copy(), "ly": _d(ly).copy(),
+        "lz": _d(lz).copy(), "t": np.full(n, t0), "status": np.ones(n, np.int32),
+        "rng": rng_streams(n) if rng is None else rng.copy(),
+        "substeps": np.zeros(n, np.int64),
+    }
