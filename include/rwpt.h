@@ -1,0 +1,169 @@
+/*
+ * rwpt.h -- C ABI of the MI355X random-walk particle-tracking engine.
+ *
+ * Drop-in boundary for MODPATH-RW's hot path: the per-particle
+ * TrackPath/TrackCell/TrackSubCell random walk and the gpkde histogram and
+ * density reconstruction.  All arguments are plain pointers and sizes
+ * (host memory unless a name says "device"); no C++ or torch types cross
+ * this boundary.  Every entry point returns 0 on success or a negative
+ * RWPT_E* code; rwpt_last_error() then holds a message.  The engine never
+ * exits the process (the reference calls ustop, UtilMisc.f90:33-49).
+ *
+ * Conventions (the reference's):
+ *  - cells are 1-based MODFLOW numbers (k-1)*nrow*ncol + (i-1)*ncol + j and
+ *    per-cell arrays are ordered the same way (layer slowest, column fastest);
+ *  - face flows use the MODFLOW-2005 signs ModpathCellData.f90:1315-1351
+ *    consumes: frf = flow from column j to j+1, fff = row i to i+1,
+ *    flf = layer k to k+1; bflow holds 6 boundary flows per cell;
+ *  - local coordinates are in [0,1]; faces 1..6 = x-, x+, y-(front),
+ *    y+(back), z-(bottom), z+(top);
+ *  - particle status codes are those of MPathRW.F90:1256-1403:
+ *    0 pending, 1 active, 2 boundary face, 3 weak sink, 4 weak source,
+ *    5 no exit possible, 6 stop zone, 7 inactive cell, 8 unreleased,
+ *    9 undefined.
+ *
+ * Threading: one host thread per engine; calls on one engine do not
+ * overlap.  State stays resident in HBM between calls.
+ */
+#ifndef RWPT_H
+#define RWPT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RWPT_OK 0
+#define RWPT_E_ARG (-1)      /* invalid argument */
+#define RWPT_E_STATE (-2)    /* call out of order (e.g. advance before flow) */
+#define RWPT_E_DEVICE (-3)   /* HIP runtime error */
+#define RWPT_E_NOMEM (-4)    /* device allocation failed */
+#define RWPT_E_UNSUPPORTED (-5)
+
+typedef struct rwpt_engine rwpt_engine;
+
+/* Structured grid (RectangularGridDis.f90 DIS arrays; the origin is the
+ * left/front edge at x = 0, y = 0 as in RectangularGridDis.f90:295-361). */
+typedef struct {
+  int32_t ncol, nrow, nlay;
+  const double *delr; /* ncol column widths (DX)                        */
+  const double *delc; /* nrow row widths (DY)                           */
+  const double *top;  /* ncell cell top elevations                      */
+  const double *bot;  /* ncell cell bottom elevations                   */
+} rwpt_grid;
+
+/* Tracking options: the random-walk fields of ParticleTrackingOptionsType
+ * (ParticleTrackingOptions.f90:20-31) read from RWOPTS
+ * (ModpathSimulationData.f90:1770-2021). */
+typedef struct {
+  int32_t advection_kind;     /* 1 EXPONENTIAL, 2 EULERIAN                */
+  int32_t time_step_kind;     /* 1 ADV, 2 DISP, 3 MIN_ADV_DISP, 4 FIXED   */
+  double time_step_params[2]; /* timeStepParameters(1:2)                  */
+  int32_t dimension_mask[3];  /* dimensionMask, 0 or 1 each              */
+  int32_t backward_tracking;  /* BackwardTracking                        */
+  int32_t stop_zone;          /* StopZone (0: none)                      */
+  int32_t stop_at_weak_sinks;
+  int32_t stop_at_weak_sources;
+  int32_t steady_state;       /* FlowModelData%SteadyState               */
+  int32_t precision;          /* 0: plain FP64; 1: FP32 cell-local       */
+  uint64_t base_seed[2];      /* baseseed, TrackSubCell.F90:553-555      */
+} rwpt_options;
+
+/* Options of the gpkde reconstruction (ParticleTrackingOptions.f90:33-63,
+ * ReadGPKDEData ModpathSimulationData.f90:816-1273). */
+typedef struct {
+  double origin[3];      /* gpkdeDomainOrigin                            */
+  double bin_size[3];    /* gpkdeBinSize                                 */
+  int32_t nbins[3];      /* domain size / bin size                       */
+  int32_t mode;          /* 0 histogram only; 1 one Gaussian density pass
+                            with the Silverman bandwidth (uniform h)     */
+  int32_t weighted;      /* 1: particles carry mass                     */
+  double scaling_factor; /* multiplies the density (gpkdeScalingFactor)  */
+} rwpt_kde_options;
+
+/* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,
+ * steady, FP64, baseseed). */
+void rwpt_default_options(rwpt_options *opt);
+
+/* ParticleTrackingEngine%Initialize, ParticleTrackingEngine.f90:393-492 */
+int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
+                rwpt_engine **out);
+int rwpt_destroy(rwpt_engine *e);
+const char *rwpt_last_error(const rwpt_engine *e);
+
+/* Flow and cell properties (FlowModelData, ModpathBasicData, IMP file):
+ * frf/fff/flf ncell each; bflow 6*ncell or NULL; retardation, ibound,
+ * icbound may be NULL (1.0, all active, 0).  default_icbound is the mass
+ * boundary of faces without a neighbour (TransportModelData.f90:194-204). */
+int rwpt_set_flow(rwpt_engine *e, const double *frf, const double *fff,
+                  const double *flf, const double *bflow,
+                  const double *porosity, const double *retardation,
+                  const int32_t *ibound, const int32_t *icbound,
+                  int32_t default_icbound);
+
+/* Optional per-cell data for convertible layers, zones and weak sinks
+ * (head/layer_type drive GetDZRW, ModpathCellData.f90:3208-3256). */
+int rwpt_set_cell_extras(rwpt_engine *e, const double *head,
+                         const int32_t *layer_type, const int32_t *zone,
+                         const double *source_flow, const double *sink_flow);
+
+/* Dispersion (DSP file, TransportModelData.f90:583-1040): model 0 linear
+ * isotropic, 1 axisymmetric.  uniform=1 => each array holds one value,
+ * else ncell values.  aLV/aTV may be NULL for model 0. */
+int rwpt_set_dispersion(rwpt_engine *e, int32_t model, int32_t uniform,
+                        const double *aL, const double *aT, const double *aLV,
+                        const double *aTV, const double *dmeff);
+
+/* Particles as structure of arrays.  rng holds 2 words per particle (the
+ * xoroshiro128+ state); NULL derives stream p = jump^(stream_offset+p) of
+ * base_seed, the stream rng_zigset gives OpenMP thread p
+ * (rng_par_zig.f90:183-191). */
+int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
+                        const double *lx, const double *ly, const double *lz,
+                        const double *t, const int32_t *status,
+                        const uint64_t *rng, int64_t stream_offset);
+int rwpt_get_particles(rwpt_engine *e, int32_t *cell, double *lx, double *ly,
+                       double *lz, double *t, int32_t *status, uint64_t *rng);
+int64_t rwpt_particle_count(const rwpt_engine *e);
+
+/* Advances every active particle to t_stop (one TRACKING_INTERVAL_LOOP
+ * pass, MPathRW.F90:1119-1725, with pr_RWPTrackPath per particle).
+ * Pending particles are released first.  Outputs (optional): particles
+ * active after the pass and RW sub-steps performed. */
+int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
+                 int64_t *n_substeps);
+
+/* Sets the engine time (the start of the next tracking interval, the
+ * `time` of MPathRW.F90:1256-1262 used for the release test). */
+int rwpt_set_time(rwpt_engine *e, double t);
+
+/* Cumulative RW sub-steps per particle since rwpt_load_particles. */
+int rwpt_get_substeps(rwpt_engine *e, int64_t *substeps);
+
+/* Global coordinates of the particles (ConvertToModelXYZ,
+ * ModflowRectangularGrid.f90:524-549). */
+int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz);
+
+/* Concentration reconstruction over the active particles
+ * (GridProjectedKDE%ComputeDensity, GridProjectedKDE.F90:3598-4392).
+ * density and histogram are nbins[0]*nbins[1]*nbins[2] doubles, x fastest
+ * (either may be NULL); mass may be NULL (unit masses). */
+int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
+             double *density, double *histogram);
+
+/* Per-cell tables as built on the device (checking aid), ncell * 104
+ * doubles in the layout documented in DESIGN.md. */
+int rwpt_get_cell_tables(rwpt_engine *e, double *out);
+
+/* Device timing of the last rwpt_advance/rwpt_kde calls, measured with HIP
+ * events on the engine stream: which = 0 advance kernel, 1 histogram,
+ * 2 density.  Returns the summed milliseconds and the launch count. */
+int rwpt_kernel_time(rwpt_engine *e, int32_t which, double *ms,
+                     int64_t *launches);
+int rwpt_reset_kernel_time(rwpt_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

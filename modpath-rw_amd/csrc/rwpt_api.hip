@@ -1,0 +1,439 @@
+// C ABI of librwpt (include/rwpt.h): engine lifetime, host<->device copies,
+// launches and error reporting.  Inputs are validated on the host so that
+// every kernel only sees indices inside its buffers.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "rwpt_internal.h"
+
+using namespace rwpt;
+
+namespace {
+
+int fail(rwpt_engine *e, int code, const char *msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+int hip_fail(rwpt_engine *e, hipError_t err, const char *where) {
+  if (e) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(err));
+    e->err = buf;
+  }
+  return err == hipErrorOutOfMemory ? RWPT_E_NOMEM : RWPT_E_DEVICE;
+}
+
+template <class T>
+hipError_t dev_alloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <class T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <class T>
+hipError_t upload(rwpt_engine *e, T **dst, const T *src, size_t count) {
+  dev_free(*dst);
+  if (!src) return hipSuccess;
+  hipError_t err = dev_alloc(dst, count);
+  if (err != hipSuccess) return err;
+  return hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice,
+                        e->stream);
+}
+
+void free_particles(Particles &P) {
+  dev_free(P.cell);
+  dev_free(P.status);
+  dev_free(P.lx);
+  dev_free(P.ly);
+  dev_free(P.lz);
+  dev_free(P.t);
+  dev_free(P.rng);
+  dev_free(P.substeps);
+  P.n = P.cap = 0;
+}
+
+int ensure_tables(rwpt_engine *e) {
+  if (!e->have_flow) return fail(e, RWPT_E_STATE, "flow not set");
+  if (!e->have_disp) return fail(e, RWPT_E_STATE, "dispersion not set");
+  if (!e->tables_dirty) return RWPT_OK;
+  hipError_t err = build_cell_tables(e, nullptr);
+  if (err != hipSuccess) return hip_fail(e, err, "cell tables");
+  e->tables_dirty = false;
+  return RWPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rwpt_default_options(rwpt_options *opt) {
+  if (!opt) return;
+  memset(opt, 0, sizeof *opt);
+  opt->advection_kind = 2;
+  opt->time_step_kind = 4;
+  opt->time_step_params[0] = 1.0;
+  opt->time_step_params[1] = 0.0;
+  opt->dimension_mask[0] = opt->dimension_mask[1] = opt->dimension_mask[2] = 1;
+  opt->steady_state = 1;
+  opt->precision = 0;
+  opt->base_seed[0] = 0x1DADBEEFBAADD0D0ull;
+  opt->base_seed[1] = 0x5BADD0D0DEADBEEFull;
+}
+
+int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
+                rwpt_engine **out) {
+  if (!out) return RWPT_E_ARG;
+  *out = nullptr;
+  if (!grid || !opt) return RWPT_E_ARG;
+  if (grid->ncol <= 0 || grid->nrow <= 0 || grid->nlay <= 0 || !grid->delr ||
+      !grid->delc || !grid->top || !grid->bot)
+    return RWPT_E_ARG;
+  const int64_t ncell = (int64_t)grid->ncol * grid->nrow * grid->nlay;
+  if (ncell > 0x7fffffffLL) return RWPT_E_ARG;
+  if (opt->advection_kind != 1 && opt->advection_kind != 2) return RWPT_E_ARG;
+  if (opt->time_step_kind < 1 || opt->time_step_kind > 4) return RWPT_E_ARG;
+  int nd = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (opt->dimension_mask[d] != 0 && opt->dimension_mask[d] != 1)
+      return RWPT_E_ARG;
+    nd += opt->dimension_mask[d];
+  }
+  if (nd == 0) return RWPT_E_ARG;
+  if (opt->precision != 0 && opt->precision != 1) return RWPT_E_ARG;
+  for (int j = 0; j < grid->ncol; ++j)
+    if (!(grid->delr[j] > 0.0)) return RWPT_E_ARG;
+  for (int i = 0; i < grid->nrow; ++i)
+    if (!(grid->delc[i] > 0.0)) return RWPT_E_ARG;
+
+  rwpt_engine *e = new rwpt_engine();
+  e->opt = *opt;
+  hipError_t err = hipGetDevice(&e->device);
+  if (err == hipSuccess) err = hipStreamCreate(&e->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(e, err, "create");
+    fprintf(stderr, "rwpt_create: %s\n", e->err.c_str());
+    delete e;
+    return rc;
+  }
+  DeviceProblem &P = e->prob;
+  P.ncol = grid->ncol;
+  P.nrow = grid->nrow;
+  P.nlay = grid->nlay;
+  P.ncell = ncell;
+  err = upload(e, &P.delr, grid->delr, P.ncol);
+  if (err == hipSuccess) err = upload(e, &P.delc, grid->delc, P.nrow);
+  if (err == hipSuccess) err = upload(e, &P.top, grid->top, ncell);
+  if (err == hipSuccess) err = upload(e, &P.bot, grid->bot, ncell);
+  if (err == hipSuccess) err = dev_alloc(&P.q6, 6 * ncell);
+  if (err == hipSuccess) err = dev_alloc(&P.dzrw, ncell);
+  if (err == hipSuccess) err = dev_alloc(&P.cells, ncell * CELL_DOUBLES);
+  if (err == hipSuccess) err = dev_alloc(&e->counters, 4);
+  std::vector<uint64_t> mats;
+  jump_power_matrices(64, mats);
+  if (err == hipSuccess) err = upload(e, &e->jump_mats, mats.data(), mats.size());
+  for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+    err = hipEventCreate(&e->timers[k].start);
+    if (err == hipSuccess) err = hipEventCreate(&e->timers[k].stop);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(e, err, "create");
+    rwpt_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return RWPT_OK;
+}
+
+int rwpt_destroy(rwpt_engine *e) {
+  if (!e) return RWPT_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  DeviceProblem &P = e->prob;
+  dev_free(P.delr); dev_free(P.delc); dev_free(P.top); dev_free(P.bot);
+  dev_free(P.frf); dev_free(P.fff); dev_free(P.flf); dev_free(P.bflow);
+  dev_free(P.porosity); dev_free(P.retardation); dev_free(P.ibound);
+  dev_free(P.icbound); dev_free(P.head); dev_free(P.layer_type);
+  dev_free(P.zone); dev_free(P.source); dev_free(P.sink); dev_free(P.disp);
+  dev_free(P.q6); dev_free(P.dzrw); dev_free(P.cells); dev_free(P.cells32);
+  free_particles(e->part);
+  dev_free(e->jump_mats);
+  dev_free(e->counters);
+  dev_free(e->kde_scratch);
+  for (auto &T : e->timers) {
+    if (T.start) (void)hipEventDestroy(T.start);
+    if (T.stop) (void)hipEventDestroy(T.stop);
+  }
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return RWPT_OK;
+}
+
+const char *rwpt_last_error(const rwpt_engine *e) {
+  return e ? e->err.c_str() : "null engine";
+}
+
+int rwpt_set_flow(rwpt_engine *e, const double *frf, const double *fff,
+                  const double *flf, const double *bflow,
+                  const double *porosity, const double *retardation,
+                  const int32_t *ibound, const int32_t *icbound,
+                  int32_t default_icbound) {
+  if (!e) return RWPT_E_ARG;
+  if (!frf || !fff || !flf || !porosity)
+    return fail(e, RWPT_E_ARG, "frf, fff, flf and porosity are required");
+  DeviceProblem &P = e->prob;
+  const int64_t n = P.ncell;
+  for (int64_t c = 0; c < n; ++c) {
+    if (!(porosity[c] > 0.0)) return fail(e, RWPT_E_ARG, "porosity must be > 0");
+    if (retardation && !(retardation[c] > 0.0))
+      return fail(e, RWPT_E_ARG, "retardation must be > 0");
+  }
+  hipError_t err = upload(e, &P.frf, frf, n);
+  if (err == hipSuccess) err = upload(e, &P.fff, fff, n);
+  if (err == hipSuccess) err = upload(e, &P.flf, flf, n);
+  if (err == hipSuccess) err = upload(e, &P.bflow, bflow, 6 * n);
+  if (err == hipSuccess) err = upload(e, &P.porosity, porosity, n);
+  if (err == hipSuccess) err = upload(e, &P.retardation, retardation, n);
+  if (err == hipSuccess) err = upload(e, &P.ibound, ibound, n);
+  if (err == hipSuccess) err = upload(e, &P.icbound, icbound, n);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "set_flow");
+  P.default_icbound = default_icbound;
+  // isUniformPorosity (ModpathBasicData.f90:208)
+  P.uniform_porosity = 1;
+  for (int64_t c = 1; c < n; ++c)
+    if (porosity[c] != porosity[0]) {
+      P.uniform_porosity = 0;
+      break;
+    }
+  e->have_flow = true;
+  e->tables_dirty = true;
+  return RWPT_OK;
+}
+
+int rwpt_set_cell_extras(rwpt_engine *e, const double *head,
+                         const int32_t *layer_type, const int32_t *zone,
+                         const double *source_flow, const double *sink_flow) {
+  if (!e) return RWPT_E_ARG;
+  DeviceProblem &P = e->prob;
+  const int64_t n = P.ncell;
+  hipError_t err = upload(e, &P.head, head, n);
+  if (err == hipSuccess) err = upload(e, &P.layer_type, layer_type, n);
+  if (err == hipSuccess) err = upload(e, &P.zone, zone, n);
+  if (err == hipSuccess) err = upload(e, &P.source, source_flow, n);
+  if (err == hipSuccess) err = upload(e, &P.sink, sink_flow, n);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "set_cell_extras");
+  e->tables_dirty = true;
+  return RWPT_OK;
+}
+
+int rwpt_set_dispersion(rwpt_engine *e, int32_t model, int32_t uniform,
+                        const double *aL, const double *aT, const double *aLV,
+                        const double *aTV, const double *dmeff) {
+  if (!e) return RWPT_E_ARG;
+  if (model != 0 && model != 1)
+    return fail(e, RWPT_E_UNSUPPORTED, "dispersion model must be 0 or 1");
+  if (!aL || !aT || !dmeff) return fail(e, RWPT_E_ARG, "aL, aT, dmeff required");
+  if (model == 1 && (!aLV || !aTV))
+    return fail(e, RWPT_E_ARG, "axisymmetric model needs aLV and aTV");
+  DeviceProblem &P = e->prob;
+  const int64_t nd = uniform ? 1 : P.ncell;
+  std::vector<double> h(5 * nd, 0.0);
+  for (int64_t i = 0; i < nd; ++i) {
+    h[i] = aL[i];
+    h[nd + i] = aT[i];
+    h[2 * nd + i] = model == 1 ? aLV[i] : 0.0;
+    h[3 * nd + i] = model == 1 ? aTV[i] : 0.0;
+    h[4 * nd + i] = dmeff[i];
+  }
+  hipError_t err = upload(e, &P.disp, h.data(), h.size());
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "set_dispersion");
+  P.ndisp = nd;
+  P.disp_model = model;
+  e->have_disp = true;
+  e->tables_dirty = true;
+  return RWPT_OK;
+}
+
+int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
+                        const double *lx, const double *ly, const double *lz,
+                        const double *t, const int32_t *status,
+                        const uint64_t *rng, int64_t stream_offset) {
+  if (!e) return RWPT_E_ARG;
+  if (n < 0 || (n > 0 && (!cell || !lx || !ly || !lz || !t || !status)))
+    return fail(e, RWPT_E_ARG, "particle arrays required");
+  if (stream_offset < 0) return fail(e, RWPT_E_ARG, "stream_offset < 0");
+  const int64_t ncell = e->prob.ncell;
+  for (int64_t p = 0; p < n; ++p) {
+    if (cell[p] < 1 || cell[p] > ncell)
+      return fail(e, RWPT_E_ARG, "particle cell outside the grid");
+    if (status[p] < 0 || status[p] > 9)
+      return fail(e, RWPT_E_ARG, "particle status outside 0..9");
+  }
+  Particles &P = e->part;
+  hipError_t err = hipSuccess;
+  if (n > P.cap) {
+    free_particles(P);
+    err = dev_alloc(&P.cell, n);
+    if (err == hipSuccess) err = dev_alloc(&P.status, n);
+    if (err == hipSuccess) err = dev_alloc(&P.lx, n);
+    if (err == hipSuccess) err = dev_alloc(&P.ly, n);
+    if (err == hipSuccess) err = dev_alloc(&P.lz, n);
+    if (err == hipSuccess) err = dev_alloc(&P.t, n);
+    if (err == hipSuccess) err = dev_alloc(&P.rng, 2 * n);
+    if (err == hipSuccess) err = dev_alloc(&P.substeps, n);
+    if (err != hipSuccess) {
+      free_particles(P);
+      return hip_fail(e, err, "load_particles alloc");
+    }
+    P.cap = n;
+  }
+  P.n = n;
+  if (n == 0) return RWPT_OK;
+  hipStream_t s = e->stream;
+  err = hipMemcpyAsync(P.cell, cell, n * 4, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(P.status, status, n * 4, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(P.lx, lx, n * 8, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(P.ly, ly, n * 8, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(P.lz, lz, n * 8, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(P.t, t, n * 8, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipMemsetAsync(P.substeps, 0, n * 8, s);
+  if (err == hipSuccess) {
+    if (rng) err = hipMemcpyAsync(P.rng, rng, n * 16, hipMemcpyHostToDevice, s);
+    else err = launch_seed_streams(e, n, stream_offset);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err != hipSuccess) return hip_fail(e, err, "load_particles");
+  return RWPT_OK;
+}
+
+int rwpt_get_particles(rwpt_engine *e, int32_t *cell, double *lx, double *ly,
+                       double *lz, double *t, int32_t *status, uint64_t *rng) {
+  if (!e) return RWPT_E_ARG;
+  Particles &P = e->part;
+  const int64_t n = P.n;
+  if (n == 0) return RWPT_OK;
+  hipStream_t s = e->stream;
+  hipError_t err = hipSuccess;
+  if (cell) err = hipMemcpyAsync(cell, P.cell, n * 4, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && lx) err = hipMemcpyAsync(lx, P.lx, n * 8, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && ly) err = hipMemcpyAsync(ly, P.ly, n * 8, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && lz) err = hipMemcpyAsync(lz, P.lz, n * 8, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && t) err = hipMemcpyAsync(t, P.t, n * 8, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && status) err = hipMemcpyAsync(status, P.status, n * 4, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess && rng) err = hipMemcpyAsync(rng, P.rng, n * 16, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err != hipSuccess) return hip_fail(e, err, "get_particles");
+  return RWPT_OK;
+}
+
+int rwpt_get_substeps(rwpt_engine *e, int64_t *substeps) {
+  if (!e || !substeps) return RWPT_E_ARG;
+  Particles &P = e->part;
+  if (P.n == 0) return RWPT_OK;
+  hipError_t err = hipMemcpy(substeps, P.substeps, P.n * 8, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return hip_fail(e, err, "get_substeps");
+  return RWPT_OK;
+}
+
+int64_t rwpt_particle_count(const rwpt_engine *e) { return e ? e->part.n : -1; }
+
+int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
+                 int64_t *n_substeps) {
+  if (!e) return RWPT_E_ARG;
+  if (!std::isfinite(t_stop)) return fail(e, RWPT_E_ARG, "t_stop not finite");
+  int rc = ensure_tables(e);
+  if (rc != RWPT_OK) return rc;
+  if (e->opt.precision != 0)
+    return fail(e, RWPT_E_UNSUPPORTED, "FP32 cell-local walk not built yet");
+  hipError_t err = launch_advance(e, t_stop);
+  if (err != hipSuccess) return hip_fail(e, err, "advance launch");
+  unsigned long long h[4] = {0, 0, 0, 0};
+  err = hipMemcpyAsync(h, e->counters, sizeof h, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "advance");
+  if (e->part.n > 0) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e->timers[0].start, e->timers[0].stop);
+    e->timers[0].total_ms += ms;
+    e->timers[0].launches += 1;
+  }
+  e->t_engine = t_stop;
+  if (n_active) *n_active = (int64_t)h[0];
+  if (n_substeps) *n_substeps = (int64_t)h[1];
+  return RWPT_OK;
+}
+
+int rwpt_set_time(rwpt_engine *e, double t) {
+  if (!e) return RWPT_E_ARG;
+  e->t_engine = t;
+  return RWPT_OK;
+}
+
+int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz) {
+  if (!e || !gx || !gy || !gz) return RWPT_E_ARG;
+  const int64_t n = e->part.n;
+  if (n == 0) return RWPT_OK;
+  double *d = nullptr;
+  hipError_t err = dev_alloc(&d, 3 * n);
+  if (err == hipSuccess) err = launch_global_xyz(e, d, d + n, d + 2 * n);
+  if (err == hipSuccess) err = hipMemcpy(gx, d, n * 8, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(gy, d + n, n * 8, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(gz, d + 2 * n, n * 8, hipMemcpyDeviceToHost);
+  dev_free(d);
+  if (err != hipSuccess) return hip_fail(e, err, "global_xyz");
+  return RWPT_OK;
+}
+
+int rwpt_get_cell_tables(rwpt_engine *e, double *out) {
+  if (!e || !out) return RWPT_E_ARG;
+  if (!e->have_flow || !e->have_disp)
+    return fail(e, RWPT_E_STATE, "flow and dispersion must be set");
+  double *d = nullptr;
+  const int64_t n = e->prob.ncell * DEBUG_FIELDS;
+  hipError_t err = dev_alloc(&d, n);
+  if (err == hipSuccess) err = build_cell_tables(e, d);
+  if (err == hipSuccess) err = hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  dev_free(d);
+  if (err != hipSuccess) return hip_fail(e, err, "cell tables");
+  e->tables_dirty = false;
+  return RWPT_OK;
+}
+
+int rwpt_kernel_time(rwpt_engine *e, int32_t which, double *ms,
+                     int64_t *launches) {
+  if (!e || which < 0 || which > 2) return RWPT_E_ARG;
+  if (ms) *ms = e->timers[which].total_ms;
+  if (launches) *launches = e->timers[which].launches;
+  return RWPT_OK;
+}
+
+int rwpt_reset_kernel_time(rwpt_engine *e) {
+  if (!e) return RWPT_E_ARG;
+  for (auto &T : e->timers) {
+    T.total_ms = 0.0;
+    T.launches = 0;
+  }
+  return RWPT_OK;
+}
+
+int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
+             double *density, double *histogram) {
+  if (!e || !opt) return RWPT_E_ARG;
+  return fail(e, RWPT_E_UNSUPPORTED, "kde not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,404 @@
+// Per-cell table build: once per flow time step, every cell gathers its
+// 18-cell stencil and stores the corner quantities a random-walk sub-step
+// needs.  The reference instead rebuilds the cell buffer, the 18 neighbour
+// buffers and the corner variables on every cell entry of every particle
+// (ParticleTrackingEngine.f90:1093-1108, TrackCell.f90:511-518); building
+// them once turns that O(entries) work into O(cells).
+//
+// Two passes, one thread per cell:
+//  1. assemble_cells: face flows with boundary flows, GetDZRW, connections
+//     and mass-boundary flags (SetMassTransportDataStructured,
+//     ModpathCellData.f90:1099-1487; GetDZRW :3208-3256).
+//  2. corner_tables: corner discharge, corner porosity, discharge products,
+//     corner dispersion and their derivative terms
+//     (TrackSubCell.F90:3667-4363, 2557-2691, 3117-3303, 4432-4543).
+// Floating-point expressions keep the reference's operand order and the
+// file is compiled with -ffp-contract=off, so every table entry is the
+// double the reference computes.
+#include "rwpt_internal.h"
+
+namespace rwpt {
+namespace {
+
+struct GridDims {
+  int ncol, nrow, nlay;
+};
+
+__device__ __forceinline__ int face_neighbor(const GridDims g, int c0, int f) {
+  // c0 is 0-based; returns the 1-based neighbour or 0 (ComputeFaceConnection)
+  const int lc = g.nrow * g.ncol;
+  const int k = c0 / lc, r = c0 % lc, i = r / g.ncol, j = r % g.ncol;
+  switch (f) {
+    case 0: return j > 0 ? c0 : 0;                       // (c0+1)-1
+    case 1: return j < g.ncol - 1 ? c0 + 2 : 0;
+    case 2: return i < g.nrow - 1 ? c0 + 1 + g.ncol : 0;
+    case 3: return i > 0 ? c0 + 1 - g.ncol : 0;
+    case 4: return k < g.nlay - 1 ? c0 + 1 + lc : 0;
+    default: return k > 0 ? c0 + 1 - lc : 0;
+  }
+}
+
+__global__ void assemble_cells(const GridDims g, int64_t ncell,
+                               const double *__restrict__ frf,
+                               const double *__restrict__ fff,
+                               const double *__restrict__ flf,
+                               const double *__restrict__ bflow,
+                               const int32_t *__restrict__ ibound,
+                               const int32_t *__restrict__ icbound,
+                               int32_t default_icbound,
+                               const double *__restrict__ top,
+                               const double *__restrict__ bot,
+                               const double *__restrict__ head,
+                               const int32_t *__restrict__ layer_type,
+                               const int32_t *__restrict__ zone,
+                               const double *__restrict__ source,
+                               const double *__restrict__ sink,
+                               double *__restrict__ q6,
+                               double *__restrict__ dzrw,
+                               double *__restrict__ cells) {
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c0 >= ncell) return;
+  int conn[6], mb[6], bcount[6];
+  double Q[6] = {0, 0, 0, 0, 0, 0};
+  for (int f = 0; f < 6; ++f) {
+    conn[f] = face_neighbor(g, (int)c0, f);
+    if (conn[f] > 0) {
+      bcount[f] = (ibound && ibound[conn[f] - 1] == 0) ? 1 : 0;
+      mb[f] = icbound ? icbound[conn[f] - 1] : 0;
+    } else {
+      bcount[f] = 1;
+      mb[f] = default_icbound;
+    }
+  }
+  if (conn[0] > 0) Q[0] = frf[conn[0] - 1];
+  Q[1] = frf[c0];
+  Q[2] = -fff[c0];
+  if (conn[3] > 0) Q[3] = -fff[conn[3] - 1];
+  Q[4] = -flf[c0];
+  if (conn[5] > 0) Q[5] = -flf[conn[5] - 1];
+  double src = source ? source[c0] : 0.0;
+  double snk = sink ? sink[c0] : 0.0;
+  if (bflow) {
+    for (int f = 0; f < 6; ++f) {
+      const double bf = bflow[6 * c0 + f];
+      if (bf != 0.0) {
+        if (bcount[f] > 0) {
+          const double flow = bf / bcount[f];
+          if (conn[f] == 0 || (ibound && ibound[conn[f] - 1] == 0))
+            Q[f] = (f % 2 == 0) ? flow : -flow;
+        } else if (bf > 0.0) {
+          src = src + bf;
+        } else {
+          snk = snk + bf;
+        }
+      }
+    }
+  }
+  // GetDZRW and VerifyDryCell
+  const double tp = top[c0], bt = bot[c0];
+  const double hd = head ? head[c0] : tp;
+  const int lt = layer_type ? layer_type[c0] : 0;
+  double dz = tp - bt;
+  const double dzc = tp - bt;
+  bool dry = false, vpdry = false;
+  bool anyflow = false;
+  for (int f = 0; f < 6; ++f) anyflow |= (Q[f] != 0.0);
+  if (lt == 1 && hd < tp) {
+    dz = hd - bt;
+    if ((dz <= 0.0) || ((hd - bt) / dzc < 0.01)) {
+      dz = tp - bt;
+      dry = !anyflow;
+    }
+    // VerifyDryCell: partially dry unless head <= bottom with no flow
+    vpdry = !((hd - bt) <= 0.0) || anyflow;
+  }
+  for (int f = 0; f < 6; ++f) q6[6 * c0 + f] = Q[f];
+  dzrw[c0] = dz;
+  double *rec = cells + c0 * CELL_DOUBLES;
+  int32_t *ic = reinterpret_cast<int32_t *>(rec + F_CONN);
+  int32_t *im = reinterpret_cast<int32_t *>(rec + F_MASSBND);
+  for (int f = 0; f < 6; ++f) {
+    ic[f] = conn[f];
+    im[f] = mb[f];
+  }
+  bool cell_exit = (Q[0] < 0.0) || (Q[1] > 0.0) || (Q[2] < 0.0) ||
+                   (Q[3] > 0.0) || (Q[4] < 0.0) || (Q[5] > 0.0);
+  int32_t *fl = reinterpret_cast<int32_t *>(rec + F_FLAGS);
+  fl[0] = ibound ? ibound[c0] : 1;
+  fl[1] = (dry ? CF_DRY : 0) | (cell_exit ? CF_CELL_EXIT : 0) |
+          (vpdry ? CF_VERIFY_PDRY : 0);
+  int32_t *zn = reinterpret_cast<int32_t *>(rec + F_ZONE);
+  zn[0] = zone ? zone[c0] : 0;
+  zn[1] = 0;
+  rec[F_SOURCE] = src;
+  rec[F_SINK] = snk;
+}
+
+// Stencil slot offsets (dk, di, dj) in the order of
+// pr_FillNeighborCellDataStructured (ParticleTrackingEngine.f90:2009-2126):
+// W, W-front, W-back, E, E-front, E-back, front, front-bottom, front-top,
+// back, back-bottom, back-top, bottom, bottom-W, bottom-E, top, top-W, top-E.
+__constant__ int kSlot[18][3] = {
+    {0, 0, -1}, {0, 1, -1}, {0, -1, -1}, {0, 0, 1},  {0, 1, 1},  {0, -1, 1},
+    {0, 1, 0},  {1, 1, 0},  {-1, 1, 0},  {0, -1, 0}, {1, -1, 0}, {-1, -1, 0},
+    {1, 0, 0},  {1, 0, -1}, {1, 0, 1},   {-1, 0, 0}, {-1, 0, -1}, {-1, 0, 1}};
+
+// Corner component and porosity index tables (1-based slots, 4th entry of
+// the component tables is the face), TrackSubCell.F90:4142-4224, 4329-4363.
+__constant__ int kCx[8][4] = {{7, 8, 13, 1},   {7, 8, 13, 2},   {10, 11, 13, 1},
+                              {10, 11, 13, 2}, {7, 9, 16, 1},   {7, 9, 16, 2},
+                              {10, 12, 16, 1}, {10, 12, 16, 2}};
+__constant__ int kCy[8][4] = {{1, 13, 14, 3}, {4, 13, 15, 3}, {1, 13, 14, 4},
+                              {4, 13, 15, 4}, {1, 16, 17, 3}, {4, 16, 18, 3},
+                              {1, 16, 17, 4}, {4, 16, 18, 4}};
+__constant__ int kCz[8][4] = {{1, 2, 7, 5},  {4, 5, 7, 5}, {1, 3, 10, 5},
+                              {4, 6, 10, 5}, {1, 2, 7, 6}, {4, 5, 7, 6},
+                              {1, 3, 10, 6}, {4, 6, 10, 6}};
+__constant__ int kCp[8][6] = {{1, 2, 7, 8, 13, 14},   {4, 5, 7, 8, 13, 15},
+                              {1, 3, 10, 11, 13, 14}, {4, 6, 10, 12, 16, 18},
+                              {1, 2, 7, 9, 16, 17},   {4, 5, 7, 9, 16, 18},
+                              {1, 3, 10, 12, 16, 17}, {4, 6, 10, 11, 13, 15}};
+
+struct TableParams {
+  int32_t disp_model, uniform_porosity, backward, time_step_kind;
+  double ts0, ts1;
+};
+
+__global__ void corner_tables(const GridDims g, int64_t ncell,
+                              const double *__restrict__ delr,
+                              const double *__restrict__ delc,
+                              const double *__restrict__ porosity,
+                              const double *__restrict__ retardation,
+                              const double *__restrict__ disp, int64_t ndisp,
+                              const double *__restrict__ q6,
+                              const double *__restrict__ dzrw,
+                              const TableParams tp, double *__restrict__ cells,
+                              double *__restrict__ debug) {
+  const int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c0 >= ncell) return;
+  const int lc = g.nrow * g.ncol;
+  const int k = (int)(c0 / lc), r = (int)(c0 % lc), i = r / g.ncol,
+            j = r % g.ncol;
+  double *rec = cells + c0 * CELL_DOUBLES;
+  // sub-cell buffer, FillMassSubCellDataBuffer (ModpathCellData.f90:2522-2684)
+  const double DX = delr[j] / 1.0, DY = delc[i] / 1.0, DZ = dzrw[c0];
+  const double phi = porosity[c0];
+  const double R = retardation ? retardation[c0] : 1.0;
+  const double sign = tp.backward ? -1.0 : 1.0;
+  double Qc[6];
+  for (int f = 0; f < 6; ++f) Qc[f] = q6[6 * c0 + f];
+  const double VX1 = sign * Qc[0] / DY / DZ / phi / R;
+  const double VX2 = sign * Qc[1] / DY / DZ / phi / R;
+  const double VY1 = sign * Qc[2] / DX / DZ / phi / R;
+  const double VY2 = sign * Qc[3] / DX / DZ / phi / R;
+  const double VZ1 = sign * Qc[4] / DX / DY / phi / R;
+  const double VZ2 = sign * Qc[5] / DX / DY / phi / R;
+  // dispersion parameters (pr_FillDispersionParameters*)
+  const int64_t d = ndisp == 1 ? 0 : c0;
+  const double aLH = disp[d], aTH = disp[ndisp + d];
+  const double aLV = tp.disp_model == 1 ? disp[2 * ndisp + d] : 0.0;
+  const double aTV = tp.disp_model == 1 ? disp[3 * ndisp + d] : 0.0;
+  const double dm = disp[4 * ndisp + d];
+
+  // stencil data (pr_FillNeighborSubCellVariables, unrefined branch)
+  double nQ[18][6], nA[18][3], nV[18], nP[18];
+  for (int n = 0; n < 18; ++n) {
+    const int kk = k + kSlot[n][0], ii = i + kSlot[n][1], jj = j + kSlot[n][2];
+    const bool in = kk >= 0 && kk < g.nlay && ii >= 0 && ii < g.nrow &&
+                    jj >= 0 && jj < g.ncol;
+    if (!in) {
+      for (int f = 0; f < 6; ++f) nQ[n][f] = 0.0;
+      nA[n][0] = nA[n][1] = nA[n][2] = 0.0;
+      nV[n] = 0.0;
+      nP[n] = 0.0;
+      continue;
+    }
+    const int64_t nc = (int64_t)kk * lc + (int64_t)ii * g.ncol + jj;
+    for (int f = 0; f < 6; ++f) nQ[n][f] = q6[6 * nc + f];
+    const double dx = delr[jj], dy = delc[ii], dz = dzrw[nc];
+    nA[n][0] = dy * dz;
+    nA[n][1] = dx * dz;
+    nA[n][2] = dx * dy;
+    nV[n] = dx * dy * dz;
+    nP[n] = porosity[nc];
+  }
+  // corner discharge (pr_ComputeCornerDischarge, GetInterpolatedCornerDischarge)
+  double q[8][4];
+  for (int dir = 0; dir < 3; ++dir) {
+    for (int ci = 0; ci < 8; ++ci) {
+      const int *ix = dir == 0 ? kCx[ci] : (dir == 1 ? kCy[ci] : kCz[ci]);
+      double sumA = dir == 0 ? DY * DZ : (dir == 1 ? DX * DZ : DX * DY);
+      double sumQ = Qc[ix[3] - 1];
+      for (int m = 0; m < 3; ++m) {
+        sumA = sumA + nA[ix[m] - 1][dir];
+        sumQ = sumQ + nQ[ix[m] - 1][ix[3] - 1];
+      }
+      q[ci][dir] = sumQ / sumA;
+    }
+  }
+  for (int ci = 0; ci < 8; ++ci)
+    q[ci][3] = sqrt(q[ci][0] * q[ci][0] + q[ci][1] * q[ci][1] +
+                    q[ci][2] * q[ci][2]);
+  // corner porosity (Uniform / Interpolated)
+  double pc[8];
+  for (int ci = 0; ci < 8; ++ci) {
+    if (tp.uniform_porosity) {
+      pc[ci] = phi;
+    } else {
+      double sV = DX * DY * DZ;
+      double sW = phi * sV;
+      for (int m = 0; m < 6; ++m) {
+        const int n = kCp[ci][m] - 1;
+        sV = sV + nV[n];
+        sW = sW + nV[n] * nP[n];
+      }
+      pc[ci] = sW / sV;
+    }
+  }
+  // discharge products and corner dispersion
+  double Dc[6][8];  // Dxxx Dyyy Dzzz Dxyx Dxzx Dyzy
+  for (int ci = 0; ci < 8; ++ci) {
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q[ci][3] > 0.0) {
+      p[0] = q[ci][0] * q[ci][0];
+      p[1] = q[ci][0] * q[ci][1];
+      p[2] = q[ci][0] * q[ci][2];
+      p[3] = q[ci][1] * q[ci][1];
+      p[4] = q[ci][1] * q[ci][2];
+      p[5] = q[ci][2] * q[ci][2];
+      p[6] = p[0] + p[3];
+      if (p[6] > 0.0) p[7] = p[5] / p[6];
+      for (int m = 0; m < 7; ++m) p[m] = p[m] / q[ci][3];
+    }
+    if (tp.disp_model == 0) {
+      // corners 010..111 take |q| of corner 100 (TrackSubCell.F90:2622-2677)
+      const double qn = ci == 0 ? q[0][3] : q[1][3];
+      const double aLmaT = aLH - aTH;
+      const double aTQ = aTH * qn + pc[ci] * dm;
+      Dc[0][ci] = aTQ + aLmaT * p[0];
+      Dc[1][ci] = aTQ + aLmaT * p[3];
+      Dc[2][ci] = aTQ + aLmaT * p[5];
+      Dc[3][ci] = aLmaT * p[1];
+      Dc[4][ci] = aLmaT * p[2];
+      Dc[5][ci] = aLmaT * p[4];
+    } else {
+      const double pD = pc[ci] * dm;
+      double cos2 = 0.0;
+      if (q[ci][3] > 0.0) cos2 = q[ci][2] * q[ci][2] / q[ci][3];
+      const double aL = aLH + cos2 * (aLV - aLH);
+      const double aT = aTV + cos2 * (aTH - aTV);
+      const double aLmaT = aL - aT;
+      Dc[0][ci] = pD + (aL + aT * p[7]) * p[0] + aTH * p[3] * (1.0 + p[7]);
+      Dc[1][ci] = pD + aTH * p[0] * (1.0 + p[7]) + p[3] * (aL + aT * p[7]);
+      Dc[2][ci] = pD + aT * p[6] + aL * p[5];
+      Dc[3][ci] = (aL - aTH * (1.0 + p[7]) + aT * p[7]) * p[0];
+      Dc[4][ci] = aLmaT * p[2];
+      Dc[5][ci] = aLmaT * p[4];
+    }
+  }
+  // derivative terms: X(a) uses corner pairs along x, etc.
+  // term order: Xxx Yyy Zzz Xxy Xxz Yxy Yyz Zxz Zyz
+  const int src[9] = {0, 1, 2, 3, 4, 3, 5, 4, 5};
+  const int axis[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+  for (int tix = 0; tix < 9; ++tix) {
+    const double *v = Dc[src[tix]];
+    double a, b, c, dd;
+    if (axis[tix] == 0) {
+      a = (v[1] - v[0]) / DX;   // 100-000
+      b = (v[3] - v[2]) / DX;   // 110-010
+      c = (v[5] - v[4]) / DX;   // 101-001
+      dd = (v[7] - v[6]) / DX;  // 111-011
+    } else if (axis[tix] == 1) {
+      a = (v[2] - v[0]) / DY;   // 010-000
+      b = (v[3] - v[1]) / DY;   // 110-100
+      c = (v[6] - v[4]) / DY;   // 011-001
+      dd = (v[7] - v[5]) / DY;  // 111-101
+    } else {
+      a = (v[4] - v[0]) / DZ;   // 001-000
+      b = (v[5] - v[1]) / DZ;   // 101-100
+      c = (v[6] - v[2]) / DZ;   // 011-010
+      dd = (v[7] - v[3]) / DZ;  // 111-110
+    }
+    rec[F_DD + 4 * tix + 0] = a;
+    rec[F_DD + 4 * tix + 1] = b;
+    rec[F_DD + 4 * tix + 2] = c;
+    rec[F_DD + 4 * tix + 3] = dd;
+  }
+  for (int comp = 0; comp < 3; ++comp)
+    for (int ci = 0; ci < 8; ++ci)
+      rec[F_VC + 8 * comp + ci] = q[ci][comp] / pc[ci];
+  // cell time step (pr_ComputeRandomWalkTimeStep, TrackSubCell.F90:1376-1458)
+  const double mx = fmax(fabs(VX1), fabs(VX2));
+  const double my = fmax(fabs(VY1), fabs(VY2));
+  const double mz = fmax(fabs(VZ1), fabs(VZ2));
+  double dtc;
+  const double adv = tp.ts0 / (mx / DX + my / DY + mz / DZ);
+  const double dis = tp.ts1 / (aLH * mx / (DX * DX) + aTH * my / (DY * DY) +
+                               aTH * mz / (DZ * DZ));
+  switch (tp.time_step_kind) {
+    case 1: dtc = adv; break;
+    case 2: dtc = dis; break;
+    case 3: {
+      double m = 1.7976931348623157e308;
+      if (adv > 0.0 && adv < m) m = adv;
+      if (dis > 0.0 && dis < m) m = dis;
+      dtc = m;
+      break;
+    }
+    default: dtc = tp.ts0;
+  }
+  rec[F_VX1] = VX1; rec[F_VX2] = VX2; rec[F_VY1] = VY1;
+  rec[F_VY2] = VY2; rec[F_VZ1] = VZ1; rec[F_VZ2] = VZ2;
+  rec[F_DX] = DX; rec[F_DY] = DY; rec[F_DZ] = DZ;
+  rec[F_PHI] = phi; rec[F_R] = R; rec[F_DTCELL] = dtc;
+  rec[F_DVDX] = (VX2 - VX1) / DX;
+  rec[F_DVDY] = (VY2 - VY1) / DY;
+  rec[F_DVDZ] = (VZ2 - VZ1) / DZ;
+  rec[F_ALH] = aLH; rec[F_ATH] = aTH; rec[F_ALV] = aLV; rec[F_ATV] = aTV;
+  rec[F_DMEFF] = dm;
+  const bool sub_exit = (VX1 < 0.0) || (VX2 > 0.0) || (VY1 < 0.0) ||
+                        (VY2 > 0.0) || (VZ1 < 0.0) || (VZ2 > 0.0);
+  int32_t *fl = reinterpret_cast<int32_t *>(rec + F_FLAGS);
+  fl[1] = (fl[1] & ~CF_SUBCELL_EXIT) | (sub_exit ? CF_SUBCELL_EXIT : 0);
+  for (int pad = 90; pad < CELL_DOUBLES; ++pad) rec[pad] = 0.0;
+
+  if (debug) {
+    double *o = debug + c0 * DEBUG_FIELDS;
+    o[0] = VX1; o[1] = VX2; o[2] = VY1; o[3] = VY2; o[4] = VZ1; o[5] = VZ2;
+    o[6] = DX; o[7] = DY; o[8] = DZ; o[9] = phi; o[10] = R; o[11] = dtc;
+    for (int ci = 0; ci < 8; ++ci)
+      for (int m = 0; m < 4; ++m) o[12 + 4 * ci + m] = q[ci][m];
+    for (int ci = 0; ci < 8; ++ci) o[44 + ci] = pc[ci];
+    for (int b6 = 0; b6 < 6; ++b6)
+      for (int ci = 0; ci < 8; ++ci) o[52 + 8 * b6 + ci] = Dc[b6][ci];
+    o[100] = (fl[1] & CF_DRY) ? 1.0 : 0.0;
+    o[101] = sub_exit ? 1.0 : 0.0;
+    o[102] = 0.0;
+    o[103] = 0.0;
+  }
+}
+
+}  // namespace
+
+hipError_t build_cell_tables(rwpt_engine *e, double *debug_out) {
+  DeviceProblem &P = e->prob;
+  const GridDims g{P.ncol, P.nrow, P.nlay};
+  const int block = 256;
+  const unsigned grid = (unsigned)((P.ncell + block - 1) / block);
+  hipLaunchKernelGGL(assemble_cells, dim3(grid), dim3(block), 0, e->stream, g,
+                     P.ncell, P.frf, P.fff, P.flf, P.bflow, P.ibound,
+                     P.icbound, P.default_icbound, P.top, P.bot, P.head,
+                     P.layer_type, P.zone, P.source, P.sink, P.q6, P.dzrw,
+                     P.cells);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  TableParams tp{P.disp_model, P.uniform_porosity, e->opt.backward_tracking,
+                 e->opt.time_step_kind, e->opt.time_step_params[0],
+                 e->opt.time_step_params[1]};
+  hipLaunchKernelGGL(corner_tables, dim3(grid), dim3(block), 0, e->stream, g,
+                     P.ncell, P.delr, P.delc, P.porosity, P.retardation,
+                     P.disp, P.ndisp, P.q6, P.dzrw, tp, P.cells, debug_out);
+  return hipGetLastError();
+}
+
+}  // namespace rwpt

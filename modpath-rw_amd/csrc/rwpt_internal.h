@@ -1,0 +1,130 @@
+// Internal declarations shared by the HIP translation units of librwpt.
+//
+// Device data layout (all resident in HBM for the engine's lifetime):
+//  * particle state, structure of arrays: cell (i32), status (i32),
+//    lx/ly/lz (f64), t (f64), rng (2 x u64), substeps (i64);
+//  * one CellRec (768 B, 6 x 128-B lines) per grid cell holding everything a
+//    random-walk sub-step reads: face pore velocities, sizes, porosity,
+//    retardation, dispersivities, the cell time step, corner velocities and
+//    the 9 dispersion-derivative terms in the form the trilinear derivative
+//    consumes, connections and mass-boundary flags.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "rwpt.h"
+
+namespace rwpt {
+
+// Field offsets (in doubles) inside a CellRec.
+enum CellField : int {
+  F_VX1 = 0, F_VX2, F_VY1, F_VY2, F_VZ1, F_VZ2,  // 0..5 face pore velocities
+  F_DX = 6, F_DY, F_DZ,                          // 6..8
+  F_PHI = 9, F_R = 10, F_DTCELL = 11,
+  F_DVDX = 12, F_DVDY, F_DVDZ,                   // exponential gradients
+  F_ALH = 15, F_ATH, F_ALV, F_ATV, F_DMEFF,      // 15..19
+  F_VC = 20,        // 20..43: corner velocity [component][corner]
+  F_DD = 44,        // 44..79: derivative terms [term][4]
+  F_CONN = 80,      // 80..82: 6 x int32 connections
+  F_MASSBND = 83,   // 83..85: 6 x int32 mass-boundary flags
+  F_FLAGS = 86,     // int32 ibound, int32 flags
+  F_ZONE = 87,      // int32 zone
+  F_SOURCE = 88, F_SINK = 89,
+  CELL_DOUBLES = 96
+};
+
+// flag bits in the second int32 of F_FLAGS
+enum : int32_t {
+  CF_DRY = 1,             // GetDZRW dry (TrackSubCell returns InactiveCell)
+  CF_SUBCELL_EXIT = 2,    // sub-cell HasExitFace
+  CF_CELL_EXIT = 4,       // cell HasExitFace
+  CF_VERIFY_PDRY = 8,     // VerifyDryCell partiallyDry (release of status 7)
+};
+
+// Layout of the checking copy of the cell tables (104 doubles per cell).
+constexpr int DEBUG_FIELDS = 104;
+
+struct KernelTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct DeviceProblem {
+  int ncol = 0, nrow = 0, nlay = 0;
+  int64_t ncell = 0;
+  // device inputs
+  double *delr = nullptr, *delc = nullptr, *top = nullptr, *bot = nullptr;
+  double *frf = nullptr, *fff = nullptr, *flf = nullptr, *bflow = nullptr;
+  double *porosity = nullptr, *retardation = nullptr;
+  int32_t *ibound = nullptr, *icbound = nullptr;
+  double *head = nullptr;
+  int32_t *layer_type = nullptr, *zone = nullptr;
+  double *source = nullptr, *sink = nullptr;
+  double *disp = nullptr;  // 5 arrays of ndisp: aL aT aLV aTV dmeff
+  int64_t ndisp = 0;
+  int32_t default_icbound = 0;
+  int32_t disp_model = 0;
+  int32_t uniform_porosity = 0;
+  // assembled per-cell scratch of the table build
+  double *q6 = nullptr;     // 6 * ncell assembled face flows
+  double *dzrw = nullptr;   // ncell
+  // tables
+  double *cells = nullptr;  // ncell * CELL_DOUBLES
+  float *cells32 = nullptr; // FP32-local tables (ncell * CELL_DOUBLES)
+};
+
+struct Particles {
+  int64_t n = 0, cap = 0;
+  int32_t *cell = nullptr, *status = nullptr;
+  double *lx = nullptr, *ly = nullptr, *lz = nullptr, *t = nullptr;
+  uint64_t *rng = nullptr;
+  int64_t *substeps = nullptr;
+};
+
+struct WalkParams {
+  int32_t advection_kind, time_step_kind, ndim, dims[3];
+  int32_t disp_model, backward, stop_zone, stop_weak_sinks, stop_weak_sources;
+  int32_t steady_state;
+  double t_engine, t_stop;
+  int64_t max_steps;  // per particle and launch
+};
+
+}  // namespace rwpt
+
+struct rwpt_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rwpt_options opt{};
+  rwpt::DeviceProblem prob;
+  rwpt::Particles part;
+  bool have_flow = false, have_disp = false, tables_dirty = true;
+  double t_engine = 0.0;
+  uint64_t *jump_mats = nullptr;  // device copies of J^(2^k), k < 64
+  unsigned long long *counters = nullptr;  // device counters
+  double *kde_scratch = nullptr;
+  int64_t kde_scratch_size = 0;
+  rwpt::KernelTimer timers[3];
+  std::string err;
+};
+
+namespace rwpt {
+
+// launchers (return hipError_t as int)
+hipError_t build_cell_tables(rwpt_engine *e, double *debug_out);
+hipError_t launch_seed_streams(rwpt_engine *e, int64_t n, int64_t offset);
+hipError_t launch_advance(rwpt_engine *e, double t_stop);
+hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
+                             double *gz);
+hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
+                      const double *mass_dev, double *density_dev,
+                      double *hist_dev, double *stats_host);
+
+// host helper: GF(2) matrices of the xoroshiro128+ jump, J^(2^k)
+void jump_power_matrices(int kmax, std::vector<uint64_t> &out);
+
+}  // namespace rwpt

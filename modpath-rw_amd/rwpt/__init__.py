@@ -1,0 +1,9 @@
+"""rwpt: MI355X-native random-walk particle tracking for MODPATH-RW's hot path.
+
+Importable once ``modpath-rw_amd`` is on ``sys.path``.  The compute lives in
+``librwpt.so`` (HIP, gfx950) behind the C ABI of ``include/rwpt.h``;
+``rwpt.engine`` binds it, ``rwpt.problems`` synthesises structured-grid
+inputs.
+"""
+from . import problems  # noqa: F401
+from .engine import Engine, RwptError, library_path  # noqa: F401

@@ -1,0 +1,267 @@
+"""Host-side mirror of MODPATH-RW's tracking engine over the librwpt C ABI.
+
+``Engine`` plays the role of ParticleTrackingEngineType
+(ParticleTrackingEngine.f90:393-492 Initialize, pr_RWPTrackPath :1051-1254)
+driven the way MPathRW.F90:1119-1725 drives it: set the flow and the
+dispersion once, load the particle groups, then ``advance(t_stop)`` once per
+tracking interval.  All compute runs in librwpt.so (HIP, gfx950); this module
+only moves numpy arrays across the C ABI and raises on error codes.  There is
+no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_LIBPATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "librwpt.so")
+_LIB = None
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class RwptGrid(C.Structure):
+    _fields_ = [("ncol", C.c_int32), ("nrow", C.c_int32), ("nlay", C.c_int32),
+                ("delr", _dp), ("delc", _dp), ("top", _dp), ("bot", _dp)]
+
+
+class RwptOptions(C.Structure):
+    _fields_ = [("advection_kind", C.c_int32), ("time_step_kind", C.c_int32),
+                ("time_step_params", C.c_double * 2),
+                ("dimension_mask", C.c_int32 * 3),
+                ("backward_tracking", C.c_int32), ("stop_zone", C.c_int32),
+                ("stop_at_weak_sinks", C.c_int32),
+                ("stop_at_weak_sources", C.c_int32),
+                ("steady_state", C.c_int32), ("precision", C.c_int32),
+                ("base_seed", C.c_uint64 * 2)]
+
+
+class RwptKdeOptions(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("bin_size", C.c_double * 3),
+                ("nbins", C.c_int32 * 3), ("mode", C.c_int32),
+                ("weighted", C.c_int32), ("scaling_factor", C.c_double)]
+
+
+EXPORTS = {
+    "rwpt_default_options": ([C.POINTER(RwptOptions)], None),
+    "rwpt_create": ([C.POINTER(RwptGrid), C.POINTER(RwptOptions),
+                     C.POINTER(C.c_void_p)], C.c_int),
+    "rwpt_destroy": ([C.c_void_p], C.c_int),
+    "rwpt_last_error": ([C.c_void_p], C.c_char_p),
+    "rwpt_set_flow": ([C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p,
+                       C.c_int32], C.c_int),
+    "rwpt_set_cell_extras": ([C.c_void_p, _dp, _i32p, _i32p, _dp, _dp], C.c_int),
+    "rwpt_set_dispersion": ([C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp,
+                             _dp, _dp], C.c_int),
+    "rwpt_load_particles": ([C.c_void_p, C.c_int64, _i32p, _dp, _dp, _dp, _dp,
+                             _i32p, _u64p, C.c_int64], C.c_int),
+    "rwpt_get_particles": ([C.c_void_p, _i32p, _dp, _dp, _dp, _dp, _i32p,
+                            _u64p], C.c_int),
+    "rwpt_particle_count": ([C.c_void_p], C.c_int64),
+    "rwpt_advance": ([C.c_void_p, C.c_double, _i64p, _i64p], C.c_int),
+    "rwpt_set_time": ([C.c_void_p, C.c_double], C.c_int),
+    "rwpt_get_substeps": ([C.c_void_p, _i64p], C.c_int),
+    "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
+    "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
+    "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
+    "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
+    "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
+}
+
+
+class RwptError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return _LIBPATH
+
+
+def lib():
+    """Loads librwpt.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIBPATH):
+            raise RwptError(f"{_LIBPATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(_LIBPATH)
+        for name, (args, res) in EXPORTS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_options(**kw) -> RwptOptions:
+    o = RwptOptions()
+    lib().rwpt_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k in ("time_step_params",):
+            o.time_step_params[0], o.time_step_params[1] = v
+        elif k == "dimension_mask":
+            for d in range(3):
+                o.dimension_mask[d] = v[d]
+        elif k == "base_seed":
+            o.base_seed[0], o.base_seed[1] = v
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Engine:
+    """One particle-tracking engine resident on the current HIP device."""
+
+    def __init__(self, problem, precision: int = 0, **opt):
+        P = problem
+        self.problem = P
+        self._keep = []
+        g = RwptGrid()
+        g.ncol, g.nrow, g.nlay = P.ncol, P.nrow, P.nlay
+        arrs = [_d(P.delr), _d(P.delc), _d(P.top), _d(P.bot)]
+        self._keep += arrs
+        g.delr, g.delc, g.top, g.bot = [_p(a, _dp) for a in arrs]
+        o = default_options(advection_kind=P.advection_kind,
+                            time_step_kind=P.time_step_kind,
+                            time_step_params=P.ts_params,
+                            dimension_mask=P.dim_mask,
+                            backward_tracking=P.backward, precision=precision,
+                            **opt)
+        h = C.c_void_p()
+        rc = lib().rwpt_create(C.byref(g), C.byref(o), C.byref(h))
+        if rc != 0:
+            raise RwptError(f"rwpt_create failed ({rc})")
+        self.h = h
+        self.n = 0
+        self.set_flow(P)
+        self.set_dispersion(P)
+
+    # -- error handling -------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().rwpt_last_error(self.h).decode()
+            raise RwptError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rwpt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- inputs ---------------------------------------------------------------
+    def set_flow(self, P):
+        a = [_d(P.frf), _d(P.fff), _d(P.flf), _d(P.bflow), _d(P.porosity),
+             _d(P.retardation)]
+        ib, ic = _i(P.ibound), _i(P.icbound)
+        rc = lib().rwpt_set_flow(self.h, *[_p(x, _dp) for x in a], _p(ib, _i32p),
+                                 _p(ic, _i32p), int(P.default_icbound))
+        self._check(rc, "rwpt_set_flow")
+
+    def set_dispersion(self, P):
+        a = [_d(P.aL), _d(P.aT), _d(P.aLV), _d(P.aTV), _d(P.dmeff)]
+        rc = lib().rwpt_set_dispersion(self.h, int(P.disp_model), int(P.disp_uniform),
+                                       *[_p(x, _dp) for x in a])
+        self._check(rc, "rwpt_set_dispersion")
+
+    def load_particles(self, cell, lx, ly, lz, t=None, status=None, rng=None,
+                       stream_offset=0):
+        n = int(np.asarray(cell).size)
+        cell, lx, ly, lz = _i(cell), _d(lx), _d(ly), _d(lz)
+        t = _d(np.zeros(n) if t is None else t)
+        status = _i(np.ones(n, np.int32) if status is None else status)
+        rng = None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64)
+        rc = lib().rwpt_load_particles(self.h, n, _p(cell, _i32p), _p(lx, _dp),
+                                       _p(ly, _dp), _p(lz, _dp), _p(t, _dp),
+                                       _p(status, _i32p), _p(rng, _u64p),
+                                       int(stream_offset))
+        self._check(rc, "rwpt_load_particles")
+        self.n = n
+
+    # -- stepping -------------------------------------------------------------
+    def set_time(self, t):
+        self._check(lib().rwpt_set_time(self.h, float(t)), "rwpt_set_time")
+
+    def advance(self, t_stop):
+        act = C.c_int64(0)
+        ns = C.c_int64(0)
+        rc = lib().rwpt_advance(self.h, float(t_stop), C.byref(act), C.byref(ns))
+        self._check(rc, "rwpt_advance")
+        return act.value, ns.value
+
+    # -- outputs --------------------------------------------------------------
+    def particles(self):
+        n = self.n
+        out = {"cell": np.zeros(n, np.int32), "lx": np.zeros(n), "ly": np.zeros(n),
+               "lz": np.zeros(n), "t": np.zeros(n), "status": np.zeros(n, np.int32),
+               "rng": np.zeros(2 * n, np.uint64)}
+        rc = lib().rwpt_get_particles(self.h, _p(out["cell"], _i32p),
+                                      _p(out["lx"], _dp), _p(out["ly"], _dp),
+                                      _p(out["lz"], _dp), _p(out["t"], _dp),
+                                      _p(out["status"], _i32p), _p(out["rng"], _u64p))
+        self._check(rc, "rwpt_get_particles")
+        sub = np.zeros(n, np.int64)
+        self._check(lib().rwpt_get_substeps(self.h, _p(sub, _i64p)), "rwpt_get_substeps")
+        out["substeps"] = sub
+        return out
+
+    def global_xyz(self):
+        n = self.n
+        gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+        rc = lib().rwpt_global_xyz(self.h, _p(gx, _dp), _p(gy, _dp), _p(gz, _dp))
+        self._check(rc, "rwpt_global_xyz")
+        return gx, gy, gz
+
+    def cell_tables(self):
+        out = np.zeros((self.problem.ncell, 104))
+        self._check(lib().rwpt_get_cell_tables(self.h, _p(out, _dp)),
+                    "rwpt_get_cell_tables")
+        return out
+
+    def kernel_time(self, which=0):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        self._check(lib().rwpt_kernel_time(self.h, which, C.byref(ms), C.byref(n)),
+                    "rwpt_kernel_time")
+        return ms.value, n.value
+
+    def reset_kernel_time(self):
+        self._check(lib().rwpt_reset_kernel_time(self.h), "rwpt_reset_kernel_time")
+
+    def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0):
+        o = RwptKdeOptions()
+        for d in range(3):
+            o.origin[d] = origin[d]
+            o.bin_size[d] = bin_size[d]
+            o.nbins[d] = nbins[d]
+        o.mode = mode
+        o.weighted = int(mass is not None)
+        o.scaling_factor = scaling_factor
+        nb = int(np.prod(nbins))
+        dens = np.zeros(nb)
+        hist = np.zeros(nb)
+        m = _d(mass)
+        rc = lib().rwpt_kde(self.h, C.byref(o), _p(m, _dp), _p(dens, _dp), _p(hist, _dp))
+        self._check(rc, "rwpt_kde")
+        return dens, hist

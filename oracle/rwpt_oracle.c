@@ -810,6 +810,14 @@ static double conv_gd(double f0, double f1, double f2, double f3) {
 /* Literals the reference writes as default REAL (single precision). */
 #define R4(x) ((double)(float)(x))
 
+/* 10d0**n with an integer n: repeated products (exact for |n| <= 22) and a
+ * reciprocal for n < 0, as an integer power is evaluated. */
+static double pow10i(int n) {
+  double p = 1.0;
+  for (int i = 0; i < (n < 0 ? -n : n); ++i) p = p * 10.0;
+  return n < 0 ? 1.0 / p : p;
+}
+
 /* pr_SetInitialGuess, :2162-2448 */
 static double nr_initial_guess(double dt, double v, double v1, double v2,
                                double dx, double dI, double divD, double dB) {
@@ -825,7 +833,7 @@ static double nr_initial_guess(double dt, double v, double v1, double v2,
   gcount = 0;
   while ((gprim > 1) && (gcount < 7)) {
     gcount = gcount + 1;
-    dtfraction = pow(10.0, (double)(-8 + gcount));
+    dtfraction = pow10i(-8 + gcount);
     dt0 = dt * dtfraction;
     nr_vars(dt0, v, v1, v2, dx, dI, divD, dB, &f0, &f1, &f2, &f3);
     gprim = conv_g(f0, f1, f2);
@@ -837,7 +845,7 @@ static double nr_initial_guess(double dt, double v, double v1, double v2,
   if ((gprim < 1) && (gcount == 1)) {
     dtoldfraction = 1e-8;
   } else if (gprim < 1) {
-    dtoldfraction = pow(10.0, (double)(-8 + gcount - 1));
+    dtoldfraction = pow10i(-8 + gcount - 1);
   } else {
     dtfraction = mindtfrac;
     dtoldfraction = 0.5 * dtfraction / 10.0;
@@ -1361,7 +1369,11 @@ static void track_cell(tsc *T, cbuf *center, cbuf nb[18], double lx,
 }
 
 /* pr_RWPTrackPath (ParticleTrackingEngine.f90:1051-1254) mapped to the
- * particle status of MPathRW.F90:1337-1403.  Returns the new status. */
+ * particle status of MPathRW.F90:1337-1403.  Returns the new status; the
+ * location written back is the last pathline point the reference records
+ * (its pCoordLast): the exit point for a boundary exit, the final point at
+ * the stopping time, the cell-entry location for statuses 3-7, and for an
+ * undefined result the last point recorded before it. */
 static int track_path(const orc_problem *P, tsc *T, cbuf *center,
                       cbuf nb[18], int *cell, double *lx, double *ly,
                       double *lz, double *t, double t_stop,
@@ -1378,32 +1390,28 @@ static int track_path(const orc_problem *P, tsc *T, cbuf *center,
     cellresult R;
     track_cell(T, center, nb, x, y, z, tt, t_stop, &R, nsteps);
     switch (R.status) {
-      case TC_UNDEFINED:
-        return 9;
       case TC_EXIT_CELLFACE: {
-        /* the final location is the exit point in the current cell */
         *cell = center->cell;
         *lx = R.x; *ly = R.y; *lz = R.z; *t = R.t;
         int next = R.next;
         if (next > 0 && ibound_of(P, next) != 0) {
-          /* ConvertFromNeighbor, ModflowRectangularGrid.f90:366-477 */
+          /* ConvertFromNeighbor, ModflowRectangularGrid.f90:366-477: the
+           * face coordinate must sit on the shared face and flips */
           double fx = R.x, fy = R.y, fz = R.z;
-          int f = R.exitFace, bad = 0;
-          switch (f) { /* face of the new cell = opposite of exit face */
-            case 2: bad = fx != 1.0; fx = 0.0; break;
+          int bad = 0;
+          switch (R.exitFace) {
             case 1: bad = fx != 0.0; fx = 1.0; break;
-            case 4: bad = fy != 1.0; fy = 0.0; break;
+            case 2: bad = fx != 1.0; fx = 0.0; break;
             case 3: bad = fy != 0.0; fy = 1.0; break;
-            case 6: bad = fz != 1.0; fz = 0.0; break;
+            case 4: bad = fy != 1.0; fy = 0.0; break;
             case 5: bad = fz != 0.0; fz = 1.0; break;
+            case 6: bad = fz != 1.0; fz = 0.0; break;
             default: bad = 1;
           }
           if (bad) return 9; /* the reference has no defined continuation */
           loc_cell = next;
           x = fx; y = fy; z = fz;
           tt = R.t;
-          *cell = loc_cell;
-          *lx = x; *ly = y; *lz = z; *t = tt;
         } else {
           return 2;
         }
@@ -1414,16 +1422,18 @@ static int track_path(const orc_problem *P, tsc *T, cbuf *center,
         *lx = R.x; *ly = R.y; *lz = R.z; *t = R.t;
         return 1;
       case TC_NOEXIT:
-        return 5;
       case TC_STOPZONE:
-        return 6;
       case TC_WEAKSINK:
-        return 3;
       case TC_WEAKSOURCE:
-        return 4;
       case TC_INACTIVE:
-        return 7;
-      default:
+        *cell = loc_cell;
+        *lx = x; *ly = y; *lz = z; *t = tt;
+        return R.status == TC_NOEXIT     ? 5
+               : R.status == TC_STOPZONE ? 6
+               : R.status == TC_WEAKSINK ? 3
+               : R.status == TC_WEAKSOURCE ? 4
+                                         : 7;
+      default: /* Undefined and InvalidLocation */
         return 9;
     }
   }
