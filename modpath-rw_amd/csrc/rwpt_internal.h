@@ -91,7 +91,6 @@ struct WalkParams {
   int32_t disp_model, backward, stop_zone, stop_weak_sinks, stop_weak_sources;
   int32_t steady_state;
   double t_engine, t_stop;
-  int64_t max_steps;  // per particle and launch
 };
 
 }  // namespace rwpt

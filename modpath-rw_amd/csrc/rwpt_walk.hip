@@ -50,6 +50,14 @@ __device__ __forceinline__ double draw_normal(Stream &s) {
 
 __device__ __forceinline__ double f32lit(float v) { return (double)v; }
 
+// Guards against walks the reference never ends (not in the reference):
+// sub-steps per particle and launch, and consecutive cell exits made with a
+// zero time step (a particle on a face whose drift points into the face from
+// both sides, with no random component across it, ping-pongs between the two
+// cells forever in pr_RWPTrackPath).  Either reports status 9.
+constexpr int64_t kMaxSteps = 20000000;
+constexpr int kMaxStalls = 1000;
+
 // 10d0**n for an integer n, evaluated as an integer power (exact products,
 // reciprocal for n < 0).
 __device__ __forceinline__ double pow10i(int n) {
@@ -466,7 +474,8 @@ enum : int { OUT_UNDEFINED = 0, OUT_EXIT = 2, OUT_MAXTIME = 3,
 __device__ int visit_cell(const double *__restrict__ rec, const WalkParams &W,
                           bool stopIfNoExit, double ix, double iy, double iz,
                           double t0, Stream &rng, int64_t &steps, double &fx,
-                          double &fy, double &fz, double &ft, int &face) {
+                          double &fy, double &fz, double &ft, int &face,
+                          bool &zero_dt) {
   const int32_t flags = reinterpret_cast<const int32_t *>(rec + F_FLAGS)[1];
   if (stopIfNoExit && !(flags & CF_SUBCELL_EXIT)) return OUT_NOEXIT;
   if (flags & CF_DRY) return OUT_INACTIVE;
@@ -481,10 +490,10 @@ __device__ int visit_cell(const double *__restrict__ rec, const WalkParams &W,
   bool atMax = false;
   StepTerms s;
   for (;;) {
+    // guard against a walk that never ends (not in the reference): the
+    // particle is reported undefined instead of holding the GPU
+    if (steps >= kMaxSteps) return OUT_UNDEFINED;
     steps += 1;
-    // safety valve against a non-terminating walk (not in the reference):
-    // the particle is reported undefined instead of holding the GPU
-    if (steps > W.max_steps) return OUT_UNDEFINED;
     dt = dtcell;
     t = t + dt;
     if (W.t_stop < t) {
@@ -628,6 +637,7 @@ __device__ int visit_cell(const double *__restrict__ rec, const WalkParams &W,
     if (exitFace > 0) {
       fx = nx; fy = ny; fz = nz; ft = t;
       face = exitFace;
+      zero_dt = dt == 0.0;
       return OUT_EXIT;
     }
     x = nx;
@@ -643,7 +653,7 @@ __device__ int track_path(const double *__restrict__ cells,
                           const WalkParams &W, int &cell, double &lx,
                           double &ly, double &lz, double &lt, Stream &rng,
                           int64_t &steps) {
-  int c = cell;
+  int c = cell, stalls = 0;
   double x = lx, y = ly, z = lz, t = lt;
   for (;;) {
     const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
@@ -681,8 +691,9 @@ __device__ int track_path(const double *__restrict__ cells,
       }
       double fx, fy, fz, ft;
       int face = 0;
+      bool zero_dt = false;
       const int out = visit_cell(rec, W, stopIfNoExit, x, y, z, t, rng, steps,
-                                 fx, fy, fz, ft, face);
+                                 fx, fy, fz, ft, face, zero_dt);
       if (out == OUT_MAXTIME) {
         cell = c; lx = fx; ly = fy; lz = fz; lt = ft;
         return 1;
@@ -690,6 +701,8 @@ __device__ int track_path(const double *__restrict__ cells,
       if (out == OUT_EXIT) {
         // exit point in this cell is the last recorded point
         cell = c; lx = fx; ly = fy; lz = fz; lt = ft;
+        stalls = zero_dt ? stalls + 1 : 0;
+        if (stalls > kMaxStalls) return 9;
         const int next = reinterpret_cast<const int32_t *>(rec + F_CONN)[face - 1];
         if (next <= 0) return 2;
         const double *nrec = cells + (int64_t)(next - 1) * CELL_DOUBLES;
@@ -899,7 +912,6 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   W.steady_state = e->opt.steady_state;
   W.t_engine = e->t_engine;
   W.t_stop = t_stop;
-  W.max_steps = 100000000;
   hipError_t err = hipMemsetAsync(e->counters, 0, 4 * sizeof(unsigned long long),
                                   e->stream);
   if (err != hipSuccess) return err;

@@ -61,6 +61,10 @@ def lib():
                                      _dp, _dp, _dp, _dp, _dp]
         L.orc_histogram.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp,
                                     _ip, _dp]
+        L.orc_newton_axis.argtypes = [C.c_double] * 8
+        L.orc_newton_axis.restype = C.c_double
+        L.orc_quadratic_axis.argtypes = [C.c_double] * 7 + [_ip]
+        L.orc_quadratic_axis.restype = C.c_double
         _LIB = L
     return _LIB
 

@@ -1037,9 +1037,18 @@ enum { TC_UNDEFINED = 0, TC_EXIT_CELLFACE = 1, TC_STOPTIME = 2,
        TC_WEAKSINK = 7, TC_WEAKSOURCE = 8 };
 
 typedef struct {
-  int status, exitFace, conn;
+  int status, exitFace, conn, zero_dt;
   double x, y, z, t;
 } subresult;
+
+/* Guards against walks the reference never ends (not in the reference):
+ * sub-steps per particle and advance call, and consecutive cell exits made
+ * with a zero time step (a particle sitting on a face whose drift points
+ * into the face from both sides, with no random component across it,
+ * ping-pongs between the two cells forever in pr_RWPTrackPath).  Either
+ * reports status 9. */
+#define ORC_MAX_STEPS 20000000LL
+#define ORC_MAX_STALLS 1000
 
 /* pr_ExecuteRandomWalkParticleTracking, TrackSubCell.F90:812-1373 */
 static void track_subcell(tsc *T, int stopIfNoExit, double ix, double iy,
@@ -1059,6 +1068,7 @@ static void track_subcell(tsc *T, int stopIfNoExit, double ix, double iy,
   R->status = SC_UNDEFINED;
   R->exitFace = 0;
   R->conn = 0;
+  R->zero_dt = 0;
   R->x = ix;
   R->y = iy;
   R->z = iz;
@@ -1085,6 +1095,11 @@ static void track_subcell(tsc *T, int stopIfNoExit, double ix, double iy,
   exitFace = 0;
   reachedMaximumTime = 0;
   for (;;) {
+    /* guard against a walk that never ends (not in the reference) */
+    if (nsteps && *nsteps >= ORC_MAX_STEPS) {
+      R->status = SC_UNDEFINED;
+      return;
+    }
     if (nsteps) *nsteps += 1;
     dt = dtcell;
     t = t + dt;
@@ -1268,6 +1283,7 @@ static void track_subcell(tsc *T, int stopIfNoExit, double ix, double iy,
       R->conn = s->conn[exitFace - 1];
       R->status = R->conn < 0 ? SC_EXIT_INTERNAL : SC_EXIT_CELLFACE;
       R->exitFace = exitFace;
+      R->zero_dt = dt == 0.0;
       R->x = nx; R->y = ny; R->z = nz; R->t = t;
       return;
     }
@@ -1282,7 +1298,7 @@ static void track_subcell(tsc *T, int stopIfNoExit, double ix, double iy,
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-  int status, exitFace, next;
+  int status, exitFace, next, zero_dt;
   double x, y, z, t;
 } cellresult;
 
@@ -1295,6 +1311,7 @@ static void track_cell(tsc *T, cbuf *center, cbuf nb[18], double lx,
   R->status = TC_UNDEFINED;
   R->exitFace = 0;
   R->next = 0;
+  R->zero_dt = 0;
   R->x = lx; R->y = ly; R->z = lz; R->t = lt;
   if ((lx < 0.0) || (lx > 1.0) || (ly < 0.0) || (ly > 1.0) || (lz < 0.0) ||
       (lz > 1.0)) {
@@ -1349,6 +1366,7 @@ static void track_cell(tsc *T, cbuf *center, cbuf nb[18], double lx,
       R->status = TC_EXIT_CELLFACE;
       R->exitFace = S.exitFace;
       R->next = S.conn;
+      R->zero_dt = S.zero_dt;
       R->x = S.x; R->y = S.y; R->z = S.z; R->t = S.t;
       return;
     case SC_MAXTIME:
@@ -1378,7 +1396,7 @@ static int track_path(const orc_problem *P, tsc *T, cbuf *center,
                       cbuf nb[18], int *cell, double *lx, double *ly,
                       double *lz, double *t, double t_stop,
                       long long *nsteps) {
-  int loc_cell = *cell;
+  int loc_cell = *cell, stalls = 0;
   double x = *lx, y = *ly, z = *lz, tt = *t;
   fill_cell(P, loc_cell, center);
   fill_neighbors(P, center, nb);
@@ -1393,6 +1411,8 @@ static int track_path(const orc_problem *P, tsc *T, cbuf *center,
       case TC_EXIT_CELLFACE: {
         *cell = center->cell;
         *lx = R.x; *ly = R.y; *lz = R.z; *t = R.t;
+        stalls = R.zero_dt ? stalls + 1 : 0;
+        if (stalls > ORC_MAX_STALLS) return 9;
         int next = R.next;
         if (next > 0 && ibound_of(P, next) != 0) {
           /* ConvertFromNeighbor, ModflowRectangularGrid.f90:366-477: the
@@ -1608,3 +1628,22 @@ void orc_histogram(long n, const double *x, const double *y, const double *z,
     counts[idx] = counts[idx] + (w ? w[p] : 1.0);
   }
 }
+
+/* Direct entry points to the exit-time solvers, for root checks. */
+double orc_newton_axis(double dt, double v, double v1, double v2, double dx,
+                       double dI, double divD, double dB) {
+  return nr_time_step(dt, v, v1, v2, dx, dI, divD, dB);
+}
+
+double orc_quadratic_axis(double dt, double n, double p, double d, double v,
+                          double divD, double dB, int *exit_face) {
+  double dtaxis = 0.0, dtout = dt;
+  int ef = 0;
+  if (quad_axis(n, p, d, v, divD, dB, 1, 2, dt, &dtaxis, &ef, &dtout)) {
+    *exit_face = ef;
+    return 0.0;
+  }
+  *exit_face = 0;
+  return dtaxis;
+}
+

@@ -88,6 +88,13 @@ void orc_histogram(long n, const double *x, const double *y, const double *z,
                    const double binsize[3], const int nbins[3],
                    double *counts);
 
+/* exit-time solvers for one axis (Newton: TrackSubCell.F90:2451-2554;
+ * quadratic: one axis of :1576-1892, exit faces 1/2) */
+double orc_newton_axis(double dt, double v, double v1, double v2, double dx,
+                       double dI, double divD, double dB);
+double orc_quadratic_axis(double dt, double n, double p, double d, double v,
+                          double divD, double dB, int *exit_face);
+
 #ifdef __cplusplus
 }
 #endif
