@@ -107,9 +107,11 @@ def main():
     bytes_per_launch = BYTES_PER_STEP_F64 * total_steps / max(launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "rw_advance_f64_config1_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "walk_kernel_config1_traffic.txt")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        for line in open(tpath):
+            if line.startswith("hbm_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
     out = {
         "metric": "particle-steps/s + KDE cells/s, 1 GPU: FP32/batched vs plain "
                   "FP64 at <=0.5% err",
@@ -136,7 +138,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "kernel": "advance_f64",
+                     "kernel": "walk_kernel",
+                     "launches": launches,
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }

@@ -138,6 +138,17 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
  * `time` of MPathRW.F90:1256-1262 used for the release test). */
 int rwpt_set_time(rwpt_engine *e, double t);
 
+/* Launch shape of the walk (no counterpart in the reference: the
+ * reference walks each particle to t_stop in one call).  budget = RW
+ * iterations per particle per launch (the loop state is saved between
+ * launches, so results do not depend on it), block = workgroup size,
+ * sort = re-sort unfinished particles by cell between launches.
+ * Defaults 64, 256, 1. */
+int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
+                    int32_t sort);
+/* Walk launches the last rwpt_advance used. */
+int rwpt_last_launches(const rwpt_engine *e);
+
 /* Cumulative RW sub-steps per particle since rwpt_load_particles. */
 int rwpt_get_substeps(rwpt_engine *e, int64_t *substeps);
 

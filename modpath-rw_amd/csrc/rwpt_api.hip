@@ -60,6 +60,18 @@ void free_particles(Particles &P) {
   dev_free(P.t);
   dev_free(P.rng);
   dev_free(P.substeps);
+  dev_free(P.aux);
+  dev_free(P.ix);
+  dev_free(P.iy);
+  dev_free(P.iz);
+  dev_free(P.t0);
+  for (int k = 0; k < 2; ++k) {
+    dev_free(P.keys[k]);
+    dev_free(P.vals[k]);
+  }
+  if (P.sort_tmp) (void)hipFree(P.sort_tmp);
+  P.sort_tmp = nullptr;
+  P.sort_tmp_bytes = 0;
   P.n = P.cap = 0;
 }
 
@@ -174,6 +186,7 @@ int rwpt_destroy(rwpt_engine *e) {
     if (T.start) (void)hipEventDestroy(T.start);
     if (T.stop) (void)hipEventDestroy(T.stop);
   }
+  for (auto ev : e->walk_events) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return RWPT_OK;
@@ -294,11 +307,21 @@ int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
     if (err == hipSuccess) err = dev_alloc(&P.t, n);
     if (err == hipSuccess) err = dev_alloc(&P.rng, 2 * n);
     if (err == hipSuccess) err = dev_alloc(&P.substeps, n);
+    if (err == hipSuccess) err = dev_alloc(&P.aux, n);
+    if (err == hipSuccess) err = dev_alloc(&P.ix, n);
+    if (err == hipSuccess) err = dev_alloc(&P.iy, n);
+    if (err == hipSuccess) err = dev_alloc(&P.iz, n);
+    if (err == hipSuccess) err = dev_alloc(&P.t0, n);
+    for (int k = 0; k < 2 && err == hipSuccess; ++k) {
+      err = dev_alloc(&P.keys[k], n);
+      if (err == hipSuccess) err = dev_alloc(&P.vals[k], n);
+    }
+    P.cap = n;
+    if (err == hipSuccess) err = ensure_sort_storage(e);
     if (err != hipSuccess) {
       free_particles(P);
       return hip_fail(e, err, "load_particles alloc");
     }
-    P.cap = n;
   }
   P.n = n;
   if (n == 0) return RWPT_OK;
@@ -364,9 +387,9 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
   err = hipMemcpyAsync(h, e->counters, sizeof h, hipMemcpyDeviceToHost, e->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return hip_fail(e, err, "advance");
-  if (e->part.n > 0) {
+  for (int64_t k = 0; k < e->last_launches; ++k) {
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e->timers[0].start, e->timers[0].stop);
+    (void)hipEventElapsedTime(&ms, e->walk_events[2 * k], e->walk_events[2 * k + 1]);
     e->timers[0].total_ms += ms;
     e->timers[0].launches += 1;
   }
@@ -374,6 +397,22 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
   if (n_active) *n_active = (int64_t)h[0];
   if (n_substeps) *n_substeps = (int64_t)h[1];
   return RWPT_OK;
+}
+
+int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
+                    int32_t sort) {
+  if (!e) return RWPT_E_ARG;
+  if (budget < 1 || (block != 64 && block != 128 && block != 256 &&
+                     block != 512 && block != 1024))
+    return fail(e, RWPT_E_ARG, "budget >= 1, block one of 64..1024");
+  e->tune.budget = budget;
+  e->tune.block = block;
+  e->tune.sort = sort ? 1 : 0;
+  return RWPT_OK;
+}
+
+int rwpt_last_launches(const rwpt_engine *e) {
+  return e ? (int)e->last_launches : -1;
 }
 
 int rwpt_set_time(rwpt_engine *e, double t) {

@@ -84,6 +84,19 @@ struct Particles {
   double *lx = nullptr, *ly = nullptr, *lz = nullptr, *t = nullptr;
   uint64_t *rng = nullptr;
   int64_t *substeps = nullptr;
+  // resumable walk state (see rwpt_walk.hip)
+  int32_t *aux = nullptr;
+  double *ix = nullptr, *iy = nullptr, *iz = nullptr, *t0 = nullptr;
+  // work lists sorted by cell between launches
+  int32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+};
+
+struct Tuning {
+  int32_t budget = 64;    // time-loop iterations per particle and launch
+  int32_t block = 256;    // threads per workgroup of the walk
+  int32_t sort = 1;       // sort the work list by cell between launches
 };
 
 struct WalkParams {
@@ -108,6 +121,9 @@ struct rwpt_engine {
   double *kde_scratch = nullptr;
   int64_t kde_scratch_size = 0;
   rwpt::KernelTimer timers[3];
+  rwpt::Tuning tune;
+  std::vector<hipEvent_t> walk_events;  // start/stop pairs per launch
+  int64_t last_launches = 0;
   std::string err;
 };
 
@@ -122,6 +138,8 @@ hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
 hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                       const double *mass_dev, double *density_dev,
                       double *hist_dev, double *stats_host);
+
+hipError_t ensure_sort_storage(rwpt_engine *e);
 
 // host helper: GF(2) matrices of the xoroshiro128+ jump, J^(2^k)
 void jump_power_matrices(int kmax, std::vector<uint64_t> &out);

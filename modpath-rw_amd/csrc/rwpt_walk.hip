@@ -13,6 +13,10 @@
 // solve, restarts and elastic rebound.  Floating-point expressions keep the
 // reference operand order and the file is built with -ffp-contract=off, so
 // with the EULERIAN solver every result is the double the reference forms.
+#include <algorithm>
+
+#include <hipcub/hipcub.hpp>
+
 #include "rwpt_internal.h"
 
 namespace rwpt {
@@ -464,251 +468,413 @@ __device__ __forceinline__ bool outside(double x, double y, double z) {
          (z < 0.0);
 }
 
-// outcomes of one cell visit (TrackSubCellResult status values)
-enum : int { OUT_UNDEFINED = 0, OUT_EXIT = 2, OUT_MAXTIME = 3,
-             OUT_NOEXIT = 4, OUT_INACTIVE = 7 };
+// outcome of one time-loop iteration of a cell visit
+enum : int { IT_CONTINUE = 0, IT_EXIT = 2, IT_MAXTIME = 3, IT_UNDEFINED = 4 };
 
-// pr_ExecuteRandomWalkParticleTracking (TrackSubCell.F90:812-1373) for one
-// cell visit starting at (ix, iy, iz, t0).  On OUT_EXIT/OUT_MAXTIME the
-// final location is returned in (fx, fy, fz, ft) and the face in face.
-__device__ int visit_cell(const double *__restrict__ rec, const WalkParams &W,
-                          bool stopIfNoExit, double ix, double iy, double iz,
-                          double t0, Stream &rng, int64_t &steps, double &fx,
-                          double &fy, double &fz, double &ft, int &face,
-                          bool &zero_dt) {
-  const int32_t flags = reinterpret_cast<const int32_t *>(rec + F_FLAGS)[1];
-  if (stopIfNoExit && !(flags & CF_SUBCELL_EXIT)) return OUT_NOEXIT;
-  if (flags & CF_DRY) return OUT_INACTIVE;
+// Persistent loop state of a cell visit: everything
+// pr_ExecuteRandomWalkParticleTracking (TrackSubCell.F90:812-1373) carries
+// from one iteration of its time loop to the next, so a visit can be paused
+// between iterations and resumed in a later launch with identical results.
+struct Visit {
+  double x, y, z, t;      // current location and time
+  double ix, iy, iz, t0;  // visit start (initialLocation, tinit)
+  int restarts;           // posRestartCounter
+  int exitFace;           // exitFace (kept across iterations)
+};
+
+// One iteration of the time loop (TrackSubCell.F90:956-1368).  On IT_EXIT
+// and IT_MAXTIME the final point is left in V.x..V.t; *zero_dt reports an
+// exit taken with a zero time step.
+__device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
+                       Visit &V, Stream &rng, bool &zero_dt) {
   const double dtcell = rec[F_DTCELL];
-  if (dtcell <= 0.0) return OUT_UNDEFINED;
   const double maxRelJump = 0.5, dtRed = f32lit(0.1f);
   const double DX = rec[F_DX], DY = rec[F_DY], DZ = rec[F_DZ];
   const int32_t *mb = reinterpret_cast<const int32_t *>(rec + F_MASSBND);
-  double x = ix, y = iy, z = iz, nx = ix, ny = iy, nz = iz;
-  double t = t0, dt;
-  int exitFace = 0, restarts = 0;
+  double x = V.x, y = V.y, z = V.z, t = V.t;
+  double nx, ny, nz, dt;
+  int exitFace = V.exitFace;
   bool atMax = false;
   StepTerms s;
-  for (;;) {
-    // guard against a walk that never ends (not in the reference): the
-    // particle is reported undefined instead of holding the GPU
-    if (steps >= kMaxSteps) return OUT_UNDEFINED;
-    steps += 1;
-    dt = dtcell;
+  dt = dtcell;
+  t = t + dt;
+  if (W.t_stop < t) {
+    t = t - dt;
+    dt = fmax(W.t_stop - t, 0.0);
+    t = W.t_stop;
+    atMax = true;
+  }
+  displacements(rec, W, x, y, z, dt, rng, s);
+  double sq = sqrt(dt);
+  double dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+  double dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+  double dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+  // jump limiter: same draw, dt cut by 0.1 until each jump <= half a cell
+  while ((fabs(dxr / DX) > maxRelJump) || (fabs(dyr / DY) > maxRelJump) ||
+         (fabs(dzr / DZ) > maxRelJump)) {
+    t = t - dt;
+    dt = dtRed * dt;
+    advect(rec, W.advection_kind, dt, s);
+    sq = sqrt(dt);
+    dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+    dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+    dzr = s.dAz + s.ddz * dt + s.dBz * sq;
     t = t + dt;
-    if (W.t_stop < t) {
-      t = t - dt;
-      dt = fmax(W.t_stop - t, 0.0);
-      t = W.t_stop;
-      atMax = true;
+    atMax = false;
+  }
+  nx = x + dxr / DX;
+  ny = y + dyr / DY;
+  nz = z + dzr / DZ;
+  int loops = 0;
+  while (outside(nx, ny, nz)) {
+    loops = loops + 1;
+    exit_time(rec, W.advection_kind, x, y, z, nx, ny, nz, s, t, dt, exitFace);
+    if (loops > 5) {  // maxInterfaceLoopCounter: restart the visit
+      nx = V.ix; ny = V.iy; nz = V.iz;
+      t = V.t0;
+      dt = dtcell;
+      V.restarts = V.restarts + 1;
+      if (V.restarts > 10) return IT_UNDEFINED;
+      break;
     }
-    displacements(rec, W, x, y, z, dt, rng, s);
-    double sq = sqrt(dt);
-    double dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-    double dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-    double dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-    // jump limiter: same draw, dt cut by 0.1 until each jump <= half a cell
-    while ((fabs(dxr / DX) > maxRelJump) || (fabs(dyr / DY) > maxRelJump) ||
-           (fabs(dzr / DZ) > maxRelJump)) {
-      t = t - dt;
-      dt = dtRed * dt;
-      advect(rec, W.advection_kind, dt, s);
-      sq = sqrt(dt);
-      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+    advect(rec, W.advection_kind, dt, s);
+    atMax = false;
+    if (exitFace == 0) {  // no face found: restart with new draws
+      nx = V.ix; ny = V.iy; nz = V.iz;
+      t = V.t0;
+      dt = dtcell;
+      V.restarts = V.restarts + 1;
+      if (V.restarts > 10) return IT_UNDEFINED;
+      break;
+    }
+    sq = sqrt(dt);
+    if (exitFace <= 2) {
+      nx = exitFace == 1 ? 0.0 : 1.0;
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      ny = y + dyr / DY;
       dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      nz = z + dzr / DZ;
+    } else if (exitFace <= 4) {
+      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      nx = x + dxr / DX;
+      ny = exitFace == 3 ? 0.0 : 1.0;
+      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      nz = z + dzr / DZ;
+    } else {
+      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      nx = x + dxr / DX;
+      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      ny = y + dyr / DY;
+      nz = exitFace == 5 ? 0.0 : 1.0;
+    }
+    if (outside(nx, ny, nz)) continue;
+    // elastic rebound on faces whose mass boundary is 1
+    int rebounds = 0;
+    while (exitFace > 0) {
+      if (mb[exitFace - 1] != 1) break;
+      rebounds = rebounds + 1;
+      if (rebounds > 10) return IT_UNDEFINED;
+      const double xi = nx, yi = ny, zi = nz;
+      if (dt == 0.0) {
+        nx = V.ix; ny = V.iy; nz = V.iz;
+        t = t - dt;
+        dt = dtcell;
+        exitFace = 0;
+        V.restarts = V.restarts + 1;
+        break;
+      }
       t = t + dt;
-      atMax = false;
-    }
-    nx = x + dxr / DX;
-    ny = y + dyr / DY;
-    nz = z + dzr / DZ;
-    int loops = 0;
-    while (outside(nx, ny, nz)) {
-      loops = loops + 1;
-      exit_time(rec, W.advection_kind, x, y, z, nx, ny, nz, s, t, dt,
-                exitFace);
-      if (loops > 5) {  // maxInterfaceLoopCounter: restart the visit
-        nx = ix; ny = iy; nz = iz;
-        t = t0;
-        dt = dtcell;
-        restarts = restarts + 1;
-        if (restarts > 10) return OUT_UNDEFINED;
-        break;
+      if (W.t_stop < t) {
+        t = t - dt;
+        dt = fmax(W.t_stop - t, 0.0);
+        t = W.t_stop;
+        atMax = true;
+        advect(rec, W.advection_kind, dt, s);
+        sq = sqrt(dt);
+        dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+        dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+        dzr = s.dAz + s.ddz * dt + s.dBz * sq;
       }
-      advect(rec, W.advection_kind, dt, s);
-      atMax = false;
-      if (exitFace == 0) {  // no face found: restart with new draws
-        nx = ix; ny = iy; nz = iz;
-        t = t0;
-        dt = dtcell;
-        restarts = restarts + 1;
-        if (restarts > 10) return OUT_UNDEFINED;
-        break;
-      }
-      sq = sqrt(dt);
       if (exitFace <= 2) {
-        nx = exitFace == 1 ? 0.0 : 1.0;
-        dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-        ny = y + dyr / DY;
-        dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-        nz = z + dzr / DZ;
+        nx = nx - dxr / DX;
+        ny = ny + dyr / DY;
+        nz = nz + dzr / DZ;
       } else if (exitFace <= 4) {
-        dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-        nx = x + dxr / DX;
-        ny = exitFace == 3 ? 0.0 : 1.0;
-        dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-        nz = z + dzr / DZ;
+        nx = nx + dxr / DX;
+        ny = ny - dyr / DY;
+        nz = nz + dzr / DZ;
       } else {
-        dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-        nx = x + dxr / DX;
-        dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-        ny = y + dyr / DY;
-        nz = exitFace == 5 ? 0.0 : 1.0;
+        nx = nx + dxr / DX;
+        ny = ny + dyr / DY;
+        nz = nz - dzr / DZ;
       }
-      if (outside(nx, ny, nz)) continue;
-      // elastic rebound on faces whose mass boundary is 1
-      int rebounds = 0;
-      while (exitFace > 0) {
-        if (mb[exitFace - 1] != 1) break;
-        rebounds = rebounds + 1;
-        if (rebounds > 10) return OUT_UNDEFINED;
-        const double xi = nx, yi = ny, zi = nz;
-        if (dt == 0.0) {
-          nx = ix; ny = iy; nz = iz;
-          t = t - dt;
-          dt = dtcell;
-          exitFace = 0;
-          restarts = restarts + 1;
-          break;
-        }
-        t = t + dt;
-        if (W.t_stop < t) {
-          t = t - dt;
-          dt = fmax(W.t_stop - t, 0.0);
-          t = W.t_stop;
-          atMax = true;
-          advect(rec, W.advection_kind, dt, s);
-          sq = sqrt(dt);
-          dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-          dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-          dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-        }
+      if (outside(nx, ny, nz)) {
+        x = xi; y = yi; z = zi;
         if (exitFace <= 2) {
-          nx = nx - dxr / DX;
-          ny = ny + dyr / DY;
-          nz = nz + dzr / DZ;
+          s.vx = -s.vx; s.ddx = -s.ddx; s.dBx = -s.dBx;
         } else if (exitFace <= 4) {
-          nx = nx + dxr / DX;
-          ny = ny - dyr / DY;
-          nz = nz + dzr / DZ;
+          s.vy = -s.vy; s.ddy = -s.ddy; s.dBy = -s.dBy;
         } else {
-          nx = nx + dxr / DX;
-          ny = ny + dyr / DY;
-          nz = nz - dzr / DZ;
+          s.vz = -s.vz; s.ddz = -s.ddz; s.dBz = -s.dBz;
         }
-        if (outside(nx, ny, nz)) {
-          x = xi; y = yi; z = zi;
-          if (exitFace <= 2) {
-            s.vx = -s.vx; s.ddx = -s.ddx; s.dBx = -s.dBx;
-          } else if (exitFace <= 4) {
-            s.vy = -s.vy; s.ddy = -s.ddy; s.dBy = -s.dBy;
-          } else {
-            s.vz = -s.vz; s.ddz = -s.ddz; s.dBz = -s.dBz;
-          }
-          exitFace = 0;
-        } else {
-          dt = dtcell;
-          exitFace = 0;
-          if (nx == 0.0) exitFace = 1;
-          if (nx == 1.0) exitFace = 2;
-          if (ny == 0.0) exitFace = 3;
-          if (ny == 1.0) exitFace = 4;
-          if (nz == 0.0) exitFace = 5;
-          if (nz == 1.0) exitFace = 6;
-        }
+        exitFace = 0;
+      } else {
+        dt = dtcell;
+        exitFace = 0;
+        if (nx == 0.0) exitFace = 1;
+        if (nx == 1.0) exitFace = 2;
+        if (ny == 0.0) exitFace = 3;
+        if (ny == 1.0) exitFace = 4;
+        if (nz == 0.0) exitFace = 5;
+        if (nz == 1.0) exitFace = 6;
       }
     }
-    if (atMax) {
-      fx = nx; fy = ny; fz = nz; ft = t;
-      face = exitFace;
-      return OUT_MAXTIME;
+  }
+  V.x = nx; V.y = ny; V.z = nz; V.t = t;
+  V.exitFace = exitFace;
+  if (atMax) return IT_MAXTIME;
+  if (exitFace > 0) {
+    zero_dt = dt == 0.0;
+    return IT_EXIT;
+  }
+  return IT_CONTINUE;
+}
+
+// Checks TrackCell makes before tracking (TrackCell.f90:375-510) and the
+// early returns of TrackSubCell (:870-945).  Returns 0 to track, else the
+// particle status to report at the cell-entry location.
+__device__ __forceinline__ int entry_checks(const double *__restrict__ rec,
+                                            const WalkParams &W, double x,
+                                            double y, double z) {
+  if (outside(x, y, z)) return 9;  // InvalidLocation -> Undefined
+  const int32_t *fl = reinterpret_cast<const int32_t *>(rec + F_FLAGS);
+  if (fl[0] == 0) return 7;
+  const int32_t zone = reinterpret_cast<const int32_t *>(rec + F_ZONE)[0];
+  if (W.stop_zone > 0 && zone == W.stop_zone) return 6;
+  bool stopIfNoExit = true;
+  const double src = rec[F_SOURCE], snk = rec[F_SINK];
+  if (W.backward) {
+    if (W.stop_weak_sources) {
+      if (src != 0.0) return 4;
+      if (!W.steady_state) stopIfNoExit = false;
     }
-    if (exitFace > 0) {
-      fx = nx; fy = ny; fz = nz; ft = t;
-      face = exitFace;
-      zero_dt = dt == 0.0;
-      return OUT_EXIT;
-    }
-    x = nx;
-    y = ny;
-    z = nz;
+  } else if (W.stop_weak_sinks) {
+    if (snk != 0.0) return 3;
+    if (!W.steady_state) stopIfNoExit = false;
+  }
+  if (!(fl[1] & CF_CELL_EXIT)) {
+    if (W.steady_state || (W.backward && src != 0.0) ||
+        (!W.backward && snk != 0.0))
+      return 5;
+  }
+  if (stopIfNoExit && !(fl[1] & CF_SUBCELL_EXIT)) return 5;
+  if (fl[1] & CF_DRY) return 7;
+  if (rec[F_DTCELL] <= 0.0) return 9;
+  return 0;
+}
+
+// Packed small per-particle loop counters.
+struct Aux {
+  int phase, restarts, exitFace, entryFace, stalls;
+};
+__device__ __forceinline__ Aux unpack_aux(int32_t a) {
+  return Aux{a & 1, (a >> 1) & 15, (a >> 5) & 7, (a >> 8) & 7, (a >> 11) & 2047};
+}
+__device__ __forceinline__ int32_t pack_aux(const Aux &a) {
+  return a.phase | (a.restarts << 1) | (a.exitFace << 5) | (a.entryFace << 8) |
+         (a.stalls << 11);
+}
+
+// The last pathline point pr_RWPTrackPath records before an Undefined
+// result, given the entry location (x, y, z, t) of the current cell: the
+// advance's start location itself, or after a transfer the exit point in
+// the previous cell (the entry point seen from the other side).
+__device__ __forceinline__ void last_point(const double *__restrict__ cells,
+                                           int entryFace, int &c, double &x,
+                                           double &y, double &z) {
+  if (entryFace == 0) return;
+  const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+  c = reinterpret_cast<const int32_t *>(rec + F_CONN)[entryFace - 1];
+  switch (entryFace) {
+    case 1: x = 1.0; break;
+    case 2: x = 0.0; break;
+    case 3: y = 1.0; break;
+    case 4: y = 0.0; break;
+    case 5: z = 1.0; break;
+    default: z = 0.0; break;
   }
 }
 
-// pr_RWPTrackPath + TrackCell (ParticleTrackingEngine.f90:1051-1254,
-// TrackCell.f90:375-669) mapped to MPathRW particle status codes.  The
-// location written back is the last pathline point the reference records.
-__device__ int track_path(const double *__restrict__ cells,
-                          const WalkParams &W, int &cell, double &lx,
-                          double &ly, double &lz, double &lt, Stream &rng,
-                          int64_t &steps) {
-  int c = cell, stalls = 0;
-  double x = lx, y = ly, z = lz, t = lt;
-  for (;;) {
-    const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-    if (outside(x, y, z)) return 9;  // InvalidLocation -> Undefined
-    const int32_t *fl = reinterpret_cast<const int32_t *>(rec + F_FLAGS);
-    // checks before the first tracking point report the entry location
-    if (fl[0] == 0) goto stop_here_7;
-    {
-      const int32_t zone = reinterpret_cast<const int32_t *>(rec + F_ZONE)[0];
-      if (W.stop_zone > 0 && zone == W.stop_zone) {
-        cell = c; lx = x; ly = y; lz = z; lt = t;
-        return 6;
+// Wave-aggregated append of (key, value) to a work list.
+__device__ __forceinline__ void append_work(bool want, int32_t key,
+                                            int32_t val, int32_t *keys,
+                                            int32_t *vals,
+                                            unsigned long long *count) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (want) {
+    const unsigned long long below = mask & ((1ull << lane) - 1ull);
+    const unsigned long long idx = base + __popcll(below);
+    keys[idx] = key;
+    vals[idx] = val;
+  }
+}
+
+// Wave sums of two counters into global memory.
+__device__ __forceinline__ void wave_count(unsigned long long a,
+                                           unsigned long long b,
+                                           unsigned long long *c0,
+                                           unsigned long long *c1) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if (__lane_id() == 0) {
+    if (a) atomicAdd(c0, a);
+    if (b) atomicAdd(c1, b);
+  }
+}
+
+// XCD-aware block order: consecutive chunks of the (cell-sorted) work list
+// go to the same XCD, whose L2 then holds neighbouring cell records
+// (workgroups are dealt to the 8 XCDs round robin).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  const int64_t q = nblocks / 8, r = nblocks % 8;
+  const int64_t x = b % 8, j = b / 8;
+  return x * q + (x < r ? x : r) + j;
+}
+
+struct ParticlePtrs {
+  int32_t *cell, *status, *aux;
+  double *lx, *ly, *lz, *t, *ix, *iy, *iz, *t0;
+  uint64_t *rng;
+  int64_t *substeps;
+};
+
+// Start of an advance: release pending particles (MPathRW.F90:1256-1302),
+// reset the per-call loop state and list the active particles.
+__global__ void prepare_advance(const double *__restrict__ cells, int64_t n,
+                                ParticlePtrs S, const WalkParams W,
+                                int32_t *__restrict__ keys,
+                                int32_t *__restrict__ vals,
+                                unsigned long long *__restrict__ counters) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool track = false;
+  int c = 0;
+  if (p < n) {
+    int st = S.status[p];
+    c = S.cell[p];
+    if (st == 0) {
+      const double t = S.t[p];
+      if (t < W.t_engine) st = 8;
+      else if (t <= W.t_stop) {
+        st = 1;
+        const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+        if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[0] == 0) st = 7;
       }
-      bool stopIfNoExit = true;
-      const double src = rec[F_SOURCE], snk = rec[F_SINK];
-      if (W.backward) {
-        if (W.stop_weak_sources) {
-          if (src != 0.0) {
-            cell = c; lx = x; ly = y; lz = z; lt = t;
-            return 4;
-          } else if (!W.steady_state) stopIfNoExit = false;
+    }
+    if (st == 7) {
+      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+      if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[1] & CF_VERIFY_PDRY)
+        st = 1;
+    }
+    S.status[p] = st;
+    if (st == 1) {
+      S.aux[p] = 0;
+      track = true;
+    }
+  }
+  append_work(track, c, (int32_t)p, keys, vals, &counters[2]);
+}
+
+// One launch of the walk: every listed particle runs up to `budget`
+// time-loop iterations; particles not done are appended to the next list.
+__global__ void __launch_bounds__(256)
+    walk_kernel(const double *__restrict__ cells,
+                const int32_t *__restrict__ work, int64_t nwork, ParticlePtrs S,
+                const WalkParams W, int budget, int32_t *__restrict__ nkeys,
+                int32_t *__restrict__ nvals,
+                unsigned long long *__restrict__ counters) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  bool more = false;
+  int c = 0, p = 0;
+  unsigned long long steps = 0, ended_active = 0;
+  if (i < nwork) {
+    p = work[i];
+    c = S.cell[p];
+    Visit V;
+    V.x = S.lx[p]; V.y = S.ly[p]; V.z = S.lz[p]; V.t = S.t[p];
+    Aux A = unpack_aux(S.aux[p]);
+    if (A.phase) {
+      V.ix = S.ix[p]; V.iy = S.iy[p]; V.iz = S.iz[p]; V.t0 = S.t0[p];
+    } else {
+      V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
+    }
+    V.restarts = A.restarts;
+    V.exitFace = A.exitFace;
+    Stream rng{S.rng[2 * p], S.rng[2 * p + 1]};
+    int status = 1;
+    bool done = false;
+    while (!done) {
+      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+      if (!A.phase) {  // entering a cell: TrackCell checks, new visit
+        const int st = entry_checks(rec, W, V.x, V.y, V.z);
+        if (st != 0) {
+          status = st;
+          if (st == 9)  // InvalidLocation: last recorded point
+            last_point(cells, A.entryFace, c, V.x, V.y, V.z);
+          done = true;
+          break;
         }
-      } else if (W.stop_weak_sinks) {
-        if (snk != 0.0) {
-          cell = c; lx = x; ly = y; lz = z; lt = t;
-          return 3;
-        } else if (!W.steady_state) stopIfNoExit = false;
+        A.phase = 1;
+        V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
+        V.restarts = 0;
+        V.exitFace = 0;
       }
-      if (!(fl[1] & CF_CELL_EXIT)) {
-        if (W.steady_state || (W.backward && src != 0.0) ||
-            (!W.backward && snk != 0.0)) {
-          cell = c; lx = x; ly = y; lz = z; lt = t;
-          return 5;
-        }
+      if ((int)steps >= budget) {
+        more = true;
+        break;
       }
-      double fx, fy, fz, ft;
-      int face = 0;
+      steps += 1;
       bool zero_dt = false;
-      const int out = visit_cell(rec, W, stopIfNoExit, x, y, z, t, rng, steps,
-                                 fx, fy, fz, ft, face, zero_dt);
-      if (out == OUT_MAXTIME) {
-        cell = c; lx = fx; ly = fy; lz = fz; lt = ft;
-        return 1;
-      }
-      if (out == OUT_EXIT) {
-        // exit point in this cell is the last recorded point
-        cell = c; lx = fx; ly = fy; lz = fz; lt = ft;
-        stalls = zero_dt ? stalls + 1 : 0;
-        if (stalls > kMaxStalls) return 9;
+      const int out = iterate(rec, W, V, rng, zero_dt);
+      if (out == IT_CONTINUE) continue;
+      if (out == IT_MAXTIME) {
+        status = 1;
+        done = true;
+      } else if (out == IT_UNDEFINED) {
+        status = 9;
+        V.x = V.ix; V.y = V.iy; V.z = V.iz; V.t = V.t0;
+        last_point(cells, A.entryFace, c, V.x, V.y, V.z);
+        done = true;
+      } else {  // IT_EXIT: the exit point is the last recorded point
+        const int face = V.exitFace;
+        A.stalls = zero_dt ? A.stalls + 1 : 0;
+        if (A.stalls > kMaxStalls) {
+          status = 9;
+          done = true;
+          break;
+        }
         const int next = reinterpret_cast<const int32_t *>(rec + F_CONN)[face - 1];
-        if (next <= 0) return 2;
-        const double *nrec = cells + (int64_t)(next - 1) * CELL_DOUBLES;
-        if (reinterpret_cast<const int32_t *>(nrec + F_FLAGS)[0] == 0) return 2;
+        bool inactive_next = next <= 0;
+        if (!inactive_next) {
+          const double *nrec = cells + (int64_t)(next - 1) * CELL_DOUBLES;
+          inactive_next = reinterpret_cast<const int32_t *>(nrec + F_FLAGS)[0] == 0;
+        }
+        if (inactive_next) {
+          status = 2;
+          done = true;
+          break;
+        }
         // ConvertFromNeighbor: full-face connection, face coordinate flips
         bool bad;
+        double fx = V.x, fy = V.y, fz = V.z;
         switch (face) {
           case 1: bad = fx != 0.0; fx = 1.0; break;
           case 2: bad = fx != 1.0; fx = 0.0; break;
@@ -717,72 +883,55 @@ __device__ int track_path(const double *__restrict__ cells,
           case 5: bad = fz != 0.0; fz = 1.0; break;
           default: bad = fz != 1.0; fz = 0.0; break;
         }
-        if (bad) return 9;
+        if (bad) {  // no defined continuation in the reference
+          status = 9;
+          done = true;
+          break;
+        }
         c = next;
-        x = fx; y = fy; z = fz; t = ft;
-        continue;
+        V.x = fx; V.y = fy; V.z = fz;
+        A.phase = 0;
+        A.entryFace = face <= 2 ? 3 - face : (face <= 4 ? 7 - face : 11 - face);
       }
-      if (out == OUT_UNDEFINED) return 9;
-      cell = c; lx = x; ly = y; lz = z; lt = t;
-      return out == OUT_NOEXIT ? 5 : 7;
     }
-  stop_here_7:
-    cell = c; lx = x; ly = y; lz = z; lt = t;
-    return 7;
+    // statuses 3-7 come from the entry checks and report the entry point,
+    // which is the current point
+    S.cell[p] = c;
+    S.lx[p] = V.x; S.ly[p] = V.y; S.lz[p] = V.z; S.t[p] = V.t;
+    S.rng[2 * p] = rng.a;
+    S.rng[2 * p + 1] = rng.b;
+    S.substeps[p] += (int64_t)steps;
+    if (more) {
+      A.restarts = V.restarts;
+      A.exitFace = V.exitFace;
+      S.aux[p] = pack_aux(A);
+      S.ix[p] = V.ix; S.iy[p] = V.iy; S.iz[p] = V.iz; S.t0[p] = V.t0;
+    } else {
+      S.status[p] = status;
+      ended_active = status == 1 ? 1ull : 0ull;
+    }
   }
+  append_work(more, c, p, nkeys, nvals, &counters[2]);
+  wave_count(ended_active, steps, &counters[0], &counters[1]);
 }
 
-__global__ void __launch_bounds__(256)
-    advance_f64(const double *__restrict__ cells, int64_t n,
-                int32_t *__restrict__ cellv, int32_t *__restrict__ statusv,
-                double *__restrict__ lxv, double *__restrict__ lyv,
-                double *__restrict__ lzv, double *__restrict__ tv,
-                uint64_t *__restrict__ rngv, int64_t *__restrict__ stepv,
-                const WalkParams W, unsigned long long *__restrict__ counters) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  int st = statusv[p];
-  int c = cellv[p];
-  double t = tv[p];
-  int64_t steps = 0;
-  if (st == 0) {  // release, MPathRW.F90:1256-1302
-    if (t < W.t_engine) st = 8;
-    else if (t <= W.t_stop) {
-      st = 1;
-      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-      if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[0] == 0) st = 7;
-    }
+// Particles still listed after the step guard: status 9 at the last point.
+__global__ void terminate_walk(const double *__restrict__ cells,
+                               const int32_t *__restrict__ work, int64_t nwork,
+                               ParticlePtrs S) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwork) return;
+  const int p = work[i];
+  Aux A = unpack_aux(S.aux[p]);
+  int c = S.cell[p];
+  double x = S.ix[p], y = S.iy[p], z = S.iz[p], t = S.t0[p];
+  if (!A.phase) {
+    x = S.lx[p]; y = S.ly[p]; z = S.lz[p]; t = S.t[p];
   }
-  if (st == 7) {
-    const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-    if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[1] & CF_VERIFY_PDRY)
-      st = 1;
-  }
-  if (st == 1) {
-    double x = lxv[p], y = lyv[p], z = lzv[p];
-    Stream rng{rngv[2 * p], rngv[2 * p + 1]};
-    st = track_path(cells, W, c, x, y, z, t, rng, steps);
-    cellv[p] = c;
-    lxv[p] = x;
-    lyv[p] = y;
-    lzv[p] = z;
-    tv[p] = t;
-    rngv[2 * p] = rng.a;
-    rngv[2 * p + 1] = rng.b;
-    stepv[p] += steps;
-  }
-  statusv[p] = st;
-  // per-wave totals of active particles and sub-steps
-  unsigned long long act = st == 1 ? 1ull : 0ull;
-  unsigned long long ns = (unsigned long long)steps;
-  for (int off = 32; off > 0; off >>= 1) {
-    act += __shfl_down(act, off, 64);
-    ns += __shfl_down(ns, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&counters[0], act);
-    atomicAdd(&counters[1], ns);
-  }
+  last_point(cells, A.entryFace, c, x, y, z);
+  S.cell[p] = c;
+  S.lx[p] = x; S.ly[p] = y; S.lz[p] = z; S.t[p] = t;
+  S.status[p] = 9;
 }
 
 // Stream p = J^(offset+p) base, J the jump matrix over GF(2)^128; the k-th
@@ -896,6 +1045,38 @@ hipError_t launch_seed_streams(rwpt_engine *e, int64_t n, int64_t offset) {
   return hipGetLastError();
 }
 
+static ParticlePtrs particle_ptrs(Particles &P) {
+  return ParticlePtrs{P.cell, P.status, P.aux, P.lx, P.ly, P.lz, P.t,
+                      P.ix,   P.iy,     P.iz,  P.t0, P.rng, P.substeps};
+}
+
+static int radix_bits(int64_t ncell) {
+  int b = 1;
+  while ((1ll << b) <= ncell) ++b;
+  return b;
+}
+
+hipError_t ensure_sort_storage(rwpt_engine *e) {
+  Particles &P = e->part;
+  size_t bytes = 0;
+  hipError_t err = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, bytes, P.keys[0], P.keys[1], P.vals[0], P.vals[1],
+      (int)std::max<int64_t>(P.cap, 1), 0, radix_bits(e->prob.ncell), e->stream);
+  if (err != hipSuccess) return err;
+  if (bytes > P.sort_tmp_bytes) {
+    if (P.sort_tmp) (void)hipFree(P.sort_tmp);
+    P.sort_tmp = nullptr;
+    P.sort_tmp_bytes = 0;
+    err = hipMalloc(&P.sort_tmp, bytes);
+    if (err != hipSuccess) return err;
+    P.sort_tmp_bytes = bytes;
+  }
+  return hipSuccess;
+}
+
+// One tracking interval: prepare, then walk launches of at most `budget`
+// iterations per particle over the cell-sorted list of unfinished
+// particles, until every particle has reached t_stop or stopped.
 hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   Particles &P = e->part;
   WalkParams W{};
@@ -912,20 +1093,74 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   W.steady_state = e->opt.steady_state;
   W.t_engine = e->t_engine;
   W.t_stop = t_stop;
+  e->last_launches = 0;
   hipError_t err = hipMemsetAsync(e->counters, 0, 4 * sizeof(unsigned long long),
                                   e->stream);
+  if (err != hipSuccess || P.n == 0) return err;
+  const ParticlePtrs S = particle_ptrs(P);
+  const int pb = 256;
+  hipLaunchKernelGGL(prepare_advance, dim3((unsigned)((P.n + pb - 1) / pb)),
+                     dim3(pb), 0, e->stream, e->prob.cells, P.n, S, W, P.keys[0],
+                     P.vals[0], e->counters);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  unsigned long long cnt = 0;
+  err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost,
+                       e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return err;
-  if (P.n == 0) return hipSuccess;
-  const int block = 256;
-  KernelTimer &T = e->timers[0];
-  (void)hipEventRecord(T.start, e->stream);
-  hipLaunchKernelGGL(advance_f64, dim3((unsigned)((P.n + block - 1) / block)),
-                     dim3(block), 0, e->stream, e->prob.cells, P.n, P.cell,
-                     P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps, W,
-                     e->counters);
-  err = hipGetLastError();
-  (void)hipEventRecord(T.stop, e->stream);
-  return err;
+  const int budget = std::max(1, e->tune.budget);
+  const int block = e->tune.block;
+  const int64_t max_launches = (kMaxSteps + budget - 1) / budget;
+  const int bits = radix_bits(e->prob.ncell);
+  int64_t launches = 0;
+  while (cnt > 0) {
+    const int32_t *work = P.vals[0];
+    if (e->tune.sort) {
+      size_t bytes = P.sort_tmp_bytes;
+      err = hipcub::DeviceRadixSort::SortPairs(P.sort_tmp, bytes, P.keys[0],
+                                               P.keys[1], P.vals[0], P.vals[1],
+                                               (int)cnt, 0, bits, e->stream);
+      if (err != hipSuccess) return err;
+      work = P.vals[1];
+    }
+    if (launches >= max_launches) {  // step guard reached
+      hipLaunchKernelGGL(terminate_walk, dim3((unsigned)((cnt + 255) / 256)),
+                         dim3(256), 0, e->stream, e->prob.cells, work,
+                         (int64_t)cnt, S);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      break;
+    }
+    // the next list is written into the (consumed) sort input buffers,
+    // or into the second pair when the list is not sorted
+    int32_t *nk = e->tune.sort ? P.keys[0] : P.keys[1];
+    int32_t *nv = e->tune.sort ? P.vals[0] : P.vals[1];
+    err = hipMemsetAsync(e->counters + 2, 0, sizeof(unsigned long long), e->stream);
+    if (err != hipSuccess) return err;
+    if ((size_t)(2 * launches + 2) > e->walk_events.size()) {
+      hipEvent_t ev[2];
+      if ((err = hipEventCreate(&ev[0])) != hipSuccess) return err;
+      if ((err = hipEventCreate(&ev[1])) != hipSuccess) return err;
+      e->walk_events.push_back(ev[0]);
+      e->walk_events.push_back(ev[1]);
+    }
+    (void)hipEventRecord(e->walk_events[2 * launches], e->stream);
+    hipLaunchKernelGGL(walk_kernel, dim3((unsigned)((cnt + block - 1) / block)),
+                       dim3(block), 0, e->stream, e->prob.cells, work,
+                       (int64_t)cnt, S, W, budget, nk, nv, e->counters);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    (void)hipEventRecord(e->walk_events[2 * launches + 1], e->stream);
+    launches += 1;
+    err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost,
+                         e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) return err;
+    if (!e->tune.sort) {
+      std::swap(P.keys[0], P.keys[1]);
+      std::swap(P.vals[0], P.vals[1]);
+    }
+  }
+  e->last_launches = launches;
+  return hipSuccess;
 }
 
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,

@@ -215,3 +215,32 @@ def test_full_size_properties():
     assert np.all((a["cell"] >= 1) & (a["cell"] <= P.ncell))
     assert set(np.unique(a["status"])) <= {1, 2, 9}
     assert (a["status"] == 9).mean() < 1e-4
+
+
+@pytest.mark.parametrize("budget,block,sort", [(1, 64, 1), (7, 128, 0),
+                                               (64, 256, 1), (100000, 512, 1)])
+def test_walk_launch_shape_does_not_change_results(budget, block, sort):
+    """The walk pauses each particle after `budget` iterations and resumes
+    it in the next launch from its saved loop state; any budget, block size
+    and list order must reproduce the oracle bit for bit."""
+    P, where, dt_out, nout = SCEN["hetero_ibound_rebound_R"]
+    n = 1500
+    cell, lx, ly, lz = _release(P, where, n)
+    E = Engine(P)
+    try:
+        E.set_tuning(budget, block, sort)
+        E.load_particles(cell, lx, ly, lz)
+        st = O.new_state(cell, lx, ly, lz)
+        OP = O.OracleProblem(P)
+        t = 0.0
+        for k in range(1, 4):
+            E.advance(k * dt_out)
+            if budget == 1:
+                assert E.last_launches() > 10
+            O.advance(OP, st, t, k * dt_out)
+            t = k * dt_out
+        g = E.particles()
+    finally:
+        E.close()
+    for f in FIELDS:
+        assert np.array_equal(g[f], st[f]), f
