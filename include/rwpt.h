@@ -143,9 +143,11 @@ int rwpt_set_time(rwpt_engine *e, double t);
  * iterations per particle per launch (the loop state is saved between
  * launches, so results do not depend on it), block = workgroup size,
  * sort = re-sort unfinished particles by cell between launches.
- * Defaults 64, 256, 1. */
+ * block must be 64, 128 or 256; min_waves (1, 3 or 4) selects a build of
+ * the walk register-limited for that many waves per SIMD.
+ * Defaults 64, 256, 1, 1. */
 int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
-                    int32_t sort);
+                    int32_t sort, int32_t min_waves);
 /* Walk launches the last rwpt_advance used. */
 int rwpt_last_launches(const rwpt_engine *e);
 

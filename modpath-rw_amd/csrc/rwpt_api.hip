@@ -400,11 +400,14 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
 }
 
 int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
-                    int32_t sort) {
+                    int32_t sort, int32_t min_waves) {
   if (!e) return RWPT_E_ARG;
-  if (budget < 1 || (block != 64 && block != 128 && block != 256 &&
-                     block != 512 && block != 1024))
-    return fail(e, RWPT_E_ARG, "budget >= 1, block one of 64..1024");
+  // the walk kernel is compiled for at most 256 threads per workgroup
+  if (budget < 1 || (block != 64 && block != 128 && block != 256))
+    return fail(e, RWPT_E_ARG, "budget >= 1, block one of 64, 128, 256");
+  if (min_waves != 1 && min_waves != 3 && min_waves != 4)
+    return fail(e, RWPT_E_ARG, "min_waves one of 1, 3, 4");
+  e->tune.min_waves = min_waves;
   e->tune.budget = budget;
   e->tune.block = block;
   e->tune.sort = sort ? 1 : 0;

@@ -97,6 +97,7 @@ struct Tuning {
   int32_t budget = 64;    // time-loop iterations per particle and launch
   int32_t block = 256;    // threads per workgroup of the walk
   int32_t sort = 1;       // sort the work list by cell between launches
+  int32_t min_waves = 1;  // occupancy hint variant of the walk (1, 3, 4)
 };
 
 struct WalkParams {

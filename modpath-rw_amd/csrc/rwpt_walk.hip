@@ -101,9 +101,10 @@ struct StepTerms {
   double ddx, ddy, ddz;     // dispersion divergence (drift correction)
 };
 
+template <int ADV>
 __device__ __forceinline__ void advect(const double *__restrict__ rec,
-                                       int kind, double dt, StepTerms &s) {
-  if (kind == 1) {  // pr_AdvectionDisplacementExponential
+                                       double dt, StepTerms &s) {
+  if constexpr (ADV == 1) {  // pr_AdvectionDisplacementExponential
     const double dvtol = 1.0e-10;
     const double gx = rec[F_DVDX], gy = rec[F_DVDY], gz = rec[F_DVDZ];
     s.dAx = fabs(gx) > dvtol ? s.vx * (exp(gx * dt) - 1.0) / gx : s.vx * dt;
@@ -117,6 +118,7 @@ __device__ __forceinline__ void advect(const double *__restrict__ rec,
 }
 
 // pr_RWPTDisplacementsLinear / ...Axisymmetric for one sub-step.
+template <int NDIM, int MODEL, int ADV>
 __device__ __forceinline__ void displacements(const double *__restrict__ rec,
                                               const WalkParams &W, double x,
                                               double y, double z, double dt,
@@ -125,7 +127,7 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
   s.vy = (1.0 - y) * rec[F_VY1] + y * rec[F_VY2];
   s.vz = (1.0 - z) * rec[F_VZ1] + z * rec[F_VZ2];
   double aL, aT, aTH;
-  if (W.disp_model == 1) {
+  if constexpr (MODEL == 1) {
     const double vn = sqrt(s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
     double c2 = 0.0;
     if (vn > 0.0) c2 = s.vz * s.vz / vn;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
   // random displacement, pr_DisplacementRandomDischarge{,2D,1D}
   const double dm = rec[F_DMEFF];
   const double *vc = rec + F_VC;
-  if (W.ndim == 3) {
+  if constexpr (NDIM == 3) {
     const double bx = tri8(vc, x, y, z), by = tri8(vc + 8, x, y, z);
     const double bz = tri8(vc + 16, x, y, z);
     const double vn = sqrt(bx * bx + by * by + bz * bz);
@@ -178,7 +180,7 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     s.dBx = B11 * r1 + B12 * r2 + B13 * r3;
     s.dBy = B21 * r1 + B22 * r2 + B23 * r3;
     s.dBz = B31 * r1 + B32 * r2;
-  } else if (W.ndim == 2) {
+  } else if constexpr (NDIM == 2) {
     const int d1 = W.dims[0] - 1, d2 = W.dims[1] - 1;
     const double b1 = tri8(vc + 8 * d1, x, y, z);
     const double b2 = tri8(vc + 8 * d2, x, y, z);
@@ -212,7 +214,7 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     s.dBy = dB[1];
     s.dBz = dB[2];
   }
-  advect(rec, W.advection_kind, dt, s);
+  advect<ADV>(rec, dt, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,7 +278,7 @@ __device__ __forceinline__ double conv_der(const NRTerms &T) {
 
 // pr_SetInitialGuess (TrackSubCell.F90:2162-2448): decade scan, gradient
 // descent on the convergence function, then a 0.2..0.9 fraction scan.
-__device__ double nr_guess(double dt, double v, double g, double dI,
+__device__ __noinline__ double nr_guess(double dt, double v, double g, double dI,
                            double divD, double dB) {
   double dtnew = 0.0;
   if (dt == 0.0) return dtnew;
@@ -382,8 +384,9 @@ __device__ double nr_guess(double dt, double v, double g, double dI,
   return dtnew;
 }
 
-// NewtonRaphsonTimeStepExponentialAdvection (TrackSubCell.F90:2451-2554)
-__device__ double newton_axis(double dt, double v, double g, double dI,
+// NewtonRaphsonTimeStepExponentialAdvection (TrackSubCell.F90:2451-2554);
+// kept out of line: it runs only on exit attempts of EXPONENTIAL runs
+__device__ __noinline__ double newton_axis(double dt, double v, double g, double dI,
                               double divD, double dB) {
   const double nrtol = f32lit(1e-6f);
   double d0 = nr_guess(dt, v, g, dI, divD, dB);
@@ -409,7 +412,8 @@ __device__ double newton_axis(double dt, double v, double g, double dI,
 // displacement reaches the first crossed face; updates t, dt, exitFace
 // exactly as the reference (including leaving exitFace untouched when no
 // axis yields a time).
-__device__ void exit_time(const double *__restrict__ rec, int kind, double x,
+template <int ADV>
+__device__ void exit_time(const double *__restrict__ rec, double x,
                           double y, double z, double nx, double ny, double nz,
                           const StepTerms &s, double &t, double &dt,
                           int &exitFace) {
@@ -438,7 +442,7 @@ __device__ void exit_time(const double *__restrict__ rec, int kind, double x,
         dt = 0.0;
         return;
       }
-      if (kind == 2) {
+      if constexpr (ADV == 2) {
         if (quadratic_axis(dI, v[a] + dv[a], db[a], dt, dtx[a])) {
           exitFace = fx[a];
           dt = 0.0;
@@ -485,6 +489,7 @@ struct Visit {
 // One iteration of the time loop (TrackSubCell.F90:956-1368).  On IT_EXIT
 // and IT_MAXTIME the final point is left in V.x..V.t; *zero_dt reports an
 // exit taken with a zero time step.
+template <int NDIM, int MODEL, int ADV>
 __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
                        Visit &V, Stream &rng, bool &zero_dt) {
   const double dtcell = rec[F_DTCELL];
@@ -504,7 +509,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
     t = W.t_stop;
     atMax = true;
   }
-  displacements(rec, W, x, y, z, dt, rng, s);
+  displacements<NDIM, MODEL, ADV>(rec, W, x, y, z, dt, rng, s);
   double sq = sqrt(dt);
   double dxr = s.dAx + s.ddx * dt + s.dBx * sq;
   double dyr = s.dAy + s.ddy * dt + s.dBy * sq;
@@ -514,7 +519,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
          (fabs(dzr / DZ) > maxRelJump)) {
     t = t - dt;
     dt = dtRed * dt;
-    advect(rec, W.advection_kind, dt, s);
+    advect<ADV>(rec, dt, s);
     sq = sqrt(dt);
     dxr = s.dAx + s.ddx * dt + s.dBx * sq;
     dyr = s.dAy + s.ddy * dt + s.dBy * sq;
@@ -528,7 +533,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
   int loops = 0;
   while (outside(nx, ny, nz)) {
     loops = loops + 1;
-    exit_time(rec, W.advection_kind, x, y, z, nx, ny, nz, s, t, dt, exitFace);
+    exit_time<ADV>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
     if (loops > 5) {  // maxInterfaceLoopCounter: restart the visit
       nx = V.ix; ny = V.iy; nz = V.iz;
       t = V.t0;
@@ -537,7 +542,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
-    advect(rec, W.advection_kind, dt, s);
+    advect<ADV>(rec, dt, s);
     atMax = false;
     if (exitFace == 0) {  // no face found: restart with new draws
       nx = V.ix; ny = V.iy; nz = V.iz;
@@ -589,7 +594,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
         dt = fmax(W.t_stop - t, 0.0);
         t = W.t_stop;
         atMax = true;
-        advect(rec, W.advection_kind, dt, s);
+        advect<ADV>(rec, dt, s);
         sq = sqrt(dt);
         dxr = s.dAx + s.ddx * dt + s.dBx * sq;
         dyr = s.dAy + s.ddy * dt + s.dBy * sq;
@@ -794,7 +799,8 @@ __global__ void prepare_advance(const double *__restrict__ cells, int64_t n,
 
 // One launch of the walk: every listed particle runs up to `budget`
 // time-loop iterations; particles not done are appended to the next list.
-__global__ void __launch_bounds__(256)
+template <int NDIM, int MODEL, int ADV, int MINW>
+__global__ void __launch_bounds__(256, MINW)
     walk_kernel(const double *__restrict__ cells,
                 const int32_t *__restrict__ work, int64_t nwork, ParticlePtrs S,
                 const WalkParams W, int budget, int32_t *__restrict__ nkeys,
@@ -843,7 +849,7 @@ __global__ void __launch_bounds__(256)
       }
       steps += 1;
       bool zero_dt = false;
-      const int out = iterate(rec, W, V, rng, zero_dt);
+      const int out = iterate<NDIM, MODEL, ADV>(rec, W, V, rng, zero_dt);
       if (out == IT_CONTINUE) continue;
       if (out == IT_MAXTIME) {
         status = 1;
@@ -1144,9 +1150,30 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
       e->walk_events.push_back(ev[1]);
     }
     (void)hipEventRecord(e->walk_events[2 * launches], e->stream);
-    hipLaunchKernelGGL(walk_kernel, dim3((unsigned)((cnt + block - 1) / block)),
-                       dim3(block), 0, e->stream, e->prob.cells, work,
-                       (int64_t)cnt, S, W, budget, nk, nv, e->counters);
+    const dim3 grid((unsigned)((cnt + block - 1) / block));
+    const int mode = (W.ndim - 1) * 4 + W.disp_model * 2 + (W.advection_kind - 1);
+#define RWPT_WALK(ND, MO, AD)                                                  \
+  case (ND - 1) * 4 + MO * 2 + (AD - 1):                                       \
+    if (ND == 3 && MO == 0 && AD == 2 && e->tune.min_waves == 3)               \
+      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 3>), grid, dim3(block), 0,   \
+                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
+                         budget, nk, nv, e->counters);                         \
+    else if (ND == 3 && MO == 0 && AD == 2 && e->tune.min_waves == 4)          \
+      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 4>), grid, dim3(block), 0,   \
+                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
+                         budget, nk, nv, e->counters);                         \
+    else                                                                       \
+      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 1>), grid, dim3(block), 0,   \
+                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
+                         budget, nk, nv, e->counters);                         \
+    break;
+    switch (mode) {
+      RWPT_WALK(1, 0, 1) RWPT_WALK(1, 0, 2) RWPT_WALK(1, 1, 1) RWPT_WALK(1, 1, 2)
+      RWPT_WALK(2, 0, 1) RWPT_WALK(2, 0, 2) RWPT_WALK(2, 1, 1) RWPT_WALK(2, 1, 2)
+      RWPT_WALK(3, 0, 1) RWPT_WALK(3, 0, 2) RWPT_WALK(3, 1, 1) RWPT_WALK(3, 1, 2)
+      default: return hipErrorInvalidValue;
+    }
+#undef RWPT_WALK
     if ((err = hipGetLastError()) != hipSuccess) return err;
     (void)hipEventRecord(e->walk_events[2 * launches + 1], e->stream);
     launches += 1;

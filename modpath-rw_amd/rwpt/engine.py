@@ -66,7 +66,8 @@ EXPORTS = {
     "rwpt_particle_count": ([C.c_void_p], C.c_int64),
     "rwpt_advance": ([C.c_void_p, C.c_double, _i64p, _i64p], C.c_int),
     "rwpt_set_time": ([C.c_void_p, C.c_double], C.c_int),
-    "rwpt_set_tuning": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "rwpt_set_tuning": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
+                        C.c_int),
     "rwpt_last_launches": ([C.c_void_p], C.c_int),
     "rwpt_get_substeps": ([C.c_void_p, _i64p], C.c_int),
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
@@ -205,9 +206,9 @@ class Engine:
     def set_time(self, t):
         self._check(lib().rwpt_set_time(self.h, float(t)), "rwpt_set_time")
 
-    def set_tuning(self, budget=64, block=256, sort=True):
-        self._check(lib().rwpt_set_tuning(self.h, int(budget), int(block), int(sort)),
-                    "rwpt_set_tuning")
+    def set_tuning(self, budget=64, block=256, sort=True, min_waves=1):
+        self._check(lib().rwpt_set_tuning(self.h, int(budget), int(block), int(sort),
+                                          int(min_waves)), "rwpt_set_tuning")
 
     def last_launches(self):
         return lib().rwpt_last_launches(self.h)

@@ -49,3 +49,9 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     # null grid pointers and a zero dimension mask are rejected on the host
     assert lib.rwpt_create(C.byref(g), C.byref(o), C.byref(h)) == -1
     assert lib.rwpt_create(None, C.byref(o), C.byref(h)) == -1
+
+
+def test_tuning_rejects_block_above_launch_bound():
+    # rwpt_set_tuning validates on the host; a null engine is rejected first
+    lib = engine.lib()
+    assert lib.rwpt_set_tuning(None, 64, 512, 1, 1) == -1
