@@ -164,6 +164,14 @@ struct TableParams {
   double ts0, ts1;
 };
 
+// 1/d when d is a power of two (then exact), else 0.
+__device__ __forceinline__ double pow2_recip(double d) {
+  int e = 0;
+  const double m = frexp(d, &e);
+  if (m != 0.5 || 1 - e < -1022 || 1 - e > 1023) return 0.0;
+  return ldexp(1.0, 1 - e);
+}
+
 __global__ void corner_tables(const GridDims g, int64_t ncell,
                               const double *__restrict__ delr,
                               const double *__restrict__ delc,
@@ -360,7 +368,11 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
                         (VY2 > 0.0) || (VZ1 < 0.0) || (VZ2 > 0.0);
   int32_t *fl = reinterpret_cast<int32_t *>(rec + F_FLAGS);
   fl[1] = (fl[1] & ~CF_SUBCELL_EXIT) | (sub_exit ? CF_SUBCELL_EXIT : 0);
-  for (int pad = 90; pad < CELL_DOUBLES; ++pad) rec[pad] = 0.0;
+  rec[F_RDX] = pow2_recip(DX);
+  rec[F_RDY] = pow2_recip(DY);
+  rec[F_RDZ] = pow2_recip(DZ);
+  rec[F_RR] = pow2_recip(R);
+  for (int pad = F_RR + 1; pad < CELL_DOUBLES; ++pad) rec[pad] = 0.0;
 
   if (debug) {
     double *o = debug + c0 * DEBUG_FIELDS;

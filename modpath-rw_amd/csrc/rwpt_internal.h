@@ -34,6 +34,7 @@ enum CellField : int {
   F_FLAGS = 86,     // int32 ibound, int32 flags
   F_ZONE = 87,      // int32 zone
   F_SOURCE = 88, F_SINK = 89,
+  F_RDX = 90, F_RDY, F_RDZ, F_RR,  // 1/DX.. 1/R when a power of two, else 0
   CELL_DOUBLES = 96
 };
 

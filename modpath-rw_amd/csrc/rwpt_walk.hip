@@ -54,6 +54,13 @@ __device__ __forceinline__ double draw_normal(Stream &s) {
 
 __device__ __forceinline__ double f32lit(float v) { return (double)v; }
 
+// a / d where the cell record stores r = 1/d when d is a power of two (else
+// 0): the product then rounds the same exact real as the quotient, so the
+// result is bit-identical and the IEEE divide sequence is skipped.
+__device__ __forceinline__ double div_p2(double a, double d, double r) {
+  return r != 0.0 ? a * r : a / d;
+}
+
 // Guards against walks the reference never ends (not in the reference):
 // sub-steps per particle and launch, and consecutive cell exits made with a
 // zero time step (a particle on a face whose drift points into the face from
@@ -146,10 +153,10 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
   const double t4 = deriv4(dd + 16, y, z), t5 = deriv4(dd + 20, x, z);
   const double t6 = deriv4(dd + 24, x, z), t7 = deriv4(dd + 28, x, y);
   const double t8 = deriv4(dd + 32, x, y);
-  const double phi = rec[F_PHI], R = rec[F_R];
-  s.ddx = (t0 + t5 + t7) / phi / R;
-  s.ddy = (t3 + t1 + t8) / phi / R;
-  s.ddz = (t4 + t6 + t2) / phi / R;
+  const double phi = rec[F_PHI], R = rec[F_R], rR = rec[F_RR];
+  s.ddx = div_p2((t0 + t5 + t7) / phi, R, rR);
+  s.ddy = div_p2((t3 + t1 + t8) / phi, R, rR);
+  s.ddz = div_p2((t4 + t6 + t2) / phi, R, rR);
   // random displacement, pr_DisplacementRandomDischarge{,2D,1D}
   const double dm = rec[F_DMEFF];
   const double *vc = rec + F_VC;
@@ -161,14 +168,14 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     double B11 = 0, B12 = 0, B13 = 0, B21 = 0, B22 = 0, B23 = 0, B31 = 0,
            B32 = 0;
     if (vn > 0.0) {
-      const double sl = sqrt(2 * (aL * vn + dm) / R);
-      const double st = sqrt(2 * (aT * vn + dm) / R);
+      const double sl = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
+      const double st = sqrt(div_p2(2 * (aT * vn + dm), R, rR));
       B11 = bx * sl / vn;
       B21 = by * sl / vn;
       B31 = bz * sl / vn;
       B32 = vxy * st / vn;
       if (vxy > 0.0) {
-        const double sh = sqrt(2 * (aTH * vn + dm) / R);
+        const double sh = sqrt(div_p2(2 * (aTH * vn + dm), R, rR));
         B12 = -(bx * bz * st / vn / vxy);
         B13 = -(by * sh / vxy);
         B22 = -(by * bz * st / vn / vxy);
@@ -187,8 +194,8 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     const double vn = sqrt(b1 * b1 + b2 * b2);
     double B11 = 0, B12 = 0, B21 = 0, B22 = 0;
     if (vn > 0.0) {
-      const double sl = sqrt(2 * (aL * vn + dm) / R);
-      const double st = sqrt(2 * (aTH * vn + dm) / R);
+      const double sl = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
+      const double st = sqrt(div_p2(2 * (aTH * vn + dm), R, rR));
       B11 = b1 * sl / vn;
       B21 = b2 * sl / vn;
       B12 = -(b2 * st / vn);
@@ -206,7 +213,7 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     const double b1 = tri8(vc + 8 * d1, x, y, z);
     const double vn = sqrt(b1 * b1);
     double B11 = 0.0;
-    if (vn > 0.0) B11 = sqrt(2 * (aL * vn + dm) / R);
+    if (vn > 0.0) B11 = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
     const double r1 = draw_normal(rng);
     double dB[3] = {0.0, 0.0, 0.0};
     dB[d1] = B11 * r1;
@@ -495,6 +502,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
   const double dtcell = rec[F_DTCELL];
   const double maxRelJump = 0.5, dtRed = f32lit(0.1f);
   const double DX = rec[F_DX], DY = rec[F_DY], DZ = rec[F_DZ];
+  const double rX = rec[F_RDX], rY = rec[F_RDY], rZ = rec[F_RDZ];
   const int32_t *mb = reinterpret_cast<const int32_t *>(rec + F_MASSBND);
   double x = V.x, y = V.y, z = V.z, t = V.t;
   double nx, ny, nz, dt;
@@ -515,8 +523,8 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
   double dyr = s.dAy + s.ddy * dt + s.dBy * sq;
   double dzr = s.dAz + s.ddz * dt + s.dBz * sq;
   // jump limiter: same draw, dt cut by 0.1 until each jump <= half a cell
-  while ((fabs(dxr / DX) > maxRelJump) || (fabs(dyr / DY) > maxRelJump) ||
-         (fabs(dzr / DZ) > maxRelJump)) {
+  while ((fabs(div_p2(dxr, DX, rX)) > maxRelJump) || (fabs(div_p2(dyr, DY, rY)) > maxRelJump) ||
+         (fabs(div_p2(dzr, DZ, rZ)) > maxRelJump)) {
     t = t - dt;
     dt = dtRed * dt;
     advect<ADV>(rec, dt, s);
@@ -527,9 +535,9 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
     t = t + dt;
     atMax = false;
   }
-  nx = x + dxr / DX;
-  ny = y + dyr / DY;
-  nz = z + dzr / DZ;
+  nx = x + div_p2(dxr, DX, rX);
+  ny = y + div_p2(dyr, DY, rY);
+  nz = z + div_p2(dzr, DZ, rZ);
   int loops = 0;
   while (outside(nx, ny, nz)) {
     loops = loops + 1;
@@ -556,20 +564,20 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
     if (exitFace <= 2) {
       nx = exitFace == 1 ? 0.0 : 1.0;
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + dyr / DY;
+      ny = y + div_p2(dyr, DY, rY);
       dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + dzr / DZ;
+      nz = z + div_p2(dzr, DZ, rZ);
     } else if (exitFace <= 4) {
       dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + dxr / DX;
+      nx = x + div_p2(dxr, DX, rX);
       ny = exitFace == 3 ? 0.0 : 1.0;
       dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + dzr / DZ;
+      nz = z + div_p2(dzr, DZ, rZ);
     } else {
       dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + dxr / DX;
+      nx = x + div_p2(dxr, DX, rX);
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + dyr / DY;
+      ny = y + div_p2(dyr, DY, rY);
       nz = exitFace == 5 ? 0.0 : 1.0;
     }
     if (outside(nx, ny, nz)) continue;
@@ -601,17 +609,17 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
         dzr = s.dAz + s.ddz * dt + s.dBz * sq;
       }
       if (exitFace <= 2) {
-        nx = nx - dxr / DX;
-        ny = ny + dyr / DY;
-        nz = nz + dzr / DZ;
+        nx = nx - div_p2(dxr, DX, rX);
+        ny = ny + div_p2(dyr, DY, rY);
+        nz = nz + div_p2(dzr, DZ, rZ);
       } else if (exitFace <= 4) {
-        nx = nx + dxr / DX;
-        ny = ny - dyr / DY;
-        nz = nz + dzr / DZ;
+        nx = nx + div_p2(dxr, DX, rX);
+        ny = ny - div_p2(dyr, DY, rY);
+        nz = nz + div_p2(dzr, DZ, rZ);
       } else {
-        nx = nx + dxr / DX;
-        ny = ny + dyr / DY;
-        nz = nz - dzr / DZ;
+        nx = nx + div_p2(dxr, DX, rX);
+        ny = ny + div_p2(dyr, DY, rY);
+        nz = nz - div_p2(dzr, DZ, rZ);
       }
       if (outside(nx, ny, nz)) {
         x = xi; y = yi; z = zi;

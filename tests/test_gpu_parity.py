@@ -64,6 +64,17 @@ def _scenarios():
     s["distributed_dispersivity"] = (P, None, 0.5, 6)
     P = _hetero(backward=1)
     s["backward"] = (P, None, 0.5, 6)
+    # non-uniform spacings: powers of two take the exact-reciprocal path in
+    # the device code, the others the IEEE division path
+    P = _hetero()
+    g = np.random.default_rng(8)
+    P.delr = g.choice([0.5, 1.0, 1.5, 2.0, 0.7], P.ncol)
+    P.delc = g.choice([0.25, 1.0, 1.3], P.nrow)
+    dz = g.choice([1.0, 2.0, 0.6], P.nlay)
+    tops = np.concatenate([[dz.sum()], dz.sum() - np.cumsum(dz)[:-1]])
+    P.top = np.repeat(tops, P.nrow * P.ncol)
+    P.bot = np.repeat(tops - dz, P.nrow * P.ncol)
+    s["nonuniform_spacing"] = (P, None, 0.5, 6)
     return s
 
 
