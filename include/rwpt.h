@@ -159,11 +159,20 @@ int rwpt_get_substeps(rwpt_engine *e, int64_t *substeps);
 int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz);
 
 /* Concentration reconstruction over the active particles
- * (GridProjectedKDE%ComputeDensity, GridProjectedKDE.F90:3598-4392).
- * density and histogram are nbins[0]*nbins[1]*nbins[2] doubles, x fastest
- * (either may be NULL); mass may be NULL (unit masses). */
+ * (GridProjectedKDE%ComputeDensity, GridProjectedKDE.F90:3598-4392, with
+ * nOptimizationLoops = 0 and the brute-force kernel: one pass with the
+ * Silverman bandwidth).  density and histogram are nbins[0]*nbins[1]*
+ * nbins[2] doubles, x fastest (either may be NULL); mass (one per particle)
+ * may be NULL for unit masses.  The histogram holds the per-bin sums of the
+ * masses; mode 0 returns it divided by the bin volume as the density. */
 int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
              double *density, double *histogram);
+
+/* Quantities of the last rwpt_kde call: the Silverman bandwidth used, the
+ * number of active particles passed, the histogram's nPoints and the
+ * effective mass (Histogram.F90:332-341). */
+int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
+                  double *npoints, double *eff_mass);
 
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles in the layout documented in DESIGN.md. */

@@ -151,6 +151,18 @@ int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
   if (err == hipSuccess) err = dev_alloc(&P.dzrw, ncell);
   if (err == hipSuccess) err = dev_alloc(&P.cells, ncell * CELL_DOUBLES);
   if (err == hipSuccess) err = dev_alloc(&e->counters, 4);
+  {
+    // face coordinates X(1..ncol+1), Y(1..nrow+1) of RectangularGridDis.f90
+    std::vector<double> X(P.ncol + 1), Y(P.nrow + 1);
+    X[0] = 0.0;
+    for (int j = 0; j < P.ncol; ++j) X[j + 1] = X[j] + grid->delr[j];
+    Y[0] = 0.0;
+    for (int i = 0; i < P.nrow; ++i) Y[0] = Y[0] + grid->delc[i];
+    for (int i = 0; i < P.nrow; ++i) Y[i + 1] = Y[i] - grid->delc[i];
+    if (err == hipSuccess) err = upload(e, &e->xface, X.data(), X.size());
+    if (err == hipSuccess) err = upload(e, &e->yface, Y.data(), Y.size());
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  }
   std::vector<uint64_t> mats;
   jump_power_matrices(64, mats);
   if (err == hipSuccess) err = upload(e, &e->jump_mats, mats.data(), mats.size());
@@ -182,6 +194,9 @@ int rwpt_destroy(rwpt_engine *e) {
   dev_free(e->jump_mats);
   dev_free(e->counters);
   dev_free(e->kde_scratch);
+  dev_free(e->kde_weights);
+  dev_free(e->xface);
+  dev_free(e->yface);
   for (auto &T : e->timers) {
     if (T.start) (void)hipEventDestroy(T.start);
     if (T.stop) (void)hipEventDestroy(T.stop);
@@ -431,6 +446,7 @@ int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz) {
   double *d = nullptr;
   hipError_t err = dev_alloc(&d, 3 * n);
   if (err == hipSuccess) err = launch_global_xyz(e, d, d + n, d + 2 * n);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err == hipSuccess) err = hipMemcpy(gx, d, n * 8, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(gy, d + n, n * 8, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(gz, d + 2 * n, n * 8, hipMemcpyDeviceToHost);
@@ -475,7 +491,54 @@ int rwpt_reset_kernel_time(rwpt_engine *e) {
 int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
              double *density, double *histogram) {
   if (!e || !opt) return RWPT_E_ARG;
-  return fail(e, RWPT_E_UNSUPPORTED, "kde not built yet");
+  if (opt->mode != 0 && opt->mode != 1)
+    return fail(e, RWPT_E_ARG, "kde mode must be 0 or 1");
+  int64_t ncells = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (opt->nbins[d] < 1) return fail(e, RWPT_E_ARG, "nbins must be >= 1");
+    if (!(opt->bin_size[d] > 0.0)) return fail(e, RWPT_E_ARG, "bin_size must be > 0");
+    ncells *= opt->nbins[d];
+  }
+  if (ncells >= 0x7fffffffLL) return fail(e, RWPT_E_ARG, "kde grid too large");
+  if (!e->have_flow) return fail(e, RWPT_E_STATE, "flow not set");
+  const int64_t n = e->part.n;
+  double *dens = nullptr, *hist = nullptr, *m = nullptr;
+  hipError_t err = dev_alloc(&dens, ncells);
+  if (err == hipSuccess) err = dev_alloc(&hist, ncells);
+  if (err == hipSuccess && mass && n > 0) {
+    err = dev_alloc(&m, n);
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(m, mass, n * 8, hipMemcpyHostToDevice, e->stream);
+  }
+  if (err == hipSuccess) err = launch_kde(e, opt, m, dens, hist, e->kde_last);
+  if (err == hipSuccess && density)
+    err = hipMemcpyAsync(density, dens, ncells * 8, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess && histogram)
+    err = hipMemcpyAsync(histogram, hist, ncells * 8, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err == hipSuccess) {
+    for (int k = 1; k <= 2; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e->timers[k].start, e->timers[k].stop);
+      e->timers[k].total_ms += ms;
+      e->timers[k].launches += 1;
+    }
+  }
+  dev_free(dens);
+  dev_free(hist);
+  dev_free(m);
+  if (err != hipSuccess) return hip_fail(e, err, "kde");
+  return RWPT_OK;
+}
+
+int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
+                  double *npoints, double *eff_mass) {
+  if (!e) return RWPT_E_ARG;
+  if (bandwidth) *bandwidth = e->kde_last[0];
+  if (points) *points = e->kde_last[1];
+  if (npoints) *npoints = e->kde_last[2];
+  if (eff_mass) *eff_mass = e->kde_last[3];
+  return RWPT_OK;
 }
 
 }  // extern "C"

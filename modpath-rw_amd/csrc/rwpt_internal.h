@@ -122,6 +122,10 @@ struct rwpt_engine {
   unsigned long long *counters = nullptr;  // device counters
   double *kde_scratch = nullptr;
   int64_t kde_scratch_size = 0;
+  double *kde_weights = nullptr;
+  int64_t kde_weights_size = 0;
+  double kde_last[4] = {0, 0, 0, 0};  // bandwidth, points, nPoints, effMass
+  double *xface = nullptr, *yface = nullptr;  // DIS face coordinates X, Y
   rwpt::KernelTimer timers[3];
   rwpt::Tuning tune;
   std::vector<hipEvent_t> walk_events;  // start/stop pairs per launch

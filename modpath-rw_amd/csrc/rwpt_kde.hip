@@ -1,0 +1,387 @@
+// gpkde reconstruction on the device: histogram of the active particles and
+// one density pass with a uniform (Silverman) bandwidth.
+//
+// GridProjectedKDE%ComputeDensity (GridProjectedKDE.F90:3598-4392) with
+// nOptimizationLoops = 0 scatters, for every active bin, the bin count times
+// the erf-integrated multi-Gaussian kernel and folds the part that falls
+// outside the grid back across the face (KernelMultiGaussian.f90:303-390).
+// With one bandwidth for all bins that kernel is a product of 1-D kernels
+// and the fold is a half-sample symmetric reflection, so the same estimate
+// is three 1-D convolutions with mirrored padding -- O(cells * taps) instead
+// of O(active bins * taps^3).
+//
+// The histogram is exact and reproducible: particles are sorted by bin
+// (stable, so ascending particle index within a bin) and every bin sums its
+// run sequentially, the order the reference's particle loop adds weights in.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "rwpt_internal.h"
+
+namespace rwpt {
+namespace {
+
+struct KdeGrid {
+  double origin[3], bin[3];
+  int nb[3];
+  int active[3];
+};
+
+// Global coordinates (ConvertToModelXYZ) and bin key of every particle;
+// inactive particles and points outside the grid get the key ncells.
+__global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
+                         const int32_t *__restrict__ cellv,
+                         const double *__restrict__ lxv,
+                         const double *__restrict__ lyv,
+                         const double *__restrict__ lzv, int ncol, int nrow,
+                         const double *__restrict__ xface,
+                         const double *__restrict__ yface,
+                         const double *__restrict__ delr,
+                         const double *__restrict__ delc,
+                         const double *__restrict__ top,
+                         const double *__restrict__ bot, const KdeGrid G,
+                         double *__restrict__ gx, double *__restrict__ gy,
+                         double *__restrict__ gz, int32_t *__restrict__ keys,
+                         int32_t *__restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+  vals[p] = (int32_t)p;
+  if (status[p] != 1) {
+    keys[p] = (int32_t)ncells;
+    return;
+  }
+  const int c0 = cellv[p] - 1;
+  const int lc = nrow * ncol, r = c0 % lc, i = r / ncol, j = r % ncol;
+  const double cx = (xface[j] + xface[j + 1]) / 2.0;
+  const double cy = (yface[i] + yface[i + 1]) / 2.0;
+  const double left = cx - (delr[j] / 2.0), right = cx + (delr[j] / 2.0);
+  const double front = cy - (delc[i] / 2.0), back = cy + (delc[i] / 2.0);
+  const double b = bot[c0], t = top[c0];
+  const double x = lxv[p], y = lyv[p], z = lzv[p];
+  const double g[3] = {fmin(fmax(((1.0 - x) * left) + (x * right), left), right),
+                       fmin(fmax(((1.0 - y) * front) + (y * back), front), back),
+                       fmin(fmax(((1.0 - z) * b) + (z * t), b), t)};
+  gx[p] = g[0];
+  gy[p] = g[1];
+  gz[p] = g[2];
+  // Histogram.F90:276-350: floor((x - origin)/bin) + 1 on active dimensions
+  int64_t key = 0, stride = 1;
+  for (int d = 0; d < 3; ++d) {
+    int id = 0;
+    if (G.active[d]) {
+      id = (int)floor((g[d] - G.origin[d]) / G.bin[d]);
+      if (id < 0 || id >= G.nb[d]) {
+        keys[p] = (int32_t)ncells;
+        return;
+      }
+    }
+    key += stride * id;
+    stride *= G.nb[d];
+  }
+  keys[p] = (int32_t)key;
+}
+
+// Per-bin sums over the sorted runs (first element of a run sums it).
+__global__ void kde_runs(int64_t n, const int32_t *__restrict__ keys,
+                         const int32_t *__restrict__ vals,
+                         const double *__restrict__ mass, int64_t ncells,
+                         double *__restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = keys[i];
+  if (k >= ncells) return;
+  if (i > 0 && keys[i - 1] == k) return;
+  double s = 0.0;
+  for (int64_t q = i; q < n && keys[q] == k; ++q)
+    s = s + (mass ? mass[vals[q]] : 1.0);
+  counts[k] = s;
+}
+
+// Deterministic sums over the active particles: each block reduces a fixed
+// contiguous chunk in a fixed tree, the partials are added in block order.
+// mode 0: sum of x, y, z, w, w^2 and the count; mode 1: sum of squared
+// deviations from mean[].
+constexpr int kStatBlocks = 512;
+__global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
+                          const double *__restrict__ gx,
+                          const double *__restrict__ gy,
+                          const double *__restrict__ gz,
+                          const double *__restrict__ mass, int mode,
+                          const double *__restrict__ mean,
+                          double *__restrict__ partial) {
+  __shared__ double sh[6][256];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  for (int64_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+    if (status[p] != 1) continue;
+    if (mode == 0) {
+      const double w = mass ? mass[p] : 1.0;
+      acc[0] += gx[p];
+      acc[1] += gy[p];
+      acc[2] += gz[p];
+      acc[3] += w;
+      acc[4] += w * w;
+      acc[5] += 1.0;
+    } else {
+      const double ex = gx[p] - mean[0], ey = gy[p] - mean[1],
+                   ez = gz[p] - mean[2];
+      acc[0] += ex * ex;
+      acc[1] += ey * ey;
+      acc[2] += ez * ez;
+    }
+  }
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) partial[6 * blockIdx.x + q] = sh[q][0];
+}
+
+__global__ void kde_stats_final(int nparts, const double *__restrict__ partial,
+                                double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nparts; ++b)
+    for (int q = 0; q < 6; ++q) s[q] += partial[6 * b + q];
+  for (int q = 0; q < 6; ++q) out[q] = s[q];
+}
+
+__global__ void kde_scale(int64_t n, const double *__restrict__ in,
+                          double *__restrict__ out, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] * scale;
+}
+
+// One 1-D pass of the separable kernel along `axis` with half-sample
+// symmetric padding (d c b a | a b c d | d c b a); weights w[0..m] are the
+// normalised 1-D kernel at offsets 0..m.
+__global__ void kde_pass(const double *__restrict__ in, double *__restrict__ out,
+                         int nx, int ny, int nz, int axis,
+                         const double *__restrict__ w, int m, double scale) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)nx * ny * nz;
+  if (idx >= total) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((int64_t)nx * ny));
+  const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? nx : (int64_t)nx * ny);
+  const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+  const int64_t base = idx - (int64_t)pos * stride;
+  double s = 0.0;
+  for (int o = -m; o <= m; ++o) {
+    int q = pos + o;
+    while (q < 0 || q >= n) q = q < 0 ? -1 - q : 2 * n - 1 - q;
+    s += w[o < 0 ? -o : o] * in[base + (int64_t)q * stride];
+  }
+  out[idx] = s * scale;
+}
+
+}  // namespace
+
+// Silverman bandwidth + normalised 1-D erf kernel (KernelMultiGaussian.f90
+// prComputeKernelWMatrix): offsets 0..m with m = ceiling(3 h / bin).
+static void kernel_1d(double h, double bin, std::vector<double> &w) {
+  const int m = (int)std::ceil(3 * h / bin);
+  const double hd = h / bin, s2 = std::sqrt(2.0);
+  std::vector<double> e(m + 1);
+  double sum = 0.0;
+  for (int i = 0; i <= m; ++i) {
+    e[i] = std::erf(((double)i + 0.5) / (hd * s2)) -
+           std::erf(((double)i - 0.5) / (hd * s2));
+    sum += (i == 0 ? 1.0 : 2.0) * e[i];
+  }
+  w.assign(m + 1, 0.0);
+  for (int i = 0; i <= m; ++i) w[i] = e[i] / sum;
+}
+
+hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
+                      const double *mass_dev, double *density_dev,
+                      double *hist_dev, double *stats_host) {
+  DeviceProblem &P = e->prob;
+  Particles &S = e->part;
+  const int64_t n = S.n;
+  KdeGrid G{};
+  int nd = 0;
+  for (int d = 0; d < 3; ++d) {
+    G.origin[d] = opt->origin[d];
+    G.bin[d] = opt->bin_size[d];
+    G.nb[d] = opt->nbins[d];
+    G.active[d] = opt->nbins[d] > 1;
+    nd += G.active[d];
+  }
+  const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+  hipStream_t st = e->stream;
+  // scratch: gx gy gz (3n), keys/vals reuse the walk's work lists, two grid
+  // buffers, stats partials
+  const int64_t need = 3 * n + 2 * ncells + 6 * kStatBlocks + 16;
+  hipError_t err = hipSuccess;
+  if (need > e->kde_scratch_size) {
+    if (e->kde_scratch) (void)hipFree(e->kde_scratch);
+    e->kde_scratch = nullptr;
+    e->kde_scratch_size = 0;
+    if ((err = hipMalloc(&e->kde_scratch, need * sizeof(double))) != hipSuccess)
+      return err;
+    e->kde_scratch_size = need;
+  }
+  double *gx = e->kde_scratch, *gy = gx + n, *gz = gy + n;
+  double *tmpA = gz + n, *tmpB = tmpA + ncells;
+  double *partial = tmpB + ncells, *sums = partial + 6 * kStatBlocks;
+  KernelTimer &TH = e->timers[1], &TD = e->timers[2];
+  (void)hipEventRecord(TH.start, st);
+  const int blk = 256;
+  if (n > 0) {
+    hipLaunchKernelGGL(kde_keys, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk),
+                       0, st, n, S.status, S.cell, S.lx, S.ly, S.lz, P.ncol, P.nrow,
+                       e->xface, e->yface, P.delr, P.delc, P.top, P.bot, G, gx, gy,
+                       gz, S.keys[0], S.vals[0]);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    int bits = 1;
+    while ((1ll << bits) <= ncells) ++bits;
+    size_t bytes = 0;
+    err = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, S.keys[0], S.keys[1],
+                                             S.vals[0], S.vals[1], (int)n, 0, bits, st);
+    if (err != hipSuccess) return err;
+    if (bytes > S.sort_tmp_bytes) {
+      if (S.sort_tmp) (void)hipFree(S.sort_tmp);
+      S.sort_tmp = nullptr;
+      S.sort_tmp_bytes = 0;
+      if ((err = hipMalloc(&S.sort_tmp, bytes)) != hipSuccess) return err;
+      S.sort_tmp_bytes = bytes;
+    }
+    bytes = S.sort_tmp_bytes;
+    err = hipcub::DeviceRadixSort::SortPairs(S.sort_tmp, bytes, S.keys[0], S.keys[1],
+                                             S.vals[0], S.vals[1], (int)n, 0, bits, st);
+    if (err != hipSuccess) return err;
+  }
+  if ((err = hipMemsetAsync(hist_dev, 0, ncells * sizeof(double), st)) != hipSuccess)
+    return err;
+  if (n > 0) {
+    hipLaunchKernelGGL(kde_runs, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
+                       st, n, S.keys[1], S.vals[1], mass_dev, ncells, hist_dev);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  (void)hipEventRecord(TH.stop, st);
+  // statistics for the bandwidth and the effective mass
+  double hs[6] = {0, 0, 0, 0, 0, 0}, dev2[6] = {0, 0, 0, 0, 0, 0};
+  if (n > 0) {
+    hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
+                       gx, gy, gz, mass_dev, 0, (const double *)nullptr, partial);
+    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(64), 0, st, kStatBlocks,
+                       partial, sums);
+    if ((err = hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+  }
+  const double nact = hs[5];
+  double mean[3] = {0, 0, 0};
+  if (nact > 0)
+    for (int d = 0; d < 3; ++d) mean[d] = hs[d] / nact;
+  if (n > 0 && nact > 0) {
+    if ((err = hipMemcpyAsync(sums + 8, mean, sizeof mean, hipMemcpyHostToDevice, st)) != hipSuccess)
+      return err;
+    hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
+                       gx, gy, gz, mass_dev, 1, (const double *)(sums + 8), partial);
+    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(64), 0, st, kStatBlocks,
+                       partial, sums);
+    if ((err = hipMemcpyAsync(dev2, sums, sizeof dev2, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+  }
+  double binVolume = 1.0;
+  for (int d = 0; d < 3; ++d)
+    if (G.bin[d] > 0.0) binVolume *= G.bin[d];
+  // effective mass (Histogram.F90:332-341, format 0) and point count: the
+  // reference divides the weighted counts by the effective mass and
+  // multiplies the density back by it; both cancel here, so the bins hold
+  // the plain sums of the masses
+  double effMass = 1.0, nPoints = nact;
+  if (mass_dev) {
+    const double neff = hs[3] * hs[3] / hs[4];
+    effMass = hs[3] / neff;
+  } else {
+    // unit masses: nPoints counts the points inside the grid
+    std::vector<double> hh(ncells);
+    if ((err = hipMemcpyAsync(hh.data(), hist_dev, ncells * sizeof(double),
+                              hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+    nPoints = 0.0;
+    for (int64_t q = 0; q < ncells; ++q) nPoints += hh[q];
+  }
+  double h = 0.0;
+  if (nact > 0 && nd > 0) {
+    double sscale = 1.0;
+    for (int d = 0; d < 3; ++d)
+      if (G.active[d]) sscale *= std::sqrt(dev2[d] / nact);
+    sscale = std::pow(sscale, 1.0 / nd);
+    h = sscale * std::pow(4.0 / ((nd + 2.0) * nPoints), 1.0 / (nd + 4.0));
+    if (sscale == 0.0) h = 2.0 * std::pow(binVolume, 1.0 / nd);
+  }
+  if (stats_host) {
+    stats_host[0] = h;
+    stats_host[1] = nact;
+    stats_host[2] = nPoints;
+    stats_host[3] = effMass;
+  }
+  (void)hipEventRecord(TD.start, st);
+  if (opt->mode == 0 || h <= 0.0) {
+    // histogram only: the raw density counts / binVolume
+    hipLaunchKernelGGL(kde_scale, dim3((unsigned)((ncells + blk - 1) / blk)),
+                       dim3(blk), 0, st, ncells, hist_dev, density_dev,
+                       opt->scaling_factor / binVolume);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  } else {
+    std::vector<double> all;
+    int off[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      if (!G.active[d]) continue;
+      std::vector<double> w;
+      kernel_1d(h, G.bin[d], w);
+      m[d] = (int)w.size() - 1;
+      off[d] = (int)all.size();
+      all.insert(all.end(), w.begin(), w.end());
+    }
+    if ((int64_t)all.size() > e->kde_weights_size) {
+      if (e->kde_weights) (void)hipFree(e->kde_weights);
+      e->kde_weights = nullptr;
+      e->kde_weights_size = 0;
+      if ((err = hipMalloc(&e->kde_weights, all.size() * sizeof(double))) != hipSuccess)
+        return err;
+      e->kde_weights_size = (int64_t)all.size();
+    }
+    if ((err = hipMemcpyAsync(e->kde_weights, all.data(), all.size() * sizeof(double),
+                              hipMemcpyHostToDevice, st)) != hipSuccess)
+      return err;
+    const double *src = hist_dev;
+    int last = -1;
+    for (int d = 0; d < 3; ++d)
+      if (G.active[d]) last = d;
+    int ping = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (!G.active[d]) continue;
+      double *dst = d == last ? density_dev : (ping ? tmpB : tmpA);
+      const double sc = d == last ? opt->scaling_factor / binVolume : 1.0;
+      hipLaunchKernelGGL(kde_pass, dim3((unsigned)((ncells + blk - 1) / blk)),
+                         dim3(blk), 0, st, src, dst, G.nb[0], G.nb[1], G.nb[2], d,
+                         e->kde_weights + off[d], m[d], sc);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      src = dst;
+      ping ^= 1;
+    }
+  }
+  (void)hipEventRecord(TD.stop, st);
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace rwpt

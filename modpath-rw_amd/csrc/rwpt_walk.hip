@@ -1200,41 +1200,15 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
 
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
                              double *gz) {
-  // face coordinates X(1..ncol+1), Y(1..nrow+1) as RectangularGridDis builds
   DeviceProblem &P = e->prob;
-  std::vector<double> hdelr(P.ncol), hdelc(P.nrow);
-  (void)hipMemcpy(hdelr.data(), P.delr, P.ncol * sizeof(double), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(hdelc.data(), P.delc, P.nrow * sizeof(double), hipMemcpyDeviceToHost);
-  std::vector<double> X(P.ncol + 1), Y(P.nrow + 1);
-  X[0] = 0.0;
-  for (int j = 0; j < P.ncol; ++j) X[j + 1] = X[j] + hdelr[j];
-  Y[0] = 0.0;
-  for (int i = 0; i < P.nrow; ++i) Y[0] = Y[0] + hdelc[i];
-  for (int i = 0; i < P.nrow; ++i) Y[i + 1] = Y[i] - hdelc[i];
-  double *dX = nullptr, *dY = nullptr;
-  hipError_t err = hipMalloc(&dX, X.size() * sizeof(double));
-  if (err != hipSuccess) return err;
-  err = hipMalloc(&dY, Y.size() * sizeof(double));
-  if (err != hipSuccess) {
-    (void)hipFree(dX);
-    return err;
-  }
-  (void)hipMemcpyAsync(dX, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice,
-                 e->stream);
-  (void)hipMemcpyAsync(dY, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice,
-                 e->stream);
   const int block = 256;
   const int64_t n = e->part.n;
   if (n > 0)
     hipLaunchKernelGGL(global_xyz_kernel, dim3((unsigned)((n + block - 1) / block)),
                        dim3(block), 0, e->stream, n, e->part.cell, e->part.lx,
-                       e->part.ly, e->part.lz, P.ncol, P.nrow, dX, dY, P.delr,
-                       P.delc, P.top, P.bot, gx, gy, gz);
-  err = hipGetLastError();
-  (void)hipStreamSynchronize(e->stream);
-  (void)hipFree(dX);
-  (void)hipFree(dY);
-  return err;
+                       e->part.ly, e->part.lz, P.ncol, P.nrow, e->xface, e->yface,
+                       P.delr, P.delc, P.top, P.bot, gx, gy, gz);
+  return hipGetLastError();
 }
 
 }  // namespace rwpt

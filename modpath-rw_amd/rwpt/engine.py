@@ -72,6 +72,7 @@ EXPORTS = {
     "rwpt_get_substeps": ([C.c_void_p, _i64p], C.c_int),
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
+    "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp], C.c_int),
     "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
     "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
     "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
@@ -275,3 +276,9 @@ class Engine:
         rc = lib().rwpt_kde(self.h, C.byref(o), _p(m, _dp), _p(dens, _dp), _p(hist, _dp))
         self._check(rc, "rwpt_kde")
         return dens, hist
+
+    def kde_info(self):
+        v = [C.c_double(0) for _ in range(4)]
+        self._check(lib().rwpt_kde_info(self.h, *[C.byref(x) for x in v]), "rwpt_kde_info")
+        return {"bandwidth": v[0].value, "points": v[1].value, "npoints": v[2].value,
+                "eff_mass": v[3].value}

@@ -1647,3 +1647,212 @@ double orc_quadratic_axis(double dt, double n, double p, double d, double v,
   return dtaxis;
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* gpkde: one density pass with the Silverman bandwidth                      */
+/* (GridProjectedKDE.F90 prComputeDensity :3598-4392 with                    */
+/*  nOptimizationLoops = 0 and the brute-force kernel prSetKernelBrute       */
+/*  :6224-6253; KernelMultiGaussian.f90 SetupMatrix :549-590, W :720-790,    */
+/*  spans and reflection :209-390)                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int m[3];      /* matrixPositiveShape */
+  int s[3];      /* full shape 2m+1 */
+  double *mat;   /* column-major (x fastest) */
+} kmat;
+
+static void kmat_setup(kmat *K, const double h[3], const double bin[3],
+                       int ndim) {
+  for (int d = 0; d < 3; ++d) {
+    K->m[d] = bin[d] != 0.0 ? (int)ceil(3 * h[d] / bin[d]) : 0;
+    K->s[d] = 2 * K->m[d] + 1;
+  }
+  size_t n = (size_t)K->s[0] * K->s[1] * K->s[2];
+  K->mat = (double *)malloc(n * sizeof(double));
+  if (h[0] == 0.0 && h[1] == 0.0 && h[2] == 0.0) {
+    memset(K->mat, 0, n * sizeof(double));
+    return;
+  }
+  double hd[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d)
+    if (bin[d] != 0.0) hd[d] = h[d] / bin[d];
+  const double sqrt2 = sqrt(2.0);
+  /* zero-positive octant: 0.5^nDim times the erf differences per active
+   * dimension, in dimension order (the (i +- 0.5) are exact) */
+  int mx = K->m[0], my = K->m[1], mz = K->m[2];
+  double *zp = (double *)malloc((size_t)(mx + 1) * (my + 1) * (mz + 1) *
+                                sizeof(double));
+  for (int k = 0; k <= mz; ++k)
+    for (int j = 0; j <= my; ++j)
+      for (int i = 0; i <= mx; ++i) {
+        double v = pow(0.5, (double)ndim);
+        int idx[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+          if (hd[d] <= 0.0) continue;
+          double a = (double)((float)idx[d] + 0.5f);
+          double b = (double)((float)idx[d] - 0.5f);
+          v = v * (erf(a / (hd[d] * sqrt2)) - erf(b / (hd[d] * sqrt2)));
+        }
+        zp[(size_t)i + (mx + 1) * ((size_t)j + (my + 1) * k)] = v;
+      }
+  /* unfold to the full matrix (prUnfoldZeroPositiveMatrix) */
+  for (int k = 0; k < K->s[2]; ++k)
+    for (int j = 0; j < K->s[1]; ++j)
+      for (int i = 0; i < K->s[0]; ++i) {
+        int a = abs(i - mx), b = abs(j - my), c = abs(k - mz);
+        K->mat[(size_t)i + K->s[0] * ((size_t)j + K->s[1] * k)] =
+            zp[(size_t)a + (mx + 1) * ((size_t)b + (my + 1) * c)];
+      }
+  free(zp);
+  double sum = 0.0;
+  for (size_t q = 0; q < n; ++q) sum = sum + K->mat[q];
+  if (sum != 0.0)
+    for (size_t q = 0; q < n; ++q) K->mat[q] = K->mat[q] / sum;
+}
+
+/* prKernelReflection along one axis of the kernel matrix: kernel indices
+ * [lo_out] fold back into the grid (1-based boundLoc semantics). */
+static void kmat_fold(kmat *K, int axis, int lower, int boundLoc) {
+  int s0 = K->s[0], s1 = K->s[1], s2 = K->s[2];
+  int len = K->s[axis];
+#define AT(i, j, k) K->mat[(size_t)(i) + s0 * ((size_t)(j) + (size_t)s1 * (k))]
+  (void)s2;
+  /* as 1-D lines along `axis` */
+  int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+  int n1 = K->s[o1], n2 = K->s[o2];
+  double *line = (double *)malloc(len * sizeof(double));
+  for (int b = 0; b < n2; ++b)
+    for (int a = 0; a < n1; ++a) {
+      int idx[3];
+      for (int q = 0; q < len; ++q) {
+        idx[axis] = q; idx[o1] = a; idx[o2] = b;
+        line[q] = AT(idx[0], idx[1], idx[2]);
+      }
+      if (lower) {
+        int L = boundLoc; /* kernel 1..L lie outside; fold onto L+1..2L */
+        for (int t = 0; t < L; ++t) line[L + t] = line[L + t] + line[L - 1 - t];
+        for (int t = 0; t < L; ++t) line[t] = 0;
+      } else {
+        int lenb = len - boundLoc + 1; /* kernel boundLoc..len outside */
+        for (int t = 0; t < lenb; ++t)
+          line[boundLoc - 1 - lenb + t] =
+              line[boundLoc - 1 - lenb + t] + line[len - 1 - t];
+        for (int t = boundLoc - 1; t < len; ++t) line[t] = 0;
+      }
+      for (int q = 0; q < len; ++q) {
+        idx[axis] = q; idx[o1] = a; idx[o2] = b;
+        AT(idx[0], idx[1], idx[2]) = line[q];
+      }
+    }
+  free(line);
+#undef AT
+}
+
+int orc_kde_density(long n, const double *x, const double *y,
+                    const double *z, const double *w, const double origin[3],
+                    const double binsize[3], const int nbins[3],
+                    double *density, double *counts, double hout[3]) {
+  const double *pts[3] = {x, y, z};
+  int dmask[3], ndim = 0;
+  for (int d = 0; d < 3; ++d) {
+    dmask[d] = nbins[d] > 1;
+    ndim += dmask[d];
+  }
+  size_t nb = (size_t)nbins[0] * nbins[1] * nbins[2];
+  /* histogram (prComputeCounts / prComputeCountsWeighted, format 0) */
+  orc_histogram(n, x, y, z, w, origin, binsize, nbins, counts);
+  double binVolume = 1.0;
+  for (int d = 0; d < 3; ++d)
+    if (binsize[d] > 0.0) binVolume = binVolume * binsize[d];
+  double effMass = 1.0;
+  long nPoints;
+  if (w) {
+    double tm = 0.0, tm2 = 0.0;
+    for (long p = 0; p < n; ++p) tm = tm + w[p];
+    for (long p = 0; p < n; ++p) tm2 = tm2 + w[p] * w[p];
+    double neff = tm * tm / tm2;
+    effMass = tm / neff;
+    for (size_t q = 0; q < nb; ++q) counts[q] = counts[q] / effMass;
+    nPoints = n;
+  } else {
+    double tm = 0.0;
+    for (size_t q = 0; q < nb; ++q) tm = tm + counts[q];
+    nPoints = (long)tm;
+  }
+  /* Silverman bandwidth (:4034-4054) over all points passed */
+  double mean[3], sd[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+    for (long p = 0; p < n; ++p) s = s + pts[d][p];
+    mean[d] = s / (double)n;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (!dmask[d]) continue;
+    double s = 0.0;
+    for (long p = 0; p < n; ++p) {
+      double e = pts[d][p] - mean[d];
+      s = s + e * e;
+    }
+    sd[d] = sqrt(s / (double)n);
+  }
+  double sscale = 1.0;
+  for (int d = 0; d < 3; ++d)
+    if (dmask[d]) sscale = sscale * sd[d];
+  sscale = pow(sscale, 1.0 / (double)ndim);
+  double hs = sscale * pow(4.0 / ((ndim + 2.0) * (double)nPoints),
+                           1.0 / (ndim + 4.0));
+  if (sscale == 0.0) {
+    double bd = pow(binVolume, 1.0 / (double)ndim);
+    hs = 2.0 * bd;
+  }
+  double h[3];
+  for (int d = 0; d < 3; ++d) h[d] = dmask[d] ? hs : 0.0;
+  if (hout)
+    for (int d = 0; d < 3; ++d) hout[d] = h[d];
+  memset(density, 0, nb * sizeof(double));
+  /* scatter of every active bin's reflected kernel (:4623-4661) */
+  for (int k = 0; k < nbins[2]; ++k)
+    for (int j = 0; j < nbins[1]; ++j)
+      for (int i = 0; i < nbins[0]; ++i) {
+        size_t b = (size_t)i + nbins[0] * ((size_t)j + (size_t)nbins[1] * k);
+        double c = counts[b];
+        if (c == 0.0) continue;
+        kmat K;
+        kmat_setup(&K, h, binsize, ndim);
+        int id[3] = {i + 1, j + 1, k + 1}, gs[3], ks[3];
+        for (int d = 0; d < 3; ++d) {
+          gs[d] = id[d] - K.m[d] < 1 ? 1 : id[d] - K.m[d];
+          ks[d] = gs[d] + K.m[d] - id[d] + 1;
+        }
+        for (int d = 0; d < 3; ++d) {
+          if (!dmask[d]) continue;
+          if (id[d] - K.m[d] < 1) kmat_fold(&K, d, 1, 1 - id[d] + K.m[d]);
+          if (id[d] + K.m[d] > nbins[d])
+            kmat_fold(&K, d, 0,
+                      2 * K.m[d] + 1 - (id[d] + K.m[d] - nbins[d]) + 1);
+        }
+        int ge[3];
+        for (int d = 0; d < 3; ++d)
+          ge[d] = id[d] + K.m[d] > nbins[d] ? nbins[d] : id[d] + K.m[d];
+        for (int gk = gs[2]; gk <= ge[2]; ++gk)
+          for (int gj = gs[1]; gj <= ge[1]; ++gj)
+            for (int gi = gs[0]; gi <= ge[0]; ++gi) {
+              int mi = ks[0] + gi - gs[0] - 1, mj = ks[1] + gj - gs[1] - 1,
+                  mk = ks[2] + gk - gs[2] - 1;
+              size_t g = (size_t)(gi - 1) +
+                         nbins[0] * ((size_t)(gj - 1) + (size_t)nbins[1] * (gk - 1));
+              density[g] = density[g] +
+                           c * K.mat[(size_t)mi + K.s[0] * ((size_t)mj + (size_t)K.s[1] * mk)];
+            }
+        free(K.mat);
+      }
+  for (size_t q = 0; q < nb; ++q) density[q] = density[q] / binVolume;
+  if (w) {
+    for (size_t q = 0; q < nb; ++q) {
+      counts[q] = counts[q] * effMass;
+      density[q] = density[q] * effMass;
+    }
+  }
+  return 0;
+}

@@ -88,6 +88,17 @@ void orc_histogram(long n, const double *x, const double *y, const double *z,
                    const double binsize[3], const int nbins[3],
                    double *counts);
 
+/* gpkde ComputeDensity with nOptimizationLoops = 0 and the brute-force
+ * kernel: histogram (weights optional, effective-weight format 0), the
+ * Silverman bandwidth over all points, one scatter of the reflected
+ * multi-Gaussian of every active bin, divided by the bin volume.  Grids are
+ * column-major (x fastest); dimensions with one bin are inactive.  hout
+ * (optional) receives the bandwidth used. */
+int orc_kde_density(long n, const double *x, const double *y,
+                    const double *z, const double *w, const double origin[3],
+                    const double binsize[3], const int nbins[3],
+                    double *density, double *counts, double hout[3]);
+
 /* exit-time solvers for one axis (Newton: TrackSubCell.F90:2451-2554;
  * quadratic: one axis of :1576-1892, exit faces 1/2) */
 double orc_newton_axis(double dt, double v, double v1, double v2, double dx,

@@ -1,0 +1,91 @@
+"""GPU parity of the gpkde reconstruction (rwpt_kde) against the oracle.
+
+Unit-mass histogram counts are integers and must agree exactly; weighted
+counts sum each bin in the reference's particle order and agree to
+rounding of the effective-mass round trip the reference performs (1e-14).
+The density is the reference's scatter restated as three separable passes,
+so it agrees to summation order (1e-10 relative, 1e-13 of the maximum).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rwpt import Engine, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _walked(P, where, n, t_end, steps):
+    if where is None:
+        rel = problems.release_in_block(P, n, (3, 8), (6, 15), (1, P.nlay), seed=3)
+    else:
+        rel = problems.release_in_cell(P, n, *where, seed=3, lz=0.5)
+    E = Engine(P)
+    E.load_particles(*rel)
+    for k in range(1, steps + 1):
+        E.advance(t_end * k / steps)
+    return E
+
+
+def _grid(P):
+    nb = (P.ncol, P.nrow, P.nlay)
+    return (0.0, 0.0, float(P.bot.min())), (float(P.delr[0]), float(P.delc[0]),
+                                            float(P.top[0] - P.bot[0])), nb
+
+
+@pytest.mark.parametrize("case", ["cfg0", "hetero3d", "hetero3d_weighted"])
+def test_kde_matches_oracle(case):
+    if case == "cfg0":
+        P = problems.config0(ts_params=(0.05, 0.0))
+        E = _walked(P, (1, 25, 10), 4000, 10.0, 4)
+    else:
+        P = problems.heterogeneous_flow(24, 20, 6, corr_len=4.0, seed=7)
+        P.aL, P.aT, P.dmeff = np.array([0.5]), np.array([0.05]), np.array([1e-4])
+        P.ts_params = (0.5, 0.0)
+        E = _walked(P, None, 4000, 4.0, 2)
+    try:
+        origin, binsize, nbins = _grid(P)
+        g = E.particles()
+        mass = None
+        if case.endswith("weighted"):
+            mass = np.random.default_rng(5).uniform(0.2, 3.0, g["cell"].size)
+        dens, hist = E.kde(origin, binsize, nbins, mode=1, mass=mass)
+        info = E.kde_info()
+        raw, hist0 = E.kde(origin, binsize, nbins, mode=0, mass=mass)
+    finally:
+        E.close()
+    act = g["status"] == 1
+    gx, gy, gz = O.global_xyz(P, g["cell"], g["lx"], g["ly"], g["lz"])
+    od, oc, oh = O.kde_density(gx[act], gy[act], gz[act], origin, binsize, nbins,
+                               None if mass is None else mass[act])
+    if mass is None:
+        assert np.array_equal(hist, oc)
+    else:
+        np.testing.assert_allclose(hist, oc, rtol=1e-14, atol=0)
+    assert abs(info["bandwidth"] - oh[0]) <= 1e-12 * oh[0]
+    np.testing.assert_allclose(dens, od, rtol=1e-10, atol=1e-13 * od.max())
+    np.testing.assert_array_equal(hist0, hist)
+    vol = np.prod(binsize)
+    np.testing.assert_allclose(raw, hist / vol, rtol=1e-15)
+    assert abs(dens.sum() * vol - hist.sum()) <= 1e-9 * hist.sum()
+
+
+def test_kde_full_size_mass_conservation():
+    P = problems.config1()
+    n = 1_000_000
+    rel = problems.release_in_block(P, n, (1, 10), (76, 125), (1, 20))
+    E = Engine(P)
+    try:
+        E.load_particles(*rel)
+        E.advance(25.0)
+        dens, hist = E.kde((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (200, 200, 20), mode=1)
+        info = E.kde_info()
+        st = E.particles()["status"]
+        dens2, hist2 = E.kde((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (200, 200, 20), mode=1)
+    finally:
+        E.close()
+    assert hist.sum() == (st == 1).sum() == info["points"]
+    assert np.all(hist == np.round(hist))
+    assert abs(dens.sum() - hist.sum()) <= 1e-9 * hist.sum()
+    assert np.array_equal(dens, dens2) and np.array_equal(hist, hist2)
+    assert info["bandwidth"] > 1.0
