@@ -8,8 +8,12 @@ dt = 0.5 d, timeseries every 50 steps -- so one bench step advances every
 particle by 50 RW time steps (25 d) and --steps 10 is the whole 500-step
 run.  Particle state is resident in HBM before the timed region.
 
+Each interval ends with the gpkde density reconstruction of the active
+particles on the 200x200x20 flow grid (timed apart, reported as KDE cells/s;
+the wall time includes the host copies of the C ABI).
+
 Prints one JSON line: particle-steps/s (accepted RW sub-steps, the unit the
-reference's TrackSubCell loop iterates), the roofline of the advance kernel
+reference's TrackSubCell loop iterates, over the walk's wall time), the roofline of the advance kernel
 (algorithmic bytes = 96 B of SoA state per particle-step, HBM bound, 8 TB/s)
 and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
 sample of the same workload).
@@ -92,17 +96,30 @@ def main():
     E.set_time(0.0)
     E.reset_kernel_time()
 
+    # each interval ends with the gpkde reconstruction of the active plume on
+    # the flow grid (1 m bins, Silverman bandwidth, reflected kernel)
+    nbins = (P.ncol, P.nrow, P.nlay)
+    kde_cells = int(np.prod(nbins))
     total_steps = 0
+    walk_s = kde_s = 0.0
     t0 = time.perf_counter()
     for k in range(args.steps):
+        ta = time.perf_counter()
         _, ns = E.advance((k + 1) * INTERVAL_STEPS * DT)
+        tb = time.perf_counter()
+        E.kde((0.0, 0.0, float(P.bot.min())), (1.0, 1.0, 1.0), nbins, mode=1)
+        kde_s += time.perf_counter() - tb
+        walk_s += tb - ta
         total_steps += ns
     el = time.perf_counter() - t0
     kms, launches = E.kernel_time(0)
+    hms, _ = E.kernel_time(1)
+    dms, _ = E.kernel_time(2)
+    bw = E.kde_info()["bandwidth"]
     state = E.particles()
     E.close()
 
-    value = total_steps / el
+    value = total_steps / walk_s
     avg_launch_s = (kms / 1e3) / max(launches, 1)
     bytes_per_launch = BYTES_PER_STEP_F64 * total_steps / max(launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -142,6 +159,12 @@ def main():
                      "launches": launches,
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "kde": {"unit": "KDE cells/s", "grid": list(nbins),
+                "reconstructions": args.steps,
+                "cells_per_s_wall": kde_cells * args.steps / kde_s,
+                "cells_per_s_kernels": kde_cells * args.steps / ((hms + dms) / 1e3),
+                "histogram_ms": hms / args.steps, "density_ms": dms / args.steps,
+                "wall_ms": kde_s * 1e3 / args.steps, "last_bandwidth": bw},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)

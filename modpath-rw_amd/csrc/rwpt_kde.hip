@@ -10,7 +10,8 @@
 // is three 1-D convolutions with mirrored padding -- O(cells * taps) instead
 // of O(active bins * taps^3).
 //
-// The histogram is exact and reproducible: particles are sorted by bin
+// The histogram is exact and reproducible: unit masses are integer counts
+// (atomic increments, order-free); weighted particles are sorted by bin
 // (stable, so ascending particle index within a bin) and every bin sums its
 // run sequentially, the order the reference's particle loop adds weights in.
 #include <algorithm>
@@ -102,19 +103,22 @@ __global__ void kde_runs(int64_t n, const int32_t *__restrict__ keys,
 }
 
 // Deterministic sums over the active particles: each block reduces a fixed
-// contiguous chunk in a fixed tree, the partials are added in block order.
-// mode 0: sum of x, y, z, w, w^2 and the count; mode 1: sum of squared
-// deviations from mean[].
+// contiguous chunk in a fixed tree, then one block reduces the partials in
+// a fixed tree. mode 0: sum of x, y, z, w, w^2, the active count and the
+// count of points inside the grid; mode 1: sum of squared deviations from
+// mean[].
 constexpr int kStatBlocks = 512;
+constexpr int kStats = 8;
 __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
+                          const int32_t *__restrict__ keys, int64_t ncells,
                           const double *__restrict__ gx,
                           const double *__restrict__ gy,
                           const double *__restrict__ gz,
                           const double *__restrict__ mass, int mode,
                           const double *__restrict__ mean,
                           double *__restrict__ partial) {
-  __shared__ double sh[6][256];
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  __shared__ double sh[kStats][256];
+  double acc[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
@@ -128,6 +132,7 @@ __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
       acc[3] += w;
       acc[4] += w * w;
       acc[5] += 1.0;
+      acc[6] += keys[p] < ncells ? 1.0 : 0.0;
     } else {
       const double ex = gx[p] - mean[0], ey = gy[p] - mean[1],
                    ez = gz[p] - mean[2];
@@ -136,24 +141,46 @@ __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
       acc[2] += ez * ez;
     }
   }
-  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < kStats; ++q) sh[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+      for (int q = 0; q < kStats; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < 6; ++q) partial[6 * blockIdx.x + q] = sh[q][0];
+    for (int q = 0; q < kStats; ++q) partial[kStats * blockIdx.x + q] = sh[q][0];
 }
 
-__global__ void kde_stats_final(int nparts, const double *__restrict__ partial,
+// One block of kStatBlocks threads: fixed-tree sum of the block partials.
+__global__ void kde_stats_final(const double *__restrict__ partial,
                                 double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < nparts; ++b)
-    for (int q = 0; q < 6; ++q) s[q] += partial[6 * b + q];
-  for (int q = 0; q < 6; ++q) out[q] = s[q];
+  __shared__ double sh[kStats][kStatBlocks];
+  const int t = threadIdx.x;
+  for (int q = 0; q < kStats; ++q) sh[q][t] = partial[kStats * t + q];
+  __syncthreads();
+  for (int s = kStatBlocks / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int q = 0; q < kStats; ++q) sh[q][t] += sh[q][t + s];
+    __syncthreads();
+  }
+  if (t < kStats) out[t] = sh[t][0];
+}
+
+// Unit masses: integer bin counts by atomics (order-free, exact), then
+// converted to the double histogram.
+__global__ void kde_count(int64_t n, const int32_t *__restrict__ keys,
+                          int64_t ncells, uint32_t *__restrict__ cnt) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t k = keys[p];
+  if (k < ncells) atomicAdd(&cnt[k], 1u);
+}
+
+__global__ void kde_count_to_hist(int64_t ncells, const uint32_t *__restrict__ cnt,
+                                  double *__restrict__ hist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncells) hist[i] = (double)cnt[i];
 }
 
 __global__ void kde_scale(int64_t n, const double *__restrict__ in,
@@ -224,7 +251,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   hipStream_t st = e->stream;
   // scratch: gx gy gz (3n), keys/vals reuse the walk's work lists, two grid
   // buffers, stats partials
-  const int64_t need = 3 * n + 2 * ncells + 6 * kStatBlocks + 16;
+  const int64_t need = 3 * n + 2 * ncells + kStats * kStatBlocks + 2 * kStats;
   hipError_t err = hipSuccess;
   if (need > e->kde_scratch_size) {
     if (e->kde_scratch) (void)hipFree(e->kde_scratch);
@@ -236,7 +263,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   }
   double *gx = e->kde_scratch, *gy = gx + n, *gz = gy + n;
   double *tmpA = gz + n, *tmpB = tmpA + ncells;
-  double *partial = tmpB + ncells, *sums = partial + 6 * kStatBlocks;
+  double *partial = tmpB + ncells, *sums = partial + kStats * kStatBlocks;
   KernelTimer &TH = e->timers[1], &TD = e->timers[2];
   (void)hipEventRecord(TH.start, st);
   const int blk = 256;
@@ -246,6 +273,23 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                        e->xface, e->yface, P.delr, P.delc, P.top, P.bot, G, gx, gy,
                        gz, S.keys[0], S.vals[0]);
     if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  if (n > 0 && !mass_dev) {
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(tmpA);
+    if ((err = hipMemsetAsync(cnt, 0, ncells * sizeof(uint32_t), st)) != hipSuccess)
+      return err;
+    hipLaunchKernelGGL(kde_count, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
+                       st, n, S.keys[0], ncells, cnt);
+    hipLaunchKernelGGL(kde_count_to_hist, dim3((unsigned)((ncells + blk - 1) / blk)),
+                       dim3(blk), 0, st, ncells, cnt, hist_dev);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  } else {
+    if ((err = hipMemsetAsync(hist_dev, 0, ncells * sizeof(double), st)) != hipSuccess)
+      return err;
+  }
+  if (n > 0 && mass_dev) {
+    // weighted: stable sort by bin so that every bin adds its masses in
+    // ascending particle order
     int bits = 1;
     while ((1ll << bits) <= ncells) ++bits;
     size_t bytes = 0;
@@ -263,22 +307,18 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     err = hipcub::DeviceRadixSort::SortPairs(S.sort_tmp, bytes, S.keys[0], S.keys[1],
                                              S.vals[0], S.vals[1], (int)n, 0, bits, st);
     if (err != hipSuccess) return err;
-  }
-  if ((err = hipMemsetAsync(hist_dev, 0, ncells * sizeof(double), st)) != hipSuccess)
-    return err;
-  if (n > 0) {
     hipLaunchKernelGGL(kde_runs, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
                        st, n, S.keys[1], S.vals[1], mass_dev, ncells, hist_dev);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   }
   (void)hipEventRecord(TH.stop, st);
   // statistics for the bandwidth and the effective mass
-  double hs[6] = {0, 0, 0, 0, 0, 0}, dev2[6] = {0, 0, 0, 0, 0, 0};
+  double hs[kStats] = {0, 0, 0, 0, 0, 0, 0, 0}, dev2[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (n > 0) {
     hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       gx, gy, gz, mass_dev, 0, (const double *)nullptr, partial);
-    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(64), 0, st, kStatBlocks,
-                       partial, sums);
+                       S.keys[0], ncells, gx, gy, gz, mass_dev, 0,
+                       (const double *)nullptr, partial);
+    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
     if ((err = hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
       return err;
     if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
@@ -288,12 +328,12 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   if (nact > 0)
     for (int d = 0; d < 3; ++d) mean[d] = hs[d] / nact;
   if (n > 0 && nact > 0) {
-    if ((err = hipMemcpyAsync(sums + 8, mean, sizeof mean, hipMemcpyHostToDevice, st)) != hipSuccess)
+    if ((err = hipMemcpyAsync(sums + kStats, mean, sizeof mean, hipMemcpyHostToDevice, st)) != hipSuccess)
       return err;
     hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       gx, gy, gz, mass_dev, 1, (const double *)(sums + 8), partial);
-    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(64), 0, st, kStatBlocks,
-                       partial, sums);
+                       S.keys[0], ncells, gx, gy, gz, mass_dev, 1,
+                       (const double *)(sums + kStats), partial);
+    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
     if ((err = hipMemcpyAsync(dev2, sums, sizeof dev2, hipMemcpyDeviceToHost, st)) != hipSuccess)
       return err;
     if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
@@ -311,13 +351,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     effMass = hs[3] / neff;
   } else {
     // unit masses: nPoints counts the points inside the grid
-    std::vector<double> hh(ncells);
-    if ((err = hipMemcpyAsync(hh.data(), hist_dev, ncells * sizeof(double),
-                              hipMemcpyDeviceToHost, st)) != hipSuccess)
-      return err;
-    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
-    nPoints = 0.0;
-    for (int64_t q = 0; q < ncells; ++q) nPoints += hh[q];
+    nPoints = hs[6];
   }
   double h = 0.0;
   if (nact > 0 && nd > 0) {

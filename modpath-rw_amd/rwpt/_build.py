@@ -47,7 +47,22 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+REFERENCE_SRC = "/root/reference/src"
+
+
+def build_reference():
+    """Compiles the reference's own Fortran engine into oracle/_ref (the
+    checker of the oracle) where the reference sources are present."""
+    if not os.path.isdir(REFERENCE_SRC):
+        return False
+    if not os.path.exists("/opt/rocm/lib/llvm/bin/amdflang"):
+        return False
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), f"REF={REFERENCE_SRC}"])
+    return True
+
+
 if __name__ == "__main__":
     build_library()
     if "--oracle" in sys.argv:
         build_oracle()
+        build_reference()

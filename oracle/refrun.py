@@ -1,0 +1,85 @@
+"""Runs the compiled reference driver (oracle/_ref/rwpt_ref, built by
+oracle/ref/Makefile from the reference's own Fortran sources) on a problem
+and a particle cloud.  Test infrastructure: it checks the oracle, nothing in
+the product calls it.
+
+The driver tracks particle k on OpenMP thread k, so the reference's
+per-thread rng_par_zig streams become per-particle streams; particle counts
+are kept to a few hundred.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BINARY = os.path.join(HERE, "_ref", "rwpt_ref")
+
+
+def available() -> bool:
+    return os.path.exists(BINARY)
+
+
+def _filled(a, n, value, dtype):
+    if a is None:
+        return np.full(n, value, dtype)
+    return np.ascontiguousarray(a, dtype)
+
+
+def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
+    """Tracks the particles from time 0 through the output times t_out.
+    Returns a list (one per output time) of dicts cell/status/lx/ly/lz/t."""
+    n = int(np.asarray(cell).size)
+    nc = P.ncell
+    nd = int(np.asarray(P.aL).size)
+    t_out = np.ascontiguousarray(t_out, np.float64)
+    hdr = np.zeros(20, np.int32)
+    hdr[:5] = (P.ncol, P.nrow, P.nlay, n, t_out.size)
+    hdr[5], hdr[6] = P.advection_kind, P.time_step_kind
+    hdr[7:10] = P.dim_mask
+    hdr[10], hdr[11], hdr[12] = P.disp_model, nd, P.default_icbound
+    hdr[13], hdr[14] = P.backward, P.uniform_porosity
+    hdr[15] = getattr(P, "stop_weak_sinks", 0)
+    hdr[16] = getattr(P, "stop_weak_sources", 0)
+    hdr[17] = getattr(P, "stop_zone", 0)
+    hdr[19] = 1
+    f8 = np.float64
+    parts = [hdr, np.asarray(P.ts_params, f8),
+             _filled(P.delr, P.ncol, 0, f8), _filled(P.delc, P.nrow, 0, f8),
+             _filled(P.top, nc, 0, f8), _filled(P.bot, nc, 0, f8),
+             _filled(P.frf, nc, 0, f8), _filled(P.fff, nc, 0, f8),
+             _filled(P.flf, nc, 0, f8), _filled(P.bflow, 6 * nc, 0, f8),
+             _filled(P.porosity, nc, 0, f8), _filled(P.retardation, nc, 1.0, f8),
+             _filled(P.top, nc, 0, f8),                     # heads at the cell top
+             np.zeros(nc, f8), np.zeros(nc, f8),            # source, sink flows
+             _filled(P.ibound, nc, 1, np.int32), _filled(P.icbound, nc, 0, np.int32),
+             np.zeros(nc, np.int32),                        # zones
+             _filled(P.aL, nd, 0, f8), _filled(P.aT, nd, 0, f8),
+             _filled(P.aLV, nd, 0, f8), _filled(P.aTV, nd, 0, f8),
+             _filled(P.dmeff, nd, 0, f8),
+             np.ascontiguousarray(cell, np.int32), np.ascontiguousarray(lx, f8),
+             np.ascontiguousarray(ly, f8), np.ascontiguousarray(lz, f8),
+             _filled(t, n, 0.0, f8), _filled(status, n, 1, np.int32), t_out]
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as fh:
+            for a in parts:
+                fh.write(np.ascontiguousarray(a).tobytes())
+        env = dict(os.environ, OMP_STACKSIZE="2M", OMP_THREAD_LIMIT=str(max(n, 1)),
+                   OMP_DYNAMIC="false")
+        subprocess.run([BINARY, fin, fout], check=True, timeout=timeout, env=env,
+                       stdout=subprocess.DEVNULL)
+        raw = open(fout, "rb").read()
+    out, off = [], 0
+    for _ in range(t_out.size):
+        rec = {}
+        for name, dt in (("cell", np.int32), ("status", np.int32), ("lx", f8),
+                         ("ly", f8), ("lz", f8), ("t", f8)):
+            sz = n * np.dtype(dt).itemsize
+            rec[name] = np.frombuffer(raw[off:off + sz], dt).copy()
+            off += sz
+        out.append(rec)
+    return out

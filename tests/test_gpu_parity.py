@@ -12,79 +12,11 @@ import pytest
 
 from oracle import oracle as O
 from rwpt import Engine, problems
+from scenarios import SCEN, release as _release
 
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("cell", "status", "lx", "ly", "lz", "t", "substeps", "rng")
-
-
-def _hetero(**kw):
-    P = problems.heterogeneous_flow(24, 20, 6, corr_len=4.0, seed=7)
-    P.aL, P.aT, P.dmeff = np.array([0.5]), np.array([0.05]), np.array([1e-4])
-    P.ts_params = (0.5, 0.0)
-    for k, v in kw.items():
-        setattr(P, k, v)
-    return P
-
-
-def _scenarios():
-    s = {}
-    s["cfg0"] = (problems.config0(), (1, 25, 10), 0.25, 12)
-    s["cfg0_reflect"] = (problems.config0(default_icbound=1), (1, 2, 97), 0.25, 12)
-    s["hetero3d"] = (_hetero(), None, 0.5, 8)
-    P = _hetero(default_icbound=1)
-    P.ibound = np.ones(P.ncell, np.int32)
-    P.ibound[P.cell_number(3, 10, 12) - 1] = 0
-    P.ibound[P.cell_number(2, 8, 9) - 1] = 0
-    P.icbound = np.zeros(P.ncell, np.int32)
-    P.icbound[P.cell_number(4, 12, 6) - 1] = 1
-    P.retardation = np.random.default_rng(2).uniform(1.0, 1.5, P.ncell)
-    s["hetero_ibound_rebound_R"] = (P, None, 0.5, 8)
-    # axisymmetric model with aLV > aLH and aTV < aTH: the weights of
-    # TrackSubCell.F90:792-797 stay positive
-    P = _hetero(disp_model=1)
-    P.aLV, P.aTV = np.array([0.8]), np.array([0.02])
-    s["axisymmetric"] = (P, None, 0.5, 8)
-    # the reference's cosinesq = vz^2/|v| (not vz^2/|v|^2) exceeds 1 and
-    # drives alphaL negative here: sqrt of a negative gives NaN positions on
-    # both sides, and the NaNs must land on the same particles
-    P = _hetero(disp_model=1)
-    P.aLV, P.aTV = np.array([0.2]), np.array([0.01])
-    s["axisymmetric_nan"] = (P, None, 0.5, 8)
-    P = _hetero(time_step_kind=3, ts_params=(0.3, 0.2))
-    s["min_adv_disp"] = (P, None, 1.0, 6)
-    P = _hetero(time_step_kind=1, ts_params=(0.5, 0.0), dim_mask=(1, 0, 1))
-    s["adv_xz_2d"] = (P, None, 1.0, 6)
-    P = _hetero(dim_mask=(0, 1, 0))
-    s["1d_y"] = (P, None, 0.5, 6)
-    P = _hetero()
-    P.aL = np.random.default_rng(4).uniform(0.1, 1.0, P.ncell)
-    P.aT = P.aL * 0.1
-    P.dmeff = np.full(P.ncell, 1e-4)
-    s["distributed_dispersivity"] = (P, None, 0.5, 6)
-    P = _hetero(backward=1)
-    s["backward"] = (P, None, 0.5, 6)
-    # non-uniform spacings: powers of two take the exact-reciprocal path in
-    # the device code, the others the IEEE division path
-    P = _hetero()
-    g = np.random.default_rng(8)
-    P.delr = g.choice([0.5, 1.0, 1.5, 2.0, 0.7], P.ncol)
-    P.delc = g.choice([0.25, 1.0, 1.3], P.nrow)
-    dz = g.choice([1.0, 2.0, 0.6], P.nlay)
-    tops = np.concatenate([[dz.sum()], dz.sum() - np.cumsum(dz)[:-1]])
-    P.top = np.repeat(tops, P.nrow * P.ncol)
-    P.bot = np.repeat(tops - dz, P.nrow * P.ncol)
-    s["nonuniform_spacing"] = (P, None, 0.5, 6)
-    return s
-
-
-SCEN = _scenarios()
-
-
-def _release(P, where, n, seed=5):
-    if where is None:
-        return problems.release_in_block(P, n, (3, 8), (6, 15), (1, P.nlay), seed=seed)
-    return problems.release_in_cell(P, n, *where, seed=seed, lz=0.5)
 
 
 @pytest.mark.parametrize("name", sorted(SCEN))
