@@ -80,6 +80,10 @@ typedef struct {
                             with the Silverman bandwidth (uniform h)     */
   int32_t weighted;      /* 1: particles carry mass                     */
   double scaling_factor; /* multiplies the density (gpkdeScalingFactor)  */
+  int32_t unit_volume;   /* mode 1: 1 returns the density times the bin
+                            volume before scaling_factor, as
+                            ComputeDensity(unitVolume=.true.) does for
+                            MPathRW.F90:1626-1642; 0 the density itself */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,
@@ -175,7 +179,9 @@ int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
                   double *npoints, double *eff_mass);
 
 /* Per-cell tables as built on the device (checking aid), ncell * 104
- * doubles in the layout documented in DESIGN.md. */
+ * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,
+ * corner discharge qx qy qz |q| (8 x 4), corner porosity (8), corner phi*D
+ * Dxx Dyy Dzz Dxy Dxz Dyz (6 x 8), dry flag, sub-cell exit flag, 2 unused. */
 int rwpt_get_cell_tables(rwpt_engine *e, double *out);
 
 /* Device timing of the last rwpt_advance/rwpt_kde calls, measured with HIP

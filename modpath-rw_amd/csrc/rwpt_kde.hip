@@ -405,7 +405,9 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     for (int d = 0; d < 3; ++d) {
       if (!G.active[d]) continue;
       double *dst = d == last ? density_dev : (ping ? tmpB : tmpA);
-      const double sc = d == last ? opt->scaling_factor / binVolume : 1.0;
+      const double sc = d != last ? 1.0
+                        : (opt->unit_volume ? opt->scaling_factor
+                                            : opt->scaling_factor / binVolume);
       hipLaunchKernelGGL(kde_pass, dim3((unsigned)((ncells + blk - 1) / blk)),
                          dim3(blk), 0, st, src, dst, G.nb[0], G.nb[1], G.nb[2], d,
                          e->kde_weights + off[d], m[d], sc);

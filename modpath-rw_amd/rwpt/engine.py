@@ -45,7 +45,8 @@ class RwptOptions(C.Structure):
 class RwptKdeOptions(C.Structure):
     _fields_ = [("origin", C.c_double * 3), ("bin_size", C.c_double * 3),
                 ("nbins", C.c_int32 * 3), ("mode", C.c_int32),
-                ("weighted", C.c_int32), ("scaling_factor", C.c_double)]
+                ("weighted", C.c_int32), ("scaling_factor", C.c_double),
+                ("unit_volume", C.c_int32)]
 
 
 EXPORTS = {
@@ -260,7 +261,8 @@ class Engine:
     def reset_kernel_time(self):
         self._check(lib().rwpt_reset_kernel_time(self.h), "rwpt_reset_kernel_time")
 
-    def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0):
+    def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0,
+            unit_volume=False):
         o = RwptKdeOptions()
         for d in range(3):
             o.origin[d] = origin[d]
@@ -269,6 +271,7 @@ class Engine:
         o.mode = mode
         o.weighted = int(mass is not None)
         o.scaling_factor = scaling_factor
+        o.unit_volume = int(bool(unit_volume))
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)

@@ -83,3 +83,33 @@ def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
             off += sz
         out.append(rec)
     return out
+
+
+GPKDE_BINARY = os.path.join(HERE, "_ref", "gpkde_ref")
+
+
+def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
+          database=False, isotropic=False, unit_volume=True, scaling_factor=1.0,
+          smoothing_selection=0, smoothing_factor=1.0, timeout=600):
+    """Runs the reference's GridProjectedKDE on the points; returns
+    (density, histogram counts, initial smoothing) with x fastest."""
+    n = int(np.asarray(x).size)
+    hdr = np.array([n, *nbins, n_opt_loops, int(database), int(isotropic),
+                    int(weights is not None), int(unit_volume), smoothing_selection,
+                    0, 0], np.int32)
+    f8 = np.float64
+    w = np.ones(n) if weights is None else np.ascontiguousarray(weights, f8)
+    parts = [hdr, np.asarray(origin, f8), np.asarray(bin_size, f8),
+             np.array([scaling_factor, smoothing_factor], f8),
+             np.ascontiguousarray(x, f8), np.ascontiguousarray(y, f8),
+             np.ascontiguousarray(z, f8), w]
+    nb = int(np.prod(nbins))
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as fh:
+            for a in parts:
+                fh.write(np.ascontiguousarray(a).tobytes())
+        subprocess.run([GPKDE_BINARY, fin, fout], check=True, timeout=timeout,
+                       stdout=subprocess.DEVNULL, cwd=d)
+        raw = np.fromfile(fout, f8)
+    return raw[:nb].copy(), raw[nb:2 * nb].copy(), raw[2 * nb:2 * nb + 3].copy()

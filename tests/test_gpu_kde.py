@@ -92,3 +92,27 @@ def test_kde_full_size_mass_conservation():
     assert abs(dens.sum() - hist.sum()) <= 1e-9 * hist.sum()
     assert np.array_equal(dens, dens2) and np.array_equal(hist, hist2)
     assert info["bandwidth"] > 1.0
+
+
+@pytest.mark.parametrize("unit_volume", [False, True])
+def test_kde_bins_unlike_cells(unit_volume):
+    """Bins of 0.5 x 2 x 1.5 m over the 1 m flow cells, a scaling factor,
+    and the unit-volume form MPathRW requests."""
+    P = problems.heterogeneous_flow(24, 20, 6, corr_len=4.0, seed=7)
+    P.aL, P.aT, P.dmeff = np.array([0.5]), np.array([0.05]), np.array([1e-4])
+    P.ts_params = (0.5, 0.0)
+    E = _walked(P, None, 5000, 4.0, 2)
+    origin, binsize, nbins = (0.0, 0.0, 0.0), (0.5, 2.0, 1.5), (48, 10, 4)
+    sf = 0.37
+    try:
+        g = E.particles()
+        dens, hist = E.kde(origin, binsize, nbins, mode=1, scaling_factor=sf,
+                           unit_volume=unit_volume)
+    finally:
+        E.close()
+    act = g["status"] == 1
+    gx, gy, gz = O.global_xyz(P, g["cell"], g["lx"], g["ly"], g["lz"])
+    od, oc, _ = O.kde_density(gx[act], gy[act], gz[act], origin, binsize, nbins)
+    assert np.array_equal(hist, oc)
+    want = od * sf * (np.prod(binsize) if unit_volume else 1.0)
+    np.testing.assert_allclose(dens, want, rtol=1e-10, atol=1e-13 * want.max())
