@@ -1,20 +1,27 @@
 """Benchmark of the random-walk hot path on one MI355X.
 
-Workload (one "step" = one tracking interval = one rwpt_advance call):
-the heterogeneous 200x200x20 lognormal field (1 m cells, divergence-free
-face flows, porosity U[0.20,0.30], aL=0.5 m, aT=0.05 m, Dm=1e-4 m2/d),
-1e6 particles released uniformly in the 10x50x20-cell inlet block, FIXED
-dt = 0.5 d, timeseries every 50 steps -- so one bench step advances every
-particle by 50 RW time steps (25 d) and --steps 10 is the whole 500-step
-run.  Particle state is resident in HBM before the timed region.
+Workload (one "step" = one tracking interval = one rwpt_advance call plus
+one gpkde reconstruction): the heterogeneous 200x200x20 lognormal field (1 m
+cells, divergence-free face flows, porosity U[0.20,0.30], aL=0.5 m,
+aT=0.05 m, Dm=1e-4 m2/d), 1e6 particles released uniformly in the
+10x50x20-cell inlet block, FIXED dt = 0.5 d, timeseries every 50 steps -- so
+one bench step advances every particle by 50 RW time steps (25 d) and
+--steps 10 is the whole 500-step run.  Particle state is resident in HBM
+before the timed region.  Each interval ends with the gpkde density of the
+active particles on the 200x200x20 flow grid (reported as KDE cells/s; the
+wall time includes the host copies of the C ABI).
 
-Each interval ends with the gpkde density reconstruction of the active
-particles on the 200x200x20 flow grid (timed apart, reported as KDE cells/s;
-the wall time includes the host copies of the C ABI).
+The headline walk is the FP32-local form (--precision 1: FP32 cell-local
+positions and sub-step arithmetic, FP64 times, the FP64 walk's random
+draws); the plain FP64 walk runs the same workload for the speedup, and an
+untimed pass reports the FP32 errors against FP64 (plume mean and variance,
+same-cell fraction, KDE density L1).  --precision 0 makes plain FP64 the
+headline.
 
 Prints one JSON line: particle-steps/s (accepted RW sub-steps, the unit the
-reference's TrackSubCell loop iterates, over the walk's wall time), the roofline of the advance kernel
-(algorithmic bytes = 96 B of SoA state per particle-step, HBM bound, 8 TB/s)
+reference's TrackSubCell loop iterates, over the walk's wall time), the
+roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
+and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
 and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
 sample of the same workload).
 """
@@ -32,10 +39,17 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-BYTES_PER_STEP_F64 = 96   # lx,ly,lz f64 + cell,status i32 + 2 x u64, read+write
+# algorithmic bytes of one particle-step: the state a step reads and writes
+# (local x y z, cell + status, 2 x u64 rng), read + write
+BYTES_PER_STEP = {0: 2 * (24 + 8 + 16), 1: 2 * (12 + 8 + 16), 2: 2 * (12 + 8 + 16)}
 HBM_PEAK_GBS = 8000.0
 INTERVAL_STEPS = 50
 DT = 0.5
+PRECISION_NAME = {0: "plain FP64", 1: "FP32 cell-local, FP64 time, the FP64 walk's draws",
+                  2: "FP32 cell-local, FP64 time, 4-draw normals"}
+TRAFFIC_FILE = {0: "walk_kernel_config1_traffic.txt",
+                1: "walk_kernel_fp32_config1_traffic.txt",
+                2: "walk_kernel_fp32n4_config1_traffic.txt"}
 
 
 def workload(n_particles):
@@ -64,6 +78,98 @@ def cpu_baseline(P, rel, n_sample, seconds_budget=20.0):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def timed_run(P, rel, precision, steps, warmup):
+    """warmup intervals, then `steps` timed intervals from the initial state,
+    each followed by the gpkde reconstruction on the flow grid."""
+    from rwpt import Engine
+    E = Engine(P, precision=precision)
+    cell, lx, ly, lz = rel
+    nbins = (P.ncol, P.nrow, P.nlay)
+    origin = (0.0, 0.0, float(P.bot.min()))
+    try:
+        E.load_particles(cell, lx, ly, lz)
+        E.set_time(0.0)
+        warm_steps = 0
+        for w in range(warmup):
+            warm_steps += E.advance((w + 1) * INTERVAL_STEPS * DT)[1]
+        warm_ms, warm_launches = E.kernel_time(0)
+        E.load_particles(cell, lx, ly, lz)
+        E.set_time(0.0)
+        E.reset_kernel_time()
+        total_steps = 0
+        walk_s = kde_s = 0.0
+        dens = []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ta = time.perf_counter()
+            _, ns = E.advance((k + 1) * INTERVAL_STEPS * DT)
+            tb = time.perf_counter()
+            d, _ = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
+            kde_s += time.perf_counter() - tb
+            walk_s += tb - ta
+            total_steps += ns
+            dens.append(d)
+        el = time.perf_counter() - t0
+        kms, launches = E.kernel_time(0)
+        hms, _ = E.kernel_time(1)
+        dms, _ = E.kernel_time(2)
+        bw = E.kde_info()["bandwidth"]
+        active = int((E.particles()["status"] == 1).sum())
+    finally:
+        E.close()
+    return {"steps": total_steps, "walk_s": walk_s, "kde_s": kde_s, "elapsed": el,
+            "kernel_ms": kms, "launches": launches, "hist_ms": hms, "dens_ms": dms,
+            "bandwidth": bw, "dens": dens, "active": active,
+            "rate": total_steps / walk_s,
+            "all": [warm_ms + kms, warm_launches + launches, warm_steps + total_steps]}
+
+
+def fp32_errors(P, rel, precision, runs, intervals=2):
+    """Errors of the FP32-local walk against the FP64 walk (untimed): plume
+    mean and variance of the active particles after each of the first
+    `intervals` intervals, the fraction of particles in the same cell with
+    the same status, and the relative L1 difference of the KDE densities of
+    the timed runs over the intervals with at least half the plume active."""
+    from oracle import oracle as O
+    from rwpt import Engine
+    snaps = {}
+    for prec in (0, precision):
+        E = Engine(P, precision=prec)
+        try:
+            E.load_particles(*rel)
+            snaps[prec] = []
+            steps = 0
+            for k in range(intervals):
+                steps += E.advance((k + 1) * INTERVAL_STEPS * DT)[1]
+                snaps[prec].append(E.particles())
+            kms, launches = E.kernel_time(0)
+            runs[prec]["all"] = [runs[prec]["all"][0] + kms, runs[prec]["all"][1] + launches,
+                                 runs[prec]["all"][2] + steps]
+        finally:
+            E.close()
+    em = ev = 0.0
+    same = []
+    for k in range(intervals):
+        mom = []
+        for prec in (0, precision):
+            g = snaps[prec][k]
+            on = g["status"] == 1
+            x = np.stack(O.global_xyz(P, g["cell"][on], g["lx"][on], g["ly"][on], g["lz"][on]))
+            mom.append((x.mean(axis=1), x.var(axis=1)))
+        em = max(em, float(np.max(np.abs(mom[1][0] - mom[0][0]) / np.abs(mom[0][0]))))
+        ev = max(ev, float(np.max(np.abs(mom[1][1] - mom[0][1]) / mom[0][1])))
+        g0, g1 = snaps[0][k], snaps[precision][k]
+        same.append(float(((g0["cell"] == g1["cell"]) & (g0["status"] == g1["status"])).mean()))
+    n = rel[0].size
+    l1 = [float(np.abs(d1 - d0).sum() / d0.sum())
+          for d0, d1 in zip(runs[0]["dens"], runs[precision]["dens"])
+          if d0.sum() >= 0.5 * n]
+    return {"moment_intervals": intervals, "mean_rel_err_max": em,
+            "variance_rel_err_max": ev, "same_cell_status_fraction": same,
+            "kde_rel_l1_max": max(l1) if l1 else None,
+            "kde_intervals_compared": len(l1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +178,11 @@ def main():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", type=int, default=1, choices=(0, 1, 2),
+                    help="walk arithmetic: 0 plain FP64, 1 FP32-local with the FP64 "
+                         "walk's draws (default), 2 FP32-local with 4-draw normals")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the plain FP64 comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -82,53 +193,31 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from rwpt import Engine
     P, rel = workload(args.particles)
-    E = Engine(P)
-    cell, lx, ly, lz = rel
-
-    # warm-up intervals from the initial state, then restart the same run
-    E.load_particles(cell, lx, ly, lz)
-    E.set_time(0.0)
-    for w in range(args.warmup):
-        E.advance((w + 1) * INTERVAL_STEPS * DT)
-    E.load_particles(cell, lx, ly, lz)
-    E.set_time(0.0)
-    E.reset_kernel_time()
-
-    # each interval ends with the gpkde reconstruction of the active plume on
-    # the flow grid (1 m bins, Silverman bandwidth, reflected kernel)
-    nbins = (P.ncol, P.nrow, P.nlay)
-    kde_cells = int(np.prod(nbins))
-    total_steps = 0
-    walk_s = kde_s = 0.0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ta = time.perf_counter()
-        _, ns = E.advance((k + 1) * INTERVAL_STEPS * DT)
-        tb = time.perf_counter()
-        E.kde((0.0, 0.0, float(P.bot.min())), (1.0, 1.0, 1.0), nbins, mode=1)
-        kde_s += time.perf_counter() - tb
-        walk_s += tb - ta
-        total_steps += ns
-    el = time.perf_counter() - t0
-    kms, launches = E.kernel_time(0)
-    hms, _ = E.kernel_time(1)
-    dms, _ = E.kernel_time(2)
-    bw = E.kde_info()["bandwidth"]
-    state = E.particles()
-    E.close()
-
-    value = total_steps / walk_s
-    avg_launch_s = (kms / 1e3) / max(launches, 1)
-    bytes_per_launch = BYTES_PER_STEP_F64 * total_steps / max(launches, 1)
+    prec = args.precision
+    runs = {prec: timed_run(P, rel, prec, args.steps, args.warmup)}
+    if prec != 0 and not args.no_fp64:
+        runs[0] = timed_run(P, rel, 0, args.steps, args.warmup)
+    r = runs[prec]
+    value = r["rate"]
+    errors = None
+    if 0 in runs and prec != 0:
+        errors = fp32_errors(P, rel, prec, runs)
+    # the roofline averages over every launch of this walk in the process
+    # (warm-up, timed and error passes), as a profiler's per-kernel
+    # statistics of the same command do
+    all_ms, all_launches, all_steps = r["all"]
+    avg_launch_s = (all_ms / 1e3) / max(all_launches, 1)
+    bytes_per_launch = BYTES_PER_STEP[prec] * all_steps / max(all_launches, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "walk_kernel_config1_traffic.txt")
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE[prec])
     if os.path.exists(tpath):
         for line in open(tpath):
             if line.startswith("hbm_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
+    nbins = (P.ncol, P.nrow, P.nlay)
+    kde_cells = int(np.prod(nbins))
     out = {
         "metric": "particle-steps/s + KDE cells/s, 1 GPU: FP32/batched vs plain "
                   "FP64 at <=0.5% err",
@@ -137,35 +226,50 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / args.steps,
+        "ms_per_step": r["elapsed"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == 0 else "f32",
         "data": "synthetic (seeded lognormal field, curl-of-potential fluxes)",
         "config": {
             "workload": "heterogeneous 200x200x20 lognormal field, 1e6 particles, "
-                        "FIXED dt 0.5 d, 50 RW steps per tracking interval",
+                        "FIXED dt 0.5 d, 50 RW steps per tracking interval, gpkde "
+                        "reconstruction per interval",
             "grid": [200, 200, 20], "particles": args.particles,
             "rw_steps_per_interval": INTERVAL_STEPS, "dt_days": DT,
-            "precision": "plain FP64", "advection": "EULERIAN",
+            "precision": PRECISION_NAME[prec], "advection": "EULERIAN",
         },
-        "particle_steps": total_steps,
-        "active_after": int((state["status"] == 1).sum()),
+        "particle_steps": r["steps"],
+        "active_after": r["active"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "kernel": "walk_kernel",
-                     "launches": launches,
+                     "launches": all_launches,
                      "avg_launch_ms": avg_launch_s * 1e3,
+                     "timed_launches": r["launches"],
+                     "timed_avg_launch_ms": r["kernel_ms"] / max(r["launches"], 1),
+                     "algorithmic_bytes_per_step": BYTES_PER_STEP[prec],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "kde": {"unit": "KDE cells/s", "grid": list(nbins),
                 "reconstructions": args.steps,
-                "cells_per_s_wall": kde_cells * args.steps / kde_s,
-                "cells_per_s_kernels": kde_cells * args.steps / ((hms + dms) / 1e3),
-                "histogram_ms": hms / args.steps, "density_ms": dms / args.steps,
-                "wall_ms": kde_s * 1e3 / args.steps, "last_bandwidth": bw},
+                "cells_per_s_wall": kde_cells * args.steps / r["kde_s"],
+                "cells_per_s_kernels": kde_cells * args.steps /
+                ((r["hist_ms"] + r["dens_ms"]) / 1e3),
+                "histogram_ms": r["hist_ms"] / args.steps,
+                "density_ms": r["dens_ms"] / args.steps,
+                "wall_ms": r["kde_s"] * 1e3 / args.steps,
+                "last_bandwidth": r["bandwidth"]},
     }
+    if 0 in runs and prec != 0:
+        b0 = runs[0]
+        out["fp64"] = {"value": b0["rate"], "ms_per_step": b0["elapsed"] * 1e3 / args.steps,
+                       "particle_steps": b0["steps"],
+                       "walk_kernel_avg_launch_ms": b0["kernel_ms"] / max(b0["launches"], 1),
+                       "launches": b0["launches"]}
+        out["speedup_vs_fp64"] = value / b0["rate"]
+        out["error_vs_fp64"] = errors
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)
     print(json.dumps(out), flush=True)

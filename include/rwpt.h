@@ -66,7 +66,12 @@ typedef struct {
   int32_t stop_at_weak_sinks;
   int32_t stop_at_weak_sources;
   int32_t steady_state;       /* FlowModelData%SteadyState               */
-  int32_t precision;          /* 0: plain FP64; 1: FP32 cell-local       */
+  int32_t precision;          /* 0: plain FP64 (the reference's numbers);
+                                 1: FP32 cell-local positions and sub-step
+                                    arithmetic, FP64 times and Newton exit
+                                    solves, the FP64 run's random draws;
+                                 2: as 1 with normals from 4 draws (16-bit
+                                    Irwin-Hall) instead of 12            */
   uint64_t base_seed[2];      /* baseseed, TrackSubCell.F90:553-555      */
 } rwpt_options;
 

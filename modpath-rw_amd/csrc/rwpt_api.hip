@@ -65,6 +65,12 @@ void free_particles(Particles &P) {
   dev_free(P.iy);
   dev_free(P.iz);
   dev_free(P.t0);
+  dev_free(P.flx);
+  dev_free(P.fly);
+  dev_free(P.flz);
+  dev_free(P.fix);
+  dev_free(P.fiy);
+  dev_free(P.fiz);
   for (int k = 0; k < 2; ++k) {
     dev_free(P.keys[k]);
     dev_free(P.vals[k]);
@@ -78,10 +84,20 @@ void free_particles(Particles &P) {
 int ensure_tables(rwpt_engine *e) {
   if (!e->have_flow) return fail(e, RWPT_E_STATE, "flow not set");
   if (!e->have_disp) return fail(e, RWPT_E_STATE, "dispersion not set");
-  if (!e->tables_dirty) return RWPT_OK;
-  hipError_t err = build_cell_tables(e, nullptr);
-  if (err != hipSuccess) return hip_fail(e, err, "cell tables");
-  e->tables_dirty = false;
+  hipError_t err = hipSuccess;
+  if (e->tables_dirty) {
+    err = build_cell_tables(e, nullptr);
+    if (err != hipSuccess) return hip_fail(e, err, "cell tables");
+    e->tables_dirty = false;
+    e->tables32_dirty = true;
+  }
+  if (e->opt.precision != 0 && e->tables32_dirty) {
+    if (!e->prob.cells32)
+      err = dev_alloc(&e->prob.cells32, e->prob.ncell * CELL_FLOATS);
+    if (err == hipSuccess) err = build_cell_tables32(e);
+    if (err != hipSuccess) return hip_fail(e, err, "FP32 cell tables");
+    e->tables32_dirty = false;
+  }
   return RWPT_OK;
 }
 
@@ -122,7 +138,7 @@ int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
     nd += opt->dimension_mask[d];
   }
   if (nd == 0) return RWPT_E_ARG;
-  if (opt->precision != 0 && opt->precision != 1) return RWPT_E_ARG;
+  if (opt->precision < 0 || opt->precision > 2) return RWPT_E_ARG;
   for (int j = 0; j < grid->ncol; ++j)
     if (!(grid->delr[j] > 0.0)) return RWPT_E_ARG;
   for (int i = 0; i < grid->nrow; ++i)
@@ -327,6 +343,14 @@ int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
     if (err == hipSuccess) err = dev_alloc(&P.iy, n);
     if (err == hipSuccess) err = dev_alloc(&P.iz, n);
     if (err == hipSuccess) err = dev_alloc(&P.t0, n);
+    if (e->opt.precision != 0) {
+      if (err == hipSuccess) err = dev_alloc(&P.flx, n);
+      if (err == hipSuccess) err = dev_alloc(&P.fly, n);
+      if (err == hipSuccess) err = dev_alloc(&P.flz, n);
+      if (err == hipSuccess) err = dev_alloc(&P.fix, n);
+      if (err == hipSuccess) err = dev_alloc(&P.fiy, n);
+      if (err == hipSuccess) err = dev_alloc(&P.fiz, n);
+    }
     for (int k = 0; k < 2 && err == hipSuccess; ++k) {
       err = dev_alloc(&P.keys[k], n);
       if (err == hipSuccess) err = dev_alloc(&P.vals[k], n);
@@ -352,6 +376,7 @@ int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
     if (rng) err = hipMemcpyAsync(P.rng, rng, n * 16, hipMemcpyHostToDevice, s);
     else err = launch_seed_streams(e, n, stream_offset);
   }
+  if (err == hipSuccess && e->opt.precision != 0) err = launch_positions_to32(e);
   if (err == hipSuccess) err = hipStreamSynchronize(s);
   if (err != hipSuccess) return hip_fail(e, err, "load_particles");
   return RWPT_OK;
@@ -394,10 +419,12 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
   if (!std::isfinite(t_stop)) return fail(e, RWPT_E_ARG, "t_stop not finite");
   int rc = ensure_tables(e);
   if (rc != RWPT_OK) return rc;
-  if (e->opt.precision != 0)
-    return fail(e, RWPT_E_UNSUPPORTED, "FP32 cell-local walk not built yet");
   hipError_t err = launch_advance(e, t_stop);
   if (err != hipSuccess) return hip_fail(e, err, "advance launch");
+  if (e->opt.precision != 0) {
+    err = launch_positions_from32(e);
+    if (err != hipSuccess) return hip_fail(e, err, "advance positions");
+  }
   unsigned long long h[4] = {0, 0, 0, 0};
   err = hipMemcpyAsync(h, e->counters, sizeof h, hipMemcpyDeviceToHost, e->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -468,6 +495,7 @@ int rwpt_get_cell_tables(rwpt_engine *e, double *out) {
   dev_free(d);
   if (err != hipSuccess) return hip_fail(e, err, "cell tables");
   e->tables_dirty = false;
+  e->tables32_dirty = true;
   return RWPT_OK;
 }
 

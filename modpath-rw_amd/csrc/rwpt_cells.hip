@@ -390,7 +390,86 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
   }
 }
 
+// FP32-local record from the FP64 one: float fields rounded once, integer
+// fields copied, full reciprocals of DX DY DZ R phi (every divisor of the
+// sub-step becomes a product).
+__global__ void float_records(int64_t ncell, const double *__restrict__ cells,
+                              float *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const double *rec = cells + c * CELL_DOUBLES;
+  float *o = out + c * CELL_FLOATS;
+  for (int f = 0; f < F_CONN; ++f) o[f] = (float)rec[f];
+  const int32_t *ri = reinterpret_cast<const int32_t *>(rec);
+  int32_t *oi = reinterpret_cast<int32_t *>(o);
+  for (int k = 0; k < 6; ++k) {
+    oi[G_CONN + k] = ri[2 * F_CONN + k];
+    oi[G_MASSBND + k] = ri[2 * F_MASSBND + k];
+  }
+  oi[G_FLAGS] = ri[2 * F_FLAGS];
+  oi[G_FLAGS + 1] = ri[2 * F_FLAGS + 1];
+  oi[G_ZONE] = ri[2 * F_ZONE];
+  o[G_SOURCE] = (float)rec[F_SOURCE];
+  o[G_SINK] = (float)rec[F_SINK];
+  o[G_RDX] = (float)(1.0 / rec[F_DX]);
+  o[G_RDY] = (float)(1.0 / rec[F_DY]);
+  o[G_RDZ] = (float)(1.0 / rec[F_DZ]);
+  o[G_RR] = (float)(1.0 / rec[F_R]);
+  o[G_RPHI] = (float)(1.0 / rec[F_PHI]);
+  for (int f = G_RPHI + 1; f < CELL_FLOATS; ++f) o[f] = 0.0f;
+}
+
+__global__ void to_float3(int64_t n, const double *__restrict__ a,
+                          const double *__restrict__ b,
+                          const double *__restrict__ c, float *__restrict__ fa,
+                          float *__restrict__ fb, float *__restrict__ fc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fa[i] = (float)a[i];
+  fb[i] = (float)b[i];
+  fc[i] = (float)c[i];
+}
+
+__global__ void to_double3(int64_t n, const float *__restrict__ fa,
+                           const float *__restrict__ fb,
+                           const float *__restrict__ fc, double *__restrict__ a,
+                           double *__restrict__ b, double *__restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a[i] = fa[i];
+  b[i] = fb[i];
+  c[i] = fc[i];
+}
+
 }  // namespace
+
+hipError_t build_cell_tables32(rwpt_engine *e) {
+  DeviceProblem &P = e->prob;
+  const int block = 256;
+  hipLaunchKernelGGL(float_records, dim3((unsigned)((P.ncell + block - 1) / block)),
+                     dim3(block), 0, e->stream, P.ncell, P.cells, P.cells32);
+  return hipGetLastError();
+}
+
+hipError_t launch_positions_to32(rwpt_engine *e) {
+  Particles &S = e->part;
+  if (S.n == 0) return hipSuccess;
+  const int block = 256;
+  hipLaunchKernelGGL(to_float3, dim3((unsigned)((S.n + block - 1) / block)),
+                     dim3(block), 0, e->stream, S.n, S.lx, S.ly, S.lz, S.flx,
+                     S.fly, S.flz);
+  return hipGetLastError();
+}
+
+hipError_t launch_positions_from32(rwpt_engine *e) {
+  Particles &S = e->part;
+  if (S.n == 0) return hipSuccess;
+  const int block = 256;
+  hipLaunchKernelGGL(to_double3, dim3((unsigned)((S.n + block - 1) / block)),
+                     dim3(block), 0, e->stream, S.n, S.flx, S.fly, S.flz, S.lx,
+                     S.ly, S.lz);
+  return hipGetLastError();
+}
 
 hipError_t build_cell_tables(rwpt_engine *e, double *debug_out) {
   DeviceProblem &P = e->prob;

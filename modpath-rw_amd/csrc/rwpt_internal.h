@@ -46,6 +46,37 @@ enum : int32_t {
   CF_VERIFY_PDRY = 8,     // VerifyDryCell partiallyDry (release of status 7)
 };
 
+// FP32-local cell record (precision 1 and 2): the float fields 0..79 at the
+// same indices as in the FP64 record, then the integer fields one per slot,
+// then full reciprocals of the divisors the sub-step uses.
+enum CellField32 : int {
+  G_CONN = 80,      // 80..85: 6 x int32 connections
+  G_MASSBND = 86,   // 86..91: 6 x int32 mass-boundary flags
+  G_FLAGS = 92,     // int32 ibound, int32 flags
+  G_ZONE = 94,
+  G_SOURCE = 95, G_SINK = 96,
+  G_RDX = 97, G_RDY, G_RDZ, G_RR, G_RPHI,  // 1/DX 1/DY 1/DZ 1/R 1/phi
+  CELL_FLOATS = 104
+};
+
+// Record layout by arithmetic type: indices of the integer fields in int32
+// units from the record start, of the other fields in R units.
+template <typename R> struct Lay;
+template <> struct Lay<double> {
+  static constexpr int STRIDE = CELL_DOUBLES;
+  static constexpr int CONN = 2 * F_CONN, MASSBND = 2 * F_MASSBND,
+                       FLAGS = 2 * F_FLAGS, ZONE = 2 * F_ZONE;
+  static constexpr int SOURCE = F_SOURCE, SINK = F_SINK, RDX = F_RDX,
+                       RDY = F_RDY, RDZ = F_RDZ, RR = F_RR;
+};
+template <> struct Lay<float> {
+  static constexpr int STRIDE = CELL_FLOATS;
+  static constexpr int CONN = G_CONN, MASSBND = G_MASSBND, FLAGS = G_FLAGS,
+                       ZONE = G_ZONE;
+  static constexpr int SOURCE = G_SOURCE, SINK = G_SINK, RDX = G_RDX,
+                       RDY = G_RDY, RDZ = G_RDZ, RR = G_RR;
+};
+
 // Layout of the checking copy of the cell tables (104 doubles per cell).
 constexpr int DEBUG_FIELDS = 104;
 
@@ -76,7 +107,7 @@ struct DeviceProblem {
   double *dzrw = nullptr;   // ncell
   // tables
   double *cells = nullptr;  // ncell * CELL_DOUBLES
-  float *cells32 = nullptr; // FP32-local tables (ncell * CELL_DOUBLES)
+  float *cells32 = nullptr; // FP32-local tables (ncell * CELL_FLOATS)
 };
 
 struct Particles {
@@ -88,6 +119,10 @@ struct Particles {
   // resumable walk state (see rwpt_walk.hip)
   int32_t *aux = nullptr;
   double *ix = nullptr, *iy = nullptr, *iz = nullptr, *t0 = nullptr;
+  // FP32-local positions (precision 1, 2): the walk's copies of lx ly lz and
+  // of the visit start; lx ly lz above are refreshed after every advance
+  float *flx = nullptr, *fly = nullptr, *flz = nullptr;
+  float *fix = nullptr, *fiy = nullptr, *fiz = nullptr;
   // work lists sorted by cell between launches
   int32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
   void *sort_tmp = nullptr;
@@ -117,6 +152,7 @@ struct rwpt_engine {
   rwpt::DeviceProblem prob;
   rwpt::Particles part;
   bool have_flow = false, have_disp = false, tables_dirty = true;
+  bool tables32_dirty = true;  // FP32-local records behind the FP64 ones
   double t_engine = 0.0;
   uint64_t *jump_mats = nullptr;  // device copies of J^(2^k), k < 64
   unsigned long long *counters = nullptr;  // device counters
@@ -146,6 +182,11 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                       double *hist_dev, double *stats_host);
 
 hipError_t ensure_sort_storage(rwpt_engine *e);
+// FP32-local state: float record table from the FP64 one, positions to and
+// from the double arrays
+hipError_t build_cell_tables32(rwpt_engine *e);
+hipError_t launch_positions_to32(rwpt_engine *e);
+hipError_t launch_positions_from32(rwpt_engine *e);
 
 // host helper: GF(2) matrices of the xoroshiro128+ jump, J^(2^k)
 void jump_power_matrices(int kmax, std::vector<uint64_t> &out);

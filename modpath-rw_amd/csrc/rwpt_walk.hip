@@ -34,31 +34,103 @@ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) {
   return (x << k) | (x >> (64 - k));
 }
 
-__device__ __forceinline__ double draw_uniform(Stream &s) {
+__device__ __forceinline__ uint64_t draw_bits(Stream &s) {
   const uint64_t s1 = s.a, s2 = s.b ^ s.a;
   const uint64_t out = s.a + s.b;
   s.a = (rotl64(s1, 24) ^ s2) ^ (s2 << 16);
   s.b = rotl64(s2, 37);
+  return out;
+}
+
+__device__ __forceinline__ double draw_uniform(Stream &s) {
+  const uint64_t out = draw_bits(s);
   return __longlong_as_double(
              (long long)(0x3FF0000000000000ull | (out >> 12))) - 1.0;
 }
 
-// Irwin-Hall normal of pr_GenerateStandardNormalRandomZig: 12 uniforms
-// summed in order, minus 6.
-__device__ __forceinline__ double draw_normal(Stream &s) {
-  double acc = 0.0;
+// Normal deviates by arithmetic type and draw policy.
+//  NRM 0 (FP64): pr_GenerateStandardNormalRandomZig, 12 uniforms summed in
+//        order, minus 6 -- the reference's numbers bit for bit.
+//  NRM 1 (FP32): the same 12 draws; the top 24 bits of each summed exactly
+//        as integers, one rounding to float: within 5e-7 of the FP64 normal,
+//        so an FP32 walk follows the FP64 walk's random path.
+//  NRM 2 (FP32): Irwin-Hall of 12 uniforms of 16 bits, three from bits
+//        16..63 of each of 4 draws; (sum + 6)/65536 - 6 is exact up to the
+//        final rounding and has mean 0 and variance 1 - 2^-32.
+template <typename R, int NRM> struct Normal;
+template <> struct Normal<double, 0> {
+  static __device__ __forceinline__ double draw(Stream &s) {
+    double acc = 0.0;
 #pragma unroll
-  for (int m = 0; m < 12; ++m) acc = acc + draw_uniform(s);
-  return acc - 6.0;
-}
+    for (int m = 0; m < 12; ++m) acc = acc + draw_uniform(s);
+    return acc - 6.0;
+  }
+};
+template <> struct Normal<float, 1> {
+  static __device__ __forceinline__ float draw(Stream &s) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int m = 0; m < 12; ++m) acc += (uint32_t)(draw_bits(s) >> 40);
+    return (float)acc * 0x1p-24f - 6.0f;
+  }
+};
+template <> struct Normal<float, 2> {
+  static __device__ __forceinline__ float draw(Stream &s) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint64_t w = draw_bits(s);
+      const uint32_t hi = (uint32_t)(w >> 32), mid = (uint32_t)w >> 16;
+      acc += (hi & 0xFFFFu) + (hi >> 16) + mid;
+    }
+    return ((float)acc + 6.0f) * 0x1p-16f - 6.0f;
+  }
+};
 
 __device__ __forceinline__ double f32lit(float v) { return (double)v; }
 
-// a / d where the cell record stores r = 1/d when d is a power of two (else
-// 0): the product then rounds the same exact real as the quotient, so the
-// result is bit-identical and the IEEE divide sequence is skipped.
-__device__ __forceinline__ double div_p2(double a, double d, double r) {
-  return r != 0.0 ? a * r : a / d;
+// Arithmetic by type.  FP64: IEEE division and square root in the
+// reference's operand order; a / d uses the record's reciprocal of d only
+// when d is a power of two (then the product rounds the same exact real as
+// the quotient, so the result is bit-identical).  FP32: reciprocal and square
+// root instructions (1 ulp), every division by a cell size, R or phi a
+// product with the record's reciprocal.
+template <typename R> struct M;
+template <> struct M<double> {
+  static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double divr(double a, double d, double r) {
+    return r != 0.0 ? a * r : a / d;
+  }
+  static __device__ __forceinline__ double div_phi(double a, const double *rec) {
+    return a / rec[F_PHI];
+  }
+  static __device__ __forceinline__ double expm1_over(double v, double g,
+                                                      double dt) {
+    return v * (exp(g * dt) - 1.0) / g;
+  }
+};
+template <> struct M<float> {
+  static __device__ __forceinline__ float sq(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+  }
+  static __device__ __forceinline__ float div(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
+  static __device__ __forceinline__ float divr(float a, float, float r) {
+    return a * r;
+  }
+  static __device__ __forceinline__ float div_phi(float a, const float *rec) {
+    return a * rec[G_RPHI];
+  }
+  static __device__ __forceinline__ float expm1_over(float v, float g, float dt) {
+    return v * expm1f(g * dt) * __builtin_amdgcn_rcpf(g);
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ const int32_t *ints(const R *rec) {
+  return reinterpret_cast<const int32_t *>(rec);
 }
 
 // Guards against walks the reference never ends (not in the reference):
@@ -82,41 +154,41 @@ __device__ __forceinline__ double pow10i(int n) {
 // ---------------------------------------------------------------------------
 
 // trilinear interpolation of 8 corner values (pr_Trilinear)
-__device__ __forceinline__ double tri8(const double *__restrict__ v, double x,
-                                       double y, double z) {
-  const double v00 = (1.0 - x) * v[0] + x * v[1];
-  const double v01 = (1.0 - x) * v[4] + x * v[5];
-  const double v10 = (1.0 - x) * v[2] + x * v[3];
-  const double v11 = (1.0 - x) * v[6] + x * v[7];
-  const double v0 = (1.0 - y) * v00 + y * v10;
-  const double v1 = (1.0 - y) * v01 + y * v11;
-  return (1.0 - z) * v0 + z * v1;
+template <typename R>
+__device__ __forceinline__ R tri8(const R *__restrict__ v, R x, R y, R z) {
+  const R v00 = (R(1) - x) * v[0] + x * v[1];
+  const R v01 = (R(1) - x) * v[4] + x * v[5];
+  const R v10 = (R(1) - x) * v[2] + x * v[3];
+  const R v11 = (R(1) - x) * v[6] + x * v[7];
+  const R v0 = (R(1) - y) * v00 + y * v10;
+  const R v1 = (R(1) - y) * v01 + y * v11;
+  return (R(1) - z) * v0 + z * v1;
 }
 
 // derivative term stored as the four edge differences (pr_TrilinearDerivative)
-__device__ __forceinline__ double deriv4(const double *__restrict__ d, double u,
-                                         double w) {
-  const double v0 = (1.0 - u) * d[0] + u * d[1];
-  const double v1 = (1.0 - u) * d[2] + u * d[3];
-  return (1.0 - w) * v0 + w * v1;
+template <typename R>
+__device__ __forceinline__ R deriv4(const R *__restrict__ d, R u, R w) {
+  const R v0 = (R(1) - u) * d[0] + u * d[1];
+  const R v1 = (R(1) - u) * d[2] + u * d[3];
+  return (R(1) - w) * v0 + w * v1;
 }
 
-struct StepTerms {
-  double vx, vy, vz;        // interpolated face velocities
-  double dAx, dAy, dAz;     // advection displacement
-  double dBx, dBy, dBz;     // random displacement (per sqrt(dt))
-  double ddx, ddy, ddz;     // dispersion divergence (drift correction)
+template <typename R> struct StepTerms {
+  R vx, vy, vz;        // interpolated face velocities
+  R dAx, dAy, dAz;     // advection displacement
+  R dBx, dBy, dBz;     // random displacement (per sqrt(dt))
+  R ddx, ddy, ddz;     // dispersion divergence (drift correction)
 };
 
-template <int ADV>
-__device__ __forceinline__ void advect(const double *__restrict__ rec,
-                                       double dt, StepTerms &s) {
+template <typename R, int ADV>
+__device__ __forceinline__ void advect(const R *__restrict__ rec, R dt,
+                                       StepTerms<R> &s) {
   if constexpr (ADV == 1) {  // pr_AdvectionDisplacementExponential
-    const double dvtol = 1.0e-10;
-    const double gx = rec[F_DVDX], gy = rec[F_DVDY], gz = rec[F_DVDZ];
-    s.dAx = fabs(gx) > dvtol ? s.vx * (exp(gx * dt) - 1.0) / gx : s.vx * dt;
-    s.dAy = fabs(gy) > dvtol ? s.vy * (exp(gy * dt) - 1.0) / gy : s.vy * dt;
-    s.dAz = fabs(gz) > dvtol ? s.vz * (exp(gz * dt) - 1.0) / gz : s.vz * dt;
+    const R dvtol = R(1.0e-10);
+    const R gx = rec[F_DVDX], gy = rec[F_DVDY], gz = rec[F_DVDZ];
+    s.dAx = fabs(gx) > dvtol ? M<R>::expm1_over(s.vx, gx, dt) : s.vx * dt;
+    s.dAy = fabs(gy) > dvtol ? M<R>::expm1_over(s.vy, gy, dt) : s.vy * dt;
+    s.dAz = fabs(gz) > dvtol ? M<R>::expm1_over(s.vz, gz, dt) : s.vz * dt;
   } else {          // pr_AdvectionDisplacementEulerian
     s.dAx = s.vx * dt;
     s.dAy = s.vy * dt;
@@ -125,19 +197,20 @@ __device__ __forceinline__ void advect(const double *__restrict__ rec,
 }
 
 // pr_RWPTDisplacementsLinear / ...Axisymmetric for one sub-step.
-template <int NDIM, int MODEL, int ADV>
-__device__ __forceinline__ void displacements(const double *__restrict__ rec,
-                                              const WalkParams &W, double x,
-                                              double y, double z, double dt,
-                                              Stream &rng, StepTerms &s) {
-  s.vx = (1.0 - x) * rec[F_VX1] + x * rec[F_VX2];
-  s.vy = (1.0 - y) * rec[F_VY1] + y * rec[F_VY2];
-  s.vz = (1.0 - z) * rec[F_VZ1] + z * rec[F_VZ2];
-  double aL, aT, aTH;
+template <typename R, int NRM, int NDIM, int MODEL, int ADV>
+__device__ __forceinline__ void displacements(const R *__restrict__ rec,
+                                              const WalkParams &W, R x, R y,
+                                              R z, R dt, Stream &rng,
+                                              StepTerms<R> &s) {
+  using Mx = M<R>;
+  s.vx = (R(1) - x) * rec[F_VX1] + x * rec[F_VX2];
+  s.vy = (R(1) - y) * rec[F_VY1] + y * rec[F_VY2];
+  s.vz = (R(1) - z) * rec[F_VZ1] + z * rec[F_VZ2];
+  R aL, aT, aTH;
   if constexpr (MODEL == 1) {
-    const double vn = sqrt(s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
-    double c2 = 0.0;
-    if (vn > 0.0) c2 = s.vz * s.vz / vn;
+    const R vn = Mx::sq(s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    R c2 = R(0);
+    if (vn > R(0)) c2 = Mx::div(s.vz * s.vz, vn);
     aL = rec[F_ALH] + c2 * (rec[F_ALV] - rec[F_ALH]);
     aT = rec[F_ATV] + c2 * (rec[F_ATH] - rec[F_ATV]);
     aTH = rec[F_ATH];
@@ -147,62 +220,61 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     aTH = aT;
   }
   // drift correction, pr_DispersionDivergence
-  const double *dd = rec + F_DD;
-  const double t0 = deriv4(dd + 0, y, z), t1 = deriv4(dd + 4, x, z);
-  const double t2 = deriv4(dd + 8, x, y), t3 = deriv4(dd + 12, y, z);
-  const double t4 = deriv4(dd + 16, y, z), t5 = deriv4(dd + 20, x, z);
-  const double t6 = deriv4(dd + 24, x, z), t7 = deriv4(dd + 28, x, y);
-  const double t8 = deriv4(dd + 32, x, y);
-  const double phi = rec[F_PHI], R = rec[F_R], rR = rec[F_RR];
-  s.ddx = div_p2((t0 + t5 + t7) / phi, R, rR);
-  s.ddy = div_p2((t3 + t1 + t8) / phi, R, rR);
-  s.ddz = div_p2((t4 + t6 + t2) / phi, R, rR);
+  const R *dd = rec + F_DD;
+  const R t0 = deriv4(dd + 0, y, z), t1 = deriv4(dd + 4, x, z);
+  const R t2 = deriv4(dd + 8, x, y), t3 = deriv4(dd + 12, y, z);
+  const R t4 = deriv4(dd + 16, y, z), t5 = deriv4(dd + 20, x, z);
+  const R t6 = deriv4(dd + 24, x, z), t7 = deriv4(dd + 28, x, y);
+  const R t8 = deriv4(dd + 32, x, y);
+  const R Rt = rec[F_R], rR = rec[Lay<R>::RR];
+  s.ddx = Mx::divr(Mx::div_phi(t0 + t5 + t7, rec), Rt, rR);
+  s.ddy = Mx::divr(Mx::div_phi(t3 + t1 + t8, rec), Rt, rR);
+  s.ddz = Mx::divr(Mx::div_phi(t4 + t6 + t2, rec), Rt, rR);
   // random displacement, pr_DisplacementRandomDischarge{,2D,1D}
-  const double dm = rec[F_DMEFF];
-  const double *vc = rec + F_VC;
+  const R dm = rec[F_DMEFF];
+  const R *vc = rec + F_VC;
   if constexpr (NDIM == 3) {
-    const double bx = tri8(vc, x, y, z), by = tri8(vc + 8, x, y, z);
-    const double bz = tri8(vc + 16, x, y, z);
-    const double vn = sqrt(bx * bx + by * by + bz * bz);
-    const double vxy = sqrt(bx * bx + by * by);
-    double B11 = 0, B12 = 0, B13 = 0, B21 = 0, B22 = 0, B23 = 0, B31 = 0,
-           B32 = 0;
-    if (vn > 0.0) {
-      const double sl = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
-      const double st = sqrt(div_p2(2 * (aT * vn + dm), R, rR));
-      B11 = bx * sl / vn;
-      B21 = by * sl / vn;
-      B31 = bz * sl / vn;
-      B32 = vxy * st / vn;
-      if (vxy > 0.0) {
-        const double sh = sqrt(div_p2(2 * (aTH * vn + dm), R, rR));
-        B12 = -(bx * bz * st / vn / vxy);
-        B13 = -(by * sh / vxy);
-        B22 = -(by * bz * st / vn / vxy);
-        B23 = bx * sh / vxy;
+    const R bx = tri8(vc, x, y, z), by = tri8(vc + 8, x, y, z);
+    const R bz = tri8(vc + 16, x, y, z);
+    const R vn = Mx::sq(bx * bx + by * by + bz * bz);
+    const R vxy = Mx::sq(bx * bx + by * by);
+    R B11 = 0, B12 = 0, B13 = 0, B21 = 0, B22 = 0, B23 = 0, B31 = 0, B32 = 0;
+    if (vn > R(0)) {
+      const R sl = Mx::sq(Mx::divr(2 * (aL * vn + dm), Rt, rR));
+      const R st = Mx::sq(Mx::divr(2 * (aT * vn + dm), Rt, rR));
+      B11 = Mx::div(bx * sl, vn);
+      B21 = Mx::div(by * sl, vn);
+      B31 = Mx::div(bz * sl, vn);
+      B32 = Mx::div(vxy * st, vn);
+      if (vxy > R(0)) {
+        const R sh = Mx::sq(Mx::divr(2 * (aTH * vn + dm), Rt, rR));
+        B12 = -Mx::div(Mx::div(bx * bz * st, vn), vxy);
+        B13 = -Mx::div(by * sh, vxy);
+        B22 = -Mx::div(Mx::div(by * bz * st, vn), vxy);
+        B23 = Mx::div(bx * sh, vxy);
       }
     }
-    const double r1 = draw_normal(rng), r2 = draw_normal(rng);
-    const double r3 = draw_normal(rng);
+    const R r1 = Normal<R, NRM>::draw(rng), r2 = Normal<R, NRM>::draw(rng);
+    const R r3 = Normal<R, NRM>::draw(rng);
     s.dBx = B11 * r1 + B12 * r2 + B13 * r3;
     s.dBy = B21 * r1 + B22 * r2 + B23 * r3;
     s.dBz = B31 * r1 + B32 * r2;
   } else if constexpr (NDIM == 2) {
     const int d1 = W.dims[0] - 1, d2 = W.dims[1] - 1;
-    const double b1 = tri8(vc + 8 * d1, x, y, z);
-    const double b2 = tri8(vc + 8 * d2, x, y, z);
-    const double vn = sqrt(b1 * b1 + b2 * b2);
-    double B11 = 0, B12 = 0, B21 = 0, B22 = 0;
-    if (vn > 0.0) {
-      const double sl = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
-      const double st = sqrt(div_p2(2 * (aTH * vn + dm), R, rR));
-      B11 = b1 * sl / vn;
-      B21 = b2 * sl / vn;
-      B12 = -(b2 * st / vn);
-      B22 = b1 * st / vn;
+    const R b1 = tri8(vc + 8 * d1, x, y, z);
+    const R b2 = tri8(vc + 8 * d2, x, y, z);
+    const R vn = Mx::sq(b1 * b1 + b2 * b2);
+    R B11 = 0, B12 = 0, B21 = 0, B22 = 0;
+    if (vn > R(0)) {
+      const R sl = Mx::sq(Mx::divr(2 * (aL * vn + dm), Rt, rR));
+      const R st = Mx::sq(Mx::divr(2 * (aTH * vn + dm), Rt, rR));
+      B11 = Mx::div(b1 * sl, vn);
+      B21 = Mx::div(b2 * sl, vn);
+      B12 = -Mx::div(b2 * st, vn);
+      B22 = Mx::div(b1 * st, vn);
     }
-    const double r1 = draw_normal(rng), r2 = draw_normal(rng);
-    double dB[3] = {0.0, 0.0, 0.0};
+    const R r1 = Normal<R, NRM>::draw(rng), r2 = Normal<R, NRM>::draw(rng);
+    R dB[3] = {R(0), R(0), R(0)};
     dB[d1] = B11 * r1 + B12 * r2;
     dB[d2] = B21 * r1 + B22 * r2;
     s.dBx = dB[0];
@@ -210,18 +282,18 @@ __device__ __forceinline__ void displacements(const double *__restrict__ rec,
     s.dBz = dB[2];
   } else {
     const int d1 = W.dims[0] - 1;
-    const double b1 = tri8(vc + 8 * d1, x, y, z);
-    const double vn = sqrt(b1 * b1);
-    double B11 = 0.0;
-    if (vn > 0.0) B11 = sqrt(div_p2(2 * (aL * vn + dm), R, rR));
-    const double r1 = draw_normal(rng);
-    double dB[3] = {0.0, 0.0, 0.0};
+    const R b1 = tri8(vc + 8 * d1, x, y, z);
+    const R vn = Mx::sq(b1 * b1);
+    R B11 = R(0);
+    if (vn > R(0)) B11 = Mx::sq(Mx::divr(2 * (aL * vn + dm), Rt, rR));
+    const R r1 = Normal<R, NRM>::draw(rng);
+    R dB[3] = {R(0), R(0), R(0)};
     dB[d1] = B11 * r1;
     s.dBx = dB[0];
     s.dBy = dB[1];
     s.dBz = dB[2];
   }
-  advect<ADV>(rec, dt, s);
+  advect<R, ADV>(rec, dt, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,10 +328,38 @@ __device__ __forceinline__ bool quadratic_axis(double dI, double A, double B,
   return false;
 }
 
+// FP32 form: the same roots from the cancellation-free pair q/A, -dI/q with
+// q = -(B + sign(B) sqrt(B^2 + 4 dI A))/2.
+__device__ __forceinline__ bool quadratic_axis(float dI, float A, float B,
+                                               float dtcur, float &dtaxis) {
+  const float arg = B * B + 4 * dI * A;
+  if ((arg >= 0.0f) && (A != 0.0f)) {
+    const float sq = __builtin_amdgcn_sqrtf(arg);
+    const float q = -0.5f * (B + copysignf(sq, B));
+    const float z1 = q * __builtin_amdgcn_rcpf(A);
+    const float z2 = q != 0.0f ? -dI * __builtin_amdgcn_rcpf(q) : 0.0f;
+    if (z1 > 0.0f || z2 > 0.0f) {
+      const float m = (z1 > 0.0f && z2 > 0.0f) ? fminf(z1, z2)
+                                               : (z1 > 0.0f ? z1 : z2);
+      dtaxis = m * m;
+      if (dtaxis == 0.0f) return true;
+    } else if (z1 == 0.0f || z2 == 0.0f) {
+      return true;
+    }
+  } else if (A == 0.0f) {
+    if (B != 0.0f) {
+      const float r = dI * __builtin_amdgcn_rcpf(B);
+      dtaxis = r * r;
+      if (dtaxis == 0.0f) return true;
+    }
+  }
+  if (dtaxis > dtcur) dtaxis = 0.0f;
+  return false;
+}
+
 struct NRTerms {
   double f0, f1, f2, f3;
 };
-
 // pr_NewtonRaphsonVariablesExponential (leaves the terms untouched at dt=0)
 __device__ __forceinline__ void nr_terms(double dt, double v, double g,
                                          double dI, double divD, double dB,
@@ -418,65 +518,67 @@ __device__ __noinline__ double newton_axis(double dt, double v, double g, double
 // ExitFaceAndUpdateTimeStep: finds the axis and time at which the
 // displacement reaches the first crossed face; updates t, dt, exitFace
 // exactly as the reference (including leaving exitFace untouched when no
-// axis yields a time).
-template <int ADV>
-__device__ void exit_time(const double *__restrict__ rec, double x,
-                          double y, double z, double nx, double ny, double nz,
-                          const StepTerms &s, double &t, double &dt,
+// axis yields a time).  The Newton solve of the exponential form runs in
+// FP64 for either arithmetic type.
+template <typename R, int ADV>
+__device__ void exit_time(const R *__restrict__ rec, R x, R y, R z, R nx, R ny,
+                          R nz, const StepTerms<R> &s, double &t, R &dt,
                           int &exitFace) {
   t = t - dt;
-  const double pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
-  const double dd[3] = {rec[F_DX], rec[F_DY], rec[F_DZ]};
-  const double v[3] = {s.vx, s.vy, s.vz};
-  const double dv[3] = {s.ddx, s.ddy, s.ddz};
-  const double db[3] = {s.dBx, s.dBy, s.dBz};
-  const double gr[3] = {rec[F_DVDX], rec[F_DVDY], rec[F_DVDZ]};
-  double dtx[3] = {0.0, 0.0, 0.0};
+  const R pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
+  const R dd[3] = {rec[F_DX], rec[F_DY], rec[F_DZ]};
+  const R v[3] = {s.vx, s.vy, s.vz};
+  const R dv[3] = {s.ddx, s.ddy, s.ddz};
+  const R db[3] = {s.dBx, s.dBy, s.dBz};
+  const R gr[3] = {rec[F_DVDX], rec[F_DVDY], rec[F_DVDZ]};
+  R dtx[3] = {R(0), R(0), R(0)};
   int fx[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if ((pn[a] > 1.0) || (pn[a] < 0.0)) {
-      double dI;
-      if (pn[a] > 1.0) {
-        dI = dd[a] * (1.0 - pp[a]);
+    if ((pn[a] > R(1)) || (pn[a] < R(0))) {
+      R dI;
+      if (pn[a] > R(1)) {
+        dI = dd[a] * (R(1) - pp[a]);
         fx[a] = 2 * a + 2;
       } else {
         dI = -dd[a] * pp[a];
         fx[a] = 2 * a + 1;
       }
-      if (dI == 0.0) {
+      if (dI == R(0)) {
         exitFace = fx[a];
-        dt = 0.0;
+        dt = R(0);
         return;
       }
       if constexpr (ADV == 2) {
         if (quadratic_axis(dI, v[a] + dv[a], db[a], dt, dtx[a])) {
           exitFace = fx[a];
-          dt = 0.0;
+          dt = R(0);
           return;
         }
       } else {
-        dtx[a] = newton_axis(dt, v[a], gr[a], dI, dv[a], db[a]);
-        if (dtx[a] > dt) dtx[a] = 0.0;
+        dtx[a] = (R)newton_axis((double)dt, (double)v[a], (double)gr[a],
+                                (double)dI, (double)dv[a], (double)db[a]);
+        if (dtx[a] > dt) dtx[a] = R(0);
       }
     }
   }
-  if (dtx[0] == 0.0 && dtx[1] == 0.0 && dtx[2] == 0.0) {
+  if (dtx[0] == R(0) && dtx[1] == R(0) && dtx[2] == R(0)) {
     t = t + dt;
     return;
   }
   int im = -1;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
-    if (dtx[a] > 0.0 && (im < 0 || dtx[a] < dtx[im])) im = a;
+    if (dtx[a] > R(0) && (im < 0 || dtx[a] < dtx[im])) im = a;
   dt = dtx[im];
   exitFace = fx[im];
   t = t + dt;
 }
 
-__device__ __forceinline__ bool outside(double x, double y, double z) {
-  return (x > 1.0) || (x < 0.0) || (y > 1.0) || (y < 0.0) || (z > 1.0) ||
-         (z < 0.0);
+template <typename R>
+__device__ __forceinline__ bool outside(R x, R y, R z) {
+  return (x > R(1)) || (x < R(0)) || (y > R(1)) || (y < R(0)) || (z > R(1)) ||
+         (z < R(0));
 }
 
 // outcome of one time-loop iteration of a cell visit
@@ -486,62 +588,68 @@ enum : int { IT_CONTINUE = 0, IT_EXIT = 2, IT_MAXTIME = 3, IT_UNDEFINED = 4 };
 // pr_ExecuteRandomWalkParticleTracking (TrackSubCell.F90:812-1373) carries
 // from one iteration of its time loop to the next, so a visit can be paused
 // between iterations and resumed in a later launch with identical results.
-struct Visit {
-  double x, y, z, t;      // current location and time
-  double ix, iy, iz, t0;  // visit start (initialLocation, tinit)
-  int restarts;           // posRestartCounter
-  int exitFace;           // exitFace (kept across iterations)
+// Times are FP64 for either arithmetic type (global time accumulation).
+template <typename R> struct Visit {
+  R x, y, z;       // current location
+  double t;        // and time
+  R ix, iy, iz;    // visit start (initialLocation, tinit)
+  double t0;
+  int restarts;    // posRestartCounter
+  int exitFace;    // exitFace (kept across iterations)
 };
 
 // One iteration of the time loop (TrackSubCell.F90:956-1368).  On IT_EXIT
 // and IT_MAXTIME the final point is left in V.x..V.t; *zero_dt reports an
 // exit taken with a zero time step.
-template <int NDIM, int MODEL, int ADV>
-__device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
-                       Visit &V, Stream &rng, bool &zero_dt) {
-  const double dtcell = rec[F_DTCELL];
-  const double maxRelJump = 0.5, dtRed = f32lit(0.1f);
-  const double DX = rec[F_DX], DY = rec[F_DY], DZ = rec[F_DZ];
-  const double rX = rec[F_RDX], rY = rec[F_RDY], rZ = rec[F_RDZ];
-  const int32_t *mb = reinterpret_cast<const int32_t *>(rec + F_MASSBND);
-  double x = V.x, y = V.y, z = V.z, t = V.t;
-  double nx, ny, nz, dt;
+template <typename R, int NRM, int NDIM, int MODEL, int ADV>
+__device__ int iterate(const R *__restrict__ rec, const WalkParams &W,
+                       Visit<R> &V, Stream &rng, bool &zero_dt) {
+  using Mx = M<R>;
+  const R dtcell = rec[F_DTCELL];
+  const R maxRelJump = R(0.5), dtRed = R(f32lit(0.1f));
+  const R DX = rec[F_DX], DY = rec[F_DY], DZ = rec[F_DZ];
+  const R rX = rec[Lay<R>::RDX], rY = rec[Lay<R>::RDY], rZ = rec[Lay<R>::RDZ];
+  const int32_t *mb = ints(rec) + Lay<R>::MASSBND;
+  R x = V.x, y = V.y, z = V.z;
+  double t = V.t;
+  R nx, ny, nz, dt;
   int exitFace = V.exitFace;
   bool atMax = false;
-  StepTerms s;
+  StepTerms<R> s;
   dt = dtcell;
   t = t + dt;
   if (W.t_stop < t) {
     t = t - dt;
-    dt = fmax(W.t_stop - t, 0.0);
+    dt = R(fmax(W.t_stop - t, 0.0));
     t = W.t_stop;
     atMax = true;
   }
-  displacements<NDIM, MODEL, ADV>(rec, W, x, y, z, dt, rng, s);
-  double sq = sqrt(dt);
-  double dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-  double dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-  double dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+  displacements<R, NRM, NDIM, MODEL, ADV>(rec, W, x, y, z, dt, rng, s);
+  R sq = Mx::sq(dt);
+  R dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+  R dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+  R dzr = s.dAz + s.ddz * dt + s.dBz * sq;
   // jump limiter: same draw, dt cut by 0.1 until each jump <= half a cell
-  while ((fabs(div_p2(dxr, DX, rX)) > maxRelJump) || (fabs(div_p2(dyr, DY, rY)) > maxRelJump) ||
-         (fabs(div_p2(dzr, DZ, rZ)) > maxRelJump)) {
+  while ((fabs(Mx::divr(dxr, DX, rX)) > maxRelJump) ||
+         (fabs(Mx::divr(dyr, DY, rY)) > maxRelJump) ||
+         (fabs(Mx::divr(dzr, DZ, rZ)) > maxRelJump)) {
     t = t - dt;
     dt = dtRed * dt;
-    advect<ADV>(rec, dt, s);
-    sq = sqrt(dt);
+    advect<R, ADV>(rec, dt, s);
+    sq = Mx::sq(dt);
     dxr = s.dAx + s.ddx * dt + s.dBx * sq;
     dyr = s.dAy + s.ddy * dt + s.dBy * sq;
     dzr = s.dAz + s.ddz * dt + s.dBz * sq;
     t = t + dt;
     atMax = false;
   }
-  nx = x + div_p2(dxr, DX, rX);
-  ny = y + div_p2(dyr, DY, rY);
-  nz = z + div_p2(dzr, DZ, rZ);
+  nx = x + Mx::divr(dxr, DX, rX);
+  ny = y + Mx::divr(dyr, DY, rY);
+  nz = z + Mx::divr(dzr, DZ, rZ);
   int loops = 0;
   while (outside(nx, ny, nz)) {
     loops = loops + 1;
-    exit_time<ADV>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
+    exit_time<R, ADV>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
     if (loops > 5) {  // maxInterfaceLoopCounter: restart the visit
       nx = V.ix; ny = V.iy; nz = V.iz;
       t = V.t0;
@@ -550,7 +658,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
-    advect<ADV>(rec, dt, s);
+    advect<R, ADV>(rec, dt, s);
     atMax = false;
     if (exitFace == 0) {  // no face found: restart with new draws
       nx = V.ix; ny = V.iy; nz = V.iz;
@@ -560,25 +668,25 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
-    sq = sqrt(dt);
+    sq = Mx::sq(dt);
     if (exitFace <= 2) {
-      nx = exitFace == 1 ? 0.0 : 1.0;
+      nx = exitFace == 1 ? R(0) : R(1);
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + div_p2(dyr, DY, rY);
+      ny = y + Mx::divr(dyr, DY, rY);
       dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + div_p2(dzr, DZ, rZ);
+      nz = z + Mx::divr(dzr, DZ, rZ);
     } else if (exitFace <= 4) {
       dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + div_p2(dxr, DX, rX);
-      ny = exitFace == 3 ? 0.0 : 1.0;
+      nx = x + Mx::divr(dxr, DX, rX);
+      ny = exitFace == 3 ? R(0) : R(1);
       dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + div_p2(dzr, DZ, rZ);
+      nz = z + Mx::divr(dzr, DZ, rZ);
     } else {
       dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + div_p2(dxr, DX, rX);
+      nx = x + Mx::divr(dxr, DX, rX);
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + div_p2(dyr, DY, rY);
-      nz = exitFace == 5 ? 0.0 : 1.0;
+      ny = y + Mx::divr(dyr, DY, rY);
+      nz = exitFace == 5 ? R(0) : R(1);
     }
     if (outside(nx, ny, nz)) continue;
     // elastic rebound on faces whose mass boundary is 1
@@ -587,8 +695,8 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       if (mb[exitFace - 1] != 1) break;
       rebounds = rebounds + 1;
       if (rebounds > 10) return IT_UNDEFINED;
-      const double xi = nx, yi = ny, zi = nz;
-      if (dt == 0.0) {
+      const R xi = nx, yi = ny, zi = nz;
+      if (dt == R(0)) {
         nx = V.ix; ny = V.iy; nz = V.iz;
         t = t - dt;
         dt = dtcell;
@@ -599,27 +707,27 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       t = t + dt;
       if (W.t_stop < t) {
         t = t - dt;
-        dt = fmax(W.t_stop - t, 0.0);
+        dt = R(fmax(W.t_stop - t, 0.0));
         t = W.t_stop;
         atMax = true;
-        advect<ADV>(rec, dt, s);
-        sq = sqrt(dt);
+        advect<R, ADV>(rec, dt, s);
+        sq = Mx::sq(dt);
         dxr = s.dAx + s.ddx * dt + s.dBx * sq;
         dyr = s.dAy + s.ddy * dt + s.dBy * sq;
         dzr = s.dAz + s.ddz * dt + s.dBz * sq;
       }
       if (exitFace <= 2) {
-        nx = nx - div_p2(dxr, DX, rX);
-        ny = ny + div_p2(dyr, DY, rY);
-        nz = nz + div_p2(dzr, DZ, rZ);
+        nx = nx - Mx::divr(dxr, DX, rX);
+        ny = ny + Mx::divr(dyr, DY, rY);
+        nz = nz + Mx::divr(dzr, DZ, rZ);
       } else if (exitFace <= 4) {
-        nx = nx + div_p2(dxr, DX, rX);
-        ny = ny - div_p2(dyr, DY, rY);
-        nz = nz + div_p2(dzr, DZ, rZ);
+        nx = nx + Mx::divr(dxr, DX, rX);
+        ny = ny - Mx::divr(dyr, DY, rY);
+        nz = nz + Mx::divr(dzr, DZ, rZ);
       } else {
-        nx = nx + div_p2(dxr, DX, rX);
-        ny = ny + div_p2(dyr, DY, rY);
-        nz = nz - div_p2(dzr, DZ, rZ);
+        nx = nx + Mx::divr(dxr, DX, rX);
+        ny = ny + Mx::divr(dyr, DY, rY);
+        nz = nz - Mx::divr(dzr, DZ, rZ);
       }
       if (outside(nx, ny, nz)) {
         x = xi; y = yi; z = zi;
@@ -634,12 +742,12 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
       } else {
         dt = dtcell;
         exitFace = 0;
-        if (nx == 0.0) exitFace = 1;
-        if (nx == 1.0) exitFace = 2;
-        if (ny == 0.0) exitFace = 3;
-        if (ny == 1.0) exitFace = 4;
-        if (nz == 0.0) exitFace = 5;
-        if (nz == 1.0) exitFace = 6;
+        if (nx == R(0)) exitFace = 1;
+        if (nx == R(1)) exitFace = 2;
+        if (ny == R(0)) exitFace = 3;
+        if (ny == R(1)) exitFace = 4;
+        if (nz == R(0)) exitFace = 5;
+        if (nz == R(1)) exitFace = 6;
       }
     }
   }
@@ -647,7 +755,7 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
   V.exitFace = exitFace;
   if (atMax) return IT_MAXTIME;
   if (exitFace > 0) {
-    zero_dt = dt == 0.0;
+    zero_dt = dt == R(0);
     return IT_EXIT;
   }
   return IT_CONTINUE;
@@ -656,33 +764,33 @@ __device__ int iterate(const double *__restrict__ rec, const WalkParams &W,
 // Checks TrackCell makes before tracking (TrackCell.f90:375-510) and the
 // early returns of TrackSubCell (:870-945).  Returns 0 to track, else the
 // particle status to report at the cell-entry location.
-__device__ __forceinline__ int entry_checks(const double *__restrict__ rec,
-                                            const WalkParams &W, double x,
-                                            double y, double z) {
+template <typename R>
+__device__ __forceinline__ int entry_checks(const R *__restrict__ rec,
+                                            const WalkParams &W, R x, R y, R z) {
   if (outside(x, y, z)) return 9;  // InvalidLocation -> Undefined
-  const int32_t *fl = reinterpret_cast<const int32_t *>(rec + F_FLAGS);
+  const int32_t *fl = ints(rec) + Lay<R>::FLAGS;
   if (fl[0] == 0) return 7;
-  const int32_t zone = reinterpret_cast<const int32_t *>(rec + F_ZONE)[0];
+  const int32_t zone = ints(rec)[Lay<R>::ZONE];
   if (W.stop_zone > 0 && zone == W.stop_zone) return 6;
   bool stopIfNoExit = true;
-  const double src = rec[F_SOURCE], snk = rec[F_SINK];
+  const R src = rec[Lay<R>::SOURCE], snk = rec[Lay<R>::SINK];
   if (W.backward) {
     if (W.stop_weak_sources) {
-      if (src != 0.0) return 4;
+      if (src != R(0)) return 4;
       if (!W.steady_state) stopIfNoExit = false;
     }
   } else if (W.stop_weak_sinks) {
-    if (snk != 0.0) return 3;
+    if (snk != R(0)) return 3;
     if (!W.steady_state) stopIfNoExit = false;
   }
   if (!(fl[1] & CF_CELL_EXIT)) {
-    if (W.steady_state || (W.backward && src != 0.0) ||
-        (!W.backward && snk != 0.0))
+    if (W.steady_state || (W.backward && src != R(0)) ||
+        (!W.backward && snk != R(0)))
       return 5;
   }
   if (stopIfNoExit && !(fl[1] & CF_SUBCELL_EXIT)) return 5;
   if (fl[1] & CF_DRY) return 7;
-  if (rec[F_DTCELL] <= 0.0) return 9;
+  if (rec[F_DTCELL] <= R(0)) return 9;
   return 0;
 }
 
@@ -702,19 +810,20 @@ __device__ __forceinline__ int32_t pack_aux(const Aux &a) {
 // result, given the entry location (x, y, z, t) of the current cell: the
 // advance's start location itself, or after a transfer the exit point in
 // the previous cell (the entry point seen from the other side).
-__device__ __forceinline__ void last_point(const double *__restrict__ cells,
-                                           int entryFace, int &c, double &x,
-                                           double &y, double &z) {
+template <typename R>
+__device__ __forceinline__ void last_point(const R *__restrict__ cells,
+                                           int entryFace, int &c, R &x, R &y,
+                                           R &z) {
   if (entryFace == 0) return;
-  const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-  c = reinterpret_cast<const int32_t *>(rec + F_CONN)[entryFace - 1];
+  const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
+  c = ints(rec)[Lay<R>::CONN + entryFace - 1];
   switch (entryFace) {
-    case 1: x = 1.0; break;
-    case 2: x = 0.0; break;
-    case 3: y = 1.0; break;
-    case 4: y = 0.0; break;
-    case 5: z = 1.0; break;
-    default: z = 0.0; break;
+    case 1: x = R(1); break;
+    case 2: x = R(0); break;
+    case 3: y = R(1); break;
+    case 4: y = R(0); break;
+    case 5: z = R(1); break;
+    default: z = R(0); break;
   }
 }
 
@@ -762,17 +871,21 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   return x * q + (x < r ? x : r) + j;
 }
 
-struct ParticlePtrs {
+// Particle state as the walk of arithmetic type R sees it: positions in R,
+// times in FP64.
+template <typename R> struct ParticlePtrs {
   int32_t *cell, *status, *aux;
-  double *lx, *ly, *lz, *t, *ix, *iy, *iz, *t0;
+  R *lx, *ly, *lz, *ix, *iy, *iz;
+  double *t, *t0;
   uint64_t *rng;
   int64_t *substeps;
 };
 
 // Start of an advance: release pending particles (MPathRW.F90:1256-1302),
 // reset the per-call loop state and list the active particles.
-__global__ void prepare_advance(const double *__restrict__ cells, int64_t n,
-                                ParticlePtrs S, const WalkParams W,
+template <typename R>
+__global__ void prepare_advance(const R *__restrict__ cells, int64_t n,
+                                ParticlePtrs<R> S, const WalkParams W,
                                 int32_t *__restrict__ keys,
                                 int32_t *__restrict__ vals,
                                 unsigned long long *__restrict__ counters) {
@@ -787,14 +900,13 @@ __global__ void prepare_advance(const double *__restrict__ cells, int64_t n,
       if (t < W.t_engine) st = 8;
       else if (t <= W.t_stop) {
         st = 1;
-        const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-        if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[0] == 0) st = 7;
+        const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
+        if (ints(rec)[Lay<R>::FLAGS] == 0) st = 7;
       }
     }
     if (st == 7) {
-      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
-      if (reinterpret_cast<const int32_t *>(rec + F_FLAGS)[1] & CF_VERIFY_PDRY)
-        st = 1;
+      const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
+      if (ints(rec)[Lay<R>::FLAGS + 1] & CF_VERIFY_PDRY) st = 1;
     }
     S.status[p] = st;
     if (st == 1) {
@@ -807,12 +919,12 @@ __global__ void prepare_advance(const double *__restrict__ cells, int64_t n,
 
 // One launch of the walk: every listed particle runs up to `budget`
 // time-loop iterations; particles not done are appended to the next list.
-template <int NDIM, int MODEL, int ADV, int MINW>
+template <typename R, int NRM, int NDIM, int MODEL, int ADV, int MINW>
 __global__ void __launch_bounds__(256, MINW)
-    walk_kernel(const double *__restrict__ cells,
-                const int32_t *__restrict__ work, int64_t nwork, ParticlePtrs S,
-                const WalkParams W, int budget, int32_t *__restrict__ nkeys,
-                int32_t *__restrict__ nvals,
+    walk_kernel(const R *__restrict__ cells,
+                const int32_t *__restrict__ work, int64_t nwork,
+                ParticlePtrs<R> S, const WalkParams W, int budget,
+                int32_t *__restrict__ nkeys, int32_t *__restrict__ nvals,
                 unsigned long long *__restrict__ counters) {
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t i = blk * blockDim.x + threadIdx.x;
@@ -822,7 +934,7 @@ __global__ void __launch_bounds__(256, MINW)
   if (i < nwork) {
     p = work[i];
     c = S.cell[p];
-    Visit V;
+    Visit<R> V;
     V.x = S.lx[p]; V.y = S.ly[p]; V.z = S.lz[p]; V.t = S.t[p];
     Aux A = unpack_aux(S.aux[p]);
     if (A.phase) {
@@ -836,7 +948,7 @@ __global__ void __launch_bounds__(256, MINW)
     int status = 1;
     bool done = false;
     while (!done) {
-      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+      const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
       if (!A.phase) {  // entering a cell: TrackCell checks, new visit
         const int st = entry_checks(rec, W, V.x, V.y, V.z);
         if (st != 0) {
@@ -857,7 +969,7 @@ __global__ void __launch_bounds__(256, MINW)
       }
       steps += 1;
       bool zero_dt = false;
-      const int out = iterate<NDIM, MODEL, ADV>(rec, W, V, rng, zero_dt);
+      const int out = iterate<R, NRM, NDIM, MODEL, ADV>(rec, W, V, rng, zero_dt);
       if (out == IT_CONTINUE) continue;
       if (out == IT_MAXTIME) {
         status = 1;
@@ -875,11 +987,11 @@ __global__ void __launch_bounds__(256, MINW)
           done = true;
           break;
         }
-        const int next = reinterpret_cast<const int32_t *>(rec + F_CONN)[face - 1];
+        const int next = ints(rec)[Lay<R>::CONN + face - 1];
         bool inactive_next = next <= 0;
         if (!inactive_next) {
-          const double *nrec = cells + (int64_t)(next - 1) * CELL_DOUBLES;
-          inactive_next = reinterpret_cast<const int32_t *>(nrec + F_FLAGS)[0] == 0;
+          const R *nrec = cells + (int64_t)(next - 1) * Lay<R>::STRIDE;
+          inactive_next = ints(nrec)[Lay<R>::FLAGS] == 0;
         }
         if (inactive_next) {
           status = 2;
@@ -888,14 +1000,14 @@ __global__ void __launch_bounds__(256, MINW)
         }
         // ConvertFromNeighbor: full-face connection, face coordinate flips
         bool bad;
-        double fx = V.x, fy = V.y, fz = V.z;
+        R fx = V.x, fy = V.y, fz = V.z;
         switch (face) {
-          case 1: bad = fx != 0.0; fx = 1.0; break;
-          case 2: bad = fx != 1.0; fx = 0.0; break;
-          case 3: bad = fy != 0.0; fy = 1.0; break;
-          case 4: bad = fy != 1.0; fy = 0.0; break;
-          case 5: bad = fz != 0.0; fz = 1.0; break;
-          default: bad = fz != 1.0; fz = 0.0; break;
+          case 1: bad = fx != R(0); fx = R(1); break;
+          case 2: bad = fx != R(1); fx = R(0); break;
+          case 3: bad = fy != R(0); fy = R(1); break;
+          case 4: bad = fy != R(1); fy = R(0); break;
+          case 5: bad = fz != R(0); fz = R(1); break;
+          default: bad = fz != R(1); fz = R(0); break;
         }
         if (bad) {  // no defined continuation in the reference
           status = 9;
@@ -930,15 +1042,17 @@ __global__ void __launch_bounds__(256, MINW)
 }
 
 // Particles still listed after the step guard: status 9 at the last point.
-__global__ void terminate_walk(const double *__restrict__ cells,
+template <typename R>
+__global__ void terminate_walk(const R *__restrict__ cells,
                                const int32_t *__restrict__ work, int64_t nwork,
-                               ParticlePtrs S) {
+                               ParticlePtrs<R> S) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nwork) return;
   const int p = work[i];
   Aux A = unpack_aux(S.aux[p]);
   int c = S.cell[p];
-  double x = S.ix[p], y = S.iy[p], z = S.iz[p], t = S.t0[p];
+  R x = S.ix[p], y = S.iy[p], z = S.iz[p];
+  double t = S.t0[p];
   if (!A.phase) {
     x = S.lx[p]; y = S.ly[p]; z = S.lz[p]; t = S.t[p];
   }
@@ -1059,9 +1173,14 @@ hipError_t launch_seed_streams(rwpt_engine *e, int64_t n, int64_t offset) {
   return hipGetLastError();
 }
 
-static ParticlePtrs particle_ptrs(Particles &P) {
-  return ParticlePtrs{P.cell, P.status, P.aux, P.lx, P.ly, P.lz, P.t,
-                      P.ix,   P.iy,     P.iz,  P.t0, P.rng, P.substeps};
+static ParticlePtrs<double> particle_ptrs64(Particles &P) {
+  return ParticlePtrs<double>{P.cell, P.status, P.aux, P.lx, P.ly, P.lz, P.ix,
+                              P.iy,   P.iz,     P.t,   P.t0, P.rng, P.substeps};
+}
+
+static ParticlePtrs<float> particle_ptrs32(Particles &P) {
+  return ParticlePtrs<float>{P.cell, P.status, P.aux, P.flx, P.fly, P.flz, P.fix,
+                             P.fiy,  P.fiz,    P.t,   P.t0,  P.rng, P.substeps};
 }
 
 static int radix_bits(int64_t ncell) {
@@ -1088,6 +1207,69 @@ hipError_t ensure_sort_storage(rwpt_engine *e) {
   return hipSuccess;
 }
 
+// Launch of the walk variant for (precision, dimensions, model, advection,
+// occupancy hint); the occupancy variants exist for the 3-D linear EULERIAN
+// walk only.
+template <typename R, int NRM, int ND, int MO, int AD>
+static void launch_walk_variant(rwpt_engine *e, const R *cells,
+                                const ParticlePtrs<R> &S, const WalkParams &W,
+                                dim3 grid, int block, const int32_t *work,
+                                int64_t cnt, int budget, int32_t *nk,
+                                int32_t *nv) {
+  const int mw = e->tune.min_waves;
+  if constexpr (ND == 3 && MO == 0 && AD == 2) {
+    if (mw == 3) {
+      hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 3>), grid, dim3(block), 0,
+                         e->stream, cells, work, cnt, S, W, budget, nk, nv,
+                         e->counters);
+      return;
+    }
+    if (mw == 4) {
+      hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 4>), grid, dim3(block), 0,
+                         e->stream, cells, work, cnt, S, W, budget, nk, nv,
+                         e->counters);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 1>), grid, dim3(block), 0,
+                     e->stream, cells, work, cnt, S, W, budget, nk, nv,
+                     e->counters);
+}
+
+template <int ND, int MO, int AD>
+static void launch_walk_mode(rwpt_engine *e, const WalkParams &W, dim3 grid,
+                             int block, const int32_t *work, int64_t cnt,
+                             int budget, int32_t *nk, int32_t *nv) {
+  Particles &P = e->part;
+  if (e->opt.precision == 0)
+    launch_walk_variant<double, 0, ND, MO, AD>(e, e->prob.cells, particle_ptrs64(P), W,
+                                               grid, block, work, cnt, budget, nk, nv);
+  else if (e->opt.precision == 1)
+    launch_walk_variant<float, 1, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P), W,
+                                              grid, block, work, cnt, budget, nk, nv);
+  else
+    launch_walk_variant<float, 2, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P), W,
+                                              grid, block, work, cnt, budget, nk, nv);
+}
+
+static hipError_t launch_walk(rwpt_engine *e, const WalkParams &W, dim3 grid,
+                              int block, const int32_t *work, int64_t cnt,
+                              int budget, int32_t *nk, int32_t *nv) {
+  const int mode = (W.ndim - 1) * 4 + W.disp_model * 2 + (W.advection_kind - 1);
+#define RWPT_WALK(ND, MO, AD)                                                  \
+  case (ND - 1) * 4 + MO * 2 + (AD - 1):                                       \
+    launch_walk_mode<ND, MO, AD>(e, W, grid, block, work, cnt, budget, nk, nv); \
+    break;
+  switch (mode) {
+    RWPT_WALK(1, 0, 1) RWPT_WALK(1, 0, 2) RWPT_WALK(1, 1, 1) RWPT_WALK(1, 1, 2)
+    RWPT_WALK(2, 0, 1) RWPT_WALK(2, 0, 2) RWPT_WALK(2, 1, 1) RWPT_WALK(2, 1, 2)
+    RWPT_WALK(3, 0, 1) RWPT_WALK(3, 0, 2) RWPT_WALK(3, 1, 1) RWPT_WALK(3, 1, 2)
+    default: return hipErrorInvalidValue;
+  }
+#undef RWPT_WALK
+  return hipGetLastError();
+}
+
 // One tracking interval: prepare, then walk launches of at most `budget`
 // iterations per particle over the cell-sorted list of unfinished
 // particles, until every particle has reached t_stop or stopped.
@@ -1111,11 +1293,17 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   hipError_t err = hipMemsetAsync(e->counters, 0, 4 * sizeof(unsigned long long),
                                   e->stream);
   if (err != hipSuccess || P.n == 0) return err;
-  const ParticlePtrs S = particle_ptrs(P);
+  const bool fp32 = e->opt.precision != 0;
   const int pb = 256;
-  hipLaunchKernelGGL(prepare_advance, dim3((unsigned)((P.n + pb - 1) / pb)),
-                     dim3(pb), 0, e->stream, e->prob.cells, P.n, S, W, P.keys[0],
-                     P.vals[0], e->counters);
+  const dim3 pgrid((unsigned)((P.n + pb - 1) / pb));
+  if (fp32)
+    hipLaunchKernelGGL(prepare_advance<float>, pgrid, dim3(pb), 0, e->stream,
+                       e->prob.cells32, P.n, particle_ptrs32(P), W, P.keys[0],
+                       P.vals[0], e->counters);
+  else
+    hipLaunchKernelGGL(prepare_advance<double>, pgrid, dim3(pb), 0, e->stream,
+                       e->prob.cells, P.n, particle_ptrs64(P), W, P.keys[0],
+                       P.vals[0], e->counters);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   unsigned long long cnt = 0;
   err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost,
@@ -1138,9 +1326,13 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
       work = P.vals[1];
     }
     if (launches >= max_launches) {  // step guard reached
-      hipLaunchKernelGGL(terminate_walk, dim3((unsigned)((cnt + 255) / 256)),
-                         dim3(256), 0, e->stream, e->prob.cells, work,
-                         (int64_t)cnt, S);
+      const dim3 tg((unsigned)((cnt + 255) / 256));
+      if (fp32)
+        hipLaunchKernelGGL(terminate_walk<float>, tg, dim3(256), 0, e->stream,
+                           e->prob.cells32, work, (int64_t)cnt, particle_ptrs32(P));
+      else
+        hipLaunchKernelGGL(terminate_walk<double>, tg, dim3(256), 0, e->stream,
+                           e->prob.cells, work, (int64_t)cnt, particle_ptrs64(P));
       if ((err = hipGetLastError()) != hipSuccess) return err;
       break;
     }
@@ -1159,30 +1351,8 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
     }
     (void)hipEventRecord(e->walk_events[2 * launches], e->stream);
     const dim3 grid((unsigned)((cnt + block - 1) / block));
-    const int mode = (W.ndim - 1) * 4 + W.disp_model * 2 + (W.advection_kind - 1);
-#define RWPT_WALK(ND, MO, AD)                                                  \
-  case (ND - 1) * 4 + MO * 2 + (AD - 1):                                       \
-    if (ND == 3 && MO == 0 && AD == 2 && e->tune.min_waves == 3)               \
-      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 3>), grid, dim3(block), 0,   \
-                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
-                         budget, nk, nv, e->counters);                         \
-    else if (ND == 3 && MO == 0 && AD == 2 && e->tune.min_waves == 4)          \
-      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 4>), grid, dim3(block), 0,   \
-                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
-                         budget, nk, nv, e->counters);                         \
-    else                                                                       \
-      hipLaunchKernelGGL((walk_kernel<ND, MO, AD, 1>), grid, dim3(block), 0,   \
-                         e->stream, e->prob.cells, work, (int64_t)cnt, S, W,   \
-                         budget, nk, nv, e->counters);                         \
-    break;
-    switch (mode) {
-      RWPT_WALK(1, 0, 1) RWPT_WALK(1, 0, 2) RWPT_WALK(1, 1, 1) RWPT_WALK(1, 1, 2)
-      RWPT_WALK(2, 0, 1) RWPT_WALK(2, 0, 2) RWPT_WALK(2, 1, 1) RWPT_WALK(2, 1, 2)
-      RWPT_WALK(3, 0, 1) RWPT_WALK(3, 0, 2) RWPT_WALK(3, 1, 1) RWPT_WALK(3, 1, 2)
-      default: return hipErrorInvalidValue;
-    }
-#undef RWPT_WALK
-    if ((err = hipGetLastError()) != hipSuccess) return err;
+    err = launch_walk(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv);
+    if (err != hipSuccess) return err;
     (void)hipEventRecord(e->walk_events[2 * launches + 1], e->stream);
     launches += 1;
     err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost,
