@@ -441,6 +441,20 @@ __global__ void to_double3(int64_t n, const float *__restrict__ fa,
   c[i] = fc[i];
 }
 
+// Clears *flag unless every cell stores the exact reciprocals of DX, DY, DZ
+// and R (all powers of two).
+__global__ void all_pow2(int64_t ncell, const double *__restrict__ cells,
+                         unsigned long long *__restrict__ flag) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = true;
+  if (c < ncell) {
+    const double *rec = cells + c * CELL_DOUBLES;
+    ok = rec[F_RDX] != 0.0 && rec[F_RDY] != 0.0 && rec[F_RDZ] != 0.0 &&
+         rec[F_RR] != 0.0;
+  }
+  if (__ballot(!ok) != 0 && __lane_id() == 0) atomicAnd(flag, 0ull);
+}
+
 }  // namespace
 
 hipError_t build_cell_tables32(rwpt_engine *e) {
@@ -489,7 +503,20 @@ hipError_t build_cell_tables(rwpt_engine *e, double *debug_out) {
   hipLaunchKernelGGL(corner_tables, dim3(grid), dim3(block), 0, e->stream, g,
                      P.ncell, P.delr, P.delc, P.porosity, P.retardation,
                      P.disp, P.ndisp, P.q6, P.dzrw, tp, P.cells, debug_out);
-  return hipGetLastError();
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // counters[3] is free outside an advance
+  unsigned long long one = 1ull, flag = 0ull;
+  err = hipMemcpyAsync(e->counters + 3, &one, sizeof one, hipMemcpyHostToDevice,
+                       e->stream);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(all_pow2, dim3(grid), dim3(block), 0, e->stream, P.ncell,
+                     P.cells, e->counters + 3);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  err = hipMemcpyAsync(&flag, e->counters + 3, sizeof flag, hipMemcpyDeviceToHost,
+                       e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  P.pow2_cells = err == hipSuccess && flag == 1ull;
+  return err;
 }
 
 }  // namespace rwpt

@@ -108,6 +108,7 @@ struct DeviceProblem {
   // tables
   double *cells = nullptr;  // ncell * CELL_DOUBLES
   float *cells32 = nullptr; // FP32-local tables (ncell * CELL_FLOATS)
+  bool pow2_cells = false;  // every cell's DX, DY, DZ, R a power of two
 };
 
 struct Particles {

@@ -128,6 +128,16 @@ template <> struct M<float> {
   }
 };
 
+// P2 = 1: every cell's DX, DY, DZ and R are powers of two (a flag of the
+// table build), so every a / d of the sub-step is the exact product a * (1/d)
+// with no per-lane test left to the compiler.
+template <typename R, int P2> struct Ops : M<R> {};
+template <> struct Ops<double, 1> : M<double> {
+  static __device__ __forceinline__ double divr(double a, double, double r) {
+    return a * r;
+  }
+};
+
 template <typename R>
 __device__ __forceinline__ const int32_t *ints(const R *rec) {
   return reinterpret_cast<const int32_t *>(rec);
@@ -197,12 +207,12 @@ __device__ __forceinline__ void advect(const R *__restrict__ rec, R dt,
 }
 
 // pr_RWPTDisplacementsLinear / ...Axisymmetric for one sub-step.
-template <typename R, int NRM, int NDIM, int MODEL, int ADV>
+template <typename R, int NRM, int P2, int NDIM, int MODEL, int ADV>
 __device__ __forceinline__ void displacements(const R *__restrict__ rec,
                                               const WalkParams &W, R x, R y,
                                               R z, R dt, Stream &rng,
                                               StepTerms<R> &s) {
-  using Mx = M<R>;
+  using Mx = Ops<R, P2>;
   s.vx = (R(1) - x) * rec[F_VX1] + x * rec[F_VX2];
   s.vy = (R(1) - y) * rec[F_VY1] + y * rec[F_VY2];
   s.vz = (R(1) - z) * rec[F_VZ1] + z * rec[F_VZ2];
@@ -601,10 +611,10 @@ template <typename R> struct Visit {
 // One iteration of the time loop (TrackSubCell.F90:956-1368).  On IT_EXIT
 // and IT_MAXTIME the final point is left in V.x..V.t; *zero_dt reports an
 // exit taken with a zero time step.
-template <typename R, int NRM, int NDIM, int MODEL, int ADV>
+template <typename R, int NRM, int P2, int NDIM, int MODEL, int ADV>
 __device__ int iterate(const R *__restrict__ rec, const WalkParams &W,
                        Visit<R> &V, Stream &rng, bool &zero_dt) {
-  using Mx = M<R>;
+  using Mx = Ops<R, P2>;
   const R dtcell = rec[F_DTCELL];
   const R maxRelJump = R(0.5), dtRed = R(f32lit(0.1f));
   const R DX = rec[F_DX], DY = rec[F_DY], DZ = rec[F_DZ];
@@ -624,7 +634,7 @@ __device__ int iterate(const R *__restrict__ rec, const WalkParams &W,
     t = W.t_stop;
     atMax = true;
   }
-  displacements<R, NRM, NDIM, MODEL, ADV>(rec, W, x, y, z, dt, rng, s);
+  displacements<R, NRM, P2, NDIM, MODEL, ADV>(rec, W, x, y, z, dt, rng, s);
   R sq = Mx::sq(dt);
   R dxr = s.dAx + s.ddx * dt + s.dBx * sq;
   R dyr = s.dAy + s.ddy * dt + s.dBy * sq;
@@ -919,7 +929,7 @@ __global__ void prepare_advance(const R *__restrict__ cells, int64_t n,
 
 // One launch of the walk: every listed particle runs up to `budget`
 // time-loop iterations; particles not done are appended to the next list.
-template <typename R, int NRM, int NDIM, int MODEL, int ADV, int MINW>
+template <typename R, int NRM, int P2, int NDIM, int MODEL, int ADV, int MINW>
 __global__ void __launch_bounds__(256, MINW)
     walk_kernel(const R *__restrict__ cells,
                 const int32_t *__restrict__ work, int64_t nwork,
@@ -969,7 +979,7 @@ __global__ void __launch_bounds__(256, MINW)
       }
       steps += 1;
       bool zero_dt = false;
-      const int out = iterate<R, NRM, NDIM, MODEL, ADV>(rec, W, V, rng, zero_dt);
+      const int out = iterate<R, NRM, P2, NDIM, MODEL, ADV>(rec, W, V, rng, zero_dt);
       if (out == IT_CONTINUE) continue;
       if (out == IT_MAXTIME) {
         status = 1;
@@ -1210,7 +1220,7 @@ hipError_t ensure_sort_storage(rwpt_engine *e) {
 // Launch of the walk variant for (precision, dimensions, model, advection,
 // occupancy hint); the occupancy variants exist for the 3-D linear EULERIAN
 // walk only.
-template <typename R, int NRM, int ND, int MO, int AD>
+template <typename R, int NRM, int P2, int ND, int MO, int AD>
 static void launch_walk_variant(rwpt_engine *e, const R *cells,
                                 const ParticlePtrs<R> &S, const WalkParams &W,
                                 dim3 grid, int block, const int32_t *work,
@@ -1219,19 +1229,19 @@ static void launch_walk_variant(rwpt_engine *e, const R *cells,
   const int mw = e->tune.min_waves;
   if constexpr (ND == 3 && MO == 0 && AD == 2) {
     if (mw == 3) {
-      hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 3>), grid, dim3(block), 0,
+      hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 3>), grid, dim3(block), 0,
                          e->stream, cells, work, cnt, S, W, budget, nk, nv,
                          e->counters);
       return;
     }
     if (mw == 4) {
-      hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 4>), grid, dim3(block), 0,
+      hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 4>), grid, dim3(block), 0,
                          e->stream, cells, work, cnt, S, W, budget, nk, nv,
                          e->counters);
       return;
     }
   }
-  hipLaunchKernelGGL((walk_kernel<R, NRM, ND, MO, AD, 1>), grid, dim3(block), 0,
+  hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 1>), grid, dim3(block), 0,
                      e->stream, cells, work, cnt, S, W, budget, nk, nv,
                      e->counters);
 }
@@ -1241,15 +1251,22 @@ static void launch_walk_mode(rwpt_engine *e, const WalkParams &W, dim3 grid,
                              int block, const int32_t *work, int64_t cnt,
                              int budget, int32_t *nk, int32_t *nv) {
   Particles &P = e->part;
-  if (e->opt.precision == 0)
-    launch_walk_variant<double, 0, ND, MO, AD>(e, e->prob.cells, particle_ptrs64(P), W,
-                                               grid, block, work, cnt, budget, nk, nv);
+  if (e->opt.precision == 0 && e->prob.pow2_cells)
+    launch_walk_variant<double, 0, 1, ND, MO, AD>(e, e->prob.cells, particle_ptrs64(P),
+                                                  W, grid, block, work, cnt, budget, nk,
+                                                  nv);
+  else if (e->opt.precision == 0)
+    launch_walk_variant<double, 0, 0, ND, MO, AD>(e, e->prob.cells, particle_ptrs64(P),
+                                                  W, grid, block, work, cnt, budget, nk,
+                                                  nv);
   else if (e->opt.precision == 1)
-    launch_walk_variant<float, 1, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P), W,
-                                              grid, block, work, cnt, budget, nk, nv);
+    launch_walk_variant<float, 1, 0, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P),
+                                                 W, grid, block, work, cnt, budget, nk,
+                                                 nv);
   else
-    launch_walk_variant<float, 2, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P), W,
-                                              grid, block, work, cnt, budget, nk, nv);
+    launch_walk_variant<float, 2, 0, ND, MO, AD>(e, e->prob.cells32, particle_ptrs32(P),
+                                                 W, grid, block, work, cnt, budget, nk,
+                                                 nv);
 }
 
 static hipError_t launch_walk(rwpt_engine *e, const WalkParams &W, dim3 grid,

@@ -29,12 +29,19 @@ def _run(cmd, cwd=None):
 
 def build_library(verbose=True):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
+    objs, procs = [], []
+    # the translation units compile in parallel
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        _run([hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
-              os.path.join(CSRC, src), "-o", obj])
+        cmd = [hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
+               os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
+    for cmd, pr in procs:
+        if pr.wait() != 0:
+            raise RuntimeError(f"build step failed: {' '.join(cmd)}")
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
