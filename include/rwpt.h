@@ -89,6 +89,14 @@ typedef struct {
                             volume before scaling_factor, as
                             ComputeDensity(unitVolume=.true.) does for
                             MPathRW.F90:1626-1642; 0 the density itself */
+  int32_t n_opt_loops;   /* gpkdeNOptLoops: 0 one pass with the Silverman
+                            bandwidth; > 0 the bandwidth optimisation of
+                            ComputeDensityOptimization with brute-force
+                            kernels (GridProjectedKDE.F90:4396-5160), 2-D
+                            and 3-D reconstructions                      */
+  int32_t isotropic;     /* gpkdeIsotropicKernels                        */
+  int32_t skip_error_convergence; /* gpkdeSkipError: run all loops        */
+  double rel_error_convergence;   /* gpkdeRelErrorConvergence (<= 0: 0.02) */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,
@@ -178,10 +186,10 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
              double *density, double *histogram);
 
 /* Quantities of the last rwpt_kde call: the Silverman bandwidth used, the
- * number of active particles passed, the histogram's nPoints and the
- * effective mass (Histogram.F90:332-341). */
+ * number of active particles passed, the histogram's nPoints, the
+ * effective mass (Histogram.F90:332-341) and the optimisation loops run. */
 int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
-                  double *npoints, double *eff_mass);
+                  double *npoints, double *eff_mass, int32_t *loops);
 
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,

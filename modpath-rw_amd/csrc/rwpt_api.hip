@@ -521,6 +521,7 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
   if (!e || !opt) return RWPT_E_ARG;
   if (opt->mode != 0 && opt->mode != 1)
     return fail(e, RWPT_E_ARG, "kde mode must be 0 or 1");
+  if (opt->n_opt_loops < 0) return fail(e, RWPT_E_ARG, "n_opt_loops < 0");
   int64_t ncells = 1;
   for (int d = 0; d < 3; ++d) {
     if (opt->nbins[d] < 1) return fail(e, RWPT_E_ARG, "nbins must be >= 1");
@@ -539,6 +540,14 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
       err = hipMemcpyAsync(m, mass, n * 8, hipMemcpyHostToDevice, e->stream);
   }
   if (err == hipSuccess) err = launch_kde(e, opt, m, dens, hist, e->kde_last);
+  if (err == hipErrorNotSupported) {
+    dev_free(dens);
+    dev_free(hist);
+    dev_free(m);
+    return fail(e, RWPT_E_UNSUPPORTED,
+                "bandwidth optimisation needs a 2-D or 3-D grid and kernels narrower "
+                "than 2500 bins");
+  }
   if (err == hipSuccess && density)
     err = hipMemcpyAsync(density, dens, ncells * 8, hipMemcpyDeviceToHost, e->stream);
   if (err == hipSuccess && histogram)
@@ -560,8 +569,9 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
 }
 
 int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
-                  double *npoints, double *eff_mass) {
+                  double *npoints, double *eff_mass, int32_t *loops) {
   if (!e) return RWPT_E_ARG;
+  if (loops) *loops = e->kde_loops;
   if (bandwidth) *bandwidth = e->kde_last[0];
   if (points) *points = e->kde_last[1];
   if (npoints) *npoints = e->kde_last[2];

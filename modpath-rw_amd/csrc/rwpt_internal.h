@@ -162,6 +162,7 @@ struct rwpt_engine {
   double *kde_weights = nullptr;
   int64_t kde_weights_size = 0;
   double kde_last[4] = {0, 0, 0, 0};  // bandwidth, points, nPoints, effMass
+  int32_t kde_loops = 0;              // optimisation loops of the last kde
   double *xface = nullptr, *yface = nullptr;  // DIS face coordinates X, Y
   rwpt::KernelTimer timers[3];
   rwpt::Tuning tune;
@@ -183,6 +184,9 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                       double *hist_dev, double *stats_host);
 
 hipError_t ensure_sort_storage(rwpt_engine *e);
+hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
+                            const double *counts_dev, double hsilverman,
+                            double nPoints, double *density_dev, int *loops_run);
 // FP32-local state: float record table from the FP64 one, positions to and
 // from the double arrays
 hipError_t build_cell_tables32(rwpt_engine *e);

@@ -189,6 +189,18 @@ __global__ void kde_scale(int64_t n, const double *__restrict__ in,
   if (i < n) out[i] = in[i] * scale;
 }
 
+// density * effectiveMass (GridProjectedKDE.F90:5074-5080), * binVolume
+// (unitVolume, :4292-4296), * scalingFactor (:4297-4300), in that order
+__global__ void kde_finish(int64_t n, double *__restrict__ d, double eff,
+                           double vol, double sf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = d[i];
+  if (eff != 1.0) v = v * eff;
+  if (vol != 1.0) v = v * vol;
+  d[i] = v * sf;
+}
+
 // One 1-D pass of the separable kernel along `axis` with half-sample
 // symmetric padding (d c b a | a b c d | d c b a); weights w[0..m] are the
 // normalised 1-D kernel at offsets 0..m.
@@ -369,7 +381,26 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     stats_host[3] = effMass;
   }
   (void)hipEventRecord(TD.start, st);
-  if (opt->mode == 0 || h <= 0.0) {
+  e->kde_loops = 0;
+  if (opt->mode == 1 && opt->n_opt_loops > 0 && h > 0.0) {
+    // bandwidth optimisation: the loop sees the counts the reference's
+    // weighted histogram holds (mass sums / effective mass)
+    if ((err = hipMemcpyAsync(tmpA, hist_dev, ncells * sizeof(double),
+                              hipMemcpyDeviceToDevice, st)) != hipSuccess)
+      return err;
+    if (mass_dev)
+      hipLaunchKernelGGL(kde_scale, dim3((unsigned)((ncells + blk - 1) / blk)),
+                         dim3(blk), 0, st, ncells, tmpA, tmpA, 1.0 / effMass);
+    int loops = 0;
+    if ((err = launch_gpkde_opt(e, opt, tmpA, h, nPoints, density_dev, &loops)) !=
+        hipSuccess)
+      return err;
+    e->kde_loops = loops;
+    hipLaunchKernelGGL(kde_finish, dim3((unsigned)((ncells + blk - 1) / blk)),
+                       dim3(blk), 0, st, ncells, density_dev, mass_dev ? effMass : 1.0,
+                       opt->unit_volume ? binVolume : 1.0, opt->scaling_factor);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  } else if (opt->mode == 0 || h <= 0.0) {
     // histogram only: the raw density counts / binVolume
     hipLaunchKernelGGL(kde_scale, dim3((unsigned)((ncells + blk - 1) / blk)),
                        dim3(blk), 0, st, ncells, hist_dev, density_dev,

@@ -11,7 +11,8 @@ CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(os.path.dirname(PKG), "librwpt.so")
 
-SOURCES = ["rwpt_api.hip", "rwpt_cells.hip", "rwpt_walk.hip", "rwpt_kde.hip"]
+SOURCES = ["rwpt_api.hip", "rwpt_cells.hip", "rwpt_walk.hip", "rwpt_kde.hip",
+           "rwpt_gpkde.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

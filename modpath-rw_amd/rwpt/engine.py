@@ -46,7 +46,9 @@ class RwptKdeOptions(C.Structure):
     _fields_ = [("origin", C.c_double * 3), ("bin_size", C.c_double * 3),
                 ("nbins", C.c_int32 * 3), ("mode", C.c_int32),
                 ("weighted", C.c_int32), ("scaling_factor", C.c_double),
-                ("unit_volume", C.c_int32)]
+                ("unit_volume", C.c_int32), ("n_opt_loops", C.c_int32),
+                ("isotropic", C.c_int32), ("skip_error_convergence", C.c_int32),
+                ("rel_error_convergence", C.c_double)]
 
 
 EXPORTS = {
@@ -73,7 +75,7 @@ EXPORTS = {
     "rwpt_get_substeps": ([C.c_void_p, _i64p], C.c_int),
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
-    "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp], C.c_int),
+    "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp, _i32p], C.c_int),
     "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
     "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
     "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
@@ -262,7 +264,8 @@ class Engine:
         self._check(lib().rwpt_reset_kernel_time(self.h), "rwpt_reset_kernel_time")
 
     def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0,
-            unit_volume=False):
+            unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
+            rel_error=0.02):
         o = RwptKdeOptions()
         for d in range(3):
             o.origin[d] = origin[d]
@@ -272,6 +275,10 @@ class Engine:
         o.weighted = int(mass is not None)
         o.scaling_factor = scaling_factor
         o.unit_volume = int(bool(unit_volume))
+        o.n_opt_loops = n_opt_loops
+        o.isotropic = int(bool(isotropic))
+        o.skip_error_convergence = int(bool(skip_error))
+        o.rel_error_convergence = rel_error
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)
@@ -282,6 +289,8 @@ class Engine:
 
     def kde_info(self):
         v = [C.c_double(0) for _ in range(4)]
-        self._check(lib().rwpt_kde_info(self.h, *[C.byref(x) for x in v]), "rwpt_kde_info")
+        loops = C.c_int32(0)
+        self._check(lib().rwpt_kde_info(self.h, *[C.byref(x) for x in v], C.byref(loops)),
+                    "rwpt_kde_info")
         return {"bandwidth": v[0].value, "points": v[1].value, "npoints": v[2].value,
-                "eff_mass": v[3].value}
+                "eff_mass": v[3].value, "loops": loops.value}

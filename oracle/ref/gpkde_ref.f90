@@ -8,7 +8,8 @@
 ! usage: gpkde_ref <input> <output>
 !   input  (stream): int32 hdr(12) = n, nbins(3), nOptLoops, database,
 !          isotropic, weighted, unitVolume, initialSmoothingSelection,
-!          scaleHistogram, exportHistogramRaw;
+!          exportOptimizationVariables (files gpkde.out<loop> in the working
+!          directory), skipErrorConvergence;
 !          real64 origin(3), binSize(3), scalingFactor, smoothingFactor;
 !          real64 x(n), y(n), z(n), w(n)
 !   output (stream): real64 density(nx*ny*nz), counts(nx*ny*nz),
@@ -44,11 +45,13 @@ program gpkde_ref
     call kde%ComputeDensity(pts, unitVolume=hdr(9) .ne. 0,                    &
       weightedHistogram=.true., weights=w, scalingFactor=sf,                   &
       histogramScalingFactor=sf, isotropic=hdr(7) .ne. 0,                      &
-      skipErrorConvergence=.false., relativeErrorConvergence=0.02d0)
+      exportOptimizationVariables=hdr(11) .ne. 0,                              &
+      skipErrorConvergence=hdr(12) .ne. 0, relativeErrorConvergence=0.02d0)
   else
     call kde%ComputeDensity(pts, unitVolume=hdr(9) .ne. 0, scalingFactor=sf,  &
       histogramScalingFactor=sf, isotropic=hdr(7) .ne. 0,                      &
-      skipErrorConvergence=.false., relativeErrorConvergence=0.02d0)
+      exportOptimizationVariables=hdr(11) .ne. 0,                              &
+      skipErrorConvergence=hdr(12) .ne. 0, relativeErrorConvergence=0.02d0)
   end if
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(11) kde%densityEstimateGrid, kde%histogram%counts, kde%initialSmoothing

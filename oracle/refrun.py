@@ -88,15 +88,24 @@ def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
 GPKDE_BINARY = os.path.join(HERE, "_ref", "gpkde_ref")
 
 
+def gpkde_available() -> bool:
+    return os.access(GPKDE_BINARY, os.X_OK)
+
+
 def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
           database=False, isotropic=False, unit_volume=True, scaling_factor=1.0,
-          smoothing_selection=0, smoothing_factor=1.0, timeout=600):
+          smoothing_selection=0, smoothing_factor=1.0, skip_error=False,
+          export_vars=False, timeout=600):
     """Runs the reference's GridProjectedKDE on the points; returns
-    (density, histogram counts, initial smoothing) with x fastest."""
+    (density, histogram counts, initial smoothing) with x fastest, and with
+    export_vars also the per-loop optimisation variables (a list of arrays,
+    one row per active bin: ix iy iz, density, smoothing(3), shape(3),
+    scale, sigma, curvature bandwidth(3), nEstimate, roughness XX YY ZZ,
+    net roughness)."""
     n = int(np.asarray(x).size)
     hdr = np.array([n, *nbins, n_opt_loops, int(database), int(isotropic),
                     int(weights is not None), int(unit_volume), smoothing_selection,
-                    0, 0], np.int32)
+                    int(export_vars), int(skip_error)], np.int32)
     f8 = np.float64
     w = np.ones(n) if weights is None else np.ascontiguousarray(weights, f8)
     parts = [hdr, np.asarray(origin, f8), np.asarray(bin_size, f8),
@@ -112,4 +121,13 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
         subprocess.run([GPKDE_BINARY, fin, fout], check=True, timeout=timeout,
                        stdout=subprocess.DEVNULL, cwd=d)
         raw = np.fromfile(fout, f8)
-    return raw[:nb].copy(), raw[nb:2 * nb].copy(), raw[2 * nb:2 * nb + 3].copy()
+        loops = []
+        if export_vars:
+            # the reference names the file of loop m gpkde.out<m>: loop 0
+            # only without optimisation loops, loops 1.. otherwise
+            for k in range(0, n_opt_loops + 1):
+                f = os.path.join(d, f"gpkde.out{k}")
+                if os.path.exists(f):
+                    loops.append(np.loadtxt(f, ndmin=2))
+    out = (raw[:nb].copy(), raw[nb:2 * nb].copy(), raw[2 * nb:2 * nb + 3].copy())
+    return out + (loops,) if export_vars else out
