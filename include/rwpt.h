@@ -97,6 +97,24 @@ typedef struct {
   int32_t isotropic;     /* gpkdeIsotropicKernels                        */
   int32_t skip_error_convergence; /* gpkdeSkipError: run all loops        */
   double rel_error_convergence;   /* gpkdeRelErrorConvergence (<= 0: 0.02) */
+  int32_t initial_smoothing;      /* gpkdeInitialSmoothingFormat
+                                     (GridProjectedKDE.F90:631-650, 4109-4117):
+                                     0 Silverman, times the factor when it
+                                     is > 0 and != 1; 1 factor * bin
+                                     distance (bin volume ^ (1/dims with a
+                                     bin size))                          */
+  double initial_smoothing_factor; /* gpkdeBinSizeFactor (0 acts as 1)   */
+  int32_t bound_kernel_size;      /* gpkdeBoundKernelSize (:788-858): 0 by
+                                     the domain, 1 by min/max h over bin
+                                     size, 2 unbounded                   */
+  double min_h_over_delta;        /* gpkdeKDBParams(1), format 1          */
+  double max_h_over_delta;        /* gpkdeKDBParams(3), format 1          */
+  int32_t effective_weight_format; /* gpkdeEffectiveWeightFormat
+                                     (Histogram.F90:276-490): 0 Kish
+                                     effective mass, 1 average mass, 2
+                                     point counts drive the loop and the
+                                     masses the last pass, 3 as 2 with the
+                                     per-bin Kish counts                 */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,
@@ -177,11 +195,14 @@ int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz);
 
 /* Concentration reconstruction over the active particles
  * (GridProjectedKDE%ComputeDensity, GridProjectedKDE.F90:3598-4392, with
- * nOptimizationLoops = 0 and the brute-force kernel: one pass with the
- * Silverman bandwidth).  density and histogram are nbins[0]*nbins[1]*
- * nbins[2] doubles, x fastest (either may be NULL); mass (one per particle)
- * may be NULL for unit masses.  The histogram holds the per-bin sums of the
- * masses; mode 0 returns it divided by the bin volume as the density. */
+ * brute-force kernels: one pass with the initial bandwidth when
+ * n_opt_loops = 0, the bandwidth optimisation otherwise).  density and
+ * histogram are nbins[0]*nbins[1]*nbins[2] doubles, x fastest (either may
+ * be NULL); mass (one per particle) may be NULL for unit masses.  The
+ * histogram holds the per-bin sums of the masses; mode 0 returns it
+ * divided by the bin volume as the density.  A dimension of one bin may
+ * have bin size 0 (compressed: no check of the coordinate); with a bin
+ * size it drops the points outside that bin, as the reference does. */
 int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
              double *density, double *histogram);
 

@@ -522,10 +522,25 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
   if (opt->mode != 0 && opt->mode != 1)
     return fail(e, RWPT_E_ARG, "kde mode must be 0 or 1");
   if (opt->n_opt_loops < 0) return fail(e, RWPT_E_ARG, "n_opt_loops < 0");
+  if (opt->initial_smoothing != 0 && opt->initial_smoothing != 1)
+    return fail(e, RWPT_E_ARG, "initial_smoothing must be 0 or 1");
+  if (opt->initial_smoothing == 1 && !(opt->initial_smoothing_factor > 0.0))
+    return fail(e, RWPT_E_ARG, "initial_smoothing 1 needs a factor > 0");
+  if (opt->bound_kernel_size < 0 || opt->bound_kernel_size > 2)
+    return fail(e, RWPT_E_ARG, "bound_kernel_size must be 0, 1 or 2");
+  // Initialize stops on these (GridProjectedKDE.F90:693-718)
+  if (opt->bound_kernel_size == 1 &&
+      !(opt->min_h_over_delta > 0.0 && opt->max_h_over_delta > opt->min_h_over_delta))
+    return fail(e, RWPT_E_ARG, "bound_kernel_size 1 needs 0 < min_h_over_delta < "
+                               "max_h_over_delta");
+  if (opt->effective_weight_format < 0 || opt->effective_weight_format > 3)
+    return fail(e, RWPT_E_ARG, "effective_weight_format must be 0..3");
   int64_t ncells = 1;
   for (int d = 0; d < 3; ++d) {
     if (opt->nbins[d] < 1) return fail(e, RWPT_E_ARG, "nbins must be >= 1");
-    if (!(opt->bin_size[d] > 0.0)) return fail(e, RWPT_E_ARG, "bin_size must be > 0");
+    // a zero bin size compresses a single-bin dimension (Histogram.F90:121)
+    if (!(opt->bin_size[d] > 0.0) && !(opt->bin_size[d] == 0.0 && opt->nbins[d] == 1))
+      return fail(e, RWPT_E_ARG, "bin_size must be > 0 (or 0 for a single bin)");
     ncells *= opt->nbins[d];
   }
   if (ncells >= 0x7fffffffLL) return fail(e, RWPT_E_ARG, "kde grid too large");
@@ -546,7 +561,7 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
     dev_free(m);
     return fail(e, RWPT_E_UNSUPPORTED,
                 "bandwidth optimisation needs a 2-D or 3-D grid and kernels narrower "
-                "than 2500 bins");
+                "than 2048 bins");
   }
   if (err == hipSuccess && density)
     err = hipMemcpyAsync(density, dens, ncells * 8, hipMemcpyDeviceToHost, e->stream);

@@ -314,12 +314,16 @@ __global__ void gk_flag_active(int64_t ncells, const double *__restrict__ counts
 }  // namespace
 
 // Host side of one adaptive reconstruction.  counts_dev: the histogram in
-// the units the reference's loop uses (weighted: mass sums / effective mass);
-// density_dev (ncells) receives the density (before the effective-mass,
-// unit-volume and scaling factors).  Returns the loops run in *loops_run.
+// the units the reference's loop uses (weighted: mass sums / effective mass,
+// point counts or per-bin Kish counts); final_weights_dev, when given, is
+// scattered with the final kernels in place of the counts (effective weight
+// formats 2 and 3, GridProjectedKDE.F90:4712-4760); density_dev (ncells)
+// receives the density (before the effective-mass, unit-volume and scaling
+// factors).  Returns the loops run in *loops_run.
 hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
-                            const double *counts_dev, double hsilverman,
-                            double nPoints, double *density_dev, int *loops_run) {
+                            const double *counts_dev, double hinit,
+                            double nPoints, const double *final_weights_dev,
+                            double *density_dev, int *loops_run) {
   hipStream_t st = e->stream;
   GkGrid G{};
   int nd = 0;
@@ -348,15 +352,22 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   const double oneOverNDimPlusFour = 1.0 / (fnd + 4.0);
   const double minusOneOverNDimPlusSix = -1.0 / (fnd + 6.0);
   const double onePlusNDimQuarter = 1.0 + 0.25 * fnd;
-  // kernel size bounds, boundKernelSizeFormat 0 (:829-858)
+  // kernel size bounds (:788-858): format 0 by the domain, 1 by the given
+  // h/delta range, 2 none
   double maxKS[3] = {0, 0, 0}, maxKSD[3] = {0, 0, 0}, minKS[3] = {0, 0, 0},
          minKSD[3] = {0, 0, 0};
+  const bool bounded = opt->bound_kernel_size != 2;
   for (int d = 0; d < 3; ++d) {
     if (!G.active[d]) continue;
-    maxKS[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeW;
-    maxKSD[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeSD;
-    minKS[d] = 1.2 * G.bin[d] / (double)kRangeW;
-    minKSD[d] = 1.2 * G.bin[d] / (double)kRangeSD;
+    if (opt->bound_kernel_size == 1) {
+      maxKS[d] = maxKSD[d] = G.bin[d] * opt->max_h_over_delta;
+      minKS[d] = minKSD[d] = G.bin[d] * opt->min_h_over_delta;
+    } else {
+      maxKS[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeW;
+      maxKSD[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeSD;
+      minKS[d] = 1.2 * G.bin[d] / (double)kRangeW;
+      minKSD[d] = 1.2 * G.bin[d] / (double)kRangeSD;
+    }
   }
   int maxDim = -1, minDim = -1;
   for (int d = 0; d < 3; ++d) {
@@ -453,7 +464,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   };
 
   // density scatter with the current smoothing (:4623-4668, :4930-4955)
-  auto density_pass = [&]() -> hipError_t {
+  auto density_pass = [&](const double *wbins) -> hipError_t {
     for (int n = 0; n < N; ++n)
       skip[n] = (hk[3 * n] < 0.0 || hk[3 * n + 1] < 0.0 || hk[3 * n + 2] < 0.0) ? 1 : 0;
     const int cap = cap_for(hk, 3, kRangeW, false);
@@ -464,7 +475,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     hipLaunchKernelGGL(gk_fill, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
                        density_dev, 0.0);
     hipLaunchKernelGGL(gk_scatter, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                       dh, 3, dcnt, G, 0, 0, kRangeW, cap, dskip, density_dev);
+                       dh, 3, wbins, G, 0, 0, kRangeW, cap, dskip, density_dev);
     hipLaunchKernelGGL(gk_div, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
                        density_dev, binVol);
     hipLaunchKernelGGL(gk_extract, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0,
@@ -491,9 +502,9 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     return r;
   };
 
-  // initial smoothing: Silverman in the active dimensions (:4059-4076)
+  // initial smoothing in the active dimensions (:4109-4117, :4616)
   for (int n = 0; n < N; ++n)
-    for (int d = 0; d < 3; ++d) hk[3 * (size_t)n + d] = G.active[d] ? hsilverman : 0.0;
+    for (int d = 0; d < 3; ++d) hk[3 * (size_t)n + d] = G.active[d] ? hinit : 0.0;
   for (int n = 0; n < N; ++n) {
     scale[n] = smoothing_scale(n);
     sigma[n] = 2.0 * scale[n];  // defaultInitialSigmaFactor
@@ -507,7 +518,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     }
   }
   std::vector<double> scaleOld = scale;
-  if ((err = density_pass()) != hipSuccess) return cleanup(), err;
+  if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
 
   auto almise = [&]() {
     double s2 = 0.0;
@@ -533,9 +544,11 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     for (int n = 0; n < N; ++n) {
       double ks = std::pow(n1[n], 0.5) * std::pow(scale[n], onePlusNDimQuarter) /
                   std::pow(4.0 * density[n], 0.25) * supportC;
-      if (ks > maxSigmaGrowth * sigma[n]) ks = maxSigmaGrowth * sigma[n];
-      ks = std::min(ks, maxKS[maxDim]);
-      if (ks < minKS[minDim]) ks = minKS[minDim];
+      if (bounded) {
+        if (ks > maxSigmaGrowth * sigma[n]) ks = maxSigmaGrowth * sigma[n];
+        ks = std::min(ks, maxKS[maxDim]);
+        if (ks < minKS[minDim]) ks = minKS[minDim];
+      }
       sigma[n] = ks;
     }
     if ((err = nest_pass(nEst)) != hipSuccess) return cleanup(), err;
@@ -559,8 +572,10 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
             (oneOverNDimPlusFour / std::pow(shape[3 * (size_t)n + did], 4.0)) * sum,
             minusOneOverNDimPlusSix);
         double cb = alphaC * nvpb * term * scale[n];
-        if (cb > maxKSD[did]) cb = maxKSD[did];
-        if (cb < minKSD[did]) cb = minKSD[did];
+        if (bounded) {
+          if (cb > maxKSD[did]) cb = maxKSD[did];
+          if (cb < minKSD[did]) cb = minKSD[did];
+        }
         curvBW[3 * (size_t)n + did] = cb;
       }
     }
@@ -679,7 +694,8 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
         if (isotropic) continue;
         if (nd == 2) {
           if (r11v[n] == 0.0 || r22v[n] == 0.0) continue;
-          sh[id1] = std::max(std::pow(net[n] / r11v[n], 1.0 / 8.0), minKS[id1] / scale[n]);
+          sh[id1] = std::pow(net[n] / r11v[n], 1.0 / 8.0);
+          if (bounded) sh[id1] = std::max(sh[id1], minKS[id1] / scale[n]);
           sh[id2] = 1.0 / sh[id1];
           const double ns = std::sqrt(sh[id1] * sh[id2]);
           if (ns == 0.0) {
@@ -690,8 +706,12 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
           sh[id2] = sh[id2] / ns;
         } else {
           if (rXX[n] == 0.0 || rYY[n] == 0.0 || rZZ[n] == 0.0) continue;
-          sh[0] = std::max(std::pow(net[n] / rXX[n], 1.0 / 12.0), minKS[0] / scale[n]);
-          sh[1] = std::max(std::pow(net[n] / rYY[n], 1.0 / 12.0), minKS[1] / scale[n]);
+          sh[0] = std::pow(net[n] / rXX[n], 1.0 / 12.0);
+          sh[1] = std::pow(net[n] / rYY[n], 1.0 / 12.0);
+          if (bounded) {
+            sh[0] = std::max(sh[0], minKS[0] / scale[n]);
+            sh[1] = std::max(sh[1], minKS[1] / scale[n]);
+          }
           sh[2] = 1.0 / sh[0] / sh[1];
           // the reference's exponent is the single-precision literal 0.333
           const double ns = std::pow(sh[0] * sh[1] * sh[2], (double)0.333f);
@@ -711,6 +731,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
           double &hv = hk[3 * (size_t)n + d];
           hv = shape[3 * (size_t)n + d] * scale[n];
         }
+        if (!bounded) continue;
         bool anyMax = false, anyMin = false;
         for (int n = 0; n < N; ++n) anyMax |= hk[3 * (size_t)n + d] > maxKS[d];
         if (anyMax) {
@@ -729,7 +750,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
         for (int n = 0; n < N; ++n) scale[n] = smoothing_scale(n);
     }
     // new density (:4930-4960)
-    if ((err = density_pass()) != hipSuccess) return cleanup(), err;
+    if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
     // convergence (:4962-5038)
     double maxD = -INFINITY, maxDO = -INFINITY;
     for (int n = 0; n < N; ++n) {
@@ -765,6 +786,14 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     cumsumOld = cumsum;
     densityOld = density;
     scaleOld = scale;
+  }
+  if (final_weights_dev) {
+    // last reconstruction over the mass sums (:4730-4758): the counts'
+    // device copy is no longer needed and takes the masses of the bins
+    hipLaunchKernelGGL(gk_extract, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       dbins, N, final_weights_dev, dcnt);
+    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
+    if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
   }
   if (loops_run) *loops_run = loops;
   cleanup();

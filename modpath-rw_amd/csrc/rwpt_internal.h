@@ -185,8 +185,9 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
 
 hipError_t ensure_sort_storage(rwpt_engine *e);
 hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
-                            const double *counts_dev, double hsilverman,
-                            double nPoints, double *density_dev, int *loops_run);
+                            const double *counts_dev, double hinit,
+                            double nPoints, const double *final_weights_dev,
+                            double *density_dev, int *loops_run);
 // FP32-local state: float record table from the FP64 one, positions to and
 // from the double arrays
 hipError_t build_cell_tables32(rwpt_engine *e);

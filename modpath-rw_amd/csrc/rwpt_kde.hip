@@ -69,11 +69,12 @@ __global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
   gx[p] = g[0];
   gy[p] = g[1];
   gz[p] = g[2];
-  // Histogram.F90:276-350: floor((x - origin)/bin) + 1 on active dimensions
+  // Histogram.F90:276-350: floor((x - origin)/bin) + 1 in every dimension
+  // with a bin size (a single-bin dimension still drops points outside it)
   int64_t key = 0, stride = 1;
   for (int d = 0; d < 3; ++d) {
     int id = 0;
-    if (G.active[d]) {
+    if (G.bin[d] > 0.0) {
       id = (int)floor((g[d] - G.origin[d]) / G.bin[d]);
       if (id < 0 || id >= G.nb[d]) {
         keys[p] = (int32_t)ncells;
@@ -86,20 +87,31 @@ __global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
   keys[p] = (int32_t)key;
 }
 
-// Per-bin sums over the sorted runs (first element of a run sums it).
+// Per-bin sums over the sorted runs (first element of a run sums it):
+// what 0 the masses, 1 the points, 2 the squared masses.
 __global__ void kde_runs(int64_t n, const int32_t *__restrict__ keys,
                          const int32_t *__restrict__ vals,
                          const double *__restrict__ mass, int64_t ncells,
-                         double *__restrict__ counts) {
+                         int what, double *__restrict__ counts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t k = keys[i];
   if (k >= ncells) return;
   if (i > 0 && keys[i - 1] == k) return;
   double s = 0.0;
-  for (int64_t q = i; q < n && keys[q] == k; ++q)
-    s = s + (mass ? mass[vals[q]] : 1.0);
+  for (int64_t q = i; q < n && keys[q] == k; ++q) {
+    const double w = mass ? mass[vals[q]] : 1.0;
+    s = s + (what == 0 ? w : (what == 1 ? 1.0 : w * w));
+  }
   counts[k] = s;
+}
+
+// Per-bin Kish counts (sum w)^2 / sum w^2 where sum w^2 != 0
+// (Histogram.F90:476-480); sq holds the squared-mass sums on entry.
+__global__ void kde_kish_counts(int64_t n, const double *__restrict__ wsum,
+                                double *__restrict__ sq) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && sq[i] != 0.0) sq[i] = (wsum[i] * wsum[i]) / sq[i];
 }
 
 // Deterministic sums over the active particles: each block reduces a fixed
@@ -320,7 +332,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                                              S.vals[0], S.vals[1], (int)n, 0, bits, st);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(kde_runs, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
-                       st, n, S.keys[1], S.vals[1], mass_dev, ncells, hist_dev);
+                       st, n, S.keys[1], S.vals[1], mass_dev, ncells, 0, hist_dev);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   }
   (void)hipEventRecord(TH.stop, st);
@@ -350,29 +362,48 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
       return err;
     if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
   }
+  // bin volume and bin distance over the dimensions with a bin size
+  // (Histogram.F90:121, 154-155)
   double binVolume = 1.0;
+  int nbd = 0;
   for (int d = 0; d < 3; ++d)
-    if (G.bin[d] > 0.0) binVolume *= G.bin[d];
-  // effective mass (Histogram.F90:332-341, format 0) and point count: the
-  // reference divides the weighted counts by the effective mass and
-  // multiplies the density back by it; both cancel here, so the bins hold
-  // the plain sums of the masses
+    if (G.bin[d] > 0.0) {
+      binVolume *= G.bin[d];
+      ++nbd;
+    }
+  const double binDistance = std::pow(binVolume, 1.0 / (nbd > 0 ? nbd : 1));
+  // effective mass (Histogram.F90:326-343; formats 2 and 3 keep 1) and
+  // point count: the reference divides the weighted counts by the
+  // effective mass and multiplies the density back by it; both cancel in
+  // a single pass, so the bins hold the plain sums of the masses
+  const int wfmt = opt->effective_weight_format;
   double effMass = 1.0, nPoints = nact;
   if (mass_dev) {
-    const double neff = hs[3] * hs[3] / hs[4];
-    effMass = hs[3] / neff;
+    if (wfmt == 0) {
+      const double neff = hs[3] * hs[3] / hs[4];
+      effMass = hs[3] / neff;
+    } else if (wfmt == 1) {
+      effMass = hs[3] / nact;
+    }
   } else {
     // unit masses: nPoints counts the points inside the grid
     nPoints = hs[6];
   }
+  // initial smoothing: Silverman (1986) over the standard deviations
+  // (GridProjectedKDE.F90:4034-4054, 4109-4117), or a factor of the bin
+  // distance (:643-648)
   double h = 0.0;
-  if (nact > 0 && nd > 0) {
+  const double fac = opt->initial_smoothing_factor;
+  if (opt->initial_smoothing == 1) {
+    if (nact > 0 && nd > 0) h = fac * binDistance;
+  } else if (nact > 0 && nd > 0) {
     double sscale = 1.0;
     for (int d = 0; d < 3; ++d)
       if (G.active[d]) sscale *= std::sqrt(dev2[d] / nact);
     sscale = std::pow(sscale, 1.0 / nd);
     h = sscale * std::pow(4.0 / ((nd + 2.0) * nPoints), 1.0 / (nd + 4.0));
-    if (sscale == 0.0) h = 2.0 * std::pow(binVolume, 1.0 / nd);
+    if (sscale == 0.0) h = 2.0 * binDistance;
+    else if (fac > 0.0 && fac != 1.0) h = h * fac;
   }
   if (stats_host) {
     stats_host[0] = h;
@@ -384,16 +415,34 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   e->kde_loops = 0;
   if (opt->mode == 1 && opt->n_opt_loops > 0 && h > 0.0) {
     // bandwidth optimisation: the loop sees the counts the reference's
-    // weighted histogram holds (mass sums / effective mass)
-    if ((err = hipMemcpyAsync(tmpA, hist_dev, ncells * sizeof(double),
-                              hipMemcpyDeviceToDevice, st)) != hipSuccess)
-      return err;
-    if (mass_dev)
-      hipLaunchKernelGGL(kde_scale, dim3((unsigned)((ncells + blk - 1) / blk)),
-                         dim3(blk), 0, st, ncells, tmpA, tmpA, 1.0 / effMass);
+    // weighted histogram holds: mass sums / effective mass (formats 0, 1),
+    // point counts (2) or per-bin Kish counts (3); formats 2 and 3 then
+    // scatter the mass sums with the final kernels
+    const unsigned gb = (unsigned)((ncells + blk - 1) / blk);
+    const bool final_masses = mass_dev && (wfmt == 2 || wfmt == 3);
+    if (mass_dev && final_masses) {
+      if ((err = hipMemsetAsync(tmpA, 0, ncells * sizeof(double), st)) != hipSuccess)
+        return err;
+      if (n > 0)
+        hipLaunchKernelGGL(kde_runs, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk),
+                           0, st, n, S.keys[1], S.vals[1], mass_dev, ncells,
+                           wfmt == 2 ? 1 : 2, tmpA);
+      if (wfmt == 3)
+        hipLaunchKernelGGL(kde_kish_counts, dim3(gb), dim3(blk), 0, st, ncells,
+                           hist_dev, tmpA);
+    } else {
+      if ((err = hipMemcpyAsync(tmpA, hist_dev, ncells * sizeof(double),
+                                hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return err;
+      if (mass_dev)
+        hipLaunchKernelGGL(kde_scale, dim3(gb), dim3(blk), 0, st, ncells, tmpA, tmpA,
+                           1.0 / effMass);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
     int loops = 0;
-    if ((err = launch_gpkde_opt(e, opt, tmpA, h, nPoints, density_dev, &loops)) !=
-        hipSuccess)
+    if ((err = launch_gpkde_opt(e, opt, tmpA, h, nPoints,
+                                final_masses ? hist_dev : nullptr, density_dev,
+                                &loops)) != hipSuccess)
       return err;
     e->kde_loops = loops;
     hipLaunchKernelGGL(kde_finish, dim3((unsigned)((ncells + blk - 1) / blk)),

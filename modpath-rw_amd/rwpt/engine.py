@@ -48,7 +48,12 @@ class RwptKdeOptions(C.Structure):
                 ("weighted", C.c_int32), ("scaling_factor", C.c_double),
                 ("unit_volume", C.c_int32), ("n_opt_loops", C.c_int32),
                 ("isotropic", C.c_int32), ("skip_error_convergence", C.c_int32),
-                ("rel_error_convergence", C.c_double)]
+                ("rel_error_convergence", C.c_double),
+                ("initial_smoothing", C.c_int32),
+                ("initial_smoothing_factor", C.c_double),
+                ("bound_kernel_size", C.c_int32), ("min_h_over_delta", C.c_double),
+                ("max_h_over_delta", C.c_double),
+                ("effective_weight_format", C.c_int32)]
 
 
 EXPORTS = {
@@ -265,7 +270,13 @@ class Engine:
 
     def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0,
             unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
-            rel_error=0.02):
+            rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
+            bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
+            effective_weight_format=0):
+        """gpkde reconstruction of the active particles; the keywords are
+        the options of the gpkde input file (ReadGPKDEData,
+        ModpathSimulationData.f90:816-1273).  Returns (density, histogram),
+        x fastest."""
         o = RwptKdeOptions()
         for d in range(3):
             o.origin[d] = origin[d]
@@ -279,6 +290,12 @@ class Engine:
         o.isotropic = int(bool(isotropic))
         o.skip_error_convergence = int(bool(skip_error))
         o.rel_error_convergence = rel_error
+        o.initial_smoothing = initial_smoothing
+        o.initial_smoothing_factor = smoothing_factor
+        o.bound_kernel_size = bound_kernel_size
+        o.min_h_over_delta = min_h_over_delta
+        o.max_h_over_delta = max_h_over_delta
+        o.effective_weight_format = effective_weight_format
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)

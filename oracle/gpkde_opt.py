@@ -4,9 +4,10 @@ Only tests/, the smoke check and bench.py's cpu_baseline leg import this.
 It restates GridProjectedKDE%ComputeDensityOptimization
 (src/lib/gpkde/GridProjectedKDE.F90:4396-5160) with brute-force kernels
 (prSetKernelBrute, prSetKernelSigmaBrute, prSetKernelSD2DBrute,
-prSetKernelSDBrute, :6222-6463), kernel size bound format 0 and the default
-effective-weight format 0, one active bin at a time in Python loops: for
-grids of a few thousand active bins only.
+prSetKernelSDBrute, :6222-6463) and the three kernel size bound formats,
+one active bin at a time in Python loops: for grids of a few thousand
+active bins only.  The effective-weight formats enter through the counts
+and final weights the caller passes.
 
 Every stage is pinned against the per-loop variables that the compiled
 reference exports (exportOptimizationVariables, :5049-5056), see
@@ -139,12 +140,18 @@ def kernel(G, c, h, kind, der=0, rng=RANGE_W):
 
 
 def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
-             isotropic=False, skip_error=False, tol=0.02, export=False):
+             isotropic=False, skip_error=False, tol=0.02, export=False,
+             bound=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
+             final_weights=None):
     """Adaptive density of a histogram.
 
     counts: histogram (x fastest) in the units the loop uses (weighted:
     mass sums / effective mass); h_init: initial (Silverman) smoothing;
-    n_points: histogram%nPoints.  Returns (density with x fastest, before the
+    n_points: histogram%nPoints; bound: boundKernelSizeFormat (0 domain,
+    1 the h/delta range given, 2 none); final_weights: histogram (x fastest)
+    scattered with the final kernels in place of the counts (effective
+    weight formats 2 and 3, GridProjectedKDE.F90:4712-4760).
+    Returns (density with x fastest, before the
     effective-mass, unit-volume and scaling factors, loops run, and with
     export the per-loop tables of the reference's export: rows of ix iy iz,
     density, smoothing(3), shape(3), scale, sigma, curvature bandwidth(3),
@@ -162,7 +169,6 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
     N = len(ids)
     cnt = grid_counts[ids[:, 0] - 1, ids[:, 1] - 1, ids[:, 2] - 1]
     binVol = G.bin_volume
-    raw_density = cnt / binVol
     dims = [d for d in range(3) if G.active[d]]
     # prInitializeModuleConstants, GridProjectedKDE.F90:1582-1611
     supportC = ((fnd + 2.0) * (8.0 * pi) ** (0.5 * fnd)) ** 0.25
@@ -170,13 +176,18 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
               ** (1.0 / (fnd + 6.0)) * (fnd + 2.0) ** (1.0 / (fnd + 4.0))
               / (fnd + 4.0) ** (1.0 / (fnd + 6.0)))
     betaC = 2.0 / (fnd + 4.0) / (fnd + 6.0)
-    # kernel size bounds, boundKernelSizeFormat 0 (:829-858)
+    # kernel size bounds (:788-858)
     maxKS = np.zeros(3); maxKSD = np.zeros(3); minKS = np.zeros(3); minKSD = np.zeros(3)
+    bounded = bound != 2
     for d in dims:
-        maxKS[d] = G.bin[d] * (0.9 * nb[d] - 1) / RANGE_W
-        maxKSD[d] = G.bin[d] * (0.9 * nb[d] - 1) / RANGE_SD
-        minKS[d] = 1.2 * G.bin[d] / RANGE_W
-        minKSD[d] = 1.2 * G.bin[d] / RANGE_SD
+        if bound == 1:
+            maxKS[d] = maxKSD[d] = G.bin[d] * max_h_over_delta
+            minKS[d] = minKSD[d] = G.bin[d] * min_h_over_delta
+        else:
+            maxKS[d] = G.bin[d] * (0.9 * nb[d] - 1) / RANGE_W
+            maxKSD[d] = G.bin[d] * (0.9 * nb[d] - 1) / RANGE_SD
+            minKS[d] = 1.2 * G.bin[d] / RANGE_W
+            minKSD[d] = 1.2 * G.bin[d] / RANGE_SD
     maxKSv = min(maxKS[d] for d in dims)
     minKSv = max(minKS[d] for d in dims)
 
@@ -242,9 +253,10 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
         # (:4816-4858)
         n1 = gather(grid, sigma)
         ks = n1 ** 0.5 * scale ** (1.0 + 0.25 * fnd) / (4.0 * density) ** 0.25 * supportC
-        ks = np.where(ks > MAX_SIGMA_GROWTH * sigma, MAX_SIGMA_GROWTH * sigma, ks)
-        ks = np.minimum(ks, maxKSv)
-        ks = np.where(ks < minKSv, minKSv, ks)
+        if bounded:
+            ks = np.where(ks > MAX_SIGMA_GROWTH * sigma, MAX_SIGMA_GROWTH * sigma, ks)
+            ks = np.minimum(ks, maxKSv)
+            ks = np.where(ks < minKSv, minKSv, ks)
         sigma = ks
         nEst = gather(grid, sigma)
         # curvature bandwidths (prComputeCurvatureBandwidth, :5300-5400)
@@ -259,8 +271,9 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
             with np.errstate(divide="ignore", invalid="ignore"):
                 term = ((1.0 / (fnd + 4.0)) / shape[:, did] ** 4 * s) ** (-1.0 / (fnd + 6.0))
             cb = alphaC * nvpb * term * scale
-            cb = np.where(cb > maxKSD[did], maxKSD[did], cb)
-            cb = np.where(cb < minKSD[did], minKSD[did], cb)
+            if bounded:
+                cb = np.where(cb > maxKSD[did], maxKSD[did], cb)
+                cb = np.where(cb < minKSD[did], minKSD[did], cb)
             curv[:, did] = np.where(ok, cb, 0.0)
         # net roughness (prComputeNetRoughness2D :1905-2289, 3D :2295-2709)
         rough = np.zeros((N, 3))
@@ -336,7 +349,9 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
                     if r11[n] == 0 or r22[n] == 0:
                         continue
                     a, b = dims
-                    shape[n, a] = max((net[n] / r11[n]) ** 0.125, minKS[a] / scale[n])
+                    shape[n, a] = (net[n] / r11[n]) ** 0.125
+                    if bounded:
+                        shape[n, a] = max(shape[n, a], minKS[a] / scale[n])
                     shape[n, b] = 1.0 / shape[n, a]
                     ns = sqrt(shape[n, a] * shape[n, b])
                     if ns == 0:
@@ -347,8 +362,11 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
                 else:
                     if (rough[n] == 0).any():
                         continue
-                    shape[n, 0] = max((net[n] / rough[n, 0]) ** (1.0 / 12.0), minKS[0] / scale[n])
-                    shape[n, 1] = max((net[n] / rough[n, 1]) ** (1.0 / 12.0), minKS[1] / scale[n])
+                    shape[n, 0] = (net[n] / rough[n, 0]) ** (1.0 / 12.0)
+                    shape[n, 1] = (net[n] / rough[n, 1]) ** (1.0 / 12.0)
+                    if bounded:
+                        shape[n, 0] = max(shape[n, 0], minKS[0] / scale[n])
+                        shape[n, 1] = max(shape[n, 1], minKS[1] / scale[n])
                     shape[n, 2] = 1.0 / shape[n, 0] / shape[n, 1]
                     # single-precision literal 0.333 in the reference (:5640)
                     ns = (shape[n, 0] * shape[n, 1] * shape[n, 2]) ** float(np.float32(0.333))
@@ -360,6 +378,8 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
         update = False
         for d in dims:
             hk[:, d] = shape[:, d] * scale
+            if not bounded:
+                continue
             if (hk[:, d] > maxKS[d]).any():
                 update = True
                 hk[:, d] = np.minimum(hk[:, d], maxKS[d])
@@ -392,5 +412,9 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
         scale_old = scale.copy()
         if export:
             tables.append(table())
+    if final_weights is not None:
+        fw = np.asarray(final_weights, float).reshape(nb[::-1]).transpose(2, 1, 0)
+        cnt = fw[ids[:, 0] - 1, ids[:, 1] - 1, ids[:, 2] - 1]
+        grid, density = density_pass(hk)
     out = grid.transpose(2, 1, 0).ravel().copy()
     return (out, loops, tables) if export else (out, loops)

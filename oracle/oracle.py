@@ -64,6 +64,9 @@ def lib():
         L.orc_kde_density.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
                                       _dp, _dp, _dp]
         L.orc_kde_density.restype = C.c_int
+        L.orc_kde_density_sel.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _ip, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_kde_density_sel.restype = C.c_int
         L.orc_newton_axis.argtypes = [C.c_double] * 8
         L.orc_newton_axis.restype = C.c_double
         L.orc_quadratic_axis.argtypes = [C.c_double] * 7 + [_ip]
@@ -206,16 +209,18 @@ def new_state(cell, lx, ly, lz, t0=0.0, rng=None):
     }
 
 
-def kde_density(x, y, z, origin, binsize, nbins, w=None):
-    """gpkde density (nOptimizationLoops = 0, brute-force kernel)."""
+def kde_density(x, y, z, origin, binsize, nbins, w=None, selection=0, factor=1.0):
+    """gpkde density (nOptimizationLoops = 0, brute-force kernel); selection
+    and factor choose the initial smoothing."""
     n = x.size
     nb = int(np.prod(nbins))
     dens = np.zeros(nb)
     counts = np.zeros(nb)
     h = np.zeros(3)
     L = lib()
-    L.orc_kde_density(n, _ptr(_d(x), _dp), _ptr(_d(y), _dp), _ptr(_d(z), _dp),
-                      _ptr(_d(w), _dp), _ptr(_d(origin), _dp), _ptr(_d(binsize), _dp),
-                      _ptr(_i(nbins), _ip), _ptr(dens, _dp), _ptr(counts, _dp),
-                      _ptr(h, _dp))
+    L.orc_kde_density_sel(n, _ptr(_d(x), _dp), _ptr(_d(y), _dp), _ptr(_d(z), _dp),
+                          _ptr(_d(w), _dp), _ptr(_d(origin), _dp),
+                          _ptr(_d(binsize), _dp), _ptr(_i(nbins), _ip),
+                          C.c_int(selection), C.c_double(factor), _ptr(dens, _dp),
+                          _ptr(counts, _dp), _ptr(h, _dp))
     return dens, counts, h

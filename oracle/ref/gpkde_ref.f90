@@ -6,11 +6,14 @@
 ! the density grid and the histogram counts.
 !
 ! usage: gpkde_ref <input> <output>
-!   input  (stream): int32 hdr(12) = n, nbins(3), nOptLoops, database,
+!   input  (stream): int32 hdr(14) = n, nbins(3), nOptLoops, database,
 !          isotropic, weighted, unitVolume, initialSmoothingSelection,
 !          exportOptimizationVariables (files gpkde.out<loop> in the working
-!          directory), skipErrorConvergence;
-!          real64 origin(3), binSize(3), scalingFactor, smoothingFactor;
+!          directory), skipErrorConvergence, boundKernelSizeFormat,
+!          effectiveWeightFormat;
+!          real64 origin(3), binSize(3), scalingFactor, smoothingFactor,
+!          minHOverDelta, deltaHOverDelta, maxHOverDelta (passed when
+!          minHOverDelta > 0);
 !          real64 x(n), y(n), z(n), w(n)
 !   output (stream): real64 density(nx*ny*nz), counts(nx*ny*nz),
 !          initialSmoothing(3)
@@ -18,8 +21,8 @@ program gpkde_ref
   use iso_fortran_env, only: int32, real64
   use GridProjectedKDEModule, only: GridProjectedKDEType
   implicit none
-  integer(int32) :: hdr(12)
-  real(real64) :: origin(3), binSize(3), domainSize(3), sf, smf
+  integer(int32) :: hdr(14)
+  real(real64) :: origin(3), binSize(3), domainSize(3), sf, smf, hod(3)
   real(real64), allocatable :: pts(:, :), w(:)
   character(len=1024) :: fin, fout
   type(GridProjectedKDEType), allocatable :: kde
@@ -28,7 +31,7 @@ program gpkde_ref
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(10) hdr, origin, binSize, sf, smf
+  read(10) hdr, origin, binSize, sf, smf, hod
   n = hdr(1)
   nb = hdr(2:4)
   allocate(pts(n, 3), w(n))
@@ -37,10 +40,22 @@ program gpkde_ref
   domainSize = nb * binSize
 
   allocate(kde)
-  call kde%Initialize(domainSize, binSize, domainOrigin=origin,               &
-    adaptGridToCoords=.false., nOptimizationLoops=int(hdr(5)),                 &
-    databaseOptimization=hdr(6) .ne. 0, initialSmoothingSelection=int(hdr(10)),&
-    initialSmoothingFactor=smf, outFileName='/dev/null')
+  if (hod(1) > 0d0) then
+    call kde%Initialize(domainSize, binSize, domainOrigin=origin,             &
+      adaptGridToCoords=.false., nOptimizationLoops=int(hdr(5)),               &
+      databaseOptimization=hdr(6) .ne. 0,                                      &
+      minHOverDelta=hod(1), deltaHOverDelta=hod(2), maxHOverDelta=hod(3),      &
+      initialSmoothingSelection=int(hdr(10)), initialSmoothingFactor=smf,      &
+      effectiveWeightFormat=int(hdr(14)),                                      &
+      boundKernelSizeFormat=int(hdr(13)), outFileName='/dev/null')
+  else
+    call kde%Initialize(domainSize, binSize, domainOrigin=origin,             &
+      adaptGridToCoords=.false., nOptimizationLoops=int(hdr(5)),               &
+      databaseOptimization=hdr(6) .ne. 0,                                      &
+      initialSmoothingSelection=int(hdr(10)), initialSmoothingFactor=smf,      &
+      effectiveWeightFormat=int(hdr(14)),                                      &
+      boundKernelSizeFormat=int(hdr(13)), outFileName='/dev/null')
+  end if
   if (hdr(8) .ne. 0) then
     call kde%ComputeDensity(pts, unitVolume=hdr(9) .ne. 0,                    &
       weightedHistogram=.true., weights=w, scalingFactor=sf,                   &

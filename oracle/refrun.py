@@ -95,21 +95,25 @@ def gpkde_available() -> bool:
 def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
           database=False, isotropic=False, unit_volume=True, scaling_factor=1.0,
           smoothing_selection=0, smoothing_factor=1.0, skip_error=False,
-          export_vars=False, timeout=600):
+          export_vars=False, bound=0, h_over_delta=(0.0, 0.0, 0.0),
+          weight_format=0, timeout=600):
     """Runs the reference's GridProjectedKDE on the points; returns
     (density, histogram counts, initial smoothing) with x fastest, and with
     export_vars also the per-loop optimisation variables (a list of arrays,
     one row per active bin: ix iy iz, density, smoothing(3), shape(3),
     scale, sigma, curvature bandwidth(3), nEstimate, roughness XX YY ZZ,
-    net roughness)."""
+    net roughness).  bound: boundKernelSizeFormat; h_over_delta: (min,
+    delta, max) h/delta passed to Initialize when min > 0; weight_format:
+    effectiveWeightFormat."""
     n = int(np.asarray(x).size)
     hdr = np.array([n, *nbins, n_opt_loops, int(database), int(isotropic),
                     int(weights is not None), int(unit_volume), smoothing_selection,
-                    int(export_vars), int(skip_error)], np.int32)
+                    int(export_vars), int(skip_error), bound, weight_format],
+                   np.int32)
     f8 = np.float64
     w = np.ones(n) if weights is None else np.ascontiguousarray(weights, f8)
     parts = [hdr, np.asarray(origin, f8), np.asarray(bin_size, f8),
-             np.array([scaling_factor, smoothing_factor], f8),
+             np.array([scaling_factor, smoothing_factor, *h_over_delta], f8),
              np.ascontiguousarray(x, f8), np.ascontiguousarray(y, f8),
              np.ascontiguousarray(z, f8), w]
     nb = int(np.prod(nbins))

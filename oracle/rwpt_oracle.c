@@ -1615,7 +1615,9 @@ void orc_histogram(long n, const double *x, const double *y, const double *z,
   for (long p = 0; p < n; ++p) {
     int gi[3] = {1, 1, 1}, inside = 1;
     for (int d = 0; d < 3; ++d) {
-      if (nbins[d] <= 1) continue; /* dimension not active */
+      /* the histogram checks every dimension with a bin size, a single
+         bin included (Histogram.F90:121, 306-317) */
+      if (!(binsize[d] > 0.0)) continue;
       gi[d] = (int)floor((pts[d][p] - origin[d]) / binsize[d]) + 1;
       if (gi[d] > nbins[d] || gi[d] <= 0) {
         inside = 0;
@@ -1753,6 +1755,15 @@ int orc_kde_density(long n, const double *x, const double *y,
                     const double *z, const double *w, const double origin[3],
                     const double binsize[3], const int nbins[3],
                     double *density, double *counts, double hout[3]) {
+  return orc_kde_density_sel(n, x, y, z, w, origin, binsize, nbins, 0, 1.0,
+                             density, counts, hout);
+}
+
+int orc_kde_density_sel(long n, const double *x, const double *y,
+                        const double *z, const double *w,
+                        const double origin[3], const double binsize[3],
+                        const int nbins[3], int selection, double factor,
+                        double *density, double *counts, double hout[3]) {
   const double *pts[3] = {x, y, z};
   int dmask[3], ndim = 0;
   for (int d = 0; d < 3; ++d) {
@@ -1763,8 +1774,14 @@ int orc_kde_density(long n, const double *x, const double *y,
   /* histogram (prComputeCounts / prComputeCountsWeighted, format 0) */
   orc_histogram(n, x, y, z, w, origin, binsize, nbins, counts);
   double binVolume = 1.0;
+  int nbd = 0;
   for (int d = 0; d < 3; ++d)
-    if (binsize[d] > 0.0) binVolume = binVolume * binsize[d];
+    if (binsize[d] > 0.0) {
+      binVolume = binVolume * binsize[d];
+      ++nbd;
+    }
+  /* binDistance over the dimensions with a bin size (Histogram.F90:155) */
+  const double binDistance = pow(binVolume, 1.0 / (double)(nbd > 0 ? nbd : 1));
   double effMass = 1.0;
   long nPoints;
   if (w) {
@@ -1802,10 +1819,11 @@ int orc_kde_density(long n, const double *x, const double *y,
   sscale = pow(sscale, 1.0 / (double)ndim);
   double hs = sscale * pow(4.0 / ((ndim + 2.0) * (double)nPoints),
                            1.0 / (ndim + 4.0));
-  if (sscale == 0.0) {
-    double bd = pow(binVolume, 1.0 / (double)ndim);
-    hs = 2.0 * bd;
-  }
+  if (sscale == 0.0)
+    hs = 2.0 * binDistance;
+  else if (factor > 0.0 && factor != 1.0)
+    hs = hs * factor;
+  if (selection == 1) hs = factor * binDistance;
   double h[3];
   for (int d = 0; d < 3; ++d) h[d] = dmask[d] ? hs : 0.0;
   if (hout)

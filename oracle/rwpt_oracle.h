@@ -98,6 +98,14 @@ int orc_kde_density(long n, const double *x, const double *y,
                     const double *z, const double *w, const double origin[3],
                     const double binsize[3], const int nbins[3],
                     double *density, double *counts, double hout[3]);
+/* as orc_kde_density with the initial smoothing selection (0 Silverman
+ * times the factor when it is > 0 and != 1, 1 factor * bin distance,
+ * GridProjectedKDE.F90:631-650, 4109-4117) */
+int orc_kde_density_sel(long n, const double *x, const double *y,
+                        const double *z, const double *w,
+                        const double origin[3], const double binsize[3],
+                        const int nbins[3], int selection, double factor,
+                        double *density, double *counts, double hout[3]);
 
 /* exit-time solvers for one axis (Newton: TrackSubCell.F90:2451-2554;
  * quadratic: one axis of :1576-1892, exit faces 1/2) */

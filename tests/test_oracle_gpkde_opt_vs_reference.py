@@ -93,3 +93,53 @@ def test_weighted():
     dens, loops = gpkde_opt.optimize(counts, nb, (1, 1, 1), h, float(w.size), 3,
                                      skip_error=True)
     np.testing.assert_allclose(dens * eff, dref, rtol=0, atol=1e-12 * dref.max())
+
+
+@pytest.mark.parametrize("fmt", [1, 2, 3])
+def test_effective_weight_formats(fmt):
+    """Effective weight formats (Histogram.F90:326-343, 352-490): 1 the
+    average mass; 2 point counts drive the loop and the mass sums the last
+    scatter; 3 per-bin Kish counts (sum w)^2 / sum w^2 with the same last
+    scatter (GridProjectedKDE.F90:4712-4760)."""
+    nb = (60, 40, 1)
+    pts, w = cloud(nb, n=20000, seed=6, weights=True)
+    dref, _, h = refrun.gpkde(*pts, (0, 0, 0), (1, 1, 1), nb, weights=w,
+                              n_opt_loops=3, unit_volume=False, skip_error=True,
+                              weight_format=fmt)
+    wsum = histogram(pts, nb, w)
+    final, scale = None, 1.0
+    if fmt == 1:
+        scale = w.sum() / w.size
+        counts = wsum / scale
+    elif fmt == 2:
+        counts, final = histogram(pts, nb), wsum
+    else:
+        sq = histogram(pts, nb, w * w)
+        counts = np.where(sq != 0, wsum ** 2 / np.where(sq != 0, sq, 1.0), 0.0)
+        final = wsum
+    dens, _ = gpkde_opt.optimize(counts, nb, (1, 1, 1), h, float(w.size), 3,
+                                 skip_error=True, final_weights=final)
+    np.testing.assert_allclose(dens * scale, dref, rtol=0, atol=1e-12 * dref.max())
+
+
+@pytest.mark.parametrize("bound,hod", [(1, (0.5, 0.0, 6.0)), (1, (0.9, 0.0, 3.0)),
+                                       (2, (0.0, 0.0, 0.0))])
+@pytest.mark.parametrize("nb", [(60, 40, 1), (24, 20, 16)])
+def test_kernel_size_bounds(bound, hod, nb):
+    """boundKernelSizeFormat 1 bounds the kernels by the given h/delta
+    range, 2 leaves them unbounded (GridProjectedKDE.F90:788-858, 4863,
+    5387, 5551, 5702).  A lower bound with 3 h/delta an integer (h/delta =
+    1) is avoided: kernels clamped there sit on the jump of the half-width
+    ceiling(3 h/delta), where a last-bit difference of h, from another
+    summation order, changes the footprint."""
+    pts, _ = cloud(nb, n=8000 if nb[2] > 1 else 20000, seed=8)
+    dref, _, h, tabs = refrun.gpkde(*pts, (0, 0, 0), (1, 1, 1), nb, n_opt_loops=2,
+                                    unit_volume=False, skip_error=True, bound=bound,
+                                    h_over_delta=hod, export_vars=True)
+    counts = histogram(pts, nb)
+    dens, _, mine = gpkde_opt.optimize(counts, nb, (1, 1, 1), h, counts.sum(), 2,
+                                       skip_error=True, export=True, bound=bound,
+                                       min_h_over_delta=hod[0], max_h_over_delta=hod[2])
+    for a, b in zip(mine, tabs):
+        compare_tables(a, b, 5e-8)
+    np.testing.assert_allclose(dens, dref, rtol=0, atol=1e-12 * dref.max())

@@ -57,3 +57,50 @@ def test_kde_single_point_cloud_matches_reference():
     np.testing.assert_array_equal(h_ref, h_or)
     np.testing.assert_array_equal(c_ref, c_or)
     np.testing.assert_allclose(d_ref, d_or, rtol=0, atol=1e-13 * d_or.max())
+
+
+def test_kde_single_point_cloud_anisotropic_bins():
+    """The fallback uses the bin distance, the bin volume to the power one
+    over the number of dimensions with a bin size (Histogram.F90:154-155),
+    three here although the reconstruction is two-dimensional.  The
+    coordinates are exact binary fractions, so the deviations are zero."""
+    x, y, z = np.full(64, 10.25), np.full(64, 7.625), np.full(64, 0.5)
+    bs, nb = (2.0, 0.5, 1.5), (20, 40, 1)
+    d_ref, c_ref, h_ref = refrun.gpkde(x, y, z, (0, 0, 0), bs, nb, unit_volume=False)
+    d_or, c_or, h_or = O.kde_density(x, y, z, (0, 0, 0), bs, nb)
+    np.testing.assert_allclose(h_or[:2], 2.0 * 1.5 ** (1.0 / 3.0), rtol=1e-15)
+    np.testing.assert_array_equal(h_ref, h_or)
+    np.testing.assert_array_equal(c_ref, c_or)
+    np.testing.assert_allclose(d_ref, d_or, rtol=0, atol=1e-13 * d_or.max())
+
+
+@pytest.mark.parametrize("zbin", [1.0, 0.0])
+def test_kde_single_bin_dimension(zbin):
+    """A dimension of one bin with a bin size drops the points outside it
+    (Histogram.F90:121, 306-317); a zero bin size compresses it."""
+    g = np.random.default_rng(4)
+    n = 4000
+    x, y, z = g.normal(20, 4, n), g.normal(15, 3, n), g.uniform(-0.6, 1.6, n)
+    nb, bs = (40, 30, 1), (1.0, 1.0, zbin)
+    d_ref, c_ref, h_ref = refrun.gpkde(x, y, z, (0, 0, 0), bs, nb, unit_volume=False)
+    d_or, c_or, h_or = O.kde_density(x, y, z, (0, 0, 0), bs, nb)
+    inside = (z >= 0) & (z < 1) if zbin else np.ones(n, bool)
+    assert c_or.sum() <= inside.sum()
+    np.testing.assert_array_equal(c_ref, c_or)
+    np.testing.assert_array_equal(h_ref, h_or)
+    np.testing.assert_allclose(d_ref, d_or, rtol=0, atol=1e-13 * d_or.max())
+
+
+@pytest.mark.parametrize("sel,factor", [(0, 1.3), (1, 1.7), (1, 0.6)])
+@pytest.mark.parametrize("case", ["2d", "3d_w"])
+def test_kde_initial_smoothing_selection(sel, factor, case):
+    """initialSmoothingSelection 0 scales Silverman's bandwidth by a factor
+    other than one; 1 takes the factor times the bin distance
+    (GridProjectedKDE.F90:631-650, 4109-4117)."""
+    (x, y, z), nb, bs, w = _cloud(case)
+    d_ref, c_ref, h_ref = refrun.gpkde(x, y, z, (0.0, 0.0, 0.0), bs, nb, weights=w,
+                                       unit_volume=False, smoothing_selection=sel,
+                                       smoothing_factor=factor)
+    d_or, c_or, h_or = O.kde_density(x, y, z, (0.0, 0.0, 0.0), bs, nb, w, sel, factor)
+    np.testing.assert_allclose(h_ref, h_or, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(d_ref, d_or, rtol=0, atol=1e-13 * d_or.max())
