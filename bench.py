@@ -23,7 +23,11 @@ reference's TrackSubCell loop iterates, over the walk's wall time), the
 roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
 and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
 and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
-sample of the same workload).
+sample of the same workload: the first interval of all particles).  Before
+the timed region the plume after the first interval (25 d, still inside the
+grid) is reconstructed with the gpkde bandwidth optimisation (--kde-loops,
+default 10, convergence tests on) and reported as kde_adaptive, beside one
+loop of the compiled reference on the same points when it is built.
 """
 from __future__ import annotations
 
@@ -78,9 +82,52 @@ def cpu_baseline(P, rel, n_sample, seconds_budget=20.0):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
-def timed_run(P, rel, precision, steps, warmup):
+def adaptive_kde(E, origin, nbins, loops):
+    """gpkde with the bandwidth optimisation (brute-force kernels,
+    convergence tests on, as the reference's example input sets it) on the
+    engine's active particles; the compiled reference, when it is built,
+    runs one loop of the same reconstruction on the host cores."""
+    cells = int(np.prod(nbins))
+    E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1, n_opt_loops=1)  # warm
+    k0, _ = E.kernel_time(2)
+    t0 = time.perf_counter()
+    _, hist = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1, n_opt_loops=loops)
+    wall = time.perf_counter() - t0
+    k1, _ = E.kernel_time(2)
+    run = E.kde_info()["loops"]
+    out = {"unit": "KDE cells/s per optimisation loop", "loops_requested": loops,
+           "loops_run": run, "active_bins": int((hist != 0).sum()),
+           "wall_ms": wall * 1e3, "device_ms": k1 - k0,
+           "ms_per_loop": wall * 1e3 / max(run, 1),
+           "cells_per_s": cells * max(run, 1) / wall}
+    try:
+        from oracle import refrun
+        if refrun.gpkde_available():
+            g = E.particles()
+            act = g["status"] == 1
+            gx, gy, gz = E.global_xyz()
+            ts = []
+            for L in (0, 1):
+                t = time.perf_counter()
+                refrun.gpkde(gx[act], gy[act], gz[act], origin, (1.0, 1.0, 1.0), nbins,
+                             n_opt_loops=L, unit_volume=False, skip_error=True)
+                ts.append(time.perf_counter() - t)
+            out["cpu_reference"] = {
+                "ms_per_loop": (ts[1] - ts[0]) * 1e3, "ms_without_loops": ts[0] * 1e3,
+                "cores": os.cpu_count() or 1, "kind": "reference",
+                "sample": f"the same {int(act.sum())} points, one optimisation loop, "
+                          "the reference's GridProjectedKDE compiled from its "
+                          "sources (amdflang -O2 -fopenmp)"}
+    except Exception as exc:  # the GPU figure stands without the CPU one
+        out["cpu_reference"] = {"error": str(exc)}
+    return out
+
+
+def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
     """warmup intervals, then `steps` timed intervals from the initial state,
-    each followed by the gpkde reconstruction on the flow grid."""
+    each followed by the gpkde reconstruction on the flow grid; with
+    adaptive_loops > 0, one adaptive reconstruction of the plume after the
+    first interval is timed before the timed region."""
     from rwpt import Engine
     E = Engine(P, precision=precision)
     cell, lx, ly, lz = rel
@@ -92,6 +139,13 @@ def timed_run(P, rel, precision, steps, warmup):
         warm_steps = 0
         for w in range(warmup):
             warm_steps += E.advance((w + 1) * INTERVAL_STEPS * DT)[1]
+        adaptive = None
+        if adaptive_loops > 0:
+            # the plume after the first tracking interval, still inside the grid
+            E.load_particles(cell, lx, ly, lz)
+            E.set_time(0.0)
+            warm_steps += E.advance(INTERVAL_STEPS * DT)[1]
+            adaptive = adaptive_kde(E, origin, nbins, adaptive_loops)
         warm_ms, warm_launches = E.kernel_time(0)
         E.load_particles(cell, lx, ly, lz)
         E.set_time(0.0)
@@ -117,7 +171,8 @@ def timed_run(P, rel, precision, steps, warmup):
         active = int((E.particles()["status"] == 1).sum())
     finally:
         E.close()
-    return {"steps": total_steps, "walk_s": walk_s, "kde_s": kde_s, "elapsed": el,
+    return {"adaptive": adaptive,
+            "steps": total_steps, "walk_s": walk_s, "kde_s": kde_s, "elapsed": el,
             "kernel_ms": kms, "launches": launches, "hist_ms": hms, "dens_ms": dms,
             "bandwidth": bw, "dens": dens, "active": active,
             "rate": total_steps / walk_s,
@@ -176,13 +231,17 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000,
+                    help="particles of the CPU baseline's first interval")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", type=int, default=1, choices=(0, 1, 2),
                     help="walk arithmetic: 0 plain FP64, 1 FP32-local with the FP64 "
                          "walk's draws (default), 2 FP32-local with 4-draw normals")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--kde-loops", type=int, default=10,
+                    help="optimisation loops of the adaptive gpkde measured on the "
+                         "final plume (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -195,7 +254,8 @@ def main():
 
     P, rel = workload(args.particles)
     prec = args.precision
-    runs = {prec: timed_run(P, rel, prec, args.steps, args.warmup)}
+    runs = {prec: timed_run(P, rel, prec, args.steps, args.warmup,
+                            adaptive_loops=args.kde_loops)}
     if prec != 0 and not args.no_fp64:
         runs[0] = timed_run(P, rel, 0, args.steps, args.warmup)
     r = runs[prec]
@@ -262,6 +322,8 @@ def main():
                 "wall_ms": r["kde_s"] * 1e3 / args.steps,
                 "last_bandwidth": r["bandwidth"]},
     }
+    if r["adaptive"] is not None:
+        out["kde_adaptive"] = r["adaptive"]
     if 0 in runs and prec != 0:
         b0 = runs[0]
         out["fp64"] = {"value": b0["rate"], "ms_per_step": b0["elapsed"] * 1e3 / args.steps,
