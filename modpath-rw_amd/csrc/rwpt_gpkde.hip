@@ -101,71 +101,92 @@ struct Factors {
   double scale;      // scalar factor of the product
 };
 
+// Per-dimension values are staged in LDS before any thread indexes them
+// by its thread id: a dynamically indexed private array or by-value
+// argument would live in scratch memory.
 __device__ void build_factors(double *lds, int cap, const GkGrid &G, const int c[3],
                               const double h[3], int kind, int der, int range,
                               Factors &F) {
-  double *arr[3] = {lds, lds + cap, lds + 2 * cap};
-  __shared__ double sums[3], sumsq[3];
-  __shared__ int shm[3];
+  __shared__ double s_h[3], s_bin[3], sums[3], sumsq[3];
+  __shared__ int s_c[3], s_nb[3], s_act[3], shm[3];
   const int tid = threadIdx.x;
+  if (tid == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      s_h[d] = h[d];
+      s_bin[d] = G.bin[d];
+      s_c[d] = c[d];
+      s_nb[d] = G.nb[d];
+      s_act[d] = G.active[d];
+    }
+  }
+  __syncthreads();
   if (tid < 3) {
     const int d = tid;
     double hd = 0.0;
-    if (G.bin[d] != 0.0) hd = h[d] / G.bin[d];
+    if (s_bin[d] != 0.0) hd = s_h[d] / s_bin[d];
     int m = 0;
-    if ((kind == 1 || G.active[d] || kind == 2) && hd > 0.0 && hd < 1.0e9)
-      m = half_width(range, h[d], G.bin[d]);
-    if (kind == 0 && h[d] == 0.0) m = 0;
+    if ((kind == 1 || s_act[d] || kind == 2) && hd > 0.0 && hd < 1.0e9)
+      m = half_width(range, s_h[d], s_bin[d]);
+    if (kind == 0 && s_h[d] == 0.0) m = 0;
     if (2 * m + 1 > cap) m = -1;  // wider than the launch allows: bin skipped
     shm[d] = m;
   }
   __syncthreads();
   bool bad = false;
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     F.m[d] = shm[d] < 0 ? 0 : shm[d];
     bad = bad || shm[d] < 0;
   }
   // fill
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int m = F.m[d], L = 2 * m + 1;
     const double hd = G.bin[d] != 0.0 ? h[d] / G.bin[d] : 0.0;
+    const bool vd = kind == 2 && d == der;
     for (int i = tid; i < L; i += blockDim.x) {
       const int off = i - m < 0 ? m - i : i - m;
       double v = 1.0;
-      if (hd > 0.0) v = (kind == 2 && d == der) ? v_factor(off, hd) : w_factor(off, hd);
-      arr[d][i] = v;
+      if (hd > 0.0) v = vd ? v_factor(off, hd) : w_factor(off, hd);
+      lds[d * cap + i] = v;
     }
   }
   __syncthreads();
   if (tid < 3) {
-    const int d = tid, m = F.m[d], L = 2 * m + 1;
-    double s = 0.0, q = 0.0;
+    const int d = tid, L = 2 * (shm[d] < 0 ? 0 : shm[d]) + 1;
+    const double *a = lds + d * cap;
+    double sm = 0.0, q = 0.0;
     for (int i = 0; i < L; ++i) {
-      s += arr[d][i];
-      q += arr[d][i] * arr[d][i];
+      sm += a[i];
+      q += a[i] * a[i];
     }
-    sums[d] = s;
+    sums[d] = sm;
     sumsq[d] = q;
   }
   __syncthreads();
   // scalar factor
   double sc = 1.0;
   if (kind == 0 || kind == 1) {
+#pragma unroll
     for (int d = 0; d < 3; ++d)
       if (F.m[d] > 0 || (G.bin[d] != 0.0 && h[d] > 0.0)) sc /= sums[d];
   } else {
     double hdv[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) hdv[d] = G.bin[d] != 0.0 ? h[d] / G.bin[d] : 0.0;
+    const double hder = der == 0 ? hdv[0] : (der == 1 ? hdv[1] : hdv[2]);
     const double fnd = (double)G.ndim;
-    const double C = -1.0 / (pow(2.0, fnd - 0.5) * kSqrtPi * pow(hdv[der], 3.0));
+    const double C = -1.0 / (pow(2.0, fnd - 0.5) * kSqrtPi * pow(hder, 3.0));
     // a kernel whose squares sum to zero is zeroed (:871-875)
     sc = C * C * sumsq[0] * sumsq[1] * sumsq[2] <= 0.0 ? 0.0 : C;
     // the correction loop divides by sqrt(h/delta) in the dimensions before
     // the derivative dimension and then returns unconditionally (the
     // `if (...) matrix = 0; return;` line of :887, :993, :1097), so the
     // sqrt(3/normprod) factor and the later dimensions are never applied
-    for (int d = 0; d < der; ++d)
-      if (hdv[d] > 0.0) sc = sc / sqrt(hdv[d]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d < der && hdv[d] > 0.0) sc = sc / sqrt(hdv[d]);
   }
   // SetupMatrix with a zero bandwidth gives a zero matrix (:431-434)
   if (h[0] == 0.0 && h[1] == 0.0 && h[2] == 0.0) sc = 0.0;
@@ -173,9 +194,10 @@ __device__ void build_factors(double *lds, int cap, const GkGrid &G, const int c
   F.scale = sc;
   // fold (active dimensions) and spans
   if (tid < 3) {
-    const int d = tid, m = F.m[d];
-    if (G.active[d] && m > 0) fold(arr[d], m, c[d], G.nb[d]);
+    const int d = tid, m = shm[d] < 0 ? 0 : shm[d];
+    if (s_act[d] && m > 0) fold(lds + d * cap, m, s_c[d], s_nb[d]);
   }
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     F.lo[d] = max(c[d] - F.m[d], 1);
     F.hi[d] = min(c[d] + F.m[d], G.nb[d]);
@@ -276,16 +298,20 @@ __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
       acc[2] += (k1 * k2) * kv;
     }
   }
-  const int nsum = mode == 0 ? 1 : (mode == 1 ? 6 : 3);
-  for (int q = 0; q < nsum; ++q) red[q][threadIdx.x] = acc[q];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st)
-      for (int q = 0; q < nsum; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < nsum; ++q) out[(int64_t)n * 6 + q] = red[q][0];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) out[(int64_t)n * 6 + q] = red[q][0];
+  }
 }
 
 __global__ void gk_fill(int64_t n, double *__restrict__ a, double v) {
