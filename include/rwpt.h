@@ -40,6 +40,7 @@ extern "C" {
 #define RWPT_E_DEVICE (-3)   /* HIP runtime error */
 #define RWPT_E_NOMEM (-4)    /* device allocation failed */
 #define RWPT_E_UNSUPPORTED (-5)
+#define RWPT_E_IO (-6)       /* a file could not be opened or written */
 
 typedef struct rwpt_engine rwpt_engine;
 
@@ -211,6 +212,24 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
  * effective mass (Histogram.F90:332-341) and the optimisation loops run. */
 int rwpt_kde_info(const rwpt_engine *e, double *bandwidth, double *points,
                   double *npoints, double *eff_mass, int32_t *loops);
+
+/* Density rows as MODPATH-RW's gpkde output file holds them
+ * (GridProjectedKDE%ExportDensityUnit / %ExportDensityUnitBinary,
+ * GridProjectedKDE.F90:6504-7021, in the branch MPathRW.F90:1626-1642
+ * reaches: time index, time and particle group present): one row per bin
+ * whose density is not <= 0, x fastest.  format 0 text (I8 and ES18.9E3
+ * fields), 1 binary stream (int32 and float64 fields, no record markers);
+ * column_format (gpkdeOutColFormat) 0 bin indices, 1 bin indices and bin
+ * centre coordinates, 2 centre coordinates.  density and histogram are the
+ * rwpt_kde outputs for the grid of opt; the histogram column is written
+ * times histogram_scale (MPathRW passes gpkdeScalingFactor as the
+ * histogramScalingFactor).  append 0 replaces the file, 1 appends (the
+ * reference keeps one unit open for the run).  Needs no engine or device. */
+int rwpt_kde_write(const char *path, int32_t append, int32_t format,
+                   int32_t column_format, int32_t time_id, double time,
+                   int32_t group_id, const rwpt_kde_options *opt,
+                   const double *density, const double *histogram,
+                   double histogram_scale);
 
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,

@@ -81,6 +81,9 @@ EXPORTS = {
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
     "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp, _i32p], C.c_int),
+    "rwpt_kde_write": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                        C.c_double, C.c_int32, C.POINTER(RwptKdeOptions), _dp, _dp,
+                        C.c_double], C.c_int),
     "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
     "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
     "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
@@ -136,6 +139,26 @@ def default_options(**kw) -> RwptOptions:
         else:
             setattr(o, k, v)
     return o
+
+
+def kde_write(path, origin, bin_size, nbins, density, histogram, time_id=1, time=0.0,
+              group_id=1, fmt=0, column_format=0, histogram_scale=1.0, append=False):
+    """Writes the density rows of a reconstruction as MODPATH-RW's gpkde
+    output file holds them (rwpt_kde_write; no engine or device needed)."""
+    o = RwptKdeOptions()
+    for d in range(3):
+        o.origin[d] = origin[d]
+        o.bin_size[d] = bin_size[d]
+        o.nbins[d] = nbins[d]
+    dens = _d(density)
+    hist = _d(histogram)
+    if dens.size != int(np.prod(nbins)) or hist.size != dens.size:
+        raise RwptError("density and histogram must hold nbins cells")
+    rc = lib().rwpt_kde_write(os.fsencode(path), int(bool(append)), fmt, column_format,
+                              time_id, time, group_id, C.byref(o), _p(dens, _dp),
+                              _p(hist, _dp), histogram_scale)
+    if rc != 0:
+        raise RwptError(f"rwpt_kde_write failed ({rc})")
 
 
 class Engine:

@@ -96,7 +96,7 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
           database=False, isotropic=False, unit_volume=True, scaling_factor=1.0,
           smoothing_selection=0, smoothing_factor=1.0, skip_error=False,
           export_vars=False, bound=0, h_over_delta=(0.0, 0.0, 0.0),
-          weight_format=0, timeout=600):
+          weight_format=0, export=None, timeout=600):
     """Runs the reference's GridProjectedKDE on the points; returns
     (density, histogram counts, initial smoothing) with x fastest, and with
     export_vars also the per-loop optimisation variables (a list of arrays,
@@ -104,16 +104,19 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
     scale, sigma, curvature bandwidth(3), nEstimate, roughness XX YY ZZ,
     net roughness).  bound: boundKernelSizeFormat; h_over_delta: (min,
     delta, max) h/delta passed to Initialize when min > 0; weight_format:
-    effectiveWeightFormat."""
+    effectiveWeightFormat; export = (format, column format, time id, time,
+    group id) makes the reference write its density rows
+    (ExportDensityUnit[Binary]) and appends their bytes to the result."""
     n = int(np.asarray(x).size)
+    ex = export if export is not None else (-1, 0, 0, 0.0, 0)
     hdr = np.array([n, *nbins, n_opt_loops, int(database), int(isotropic),
                     int(weights is not None), int(unit_volume), smoothing_selection,
-                    int(export_vars), int(skip_error), bound, weight_format],
-                   np.int32)
+                    int(export_vars), int(skip_error), bound, weight_format,
+                    ex[0], ex[1], ex[2], ex[4]], np.int32)
     f8 = np.float64
     w = np.ones(n) if weights is None else np.ascontiguousarray(weights, f8)
     parts = [hdr, np.asarray(origin, f8), np.asarray(bin_size, f8),
-             np.array([scaling_factor, smoothing_factor, *h_over_delta], f8),
+             np.array([scaling_factor, smoothing_factor, *h_over_delta, ex[3]], f8),
              np.ascontiguousarray(x, f8), np.ascontiguousarray(y, f8),
              np.ascontiguousarray(z, f8), w]
     nb = int(np.prod(nbins))
@@ -133,5 +136,10 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
                 f = os.path.join(d, f"gpkde.out{k}")
                 if os.path.exists(f):
                     loops.append(np.loadtxt(f, ndmin=2))
+        if export is not None:
+            with open(os.path.join(d, "export.out"), "rb") as fh:
+                exported = fh.read()
     out = (raw[:nb].copy(), raw[nb:2 * nb].copy(), raw[2 * nb:2 * nb + 3].copy())
-    return out + (loops,) if export_vars else out
+    if export_vars:
+        out = out + (loops,)
+    return out + (exported,) if export is not None else out
