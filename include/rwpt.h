@@ -108,14 +108,22 @@ typedef struct {
   int32_t bound_kernel_size;      /* gpkdeBoundKernelSize (:788-858): 0 by
                                      the domain, 1 by min/max h over bin
                                      size, 2 unbounded                   */
-  double min_h_over_delta;        /* gpkdeKDBParams(1), format 1          */
-  double max_h_over_delta;        /* gpkdeKDBParams(3), format 1          */
+  double min_h_over_delta;        /* gpkdeKDBParams(1): smallest h/delta
+                                     of the kernel database and the lower
+                                     bound of bound format 1              */
+  double max_h_over_delta;        /* gpkdeKDBParams(3): largest h/delta   */
   int32_t effective_weight_format; /* gpkdeEffectiveWeightFormat
                                      (Histogram.F90:276-490): 0 Kish
                                      effective mass, 1 average mass, 2
                                      point counts drive the loop and the
                                      masses the last pass, 3 as 2 with the
                                      per-bin Kish counts                 */
+  int32_t kernel_database;        /* gpkdeKernelDatabase: 1 kernels come
+                                     from the log-spaced database of
+                                     InitializeKernelDatabaseFlat
+                                     (GridProjectedKDE.F90:3138-3526) with
+                                     its lookups (:5727-6221); 0 brute force */
+  double delta_h_over_delta;      /* gpkdeKDBParams(2): database spacing   */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,

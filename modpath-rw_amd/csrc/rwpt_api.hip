@@ -535,6 +535,13 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
                                "max_h_over_delta");
   if (opt->effective_weight_format < 0 || opt->effective_weight_format > 3)
     return fail(e, RWPT_E_ARG, "effective_weight_format must be 0..3");
+  // InitializeKernelDatabaseFlat's parameter checks (GridProjectedKDE.F90:693-740)
+  if (opt->kernel_database &&
+      !(opt->min_h_over_delta > 0.0 && opt->max_h_over_delta > opt->min_h_over_delta &&
+        opt->delta_h_over_delta > 0.0 && opt->delta_h_over_delta < opt->max_h_over_delta))
+    return fail(e, RWPT_E_ARG, "kernel_database needs 0 < min_h_over_delta < "
+                               "max_h_over_delta and 0 < delta_h_over_delta < "
+                               "max_h_over_delta");
   int64_t ncells = 1;
   for (int d = 0; d < 3; ++d) {
     if (opt->nbins[d] < 1) return fail(e, RWPT_E_ARG, "nbins must be >= 1");

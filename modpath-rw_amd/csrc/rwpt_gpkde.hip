@@ -106,7 +106,7 @@ struct Factors {
 // argument would live in scratch memory.
 __device__ void build_factors(double *lds, int cap, const GkGrid &G, const int c[3],
                               const double h[3], int kind, int der, int range,
-                              Factors &F) {
+                              int collapse, Factors &F) {
   __shared__ double s_h[3], s_bin[3], sums[3], sumsq[3];
   __shared__ int s_c[3], s_nb[3], s_act[3], shm[3];
   const int tid = threadIdx.x;
@@ -165,6 +165,17 @@ __device__ void build_factors(double *lds, int cap, const GkGrid &G, const int c
     sumsq[d] = q;
   }
   __syncthreads();
+  // a dimension collapsed to its centre entry keeps the normalisation over
+  // its full width (the x-y transposed x-z / y-z database lookup)
+  if (collapse >= 0 && collapse < 3) {
+    if (tid == 0 && shm[collapse] > 0) lds[collapse * cap] = lds[collapse * cap + shm[collapse]];
+    __syncthreads();
+    if (tid == 0) shm[collapse] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d == collapse) F.m[d] = 0;
+  }
   // scalar factor
   double sc = 1.0;
   if (kind == 0 || kind == 1) {
@@ -211,6 +222,7 @@ __global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
                            const double *__restrict__ w, const GkGrid G,
                            int kind, int der, int range, int cap,
                            const uint8_t *__restrict__ skip,
+                           const int8_t *__restrict__ collapse,
                            double *__restrict__ grid) {
   extern __shared__ double lds[];
   const int n = blockIdx.x;
@@ -224,7 +236,7 @@ __global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
   if (kind == 2)  // second-derivative kernels carry one bandwidth in active dims
     for (int d = 0; d < 3; ++d) h[d] = G.active[d] ? h[d] : 0.0;
   Factors F;
-  build_factors(lds, cap, G, c, h, kind, der, range, F);
+  build_factors(lds, cap, G, c, h, kind, der, range, collapse ? collapse[n] : -1, F);
   const double *ax = lds, *ay = lds + cap, *az = lds + 2 * cap;
   const int sx = F.hi[0] - F.lo[0] + 1, sy = F.hi[1] - F.lo[1] + 1,
             sz = F.hi[2] - F.lo[2] + 1;
@@ -245,7 +257,8 @@ __global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
 // kx^2 ky^2, kx^2 kz^2, ky^2 kz^2 (cross terms after squaring,
 // GridProjectedKDE.F90:2464-2469); mode 2 (2-D): k1^2, k2^2, k1 k2.
 __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
-                         const double *__restrict__ sigma, const GkGrid G,
+                         const double *__restrict__ hv, int hstride, int kind,
+                         const GkGrid G,
                          int mode, int cap, const double *__restrict__ g0,
                          const double *__restrict__ g1,
                          const double *__restrict__ g2, double binVol,
@@ -258,10 +271,11 @@ __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
   const int id = bins[n];
   const int c[3] = {id % G.nb[0] + 1, (id / G.nb[0]) % G.nb[1] + 1,
                     id / (G.nb[0] * G.nb[1]) + 1};
-  const double s = sigma[n];
-  const double h[3] = {s, s, s};
+  double h[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 3 ? d : 0)];
   Factors F;
-  build_factors(lds, cap, G, c, h, 1, 0, kRangeW, F);
+  build_factors(lds, cap, G, c, h, kind, 0, kRangeW, -1, F);
   const double *ax = lds, *ay = lds + cap, *az = lds + 2 * cap;
   const int sx = F.hi[0] - F.lo[0] + 1, sy = F.hi[1] - F.lo[1] + 1,
             sz = F.hi[2] - F.lo[2] + 1;
@@ -433,17 +447,19 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   double *dcnt = nullptr, *dh = nullptr, *dsig = nullptr, *dsd = nullptr,
          *dsums = nullptr, *darr = nullptr, *grids = nullptr;
   uint8_t *dskip = nullptr;
+  int8_t *dcol = nullptr;
   auto cleanup = [&]() {
     for (void *p : {(void *)dbins, (void *)dcnt, (void *)dh, (void *)dsig,
                     (void *)dsd, (void *)dsums, (void *)darr, (void *)grids,
-                    (void *)dskip})
+                    (void *)dskip, (void *)dcol})
       if (p) (void)hipFree(p);
   };
   if ((err = hipMalloc(&dbins, N * 4)) != hipSuccess) return cleanup(), err;
   if ((err = hipMalloc(&dcnt, N * 8)) != hipSuccess) return cleanup(), err;
   if ((err = hipMalloc(&dh, N * 24)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dsig, N * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dsd, N * 8)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&dsig, N * 24)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&dsd, N * 24)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&dcol, N)) != hipSuccess) return cleanup(), err;
   if ((err = hipMalloc(&dsums, N * 48)) != hipSuccess) return cleanup(), err;
   if ((err = hipMalloc(&darr, N * 8)) != hipSuccess) return cleanup(), err;
   if ((err = hipMalloc(&dskip, N)) != hipSuccess) return cleanup(), err;
@@ -489,19 +505,35 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     return dc != 0 ? std::pow(s, 1.0 / fnd) : 0.0;
   };
 
+  // kernel database: every lookup becomes the bandwidths of its entry
+  KernelDb db;
+  const bool useDb = opt->kernel_database != 0;
+  if (useDb)
+    db.build(opt->min_h_over_delta, opt->delta_h_over_delta, opt->max_h_over_delta,
+             opt->bin_size, opt->nbins);
+  std::vector<double> hw(3 * (size_t)N), hs(3 * (size_t)N), hsd(3 * (size_t)N);
+  std::vector<int8_t> col(N, -1);
+
   // density scatter with the current smoothing (:4623-4668, :4930-4955)
   auto density_pass = [&](const double *wbins) -> hipError_t {
-    for (int n = 0; n < N; ++n)
-      skip[n] = (hk[3 * n] < 0.0 || hk[3 * n + 1] < 0.0 || hk[3 * n + 2] < 0.0) ? 1 : 0;
-    const int cap = cap_for(hk, 3, kRangeW, false);
+    for (int n = 0; n < N; ++n) {
+      const double *hn = &hk[3 * (size_t)n];
+      skip[n] = (hn[0] < 0.0 || hn[1] < 0.0 || hn[2] < 0.0) ? 1 : 0;
+      if (useDb) col[n] = (int8_t)db.w(hn, &hw[3 * (size_t)n]);
+      else for (int d = 0; d < 3; ++d) hw[3 * (size_t)n + d] = hn[d];
+    }
+    const int cap = cap_for(hw, 3, kRangeW, false);
     if (!lds_ok(cap)) return hipErrorNotSupported;
-    hipError_t r = hipMemcpyAsync(dh, hk.data(), N * 24, hipMemcpyHostToDevice, st);
+    hipError_t r = hipMemcpyAsync(dh, hw.data(), N * 24, hipMemcpyHostToDevice, st);
     if (r == hipSuccess) r = hipMemcpyAsync(dskip, skip.data(), N, hipMemcpyHostToDevice, st);
+    if (r == hipSuccess && useDb)
+      r = hipMemcpyAsync(dcol, col.data(), N, hipMemcpyHostToDevice, st);
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(gk_fill, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
                        density_dev, 0.0);
     hipLaunchKernelGGL(gk_scatter, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                       dh, 3, wbins, G, 0, 0, kRangeW, cap, dskip, density_dev);
+                       dh, 3, wbins, G, 0, 0, kRangeW, cap, dskip,
+                       useDb ? (const int8_t *)dcol : (const int8_t *)nullptr, density_dev);
     hipLaunchKernelGGL(gk_div, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
                        density_dev, binVol);
     hipLaunchKernelGGL(gk_extract, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0,
@@ -511,14 +543,30 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     if (r == hipSuccess) r = hipStreamSynchronize(st);
     return r;
   };
+  // sigma kernels of the current support scales: brute force carries the
+  // scale in every dimension with a bin size (kind 1); a database entry
+  // has zero bandwidth outside the active dimensions (kind 0)
+  int sigCap = 0;
+  auto upload_sigma = [&]() -> hipError_t {
+    if (useDb) {
+      for (int n = 0; n < N; ++n) db.sigma(sigma[n], &hs[3 * (size_t)n]);
+      sigCap = cap_for(hs, 3, kRangeW, false);
+    } else {
+      for (int n = 0; n < N; ++n)
+        for (int d = 0; d < 3; ++d) hs[3 * (size_t)n + d] = sigma[n];
+      sigCap = cap_for(sigma, 1, kRangeW, true);
+    }
+    if (!lds_ok(sigCap)) return hipErrorNotSupported;
+    return hipMemcpyAsync(dsig, hs.data(), N * 24, hipMemcpyHostToDevice, st);
+  };
+  const int sigKind = useDb ? 0 : 1;
   // sigma gather of the density grid (nEstimate)
   auto nest_pass = [&](std::vector<double> &out) -> hipError_t {
-    const int cap = cap_for(sigma, 1, kRangeW, true);
-    if (!lds_ok(cap)) return hipErrorNotSupported;
-    hipError_t r = hipMemcpyAsync(dsig, sigma.data(), N * 8, hipMemcpyHostToDevice, st);
+    hipError_t r = upload_sigma();
     if (r != hipSuccess) return r;
+    const int cap = sigCap;
     hipLaunchKernelGGL(gk_gather, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                       dsig, G, 0, cap, density_dev, (const double *)nullptr,
+                       dsig, 3, sigKind, G, 0, cap, density_dev, (const double *)nullptr,
                        (const double *)nullptr, 1.0, 1.0, 1.0, 1.0, dsums);
     if ((r = hipGetLastError()) != hipSuccess) return r;
     std::vector<double> s6(6 * (size_t)N);
@@ -611,9 +659,11 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     const int nder = nd == 3 ? 3 : 2;
     for (int q = 0; q < nder; ++q) {
       const int der = nd == 3 ? q : dims[q];
-      std::vector<double> hb(N);
       for (int n = 0; n < N; ++n) {
-        hb[n] = curvBW[3 * (size_t)n + der];
+        const double cb = curvBW[3 * (size_t)n + der];
+        double *ho = &hsd[3 * (size_t)n];
+        if (useDb) db.sd(cb, der, ho);
+        else for (int d = 0; d < 3; ++d) ho[d] = G.active[d] ? cb : 0.0;
         skip[n] = 0;
         if (nd == 2) {
           for (int d = 0; d < 3; ++d) {
@@ -622,27 +672,26 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
           }
         }
       }
-      const int cap = cap_for(hb, 1, kRangeSD, false);
+      const int cap = cap_for(hsd, 3, kRangeSD, false);
       if (!lds_ok(cap)) return cleanup(), hipErrorNotSupported;
-      err = hipMemcpyAsync(dsd, hb.data(), N * 8, hipMemcpyHostToDevice, st);
+      err = hipMemcpyAsync(dsd, hsd.data(), N * 24, hipMemcpyHostToDevice, st);
       if (err == hipSuccess)
         err = hipMemcpyAsync(dskip, skip.data(), N, hipMemcpyHostToDevice, st);
       if (err != hipSuccess) return cleanup(), err;
       hipLaunchKernelGGL(gk_scatter, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                         dsd, 1, dcnt, G, 2, der, kRangeSD, cap, dskip, cg[q]);
+                         dsd, 3, dcnt, G, 2, der, kRangeSD, cap, dskip,
+                         (const int8_t *)nullptr, cg[q]);
       if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
     }
     {
-      const int cap = cap_for(sigma, 1, kRangeW, true);
-      if (!lds_ok(cap)) return cleanup(), hipErrorNotSupported;
-      err = hipMemcpyAsync(dsig, sigma.data(), N * 8, hipMemcpyHostToDevice, st);
-      if (err != hipSuccess) return cleanup(), err;
+      if ((err = upload_sigma()) != hipSuccess) return cleanup(), err;
+      const int cap = sigCap;
       const double c0 = G.bin[nd == 3 ? 0 : id1] * G.bin[nd == 3 ? 0 : id1];
       const double c1 = G.bin[nd == 3 ? 1 : id2] * G.bin[nd == 3 ? 1 : id2];
       const double c2 = G.bin[2] * G.bin[2];
       hipLaunchKernelGGL(gk_gather, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                         dsig, G, nd == 3 ? 1 : 2, cap, cg[0], cg[1], cg[2], binVol,
-                         c0, c1, c2, dsums);
+                         dsig, 3, sigKind, G, nd == 3 ? 1 : 2, cap, cg[0], cg[1], cg[2],
+                         binVol, c0, c1, c2, dsums);
       if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
       std::vector<double> s6(6 * (size_t)N);
       err = hipMemcpyAsync(s6.data(), dsums, N * 48, hipMemcpyDeviceToHost, st);
@@ -824,6 +873,92 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   if (loops_run) *loops_run = loops;
   cleanup();
   return hipSuccess;
+}
+
+void KernelDb::build(double min_hod, double delta_hod, double max_hod,
+                     const double binsz[3], const int nbins[3]) {
+  min = min_hod;
+  // nDelta = ceiling(log10(max/min)/log10(1 + delta)) + 1 and the log-spaced
+  // grid of prGenerateLogSpaceData (GridProjectedKDE.F90:3197-3207, 6467-6500)
+  n = (int)std::ceil(std::log10(max_hod / min_hod) / std::log10(1.0 + delta_hod)) + 1;
+  const double lo = std::log10(min_hod), hi = std::log10(max_hod);
+  const double step = (hi - lo) / (double)(n - 1);
+  double acc = 0.0;
+  hod.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    hod[i] = std::pow(10.0, lo + acc);
+    acc = acc + step;
+  }
+  // the lookups use log(h2/h1) in place of the given spacing (:3207)
+  delta = std::log(hod[1] / hod[0]);
+  ndim = 0;
+  for (int d = 0; d < 3; ++d) {
+    bin[d] = binsz[d];
+    active[d] = nbins[d] > 1;
+    if (active[d]) dims[ndim++] = d;
+  }
+}
+
+int KernelDb::index(double h, int d) const {
+  // :5740-5750, 1-based and clamped to the grid
+  if (!(h > 0.0)) return 1;
+  const int i = (int)std::floor(std::log(h / bin[d] / min) / delta) + 1;
+  return std::min(std::max(i, 1), n);
+}
+
+int KernelDb::w(const double h[3], double out[3]) const {
+  int idx[3] = {1, 1, 1};
+  for (int d = 0; d < 3; ++d)
+    if (active[d]) idx[d] = index(h[d], d);
+  out[0] = out[1] = out[2] = 0.0;
+  if (ndim == 3) {
+    // the x-y transpose of a lookup with idx(x) < idx(y) restores the
+    // bandwidths in 3-D
+    for (int d = 0; d < 3; ++d) out[d] = hod[idx[d] - 1] * bin[d];
+    return -1;
+  }
+  if (ndim == 1) {
+    // the 1-D entry is the largest index over the three dimensions (:5752-5757)
+    const int a = dims[0];
+    out[a] = hod[std::max(idx[0], std::max(idx[1], idx[2])) - 1] * bin[a];
+    return -1;
+  }
+  const int a = dims[0], b = dims[1];
+  if (idx[a] >= idx[b] || (a == 0 && b == 1)) {
+    out[a] = hod[idx[a] - 1] * bin[a];
+    out[b] = hod[idx[b] - 1] * bin[b];
+    return -1;
+  }
+  // x-z or y-z plane, transposed entry: the x-y transpose leaves the first
+  // active dimension at its centre entry and the bandwidths swapped
+  // (ComputeSpansBoundedTranspose, KernelMultiGaussian.f90:393-484)
+  out[a] = hod[idx[b] - 1] * bin[a];
+  out[b] = hod[idx[a] - 1] * bin[b];
+  return a;
+}
+
+void KernelDb::sigma(double s, double out[3]) const {
+  int idx[3] = {1, 1, 1};
+  for (int d = 0; d < 3; ++d)
+    if (active[d]) idx[d] = index(s, d);
+  out[0] = out[1] = out[2] = 0.0;
+  if (ndim == 1) {
+    const int a = dims[0];
+    out[a] = hod[std::max(idx[0], std::max(idx[1], idx[2])) - 1] * bin[a];
+    return;
+  }
+  // the flat entry without the transpose: the first two (active)
+  // dimensions take the larger and the smaller index
+  const int a = ndim == 3 ? 0 : dims[0], b = ndim == 3 ? 1 : dims[1];
+  const int hi = std::max(idx[a], idx[b]), lo = std::min(idx[a], idx[b]);
+  out[a] = hod[hi - 1] * bin[a];
+  out[b] = hod[lo - 1] * bin[b];
+  if (ndim == 3) out[2] = hod[idx[2] - 1] * bin[2];
+}
+
+void KernelDb::sd(double cb, int der, double out[3]) const {
+  const int i = cb > 0.0 ? index(cb, der) : 1;
+  for (int d = 0; d < 3; ++d) out[d] = active[d] ? hod[i - 1] * bin[d] : 0.0;
 }
 
 }  // namespace rwpt

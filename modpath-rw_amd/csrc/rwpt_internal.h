@@ -197,4 +197,29 @@ hipError_t launch_positions_from32(rwpt_engine *e);
 // host helper: GF(2) matrices of the xoroshiro128+ jump, J^(2^k)
 void jump_power_matrices(int kmax, std::vector<uint64_t> &out);
 
+// gpkde kernel database (InitializeKernelDatabaseFlat,
+// GridProjectedKDE.F90:3138-3526, log form): the h/delta grid and, for
+// each lookup, the bandwidths of the database entry it returns.  Every
+// entry is a brute-force kernel at grid bandwidths, so the device builds
+// that kernel directly instead of storing the database.
+struct KernelDb {
+  int n = 0;
+  double min = 0.0, delta = 0.0;
+  std::vector<double> hod;  // h/delta of entries 1..n
+  double bin[3] = {0, 0, 0};
+  int active[3] = {0, 0, 0};
+  int ndim = 0, dims[3] = {0, 0, 0};
+  void build(double min_hod, double delta_hod, double max_hod, const double binsz[3],
+             const int nbins[3]);
+  int index(double h, int d) const;
+  // density kernel (prSetKernelFromDatabase :5965-6003): bandwidths and
+  // the dimension left one bin wide by the x-y transpose of an x-z or y-z
+  // lookup (-1 none)
+  int w(const double h[3], double out[3]) const;
+  // sigma kernel (prSetKernelSigmaFromDatabase :6006-6043)
+  void sigma(double s, double out[3]) const;
+  // second-derivative kernel along der (:6081-6124, :6184-6221)
+  void sd(double cb, int der, double out[3]) const;
+};
+
 }  // namespace rwpt

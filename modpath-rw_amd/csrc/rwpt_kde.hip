@@ -240,9 +240,12 @@ __global__ void kde_pass(const double *__restrict__ in, double *__restrict__ out
 
 }  // namespace
 
-// Silverman bandwidth + normalised 1-D erf kernel (KernelMultiGaussian.f90
-// prComputeKernelWMatrix): offsets 0..m with m = ceiling(3 h / bin).
-static void kernel_1d(double h, double bin, std::vector<double> &w) {
+// Normalised 1-D erf kernel (KernelMultiGaussian.f90
+// prComputeKernelWMatrix): offsets 0..m with m = ceiling(3 h / bin); with
+// centre_only the centre weight alone, normalised over the full width (a
+// dimension left one bin wide by a transposed database lookup).
+static void kernel_1d(double h, double bin, std::vector<double> &w,
+                      bool centre_only = false) {
   const int m = (int)std::ceil(3 * h / bin);
   const double hd = h / bin, s2 = std::sqrt(2.0);
   std::vector<double> e(m + 1);
@@ -252,8 +255,8 @@ static void kernel_1d(double h, double bin, std::vector<double> &w) {
            std::erf(((double)i - 0.5) / (hd * s2));
     sum += (i == 0 ? 1.0 : 2.0) * e[i];
   }
-  w.assign(m + 1, 0.0);
-  for (int i = 0; i <= m; ++i) w[i] = e[i] / sum;
+  w.assign(centre_only ? 1 : m + 1, 0.0);
+  for (int i = 0; i < (int)w.size(); ++i) w[i] = e[i] / sum;
 }
 
 hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
@@ -456,12 +459,24 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                        opt->scaling_factor / binVolume);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   } else {
+    // the same bandwidth in every active dimension, or the database entry
+    // it looks up
+    double hv[3] = {0, 0, 0};
+    int collapse = -1;
+    for (int d = 0; d < 3; ++d) hv[d] = G.active[d] ? h : 0.0;
+    if (opt->kernel_database) {
+      KernelDb db;
+      db.build(opt->min_h_over_delta, opt->delta_h_over_delta, opt->max_h_over_delta,
+               opt->bin_size, opt->nbins);
+      const double hin[3] = {hv[0], hv[1], hv[2]};
+      collapse = db.w(hin, hv);
+    }
     std::vector<double> all;
     int off[3] = {0, 0, 0}, m[3] = {0, 0, 0};
     for (int d = 0; d < 3; ++d) {
       if (!G.active[d]) continue;
       std::vector<double> w;
-      kernel_1d(h, G.bin[d], w);
+      kernel_1d(hv[d], G.bin[d], w, d == collapse);
       m[d] = (int)w.size() - 1;
       off[d] = (int)all.size();
       all.insert(all.end(), w.begin(), w.end());

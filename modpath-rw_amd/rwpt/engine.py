@@ -53,7 +53,8 @@ class RwptKdeOptions(C.Structure):
                 ("initial_smoothing_factor", C.c_double),
                 ("bound_kernel_size", C.c_int32), ("min_h_over_delta", C.c_double),
                 ("max_h_over_delta", C.c_double),
-                ("effective_weight_format", C.c_int32)]
+                ("effective_weight_format", C.c_int32),
+                ("kernel_database", C.c_int32), ("delta_h_over_delta", C.c_double)]
 
 
 EXPORTS = {
@@ -295,7 +296,7 @@ class Engine:
             unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
             rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
             bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
-            effective_weight_format=0):
+            effective_weight_format=0, kernel_database=False, delta_h_over_delta=0.0):
         """gpkde reconstruction of the active particles; the keywords are
         the options of the gpkde input file (ReadGPKDEData,
         ModpathSimulationData.f90:816-1273).  Returns (density, histogram),
@@ -319,6 +320,8 @@ class Engine:
         o.min_h_over_delta = min_h_over_delta
         o.max_h_over_delta = max_h_over_delta
         o.effective_weight_format = effective_weight_format
+        o.kernel_database = int(bool(kernel_database))
+        o.delta_h_over_delta = delta_h_over_delta
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)

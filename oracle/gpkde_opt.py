@@ -15,7 +15,7 @@ tests/test_oracle_gpkde_opt_vs_reference.py.
 """
 from __future__ import annotations
 
-from math import ceil, pi, sqrt
+from math import ceil, floor, log, log10, pi, sqrt
 
 import numpy as np
 from scipy.special import erf
@@ -84,7 +84,88 @@ class Grid:
         self.bin_volume = float(np.prod([b for b in self.bin if b > 0]))
 
 
-def kernel(G, c, h, kind, der=0, rng=RANGE_W):
+class KernelDatabase:
+    """The log-spaced h/delta grid of the kernel database and the kernel each
+    lookup returns (InitializeKernelDatabaseFlat, GridProjectedKDE.F90:
+    3138-3526; lookups :5727-6221; prGenerateLogSpaceData :6467-6500).
+    Every database kernel is a brute-force kernel at a grid bandwidth, so a
+    lookup is restated as the bandwidths it stands for."""
+
+    def __init__(self, G, min_hod, delta_hod, max_hod):
+        self.G = G
+        self.min = float(min_hod)
+        n = int(ceil(log10(max_hod / min_hod) / log10(1.0 + delta_hod))) + 1
+        lo, hi = log10(min_hod), log10(max_hod)
+        step = (hi - lo) / (n - 1)
+        acc = 0.0
+        self.hod = []
+        for _ in range(n):
+            self.hod.append(10.0 ** (lo + acc))
+            acc = acc + step
+        self.n = n
+        # the spacing used by the lookups is overwritten with the natural
+        # log of the ratio of the first two entries (:3207)
+        self.delta = log(self.hod[1] / self.hod[0])
+        self.dims = [d for d in range(3) if G.active[d]]
+
+    def index(self, h, d):
+        # 1-based, clamped (:5740-5750)
+        if h <= 0:
+            return 1
+        return min(max(int(floor(log(h / self.G.bin[d] / self.min) / self.delta)) + 1, 1),
+                   self.n)
+
+    def w(self, h):
+        """Density kernel (prSetKernelFromDatabase :5965-6003): bandwidths
+        and the dimension collapsed to its centre weight (-1 none).  A 2-D
+        lookup with the first index below the second takes the transposed
+        entry; the x-y transpose restores the bandwidths in the x-y plane
+        and in 3-D, while in the x-z and y-z planes it leaves the first
+        active dimension one bin wide (KernelMultiGaussian.f90:393-484)."""
+        G = self.G
+        idx = [self.index(h[d], d) if d in self.dims else 1 for d in range(3)]
+        out = [0.0, 0.0, 0.0]
+        if G.ndim == 3:
+            for d in range(3):
+                out[d] = self.hod[idx[d] - 1] * G.bin[d]
+            return out, -1
+        a, b = self.dims
+        if idx[a] >= idx[b] or (a, b) == (0, 1):
+            out[a] = self.hod[idx[a] - 1] * G.bin[a]
+            out[b] = self.hod[idx[b] - 1] * G.bin[b]
+            return out, -1
+        out[a] = self.hod[idx[b] - 1] * G.bin[a]
+        out[b] = self.hod[idx[a] - 1] * G.bin[b]
+        return out, a
+
+    def sigma(self, s):
+        """Sigma kernel (prSetKernelSigmaFromDatabase :6006-6043): the flat
+        entry of the per-dimension indices without the transpose, so the
+        first two dimensions take the larger and the smaller index."""
+        G = self.G
+        idx = [self.index(s, d) if d in self.dims else 1 for d in range(3)]
+        out = [0.0, 0.0, 0.0]
+        if G.ndim == 3:
+            hi, lo = max(idx[0], idx[1]), min(idx[0], idx[1])
+            out = [self.hod[hi - 1] * G.bin[0], self.hod[lo - 1] * G.bin[1],
+                   self.hod[idx[2] - 1] * G.bin[2]]
+            return out
+        a, b = self.dims
+        hi, lo = max(idx[a], idx[b]), min(idx[a], idx[b])
+        out[a] = self.hod[hi - 1] * G.bin[a]
+        out[b] = self.hod[lo - 1] * G.bin[b]
+        return out
+
+    def sd(self, cb, der):
+        """Second-derivative kernel along der (prSetKernelSD2DFromDatabase
+        :6081-6124, prSetKernelSDFromDatabase :6184-6221): the grid h/delta
+        of the derivative dimension's bandwidth in every active dimension."""
+        i = self.index(cb, der) if cb > 0 else 1
+        return [self.hod[i - 1] * self.G.bin[d] if self.G.active[d] else 0.0
+                for d in range(3)]
+
+
+def kernel(G, c, h, kind, der=0, rng=RANGE_W, collapse=-1):
     """Grid slices and the bounded kernel matrix of one bin.
 
     kind 0: W kernel (SetupMatrix of KernelMultiGaussianType) with bandwidth
@@ -127,6 +208,11 @@ def kernel(G, c, h, kind, der=0, rng=RANGE_W):
                 sc /= sqrt(hdv[d])
     if h[0] == 0 and h[1] == 0 and h[2] == 0:
         sc = 0.0
+    if collapse >= 0:
+        # only the centre entry of this dimension remains, the normalisation
+        # over the full kernel stays
+        arrs[collapse] = arrs[collapse][ms[collapse]:ms[collapse] + 1].copy()
+        ms[collapse] = 0
     for d in range(3):
         if G.active[d] and ms[d] > 0:
             arrs[d] = _fold(arrs[d], ms[d], c[d], G.nb[d])
@@ -142,7 +228,7 @@ def kernel(G, c, h, kind, der=0, rng=RANGE_W):
 def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
              isotropic=False, skip_error=False, tol=0.02, export=False,
              bound=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
-             final_weights=None):
+             final_weights=None, database=None):
     """Adaptive density of a histogram.
 
     counts: histogram (x fastest) in the units the loop uses (weighted:
@@ -150,7 +236,9 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
     n_points: histogram%nPoints; bound: boundKernelSizeFormat (0 domain,
     1 the h/delta range given, 2 none); final_weights: histogram (x fastest)
     scattered with the final kernels in place of the counts (effective
-    weight formats 2 and 3, GridProjectedKDE.F90:4712-4760).
+    weight formats 2 and 3, GridProjectedKDE.F90:4712-4760); database:
+    (min, delta, max) h/delta of the kernel database, None for brute-force
+    kernels.
     Returns (density with x fastest, before the
     effective-mass, unit-volume and scaling factors, loops run, and with
     export the per-loop tables of the reference's export: rows of ix iy iz,
@@ -201,21 +289,32 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
             dc = dc - (~nz)
         return np.where(dc != 0, s ** (1.0 / fnd), 0.0)
 
+    db = KernelDatabase(G, *database) if database is not None else None
+
     def density_pass(hk):
         # (:4623-4668, :4930-4960)
         grid = np.zeros(nb)
         for n in range(N):
             if (hk[n] < 0).any():
                 continue
-            sl, K = kernel(G, ids[n], hk[n], 0)
+            if db is None:
+                sl, K = kernel(G, ids[n], hk[n], 0)
+            else:
+                hw, col = db.w(hk[n])
+                sl, K = kernel(G, ids[n], hw, 0, collapse=col)
             grid[sl] += cnt[n] * K
         grid /= binVol
         return grid, grid[ids[:, 0] - 1, ids[:, 1] - 1, ids[:, 2] - 1].copy()
 
+    def sigma_kernel(n, s_):
+        if db is None:
+            return kernel(G, ids[n], (s_,) * 3, 1)
+        return kernel(G, ids[n], db.sigma(s_), 0)
+
     def gather(grid, sig, f=None):
         out = np.zeros(N)
         for n in range(N):
-            sl, K = kernel(G, ids[n], (sig[n],) * 3, 1)
+            sl, K = sigma_kernel(n, sig[n])
             out[n] = (grid[sl] * K).sum()
         return out
 
@@ -284,7 +383,10 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
             for n in range(N):
                 if nd == 2 and ((curv[n] < 0).any() or np.isnan(curv[n]).any()):
                     continue
-                h = [curv[n, der] if G.active[d] else 0.0 for d in range(3)]
+                if db is None:
+                    h = [curv[n, der] if G.active[d] else 0.0 for d in range(3)]
+                else:
+                    h = db.sd(curv[n, der], der)
                 sl, K = kernel(G, ids[n], h, 2, der, RANGE_SD)
                 cg[sl] += cnt[n] * K
             cgs.append(cg / binVol / G.bin[der] ** 2)
@@ -293,7 +395,7 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
             c12 = c1 * c2
             s = np.zeros((N, 3))
             for n in range(N):
-                sl, K = kernel(G, ids[n], (sigma[n],) * 3, 1)
+                sl, K = sigma_kernel(n, sigma[n])
                 s[n] = [(c1[sl] ** 2 * K).sum(), (c2[sl] ** 2 * K).sum(), (c12[sl] * K).sum()]
             rough[:, dims[0]] = s[:, 0]
             rough[:, dims[1]] = s[:, 1]
@@ -306,7 +408,7 @@ def optimize(counts, nbins, bin_size, h_init, n_points, n_opt_loops,
             cx, cy, cz = (c * c for c in cgs)
             s = np.zeros((N, 6))
             for n in range(N):
-                sl, K = kernel(G, ids[n], (sigma[n],) * 3, 1)
+                sl, K = sigma_kernel(n, sigma[n])
                 s[n] = [(cx[sl] * K).sum(), (cy[sl] * K).sum(), (cz[sl] * K).sum(),
                         (cx[sl] * cy[sl] * K).sum(), (cx[sl] * cz[sl] * K).sum(),
                         (cy[sl] * cz[sl] * K).sum()]

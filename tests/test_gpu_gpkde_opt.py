@@ -1,4 +1,5 @@
-"""GPU parity of the gpkde bandwidth optimisation (nOptimizationLoops > 0).
+"""GPU parity of the gpkde bandwidth optimisation (nOptimizationLoops > 0),
+its options and the kernel database.
 
 The device loop is compared with the numpy restatement of
 GridProjectedKDE%ComputeDensityOptimization (itself pinned loop by loop to
@@ -70,6 +71,25 @@ CASES = {
                             False, 1.0, {"initial_smoothing": 1, "smoothing_factor": 1.7}),
     "2d_compressed_z": ((60, 40, 1), ((1.0, 1.0, 0.0), (60, 40, 1)), 4, False, False,
                         False, 1.0, {}),
+    "2d_database": ((60, 40, 1), ((1.0, 1.0, 1.0), (60, 40, 1)), 4, False, False, False,
+                    1.0, {"kernel_database": True, "min_h_over_delta": 0.3,
+                          "delta_h_over_delta": 0.3, "max_h_over_delta": 5.0}),
+    "2d_database_aniso_bins": ((60, 40, 1), ((2.0, 0.5, 1.0), (30, 80, 1)), 4, True, False,
+                               False, 1.0, {"kernel_database": True,
+                                            "min_h_over_delta": 0.3,
+                                            "delta_h_over_delta": 0.3,
+                                            "max_h_over_delta": 6.0}),
+    "2d_xz_database": ((50, 1, 30), ((1.0, 1.0, 1.0), (50, 1, 30)), 3, False, False, False,
+                       1.0, {"kernel_database": True, "min_h_over_delta": 0.3,
+                             "delta_h_over_delta": 0.3, "max_h_over_delta": 5.0}),
+    "2d_xz_database_no_loops": ((50, 1, 30), ((1.0, 1.0, 1.0), (50, 1, 30)), 0, False,
+                                False, True, 1.0, {"kernel_database": True,
+                                                   "min_h_over_delta": 0.3,
+                                                   "delta_h_over_delta": 0.3,
+                                                   "max_h_over_delta": 5.0}),
+    "3d_database": ((24, 20, 12), ((1.0, 0.5, 2.0), (24, 20, 12)), 3, False, False, False,
+                    1.0, {"kernel_database": True, "min_h_over_delta": 0.3,
+                          "delta_h_over_delta": 0.3, "max_h_over_delta": 5.0}),
 }
 
 
@@ -80,8 +100,12 @@ def _ref_kwargs(kw):
         out["weight_format"] = kw["effective_weight_format"]
     if "bound_kernel_size" in kw:
         out["bound"] = kw["bound_kernel_size"]
-        out["h_over_delta"] = (kw.get("min_h_over_delta", 0.0), 0.0,
+    if "bound_kernel_size" in kw or "kernel_database" in kw:
+        out["h_over_delta"] = (kw.get("min_h_over_delta", 0.0),
+                               kw.get("delta_h_over_delta", 0.0),
                                kw.get("max_h_over_delta", 0.0))
+    if kw.get("kernel_database"):
+        out["database"] = True
     if "initial_smoothing" in kw:
         out["smoothing_selection"] = kw["initial_smoothing"]
         out["smoothing_factor"] = kw["smoothing_factor"]
@@ -133,7 +157,9 @@ def test_adaptive_density(name):
         counts, nb, bs, oh, npts, loops, isotropic=iso,
         bound=kw.get("bound_kernel_size", 0),
         min_h_over_delta=kw.get("min_h_over_delta", 0.0),
-        max_h_over_delta=kw.get("max_h_over_delta", 0.0), final_weights=final)
+        max_h_over_delta=kw.get("max_h_over_delta", 0.0), final_weights=final,
+        database=((kw["min_h_over_delta"], kw["delta_h_over_delta"], kw["max_h_over_delta"])
+                  if kw.get("kernel_database") else None))
     want = want * eff * sf * (np.prod([b for b in bs if b > 0]) if uv else 1.0)
     assert info["loops"] == oloops
     np.testing.assert_allclose(dens, want, rtol=0, atol=1e-9 * want.max())

@@ -143,3 +143,41 @@ def test_kernel_size_bounds(bound, hod, nb):
     for a, b in zip(mine, tabs):
         compare_tables(a, b, 5e-8)
     np.testing.assert_allclose(dens, dref, rtol=0, atol=1e-12 * dref.max())
+
+
+DB_CASES = {
+    "2d_xy": ((60, 40, 1), (1.0, 1.0, 1.0)),
+    "2d_xz": ((50, 1, 30), (1.0, 1.0, 1.0)),
+    "2d_yz": ((1, 36, 44), (1.0, 1.0, 1.0)),
+    "2d_xy_aniso": ((60, 40, 1), (2.0, 0.5, 1.0)),
+    "3d": ((24, 20, 16), (1.0, 1.0, 1.0)),
+    "3d_aniso": ((24, 20, 16), (1.0, 0.5, 2.0)),
+}
+
+
+@pytest.mark.parametrize("loops", [0, 2])
+@pytest.mark.parametrize("name", list(DB_CASES))
+def test_kernel_database(name, loops):
+    """Kernels from the log-spaced database (InitializeKernelDatabaseFlat,
+    GridProjectedKDE.F90:3138-3526; lookups :5727-6221), with the lookups'
+    quirks: the sigma entry taken without its transpose (larger index in the
+    first dimension), and the x-y transpose of an x-z / y-z density lookup
+    that leaves the first active dimension one bin wide."""
+    nb, bs = DB_CASES[name]
+    hod = (0.3, 0.3, 5.0)
+    pts, _ = cloud(nb, n=8000 if nb[2] > 1 and nb[0] > 1 else 20000, seed=2)
+    pts = [p * b for p, b in zip(pts, bs)]
+    res = refrun.gpkde(*pts, (0, 0, 0), bs, nb, n_opt_loops=loops, database=True,
+                       h_over_delta=hod, unit_volume=False, skip_error=True,
+                       export_vars=loops > 0)
+    counts = histogram([p / b for p, b in zip(pts, bs)], nb)
+    mine = gpkde_opt.optimize(counts, nb, bs, res[2], counts.sum(), loops,
+                              skip_error=True, export=loops > 0, database=hod)
+    if loops:
+        for a, b in zip(mine[2], res[3]):
+            compare_tables(a, b, 5e-8)
+    np.testing.assert_allclose(mine[0], res[0], rtol=0, atol=1e-12 * res[0].max())
+    brute = refrun.gpkde(*pts, (0, 0, 0), bs, nb, n_opt_loops=loops, unit_volume=False,
+                         skip_error=True)[0]
+    # the database is in effect: its grid bandwidths change the density
+    assert np.abs(brute - res[0]).max() > 1e-6 * res[0].max()
