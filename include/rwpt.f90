@@ -1,0 +1,180 @@
+! Fortran binding of the C ABI of include/rwpt.h for MODPATH-RW
+! (ISO_C_BINDING): derived types with the field order and kinds of the C
+! structs, one interface per entry point.  The reference's driver uses it in
+! place of the OpenMP particle loop of MPathRW.F90:1239-1403 and of the gpkde
+! calls of MPathRW.F90:556-574 and :1626-1642.
+module RwptGpuModule
+  use iso_c_binding
+  implicit none
+  private
+  public :: rwpt_grid, rwpt_options, rwpt_kde_options
+  public :: rwpt_default_options, rwpt_create, rwpt_destroy, rwpt_last_error
+  public :: rwpt_set_flow, rwpt_set_cell_extras, rwpt_set_dispersion
+  public :: rwpt_load_particles, rwpt_get_particles, rwpt_particle_count
+  public :: rwpt_advance, rwpt_set_time, rwpt_global_xyz, rwpt_kde, rwpt_kde_info
+  public :: rwpt_kde_write, rwpt_get_substeps, rwpt_set_tuning, rwpt_last_launches
+  public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
+
+  type, bind(C) :: rwpt_grid
+    integer(c_int32_t) :: ncol, nrow, nlay
+    type(c_ptr) :: delr, delc, top, bot          ! c_loc of real(c_double) arrays
+  end type
+  type, bind(C) :: rwpt_options
+    integer(c_int32_t) :: advection_kind, time_step_kind
+    real(c_double)     :: time_step_params(2)
+    integer(c_int32_t) :: dimension_mask(3), backward_tracking, stop_zone
+    integer(c_int32_t) :: stop_at_weak_sinks, stop_at_weak_sources, steady_state
+    integer(c_int32_t) :: precision
+    integer(c_int64_t) :: base_seed(2)
+  end type
+  type, bind(C) :: rwpt_kde_options
+    real(c_double)     :: origin(3), bin_size(3)
+    integer(c_int32_t) :: nbins(3), mode, weighted
+    real(c_double)     :: scaling_factor
+    integer(c_int32_t) :: unit_volume, n_opt_loops, isotropic, skip_error_convergence
+    real(c_double)     :: rel_error_convergence
+    integer(c_int32_t) :: initial_smoothing
+    real(c_double)     :: initial_smoothing_factor
+    integer(c_int32_t) :: bound_kernel_size
+    real(c_double)     :: min_h_over_delta, max_h_over_delta
+    integer(c_int32_t) :: effective_weight_format, kernel_database
+    real(c_double)     :: delta_h_over_delta
+  end type
+
+  interface
+    subroutine rwpt_default_options(opt) bind(C, name='rwpt_default_options')
+      import :: rwpt_options
+      type(rwpt_options), intent(out) :: opt
+    end subroutine
+    integer(c_int) function rwpt_create(grid, opt, e) bind(C, name='rwpt_create')
+      import :: c_int, c_ptr, rwpt_grid, rwpt_options
+      type(rwpt_grid), intent(in) :: grid
+      type(rwpt_options), intent(in) :: opt
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function rwpt_destroy(e) bind(C, name='rwpt_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    type(c_ptr) function rwpt_last_error(e) bind(C, name='rwpt_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function rwpt_set_flow(e, frf, fff, flf, bflow, porosity, &
+        retardation, ibound, icbound, default_icbound) bind(C, name='rwpt_set_flow')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      real(c_double), intent(in) :: frf(*), fff(*), flf(*), bflow(*), porosity(*), retardation(*)
+      integer(c_int32_t), intent(in) :: ibound(*), icbound(*)
+      integer(c_int32_t), value :: default_icbound
+    end function
+    integer(c_int) function rwpt_set_cell_extras(e, head, layer_type, zone, &
+        source_flow, sink_flow) bind(C, name='rwpt_set_cell_extras')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      real(c_double), intent(in) :: head(*), source_flow(*), sink_flow(*)
+      integer(c_int32_t), intent(in) :: layer_type(*), zone(*)
+    end function
+    integer(c_int) function rwpt_set_dispersion(e, model, uniform, aL, aT, aLV, aTV, &
+        dmeff) bind(C, name='rwpt_set_dispersion')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: model, uniform
+      real(c_double), intent(in) :: aL(*), aT(*), aLV(*), aTV(*), dmeff(*)
+    end function
+    integer(c_int) function rwpt_load_particles(e, n, cell, lx, ly, lz, t, status, &
+        rng, stream_offset) bind(C, name='rwpt_load_particles')
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: e
+      integer(c_int64_t), value :: n, stream_offset
+      integer(c_int32_t), intent(in) :: cell(*), status(*)
+      real(c_double), intent(in) :: lx(*), ly(*), lz(*), t(*)
+      type(c_ptr), value :: rng                  ! c_null_ptr: per-particle streams
+    end function
+    integer(c_int) function rwpt_get_particles(e, cell, lx, ly, lz, t, status, rng) &
+        bind(C, name='rwpt_get_particles')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), intent(out) :: cell(*), status(*)
+      real(c_double), intent(out) :: lx(*), ly(*), lz(*), t(*)
+      type(c_ptr), value :: rng
+    end function
+    integer(c_int64_t) function rwpt_particle_count(e) bind(C, name='rwpt_particle_count')
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function rwpt_set_time(e, t) bind(C, name='rwpt_set_time')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), value :: t
+    end function
+    integer(c_int) function rwpt_advance(e, t_stop, n_active, n_substeps) &
+        bind(C, name='rwpt_advance')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: e
+      real(c_double), value :: t_stop
+      integer(c_int64_t), intent(out) :: n_active, n_substeps
+    end function
+    integer(c_int) function rwpt_global_xyz(e, gx, gy, gz) bind(C, name='rwpt_global_xyz')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: gx(*), gy(*), gz(*)
+    end function
+    integer(c_int) function rwpt_kde(e, opt, mass, density, histogram) bind(C, name='rwpt_kde')
+      import :: c_int, c_ptr, c_double, rwpt_kde_options
+      type(c_ptr), value :: e
+      type(rwpt_kde_options), intent(in) :: opt
+      real(c_double), intent(in) :: mass(*)
+      real(c_double), intent(out) :: density(*), histogram(*)
+    end function
+    integer(c_int) function rwpt_kde_info(e, bandwidth, points, npoints, eff_mass, &
+        loops) bind(C, name='rwpt_kde_info')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: bandwidth, points, npoints, eff_mass
+      integer(c_int32_t), intent(out) :: loops
+    end function
+    integer(c_int) function rwpt_kde_write(path, append, format, column_format, &
+        time_id, time, group_id, opt, density, histogram, histogram_scale) &
+        bind(C, name='rwpt_kde_write')
+      import :: c_int, c_char, c_double, c_int32_t, rwpt_kde_options
+      character(kind=c_char), intent(in) :: path(*)   ! trim(name)//c_null_char
+      integer(c_int32_t), value :: append, format, column_format, time_id, group_id
+      real(c_double), value :: time, histogram_scale
+      type(rwpt_kde_options), intent(in) :: opt
+      real(c_double), intent(in) :: density(*), histogram(*)
+    end function
+    integer(c_int) function rwpt_get_substeps(e, substeps) bind(C, name='rwpt_get_substeps')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: e
+      integer(c_int64_t), intent(out) :: substeps(*)
+    end function
+    integer(c_int) function rwpt_set_tuning(e, budget, block, sort, min_waves) &
+        bind(C, name='rwpt_set_tuning')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: budget, block, sort, min_waves
+    end function
+    integer(c_int) function rwpt_last_launches(e) bind(C, name='rwpt_last_launches')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function rwpt_kernel_time(e, which, ms, launches) &
+        bind(C, name='rwpt_kernel_time')
+      import :: c_int, c_ptr, c_int32_t, c_double, c_int64_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: which
+      real(c_double), intent(out) :: ms
+      integer(c_int64_t), intent(out) :: launches
+    end function
+    integer(c_int) function rwpt_reset_kernel_time(e) bind(C, name='rwpt_reset_kernel_time')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function rwpt_get_cell_tables(e, out) bind(C, name='rwpt_get_cell_tables')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: out(*)
+    end function
+  end interface
+end module RwptGpuModule
