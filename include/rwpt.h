@@ -124,6 +124,10 @@ typedef struct {
                                      (GridProjectedKDE.F90:3138-3526) with
                                      its lookups (:5727-6221); 0 brute force */
   double delta_h_over_delta;      /* gpkdeKDBParams(2): database spacing   */
+  int32_t member;                 /* ensemble engines: the member whose
+                                     particles are reconstructed (0-based),
+                                     -1 all members together; ignored with
+                                     one member                          */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,
@@ -133,6 +137,20 @@ void rwpt_default_options(rwpt_options *opt);
 /* ParticleTrackingEngine%Initialize, ParticleTrackingEngine.f90:393-492 */
 int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
                 rwpt_engine **out);
+
+/* An ensemble of `members` independent copies of the grid in one engine
+ * (no counterpart in the reference, which runs one model per process): the
+ * members are stacked in HBM as one grid of members * nlay layers whose
+ * vertical connections are cut between members, so one rwpt_advance walks
+ * every member's particles in the same launches.  Cells are numbered
+ * member-major: cell c of member m (both 0-based) is m * ncell + c + 1 in
+ * rwpt_load_particles / rwpt_get_particles, and every per-cell array of
+ * rwpt_set_flow, rwpt_set_cell_extras and rwpt_set_dispersion (uniform 0)
+ * holds members * ncell values.  A member's results are those of the same
+ * member run alone with stream_offset = its first particle's index.
+ * rwpt_create is rwpt_create_ensemble with one member. */
+int rwpt_create_ensemble(const rwpt_grid *grid, const rwpt_options *opt,
+                         int32_t members, rwpt_engine **out);
 int rwpt_destroy(rwpt_engine *e);
 const char *rwpt_last_error(const rwpt_engine *e);
 
@@ -154,7 +172,8 @@ int rwpt_set_cell_extras(rwpt_engine *e, const double *head,
 
 /* Dispersion (DSP file, TransportModelData.f90:583-1040): model 0 linear
  * isotropic, 1 axisymmetric.  uniform=1 => each array holds one value,
- * else ncell values.  aLV/aTV may be NULL for model 0. */
+ * uniform=2 => one value per ensemble member, uniform=0 => one value per
+ * cell.  aLV/aTV may be NULL for model 0. */
 int rwpt_set_dispersion(rwpt_engine *e, int32_t model, int32_t uniform,
                         const double *aL, const double *aT, const double *aLV,
                         const double *aTV, const double *dmeff);

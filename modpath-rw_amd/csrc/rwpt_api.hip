@@ -121,14 +121,21 @@ void rwpt_default_options(rwpt_options *opt) {
 
 int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
                 rwpt_engine **out) {
+  return rwpt_create_ensemble(grid, opt, 1, out);
+}
+
+int rwpt_create_ensemble(const rwpt_grid *grid, const rwpt_options *opt,
+                         int32_t members, rwpt_engine **out) {
   if (!out) return RWPT_E_ARG;
   *out = nullptr;
-  if (!grid || !opt) return RWPT_E_ARG;
+  if (!grid || !opt || members < 1) return RWPT_E_ARG;
   if (grid->ncol <= 0 || grid->nrow <= 0 || grid->nlay <= 0 || !grid->delr ||
       !grid->delc || !grid->top || !grid->bot)
     return RWPT_E_ARG;
-  const int64_t ncell = (int64_t)grid->ncol * grid->nrow * grid->nlay;
-  if (ncell > 0x7fffffffLL) return RWPT_E_ARG;
+  const int64_t ncell_member = (int64_t)grid->ncol * grid->nrow * grid->nlay;
+  const int64_t ncell = ncell_member * members;
+  if (ncell > 0x7fffffffLL || (int64_t)grid->nlay * members > 0x7fffffffLL)
+    return RWPT_E_ARG;
   if (opt->advection_kind != 1 && opt->advection_kind != 2) return RWPT_E_ARG;
   if (opt->time_step_kind < 1 || opt->time_step_kind > 4) return RWPT_E_ARG;
   int nd = 0;
@@ -157,12 +164,23 @@ int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
   DeviceProblem &P = e->prob;
   P.ncol = grid->ncol;
   P.nrow = grid->nrow;
-  P.nlay = grid->nlay;
+  P.nlay = grid->nlay * members;
+  P.nlay_member = grid->nlay;
+  P.members = members;
   P.ncell = ncell;
+  P.ncell_member = ncell_member;
   err = upload(e, &P.delr, grid->delr, P.ncol);
   if (err == hipSuccess) err = upload(e, &P.delc, grid->delc, P.nrow);
-  if (err == hipSuccess) err = upload(e, &P.top, grid->top, ncell);
-  if (err == hipSuccess) err = upload(e, &P.bot, grid->bot, ncell);
+  {
+    // every member has the geometry of the grid given
+    std::vector<double> tb(ncell);
+    for (int64_t c = 0; c < ncell; ++c) tb[c] = grid->top[c % ncell_member];
+    if (err == hipSuccess) err = upload(e, &P.top, tb.data(), ncell);
+    for (int64_t c = 0; c < ncell; ++c) tb[c] = grid->bot[c % ncell_member];
+    if (err == hipSuccess) err = upload(e, &P.bot, tb.data(), ncell);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  }
+  if (err == hipSuccess) err = dev_alloc(&P.uniform_member, members);
   if (err == hipSuccess) err = dev_alloc(&P.q6, 6 * ncell);
   if (err == hipSuccess) err = dev_alloc(&P.dzrw, ncell);
   if (err == hipSuccess) err = dev_alloc(&P.cells, ncell * CELL_DOUBLES);
@@ -206,6 +224,7 @@ int rwpt_destroy(rwpt_engine *e) {
   dev_free(P.icbound); dev_free(P.head); dev_free(P.layer_type);
   dev_free(P.zone); dev_free(P.source); dev_free(P.sink); dev_free(P.disp);
   dev_free(P.q6); dev_free(P.dzrw); dev_free(P.cells); dev_free(P.cells32);
+  dev_free(P.uniform_member);
   free_particles(e->part);
   dev_free(e->jump_mats);
   dev_free(e->counters);
@@ -242,7 +261,21 @@ int rwpt_set_flow(rwpt_engine *e, const double *frf, const double *fff,
     if (retardation && !(retardation[c] > 0.0))
       return fail(e, RWPT_E_ARG, "retardation must be > 0");
   }
-  hipError_t err = upload(e, &P.frf, frf, n);
+  // isUniformPorosity (ModpathBasicData.f90:208), per ensemble member: a
+  // member whose cells share one porosity takes its corner porosities from
+  // the cell (TrackSubCell.F90:4227-4283)
+  std::vector<int32_t> uni(P.members, 1);
+  for (int m = 0; m < P.members; ++m) {
+    const double *pm = porosity + (int64_t)m * P.ncell_member;
+    for (int64_t c = 1; c < P.ncell_member; ++c)
+      if (pm[c] != pm[0]) {
+        uni[m] = 0;
+        break;
+      }
+  }
+  hipError_t err = hipMemcpyAsync(P.uniform_member, uni.data(), P.members * 4,
+                                  hipMemcpyHostToDevice, e->stream);
+  if (err == hipSuccess) err = upload(e, &P.frf, frf, n);
   if (err == hipSuccess) err = upload(e, &P.fff, fff, n);
   if (err == hipSuccess) err = upload(e, &P.flf, flf, n);
   if (err == hipSuccess) err = upload(e, &P.bflow, bflow, 6 * n);
@@ -253,13 +286,8 @@ int rwpt_set_flow(rwpt_engine *e, const double *frf, const double *fff,
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return hip_fail(e, err, "set_flow");
   P.default_icbound = default_icbound;
-  // isUniformPorosity (ModpathBasicData.f90:208)
   P.uniform_porosity = 1;
-  for (int64_t c = 1; c < n; ++c)
-    if (porosity[c] != porosity[0]) {
-      P.uniform_porosity = 0;
-      break;
-    }
+  for (int m = 0; m < P.members; ++m) P.uniform_porosity &= uni[m];
   e->have_flow = true;
   e->tables_dirty = true;
   return RWPT_OK;
@@ -291,15 +319,20 @@ int rwpt_set_dispersion(rwpt_engine *e, int32_t model, int32_t uniform,
   if (!aL || !aT || !dmeff) return fail(e, RWPT_E_ARG, "aL, aT, dmeff required");
   if (model == 1 && (!aLV || !aTV))
     return fail(e, RWPT_E_ARG, "axisymmetric model needs aLV and aTV");
+  if (uniform < 0 || uniform > 2)
+    return fail(e, RWPT_E_ARG, "uniform must be 0 (per cell), 1 (one value) or 2 "
+                               "(per ensemble member)");
   DeviceProblem &P = e->prob;
-  const int64_t nd = uniform ? 1 : P.ncell;
+  // per member values become per cell values of that member's cells
+  const int64_t nd = uniform == 1 ? 1 : P.ncell;
   std::vector<double> h(5 * nd, 0.0);
   for (int64_t i = 0; i < nd; ++i) {
-    h[i] = aL[i];
-    h[nd + i] = aT[i];
-    h[2 * nd + i] = model == 1 ? aLV[i] : 0.0;
-    h[3 * nd + i] = model == 1 ? aTV[i] : 0.0;
-    h[4 * nd + i] = dmeff[i];
+    const int64_t s = uniform == 2 ? i / P.ncell_member : i;
+    h[i] = aL[s];
+    h[nd + i] = aT[s];
+    h[2 * nd + i] = model == 1 ? aLV[s] : 0.0;
+    h[3 * nd + i] = model == 1 ? aTV[s] : 0.0;
+    h[4 * nd + i] = dmeff[s];
   }
   hipError_t err = upload(e, &P.disp, h.data(), h.size());
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);

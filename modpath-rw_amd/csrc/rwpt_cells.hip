@@ -22,6 +22,7 @@ namespace {
 
 struct GridDims {
   int ncol, nrow, nlay;
+  int nlay_member;  // layers of one ensemble member (nlay for one member)
 };
 
 __device__ __forceinline__ int face_neighbor(const GridDims g, int c0, int f) {
@@ -33,8 +34,9 @@ __device__ __forceinline__ int face_neighbor(const GridDims g, int c0, int f) {
     case 1: return j < g.ncol - 1 ? c0 + 2 : 0;
     case 2: return i < g.nrow - 1 ? c0 + 1 + g.ncol : 0;
     case 3: return i > 0 ? c0 + 1 - g.ncol : 0;
-    case 4: return k < g.nlay - 1 ? c0 + 1 + lc : 0;
-    default: return k > 0 ? c0 + 1 - lc : 0;
+    // vertical connections stop at the bottom/top of each ensemble member
+    case 4: return (k % g.nlay_member) < g.nlay_member - 1 ? c0 + 1 + lc : 0;
+    default: return (k % g.nlay_member) > 0 ? c0 + 1 - lc : 0;
   }
 }
 
@@ -160,7 +162,7 @@ __constant__ int kCp[8][6] = {{1, 2, 7, 8, 13, 14},   {4, 5, 7, 8, 13, 15},
                               {1, 3, 10, 12, 16, 17}, {4, 6, 10, 11, 13, 15}};
 
 struct TableParams {
-  int32_t disp_model, uniform_porosity, backward, time_step_kind;
+  int32_t disp_model, backward, time_step_kind;
   double ts0, ts1;
 };
 
@@ -180,7 +182,9 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
                               const double *__restrict__ disp, int64_t ndisp,
                               const double *__restrict__ q6,
                               const double *__restrict__ dzrw,
-                              const TableParams tp, double *__restrict__ cells,
+                              const TableParams tp,
+                              const int32_t *__restrict__ uniform_member,
+                              double *__restrict__ cells,
                               double *__restrict__ debug) {
   const int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c0 >= ncell) return;
@@ -212,8 +216,8 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
   double nQ[18][6], nA[18][3], nV[18], nP[18];
   for (int n = 0; n < 18; ++n) {
     const int kk = k + kSlot[n][0], ii = i + kSlot[n][1], jj = j + kSlot[n][2];
-    const bool in = kk >= 0 && kk < g.nlay && ii >= 0 && ii < g.nrow &&
-                    jj >= 0 && jj < g.ncol;
+    const bool in = kk >= 0 && kk < g.nlay && kk / g.nlay_member == k / g.nlay_member &&
+                    ii >= 0 && ii < g.nrow && jj >= 0 && jj < g.ncol;
     if (!in) {
       for (int f = 0; f < 6; ++f) nQ[n][f] = 0.0;
       nA[n][0] = nA[n][1] = nA[n][2] = 0.0;
@@ -250,7 +254,7 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
   // corner porosity (Uniform / Interpolated)
   double pc[8];
   for (int ci = 0; ci < 8; ++ci) {
-    if (tp.uniform_porosity) {
+    if (uniform_member[k / g.nlay_member]) {
       pc[ci] = phi;
     } else {
       double sV = DX * DY * DZ;
@@ -487,7 +491,7 @@ hipError_t launch_positions_from32(rwpt_engine *e) {
 
 hipError_t build_cell_tables(rwpt_engine *e, double *debug_out) {
   DeviceProblem &P = e->prob;
-  const GridDims g{P.ncol, P.nrow, P.nlay};
+  const GridDims g{P.ncol, P.nrow, P.nlay, P.nlay_member};
   const int block = 256;
   const unsigned grid = (unsigned)((P.ncell + block - 1) / block);
   hipLaunchKernelGGL(assemble_cells, dim3(grid), dim3(block), 0, e->stream, g,
@@ -497,12 +501,13 @@ hipError_t build_cell_tables(rwpt_engine *e, double *debug_out) {
                      P.cells);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  TableParams tp{P.disp_model, P.uniform_porosity, e->opt.backward_tracking,
+  TableParams tp{P.disp_model, e->opt.backward_tracking,
                  e->opt.time_step_kind, e->opt.time_step_params[0],
                  e->opt.time_step_params[1]};
   hipLaunchKernelGGL(corner_tables, dim3(grid), dim3(block), 0, e->stream, g,
                      P.ncell, P.delr, P.delc, P.porosity, P.retardation,
-                     P.disp, P.ndisp, P.q6, P.dzrw, tp, P.cells, debug_out);
+                     P.disp, P.ndisp, P.q6, P.dzrw, tp, P.uniform_member, P.cells,
+                     debug_out);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // counters[3] is free outside an advance
   unsigned long long one = 1ull, flag = 0ull;

@@ -87,8 +87,12 @@ struct KernelTimer {
 };
 
 struct DeviceProblem {
-  int ncol = 0, nrow = 0, nlay = 0;
-  int64_t ncell = 0;
+  // an ensemble of `members` copies of one grid is stored as one grid of
+  // members * nlay_member layers whose vertical connections are cut at
+  // every member boundary; cells are numbered member-major
+  int ncol = 0, nrow = 0, nlay = 0, nlay_member = 0, members = 1;
+  int64_t ncell = 0;           // all members
+  int64_t ncell_member = 0;    // one member
   // device inputs
   double *delr = nullptr, *delc = nullptr, *top = nullptr, *bot = nullptr;
   double *frf = nullptr, *fff = nullptr, *flf = nullptr, *bflow = nullptr;
@@ -101,7 +105,8 @@ struct DeviceProblem {
   int64_t ndisp = 0;
   int32_t default_icbound = 0;
   int32_t disp_model = 0;
-  int32_t uniform_porosity = 0;
+  int32_t uniform_porosity = 0;      // every member's porosity uniform
+  int32_t *uniform_member = nullptr;  // per member: its porosity is uniform
   // assembled per-cell scratch of the table build
   double *q6 = nullptr;     // 6 * ncell assembled face flows
   double *dzrw = nullptr;   // ncell

@@ -29,7 +29,14 @@ struct KdeGrid {
   double origin[3], bin[3];
   int nb[3];
   int active[3];
+  int member;             // ensemble member reconstructed, -1 all
+  int64_t ncell_member;   // flow cells of one member
 };
+
+// the particle enters the reconstruction: active, and in the member asked for
+__device__ __forceinline__ bool kde_takes(int32_t status, int32_t cell, const KdeGrid &G) {
+  return status == 1 && (G.member < 0 || (int64_t)(cell - 1) / G.ncell_member == G.member);
+}
 
 // Global coordinates (ConvertToModelXYZ) and bin key of every particle;
 // inactive particles and points outside the grid get the key ncells.
@@ -51,7 +58,7 @@ __global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
   if (p >= n) return;
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
   vals[p] = (int32_t)p;
-  if (status[p] != 1) {
+  if (!kde_takes(status[p], cellv[p], G)) {
     keys[p] = (int32_t)ncells;
     return;
   }
@@ -122,6 +129,7 @@ __global__ void kde_kish_counts(int64_t n, const double *__restrict__ wsum,
 constexpr int kStatBlocks = 512;
 constexpr int kStats = 8;
 __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
+                          const int32_t *__restrict__ cellv, const KdeGrid G,
                           const int32_t *__restrict__ keys, int64_t ncells,
                           const double *__restrict__ gx,
                           const double *__restrict__ gy,
@@ -135,7 +143,7 @@ __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   for (int64_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
-    if (status[p] != 1) continue;
+    if (!kde_takes(status[p], cellv[p], G)) continue;
     if (mode == 0) {
       const double w = mass ? mass[p] : 1.0;
       acc[0] += gx[p];
@@ -274,6 +282,8 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     G.active[d] = opt->nbins[d] > 1;
     nd += G.active[d];
   }
+  G.member = P.members > 1 ? opt->member : -1;
+  G.ncell_member = P.ncell_member;
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
   hipStream_t st = e->stream;
   // scratch: gx gy gz (3n), keys/vals reuse the walk's work lists, two grid
@@ -343,7 +353,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   double hs[kStats] = {0, 0, 0, 0, 0, 0, 0, 0}, dev2[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (n > 0) {
     hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       S.keys[0], ncells, gx, gy, gz, mass_dev, 0,
+                       S.cell, G, S.keys[0], ncells, gx, gy, gz, mass_dev, 0,
                        (const double *)nullptr, partial);
     hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
     if ((err = hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
@@ -358,7 +368,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     if ((err = hipMemcpyAsync(sums + kStats, mean, sizeof mean, hipMemcpyHostToDevice, st)) != hipSuccess)
       return err;
     hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       S.keys[0], ncells, gx, gy, gz, mass_dev, 1,
+                       S.cell, G, S.keys[0], ncells, gx, gy, gz, mass_dev, 1,
                        (const double *)(sums + kStats), partial);
     hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
     if ((err = hipMemcpyAsync(dev2, sums, sizeof dev2, hipMemcpyDeviceToHost, st)) != hipSuccess)

@@ -54,13 +54,16 @@ class RwptKdeOptions(C.Structure):
                 ("bound_kernel_size", C.c_int32), ("min_h_over_delta", C.c_double),
                 ("max_h_over_delta", C.c_double),
                 ("effective_weight_format", C.c_int32),
-                ("kernel_database", C.c_int32), ("delta_h_over_delta", C.c_double)]
+                ("kernel_database", C.c_int32), ("delta_h_over_delta", C.c_double),
+                ("member", C.c_int32)]
 
 
 EXPORTS = {
     "rwpt_default_options": ([C.POINTER(RwptOptions)], None),
     "rwpt_create": ([C.POINTER(RwptGrid), C.POINTER(RwptOptions),
                      C.POINTER(C.c_void_p)], C.c_int),
+    "rwpt_create_ensemble": ([C.POINTER(RwptGrid), C.POINTER(RwptOptions), C.c_int32,
+                              C.POINTER(C.c_void_p)], C.c_int),
     "rwpt_destroy": ([C.c_void_p], C.c_int),
     "rwpt_last_error": ([C.c_void_p], C.c_char_p),
     "rwpt_set_flow": ([C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p,
@@ -166,12 +169,16 @@ class Engine:
     """One particle-tracking engine resident on the current HIP device."""
 
     def __init__(self, problem, precision: int = 0, **opt):
+        """problem: a Problem, or an ensemble from problems.stack (its
+        members share the first member's grid)."""
         P = problem
         self.problem = P
+        self.members = getattr(P, "members", 1)
         self._keep = []
         g = RwptGrid()
         g.ncol, g.nrow, g.nlay = P.ncol, P.nrow, P.nlay
-        arrs = [_d(P.delr), _d(P.delc), _d(P.top), _d(P.bot)]
+        ncm = P.ncol * P.nrow * P.nlay
+        arrs = [_d(P.delr), _d(P.delc), _d(P.top)[:ncm].copy(), _d(P.bot)[:ncm].copy()]
         self._keep += arrs
         g.delr, g.delc, g.top, g.bot = [_p(a, _dp) for a in arrs]
         o = default_options(advection_kind=P.advection_kind,
@@ -181,9 +188,10 @@ class Engine:
                             backward_tracking=P.backward, precision=precision,
                             **opt)
         h = C.c_void_p()
-        rc = lib().rwpt_create(C.byref(g), C.byref(o), C.byref(h))
+        rc = lib().rwpt_create_ensemble(C.byref(g), C.byref(o), int(self.members),
+                                        C.byref(h))
         if rc != 0:
-            raise RwptError(f"rwpt_create failed ({rc})")
+            raise RwptError(f"rwpt_create_ensemble failed ({rc})")
         self.h = h
         self.n = 0
         self.set_flow(P)
@@ -296,7 +304,8 @@ class Engine:
             unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
             rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
             bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
-            effective_weight_format=0, kernel_database=False, delta_h_over_delta=0.0):
+            effective_weight_format=0, kernel_database=False, delta_h_over_delta=0.0,
+            member=-1):
         """gpkde reconstruction of the active particles; the keywords are
         the options of the gpkde input file (ReadGPKDEData,
         ModpathSimulationData.f90:816-1273).  Returns (density, histogram),
@@ -322,6 +331,7 @@ class Engine:
         o.effective_weight_format = effective_weight_format
         o.kernel_database = int(bool(kernel_database))
         o.delta_h_over_delta = delta_h_over_delta
+        o.member = member
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)

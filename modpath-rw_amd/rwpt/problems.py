@@ -231,3 +231,81 @@ def release_in_block(P: Problem, n, cols, rows, lays, seed=1):
 
 BASE_SEED = (0x1DADBEEFBAADD0D0, 0x5BADD0D0DEADBEEF)
 """baseseed of pr_InitializeRandomWalk, TrackSubCell.F90:553-555."""
+
+
+@dataclasses.dataclass
+class Ensemble(Problem):
+    """Members sharing one grid (ncol, nrow, nlay, delr, delc, top, bot of the
+    first member), with their per-cell arrays concatenated member-major as
+    rwpt_create_ensemble expects them; dispersion is one value per member
+    when every member's is uniform."""
+    members: int = 1
+
+    @property
+    def ncell(self) -> int:
+        return self.ncol * self.nrow * self.nlay * self.members
+
+    @property
+    def ncell_member(self) -> int:
+        return self.ncol * self.nrow * self.nlay
+
+    @property
+    def disp_uniform(self) -> int:
+        return 2 if self.aL.size == self.members and self._per_member else 0
+
+    _per_member: bool = True
+
+    def global_cell(self, member, cell):
+        """Member-major cell number of a member's 1-based cell."""
+        return np.asarray(member) * self.ncell_member + np.asarray(cell)
+
+
+def stack(problems):
+    """Ensemble of member problems on the same grid."""
+    p0 = problems[0]
+    for q in problems[1:]:
+        if (q.ncol, q.nrow, q.nlay) != (p0.ncol, p0.nrow, p0.nlay):
+            raise ValueError("ensemble members must share the grid")
+        for name in ("delr", "delc", "top", "bot"):
+            if not np.array_equal(getattr(q, name), getattr(p0, name)):
+                raise ValueError(f"ensemble members must share {name}")
+        for name in ("default_icbound", "disp_model", "advection_kind",
+                     "time_step_kind", "ts_params", "dim_mask", "backward"):
+            if getattr(q, name) != getattr(p0, name):
+                raise ValueError(f"ensemble members must share {name}")
+    n = p0.ncell
+
+    def cat(name, size, fill):
+        parts = []
+        for q in problems:
+            a = getattr(q, name)
+            parts.append(np.full(size, fill) if a is None else np.asarray(a))
+        return np.concatenate(parts)
+
+    per_member = all(q.aL.size == 1 for q in problems)
+
+    def disp(name, fill):
+        if per_member:
+            return np.array([fill if getattr(q, name) is None else
+                             float(np.asarray(getattr(q, name)).ravel()[0])
+                             for q in problems])
+        return np.concatenate([np.full(n, fill) if getattr(q, name) is None else
+                               np.broadcast_to(np.asarray(getattr(q, name), float), (n,))
+                               for q in problems])
+
+    E = Ensemble(
+        ncol=p0.ncol, nrow=p0.nrow, nlay=p0.nlay, delr=p0.delr, delc=p0.delc,
+        top=p0.top, bot=p0.bot, frf=cat("frf", n, 0.0), fff=cat("fff", n, 0.0),
+        flf=cat("flf", n, 0.0), porosity=cat("porosity", n, 0.0),
+        bflow=cat("bflow", 6 * n, 0.0), retardation=cat("retardation", n, 1.0),
+        ibound=cat("ibound", n, 1).astype(np.int32),
+        icbound=cat("icbound", n, 0).astype(np.int32),
+        default_icbound=p0.default_icbound, disp_model=p0.disp_model,
+        aL=disp("aL", 0.0), aT=disp("aT", 0.0),
+        aLV=disp("aLV", 0.0) if p0.disp_model == 1 else None,
+        aTV=disp("aTV", 0.0) if p0.disp_model == 1 else None,
+        dmeff=disp("dmeff", 0.0), advection_kind=p0.advection_kind,
+        time_step_kind=p0.time_step_kind, ts_params=p0.ts_params,
+        dim_mask=p0.dim_mask, backward=p0.backward, members=len(problems))
+    E._per_member = per_member
+    return E
