@@ -23,7 +23,10 @@ reference's TrackSubCell loop iterates, over the walk's wall time), the
 roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
 and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
 and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
-sample of the same workload: the first interval of all particles).  Before
+sample of the same workload: the first interval of all particles).  The
+batched workload (256 ensemble members, about 5.5e7 particles, one walk over
+all members) is timed for --ensemble-intervals intervals and reported as
+ensemble.  Before
 the timed region the plume after the first interval (25 d, still inside the
 grid) is reconstructed with the gpkde bandwidth optimisation (--kde-loops,
 default 10, convergence tests on) and reported as kde_adaptive, beside one
@@ -121,6 +124,45 @@ def adaptive_kde(E, origin, nbins, loops):
     except Exception as exc:  # the GPU figure stands without the CPU one
         out["cpu_reference"] = {"error": str(exc)}
     return out
+
+
+def ensemble_run(intervals, warmup):
+    """The batched workload: 256 ensemble members on one 100x100x10 field,
+    per-member dispersivities and porosity, log-uniform particle counts
+    (1e4..1e6, about 5.5e7 in all), FP32-local, all members in the same
+    walk launches; `intervals` tracking intervals of 50 RW steps timed."""
+    from rwpt import Engine, problems
+    E, counts = problems.config3()
+    rel = problems.release_ensemble(E, counts, (1, 5), (38, 63), (1, 10))
+    eng = Engine(E, precision=1)
+    try:
+        eng.load_particles(*rel)
+        eng.set_time(0.0)
+        for w in range(warmup):
+            eng.advance((w + 1) * INTERVAL_STEPS * DT)
+        eng.load_particles(*rel)
+        eng.set_time(0.0)
+        eng.reset_kernel_time()
+        steps = 0
+        t0 = time.perf_counter()
+        for k in range(intervals):
+            steps += eng.advance((k + 1) * INTERVAL_STEPS * DT)[1]
+        el = time.perf_counter() - t0
+        kms, launches = eng.kernel_time(0)
+        active = int((eng.particles()["status"] == 1).sum())
+    finally:
+        eng.close()
+    return {"unit": "particle-steps/s", "value": steps / el,
+            "workload": "256 members on one seeded 100x100x10 lognormal field, "
+                        "aL ~ U[0.05,1], aT = aL*U[0.05,0.2], porosity ~ U[0.2,0.35] "
+                        "per member, particles log-uniform in [1e4, 1e6] per member, "
+                        "FP32-local, FIXED dt 0.5 d",
+            "members": int(E.members), "particles": int(counts.sum()),
+            "intervals": intervals, "rw_steps_per_interval": INTERVAL_STEPS,
+            "particle_steps": steps, "ms_per_interval": el * 1e3 / intervals,
+            "walk_launches": launches,
+            "walk_kernel_avg_launch_ms": kms / max(launches, 1),
+            "active_after": active}
 
 
 def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
@@ -239,6 +281,9 @@ def main():
                          "walk's draws (default), 2 FP32-local with 4-draw normals")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--ensemble-intervals", type=int, default=1,
+                    help="tracking intervals of the 256-member ensemble timed "
+                         "(0: skip)")
     ap.add_argument("--kde-loops", type=int, default=10,
                     help="optimisation loops of the adaptive gpkde measured on the "
                          "final plume (0: skip)")
@@ -332,6 +377,8 @@ def main():
                        "launches": b0["launches"]}
         out["speedup_vs_fp64"] = value / b0["rate"]
         out["error_vs_fp64"] = errors
+    if args.ensemble_intervals > 0:
+        out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)
     print(json.dumps(out), flush=True)

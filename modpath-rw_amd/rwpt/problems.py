@@ -309,3 +309,35 @@ def stack(problems):
         dim_mask=p0.dim_mask, backward=p0.backward, members=len(problems))
     E._per_member = per_member
     return E
+
+
+def config3(members=256, seed=20240602, n_min=10_000, n_max=1_000_000):
+    """Ensemble of `members` on one seeded 100x100x10 heterogeneous field
+    (the generator of config1): per member aL ~ U[0.05, 1] m,
+    aT = aL * U[0.05, 0.2], a uniform porosity ~ U[0.2, 0.35] and a particle
+    count log-uniform in [n_min, n_max].  Returns (ensemble, counts)."""
+    base = heterogeneous_flow(100, 100, 10, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    aL = rng.uniform(0.05, 1.0, members)
+    aT = aL * rng.uniform(0.05, 0.2, members)
+    phi = rng.uniform(0.2, 0.35, members)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), members)).astype(np.int64)
+    n = base.ncell
+    E = Ensemble(ncol=base.ncol, nrow=base.nrow, nlay=base.nlay, delr=base.delr,
+                 delc=base.delc, top=base.top, bot=base.bot,
+                 frf=np.tile(base.frf, members), fff=np.tile(base.fff, members),
+                 flf=np.tile(base.flf, members), porosity=np.repeat(phi, n),
+                 bflow=np.tile(base.bflow, members), aL=aL, aT=aT,
+                 dmeff=np.full(members, 1e-4), ts_params=(0.5, 0.0),
+                 members=members)
+    return E, counts
+
+
+def release_ensemble(E, counts, cols, rows, lays, seed=1):
+    """counts[m] particles of member m uniformly inside the block of cells,
+    member-major (the order of the members' rng stream offsets)."""
+    parts = []
+    for m, n in enumerate(counts):
+        c, x, y, z = release_in_block(E, int(n), cols, rows, lays, seed=seed + m)
+        parts.append((E.global_cell(m, c).astype(np.int32), x, y, z))
+    return tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
