@@ -140,6 +140,7 @@ def ensemble_run(intervals, warmup):
         eng.set_time(0.0)
         for w in range(warmup):
             eng.advance((w + 1) * INTERVAL_STEPS * DT)
+        tune = eng.autotune(warmup * INTERVAL_STEPS * DT + 10 * DT)
         eng.load_particles(*rel)
         eng.set_time(0.0)
         eng.reset_kernel_time()
@@ -162,7 +163,7 @@ def ensemble_run(intervals, warmup):
             "particle_steps": steps, "ms_per_interval": el * 1e3 / intervals,
             "walk_launches": launches,
             "walk_kernel_avg_launch_ms": kms / max(launches, 1),
-            "active_after": active}
+            "active_after": active, "tuning": tune}
 
 
 def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
@@ -179,15 +180,21 @@ def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
         E.load_particles(cell, lx, ly, lz)
         E.set_time(0.0)
         warm_steps = 0
+        t_now = 0.0
         for w in range(warmup):
-            warm_steps += E.advance((w + 1) * INTERVAL_STEPS * DT)[1]
+            t_now = (w + 1) * INTERVAL_STEPS * DT
+            warm_steps += E.advance(t_now)[1]
         adaptive = None
         if adaptive_loops > 0:
             # the plume after the first tracking interval, still inside the grid
             E.load_particles(cell, lx, ly, lz)
             E.set_time(0.0)
-            warm_steps += E.advance(INTERVAL_STEPS * DT)[1]
+            t_now = INTERVAL_STEPS * DT
+            warm_steps += E.advance(t_now)[1]
             adaptive = adaptive_kde(E, origin, nbins, adaptive_loops)
+        # launch shape from the autotuner, probed on the plume at hand
+        tune = E.autotune(t_now + INTERVAL_STEPS * DT)
+        warm_steps += tune["trial_substeps"]
         warm_ms, warm_launches = E.kernel_time(0)
         E.load_particles(cell, lx, ly, lz)
         E.set_time(0.0)
@@ -213,7 +220,7 @@ def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
         active = int((E.particles()["status"] == 1).sum())
     finally:
         E.close()
-    return {"adaptive": adaptive,
+    return {"adaptive": adaptive, "tuning": tune,
             "steps": total_steps, "walk_s": walk_s, "kde_s": kde_s, "elapsed": el,
             "kernel_ms": kms, "launches": launches, "hist_ms": hms, "dens_ms": dms,
             "bandwidth": bw, "dens": dens, "active": active,
@@ -357,6 +364,7 @@ def main():
                      "timed_avg_launch_ms": r["kernel_ms"] / max(r["launches"], 1),
                      "algorithmic_bytes_per_step": BYTES_PER_STEP[prec],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "tuning": r["tuning"],
         "kde": {"unit": "KDE cells/s", "grid": list(nbins),
                 "reconstructions": args.steps,
                 "cells_per_s_wall": kde_cells * args.steps / r["kde_s"],

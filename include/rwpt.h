@@ -211,6 +211,19 @@ int rwpt_set_time(rwpt_engine *e, double t);
  * Defaults 64, 256, 1, 1. */
 int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
                     int32_t sort, int32_t min_waves);
+/* Launch-shape autotuner (no counterpart in the reference): times trial
+ * advances from the current state to t_probe over workgroup size and
+ * occupancy variant, per-launch iteration budget and work-list sort (a
+ * staged search of rwpt_set_tuning's space), restores the particle state
+ * and engine time, and keeps the fastest shape.  Results never depend on
+ * the shape (the walk is bit-identical under every tuning), only the time.
+ * Any output pointer may be NULL; ms_best is the wall time of the best
+ * trial, trial_substeps the RW sub-steps of all trials (their walk
+ * launches count in rwpt_kernel_time like those of rwpt_advance).  Needs
+ * twice the particle state in device memory while it runs. */
+int rwpt_autotune(rwpt_engine *e, double t_probe, int32_t *budget, int32_t *block,
+                  int32_t *sort, int32_t *min_waves, double *ms_best,
+                  int64_t *trial_substeps);
 /* Walk launches the last rwpt_advance used. */
 int rwpt_last_launches(const rwpt_engine *e);
 

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -486,6 +487,156 @@ int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
   e->tune.budget = budget;
   e->tune.block = block;
   e->tune.sort = sort ? 1 : 0;
+  return RWPT_OK;
+}
+
+namespace {
+
+// Device copy of every particle array an advance writes, so that trial
+// advances can start from the same state.
+struct StateCopy {
+  std::vector<std::pair<void *, size_t>> arrays;  // (device array, bytes)
+  char *buf = nullptr;
+  size_t bytes = 0;
+  hipError_t save(rwpt_engine *e) {
+    Particles &S = e->part;
+    const size_t n = (size_t)S.n;
+    arrays = {{S.cell, 4 * n},  {S.status, 4 * n}, {S.lx, 8 * n},  {S.ly, 8 * n},
+              {S.lz, 8 * n},    {S.t, 8 * n},      {S.rng, 16 * n}, {S.substeps, 8 * n},
+              {S.aux, 4 * n},   {S.ix, 8 * n},     {S.iy, 8 * n},   {S.iz, 8 * n},
+              {S.t0, 8 * n}};
+    if (S.flx)
+      for (float *f : {S.flx, S.fly, S.flz, S.fix, S.fiy, S.fiz}) arrays.push_back({f, 4 * n});
+    bytes = 0;
+    for (auto &a : arrays)
+      if (a.first) bytes += a.second;
+    hipError_t err = hipMalloc(&buf, bytes);
+    size_t off = 0;
+    for (auto &a : arrays) {
+      if (err != hipSuccess) break;
+      if (!a.first) continue;
+      err = hipMemcpyAsync(buf + off, a.first, a.second, hipMemcpyDeviceToDevice, e->stream);
+      off += a.second;
+    }
+    return err;
+  }
+  hipError_t restore(rwpt_engine *e) {
+    size_t off = 0;
+    hipError_t err = hipSuccess;
+    for (auto &a : arrays) {
+      if (err != hipSuccess) break;
+      if (!a.first) continue;
+      err = hipMemcpyAsync(a.first, buf + off, a.second, hipMemcpyDeviceToDevice, e->stream);
+      off += a.second;
+    }
+    return err;
+  }
+  ~StateCopy() {
+    if (buf) (void)hipFree(buf);
+  }
+};
+
+}  // namespace
+
+int rwpt_autotune(rwpt_engine *e, double t_probe, int32_t *budget, int32_t *block,
+                  int32_t *sort, int32_t *min_waves, double *ms_best,
+                  int64_t *trial_substeps) {
+  if (!e) return RWPT_E_ARG;
+  if (!std::isfinite(t_probe) || t_probe == e->t_engine)
+    return fail(e, RWPT_E_ARG, "t_probe must differ from the engine time");
+  int rc = ensure_tables(e);
+  if (rc != RWPT_OK) return rc;
+  if (e->part.n == 0) return fail(e, RWPT_E_STATE, "no particles loaded");
+  StateCopy snap;
+  hipError_t err = snap.save(e);
+  if (err != hipSuccess) return hip_fail(e, err, "autotune snapshot");
+  const double t0 = e->t_engine;
+  const Tuning keep = e->tune;
+  // one trial advance from the snapshot, wall time of the whole launch loop;
+  // its walk launches count in the advance kernel's timing like any others
+  int64_t steps = 0;
+  auto trial = [&](const Tuning &T, double *ms) -> hipError_t {
+    hipError_t r = snap.restore(e);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    if (r != hipSuccess) return r;
+    e->tune = T;
+    e->t_engine = t0;
+    const auto a = std::chrono::steady_clock::now();
+    r = launch_advance(e, t_probe);
+    if (r == hipSuccess && e->opt.precision != 0) r = launch_positions_from32(e);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    const auto b = std::chrono::steady_clock::now();
+    *ms = std::chrono::duration<double, std::milli>(b - a).count();
+    unsigned long long h[2] = {0, 0};
+    if (r == hipSuccess)
+      r = hipMemcpy(h, e->counters, sizeof h, hipMemcpyDeviceToHost);
+    steps += (int64_t)h[1];
+    for (int64_t k = 0; r == hipSuccess && k < e->last_launches; ++k) {
+      float km = 0.f;
+      (void)hipEventElapsedTime(&km, e->walk_events[2 * k], e->walk_events[2 * k + 1]);
+      e->timers[0].total_ms += km;
+      e->timers[0].launches += 1;
+    }
+    return r;
+  };
+  // each shape is timed twice and keeps its faster run
+  auto timed = [&](const Tuning &T, double *ms) -> hipError_t {
+    double a = 0.0, b = 0.0;
+    hipError_t r = trial(T, &a);
+    if (r == hipSuccess) r = trial(T, &b);
+    *ms = std::min(a, b);
+    return r;
+  };
+  Tuning best = keep;
+  double bestMs = 0.0;
+  err = trial(best, &bestMs);  // warm-up (first launches compile and cache)
+  if (err == hipSuccess) err = timed(best, &bestMs);
+  auto consider = [&](Tuning T) -> hipError_t {
+    double ms = 0.0;
+    hipError_t r = timed(T, &ms);
+    if (r == hipSuccess && ms < bestMs) {
+      bestMs = ms;
+      best = T;
+    }
+    return r;
+  };
+  // staged search: workgroup size and occupancy variant, then the
+  // per-launch iteration budget, then the cell sort of the work list
+  for (int bl : {64, 128, 256})
+    for (int mw : {1, 3, 4}) {
+      if (err != hipSuccess) break;
+      Tuning T = best;
+      T.block = bl;
+      T.min_waves = mw;
+      err = consider(T);
+    }
+  for (int bu : {16, 32, 64, 128, 256}) {
+    if (err != hipSuccess) break;
+    Tuning T = best;
+    T.budget = bu;
+    err = consider(T);
+  }
+  for (int so : {0, 1}) {
+    if (err != hipSuccess) break;
+    Tuning T = best;
+    T.sort = so;
+    err = consider(T);
+  }
+  // back to the state before the call, with the tuning found
+  if (err == hipSuccess) err = snap.restore(e);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  e->t_engine = t0;
+  if (err != hipSuccess) {
+    e->tune = keep;
+    return hip_fail(e, err, "autotune");
+  }
+  e->tune = best;
+  if (budget) *budget = best.budget;
+  if (block) *block = best.block;
+  if (sort) *sort = best.sort;
+  if (min_waves) *min_waves = best.min_waves;
+  if (ms_best) *ms_best = bestMs;
+  if (trial_substeps) *trial_substeps = steps;
   return RWPT_OK;
 }
 

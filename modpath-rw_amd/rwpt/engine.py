@@ -82,6 +82,8 @@ EXPORTS = {
                         C.c_int),
     "rwpt_last_launches": ([C.c_void_p], C.c_int),
     "rwpt_get_substeps": ([C.c_void_p, _i64p], C.c_int),
+    "rwpt_autotune": ([C.c_void_p, C.c_double, _i32p, _i32p, _i32p, _i32p, _dp,
+                       _i64p], C.c_int),
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
     "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp, _i32p], C.c_int),
@@ -250,6 +252,17 @@ class Engine:
     def set_tuning(self, budget=64, block=256, sort=True, min_waves=1):
         self._check(lib().rwpt_set_tuning(self.h, int(budget), int(block), int(sort),
                                           int(min_waves)), "rwpt_set_tuning")
+
+    def autotune(self, t_probe):
+        """rwpt_autotune: keeps the fastest launch shape for advances like
+        the one to t_probe; the particle state is left as it was."""
+        v = [C.c_int32(0) for _ in range(4)]
+        ms = C.c_double(0)
+        steps = C.c_int64(0)
+        self._check(lib().rwpt_autotune(self.h, float(t_probe), *[C.byref(x) for x in v],
+                                        C.byref(ms), C.byref(steps)), "rwpt_autotune")
+        return {"budget": v[0].value, "block": v[1].value, "sort": v[2].value,
+                "min_waves": v[3].value, "ms": ms.value, "trial_substeps": steps.value}
 
     def last_launches(self):
         return lib().rwpt_last_launches(self.h)
