@@ -126,6 +126,22 @@ def adaptive_kde(E, origin, nbins, loops):
     return out
 
 
+TUNED_NAME = {0: "config1_fp64", 1: "config1_fp32", 2: "config1_fp32n4"}
+
+
+def apply_tuning(E, name):
+    """The walk's launch shape for this workload as the autotuner
+    (python -m rwpt.autotune, rwpt_autotune) stored it; defaults when the
+    workload has no entry."""
+    from rwpt import autotune
+    t = autotune.tuned(name)
+    if t is None:
+        return {"source": "defaults"}
+    E.set_tuning(budget=t["budget"], block=t["block"], sort=bool(t["sort"]),
+                 min_waves=t["min_waves"])
+    return dict(t, source="modpath-rw_amd/rwpt/tuning.json (rwpt_autotune)")
+
+
 def ensemble_run(intervals, warmup):
     """The batched workload: 256 ensemble members on one 100x100x10 field,
     per-member dispersivities and porosity, log-uniform particle counts
@@ -135,12 +151,12 @@ def ensemble_run(intervals, warmup):
     E, counts = problems.config3()
     rel = problems.release_ensemble(E, counts, (1, 5), (38, 63), (1, 10))
     eng = Engine(E, precision=1)
+    tune = apply_tuning(eng, "ensemble256_fp32")
     try:
         eng.load_particles(*rel)
         eng.set_time(0.0)
         for w in range(warmup):
             eng.advance((w + 1) * INTERVAL_STEPS * DT)
-        tune = eng.autotune(warmup * INTERVAL_STEPS * DT + 10 * DT)
         eng.load_particles(*rel)
         eng.set_time(0.0)
         eng.reset_kernel_time()
@@ -173,6 +189,7 @@ def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
     first interval is timed before the timed region."""
     from rwpt import Engine
     E = Engine(P, precision=precision)
+    tune = apply_tuning(E, TUNED_NAME[precision])
     cell, lx, ly, lz = rel
     nbins = (P.ncol, P.nrow, P.nlay)
     origin = (0.0, 0.0, float(P.bot.min()))
@@ -192,9 +209,6 @@ def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
             t_now = INTERVAL_STEPS * DT
             warm_steps += E.advance(t_now)[1]
             adaptive = adaptive_kde(E, origin, nbins, adaptive_loops)
-        # launch shape from the autotuner, probed on the plume at hand
-        tune = E.autotune(t_now + INTERVAL_STEPS * DT)
-        warm_steps += tune["trial_substeps"]
         warm_ms, warm_launches = E.kernel_time(0)
         E.load_particles(cell, lx, ly, lz)
         E.set_time(0.0)
@@ -239,6 +253,7 @@ def fp32_errors(P, rel, precision, runs, intervals=2):
     snaps = {}
     for prec in (0, precision):
         E = Engine(P, precision=prec)
+        apply_tuning(E, TUNED_NAME[prec])
         try:
             E.load_particles(*rel)
             snaps[prec] = []
