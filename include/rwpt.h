@@ -271,6 +271,45 @@ int rwpt_kde_write(const char *path, int32_t append, int32_t format,
                    const double *density, const double *histogram,
                    double histogram_scale);
 
+/* MODPATH-RW's timeseries file (no counterpart needs an engine there: the
+ * reference writes it from its ParticleType array): the header of
+ * WriteTimeseriesHeader (ParticleManager.F90:208-226), which replaces the
+ * file, and the records of WriteTimeseriesRecord (:229-250) for the
+ * engine's particles at one time point, appended in particle order:
+ * (2I8,ES18.9E3,I10,I5,2I10,6ES18.9E3,I10) = time point index, time step,
+ * time, sequence number, group, particle ID, cell, local x y z, model x y,
+ * global z, layer.  sequence, id and group hold one value per particle;
+ * select (NULL: the particles with status 1) picks the records written.
+ * Cell numbers and layers are those of the particle's ensemble member. */
+int rwpt_write_timeseries_header(const char *path, int32_t tracking_direction,
+                                 double reference_time, double origin_x,
+                                 double origin_y, double rotation);
+int rwpt_write_timeseries(rwpt_engine *e, const char *path, int32_t time_point,
+                          int32_t time_step, double time, const int32_t *sequence,
+                          const int32_t *id, const int32_t *group,
+                          const uint8_t *select);
+
+/* MODPATH-RW's endpoint file for the engine's particles: format 0
+ * WriteEndpoints (MODPATH_ENDPOINT_FILE 7 2, ParticleManager.F90:51-125),
+ * format 1 WriteEndpointsMassParticles (MODPATHRW_ENDPOINT_FILE 1 0,
+ * :128-205).  Particles are listed group by group as the engine holds them;
+ * statuses 0 and 8 are counted but not written.  group (1-based),
+ * sequence, id and the initial cell, local coordinates, time and face
+ * (format 0) hold one value per particle; group_names (written padded to
+ * 20 characters) and group_solute (format 1) one per group; mass (format
+ * 1) one per particle.  Zones are those of rwpt_set_cell_extras (1 when
+ * none were set).  Replaces the file. */
+int rwpt_write_endpoints(rwpt_engine *e, const char *path, int32_t format,
+                         int32_t tracking_direction, double reference_time,
+                         double origin_x, double origin_y, double rotation,
+                         int32_t n_groups, const char *const *group_names,
+                         const int32_t *group_solute, const int32_t *group,
+                         const int32_t *sequence, const int32_t *id,
+                         const int32_t *initial_cell, const double *initial_lx,
+                         const double *initial_ly, const double *initial_lz,
+                         const double *initial_time, const int32_t *initial_face,
+                         const double *mass);
+
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,
  * corner discharge qx qy qz |q| (8 x 4), corner porosity (8), corner phi*D

@@ -112,3 +112,307 @@ extern "C" int rwpt_kde_write(const char *path, int32_t append, int32_t format,
   if (std::fclose(f) != 0 && rc == RWPT_OK) rc = RWPT_E_IO;
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Timeseries and endpoint files (ParticleManager.F90:51-250), written from the
+// engine's particle state.  Cell numbers and layers are those of the particle's
+// own ensemble member (the same numbers for one member).
+
+#include <vector>
+
+#include "rwpt_internal.h"
+
+namespace {
+
+// Fortran Ew.dEe with w = 18, d = 10, e = 2 (the `e18.10` of the headers):
+// 0.dddddddddd mantissa, E, sign, two exponent digits, right-aligned.
+void put_e18_10(std::string &out, double v) {
+  char f[64];
+  if (v == 0.0) {
+    std::snprintf(f, sizeof f, "%18s", "0.0000000000E+00");
+  } else {
+    char m[48];
+    std::snprintf(m, sizeof m, "%.9E", v);  // d.dddddddddE+xx: 10 digits
+    // 0.dddddddddd x 10^(x+1): the digits of d.ddddddddd with the point moved
+    const char *p = m;
+    const bool neg = *p == '-';
+    if (neg) ++p;
+    char digits[16];
+    int nd = 0;
+    for (; *p && *p != 'E'; ++p)
+      if (*p != '.') digits[nd++] = *p;
+    digits[nd] = '\0';
+    const int ex = std::atoi(p + 1) + 1;
+    std::snprintf(f, sizeof f, "%s0.%sE%c%02d", neg ? "-" : "", digits, ex < 0 ? '-' : '+',
+                  ex < 0 ? -ex : ex);
+    char g[64];
+    std::snprintf(g, sizeof g, "%18s", f);
+    out += g;
+    return;
+  }
+  out += f;
+}
+
+void put_i(std::string &out, int64_t v, int w) {
+  char f[32];
+  std::snprintf(f, sizeof f, "%*lld", w, (long long)v);
+  out += f;
+}
+
+struct HostGrid {
+  int ncol = 0, nrow = 0;
+  int64_t ncm = 0;
+  std::vector<double> xf, yf, delr, delc, top, bot;
+  std::vector<int32_t> zone;
+  bool load(rwpt_engine *e) {
+    rwpt::DeviceProblem &P = e->prob;
+    ncol = P.ncol;
+    nrow = P.nrow;
+    ncm = P.ncell_member;
+    xf.resize(ncol + 1);
+    yf.resize(nrow + 1);
+    delr.resize(ncol);
+    delc.resize(nrow);
+    top.resize(P.ncell);
+    bot.resize(P.ncell);
+    hipError_t err = hipMemcpy(xf.data(), e->xface, xf.size() * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(yf.data(), e->yface, yf.size() * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(delr.data(), P.delr, ncol * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(delc.data(), P.delc, nrow * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(top.data(), P.top, P.ncell * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(bot.data(), P.bot, P.ncell * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess && P.zone) {
+      zone.resize(P.ncell);
+      err = hipMemcpy(zone.data(), P.zone, P.ncell * 4, hipMemcpyDeviceToHost);
+    }
+    return err == hipSuccess;
+  }
+  // ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549); c is 1-based
+  void xyz(int32_t c, double x, double y, double z, double out[3]) const {
+    const int64_t c0 = c - 1;
+    const int64_t lc = (int64_t)nrow * ncol, r = c0 % lc;
+    const int i = (int)(r / ncol), j = (int)(r % ncol);
+    const double cx = (xf[j] + xf[j + 1]) / 2.0, cy = (yf[i] + yf[i + 1]) / 2.0;
+    const double left = cx - (delr[j] / 2.0), right = cx + (delr[j] / 2.0);
+    const double front = cy - (delc[i] / 2.0), back = cy + (delc[i] / 2.0);
+    const double b = bot[c0], t = top[c0];
+    out[0] = std::fmin(std::fmax(((1.0 - x) * left) + (x * right), left), right);
+    out[1] = std::fmin(std::fmax(((1.0 - y) * front) + (y * back), front), back);
+    out[2] = std::fmin(std::fmax(((1.0 - z) * b) + (z * t), b), t);
+  }
+  int32_t local_cell(int32_t c) const { return (int32_t)((c - 1) % ncm + 1); }
+  int32_t layer(int32_t c) const {
+    return (int32_t)(((c - 1) % ncm) / ((int64_t)nrow * ncol) + 1);
+  }
+  int32_t zone_of(int32_t c) const { return zone.empty() ? 1 : zone[c - 1]; }
+};
+
+struct HostParticles {
+  std::vector<int32_t> cell, status;
+  std::vector<double> lx, ly, lz, t;
+  bool load(rwpt_engine *e) {
+    rwpt::Particles &S = e->part;
+    const size_t n = (size_t)S.n;
+    cell.resize(n);
+    status.resize(n);
+    lx.resize(n);
+    ly.resize(n);
+    lz.resize(n);
+    t.resize(n);
+    if (n == 0) return true;
+    hipError_t err = hipStreamSynchronize(e->stream);
+    if (err == hipSuccess) err = hipMemcpy(cell.data(), S.cell, n * 4, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(status.data(), S.status, n * 4, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(lx.data(), S.lx, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(ly.data(), S.ly, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(lz.data(), S.lz, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(t.data(), S.t, n * 8, hipMemcpyDeviceToHost);
+    return err == hipSuccess;
+  }
+};
+
+int write_all(const char *path, bool append, const std::string &buf) {
+  FILE *f = std::fopen(path, append ? "ab" : "wb");
+  if (!f) return RWPT_E_IO;
+  int rc = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size() ? RWPT_OK : RWPT_E_IO;
+  if (std::fclose(f) != 0) rc = RWPT_E_IO;
+  return rc;
+}
+
+// FindFace (ParticleManager.F90:665-690)
+int find_face(double x, double y, double z) {
+  if (x == 0.0) return 1;
+  if (x == 1.0) return 2;
+  if (y == 0.0) return 3;
+  if (y == 1.0) return 4;
+  if (z == 0.0) return 5;
+  if (z == 1.0) return 6;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int rwpt_write_timeseries_header(const char *path, int32_t tracking_direction,
+                                            double reference_time, double origin_x,
+                                            double origin_y, double rotation) {
+  if (!path) return RWPT_E_ARG;
+  // WriteTimeseriesHeader (ParticleManager.F90:208-226)
+  std::string b = "MODPATH_TIMESERIES_FILE";
+  put_i(b, 7, 10);
+  put_i(b, 2, 10);
+  b += '\n';
+  put_i(b, tracking_direction, 5);
+  b += ' ';
+  put_e18_10(b, reference_time);
+  put_e18_10(b, origin_x);
+  put_e18_10(b, origin_y);
+  put_e18_10(b, rotation);
+  b += "\nEND HEADER\n";
+  return write_all(path, false, b);
+}
+
+extern "C" int rwpt_write_timeseries(rwpt_engine *e, const char *path, int32_t time_point,
+                                     int32_t time_step, double time,
+                                     const int32_t *sequence, const int32_t *id,
+                                     const int32_t *group, const uint8_t *select) {
+  if (!e || !path || !sequence || !id || !group) return RWPT_E_ARG;
+  HostGrid G;
+  HostParticles H;
+  if (!G.load(e) || !H.load(e)) return RWPT_E_DEVICE;
+  std::string b;
+  for (size_t p = 0; p < H.cell.size(); ++p) {
+    if (select ? !select[p] : H.status[p] != 1) continue;
+    double g[3];
+    G.xyz(H.cell[p], H.lx[p], H.ly[p], H.lz[p], g);
+    // WriteTimeseriesRecord (ParticleManager.F90:229-250):
+    // (2I8,es18.9e3,i10,i5,2i10,6es18.9e3,i10)
+    put_i(b, time_point, 8);
+    put_i(b, time_step, 8);
+    put_es18_9e3(b, time);
+    put_i(b, sequence[p], 10);
+    put_i(b, group[p], 5);
+    put_i(b, id[p], 10);
+    put_i(b, G.local_cell(H.cell[p]), 10);
+    put_es18_9e3(b, H.lx[p]);
+    put_es18_9e3(b, H.ly[p]);
+    put_es18_9e3(b, H.lz[p]);
+    put_es18_9e3(b, g[0]);
+    put_es18_9e3(b, g[1]);
+    put_es18_9e3(b, g[2]);
+    put_i(b, G.layer(H.cell[p]), 10);
+    b += '\n';
+  }
+  return write_all(path, true, b);
+}
+
+extern "C" int rwpt_write_endpoints(
+    rwpt_engine *e, const char *path, int32_t format, int32_t tracking_direction,
+    double reference_time, double origin_x, double origin_y, double rotation,
+    int32_t n_groups, const char *const *group_names, const int32_t *group_solute,
+    const int32_t *group, const int32_t *sequence, const int32_t *id,
+    const int32_t *initial_cell, const double *initial_lx, const double *initial_ly,
+    const double *initial_lz, const double *initial_time, const int32_t *initial_face,
+    const double *mass) {
+  if (!e || !path || (format != 0 && format != 1) || n_groups < 0 || !group ||
+      !sequence || !id || !initial_cell || !initial_lx || !initial_ly || !initial_lz ||
+      !initial_time || (n_groups > 0 && !group_names))
+    return RWPT_E_ARG;
+  if (format == 0 && !initial_face) return RWPT_E_ARG;
+  if (format == 1 && (!group_solute || !mass)) return RWPT_E_ARG;
+  HostGrid G;
+  HostParticles H;
+  if (!G.load(e) || !H.load(e)) return RWPT_E_DEVICE;
+  const size_t n = H.cell.size();
+  // status sums, release count, largest released ID (:67-86)
+  int64_t sums[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t maxId = 0;
+  for (size_t p = 0; p < n; ++p) {
+    const int s = H.status[p];
+    if (s >= 0 && s <= 9) {
+      sums[s] += 1;
+      if (s != 0 && s != 8 && id[p] > maxId) maxId = id[p];
+    }
+  }
+  const int64_t total = (int64_t)n, released = total - sums[0] - sums[8];
+  std::string b = format == 0 ? "MODPATH_ENDPOINT_FILE" : "MODPATHRW_ENDPOINT_FILE";
+  put_i(b, format == 0 ? 7 : 1, 10);
+  put_i(b, format == 0 ? 2 : 0, 10);
+  b += '\n';
+  put_i(b, tracking_direction, 5);
+  put_i(b, total, 10);
+  put_i(b, released, 10);
+  put_i(b, maxId, 10);
+  b += ' ';
+  put_e18_10(b, reference_time);
+  put_e18_10(b, origin_x);
+  put_e18_10(b, origin_y);
+  put_e18_10(b, rotation);
+  b += '\n';
+  for (int s = 0; s < 10; ++s) put_i(b, sums[s], 8);
+  b += '\n';
+  put_i(b, n_groups, 10);
+  b += '\n';
+  for (int g = 0; g < n_groups; ++g) {
+    // ParticleGroupType%Name is character(len=20): padded or cut to 20
+    std::string nm = group_names[g] ? group_names[g] : "";
+    nm.resize(20, ' ');
+    b += nm;
+    b += '\n';
+  }
+  b += "END HEADER\n";
+  for (size_t p = 0; p < n; ++p) {
+    if (H.status[p] == 0 || H.status[p] == 8) continue;
+    double gi[3], gf[3];
+    G.xyz(initial_cell[p], initial_lx[p], initial_ly[p], initial_lz[p], gi);
+    G.xyz(H.cell[p], H.lx[p], H.ly[p], H.lz[p], gf);
+    const int32_t c0 = G.local_cell(initial_cell[p]), c1 = G.local_cell(H.cell[p]);
+    if (format == 0) {
+      // WriteEndpoints (:100-121):
+      // (3i10,i5,2es18.9e3,i10,i5,6es18.9e3,2i5,i10,i5,6es18.9e3,2i5)
+      put_i(b, sequence[p], 10);
+      put_i(b, group[p], 10);
+      put_i(b, id[p], 10);
+      put_i(b, H.status[p], 5);
+      put_es18_9e3(b, initial_time[p]);
+      put_es18_9e3(b, H.t[p]);
+      put_i(b, c0, 10);
+      put_i(b, G.layer(initial_cell[p]), 5);
+      put_es18_9e3(b, initial_lx[p]);
+      put_es18_9e3(b, initial_ly[p]);
+      put_es18_9e3(b, initial_lz[p]);
+      for (int d = 0; d < 3; ++d) put_es18_9e3(b, gi[d]);
+      put_i(b, G.zone_of(initial_cell[p]), 5);
+      put_i(b, initial_face[p], 5);
+      put_i(b, c1, 10);
+      put_i(b, G.layer(H.cell[p]), 5);
+      put_es18_9e3(b, H.lx[p]);
+      put_es18_9e3(b, H.ly[p]);
+      put_es18_9e3(b, H.lz[p]);
+      for (int d = 0; d < 3; ++d) put_es18_9e3(b, gf[d]);
+      put_i(b, G.zone_of(H.cell[p]), 5);
+      put_i(b, find_face(H.lx[p], H.ly[p], H.lz[p]), 5);
+    } else {
+      // WriteEndpointsMassParticles (:180-200):
+      // (4i10,i5,3es18.9e3,i10,i5,3es18.9e3,1i5,i10,i5,3es18.9e3,1i5)
+      put_i(b, sequence[p], 10);
+      put_i(b, group[p], 10);
+      const int32_t gidx = group[p] - 1;
+      put_i(b, gidx >= 0 && gidx < n_groups ? group_solute[gidx] : 0, 10);
+      put_i(b, id[p], 10);
+      put_i(b, H.status[p], 5);
+      put_es18_9e3(b, mass[p]);
+      put_es18_9e3(b, initial_time[p]);
+      put_es18_9e3(b, H.t[p]);
+      put_i(b, c0, 10);
+      put_i(b, G.layer(initial_cell[p]), 5);
+      for (int d = 0; d < 3; ++d) put_es18_9e3(b, gi[d]);
+      put_i(b, G.zone_of(initial_cell[p]), 5);
+      put_i(b, c1, 10);
+      put_i(b, G.layer(H.cell[p]), 5);
+      for (int d = 0; d < 3; ++d) put_es18_9e3(b, gf[d]);
+      put_i(b, G.zone_of(H.cell[p]), 5);
+    }
+    b += '\n';
+  }
+  return write_all(path, false, b);
+}

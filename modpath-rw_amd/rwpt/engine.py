@@ -87,6 +87,14 @@ EXPORTS = {
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
     "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp, _i32p], C.c_int),
+    "rwpt_write_timeseries_header": ([C.c_char_p, C.c_int32, C.c_double, C.c_double,
+                                      C.c_double, C.c_double], C.c_int),
+    "rwpt_write_timeseries": ([C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_double,
+                               _i32p, _i32p, _i32p, C.POINTER(C.c_uint8)], C.c_int),
+    "rwpt_write_endpoints": ([C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_double,
+                              C.c_double, C.c_double, C.c_double, C.c_int32,
+                              C.POINTER(C.c_char_p), _i32p, _i32p, _i32p, _i32p, _i32p,
+                              _dp, _dp, _dp, _dp, _i32p, _dp], C.c_int),
     "rwpt_kde_write": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                         C.c_double, C.c_int32, C.POINTER(RwptKdeOptions), _dp, _dp,
                         C.c_double], C.c_int),
@@ -145,6 +153,15 @@ def default_options(**kw) -> RwptOptions:
         else:
             setattr(o, k, v)
     return o
+
+
+def write_timeseries_header(path, tracking_direction, reference_time, origin, rotation):
+    """Replaces path with MODPATH-RW's timeseries header."""
+    rc = lib().rwpt_write_timeseries_header(os.fsencode(path), int(tracking_direction),
+                                            float(reference_time), float(origin[0]),
+                                            float(origin[1]), float(rotation))
+    if rc != 0:
+        raise RwptError(f"rwpt_write_timeseries_header failed ({rc})")
 
 
 def kde_write(path, origin, bin_size, nbins, density, histogram, time_id=1, time=0.0,
@@ -231,6 +248,16 @@ class Engine:
                                        *[_p(x, _dp) for x in a])
         self._check(rc, "rwpt_set_dispersion")
 
+    def set_cell_extras(self, head=None, layer_type=None, zone=None, source_flow=None,
+                        sink_flow=None):
+        """rwpt_set_cell_extras: heads and layer types of convertible layers,
+        zones, weak source and sink flows (each None or one value per cell)."""
+        a = [_d(head), _i(layer_type), _i(zone), _d(source_flow), _d(sink_flow)]
+        rc = lib().rwpt_set_cell_extras(self.h, _p(a[0], _dp), _p(a[1], _i32p),
+                                        _p(a[2], _i32p), _p(a[3], _dp), _p(a[4], _dp))
+        self._check(rc, "rwpt_set_cell_extras")
+        self._keep_extras = a
+
     def load_particles(self, cell, lx, ly, lz, t=None, status=None, rng=None,
                        stream_offset=0):
         n = int(np.asarray(cell).size)
@@ -263,6 +290,34 @@ class Engine:
                                         C.byref(ms), C.byref(steps)), "rwpt_autotune")
         return {"budget": v[0].value, "block": v[1].value, "sort": v[2].value,
                 "min_waves": v[3].value, "ms": ms.value, "trial_substeps": steps.value}
+
+    def write_timeseries(self, path, time_point, time_step, time, sequence, ids, group,
+                         select=None):
+        """Appends the timeseries records of the particles (rwpt_write_timeseries)."""
+        seq, idv, grp = _i(sequence), _i(ids), _i(group)
+        sel = None if select is None else np.ascontiguousarray(select, dtype=np.uint8)
+        rc = lib().rwpt_write_timeseries(
+            self.h, os.fsencode(path), int(time_point), int(time_step), float(time),
+            _p(seq, _i32p), _p(idv, _i32p), _p(grp, _i32p),
+            None if sel is None else sel.ctypes.data_as(C.POINTER(C.c_uint8)))
+        self._check(rc, "rwpt_write_timeseries")
+
+    def write_endpoints(self, path, fmt, tracking_direction, reference_time, origin,
+                        rotation, group_names, group_solute, group, sequence, ids,
+                        initial_cell, initial_lx, initial_ly, initial_lz, initial_time,
+                        initial_face=None, mass=None):
+        """Writes the endpoint file (rwpt_write_endpoints)."""
+        names = (C.c_char_p * len(group_names))(*[n.encode() for n in group_names])
+        arr_i = [_i(group_solute), _i(group), _i(sequence), _i(ids), _i(initial_cell)]
+        arr_d = [_d(initial_lx), _d(initial_ly), _d(initial_lz), _d(initial_time)]
+        face = None if initial_face is None else _i(initial_face)
+        m = None if mass is None else _d(mass)
+        rc = lib().rwpt_write_endpoints(
+            self.h, os.fsencode(path), int(fmt), int(tracking_direction),
+            float(reference_time), float(origin[0]), float(origin[1]), float(rotation),
+            len(group_names), names, *[_p(a, _i32p) for a in arr_i],
+            *[_p(a, _dp) for a in arr_d], _p(face, _i32p), _p(m, _dp))
+        self._check(rc, "rwpt_write_endpoints")
 
     def last_launches(self):
         return lib().rwpt_last_launches(self.h)

@@ -143,3 +143,45 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
     if export_vars:
         out = out + (loops,)
     return out + (exported,) if export is not None else out
+
+
+WRITERS_BINARY = os.path.join(HERE, "_ref", "writers_ref")
+
+
+def writers_available() -> bool:
+    return os.access(WRITERS_BINARY, os.X_OK)
+
+
+def writers(P, state, initial, meta, timeout=120):
+    """Runs the reference's timeseries and endpoint writers (writers_ref) on
+    one grid and particle set; returns the bytes of (timeseries, endpoints,
+    mass endpoints).  state: cell, lx, ly, lz, t, status; initial: cell,
+    lx, ly, lz, t, face; meta: group, sequence, id, mass, solute (per
+    group), names (per group), tracking_direction, time_point, time_step,
+    reference_time, origin, rotation, time, zones (per cell)."""
+    f8, i4 = np.float64, np.int32
+    nb = P.ncell
+    hdr = np.array([P.ncol, P.nrow, P.nlay, state["cell"].size, len(meta["names"]),
+                    meta["tracking_direction"], meta["time_point"], meta["time_step"]], i4)
+    names = b"".join(n.encode().ljust(20)[:20] for n in meta["names"])
+    parts = [hdr, np.array([meta["reference_time"], *meta["origin"], meta["rotation"],
+                            meta["time"]], f8),
+             np.asarray(P.delr, f8), np.asarray(P.delc, f8), np.asarray(P.top, f8),
+             np.asarray(P.bot, f8), np.asarray(meta["zones"], i4).reshape(nb)]
+    for a in (meta["group"], meta["sequence"], meta["id"], state["status"], state["cell"],
+              initial["cell"], initial["face"]):
+        parts.append(np.ascontiguousarray(a, i4))
+    for a in (state["lx"], state["ly"], state["lz"], state["t"], initial["lx"],
+              initial["ly"], initial["lz"], initial["t"], meta["mass"]):
+        parts.append(np.ascontiguousarray(a, f8))
+    parts.append(np.ascontiguousarray(meta["solute"], i4))
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "in.bin")
+        with open(fin, "wb") as fh:
+            for a in parts:
+                fh.write(np.ascontiguousarray(a).tobytes())
+            fh.write(names)
+        subprocess.run([WRITERS_BINARY, fin], check=True, timeout=timeout,
+                       stdout=subprocess.DEVNULL, cwd=d)
+        return tuple(open(os.path.join(d, f), "rb").read()
+                     for f in ("ts.out", "ep.out", "epm.out"))
