@@ -24,7 +24,10 @@ roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
 and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
 and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
 sample of the same workload: the first interval of all particles).  The
-batched workload (256 ensemble members, about 5.5e7 particles, one walk over
+precision workload (config 2: 1e7 particles, 1000 steps, FP32-local and
+FP64, errors on moments and on the concentration at 64 observation cells)
+is reported as config2 (--no-config2 skips it).  The batched workload
+(256 ensemble members, about 6.3e7 particles, one walk over
 all members) is timed for --ensemble-intervals intervals and reported as
 ensemble.  Before
 the timed region the plume after the first interval (25 d, still inside the
@@ -127,6 +130,81 @@ def adaptive_kde(E, origin, nbins, loops):
 
 
 TUNED_NAME = {0: "config1_fp64", 1: "config1_fp32", 2: "config1_fp32n4"}
+
+
+def observation_cells(P):
+    """64 observation cells (0-based flat indices, x fastest) along the
+    plume's path: 8 columns x 8 rows in the middle layer."""
+    cols = np.linspace(15, 150, 8).astype(int)
+    rows = np.linspace(80, 120, 8).astype(int)
+    k = P.nlay // 2
+    return np.array([c + P.ncol * (r + P.nrow * k) for r in rows for c in cols])
+
+
+def config2_run(precision, report_steps=(50, 100, 200, 400, 1000)):
+    """The precision workload: the config-1 field with 1e7 particles and
+    1000 RW steps, run in FP32-local and in plain FP64.  Reports both walks'
+    throughput and the FP32 errors against FP64 on the plume mean and
+    variance (moments summed in FP64 over the active particles) and on the
+    KDE concentration at 64 observation cells, after the steps listed."""
+    from rwpt import Engine, problems
+    P = problems.config1()
+    rel = problems.release_in_block(P, 10_000_000, (1, 10), (76, 125), (1, 20))
+    nbins = (P.ncol, P.nrow, P.nlay)
+    origin = (0.0, 0.0, float(P.bot.min()))
+    obs = observation_cells(P)
+    out, snaps = {}, {}
+    for prec in (0, precision):
+        E = Engine(P, precision=prec)
+        tune = apply_tuning(E, TUNED_NAME[prec])
+        try:
+            E.load_particles(*rel)
+            walk = 0.0
+            steps = 0
+            res = []
+            done = 0
+            for target in report_steps:
+                t0 = time.perf_counter()
+                steps += E.advance(target * DT)[1]
+                walk += time.perf_counter() - t0
+                g = E.particles()
+                on = g["status"] == 1
+                gx, gy, gz = E.global_xyz()
+                x = np.stack([gx[on], gy[on], gz[on]])
+                dens, _ = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
+                res.append({"steps": target, "active": int(on.sum()),
+                            "mean": x.mean(axis=1), "var": x.var(axis=1),
+                            "obs": dens[obs]})
+                done = target
+            kms, launches = E.kernel_time(0)
+        finally:
+            E.close()
+        snaps[prec] = res
+        out["fp64" if prec == 0 else "fp32"] = {
+            "value": steps / walk, "particle_steps": steps, "walk_s": walk,
+            "walk_launches": launches, "walk_kernel_avg_launch_ms": kms / max(launches, 1),
+            "roofline_frac": BYTES_PER_STEP[prec] * steps / (kms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "tuning": tune, "steps_run": done}
+    errs = []
+    for a, b in zip(snaps[0], snaps[precision]):
+        if a["active"] < 0.01 * rel[0].size:
+            continue  # the plume has left the grid: nothing left to compare
+        o = np.abs(b["obs"] - a["obs"])
+        errs.append({"steps": a["steps"], "active_fp64": a["active"],
+                     "active_fp32": b["active"],
+                     "mean_rel_err": float(np.max(np.abs(b["mean"] - a["mean"]) /
+                                                  np.abs(a["mean"]))),
+                     "variance_rel_err": float(np.max(np.abs(b["var"] - a["var"]) / a["var"])),
+                     "obs_conc_rel_err_mean": float(o.sum() / max(a["obs"].sum(), 1e-300)),
+                     "obs_conc_rel_err_max": float(o.max() / max(a["obs"].max(), 1e-300))})
+    out["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["errors"] = errs
+    out["workload"] = ("config-1 field, 1e7 particles in the inlet block, 1000 RW steps of "
+                       "0.5 d; errors of the FP32-local walk against the FP64 walk while "
+                       "at least 1 % of the particles are in the grid (observation "
+                       "concentrations: relative L1 over the 64 cells and largest cell "
+                       "difference over the largest FP64 value)")
+    return out
 
 
 def apply_tuning(E, name):
@@ -306,6 +384,8 @@ def main():
     ap.add_argument("--ensemble-intervals", type=int, default=1,
                     help="tracking intervals of the 256-member ensemble timed "
                          "(0: skip)")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the 1e7-particle, 1000-step precision workload")
     ap.add_argument("--kde-loops", type=int, default=10,
                     help="optimisation loops of the adaptive gpkde measured on the "
                          "final plume (0: skip)")
@@ -400,6 +480,8 @@ def main():
                        "launches": b0["launches"]}
         out["speedup_vs_fp64"] = value / b0["rate"]
         out["error_vs_fp64"] = errors
+    if not args.no_config2 and prec != 0:
+        out["config2"] = config2_run(prec)
     if args.ensemble_intervals > 0:
         out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
     if not args.no_cpu_baseline:
