@@ -17,12 +17,16 @@
 // builds three 1-D factor arrays in LDS, applies the reflection fold of
 // :303-390 to each (the fold of a product kernel is the product of the folded
 // factors), and walks the footprint with the outer product.  The per-bin
-// formulas between the passes run on the host in FP64 with the same libm
-// calls as the reference.  Scatters add with atomics, so the summation order
-// (and the last bits of the density) can vary from run to run.
+// formulas between the passes run on the device too, one thread per active
+// bin in FP64, so the loop keeps every per-bin array in HBM and the host
+// only reads back the widest kernel before each pass and three sums per
+// loop.  Scatters add with atomics, so the summation order (and the last
+// bits of the density) can vary from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "rwpt_internal.h"
 
@@ -232,7 +236,7 @@ __global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
   const int c[3] = {id % G.nb[0] + 1, (id / G.nb[0]) % G.nb[1] + 1,
                     id / (G.nb[0] * G.nb[1]) + 1};
   double h[3];
-  for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 3 ? d : 0)];
+  for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 1 ? 0 : d)];
   if (kind == 2)  // second-derivative kernels carry one bandwidth in active dims
     for (int d = 0; d < 3; ++d) h[d] = G.active[d] ? h[d] : 0.0;
   Factors F;
@@ -273,7 +277,7 @@ __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
                     id / (G.nb[0] * G.nb[1]) + 1};
   double h[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 3 ? d : 0)];
+  for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 1 ? 0 : d)];
   Factors F;
   build_factors(lds, cap, G, c, h, kind, 0, kRangeW, -1, F);
   const double *ax = lds, *ay = lds + cap, *az = lds + 2 * cap;
@@ -351,6 +355,515 @@ __global__ void gk_flag_active(int64_t ncells, const double *__restrict__ counts
   if (i < ncells) flags[i] = counts[i] != 0.0 ? 1 : 0;
 }
 
+
+// indices of the non-zero histogram bins, ascending, and their number
+hipError_t compact_nonzero(const double *counts, int64_t ncells, int32_t *bins,
+                           int32_t *dcount, int *count, hipStream_t st) {
+  int32_t *flags = nullptr;
+  void *tmp = nullptr;
+  size_t bytes = 0;
+  const hipcub::CountingInputIterator<int32_t> ids(0);
+  hipError_t err = hipcub::DeviceSelect::Flagged(nullptr, bytes, ids, (const int32_t *)nullptr,
+                                                 bins, dcount, (int)ncells, st);
+  if (err == hipSuccess) err = hipMalloc(&flags, (size_t)ncells * 4);
+  if (err == hipSuccess) err = hipMalloc(&tmp, std::max(bytes, (size_t)16));
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(gk_flag_active, dim3((int)((ncells + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, st, ncells, counts, flags);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess)
+    err = hipcub::DeviceSelect::Flagged(tmp, bytes, ids, (const int32_t *)flags, bins, dcount,
+                                        (int)ncells, st);
+  int32_t c = 0;
+  if (err == hipSuccess) err = hipMemcpyAsync(&c, dcount, 4, hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (flags) (void)hipFree(flags);
+  if (tmp) (void)hipFree(tmp);
+  *count = c;
+  return err;
+}
+
+// ---------------------------------------------------------------------------
+// The per-bin formulas between the passes, one thread per active bin, in
+// FP64.  Whole-set quantities (the widest kernel, the smallest net
+// roughness, the largest scale and density, whether a bound was hit) are
+// folded with atomics on the bit patterns of non-negative doubles, which
+// order like the values.
+
+struct GkC {
+  int nd, dims[3], active[3];
+  double bin[3], maxKS[3], maxKSD[3], minKS[3], minKSD[3];
+  double maxKSv, minKSv;
+  int bounded, isotropic, useDb;
+  double supportC, alphaC, betaC, fnd, opq, o4, m6, sqrtEightPi, pi, nPoints;
+  // kernel database: h/delta of entries 1..ndb and the log-spaced index
+  const double *hod;
+  int ndb;
+  double dbmin, dbdelta;
+};
+
+struct GkA {
+  double *hk, *scale, *shape, *sigma, *nEst, *dens, *densOld, *scaleOld, *curv,
+      *rough, *net, *hw, *hs, *hsd, *s6, *part;
+  int8_t *col;
+  uint8_t *skip;
+  // 0 widest half-width, 1 smallest positive net roughness, 2 largest
+  // positive scale, 3 a smoothing was bounded, 4 largest density, 5 largest
+  // previous density
+  unsigned long long *u;
+};
+
+constexpr int kSumBlocks = 256;
+
+__device__ inline void amax_pos(unsigned long long *a, double v) {
+  atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline void amin_pos(unsigned long long *a, double v) {
+  atomicMin(a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline double as_double(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+
+// prComputeKernelSmoothingScale (GridProjectedKDE.F90:5164-5203)
+__device__ inline double smoothing_scale_d(const GkC &K, const double *h) {
+  double s = 1.0;
+  int dc = K.nd;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!K.active[d]) continue;
+    if (h[d] != 0.0) s = s * h[d];
+    else dc = dc - 1;
+  }
+  return dc != 0 ? pow(s, 1.0 / K.fnd) : 0.0;
+}
+
+// database index of a bandwidth, 1-based and clamped (:5740-5750)
+__device__ inline int db_index(const GkC &K, double h, double bin) {
+  if (!(h > 0.0)) return 1;
+  const int i = (int)floor(log(h / bin / K.dbmin) / K.dbdelta) + 1;
+  return min(max(i, 1), K.ndb);
+}
+
+__device__ inline double pick3(const double v[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
+}
+__device__ inline int pick3i(const int v[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
+}
+
+// density kernel from the database (prSetKernelFromDatabase :5965-6003):
+// bandwidths, and the dimension left at its centre entry by the x-y
+// transpose of an x-z or y-z lookup (-1 none)
+__device__ inline int db_w(const GkC &K, const double h[3], double out[3]) {
+  int idx[3] = {1, 1, 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (K.active[d]) idx[d] = db_index(K, h[d], K.bin[d]);
+  if (K.nd == 3) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = K.hod[idx[d] - 1] * K.bin[d];
+    return -1;
+  }
+  const int a = K.dims[0], b = K.dims[1];
+  const int ia = pick3i(idx, a), ib = pick3i(idx, b);
+  const double ba = pick3(K.bin, a), bb = pick3(K.bin, b);
+  double ha, hb;
+  int c = -1;
+  if (ia >= ib || (a == 0 && b == 1)) {
+    ha = K.hod[ia - 1] * ba;
+    hb = K.hod[ib - 1] * bb;
+  } else {
+    ha = K.hod[ib - 1] * ba;
+    hb = K.hod[ia - 1] * bb;
+    c = a;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[d] = d == a ? ha : (d == b ? hb : 0.0);
+  return c;
+}
+
+// sigma kernel from the database (:6006-6043): the flat entry, the first
+// two active dimensions taking the larger and the smaller index
+__device__ inline void db_sigma(const GkC &K, double s, double out[3]) {
+  int idx[3] = {1, 1, 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (K.active[d]) idx[d] = db_index(K, s, K.bin[d]);
+  const int a = K.nd == 3 ? 0 : K.dims[0], b = K.nd == 3 ? 1 : K.dims[1];
+  const int ia = pick3i(idx, a), ib = pick3i(idx, b);
+  const int hi = max(ia, ib), lo = min(ia, ib);
+  const double va = K.hod[hi - 1] * pick3(K.bin, a), vb = K.hod[lo - 1] * pick3(K.bin, b);
+  const double vz = K.nd == 3 ? K.hod[idx[2] - 1] * K.bin[2] : 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[d] = d == a ? va : (d == b ? vb : vz);
+}
+
+// widest half-width of a set of kernels, for the LDS size of the next pass
+__device__ inline void cap_update(const GkC &K, unsigned long long *u, const double h[3],
+                                  int range, bool allDims) {
+  int m = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!allDims && !K.active[d]) continue;
+    if (h[d] > 0.0 && isfinite(h[d])) m = max(m, half_width(range, h[d], K.bin[d]));
+  }
+  if (m > 0) atomicMax(u, (unsigned long long)m);
+}
+
+__global__ void gk_reset(unsigned long long *u) {
+  const int t = threadIdx.x;
+  if (t < 6) u[t] = t == 1 ? (unsigned long long)__double_as_longlong(INFINITY) : 0ull;
+}
+
+// initial smoothing in the active dimensions (:4109-4117, :4606-4618)
+__global__ void gk_init(int N, const GkC K, GkA A, double hinit) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double h[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) h[d] = K.active[d] ? hinit : 0.0;
+  const double sc = smoothing_scale_d(K, h);
+  A.scale[n] = sc;
+  A.scaleOld[n] = sc;
+  A.sigma[n] = 2.0 * sc;  // defaultInitialSigmaFactor
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    A.hk[3 * (size_t)n + d] = h[d];
+    A.shape[3 * (size_t)n + d] = (K.active[d] && sc > 0.0) ? h[d] / sc : 0.0;
+  }
+}
+
+// density kernels of the current smoothing
+__global__ void gk_wparams(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double h[3], o[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) h[d] = A.hk[3 * (size_t)n + d];
+  A.skip[n] = (h[0] < 0.0 || h[1] < 0.0 || h[2] < 0.0) ? 1 : 0;
+  int c = -1;
+  if (K.useDb) {
+    c = db_w(K, h, o);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] = h[d];
+  }
+  A.col[n] = (int8_t)c;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) A.hw[3 * (size_t)n + d] = o[d];
+  cap_update(K, A.u, o, kRangeW, false);
+}
+
+// sigma kernels of the current support scales: brute force carries the
+// scale in every dimension with a bin size, a database entry only in the
+// active ones
+__global__ void gk_sigparams(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double s = A.sigma[n];
+  double o[3];
+  if (K.useDb) {
+    db_sigma(K, s, o);
+  } else {
+    o[0] = o[1] = o[2] = s;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) A.hs[3 * (size_t)n + d] = o[d];
+  cap_update(K, A.u, o, kRangeW, !K.useDb);
+}
+
+// support scale from the first n-estimate (:4816-4858)
+__global__ void gk_support(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double n1 = A.s6[6 * (size_t)n];
+  double ks = pow(n1, 0.5) * pow(A.scale[n], K.opq) / pow(4.0 * A.dens[n], 0.25) * K.supportC;
+  if (K.bounded) {
+    const double g = 1.5 * A.sigma[n];  // defaultMaxSigmaGrowth
+    if (ks > g) ks = g;
+    if (K.maxKSv < ks) ks = K.maxKSv;
+    if (ks < K.minKSv) ks = K.minKSv;
+  }
+  A.sigma[n] = ks;
+}
+
+// n-estimate, then the curvature bandwidths (:5300-5400), then the
+// second-derivative kernels along each derivative dimension and whether a
+// bin's curvature kernels are skipped
+__global__ void gk_curv(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double ne = A.s6[6 * (size_t)n];
+  A.nEst[n] = ne;
+  const double dn = A.dens[n], sc = A.scale[n];
+  double sh[3], cv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sh[d] = A.shape[3 * (size_t)n + d];
+  if (dn > 0.0) {
+    const double nvpb =
+        pow(pow(K.sqrtEightPi * A.sigma[n], K.fnd) * pow(ne, 2.0) / dn, K.betaC);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!K.active[d]) continue;
+      double sum = 0.0;
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+        if (K.active[e]) sum = sum + (e == d ? 5.0 : 1.0) / pow(sh[e], 2.0);
+      const double term = pow((K.o4 / pow(sh[d], 4.0)) * sum, K.m6);
+      double cb = K.alphaC * nvpb * term * sc;
+      if (K.bounded) {
+        if (cb > K.maxKSD[d]) cb = K.maxKSD[d];
+        if (cb < K.minKSD[d]) cb = K.minKSD[d];
+      }
+      cv[d] = cb;
+    }
+  }
+  uint8_t sk = 0;
+  if (K.nd == 2)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (cv[d] < 0.0 || cv[d] != cv[d]) sk = 1;
+  A.skip[n] = sk;
+  int m = 0;
+#pragma unroll
+  for (int der = 0; der < 3; ++der) {
+    A.curv[3 * (size_t)n + der] = cv[der];
+    const double cb = cv[der];
+    double o[3];
+    if (K.useDb) {
+      // prSetKernelSD*FromDatabase (:6081-6124, :6184-6221)
+      const int i = cb > 0.0 ? db_index(K, cb, K.bin[der]) : 1;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = K.active[d] ? K.hod[i - 1] * K.bin[d] : 0.0;
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = K.active[d] ? cb : 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      A.hsd[9 * (size_t)n + 3 * der + d] = o[d];
+      if (K.active[der] && K.active[d] && o[d] > 0.0 && isfinite(o[d]))
+        m = max(m, half_width(kRangeSD, o[d], K.bin[d]));
+    }
+  }
+  if (m > 0) atomicMax(A.u, (unsigned long long)m);
+}
+
+// net roughness from the gathered sums (:2137-2289, :2464-2709)
+__global__ void gk_rough(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double *s = A.s6 + 6 * (size_t)n;
+  double r[3] = {0.0, 0.0, 0.0};
+  double net;
+  if (K.nd == 3) {
+    const double rXX = s[0], rYY = s[1], rZZ = s[2], rXY = s[3], rXZ = s[4], rYZ = s[5];
+    r[0] = rXX;
+    r[1] = rYY;
+    r[2] = rZZ;
+    if (!K.isotropic) {
+      double v = 3.0 * pow(rXX * rYY * rZZ, 1.0 / 3.0);
+      if (rYZ > 0.0) v = v + 2.0 * rYZ * pow(pow(rXX, 2.0) / rYY / rZZ, 1.0 / 6.0);
+      if (rXZ > 0.0) v = v + 2.0 * rXZ * pow(pow(rYY, 2.0) / rXX / rZZ, 1.0 / 6.0);
+      if (rXY > 0.0) v = v + 2.0 * rXY * pow(pow(rZZ, 2.0) / rXX / rYY, 1.0 / 6.0);
+      net = v;
+    } else {
+      net = rXX + 2.0 * rXY + 2.0 * rXZ + rYY + 2.0 * rYZ + rZZ;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = d == K.dims[0] ? s[0] : (d == K.dims[1] ? s[1] : 0.0);
+    if (!K.isotropic) net = 2.0 * s[2] + 2.0 * sqrt(s[0] * s[1]);
+    else net = 2.0 * s[2] + s[0] + s[1];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) A.rough[3 * (size_t)n + d] = r[d];
+  A.net[n] = net;
+  if (net > 0.0) amin_pos(&A.u[1], net);
+}
+
+// isotropy correction of the net roughness and the optimal scale
+// (:5404-5540)
+__global__ void gk_scale(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double net = A.net[n];
+  if (!K.isotropic) {
+    double r[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = A.rough[3 * (size_t)n + d];
+    const double r11 = pick3(r, K.dims[0]), r22 = pick3(r, K.dims[1]);
+    double sumMain = K.nd == 2 ? r11 + r22 : r[0] + r[1] + r[2];
+    if (sumMain == 0.0) sumMain = as_double(A.u[1]);
+    bool iso;
+    if (K.nd == 2)
+      iso = (r11 / sumMain > 0.85) || (r22 / sumMain > 0.85);
+    else
+      iso = (r[0] / sumMain > 0.85) || (r[1] / sumMain > 0.85) || (r[2] / sumMain > 0.85);
+    if (iso) net = (1.0 - 0.5) * net + 0.5 * sumMain;
+    A.net[n] = net;
+  }
+  double sc = 0.0;
+  if (net > 0.0) sc = pow(K.fnd * A.nEst[n] / (pow(4.0 * K.pi, 0.5 * K.fnd) * net), K.o4);
+  A.scale[n] = sc;
+  if (sc > 0.0) amax_pos(&A.u[2], sc);
+}
+
+// shape factors and the new smoothing (:5540-5724), bounded per dimension
+__global__ void gk_shape(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double net = A.net[n];
+  double sc = A.scale[n];
+  if (net == 0.0) {
+    sc = A.u[2] ? as_double(A.u[2]) : -INFINITY;
+    A.scale[n] = sc;
+  }
+  double r[3], sh[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r[d] = A.rough[3 * (size_t)n + d];
+  if (!K.isotropic) {
+    if (K.nd == 2) {
+      const int a = K.dims[0], b = K.dims[1];
+      const double r11 = pick3(r, a), r22 = pick3(r, b);
+      if (r11 != 0.0 && r22 != 0.0) {
+        double sa = pow(net / r11, 1.0 / 8.0);
+        if (K.bounded) {
+          const double lo = pick3(K.minKS, a) / sc;
+          if (sa < lo) sa = lo;
+        }
+        double sb = 1.0 / sa;
+        const double ns = sqrt(sa * sb);
+        if (ns == 0.0) {
+          sa = sb = 1.0;
+        } else {
+          sa = sa / ns;
+          sb = sb / ns;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sh[d] = d == a ? sa : (d == b ? sb : 1.0);
+      }
+    } else if (r[0] != 0.0 && r[1] != 0.0 && r[2] != 0.0) {
+      double s0 = pow(net / r[0], 1.0 / 12.0), s1 = pow(net / r[1], 1.0 / 12.0);
+      if (K.bounded) {
+        if (s0 < K.minKS[0] / sc) s0 = K.minKS[0] / sc;
+        if (s1 < K.minKS[1] / sc) s1 = K.minKS[1] / sc;
+      }
+      double s2 = 1.0 / s0 / s1;
+      // the reference's exponent is the single-precision literal 0.333
+      const double ns = pow(s0 * s1 * s2, (double)0.333f);
+      if (ns == 0.0) {
+        s0 = s1 = s2 = 1.0;
+      } else {
+        s0 = s0 / ns;
+        s1 = s1 / ns;
+        s2 = s2 / ns;
+      }
+      sh[0] = s0;
+      sh[1] = s1;
+      sh[2] = s2;
+    }
+  }
+  // a smoothing above the maximum is set to it, then one below the minimum
+  // to that; any bounded smoothing recomputes every bin's scale
+  bool clamped = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double h = 0.0;
+    if (K.active[d]) {
+      h = sh[d] * sc;
+      if (K.bounded) {
+        if (h > K.maxKS[d]) {
+          h = K.maxKS[d];
+          clamped = true;
+        }
+        if (h < K.minKS[d]) {
+          h = K.minKS[d];
+          clamped = true;
+        }
+      }
+    }
+    A.shape[3 * (size_t)n + d] = sh[d];
+    A.hk[3 * (size_t)n + d] = h;
+  }
+  if (clamped) atomicOr(&A.u[3], 1ull);
+}
+
+__global__ void gk_rescale(int N, const GkC K, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N || A.u[3] == 0ull) return;
+  A.scale[n] = smoothing_scale_d(K, A.hk + 3 * (size_t)n);
+}
+
+__global__ void gk_max_density(int N, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  amax_pos(&A.u[4], fmax(A.dens[n], 0.0));
+  amax_pos(&A.u[5], fmax(A.densOld[n], 0.0));
+}
+
+// squared relative density and smoothing changes and squared ALMISE terms
+// (:4962-4991), summed per block in a fixed order; then the previous
+// density and scale take the new ones
+__global__ void __launch_bounds__(kBlock) gk_conv(int N, const GkC K, GkA A) {
+  __shared__ double red[3][kBlock];
+  const double maxD = as_double(A.u[4]), maxDO = as_double(A.u[5]);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < N; n += gridDim.x * kBlock) {
+    const double dO = A.densOld[n], dN = A.dens[n];
+    double em = 0.0;
+    if (dO != 0.0) em = fabs(dN / maxD - dO / maxDO) / (dO / maxDO);
+    const double sO = A.scaleOld[n], sN = A.scale[n];
+    double es = 0.0;
+    if (sO != 0.0) es = fabs(sN - sO) / sO;
+    double ea = 0.0;
+    if (sN != 0.0)
+      ea = (A.nEst[n] / (pow(sN, K.fnd) * pow(4.0 * K.pi, 0.5 * K.fnd)) +
+            0.25 * A.net[n] * pow(sN, 4.0)) /
+           pow(K.nPoints, 2.0);
+    acc[0] += em * em;
+    acc[1] += es * es;
+    acc[2] += ea * ea;
+    A.densOld[n] = dN;
+    A.scaleOld[n] = sN;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) A.part[3 * blockIdx.x + q] = red[q][0];
+}
+
+__global__ void __launch_bounds__(kSumBlocks) gk_conv_total(GkA A, double *out) {
+  __shared__ double red[3][kSumBlocks];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = A.part[3 * threadIdx.x + q];
+  __syncthreads();
+  for (int s = kSumBlocks / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[q] = red[q][0];
+}
+
+__global__ void gk_keep_density(int N, GkA A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) A.densOld[n] = A.dens[n];
+}
+
 }  // namespace
 
 // Host side of one adaptive reconstruction.  counts_dev: the histogram in
@@ -359,7 +872,10 @@ __global__ void gk_flag_active(int64_t ncells, const double *__restrict__ counts
 // scattered with the final kernels in place of the counts (effective weight
 // formats 2 and 3, GridProjectedKDE.F90:4712-4760); density_dev (ncells)
 // receives the density (before the effective-mass, unit-volume and scaling
-// factors).  Returns the loops run in *loops_run.
+// factors).  Returns the loops run in *loops_run.  Every per-bin quantity
+// stays on the device: the host reads back the widest kernel before each
+// scatter or gather (its LDS size) and three sums per loop for the
+// convergence tests.
 hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
                             const double *counts_dev, double hinit,
                             double nPoints, const double *final_weights_dev,
@@ -380,471 +896,241 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
   const double fnd = (double)nd;
   const double pi = 4.0 * std::atan(1.0);
-  const double sqrtEightPi = std::sqrt(8.0 * pi);
+  GkC K{};
+  K.nd = nd;
+  for (int d = 0, di = 0; d < 3; ++d) {
+    K.active[d] = G.active[d];
+    K.bin[d] = G.bin[d];
+    if (G.active[d]) K.dims[di++] = d;
+  }
   // prInitializeModuleConstants, GridProjectedKDE.F90:1582-1611
-  const double supportC = std::pow((fnd + 2.0) * std::pow(8.0 * pi, 0.5 * fnd), 0.25);
-  const double alphaC =
-      std::pow((1.0 + std::pow(2.0, 0.5 * fnd + 2.0)) /
-                   (3.0 * std::pow(2.0, 4.0 / (fnd + 4.0))),
-               1.0 / (fnd + 6.0)) *
-      std::pow(fnd + 2.0, 1.0 / (fnd + 4.0)) / std::pow(fnd + 4.0, 1.0 / (fnd + 6.0));
-  const double betaC = 2.0 / (fnd + 4.0) / (fnd + 6.0);
-  const double oneOverNDimPlusFour = 1.0 / (fnd + 4.0);
-  const double minusOneOverNDimPlusSix = -1.0 / (fnd + 6.0);
-  const double onePlusNDimQuarter = 1.0 + 0.25 * fnd;
+  K.supportC = std::pow((fnd + 2.0) * std::pow(8.0 * pi, 0.5 * fnd), 0.25);
+  K.alphaC = std::pow((1.0 + std::pow(2.0, 0.5 * fnd + 2.0)) /
+                          (3.0 * std::pow(2.0, 4.0 / (fnd + 4.0))),
+                      1.0 / (fnd + 6.0)) *
+             std::pow(fnd + 2.0, 1.0 / (fnd + 4.0)) / std::pow(fnd + 4.0, 1.0 / (fnd + 6.0));
+  K.betaC = 2.0 / (fnd + 4.0) / (fnd + 6.0);
+  K.fnd = fnd;
+  K.o4 = 1.0 / (fnd + 4.0);
+  K.m6 = -1.0 / (fnd + 6.0);
+  K.opq = 1.0 + 0.25 * fnd;
+  K.sqrtEightPi = std::sqrt(8.0 * pi);
+  K.pi = pi;
+  K.nPoints = nPoints;
+  K.isotropic = opt->isotropic != 0;
   // kernel size bounds (:788-858): format 0 by the domain, 1 by the given
   // h/delta range, 2 none
-  double maxKS[3] = {0, 0, 0}, maxKSD[3] = {0, 0, 0}, minKS[3] = {0, 0, 0},
-         minKSD[3] = {0, 0, 0};
-  const bool bounded = opt->bound_kernel_size != 2;
+  K.bounded = opt->bound_kernel_size != 2;
   for (int d = 0; d < 3; ++d) {
     if (!G.active[d]) continue;
     if (opt->bound_kernel_size == 1) {
-      maxKS[d] = maxKSD[d] = G.bin[d] * opt->max_h_over_delta;
-      minKS[d] = minKSD[d] = G.bin[d] * opt->min_h_over_delta;
+      K.maxKS[d] = K.maxKSD[d] = G.bin[d] * opt->max_h_over_delta;
+      K.minKS[d] = K.minKSD[d] = G.bin[d] * opt->min_h_over_delta;
     } else {
-      maxKS[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeW;
-      maxKSD[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeSD;
-      minKS[d] = 1.2 * G.bin[d] / (double)kRangeW;
-      minKSD[d] = 1.2 * G.bin[d] / (double)kRangeSD;
+      K.maxKS[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeW;
+      K.maxKSD[d] = G.bin[d] * (0.9 * G.nb[d] - 1) / (double)kRangeSD;
+      K.minKS[d] = 1.2 * G.bin[d] / (double)kRangeW;
+      K.minKSD[d] = 1.2 * G.bin[d] / (double)kRangeSD;
     }
   }
-  int maxDim = -1, minDim = -1;
-  for (int d = 0; d < 3; ++d) {
-    if (maxKS[d] > 0.0 && (maxDim < 0 || maxKS[d] < maxKS[maxDim])) maxDim = d;
-    if (minKS[d] > 0.0 && (minDim < 0 || minKS[d] > minKS[minDim])) minDim = d;
+  {
+    int maxDim = -1, minDim = -1;
+    for (int d = 0; d < 3; ++d) {
+      if (K.maxKS[d] > 0.0 && (maxDim < 0 || K.maxKS[d] < K.maxKS[maxDim])) maxDim = d;
+      if (K.minKS[d] > 0.0 && (minDim < 0 || K.minKS[d] > K.minKS[minDim])) minDim = d;
+    }
+    K.maxKSv = maxDim >= 0 ? K.maxKS[maxDim] : 0.0;
+    K.minKSv = minDim >= 0 ? K.minKS[minDim] : 0.0;
   }
-  const double maxSigmaGrowth = 1.5, isoThreshold = 0.85, isoFactor = 0.5;
-  const bool isotropic = opt->isotropic != 0;
   const double tol = opt->rel_error_convergence > 0.0 ? opt->rel_error_convergence : 0.02;
   const bool skipBreak = opt->skip_error_convergence != 0;
-  int dims[3], di = 0;
-  for (int d = 0; d < 3; ++d)
-    if (G.active[d]) dims[di++] = d;
-  const int id1 = dims[0], id2 = dims[1];
 
   // active bins (ascending linear index = the reference's column-major order)
-  std::vector<double> hcounts(ncells);
-  hipError_t err = hipMemcpyAsync(hcounts.data(), counts_dev, ncells * 8,
-                                  hipMemcpyDeviceToHost, st);
-  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  int32_t *dbins = nullptr, *dcount = nullptr;
+  hipError_t err = hipMalloc(&dcount, 4);
   if (err != hipSuccess) return err;
-  std::vector<int32_t> hbins;
-  for (int64_t i = 0; i < ncells; ++i)
-    if (hcounts[i] != 0.0) hbins.push_back((int32_t)i);
-  const int N = (int)hbins.size();
+  err = hipMalloc(&dbins, (size_t)ncells * 4);
+  if (err != hipSuccess) return (void)hipFree(dcount), err;
+  int N = 0;
+  err = compact_nonzero(counts_dev, ncells, dbins, dcount, &N, st);
+  (void)hipFree(dcount);
   if (loops_run) *loops_run = 0;
-  if (N == 0) {
-    return hipMemsetAsync(density_dev, 0, ncells * 8, st);
+  if (err != hipSuccess || N == 0) {
+    (void)hipFree(dbins);
+    return err != hipSuccess ? err : hipMemsetAsync(density_dev, 0, ncells * 8, st);
   }
-  std::vector<double> cnt(N), rawDensity(N);
-  for (int n = 0; n < N; ++n) {
-    cnt[n] = hcounts[hbins[n]];
-    rawDensity[n] = cnt[n] / binVol;
-  }
-  // device buffers
-  int32_t *dbins = nullptr;
-  double *dcnt = nullptr, *dh = nullptr, *dsig = nullptr, *dsd = nullptr,
-         *dsums = nullptr, *darr = nullptr, *grids = nullptr;
-  uint8_t *dskip = nullptr;
-  int8_t *dcol = nullptr;
-  auto cleanup = [&]() {
-    for (void *p : {(void *)dbins, (void *)dcnt, (void *)dh, (void *)dsig,
-                    (void *)dsd, (void *)dsums, (void *)darr, (void *)grids,
-                    (void *)dskip, (void *)dcol})
-      if (p) (void)hipFree(p);
-  };
-  if ((err = hipMalloc(&dbins, N * 4)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dcnt, N * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dh, N * 24)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dsig, N * 24)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dsd, N * 24)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dcol, N)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dsums, N * 48)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&darr, N * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dskip, N)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&grids, 3 * ncells * 8)) != hipSuccess) return cleanup(), err;
-  double *cg[3] = {grids, grids + ncells, grids + 2 * ncells};
-  err = hipMemcpyAsync(dbins, hbins.data(), N * 4, hipMemcpyHostToDevice, st);
-  if (err == hipSuccess)
-    err = hipMemcpyAsync(dcnt, cnt.data(), N * 8, hipMemcpyHostToDevice, st);
-  if (err != hipSuccess) return cleanup(), err;
 
-  const int gridBlocks = (int)((ncells + kBlock - 1) / kBlock);
-  int maxnb = std::max(G.nb[0], std::max(G.nb[1], G.nb[2]));
-  // LDS capacity per factor array: the widest kernel a launch needs
-  auto cap_for = [&](const std::vector<double> &hv, int stride, int range,
-                     bool allDims) {
-    int m = 0;
-    for (int n = 0; n < N; ++n)
-      for (int d = 0; d < 3; ++d) {
-        if (!allDims && !G.active[d]) continue;
-        const double h = hv[(size_t)n * stride + (stride == 3 ? d : 0)];
-        if (h > 0.0 && std::isfinite(h)) m = std::max(m, half_width(range, h, G.bin[d]));
-      }
-    (void)maxnb;
-    return 2 * m + 1;
-  };
-  // dynamic LDS beside the 12 KiB reduction buffer of the gather
-  auto lds_ok = [&](int cap) { return (size_t)cap * 3 * 8 <= 49152; };
-
-  std::vector<double> hk(3 * (size_t)N), scale(N), shape(3 * (size_t)N), sigma(N);
-  std::vector<double> nEst(N, 0.0), density(N, 0.0), curvBW(3 * (size_t)N, 0.0);
-  std::vector<double> rXX(N, 0.0), rYY(N, 0.0), rZZ(N, 0.0), net(N, 0.0);
-  std::vector<uint8_t> skip(N, 0);
-
-  // prComputeKernelSmoothingScale (:5164-5203)
-  auto smoothing_scale = [&](int n) {
-    double s = 1.0;
-    int dc = nd;
-    for (int d = 0; d < 3; ++d) {
-      if (!G.active[d]) continue;
-      if (hk[3 * (size_t)n + d] != 0.0) s = s * hk[3 * (size_t)n + d];
-      else dc = dc - 1;
-    }
-    return dc != 0 ? std::pow(s, 1.0 / fnd) : 0.0;
-  };
-
-  // kernel database: every lookup becomes the bandwidths of its entry
   KernelDb db;
-  const bool useDb = opt->kernel_database != 0;
-  if (useDb)
+  K.useDb = opt->kernel_database != 0;
+  if (K.useDb) {
     db.build(opt->min_h_over_delta, opt->delta_h_over_delta, opt->max_h_over_delta,
              opt->bin_size, opt->nbins);
-  std::vector<double> hw(3 * (size_t)N), hs(3 * (size_t)N), hsd(3 * (size_t)N);
-  std::vector<int8_t> col(N, -1);
+    K.ndb = db.n;
+    K.dbmin = db.min;
+    K.dbdelta = db.delta;
+  }
+
+  // per-bin doubles in one allocation: hk 3, scale, shape 3, sigma, nEst,
+  // dens, densOld, scaleOld, curv 3, rough 3, net, hw 3, hs 3, hsd 9, s6 6,
+  // weights
+  const size_t nB = (size_t)N;
+  const size_t nDoubles = nB * 41 + 3 * kSumBlocks + 3 + (size_t)std::max(db.n, 1);
+  double *pool = nullptr, *grids = nullptr;
+  unsigned long long *du = nullptr;
+  int8_t *dcol = nullptr;
+  uint8_t *dskip = nullptr;
+  auto cleanup = [&]() {
+    for (void *p : {(void *)dbins, (void *)pool, (void *)grids, (void *)du, (void *)dcol,
+                    (void *)dskip})
+      if (p) (void)hipFree(p);
+  };
+  if ((err = hipMalloc(&pool, nDoubles * 8)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&grids, 3 * (size_t)ncells * 8)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&du, 8 * 8)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&dcol, nB)) != hipSuccess) return cleanup(), err;
+  if ((err = hipMalloc(&dskip, nB)) != hipSuccess) return cleanup(), err;
+  GkA A{};
+  double *p = pool;
+  auto take = [&](size_t k) {
+    double *q = p;
+    p += k;
+    return q;
+  };
+  A.hk = take(3 * nB);
+  A.scale = take(nB);
+  A.shape = take(3 * nB);
+  A.sigma = take(nB);
+  A.nEst = take(nB);
+  A.dens = take(nB);
+  A.densOld = take(nB);
+  A.scaleOld = take(nB);
+  A.curv = take(3 * nB);
+  A.rough = take(3 * nB);
+  A.net = take(nB);
+  A.hw = take(3 * nB);
+  A.hs = take(3 * nB);
+  A.hsd = take(9 * nB);
+  A.s6 = take(6 * nB);
+  double *wbins = take(nB);
+  A.part = take(3 * kSumBlocks);
+  double *dsums = take(3);
+  double *dhod = take((size_t)std::max(db.n, 1));
+  A.col = dcol;
+  A.skip = dskip;
+  A.u = du;
+  K.hod = dhod;
+  if (K.useDb &&
+      (err = hipMemcpyAsync(dhod, db.hod.data(), db.n * 8, hipMemcpyHostToDevice, st)) !=
+          hipSuccess)
+    return cleanup(), err;
+  double *cg[3] = {grids, grids + ncells, grids + 2 * ncells};
+
+  const dim3 binGrid((N + kBlock - 1) / kBlock), blk(kBlock);
+  const int cellBlocks = (int)((ncells + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(gk_extract, binGrid, blk, 0, st, dbins, N, counts_dev, wbins);
+
+  // the widest kernel of the last parameter kernel -> LDS entries per factor
+  auto read_cap = [&](int *cap) -> hipError_t {
+    unsigned long long m = 0;
+    hipError_t r = hipGetLastError();
+    if (r == hipSuccess) r = hipMemcpyAsync(&m, du, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return r;
+    *cap = 2 * (int)m + 1;
+    // dynamic LDS beside the 12 KiB reduction buffer of the gather
+    return (size_t)*cap * 3 * 8 <= 49152 ? hipSuccess : hipErrorNotSupported;
+  };
+  auto reset_cap = [&]() { return hipMemsetAsync(du, 0, 8, st); };
 
   // density scatter with the current smoothing (:4623-4668, :4930-4955)
-  auto density_pass = [&](const double *wbins) -> hipError_t {
-    for (int n = 0; n < N; ++n) {
-      const double *hn = &hk[3 * (size_t)n];
-      skip[n] = (hn[0] < 0.0 || hn[1] < 0.0 || hn[2] < 0.0) ? 1 : 0;
-      if (useDb) col[n] = (int8_t)db.w(hn, &hw[3 * (size_t)n]);
-      else for (int d = 0; d < 3; ++d) hw[3 * (size_t)n + d] = hn[d];
-    }
-    const int cap = cap_for(hw, 3, kRangeW, false);
-    if (!lds_ok(cap)) return hipErrorNotSupported;
-    hipError_t r = hipMemcpyAsync(dh, hw.data(), N * 24, hipMemcpyHostToDevice, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(dskip, skip.data(), N, hipMemcpyHostToDevice, st);
-    if (r == hipSuccess && useDb)
-      r = hipMemcpyAsync(dcol, col.data(), N, hipMemcpyHostToDevice, st);
+  auto density_pass = [&](const double *w) -> hipError_t {
+    hipError_t r = reset_cap();
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(gk_fill, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
-                       density_dev, 0.0);
-    hipLaunchKernelGGL(gk_scatter, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                       dh, 3, wbins, G, 0, 0, kRangeW, cap, dskip,
-                       useDb ? (const int8_t *)dcol : (const int8_t *)nullptr, density_dev);
-    hipLaunchKernelGGL(gk_div, dim3(gridBlocks), dim3(kBlock), 0, st, ncells,
-                       density_dev, binVol);
-    hipLaunchKernelGGL(gk_extract, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                       st, dbins, N, density_dev, darr);
-    if ((r = hipGetLastError()) != hipSuccess) return r;
-    r = hipMemcpyAsync(density.data(), darr, N * 8, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipStreamSynchronize(st);
-    return r;
+    hipLaunchKernelGGL(gk_wparams, binGrid, blk, 0, st, N, K, A);
+    int cap = 0;
+    if ((r = read_cap(&cap)) != hipSuccess) return r;
+    hipLaunchKernelGGL(gk_fill, dim3(cellBlocks), blk, 0, st, ncells, density_dev, 0.0);
+    hipLaunchKernelGGL(gk_scatter, dim3(N), blk, cap * 3 * 8, st, dbins, N,
+                       (const double *)A.hw, 3, w, G, 0, 0, kRangeW, cap,
+                       (const uint8_t *)dskip,
+                       K.useDb ? (const int8_t *)dcol : (const int8_t *)nullptr, density_dev);
+    hipLaunchKernelGGL(gk_div, dim3(cellBlocks), blk, 0, st, ncells, density_dev, binVol);
+    hipLaunchKernelGGL(gk_extract, binGrid, blk, 0, st, dbins, N, density_dev, A.dens);
+    return hipGetLastError();
   };
-  // sigma kernels of the current support scales: brute force carries the
-  // scale in every dimension with a bin size (kind 1); a database entry
-  // has zero bandwidth outside the active dimensions (kind 0)
+  // sigma gather of the density grid (the n-estimate) into s6[6n]
+  const int sigKind = K.useDb ? 0 : 1;
   int sigCap = 0;
-  auto upload_sigma = [&]() -> hipError_t {
-    if (useDb) {
-      for (int n = 0; n < N; ++n) db.sigma(sigma[n], &hs[3 * (size_t)n]);
-      sigCap = cap_for(hs, 3, kRangeW, false);
-    } else {
-      for (int n = 0; n < N; ++n)
-        for (int d = 0; d < 3; ++d) hs[3 * (size_t)n + d] = sigma[n];
-      sigCap = cap_for(sigma, 1, kRangeW, true);
-    }
-    if (!lds_ok(sigCap)) return hipErrorNotSupported;
-    return hipMemcpyAsync(dsig, hs.data(), N * 24, hipMemcpyHostToDevice, st);
-  };
-  const int sigKind = useDb ? 0 : 1;
-  // sigma gather of the density grid (nEstimate)
-  auto nest_pass = [&](std::vector<double> &out) -> hipError_t {
-    hipError_t r = upload_sigma();
+  auto nest_pass = [&]() -> hipError_t {
+    hipError_t r = reset_cap();
     if (r != hipSuccess) return r;
-    const int cap = sigCap;
-    hipLaunchKernelGGL(gk_gather, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                       dsig, 3, sigKind, G, 0, cap, density_dev, (const double *)nullptr,
-                       (const double *)nullptr, 1.0, 1.0, 1.0, 1.0, dsums);
-    if ((r = hipGetLastError()) != hipSuccess) return r;
-    std::vector<double> s6(6 * (size_t)N);
-    r = hipMemcpyAsync(s6.data(), dsums, N * 48, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipStreamSynchronize(st);
-    for (int n = 0; n < N; ++n) out[n] = s6[6 * (size_t)n];
-    return r;
+    hipLaunchKernelGGL(gk_sigparams, binGrid, blk, 0, st, N, K, A);
+    if ((r = read_cap(&sigCap)) != hipSuccess) return r;
+    hipLaunchKernelGGL(gk_gather, dim3(N), blk, sigCap * 3 * 8, st, dbins, N,
+                       (const double *)A.hs, 3, sigKind, G, 0, sigCap,
+                       (const double *)density_dev, (const double *)nullptr,
+                       (const double *)nullptr, 1.0, 1.0, 1.0, 1.0, A.s6);
+    return hipGetLastError();
   };
 
-  // initial smoothing in the active dimensions (:4109-4117, :4616)
-  for (int n = 0; n < N; ++n)
-    for (int d = 0; d < 3; ++d) hk[3 * (size_t)n + d] = G.active[d] ? hinit : 0.0;
-  for (int n = 0; n < N; ++n) {
-    scale[n] = smoothing_scale(n);
-    sigma[n] = 2.0 * scale[n];  // defaultInitialSigmaFactor
-    for (int d = 0; d < 3; ++d) {
-      if (G.active[d]) {
-        if (scale[n] > 0.0) shape[3 * (size_t)n + d] = hk[3 * (size_t)n + d] / scale[n];
-      } else {
-        hk[3 * (size_t)n + d] = 0.0;
-        shape[3 * (size_t)n + d] = 0.0;
-      }
-    }
-  }
-  std::vector<double> scaleOld = scale;
-  if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
-
-  auto almise = [&]() {
-    double s2 = 0.0;
-    for (int n = 0; n < N; ++n) {
-      double em = 0.0;
-      if (scale[n] != 0.0)
-        em = (nEst[n] / (std::pow(scale[n], fnd) * std::pow(4.0 * pi, 0.5 * fnd)) +
-              0.25 * net[n] * std::pow(scale[n], 4.0)) /
-             std::pow(nPoints, 2.0);
-      s2 += std::pow(em, 2.0);
-    }
-    return std::sqrt(s2 / (double)N);
-  };
-  double almiseOld = almise();
+  hipLaunchKernelGGL(gk_init, binGrid, blk, 0, st, N, K, A, hinit);
+  if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
+  hipLaunchKernelGGL(gk_keep_density, binGrid, blk, 0, st, N, A);
+  // nEstimate and the net roughness start at zero, so the first ALMISE is 0
+  double almiseOld = 0.0;
   double cumsum = almiseOld, cumsumOld = cumsum;
-  std::vector<double> densityOld = density;
   int loops = 0;
+  const double c0 = G.bin[K.dims[0]] * G.bin[K.dims[0]];
+  const double c1 = G.bin[K.dims[1]] * G.bin[K.dims[1]];
+  const double c2 = G.bin[2] * G.bin[2];
   for (int m = 1; m <= opt->n_opt_loops; ++m) {
     loops = m;
+    hipLaunchKernelGGL(gk_reset, dim3(1), dim3(64), 0, st, du);
     // support sizes and nEstimate (:4816-4858)
-    std::vector<double> n1(N);
-    if ((err = nest_pass(n1)) != hipSuccess) return cleanup(), err;
-    for (int n = 0; n < N; ++n) {
-      double ks = std::pow(n1[n], 0.5) * std::pow(scale[n], onePlusNDimQuarter) /
-                  std::pow(4.0 * density[n], 0.25) * supportC;
-      if (bounded) {
-        if (ks > maxSigmaGrowth * sigma[n]) ks = maxSigmaGrowth * sigma[n];
-        ks = std::min(ks, maxKS[maxDim]);
-        if (ks < minKS[minDim]) ks = minKS[minDim];
-      }
-      sigma[n] = ks;
-    }
-    if ((err = nest_pass(nEst)) != hipSuccess) return cleanup(), err;
-    // curvature bandwidths (:5300-5400)
-    for (int n = 0; n < N; ++n) {
-      for (int d = 0; d < 3; ++d) curvBW[3 * (size_t)n + d] = 0.0;
-      if (density[n] <= 0.0) continue;
-      const double nvpb = std::pow(std::pow(sqrtEightPi * sigma[n], fnd) *
-                                       std::pow(nEst[n], 2.0) / density[n],
-                                   betaC);
-      for (int a = 0; a < nd; ++a) {
-        const int did = dims[a];
-        double num[3] = {1.0, 1.0, 1.0};
-        num[did] = 5.0;
-        double sum = 0.0;
-        for (int b = 0; b < nd; ++b) {
-          const int dd = dims[b];
-          sum = sum + num[dd] / std::pow(shape[3 * (size_t)n + dd], 2.0);
-        }
-        const double term = std::pow(
-            (oneOverNDimPlusFour / std::pow(shape[3 * (size_t)n + did], 4.0)) * sum,
-            minusOneOverNDimPlusSix);
-        double cb = alphaC * nvpb * term * scale[n];
-        if (bounded) {
-          if (cb > maxKSD[did]) cb = maxKSD[did];
-          if (cb < minKSD[did]) cb = minKSD[did];
-        }
-        curvBW[3 * (size_t)n + did] = cb;
-      }
-    }
-    // curvature grids and roughness (:1905-2709)
+    if ((err = nest_pass()) != hipSuccess) return cleanup(), err;
+    hipLaunchKernelGGL(gk_support, binGrid, blk, 0, st, N, K, A);
+    if ((err = nest_pass()) != hipSuccess) return cleanup(), err;
+    // curvature bandwidths and kernels (:5300-5400), curvature grids
+    // (:1905-2136)
+    if ((err = reset_cap()) != hipSuccess) return cleanup(), err;
+    hipLaunchKernelGGL(gk_curv, binGrid, blk, 0, st, N, K, A);
+    int sdCap = 0;
+    if ((err = read_cap(&sdCap)) != hipSuccess) return cleanup(), err;
     for (int g = 0; g < 3; ++g)
-      hipLaunchKernelGGL(gk_fill, dim3(gridBlocks), dim3(kBlock), 0, st, ncells, cg[g], 0.0);
+      hipLaunchKernelGGL(gk_fill, dim3(cellBlocks), blk, 0, st, ncells, cg[g], 0.0);
     const int nder = nd == 3 ? 3 : 2;
     for (int q = 0; q < nder; ++q) {
-      const int der = nd == 3 ? q : dims[q];
-      for (int n = 0; n < N; ++n) {
-        const double cb = curvBW[3 * (size_t)n + der];
-        double *ho = &hsd[3 * (size_t)n];
-        if (useDb) db.sd(cb, der, ho);
-        else for (int d = 0; d < 3; ++d) ho[d] = G.active[d] ? cb : 0.0;
-        skip[n] = 0;
-        if (nd == 2) {
-          for (int d = 0; d < 3; ++d) {
-            const double v = curvBW[3 * (size_t)n + d];
-            if (v < 0.0 || v != v) skip[n] = 1;
-          }
-        }
-      }
-      const int cap = cap_for(hsd, 3, kRangeSD, false);
-      if (!lds_ok(cap)) return cleanup(), hipErrorNotSupported;
-      err = hipMemcpyAsync(dsd, hsd.data(), N * 24, hipMemcpyHostToDevice, st);
-      if (err == hipSuccess)
-        err = hipMemcpyAsync(dskip, skip.data(), N, hipMemcpyHostToDevice, st);
-      if (err != hipSuccess) return cleanup(), err;
-      hipLaunchKernelGGL(gk_scatter, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                         dsd, 3, dcnt, G, 2, der, kRangeSD, cap, dskip,
+      const int der = nd == 3 ? q : K.dims[q];
+      hipLaunchKernelGGL(gk_scatter, dim3(N), blk, sdCap * 3 * 8, st, dbins, N,
+                         (const double *)(A.hsd + 3 * der), 9, (const double *)wbins, G, 2,
+                         der, kRangeSD, sdCap, (const uint8_t *)dskip,
                          (const int8_t *)nullptr, cg[q]);
-      if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
     }
-    {
-      if ((err = upload_sigma()) != hipSuccess) return cleanup(), err;
-      const int cap = sigCap;
-      const double c0 = G.bin[nd == 3 ? 0 : id1] * G.bin[nd == 3 ? 0 : id1];
-      const double c1 = G.bin[nd == 3 ? 1 : id2] * G.bin[nd == 3 ? 1 : id2];
-      const double c2 = G.bin[2] * G.bin[2];
-      hipLaunchKernelGGL(gk_gather, dim3(N), dim3(kBlock), cap * 3 * 8, st, dbins, N,
-                         dsig, 3, sigKind, G, nd == 3 ? 1 : 2, cap, cg[0], cg[1], cg[2],
-                         binVol, c0, c1, c2, dsums);
-      if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
-      std::vector<double> s6(6 * (size_t)N);
-      err = hipMemcpyAsync(s6.data(), dsums, N * 48, hipMemcpyDeviceToHost, st);
-      if (err == hipSuccess) err = hipStreamSynchronize(st);
-      if (err != hipSuccess) return cleanup(), err;
-      for (int n = 0; n < N; ++n) {
-        const double *s = &s6[6 * (size_t)n];
-        rXX[n] = rYY[n] = rZZ[n] = 0.0;
-        if (nd == 3) {
-          rXX[n] = s[0];
-          rYY[n] = s[1];
-          rZZ[n] = s[2];
-          const double rXY = s[3], rXZ = s[4], rYZ = s[5];
-          if (!isotropic) {
-            double v = 3.0 * std::pow(rXX[n] * rYY[n] * rZZ[n], 1.0 / 3.0);
-            if (rYZ > 0.0)
-              v = v + 2.0 * rYZ * std::pow(std::pow(rXX[n], 2.0) / rYY[n] / rZZ[n], 1.0 / 6.0);
-            if (rXZ > 0.0)
-              v = v + 2.0 * rXZ * std::pow(std::pow(rYY[n], 2.0) / rXX[n] / rZZ[n], 1.0 / 6.0);
-            if (rXY > 0.0)
-              v = v + 2.0 * rXY * std::pow(std::pow(rZZ[n], 2.0) / rXX[n] / rYY[n], 1.0 / 6.0);
-            net[n] = v;
-          } else {
-            net[n] = rXX[n] + 2.0 * rXY + 2.0 * rXZ + rYY[n] + 2.0 * rYZ + rZZ[n];
-          }
-        } else {
-          double *r11 = id1 == 0 ? &rXX[n] : (id1 == 1 ? &rYY[n] : &rZZ[n]);
-          double *r22 = id2 == 0 ? &rXX[n] : (id2 == 1 ? &rYY[n] : &rZZ[n]);
-          *r11 = s[0];
-          *r22 = s[1];
-          if (!isotropic) net[n] = 2.0 * s[2] + 2.0 * std::sqrt(s[0] * s[1]);
-          else net[n] = 2.0 * s[2] + s[0] + s[1];
-        }
-      }
-    }
+    // roughness: sigma gathers of the squared curvatures (:2137-2709)
+    hipLaunchKernelGGL(gk_gather, dim3(N), blk, sigCap * 3 * 8, st, dbins, N,
+                       (const double *)A.hs, 3, sigKind, G, nd == 3 ? 1 : 2, sigCap,
+                       (const double *)cg[0], (const double *)cg[1], (const double *)cg[2],
+                       binVol, c0, c1, c2, A.s6);
+    hipLaunchKernelGGL(gk_rough, binGrid, blk, 0, st, N, K, A);
     // optimal smoothing and shape (:5404-5724)
-    {
-      std::vector<double> r11v(N), r22v(N);
-      for (int n = 0; n < N; ++n) {
-        r11v[n] = id1 == 0 ? rXX[n] : (id1 == 1 ? rYY[n] : rZZ[n]);
-        r22v[n] = id2 == 0 ? rXX[n] : (id2 == 1 ? rYY[n] : rZZ[n]);
-      }
-      if (!isotropic) {
-        double minNet = INFINITY;
-        for (int n = 0; n < N; ++n)
-          if (net[n] > 0.0) minNet = std::min(minNet, net[n]);
-        for (int n = 0; n < N; ++n) {
-          double sumMain = nd == 2 ? r11v[n] + r22v[n] : rXX[n] + rYY[n] + rZZ[n];
-          if (sumMain == 0.0) sumMain = minNet;
-          bool iso;
-          if (nd == 2)
-            iso = (r11v[n] / sumMain > isoThreshold) || (r22v[n] / sumMain > isoThreshold);
-          else
-            iso = (rXX[n] / sumMain > isoThreshold) || (rYY[n] / sumMain > isoThreshold) ||
-                  (rZZ[n] / sumMain > isoThreshold);
-          if (iso) net[n] = (1.0 - isoFactor) * net[n] + isoFactor * sumMain;
-        }
-      }
-      for (int n = 0; n < N; ++n) scale[n] = 0.0;
-      for (int n = 0; n < N; ++n)
-        if (net[n] > 0.0)
-          scale[n] = std::pow(fnd * nEst[n] / (std::pow(4.0 * pi, 0.5 * fnd) * net[n]),
-                              oneOverNDimPlusFour);
-      double maxScale = -INFINITY;
-      for (int n = 0; n < N; ++n)
-        if (scale[n] > 0.0) maxScale = std::max(maxScale, scale[n]);
-      for (int n = 0; n < N; ++n)
-        if (net[n] == 0.0) scale[n] = maxScale;
-      for (int n = 0; n < N; ++n) {
-        double *sh = &shape[3 * (size_t)n];
-        for (int d = 0; d < 3; ++d) {
-          hk[3 * (size_t)n + d] = 0.0;
-          sh[d] = 1.0;
-        }
-        if (isotropic) continue;
-        if (nd == 2) {
-          if (r11v[n] == 0.0 || r22v[n] == 0.0) continue;
-          sh[id1] = std::pow(net[n] / r11v[n], 1.0 / 8.0);
-          if (bounded) sh[id1] = std::max(sh[id1], minKS[id1] / scale[n]);
-          sh[id2] = 1.0 / sh[id1];
-          const double ns = std::sqrt(sh[id1] * sh[id2]);
-          if (ns == 0.0) {
-            sh[id1] = sh[id2] = 1.0;
-            continue;
-          }
-          sh[id1] = sh[id1] / ns;
-          sh[id2] = sh[id2] / ns;
-        } else {
-          if (rXX[n] == 0.0 || rYY[n] == 0.0 || rZZ[n] == 0.0) continue;
-          sh[0] = std::pow(net[n] / rXX[n], 1.0 / 12.0);
-          sh[1] = std::pow(net[n] / rYY[n], 1.0 / 12.0);
-          if (bounded) {
-            sh[0] = std::max(sh[0], minKS[0] / scale[n]);
-            sh[1] = std::max(sh[1], minKS[1] / scale[n]);
-          }
-          sh[2] = 1.0 / sh[0] / sh[1];
-          // the reference's exponent is the single-precision literal 0.333
-          const double ns = std::pow(sh[0] * sh[1] * sh[2], (double)0.333f);
-          if (ns == 0.0) {
-            sh[0] = sh[1] = sh[2] = 1.0;
-            continue;
-          }
-          sh[0] = sh[0] / ns;
-          sh[1] = sh[1] / ns;
-          sh[2] = sh[2] / ns;
-        }
-      }
-      bool update = false;
-      for (int d = 0; d < 3; ++d) {
-        if (!G.active[d]) continue;
-        for (int n = 0; n < N; ++n) {
-          double &hv = hk[3 * (size_t)n + d];
-          hv = shape[3 * (size_t)n + d] * scale[n];
-        }
-        if (!bounded) continue;
-        bool anyMax = false, anyMin = false;
-        for (int n = 0; n < N; ++n) anyMax |= hk[3 * (size_t)n + d] > maxKS[d];
-        if (anyMax) {
-          update = true;
-          for (int n = 0; n < N; ++n)
-            if (hk[3 * (size_t)n + d] > maxKS[d]) hk[3 * (size_t)n + d] = maxKS[d];
-        }
-        for (int n = 0; n < N; ++n) anyMin |= hk[3 * (size_t)n + d] < minKS[d];
-        if (anyMin) {
-          update = true;
-          for (int n = 0; n < N; ++n)
-            if (hk[3 * (size_t)n + d] < minKS[d]) hk[3 * (size_t)n + d] = minKS[d];
-        }
-      }
-      if (update)
-        for (int n = 0; n < N; ++n) scale[n] = smoothing_scale(n);
-    }
+    hipLaunchKernelGGL(gk_scale, binGrid, blk, 0, st, N, K, A);
+    hipLaunchKernelGGL(gk_shape, binGrid, blk, 0, st, N, K, A);
+    hipLaunchKernelGGL(gk_rescale, binGrid, blk, 0, st, N, K, A);
+    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
     // new density (:4930-4960)
-    if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
+    if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
     // convergence (:4962-5038)
-    double maxD = -INFINITY, maxDO = -INFINITY;
-    for (int n = 0; n < N; ++n) {
-      maxD = std::max(maxD, density[n]);
-      maxDO = std::max(maxDO, densityOld[n]);
-    }
-    double sDen = 0.0, sSm = 0.0;
-    for (int n = 0; n < N; ++n) {
-      double em = 0.0;
-      if (densityOld[n] != 0.0)
-        em = std::fabs(density[n] / maxD - densityOld[n] / maxDO) / (densityOld[n] / maxDO);
-      sDen += std::pow(em, 2.0);
-      double es = 0.0;
-      if (scaleOld[n] != 0.0) es = std::fabs(scale[n] - scaleOld[n]) / scaleOld[n];
-      sSm += std::pow(es, 2.0);
-    }
-    const double errDensity = std::sqrt(sDen / (double)N);
-    const double errSmoothing = std::sqrt(sSm / (double)N);
-    const double alm = almise();
+    hipLaunchKernelGGL(gk_max_density, binGrid, blk, 0, st, N, A);
+    hipLaunchKernelGGL(gk_conv, dim3(kSumBlocks), blk, 0, st, N, K, A);
+    hipLaunchKernelGGL(gk_conv_total, dim3(1), dim3(kSumBlocks), 0, st, A, dsums);
+    double sums[3];
+    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
+    if ((err = hipMemcpyAsync(sums, dsums, 24, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return cleanup(), err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return cleanup(), err;
+    const double errDensity = std::sqrt(sums[0] / (double)N);
+    const double errSmoothing = std::sqrt(sums[1] / (double)N);
+    const double alm = std::sqrt(sums[2] / (double)N);
     cumsum = cumsum + alm;
     if (!skipBreak) {
       if (almiseOld > 0.0 && std::fabs(alm - almiseOld) / almiseOld < tol) break;
@@ -859,20 +1145,16 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     }
     almiseOld = alm;
     cumsumOld = cumsum;
-    densityOld = density;
-    scaleOld = scale;
   }
   if (final_weights_dev) {
-    // last reconstruction over the mass sums (:4730-4758): the counts'
-    // device copy is no longer needed and takes the masses of the bins
-    hipLaunchKernelGGL(gk_extract, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                       dbins, N, final_weights_dev, dcnt);
-    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
-    if ((err = density_pass(dcnt)) != hipSuccess) return cleanup(), err;
+    // last reconstruction over the mass sums (:4730-4758)
+    hipLaunchKernelGGL(gk_extract, binGrid, blk, 0, st, dbins, N, final_weights_dev, wbins);
+    if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
   }
   if (loops_run) *loops_run = loops;
+  err = hipStreamSynchronize(st);
   cleanup();
-  return hipSuccess;
+  return err;
 }
 
 void KernelDb::build(double min_hod, double delta_hod, double max_hod,
