@@ -26,7 +26,10 @@ and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
 sample of the same workload: the first interval of all particles).  The
 precision workload (config 2: 1e7 particles, 1000 steps, FP32-local and
 FP64, errors on moments and on the concentration at 64 observation cells)
-is reported as config2 (--no-config2 skips it).  The batched workload
+is reported as config2 (--no-config2 skips it).  The calibration workload
+(config 4: 18 zoned parameters, the pathwise adjoint of an observation
+misfit) is reported as adjoint with its cost against the forward pass and
+its check against central differences (--no-adjoint skips it).  The batched workload
 (256 ensemble members, about 6.3e7 particles, one walk over
 all members) is timed for --ensemble-intervals intervals and reported as
 ensemble.  Before
@@ -270,6 +273,99 @@ def ensemble_run(intervals, warmup):
             "active_after": active, "tuning": tune}
 
 
+def adjoint_run(n=1_000_000):
+    """The calibration workload: the config-1 field with 8 x-bands of 25
+    columns (own aL, aT each), a common Dm and porosity scale (18
+    parameters), 1e6 particles, 300 RW steps of 0.25 d in 12 checkpoint
+    windows of 25 steps, misfit of the Gaussian-smoothed (h = 2, 2, 1 m)
+    concentrations at 32 observation points over the plume against 0.8 x
+    the model's own.
+    Times the forward (rwpt_adjoint without gradient) and the adjoint, then
+    checks the pathwise gradient against central differences with common
+    random numbers: on a 4-step horizon, where no particle changes branch
+    at a relative step of 1e-8, and on the full horizon, where the tangent
+    of the walk grows multiplicatively (position-dependent dispersion) and
+    steps down to 1e-10 still move most particles' end points."""
+    from rwpt import Engine, problems
+    dt, every, total = 0.25, 25, 300
+    P, zones, (aL, aT, _, dm) = problems.config4(ts_params=(dt, 0.0))
+    rel = problems.release_in_block(P, n, (1, 10), (76, 125), (1, 20))
+    bw = (2.0, 2.0, 1.0)
+
+    def run(windows, obs, a_l, s=1.0, grad=False, target=None):
+        Q = problems.zoned_dispersion(P, zones, a_l, aT, dm)
+        Q.porosity = P.porosity * s
+        E = Engine(Q)
+        try:
+            apply_tuning(E, TUNED_NAME[0])
+            E.load_particles(*rel)
+            out = E.adjoint(windows, obs, np.zeros(len(obs)) if target is None else target,
+                            bw, gradient=grad)
+            g = E.particles()
+            out["pos"] = np.stack([g["lx"], g["ly"], g["lz"]])
+            out["cell"] = g["cell"].copy()
+        finally:
+            E.close()
+        return out
+
+    def plume_points(windows):
+        """8 x 4 observation points over the active plume after the windows
+        (mean +- 1.5 standard deviations in x and y, middle layer)."""
+        Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+        E = Engine(Q)
+        try:
+            E.load_particles(*rel)
+            for t in windows:
+                E.advance(t)
+            on = E.particles()["status"] == 1
+            gx, gy, _ = E.global_xyz()
+        finally:
+            E.close()
+        mx, my, sx, sy = gx[on].mean(), gy[on].mean(), gx[on].std(), gy[on].std()
+        k = P.nlay // 2
+        zmid = 0.5 * (P.top[k * P.nrow * P.ncol] + P.bot[k * P.nrow * P.ncol])
+        return np.array([(x, y, zmid) for y in np.linspace(my - 1.5 * sy, my + 1.5 * sy, 4)
+                         for x in np.linspace(mx - 1.5 * sx, mx + 1.5 * sx, 8)])
+
+    def fd_check(windows, rel_step):
+        obs = plume_points(windows)
+        ref = run(windows, obs, aL)
+        tgt = 0.8 * ref["conc"]
+        fwd = run(windows, obs, aL, target=tgt)
+        A = run(windows, obs, aL, grad=True, target=tgt)
+        g = float(A["grad_al"][zones == 0].sum())
+        vals = []
+        for sg in (1, -1):
+            a2 = aL.copy()
+            a2[0] *= 1 + sg * rel_step
+            vals.append(run(windows, obs, a2, target=tgt))
+        fd = (vals[0]["misfit"] - vals[1]["misfit"]) / (2 * rel_step * aL[0])
+        moved = int(((np.abs(vals[0]["pos"] - vals[1]["pos"]).max(axis=0) > 1e2 * rel_step) |
+                     (vals[0]["cell"] != vals[1]["cell"])).sum())
+        return fwd, A, g, fd, moved
+
+    full = [dt * every * (k + 1) for k in range(total // every)]
+    fwd, A, g_full, fd_full, moved_full = fd_check(full, 1e-10)
+    _, _, g_short, fd_short, moved_short = fd_check([dt * 4], 1e-8)
+    return {"workload": "config-1 field, 8 x-bands of 25 columns with own aL, aT; Dm and "
+                        "porosity scale; 1e6 particles, 300 RW steps of 0.25 d, 12 "
+                        "checkpoint windows; misfit at 32 observation points",
+            "particles": n, "steps": total, "windows": len(full),
+            "forward_ms": fwd["ms"], "adjoint_ms": A["ms"],
+            "adjoint_over_forward": A["ms"] / fwd["ms"],
+            "misfit": A["misfit"],
+            "fd_check_4_steps": {"grad_aL_zone1": g_short, "central_difference": fd_short,
+                                 "rel_step": 1e-8, "ratio": g_short / fd_short,
+                                 "particles_moved": moved_short},
+            "fd_check_300_steps": {"grad_aL_zone1": g_full, "central_difference": fd_full,
+                                   "rel_step": 1e-10, "ratio": g_full / fd_full,
+                                   "particles_moved": moved_full,
+                                   "note": "particles whose end point moved by more than "
+                                           "100 x the relative step (cells): the "
+                                           "pathwise map is not differentiable in "
+                                           "practice over this horizon"}}
+
+
 def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
     """warmup intervals, then `steps` timed intervals from the initial state,
     each followed by the gpkde reconstruction on the flow grid; with
@@ -396,6 +492,8 @@ def main():
                          "(0: skip)")
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the 1e7-particle, 1000-step precision workload")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint (calibration) workload")
     ap.add_argument("--kde-loops", type=int, default=10,
                     help="optimisation loops of the adaptive gpkde measured on the "
                          "final plume (0: skip)")
@@ -494,6 +592,8 @@ def main():
         out["config2"] = config2_run(prec)
     if args.ensemble_intervals > 0:
         out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_run()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)
     print(json.dumps(out), flush=True)

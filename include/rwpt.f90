@@ -14,6 +14,8 @@ module RwptGpuModule
   public :: rwpt_advance, rwpt_set_time, rwpt_global_xyz, rwpt_kde, rwpt_kde_info
   public :: rwpt_kde_write, rwpt_get_substeps, rwpt_set_tuning, rwpt_last_launches
   public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
+  public :: rwpt_create_ensemble, rwpt_autotune, rwpt_write_timeseries_header
+  public :: rwpt_write_timeseries, rwpt_write_endpoints, rwpt_adjoint
 
   type, bind(C) :: rwpt_grid
     integer(c_int32_t) :: ncol, nrow, nlay
@@ -175,6 +177,67 @@ module RwptGpuModule
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: e
       real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function rwpt_create_ensemble(grid, opt, members, e) &
+        bind(C, name='rwpt_create_ensemble')
+      import :: c_int, c_ptr, c_int32_t, rwpt_grid, rwpt_options
+      type(rwpt_grid), intent(in) :: grid
+      type(rwpt_options), intent(in) :: opt
+      integer(c_int32_t), value :: members
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function rwpt_autotune(e, t_probe, budget, block, sort, min_waves, &
+        ms_best, trial_substeps) bind(C, name='rwpt_autotune')
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+      type(c_ptr), value :: e
+      real(c_double), value :: t_probe
+      integer(c_int32_t), intent(out) :: budget, block, sort, min_waves
+      real(c_double), intent(out) :: ms_best
+      integer(c_int64_t), intent(out) :: trial_substeps
+    end function
+    integer(c_int) function rwpt_write_timeseries_header(path, tracking_direction, &
+        reference_time, origin_x, origin_y, rotation) &
+        bind(C, name='rwpt_write_timeseries_header')
+      import :: c_int, c_char, c_double, c_int32_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: tracking_direction
+      real(c_double), value :: reference_time, origin_x, origin_y, rotation
+    end function
+    integer(c_int) function rwpt_write_timeseries(e, path, time_point, time_step, time, &
+        sequence, id, group, select) bind(C, name='rwpt_write_timeseries')
+      import :: c_int, c_ptr, c_char, c_double, c_int32_t
+      type(c_ptr), value :: e
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: time_point, time_step
+      real(c_double), value :: time
+      integer(c_int32_t), intent(in) :: sequence(*), id(*), group(*)
+      type(c_ptr), value :: select                  ! c_loc of integer(c_int8_t) or c_null_ptr
+    end function
+    integer(c_int) function rwpt_write_endpoints(e, path, format, tracking_direction, &
+        reference_time, origin_x, origin_y, rotation, n_groups, group_names, &
+        group_solute, group, sequence, id, initial_cell, initial_lx, initial_ly, &
+        initial_lz, initial_time, initial_face, mass) bind(C, name='rwpt_write_endpoints')
+      import :: c_int, c_ptr, c_char, c_double, c_int32_t
+      type(c_ptr), value :: e
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: format, tracking_direction, n_groups
+      real(c_double), value :: reference_time, origin_x, origin_y, rotation
+      type(c_ptr), intent(in) :: group_names(*)     ! c_loc of null-terminated names
+      integer(c_int32_t), intent(in) :: group_solute(*), group(*), sequence(*), id(*)
+      integer(c_int32_t), intent(in) :: initial_cell(*), initial_face(*)
+      real(c_double), intent(in) :: initial_lx(*), initial_ly(*), initial_lz(*)
+      real(c_double), intent(in) :: initial_time(*), mass(*)
+    end function
+    integer(c_int) function rwpt_adjoint(e, n_windows, t_windows, n_obs, obs_xyz, target, &
+        bandwidth, mass, misfit, conc, grad_al, grad_at, grad_dm, grad_porosity_scale, ms) &
+        bind(C, name='rwpt_adjoint')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: n_windows, n_obs
+      real(c_double), intent(in) :: t_windows(*), obs_xyz(*), target(*), bandwidth(3)
+      type(c_ptr), value :: mass                    ! c_loc of the masses or c_null_ptr
+      real(c_double), intent(out) :: misfit, conc(*)
+      type(c_ptr), value :: grad_al, grad_at, grad_dm, grad_porosity_scale, ms
     end function
   end interface
 end module RwptGpuModule

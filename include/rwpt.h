@@ -316,6 +316,30 @@ int rwpt_write_endpoints(rwpt_engine *e, const char *path, int32_t format,
  * Dxx Dyy Dzz Dxy Dxz Dyz (6 x 8), dry flag, sub-cell exit flag, 2 unused. */
 int rwpt_get_cell_tables(rwpt_engine *e, double *out);
 
+/* Pathwise adjoint of an observation misfit (no counterpart in the
+ * reference; the gradient of a calibration objective with respect to the
+ * transport parameters).  Advances the engine through t_windows[0..n_windows-1]
+ * exactly as n_windows rwpt_advance calls would (the windows are also the
+ * checkpoints of the reverse sweep), then evaluates at the final time
+ *   conc[o] = sum over active particles of m_p prod_d N((X_pd - x_od)/h_d)/h_d
+ *   misfit  = sum_o (conc[o] - target[o])^2
+ * with X_p the global coordinates (rwpt_global_xyz), N the standard normal
+ * density, obs_xyz 3 * n_obs global coordinates, bandwidth h[3] (dimensions
+ * with h_d = 0 are left out), mass per particle (NULL: 1).  With any
+ * gradient pointer given, also the derivative of misfit with common random
+ * numbers: grad_al, grad_at, grad_dm (ncell each, may be NULL) with respect
+ * to each cell's aLH, aTH and Dm, and grad_porosity_scale with respect to a
+ * factor s multiplying every cell's porosity (at s = 1).  The derivative is
+ * the pathwise one: each particle's branches (cells crossed, jump-limiter
+ * cuts, faces) held fixed.  Gradients need an FP64 engine, EULERIAN
+ * advection, the linear dispersion model and a 2-D or 3-D walk.  ms (may be
+ * NULL) receives the wall time of the call. */
+int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_windows,
+                 int32_t n_obs, const double *obs_xyz, const double *target,
+                 const double *bandwidth, const double *mass, double *misfit,
+                 double *conc, double *grad_al, double *grad_at, double *grad_dm,
+                 double *grad_porosity_scale, double *ms);
+
 /* Device timing of the last rwpt_advance/rwpt_kde calls, measured with HIP
  * events on the engine stream: which = 0 advance kernel, 1 histogram,
  * 2 density.  Returns the summed milliseconds and the launch count. */

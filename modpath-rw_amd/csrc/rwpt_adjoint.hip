@@ -1,0 +1,1190 @@
+// Pathwise adjoint of an observation misfit through the random walk.
+//
+// With common random numbers a particle's path is a deterministic, piecewise
+// smooth function of the flow and transport parameters: for fixed draws the
+// branches of a sub-step (jump limiter, interface loop, face choice) are
+// fixed near almost every parameter value, and inside a branch the position
+// after the sub-step is a smooth function of the position before it, of the
+// time, and of the current cell's record.  The misfit
+//   J = sum_o (c_o - target_o)^2,  c_o = sum_p m_p K_h(X_p(T) - x_o)
+// smooths the particles with a Gaussian kernel, so dJ/dX_p is defined for
+// every active particle, also across cell crossings.  The gradient of J with
+// respect to every cell's dispersivities and diffusion and to a common
+// porosity scale is then the sum over particles and sub-steps of the local
+// sensitivities, taken in reverse order (adjoint) from dJ/dX_p(T).
+//
+// Forward: the advance of the engine itself (rwpt_walk.hip) through the
+// caller's output times, with the particle state copied at every window
+// start (checkpoints).  Reverse, window by window from the last: the
+// window's walk is replayed from its checkpoint by a walk kernel that
+// records the input of every time-loop iteration (a tape) and must end
+// bit-identical to the forward state after the window (checked); then one
+// thread per active particle walks its tape backwards.  For each iteration
+// it re-runs the iteration twice in forward-mode dual numbers of four
+// directions, first the particle's x, y, z and t, then the cell's aLH, aTH,
+// Dm and the porosity scale, which gives the iteration's 4 x 8 Jacobian on
+// the branch the forward took, and applies its transpose to the adjoint of
+// the iteration's output.  Restarts to the visit start carry their adjoint to
+// the visit's first iteration.  The record's derivatives with respect to
+// the cell parameters come from central differences of the table build
+// (the records are linear in aLH, aTH, Dm and rational in the porosity).
+//
+// Scope: FP64 engines, EULERIAN advection, the linear dispersion model,
+// 2-D and 3-D walks.
+#include <stdio.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "rwpt_internal.h"
+#include "rwpt_walk_common.h"
+
+namespace rwpt {
+namespace {
+
+constexpr int kTan = F_CONN;  // record fields carrying parameter tangents
+constexpr int kBlk = 64;
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers: the value is formed by the same double
+// operation as in the plain walk, so a dual iteration takes the plain
+// iteration's branches bit for bit.
+// ---------------------------------------------------------------------------
+template <int N> struct Dual {
+  double v;
+  double d[N];
+};
+
+template <int N> __device__ __forceinline__ Dual<N> dconst(double v) {
+  Dual<N> r;
+  r.v = v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = 0.0;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N> &a, double b) {
+  Dual<N> r = a;
+  r.v = a.v + b;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(double a, const Dual<N> &b) {
+  Dual<N> r = b;
+  r.v = a + b.v;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N> &a, double b) {
+  Dual<N> r = a;
+  r.v = a.v - b;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(double a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a - b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = -b.d[k];
+  return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N> &a) {
+  Dual<N> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = -a.d[k];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N> &a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N> &a, double b) {
+  Dual<N> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(double a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a * b.d[k];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(const Dual<N> &a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(const Dual<N> &a, double b) {
+  Dual<N> r;
+  r.v = a.v / b;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / b;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(double a, const Dual<N> &b) {
+  Dual<N> r;
+  r.v = a / b.v;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = -(r.v * b.d[k]) / b.v;
+  return r;
+}
+// At 0 (a zero time step after an exit from a face, a zero velocity) the
+// derivative is taken as 0: such points have measure zero on a path.
+template <int N> __device__ __forceinline__ Dual<N> dsqrt(const Dual<N> &a) {
+  Dual<N> r;
+  r.v = sqrt(a.v);
+  const double q = r.v > 0.0 ? 1.0 / (2.0 * r.v) : 0.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * q;
+  return r;
+}
+
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double val(double a) { return a; }
+template <int N> __device__ __forceinline__ double val(const Dual<N> &a) { return a.v; }
+template <typename T> struct Cst;
+template <> struct Cst<double> {
+  static __device__ __forceinline__ double of(double v) { return v; }
+};
+template <int N> struct Cst<Dual<N>> {
+  static __device__ __forceinline__ Dual<N> of(double v) { return dconst<N>(v); }
+};
+
+// Cell record as the arithmetic type sees it: plain doubles, or duals whose
+// parameter directions carry the record's tangents.
+template <typename T> struct RecT;
+template <> struct RecT<double> {
+  const double *rec;
+  __device__ __forceinline__ double operator()(int f) const { return rec[f]; }
+};
+// Directions P0..N-1 of a dual record carry the tangents of the cell
+// parameters (aLH, aTH, Dm, porosity scale); P0 = N: no parameter
+// directions.
+template <int N, int P0> struct RecD {
+  const double *rec;
+  const double *tan;  // this cell's tangent rows: tan[k * stride + f]
+  int64_t stride;     // ncell * kTan
+  __device__ __forceinline__ Dual<N> operator()(int f) const {
+    Dual<N> r = dconst<N>(rec[f]);
+    if (P0 < N && f < kTan) {
+#pragma unroll
+      for (int k = 0; k < N - P0; ++k) r.d[P0 + k] = tan[k * stride + f];
+    }
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// One time-loop iteration of a cell visit for the EULERIAN linear-model walk,
+// written once for doubles and duals.  Every double operation, its operand
+// order and every branch test are those of iterate() in rwpt_walk.hip
+// (TrackSubCell.F90:956-1368); the replay checks the equality bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T> struct VisitG {
+  T x, y, z, t;      // location and time
+  T ix, iy, iz, t0;  // visit start
+  int restarts, exitFace;
+};
+template <typename T> struct TermsG {
+  T vx, vy, vz, dAx, dAy, dAz, dBx, dBy, dBz, ddx, ddy, ddz;
+};
+
+template <typename T, typename Rec>
+__device__ __forceinline__ T tri8g(const Rec &rec, int b, const T &x, const T &y,
+                                   const T &z) {
+  const T v00 = (1.0 - x) * rec(b + 0) + x * rec(b + 1);
+  const T v01 = (1.0 - x) * rec(b + 4) + x * rec(b + 5);
+  const T v10 = (1.0 - x) * rec(b + 2) + x * rec(b + 3);
+  const T v11 = (1.0 - x) * rec(b + 6) + x * rec(b + 7);
+  const T v0 = (1.0 - y) * v00 + y * v10;
+  const T v1 = (1.0 - y) * v01 + y * v11;
+  return (1.0 - z) * v0 + z * v1;
+}
+
+template <typename T, typename Rec>
+__device__ __forceinline__ T deriv4g(const Rec &rec, int b, const T &u, const T &w) {
+  const T v0 = (1.0 - u) * rec(b + 0) + u * rec(b + 1);
+  const T v1 = (1.0 - u) * rec(b + 2) + u * rec(b + 3);
+  return (1.0 - w) * v0 + w * v1;
+}
+
+template <typename T>
+__device__ __forceinline__ void advect_g(const T &dt, TermsG<T> &s) {
+  s.dAx = s.vx * dt;
+  s.dAy = s.vy * dt;
+  s.dAz = s.vz * dt;
+}
+
+template <typename T, typename Rec, int NDIM>
+__device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
+                                const T &y, const T &z, const T &dt, Stream &rng,
+                                TermsG<T> &s) {
+  s.vx = (1.0 - x) * rec(F_VX1) + x * rec(F_VX2);
+  s.vy = (1.0 - y) * rec(F_VY1) + y * rec(F_VY2);
+  s.vz = (1.0 - z) * rec(F_VZ1) + z * rec(F_VZ2);
+  const T aL = rec(F_ALH), aT = rec(F_ATH), aTH = aT;
+  const T t0 = deriv4g<T>(rec, F_DD + 0, y, z), t1 = deriv4g<T>(rec, F_DD + 4, x, z);
+  const T t2 = deriv4g<T>(rec, F_DD + 8, x, y), t3 = deriv4g<T>(rec, F_DD + 12, y, z);
+  const T t4 = deriv4g<T>(rec, F_DD + 16, y, z), t5 = deriv4g<T>(rec, F_DD + 20, x, z);
+  const T t6 = deriv4g<T>(rec, F_DD + 24, x, z), t7 = deriv4g<T>(rec, F_DD + 28, x, y);
+  const T t8 = deriv4g<T>(rec, F_DD + 32, x, y);
+  const T Rt = rec(F_R), phi = rec(F_PHI);
+  s.ddx = ((t0 + t5 + t7) / phi) / Rt;
+  s.ddy = ((t3 + t1 + t8) / phi) / Rt;
+  s.ddz = ((t4 + t6 + t2) / phi) / Rt;
+  const T dm = rec(F_DMEFF);
+  const T zero = Cst<T>::of(0.0);
+  if constexpr (NDIM == 3) {
+    const T bx = tri8g<T>(rec, F_VC, x, y, z), by = tri8g<T>(rec, F_VC + 8, x, y, z);
+    const T bz = tri8g<T>(rec, F_VC + 16, x, y, z);
+    const T vn = dsqrt(bx * bx + by * by + bz * bz);
+    const T vxy = dsqrt(bx * bx + by * by);
+    T B11 = zero, B12 = zero, B13 = zero, B21 = zero, B22 = zero, B23 = zero, B31 = zero,
+      B32 = zero;
+    if (val(vn) > 0.0) {
+      const T sl = dsqrt((2.0 * (aL * vn + dm)) / Rt);
+      const T st = dsqrt((2.0 * (aT * vn + dm)) / Rt);
+      B11 = (bx * sl) / vn;
+      B21 = (by * sl) / vn;
+      B31 = (bz * sl) / vn;
+      B32 = (vxy * st) / vn;
+      if (val(vxy) > 0.0) {
+        const T sh = dsqrt((2.0 * (aTH * vn + dm)) / Rt);
+        B12 = -((((bx * bz) * st) / vn) / vxy);
+        B13 = -((by * sh) / vxy);
+        B22 = -((((by * bz) * st) / vn) / vxy);
+        B23 = (bx * sh) / vxy;
+      }
+    }
+    const double r1 = Normal<double, 0>::draw(rng), r2 = Normal<double, 0>::draw(rng);
+    const double r3 = Normal<double, 0>::draw(rng);
+    s.dBx = B11 * r1 + B12 * r2 + B13 * r3;
+    s.dBy = B21 * r1 + B22 * r2 + B23 * r3;
+    s.dBz = B31 * r1 + B32 * r2;
+  } else {
+    const int d1 = W.dims[0] - 1, d2 = W.dims[1] - 1;
+    const T b1 = tri8g<T>(rec, F_VC + 8 * d1, x, y, z);
+    const T b2 = tri8g<T>(rec, F_VC + 8 * d2, x, y, z);
+    const T vn = dsqrt(b1 * b1 + b2 * b2);
+    T B11 = zero, B12 = zero, B21 = zero, B22 = zero;
+    if (val(vn) > 0.0) {
+      const T sl = dsqrt((2.0 * (aL * vn + dm)) / Rt);
+      const T st = dsqrt((2.0 * (aTH * vn + dm)) / Rt);
+      B11 = (b1 * sl) / vn;
+      B21 = (b2 * sl) / vn;
+      B12 = -((b2 * st) / vn);
+      B22 = (b1 * st) / vn;
+    }
+    const double r1 = Normal<double, 0>::draw(rng), r2 = Normal<double, 0>::draw(rng);
+    const T u1 = B11 * r1 + B12 * r2, u2 = B21 * r1 + B22 * r2;
+    s.dBx = d1 == 0 ? u1 : (d2 == 0 ? u2 : zero);
+    s.dBy = d1 == 1 ? u1 : (d2 == 1 ? u2 : zero);
+    s.dBz = d1 == 2 ? u1 : (d2 == 2 ? u2 : zero);
+  }
+  advect_g(dt, s);
+}
+
+// One axis of pr_DetectExitFaceAndUpdateTimeStepQuadratic.
+template <typename T>
+__device__ bool quadratic_axis_g(const T &dI, const T &A, const T &B, const T &dtcur,
+                                 T &dtaxis) {
+  const T arg = B * B + 4.0 * dI * A;
+  if ((val(arg) >= 0.0) && (val(A) != 0.0)) {
+    const T sq = dsqrt(arg);
+    const T z1 = (-B + sq) / (2.0 * A);
+    const T z2 = (-B - sq) / (2.0 * A);
+    if (val(z1) > 0.0 || val(z2) > 0.0) {
+      const T m = (val(z1) > 0.0 && val(z2) > 0.0) ? (val(z2) < val(z1) ? z2 : z1)
+                                                   : (val(z1) > 0.0 ? z1 : z2);
+      dtaxis = m * m;
+      if (val(dtaxis) == 0.0) return true;
+    } else if (val(z1) == 0.0 || val(z2) == 0.0) {
+      return true;
+    }
+  } else if (val(A) == 0.0) {
+    if (val(B) != 0.0) {
+      const T r = dI / B;
+      dtaxis = r * r;
+      if (val(dtaxis) == 0.0) return true;
+    }
+  }
+  if (val(dtaxis) > val(dtcur)) dtaxis = Cst<T>::of(0.0);
+  return false;
+}
+
+template <typename T, typename Rec>
+__device__ void exit_time_g(const Rec &rec, const T &x, const T &y, const T &z,
+                            const T &nx, const T &ny, const T &nz, const TermsG<T> &s,
+                            T &t, T &dt, int &exitFace) {
+  t = t - dt;
+  const T pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
+  const T dd[3] = {rec(F_DX), rec(F_DY), rec(F_DZ)};
+  const T v[3] = {s.vx, s.vy, s.vz};
+  const T dv[3] = {s.ddx, s.ddy, s.ddz};
+  const T db[3] = {s.dBx, s.dBy, s.dBz};
+  T dtx[3] = {Cst<T>::of(0.0), Cst<T>::of(0.0), Cst<T>::of(0.0)};
+  int fx[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if ((val(pn[a]) > 1.0) || (val(pn[a]) < 0.0)) {
+      T dI;
+      if (val(pn[a]) > 1.0) {
+        dI = dd[a] * (1.0 - pp[a]);
+        fx[a] = 2 * a + 2;
+      } else {
+        dI = (-dd[a]) * pp[a];
+        fx[a] = 2 * a + 1;
+      }
+      if (val(dI) == 0.0) {
+        exitFace = fx[a];
+        dt = Cst<T>::of(0.0);
+        return;
+      }
+      if (quadratic_axis_g(dI, v[a] + dv[a], db[a], dt, dtx[a])) {
+        exitFace = fx[a];
+        dt = Cst<T>::of(0.0);
+        return;
+      }
+    }
+  }
+  if (val(dtx[0]) == 0.0 && val(dtx[1]) == 0.0 && val(dtx[2]) == 0.0) {
+    t = t + dt;
+    return;
+  }
+  int im = -1;
+  double best = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double va = val(dtx[a]);
+    if (va > 0.0 && (im < 0 || va < best)) {
+      im = a;
+      best = va;
+    }
+  }
+  dt = im == 0 ? dtx[0] : (im == 1 ? dtx[1] : dtx[2]);
+  exitFace = im == 0 ? fx[0] : (im == 1 ? fx[1] : fx[2]);
+  t = t + dt;
+}
+
+template <typename T>
+__device__ __forceinline__ bool outside_g(const T &x, const T &y, const T &z) {
+  return outside(val(x), val(y), val(z));
+}
+
+// *restart: 1 when the iteration returned to the visit start (location and
+// time), 2 when an elastic rebound with a zero time step returned it to the
+// visit start location only.
+template <typename T, typename Rec, int NDIM>
+__device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
+                         Stream &rng, bool &zero_dt, int &restart) {
+  const T dtcell = rec(F_DTCELL);
+  const double maxRelJump = 0.5, dtRed = f32lit(0.1f);
+  const T DX = rec(F_DX), DY = rec(F_DY), DZ = rec(F_DZ);
+  const int32_t *mb = ints(rec.rec) + Lay<double>::MASSBND;
+  T x = V.x, y = V.y, z = V.z;
+  T t = V.t;
+  T nx, ny, nz, dt;
+  int exitFace = V.exitFace;
+  bool atMax = false;
+  TermsG<T> s;
+  dt = dtcell;
+  t = t + dt;
+  if (W.t_stop < val(t)) {
+    t = t - dt;
+    const T rem = W.t_stop - t;
+    dt = val(rem) > 0.0 ? rem : Cst<T>::of(fmax(val(rem), 0.0));
+    t = Cst<T>::of(W.t_stop);
+    atMax = true;
+  }
+  displacements_g<T, Rec, NDIM>(rec, W, x, y, z, dt, rng, s);
+  T sq = dsqrt(dt);
+  T dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+  T dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+  T dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+  while ((fabs(val(dxr) / val(DX)) > maxRelJump) || (fabs(val(dyr) / val(DY)) > maxRelJump) ||
+         (fabs(val(dzr) / val(DZ)) > maxRelJump)) {
+    t = t - dt;
+    dt = dtRed * dt;
+    advect_g(dt, s);
+    sq = dsqrt(dt);
+    dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+    dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+    dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+    t = t + dt;
+    atMax = false;
+  }
+  nx = x + dxr / DX;
+  ny = y + dyr / DY;
+  nz = z + dzr / DZ;
+  int loops = 0;
+  while (outside_g(nx, ny, nz)) {
+    loops = loops + 1;
+    exit_time_g<T>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
+    if (loops > 5) {
+      nx = V.ix; ny = V.iy; nz = V.iz;
+      t = V.t0;
+      dt = dtcell;
+      V.restarts = V.restarts + 1;
+      restart = 1;
+      if (V.restarts > 10) return IT_UNDEFINED;
+      break;
+    }
+    advect_g(dt, s);
+    atMax = false;
+    if (exitFace == 0) {
+      nx = V.ix; ny = V.iy; nz = V.iz;
+      t = V.t0;
+      dt = dtcell;
+      V.restarts = V.restarts + 1;
+      restart = 1;
+      if (V.restarts > 10) return IT_UNDEFINED;
+      break;
+    }
+    sq = dsqrt(dt);
+    if (exitFace <= 2) {
+      nx = Cst<T>::of(exitFace == 1 ? 0.0 : 1.0);
+      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      ny = y + dyr / DY;
+      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      nz = z + dzr / DZ;
+    } else if (exitFace <= 4) {
+      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      nx = x + dxr / DX;
+      ny = Cst<T>::of(exitFace == 3 ? 0.0 : 1.0);
+      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      nz = z + dzr / DZ;
+    } else {
+      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      nx = x + dxr / DX;
+      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      ny = y + dyr / DY;
+      nz = Cst<T>::of(exitFace == 5 ? 0.0 : 1.0);
+    }
+    if (outside_g(nx, ny, nz)) continue;
+    int rebounds = 0;
+    while (exitFace > 0) {
+      if (mb[exitFace - 1] != 1) break;
+      rebounds = rebounds + 1;
+      if (rebounds > 10) return IT_UNDEFINED;
+      const T xi = nx, yi = ny, zi = nz;
+      if (val(dt) == 0.0) {
+        nx = V.ix; ny = V.iy; nz = V.iz;
+        t = t - dt;
+        dt = dtcell;
+        exitFace = 0;
+        V.restarts = V.restarts + 1;
+        restart = 2;
+        break;
+      }
+      t = t + dt;
+      if (W.t_stop < val(t)) {
+        t = t - dt;
+        const T rem = W.t_stop - t;
+        dt = val(rem) > 0.0 ? rem : Cst<T>::of(fmax(val(rem), 0.0));
+        t = Cst<T>::of(W.t_stop);
+        atMax = true;
+        advect_g(dt, s);
+        sq = dsqrt(dt);
+        dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+        dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+        dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      }
+      if (exitFace <= 2) {
+        nx = nx - dxr / DX;
+        ny = ny + dyr / DY;
+        nz = nz + dzr / DZ;
+      } else if (exitFace <= 4) {
+        nx = nx + dxr / DX;
+        ny = ny - dyr / DY;
+        nz = nz + dzr / DZ;
+      } else {
+        nx = nx + dxr / DX;
+        ny = ny + dyr / DY;
+        nz = nz - dzr / DZ;
+      }
+      if (outside_g(nx, ny, nz)) {
+        x = xi; y = yi; z = zi;
+        if (exitFace <= 2) {
+          s.vx = -s.vx; s.ddx = -s.ddx; s.dBx = -s.dBx;
+        } else if (exitFace <= 4) {
+          s.vy = -s.vy; s.ddy = -s.ddy; s.dBy = -s.dBy;
+        } else {
+          s.vz = -s.vz; s.ddz = -s.ddz; s.dBz = -s.dBz;
+        }
+        exitFace = 0;
+      } else {
+        dt = dtcell;
+        exitFace = 0;
+        if (val(nx) == 0.0) exitFace = 1;
+        if (val(nx) == 1.0) exitFace = 2;
+        if (val(ny) == 0.0) exitFace = 3;
+        if (val(ny) == 1.0) exitFace = 4;
+        if (val(nz) == 0.0) exitFace = 5;
+        if (val(nz) == 1.0) exitFace = 6;
+      }
+    }
+  }
+  V.x = nx; V.y = ny; V.z = nz; V.t = t;
+  V.exitFace = exitFace;
+  if (atMax) return IT_MAXTIME;
+  if (exitFace > 0) {
+    zero_dt = val(dt) == 0.0;
+    return IT_EXIT;
+  }
+  return IT_CONTINUE;
+}
+
+// ---------------------------------------------------------------------------
+// Tape of one window: the input of every time-loop iteration of the
+// particles that are active at the final time.
+// ---------------------------------------------------------------------------
+struct TapeEntry {
+  double x, y, z, t, ix, iy, iz, t0;
+  uint64_t ra, rb;
+  int32_t cell;
+  int32_t flags;  // bit 0 first iteration of a visit, 1-4 restarts, 5-7 exitFace
+};
+
+// Particle arrays of a checkpoint (or of the engine after the window).
+struct CkPtrs {
+  int32_t *cell, *status;
+  double *lx, *ly, *lz, *t;
+  uint64_t *rng;
+  int64_t *substeps;
+};
+
+__device__ __forceinline__ bool same(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// Replay of one window for the listed particles from checkpoint `in`,
+// recording the tape; the end state must equal `want` bit for bit (the
+// listed particle count of mismatches goes to *bad).  The start of an
+// advance (prepare_advance) and the visit loop (walk_kernel) without launch
+// budget: one thread walks its particle to the end of the window.
+template <int NDIM>
+__global__ void __launch_bounds__(kBlk) adj_replay(
+    const double *__restrict__ cells, const int32_t *__restrict__ sel, int64_t nsel,
+    CkPtrs in, CkPtrs want, const WalkParams W, const int64_t *__restrict__ off,
+    TapeEntry *__restrict__ tape, unsigned long long *__restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const int p = sel[i];
+  int st = in.status[p];
+  int c = in.cell[p];
+  VisitG<double> V;
+  V.x = in.lx[p]; V.y = in.ly[p]; V.z = in.lz[p]; V.t = in.t[p];
+  V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
+  V.restarts = 0;
+  V.exitFace = 0;
+  Stream rng{in.rng[2 * p], in.rng[2 * p + 1]};
+  if (st == 0) {
+    if (V.t < W.t_engine) st = 8;
+    else if (V.t <= W.t_stop) {
+      st = 1;
+      if (ints(cells + (int64_t)(c - 1) * CELL_DOUBLES)[Lay<double>::FLAGS] == 0) st = 7;
+    }
+  }
+  if (st == 7 && (ints(cells + (int64_t)(c - 1) * CELL_DOUBLES)[Lay<double>::FLAGS + 1] &
+                  CF_VERIFY_PDRY))
+    st = 1;
+  const int64_t base = off[i], cap = off[i + 1] - off[i];
+  int64_t k = 0;
+  int status = st;
+  bool overflow = false;
+  if (st == 1) {
+    Aux A{0, 0, 0, 0, 0};
+    bool first = false;
+    while (true) {
+      const double *rec = cells + (int64_t)(c - 1) * CELL_DOUBLES;
+      if (!A.phase) {
+        const int ec = entry_checks(rec, W, V.x, V.y, V.z);
+        if (ec != 0) {
+          status = ec;
+          if (ec == 9) last_point(cells, A.entryFace, c, V.x, V.y, V.z);
+          break;
+        }
+        A.phase = 1;
+        V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
+        V.restarts = 0;
+        V.exitFace = 0;
+        first = true;
+      }
+      if (k >= cap) {
+        overflow = true;
+        break;
+      }
+      TapeEntry E;
+      E.x = V.x; E.y = V.y; E.z = V.z; E.t = V.t;
+      E.ix = V.ix; E.iy = V.iy; E.iz = V.iz; E.t0 = V.t0;
+      E.ra = rng.a; E.rb = rng.b;
+      E.cell = c;
+      E.flags = (first ? 1 : 0) | (V.restarts << 1) | (V.exitFace << 5);
+      tape[base + k] = E;
+      k += 1;
+      first = false;
+      bool zero_dt = false;
+      int restart = 0;
+      const int out = iterate_g<double, RecT<double>, NDIM>(RecT<double>{rec}, W, V, rng, zero_dt, restart);
+      if (out == IT_CONTINUE) continue;
+      if (out == IT_MAXTIME) {
+        status = 1;
+        break;
+      }
+      if (out == IT_UNDEFINED) {
+        status = 9;
+        V.x = V.ix; V.y = V.iy; V.z = V.iz; V.t = V.t0;
+        last_point(cells, A.entryFace, c, V.x, V.y, V.z);
+        break;
+      }
+      const int face = V.exitFace;
+      A.stalls = zero_dt ? A.stalls + 1 : 0;
+      if (A.stalls > kMaxStalls) {
+        status = 9;
+        break;
+      }
+      const int next = ints(rec)[Lay<double>::CONN + face - 1];
+      bool inactive_next = next <= 0;
+      if (!inactive_next)
+        inactive_next =
+            ints(cells + (int64_t)(next - 1) * CELL_DOUBLES)[Lay<double>::FLAGS] == 0;
+      if (inactive_next) {
+        status = 2;
+        break;
+      }
+      bool badface;
+      double fx = V.x, fy = V.y, fz = V.z;
+      switch (face) {
+        case 1: badface = fx != 0.0; fx = 1.0; break;
+        case 2: badface = fx != 1.0; fx = 0.0; break;
+        case 3: badface = fy != 0.0; fy = 1.0; break;
+        case 4: badface = fy != 1.0; fy = 0.0; break;
+        case 5: badface = fz != 0.0; fz = 1.0; break;
+        default: badface = fz != 1.0; fz = 0.0; break;
+      }
+      if (badface) {
+        status = 9;
+        break;
+      }
+      c = next;
+      V.x = fx; V.y = fy; V.z = fz;
+      A.phase = 0;
+      A.entryFace = face <= 2 ? 3 - face : (face <= 4 ? 7 - face : 11 - face);
+    }
+  }
+  const bool ok = !overflow && k == cap && c == want.cell[p] && status == want.status[p] &&
+                  same(V.x, want.lx[p]) && same(V.y, want.ly[p]) && same(V.z, want.lz[p]) &&
+                  same(V.t, want.t[p]) && rng.a == want.rng[2 * p] &&
+                  rng.b == want.rng[2 * p + 1];
+  if (!ok) atomicAdd(bad, 1ull);
+}
+
+// Iterations of each listed particle in the window, from the sub-step
+// counters before and after it.
+__global__ void adj_counts(const int32_t *__restrict__ sel, int64_t nsel,
+                           const int64_t *__restrict__ before,
+                           const int64_t *__restrict__ after, int64_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsel) cnt[i] = after[sel[i]] - before[sel[i]];
+  if (i == nsel) cnt[i] = 0;
+}
+
+// Gradient sums of one particle's visit to one cell, added to the cell.
+__device__ __forceinline__ void flush(int c, const double g[4], double *g_al, double *g_at,
+                                      double *g_dm, double &g_s) {
+  if (c <= 0) return;
+  if (g_al) unsafeAtomicAdd(&g_al[c - 1], g[0]);
+  if (g_at) unsafeAtomicAdd(&g_at[c - 1], g[1]);
+  if (g_dm) unsafeAtomicAdd(&g_dm[c - 1], g[2]);
+  g_s += g[3];
+}
+
+// Reverse sweep of one window: lam (4 per listed particle: x y z t) holds
+// the adjoint of the state after the window on entry and of the state
+// before it on return.
+template <int NDIM>
+__global__ void __launch_bounds__(kBlk) adj_backward(
+    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
+    int64_t nsel, const WalkParams W, const int64_t *__restrict__ off,
+    const TapeEntry *__restrict__ tape, double *__restrict__ lam, double *g_al,
+    double *g_at, double *g_dm, double *g_s) {
+  using D = Dual<4>;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double gs = 0.0;
+  if (i < nsel) {
+    double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
+    double mu[4] = {0.0, 0.0, 0.0, 0.0};
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    int cur = 0;
+    const bool any = l[0] != 0.0 || l[1] != 0.0 || l[2] != 0.0 || l[3] != 0.0;
+    for (int64_t k = off[i + 1] - 1; any && k >= off[i]; --k) {
+      const TapeEntry E = tape[k];
+      if (E.cell != cur) {
+        flush(cur, g, g_al, g_at, g_dm, gs);
+        g[0] = g[1] = g[2] = g[3] = 0.0;
+        cur = E.cell;
+      }
+      const int64_t c0 = (int64_t)(E.cell - 1);
+      const double *crec = cells + c0 * CELL_DOUBLES, *ctan = tan + c0 * kTan;
+      const int rs = (E.flags >> 1) & 15, ef = (E.flags >> 5) & 7;
+      // pass 1: directions x y z t, the record held fixed
+      VisitG<D> V;
+      V.x = dconst<4>(E.x); V.x.d[0] = 1.0;
+      V.y = dconst<4>(E.y); V.y.d[1] = 1.0;
+      V.z = dconst<4>(E.z); V.z.d[2] = 1.0;
+      V.t = dconst<4>(E.t); V.t.d[3] = 1.0;
+      V.ix = dconst<4>(E.ix); V.iy = dconst<4>(E.iy);
+      V.iz = dconst<4>(E.iz); V.t0 = dconst<4>(E.t0);
+      V.restarts = rs;
+      V.exitFace = ef;
+      Stream rng{E.ra, E.rb};
+      bool zero_dt = false;
+      int restart = 0;
+      const int out = iterate_g<D, RecD<4, 4>, NDIM>(RecD<4, 4>{crec, ctan, ncell * kTan}, W,
+                                                    V, rng, zero_dt, restart);
+      // the face coordinate after an exit is the constant face value
+      const int ax = (out == IT_EXIT && V.exitFace > 0) ? (V.exitFace - 1) / 2 : -1;
+      double lo[4] = {ax == 0 ? 0.0 : l[0], ax == 1 ? 0.0 : l[1], ax == 2 ? 0.0 : l[2], l[3]};
+      double ln[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        ln[j] = lo[0] * V.x.d[j] + lo[1] * V.y.d[j] + lo[2] * V.z.d[j] + lo[3] * V.t.d[j];
+      // pass 2: directions aLH aTH Dm porosity scale, the state held fixed
+      {
+        VisitG<D> U;
+        U.x = dconst<4>(E.x); U.y = dconst<4>(E.y); U.z = dconst<4>(E.z);
+        U.t = dconst<4>(E.t);
+        U.ix = dconst<4>(E.ix); U.iy = dconst<4>(E.iy);
+        U.iz = dconst<4>(E.iz); U.t0 = dconst<4>(E.t0);
+        U.restarts = rs;
+        U.exitFace = ef;
+        Stream r2{E.ra, E.rb};
+        bool z2 = false;
+        int rs2 = 0;
+        (void)iterate_g<D, RecD<4, 0>, NDIM>(RecD<4, 0>{crec, ctan, ncell * kTan}, W, U, r2,
+                                            z2, rs2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          g[j] += lo[0] * U.x.d[j] + lo[1] * U.y.d[j] + lo[2] * U.z.d[j] + lo[3] * U.t.d[j];
+      }
+      if (restart == 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mu[q] += lo[q];
+      } else if (restart == 2) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) mu[q] += lo[q];
+      }
+      if (E.flags & 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ln[q] += mu[q];
+          mu[q] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) l[q] = ln[q];
+    }
+    flush(cur, g, g_al, g_at, g_dm, gs);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lam[4 * i + q] = l[q];
+  }
+  for (int o = 32; o > 0; o >>= 1) gs += __shfl_down(gs, o, 64);
+  if (__lane_id() == 0 && gs != 0.0) unsafeAtomicAdd(g_s, gs);
+}
+
+// Concentrations at the observation points from the active particles'
+// global coordinates: c_o += m_p prod_d N((X_d - x_od) / h_d) / h_d over the
+// dimensions with h_d > 0.
+__global__ void adj_conc(int64_t n, const int32_t *__restrict__ status,
+                         const double *__restrict__ gx, const double *__restrict__ gy,
+                         const double *__restrict__ gz, const double *__restrict__ mass,
+                         int nobs, const double *__restrict__ obs, double h0, double h1,
+                         double h2, double *__restrict__ conc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = p < n && status[p] == 1;
+  const double X[3] = {on ? gx[p] : 0.0, on ? gy[p] : 0.0, on ? gz[p] : 0.0};
+  const double m = on ? (mass ? mass[p] : 1.0) : 0.0;
+  const double h[3] = {h0, h1, h2};
+  const double c = 0.3989422804014327;  // 1 / sqrt(2 pi)
+  for (int o = 0; o < nobs; ++o) {
+    double k = m;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (h[d] > 0.0) {
+        const double u = (X[d] - obs[3 * o + d]) / h[d];
+        k = k * (c * exp(-0.5 * u * u) / h[d]);
+      }
+    for (int s = 32; s > 0; s >>= 1) k += __shfl_down(k, s, 64);
+    if (__lane_id() == 0 && k != 0.0) unsafeAtomicAdd(&conc[o], k);
+  }
+}
+
+// Adjoint of the listed particles' local coordinates from the misfit:
+// dJ/dX_d = sum_o 2 (c_o - target_o) m K_o (-(X_d - x_od) / h_d^2), times
+// dX/dlx = right - left of the cell (ConvertToModelXYZ), and 0 for t.
+__global__ void adj_seed(const int32_t *__restrict__ sel, int64_t nsel,
+                         const int32_t *__restrict__ cellv, const double *__restrict__ gx,
+                         const double *__restrict__ gy, const double *__restrict__ gz,
+                         const double *__restrict__ mass, int nobs,
+                         const double *__restrict__ obs, const double *__restrict__ resid,
+                         double h0, double h1, double h2, int ncol, int nrow,
+                         const double *__restrict__ delr, const double *__restrict__ delc,
+                         const double *__restrict__ top, const double *__restrict__ bot,
+                         double *__restrict__ lam) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const int p = sel[i];
+  const double X[3] = {gx[p], gy[p], gz[p]};
+  const double m = mass ? mass[p] : 1.0;
+  const double h[3] = {h0, h1, h2};
+  const double c = 0.3989422804014327;
+  double gX[3] = {0.0, 0.0, 0.0};
+  for (int o = 0; o < nobs; ++o) {
+    double k = m, u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (h[d] > 0.0) {
+        u[d] = (X[d] - obs[3 * o + d]) / h[d];
+        k = k * (c * exp(-0.5 * u[d] * u[d]) / h[d]);
+      }
+    const double r = 2.0 * resid[o] * k;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (h[d] > 0.0) gX[d] += r * (-u[d] / h[d]);
+  }
+  const int c0 = cellv[p] - 1;
+  const int lc = nrow * ncol, rr = c0 % lc, ii = rr / ncol, jj = rr % ncol;
+  // right - left, back - front and top - bottom of the cell as
+  // global_xyz_kernel forms them
+  const double wx = (delr[jj] / 2.0) + (delr[jj] / 2.0);
+  const double wy = (delc[ii] / 2.0) + (delc[ii] / 2.0);
+  const double wz = top[c0] - bot[c0];
+  lam[4 * i + 0] = gX[0] * wx;
+  lam[4 * i + 1] = gX[1] * wy;
+  lam[4 * i + 2] = gX[2] * wz;
+  lam[4 * i + 3] = 0.0;
+}
+
+__global__ void copy_ck(int64_t n, const int32_t *__restrict__ cell,
+                        const int32_t *__restrict__ status, const double *__restrict__ lx,
+                        const double *__restrict__ ly, const double *__restrict__ lz,
+                        const double *__restrict__ t, const uint64_t *__restrict__ rng,
+                        const int64_t *__restrict__ substeps, CkPtrs out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  out.cell[p] = cell[p];
+  out.status[p] = status[p];
+  out.lx[p] = lx[p];
+  out.ly[p] = ly[p];
+  out.lz[p] = lz[p];
+  out.t[p] = t[p];
+  out.rng[2 * p] = rng[2 * p];
+  out.rng[2 * p + 1] = rng[2 * p + 1];
+  out.substeps[p] = substeps[p];
+}
+
+}  // namespace
+}  // namespace rwpt
+
+using namespace rwpt;
+
+namespace {
+
+int adj_fail(rwpt_engine *e, int code, const char *msg) {
+  e->err = msg;
+  return code;
+}
+
+int adj_hip(rwpt_engine *e, hipError_t err, const char *where) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "adjoint %s: %s", where, hipGetErrorString(err));
+  e->err = buf;
+  return err == hipErrorOutOfMemory ? RWPT_E_NOMEM : RWPT_E_DEVICE;
+}
+
+// Device buffers of one rwpt_adjoint call, freed on every return.
+struct AdjBuffers {
+  std::vector<void *> ptrs;
+  template <class T> hipError_t alloc(T **p, size_t count) {
+    *p = nullptr;
+    hipError_t err = hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+    if (err == hipSuccess) ptrs.push_back(*p);
+    return err;
+  }
+  ~AdjBuffers() {
+    for (void *q : ptrs) (void)hipFree(q);
+  }
+};
+
+}  // namespace
+
+extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_windows,
+                            int32_t n_obs, const double *obs_xyz, const double *target,
+                            const double *bandwidth, const double *mass, double *misfit,
+                            double *conc, double *grad_al, double *grad_at, double *grad_dm,
+                            double *grad_porosity_scale, double *ms) {
+  if (!e) return RWPT_E_ARG;
+  if (n_windows < 1 || !t_windows) return adj_fail(e, RWPT_E_ARG, "n_windows >= 1 times");
+  for (int w = 0; w < n_windows; ++w)
+    if (!std::isfinite(t_windows[w])) return adj_fail(e, RWPT_E_ARG, "window time not finite");
+  if (n_obs < 1 || !obs_xyz || !target || !bandwidth)
+    return adj_fail(e, RWPT_E_ARG, "observation points, targets and bandwidths required");
+  for (int d = 0; d < 3; ++d)
+    if (!(bandwidth[d] >= 0.0) || !std::isfinite(bandwidth[d]))
+      return adj_fail(e, RWPT_E_ARG, "bandwidths must be finite and >= 0");
+  if (!(bandwidth[0] > 0.0 || bandwidth[1] > 0.0 || bandwidth[2] > 0.0))
+    return adj_fail(e, RWPT_E_ARG, "at least one bandwidth must be > 0");
+  const bool want_grad = grad_al || grad_at || grad_dm || grad_porosity_scale;
+  if (want_grad) {
+    if (e->opt.precision != 0)
+      return adj_fail(e, RWPT_E_UNSUPPORTED, "the adjoint needs an FP64 engine");
+    if (e->opt.advection_kind != 2)
+      return adj_fail(e, RWPT_E_UNSUPPORTED, "the adjoint needs EULERIAN advection");
+    if (e->prob.disp_model != 0)
+      return adj_fail(e, RWPT_E_UNSUPPORTED, "the adjoint needs the linear dispersion model");
+  }
+  int ndim = 0;
+  for (int d = 0; d < 3; ++d) ndim += e->opt.dimension_mask[d] ? 1 : 0;
+  if (want_grad && ndim < 2)
+    return adj_fail(e, RWPT_E_UNSUPPORTED, "the adjoint needs a 2-D or 3-D walk");
+  Particles &P = e->part;
+  DeviceProblem &G = e->prob;
+  const int64_t n = P.n;
+  hipStream_t st = e->stream;
+  AdjBuffers B;
+  hipError_t err = hipSuccess;
+  const int blk = 256;
+  const dim3 pgrid((unsigned)((std::max<int64_t>(n, 1) + blk - 1) / blk));
+  const auto t_begin = std::chrono::steady_clock::now();
+
+  // checkpoints at every window start
+  std::vector<CkPtrs> ck(want_grad ? n_windows : 0);
+  std::vector<WalkParams> Ws(n_windows);
+  for (size_t w = 0; w < ck.size(); ++w) {
+    CkPtrs &C = ck[w];
+    if ((err = B.alloc(&C.cell, n)) != hipSuccess || (err = B.alloc(&C.status, n)) != hipSuccess ||
+        (err = B.alloc(&C.lx, n)) != hipSuccess || (err = B.alloc(&C.ly, n)) != hipSuccess ||
+        (err = B.alloc(&C.lz, n)) != hipSuccess || (err = B.alloc(&C.t, n)) != hipSuccess ||
+        (err = B.alloc(&C.rng, 2 * n)) != hipSuccess ||
+        (err = B.alloc(&C.substeps, n)) != hipSuccess)
+      return adj_hip(e, err, "checkpoint memory");
+  }
+  // forward: the engine's own advances
+  for (int w = 0; w < n_windows; ++w) {
+    if (want_grad && n > 0) {
+      hipLaunchKernelGGL(copy_ck, pgrid, dim3(blk), 0, st, n, P.cell, P.status, P.lx, P.ly,
+                         P.lz, P.t, P.rng, P.substeps, ck[w]);
+      if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "checkpoint");
+    }
+    Ws[w] = walk_params(e, t_windows[w]);
+    const int rc = rwpt_advance(e, t_windows[w], nullptr, nullptr);
+    if (rc != RWPT_OK) return rc;
+  }
+  // misfit
+  double *gx, *gy, *gz, *dobs, *dconc, *dmass = nullptr;
+  if ((err = B.alloc(&gx, n)) != hipSuccess || (err = B.alloc(&gy, n)) != hipSuccess ||
+      (err = B.alloc(&gz, n)) != hipSuccess || (err = B.alloc(&dobs, 3 * (size_t)n_obs)) != hipSuccess ||
+      (err = B.alloc(&dconc, n_obs)) != hipSuccess)
+    return adj_hip(e, err, "misfit memory");
+  if (mass) {
+    if ((err = B.alloc(&dmass, n)) != hipSuccess) return adj_hip(e, err, "misfit memory");
+    if ((err = hipMemcpyAsync(dmass, mass, n * 8, hipMemcpyHostToDevice, st)) != hipSuccess)
+      return adj_hip(e, err, "masses");
+  }
+  if ((err = hipMemcpyAsync(dobs, obs_xyz, 24 * (size_t)n_obs, hipMemcpyHostToDevice, st)) !=
+          hipSuccess ||
+      (err = hipMemsetAsync(dconc, 0, 8 * (size_t)n_obs, st)) != hipSuccess)
+    return adj_hip(e, err, "observations");
+  if (n > 0) {
+    if ((err = launch_global_xyz(e, gx, gy, gz)) != hipSuccess)
+      return adj_hip(e, err, "global coordinates");
+    hipLaunchKernelGGL(adj_conc, pgrid, dim3(blk), 0, st, n, (const int32_t *)P.status,
+                       (const double *)gx, (const double *)gy, (const double *)gz,
+                       (const double *)dmass, (int)n_obs, (const double *)dobs, bandwidth[0],
+                       bandwidth[1], bandwidth[2], dconc);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "concentrations");
+  }
+  std::vector<double> hc(n_obs), resid(n_obs);
+  if ((err = hipMemcpyAsync(hc.data(), dconc, 8 * (size_t)n_obs, hipMemcpyDeviceToHost, st)) !=
+          hipSuccess ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return adj_hip(e, err, "concentrations");
+  double J = 0.0;
+  for (int o = 0; o < n_obs; ++o) {
+    resid[o] = hc[o] - target[o];
+    J += resid[o] * resid[o];
+  }
+  if (misfit) *misfit = J;
+  if (conc) std::memcpy(conc, hc.data(), 8 * (size_t)n_obs);
+  if (!want_grad) {
+    if (ms)
+      *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                      t_begin).count();
+    return RWPT_OK;
+  }
+
+  // reverse: the particles active at the final time carry the adjoint
+  std::vector<int32_t> hs(n);
+  if (n > 0 &&
+      ((err = hipMemcpyAsync(hs.data(), P.status, 4 * n, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+       (err = hipStreamSynchronize(st)) != hipSuccess))
+    return adj_hip(e, err, "statuses");
+  std::vector<int32_t> hsel;
+  for (int64_t p = 0; p < n; ++p)
+    if (hs[p] == 1) hsel.push_back((int32_t)p);
+  const int64_t nsel = (int64_t)hsel.size();
+  int32_t *sel;
+  double *lam, *tan, *resid_d, *g_al, *g_at, *g_dm, *g_s;
+  int64_t *cnt, *off;
+  unsigned long long *bad;
+  const int64_t ncell = G.ncell;
+  if ((err = B.alloc(&sel, nsel)) != hipSuccess || (err = B.alloc(&lam, 4 * nsel)) != hipSuccess ||
+      (err = B.alloc(&tan, 4 * (size_t)ncell * kTan)) != hipSuccess ||
+      (err = B.alloc(&resid_d, n_obs)) != hipSuccess || (err = B.alloc(&g_al, ncell)) != hipSuccess ||
+      (err = B.alloc(&g_at, ncell)) != hipSuccess || (err = B.alloc(&g_dm, ncell)) != hipSuccess ||
+      (err = B.alloc(&g_s, 1)) != hipSuccess || (err = B.alloc(&cnt, nsel + 1)) != hipSuccess ||
+      (err = B.alloc(&off, nsel + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess)
+    return adj_hip(e, err, "adjoint memory");
+  if ((nsel > 0 && (err = hipMemcpyAsync(sel, hsel.data(), 4 * nsel, hipMemcpyHostToDevice,
+                                         st)) != hipSuccess) ||
+      (err = hipMemcpyAsync(resid_d, resid.data(), 8 * (size_t)n_obs, hipMemcpyHostToDevice,
+                            st)) != hipSuccess ||
+      (err = hipMemsetAsync(g_al, 0, 8 * ncell, st)) != hipSuccess ||
+      (err = hipMemsetAsync(g_at, 0, 8 * ncell, st)) != hipSuccess ||
+      (err = hipMemsetAsync(g_dm, 0, 8 * ncell, st)) != hipSuccess ||
+      (err = hipMemsetAsync(g_s, 0, 8, st)) != hipSuccess)
+    return adj_hip(e, err, "adjoint set-up");
+  if ((err = build_record_tangents(e, tan)) != hipSuccess)
+    return adj_hip(e, err, "record tangents");
+  const dim3 sgrid((unsigned)((std::max<int64_t>(nsel, 1) + kBlk - 1) / kBlk));
+  if (nsel > 0) {
+    hipLaunchKernelGGL(adj_seed, sgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nsel,
+                       (const int32_t *)P.cell, (const double *)gx, (const double *)gy,
+                       (const double *)gz, (const double *)dmass, (int)n_obs,
+                       (const double *)dobs, (const double *)resid_d, bandwidth[0],
+                       bandwidth[1], bandwidth[2], G.ncol, G.nrow, (const double *)G.delr,
+                       (const double *)G.delc, (const double *)G.top, (const double *)G.bot,
+                       lam);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "seed");
+  }
+  TapeEntry *tape = nullptr;
+  size_t tape_cap = 0;
+  char *scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  if ((err = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(nsel + 1),
+                                              st)) != hipSuccess)
+    return adj_hip(e, err, "scan");
+  if ((err = B.alloc(&scan_tmp, scan_bytes)) != hipSuccess)
+    return adj_hip(e, err, "scan memory");
+  const CkPtrs now{P.cell, P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps};
+  for (int w = n_windows - 1; w >= 0 && nsel > 0; --w) {
+    const CkPtrs &in = ck[w];
+    const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
+    hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nsel + 1 + blk - 1) / blk)), dim3(blk), 0,
+                       st, (const int32_t *)sel, nsel, (const int64_t *)in.substeps,
+                       (const int64_t *)want.substeps, cnt);
+    size_t bytes = scan_bytes;
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, cnt, off, (int)(nsel + 1),
+                                                st)) != hipSuccess)
+      return adj_hip(e, err, "tape offsets");
+    int64_t total = 0;
+    if ((err = hipMemcpyAsync(&total, off + nsel, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (err = hipStreamSynchronize(st)) != hipSuccess)
+      return adj_hip(e, err, "tape size");
+    if ((size_t)total > tape_cap) {
+      if (tape) (void)hipFree(tape);
+      B.ptrs.erase(std::remove(B.ptrs.begin(), B.ptrs.end(), (void *)tape), B.ptrs.end());
+      tape = nullptr;
+      tape_cap = std::max<size_t>((size_t)total, tape_cap + tape_cap / 4);
+      if ((err = B.alloc(&tape, tape_cap)) != hipSuccess) return adj_hip(e, err, "tape memory");
+    }
+    if ((err = hipMemsetAsync(bad, 0, 8, st)) != hipSuccess) return adj_hip(e, err, "replay");
+    const WalkParams &W = Ws[w];
+    if (ndim == 3)
+      hipLaunchKernelGGL(adj_replay<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const int32_t *)sel, nsel, in, want, W, (const int64_t *)off, tape,
+                         bad);
+    else
+      hipLaunchKernelGGL(adj_replay<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const int32_t *)sel, nsel, in, want, W, (const int64_t *)off, tape,
+                         bad);
+    unsigned long long nbad = 0;
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (err = hipStreamSynchronize(st)) != hipSuccess)
+      return adj_hip(e, err, "replay");
+    if (nbad != 0) {
+      char buf[160];
+      snprintf(buf, sizeof buf,
+               "adjoint: the replay of window %d differs from the forward walk for %llu "
+               "particles",
+               w, nbad);
+      e->err = buf;
+      return RWPT_E_STATE;
+    }
+    if (ndim == 3)
+      hipLaunchKernelGGL(adj_backward<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, ncell, nsel, W, (const int64_t *)off,
+                         (const TapeEntry *)tape, lam, grad_al ? g_al : nullptr,
+                         grad_at ? g_at : nullptr, grad_dm ? g_dm : nullptr, g_s);
+    else
+      hipLaunchKernelGGL(adj_backward<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, ncell, nsel, W, (const int64_t *)off,
+                         (const TapeEntry *)tape, lam, grad_al ? g_al : nullptr,
+                         grad_at ? g_at : nullptr, grad_dm ? g_dm : nullptr, g_s);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "reverse sweep");
+  }
+  if ((grad_al && (err = hipMemcpyAsync(grad_al, g_al, 8 * ncell, hipMemcpyDeviceToHost, st)) !=
+                      hipSuccess) ||
+      (grad_at && (err = hipMemcpyAsync(grad_at, g_at, 8 * ncell, hipMemcpyDeviceToHost, st)) !=
+                      hipSuccess) ||
+      (grad_dm && (err = hipMemcpyAsync(grad_dm, g_dm, 8 * ncell, hipMemcpyDeviceToHost, st)) !=
+                      hipSuccess) ||
+      (grad_porosity_scale &&
+       (err = hipMemcpyAsync(grad_porosity_scale, g_s, 8, hipMemcpyDeviceToHost, st)) !=
+           hipSuccess) ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return adj_hip(e, err, "gradients");
+  if (ms)
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin)
+              .count();
+  return RWPT_OK;
+}

@@ -459,7 +459,109 @@ __global__ void all_pow2(int64_t ncell, const double *__restrict__ cells,
   if (__ballot(!ok) != 0 && __lane_id() == 0) atomicAnd(flag, 0ull);
 }
 
+// a[i] * f (porosity) or a[i] + sign * h[i] over the blocks of the
+// dispersion array that hold the perturbed parameter
+__global__ void scaled_copy(int64_t n, const double *__restrict__ a, double f,
+                            double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] * f;
+}
+
+__global__ void shifted_copy(int64_t n, const double *__restrict__ a, int64_t lo,
+                             int64_t hi, double sign, const double *__restrict__ h,
+                             double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (i >= lo && i < hi) ? a[i] + sign * h[i - lo] : a[i];
+}
+
+// step of the central difference: relative to the value, absolute 1 at 0
+__global__ void fd_steps(int64_t n, const double *__restrict__ a, double rel,
+                         double *__restrict__ h) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) h[i] = (a && a[i] != 0.0) ? rel * fabs(a[i]) : rel;
+}
+
+// (plus - minus) / (2 h) of the double fields 0..F_CONN-1 of every record;
+// h per cell from the step array (one value when nh == 1)
+__global__ void fd_tangent(int64_t ncell, const double *__restrict__ plus,
+                           const double *__restrict__ minus,
+                           const double *__restrict__ h, int64_t nh,
+                           double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncell * F_CONN) return;
+  const int64_t c = i / F_CONN, f = i % F_CONN;
+  const double hc = h[nh == 1 ? 0 : c];
+  out[i] = (plus[c * CELL_DOUBLES + f] - minus[c * CELL_DOUBLES + f]) / (2.0 * hc);
+}
+
 }  // namespace
+
+hipError_t build_record_tangents(rwpt_engine *e, double *out) {
+  DeviceProblem &P = e->prob;
+  const GridDims g{P.ncol, P.nrow, P.nlay, P.nlay_member};
+  const int block = 256;
+  const unsigned grid = (unsigned)((P.ncell + block - 1) / block);
+  const TableParams tp{P.disp_model, e->opt.backward_tracking, e->opt.time_step_kind,
+                       e->opt.time_step_params[0], e->opt.time_step_params[1]};
+  // the records are rational in the porosity scale and linear in aLH, aTH
+  // and Dm (for FIXED time steps): a relative step of 1e-5 leaves the
+  // truncation error near 1e-10 and the rounding error near 1e-11 relative
+  const double rel = 1.0e-5;
+  const size_t recBytes = (size_t)P.ncell * CELL_DOUBLES * sizeof(double);
+  const int64_t nd = 5 * P.ndisp;
+  double *plus = nullptr, *minus = nullptr, *por = nullptr, *disp = nullptr, *h = nullptr;
+  auto release = [&]() {
+    for (double *q : {plus, minus, por, disp, h})
+      if (q) (void)hipFree(q);
+  };
+  hipError_t err = hipMalloc(&plus, recBytes);
+  if (err == hipSuccess) err = hipMalloc(&minus, recBytes);
+  if (err == hipSuccess) err = hipMalloc(&por, (size_t)P.ncell * sizeof(double));
+  if (err == hipSuccess) err = hipMalloc(&disp, (size_t)nd * sizeof(double));
+  if (err == hipSuccess)
+    err = hipMalloc(&h, (size_t)std::max<int64_t>(P.ncell, P.ndisp) * sizeof(double));
+  if (err != hipSuccess) return release(), err;
+  const unsigned dgrid = (unsigned)((nd + block - 1) / block);
+  const unsigned tgrid = (unsigned)((P.ncell * F_CONN + block - 1) / block);
+  for (int k = 0; k < 4; ++k) {
+    // dispersion block of parameter k: aL [0, n), aT [n, 2n), Dm [4n, 5n)
+    const int64_t lo = k == 0 ? 0 : (k == 1 ? P.ndisp : 4 * P.ndisp), hi = lo + P.ndisp;
+    if (k < 3)
+      hipLaunchKernelGGL(fd_steps, dim3((unsigned)((P.ndisp + block - 1) / block)),
+                         dim3(block), 0, e->stream, P.ndisp, P.disp + lo, rel, h);
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      double *dst = sgn == 0 ? plus : minus;
+      const double sign = sgn == 0 ? 1.0 : -1.0;
+      err = hipMemcpyAsync(dst, P.cells, recBytes, hipMemcpyDeviceToDevice, e->stream);
+      if (err != hipSuccess) return release(), err;
+      const double *pin = P.porosity, *din = P.disp;
+      if (k < 3) {
+        hipLaunchKernelGGL(shifted_copy, dim3(dgrid), dim3(block), 0, e->stream, nd, P.disp,
+                           lo, hi, sign, (const double *)h, disp);
+        din = disp;
+      } else {
+        hipLaunchKernelGGL(scaled_copy, dim3(grid), dim3(block), 0, e->stream, P.ncell,
+                           P.porosity, 1.0 + sign * rel, por);
+        pin = por;
+      }
+      hipLaunchKernelGGL(corner_tables, dim3(grid), dim3(block), 0, e->stream, g, P.ncell,
+                         P.delr, P.delc, pin, P.retardation, din, P.ndisp, P.q6, P.dzrw,
+                         tp, P.uniform_member, dst, (double *)nullptr);
+      if ((err = hipGetLastError()) != hipSuccess) return release(), err;
+    }
+    if (k == 3)
+      hipLaunchKernelGGL(fd_steps, dim3(1), dim3(1), 0, e->stream, (int64_t)1,
+                         (const double *)nullptr, rel, h);
+    hipLaunchKernelGGL(fd_tangent, dim3(tgrid), dim3(block), 0, e->stream, P.ncell,
+                       (const double *)plus, (const double *)minus, (const double *)h,
+                       k == 3 ? (int64_t)1 : P.ndisp, out + (size_t)k * P.ncell * F_CONN);
+    if ((err = hipGetLastError()) != hipSuccess) return release(), err;
+  }
+  err = hipStreamSynchronize(e->stream);
+  release();
+  return err;
+}
 
 hipError_t build_cell_tables32(rwpt_engine *e) {
   DeviceProblem &P = e->prob;

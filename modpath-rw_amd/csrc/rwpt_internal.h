@@ -182,6 +182,12 @@ namespace rwpt {
 hipError_t build_cell_tables(rwpt_engine *e, double *debug_out);
 hipError_t launch_seed_streams(rwpt_engine *e, int64_t n, int64_t offset);
 hipError_t launch_advance(rwpt_engine *e, double t_stop);
+// the walk's parameters for an advance from the engine time to t_stop
+WalkParams walk_params(const rwpt_engine *e, double t_stop);
+// d(record)/d(parameter) of every cell, fields 0..F_CONN-1, for the
+// adjoint: 0 aLH, 1 aTH, 2 Dm (each cell's own value), 3 a common scale s
+// of all porosities (at s = 1); out holds 4 * ncell * F_CONN doubles
+hipError_t build_record_tangents(rwpt_engine *e, double *out);
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
                              double *gz);
 hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,

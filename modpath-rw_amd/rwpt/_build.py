@@ -12,7 +12,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(os.path.dirname(PKG), "librwpt.so")
 
 SOURCES = ["rwpt_api.hip", "rwpt_cells.hip", "rwpt_walk.hip", "rwpt_kde.hip",
-           "rwpt_gpkde.hip", "rwpt_export.hip"]
+           "rwpt_gpkde.hip", "rwpt_export.hip", "rwpt_adjoint.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

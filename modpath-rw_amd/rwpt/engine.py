@@ -99,6 +99,8 @@ EXPORTS = {
                         C.c_double, C.c_int32, C.POINTER(RwptKdeOptions), _dp, _dp,
                         C.c_double], C.c_int),
     "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
+    "rwpt_adjoint": ([C.c_void_p, C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp,
+                      _dp, _dp, _dp, _dp, _dp], C.c_int),
     "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
     "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
 }
@@ -213,6 +215,7 @@ class Engine:
             raise RwptError(f"rwpt_create_ensemble failed ({rc})")
         self.h = h
         self.n = 0
+        self.ncell_total = ncm * self.members
         self.set_flow(P)
         self.set_dispersion(P)
 
@@ -290,6 +293,35 @@ class Engine:
                                         C.byref(ms), C.byref(steps)), "rwpt_autotune")
         return {"budget": v[0].value, "block": v[1].value, "sort": v[2].value,
                 "min_waves": v[3].value, "ms": ms.value, "trial_substeps": steps.value}
+
+    def adjoint(self, t_windows, obs_xyz, target, bandwidth, mass=None, gradient=True):
+        """rwpt_adjoint: advances through the window times, returns the misfit
+        sum_o (c_o - target_o)^2 of the Gaussian-smoothed concentrations at
+        the observation points and, with gradient, its pathwise derivative
+        with respect to every cell's aLH, aTH, Dm (arrays of ncell) and to a
+        common porosity scale."""
+        tw = np.ascontiguousarray(t_windows, dtype=np.float64)
+        obs = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
+        tgt = np.ascontiguousarray(target, dtype=np.float64)
+        if tgt.size != obs.shape[0]:
+            raise ValueError("one target per observation point")
+        bw = np.ascontiguousarray(bandwidth, dtype=np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        conc = np.zeros(obs.shape[0])
+        J, ms, gs = C.c_double(0), C.c_double(0), C.c_double(0)
+        nc = self.ncell_total
+        g = [np.zeros(nc) for _ in range(3)] if gradient else [None, None, None]
+        rc = lib().rwpt_adjoint(self.h, int(tw.size), _p(tw, _dp), int(obs.shape[0]),
+                                _p(obs, _dp), _p(tgt, _dp), _p(bw, _dp), _p(m, _dp),
+                                C.byref(J), _p(conc, _dp), _p(g[0], _dp), _p(g[1], _dp),
+                                _p(g[2], _dp), C.byref(gs) if gradient else None,
+                                C.byref(ms))
+        self._check(rc, "rwpt_adjoint")
+        out = {"misfit": J.value, "conc": conc, "ms": ms.value}
+        if gradient:
+            out.update({"grad_al": g[0], "grad_at": g[1], "grad_dm": g[2],
+                        "grad_porosity_scale": gs.value})
+        return out
 
     def write_timeseries(self, path, time_point, time_step, time, sequence, ids, group,
                          select=None):

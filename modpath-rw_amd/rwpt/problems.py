@@ -341,3 +341,43 @@ def release_ensemble(E, counts, cols, rows, lays, seed=1):
         c, x, y, z = release_in_block(E, int(n), cols, rows, lays, seed=seed + m)
         parts.append((E.global_cell(m, c).astype(np.int32), x, y, z))
     return tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
+
+
+def column_zones(P: Problem, nzones: int) -> np.ndarray:
+    """0-based zone of every cell: nzones bands of equal column count along x."""
+    j = np.arange(P.ncell) % P.ncol
+    return (j * nzones // P.ncol).astype(np.int32)
+
+
+def zoned_dispersion(P: Problem, zones: np.ndarray, aL_z, aT_z, dm) -> Problem:
+    """P with per-cell aL, aT from zone values and a uniform Dm (per cell)."""
+    Q = dataclasses.replace(P)
+    Q.aL = np.asarray(aL_z, dtype=np.float64)[zones]
+    Q.aT = np.asarray(aT_z, dtype=np.float64)[zones]
+    Q.dmeff = np.full(P.ncell, float(dm))
+    return Q
+
+
+def config4(nzones=8, seed=20240601, **kw):
+    """The calibration workload: the config-1 field with 8 x-bands of 25
+    columns, each with its own aL and aT, a common Dm; returns the problem,
+    the zone of every cell and the 18 parameters (aL_z, aT_z, porosity
+    scale 1, Dm)."""
+    P = config1(seed=seed, **kw)
+    zones = column_zones(P, nzones)
+    aL = np.linspace(0.3, 0.7, nzones)
+    aT = 0.1 * aL
+    return zoned_dispersion(P, zones, aL, aT, 1e-4), zones, (aL, aT, 1.0, 1e-4)
+
+
+def observation_points(P: Problem, n_x=8, n_y=4, cols=(20, 160), rows=(85, 115)):
+    """Global (x, y, z) of n_x * n_y points at cell centres of the middle
+    layer along the plume path (x along columns, y along rows)."""
+    xc = np.cumsum(P.delr) - P.delr / 2
+    yc = np.cumsum(P.delc[::-1])[::-1] - P.delc / 2  # row 1 at the top (largest y)
+    k = P.nlay // 2
+    c = k * P.nrow * P.ncol
+    zmid = 0.5 * (P.top[c] + P.bot[c])
+    js = np.linspace(cols[0], cols[1], n_x).astype(int) - 1
+    is_ = np.linspace(rows[0], rows[1], n_y).astype(int) - 1
+    return np.array([(xc[j], yc[i], zmid) for i in is_ for j in js])

@@ -1,0 +1,173 @@
+"""Pathwise adjoint of the observation misfit (rwpt_adjoint).
+
+The forward part must advance exactly as rwpt_advance does (bit for bit),
+and the gradient must be the derivative of the misfit with common random
+numbers: for a step small enough that no particle changes branch, the
+central finite difference of the misfit over rebuilt engines equals the
+pathwise derivative up to O(h^2) and rounding; particles near a grazing
+exit have large second derivatives, so the two are required to agree to
+1e-4 relative at one of the steps tried.  A branch change (a jump-limiter cut or a cell exit taken on one
+side only) moves a particle by a fraction of a cell instead of O(h), so the
+check first compares the two runs' final cells, statuses, sub-step counts
+and positions.  Parameters: per-zone aL and aT (zones = column bands), a
+common Dm and a common porosity scale."""
+import numpy as np
+import pytest
+
+from rwpt import Engine, RwptError, problems
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS = (1.0, 2.0, 3.0)
+NZ = 3
+
+
+def base(ndim=3):
+    P = problems.heterogeneous_flow(24, 20, 6, corr_len=4.0, seed=7)
+    P.ts_params = (0.5, 0.0)
+    P.time_step_kind = 4
+    if ndim == 2:
+        P.dim_mask = (1, 1, 0)
+    zones = problems.column_zones(P, NZ)
+    aL = np.array([0.3, 0.5, 0.4])
+    aT = np.array([0.04, 0.05, 0.03])
+    return P, zones, aL, aT, 1e-3
+
+
+def release(P):
+    return problems.release_in_block(P, 400, (2, 6), (6, 15), (1, 6), seed=5)
+
+
+def observations(P):
+    return problems.observation_points(P, 6, 3, (5, 18), (7, 14))
+
+
+def run(P, zones, aL, aT, dm, por_scale=1.0, gradient=False, target=None):
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    Q.porosity = P.porosity * por_scale
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        obs = observations(P)
+        tgt = np.zeros(len(obs)) if target is None else target
+        out = E.adjoint(WINDOWS, obs, tgt, (1.5, 1.5, 1.0), gradient=gradient)
+        g = E.particles()
+        out["state"] = (g["cell"].copy(), g["status"].copy(), g["substeps"].copy())
+        out["pos"] = np.stack([g["lx"], g["ly"], g["lz"], g["t"]])
+    finally:
+        E.close()
+    return out
+
+
+def same_branches(a, b, rel):
+    same = all(np.array_equal(x, y) for x, y in zip(a["state"], b["state"]))
+    return same and float(np.max(np.abs(a["pos"] - b["pos"]))) < 1e4 * rel
+
+
+def central_difference(P, zones, aL, aT, dm, target, name, z, rel):
+    """(J(p + h) - J(p - h)) / 2h for one parameter and h = rel * p, or None
+    when a particle changed branch between the two runs."""
+    vals = []
+    for sgn in (1.0, -1.0):
+        aL2, aT2, dm2, s2 = aL.copy(), aT.copy(), dm, 1.0
+        if name == "aL":
+            h = rel * aL[z]
+            aL2[z] += sgn * h
+        elif name == "aT":
+            h = rel * aT[z]
+            aT2[z] += sgn * h
+        elif name == "dm":
+            h = rel * dm
+            dm2 = dm + sgn * h
+        else:
+            h = rel
+            s2 = 1.0 + sgn * h
+        vals.append(run(P, zones, aL2, aT2, dm2, s2, target=target))
+    if not same_branches(vals[0], vals[1], rel):
+        return None
+    return (vals[0]["misfit"] - vals[1]["misfit"]) / (2 * h), h
+
+
+@pytest.mark.parametrize("ndim", [3, 2])
+def test_adjoint_matches_central_differences(ndim):
+    P, zones, aL, aT, dm = base(ndim)
+    print(f"adjoint ndim {ndim}: 400 particles, windows {WINDOWS}", flush=True)
+    ref = run(P, zones, aL, aT, dm)
+    target = 0.7 * ref["conc"][::-1]
+    A = run(P, zones, aL, aT, dm, gradient=True, target=target)
+    assert A["misfit"] > 0
+    for k in ("grad_al", "grad_at", "grad_dm"):
+        assert np.isfinite(A[k]).all(), (k, int((~np.isfinite(A[k])).sum()))
+    assert np.isfinite(A["grad_porosity_scale"])
+    adj = {}
+    for z in range(NZ):
+        adj[("aL", z)] = A["grad_al"][zones == z].sum()
+        adj[("aT", z)] = A["grad_at"][zones == z].sum()
+    adj[("dm", 0)] = A["grad_dm"].sum()
+    adj[("por", 0)] = A["grad_porosity_scale"]
+    checked = 0
+    for (name, z), g in adj.items():
+        # steps without a branch change; a particle near a grazing exit has
+        # a large second derivative, so one of the steps may still be outside
+        # the linear range: the adjoint must agree with at least one of them,
+        # allowing the misfit's rounding (order of the atomic sums, ~1e-15
+        # relative) divided by 2h
+        seen = []
+        for rel in (1e-6, 1e-7, 1e-8, 1e-9):
+            r = central_difference(P, zones, aL, aT, dm, target, name, z, rel)
+            if r is None:
+                print(f"  {name}{z} rel {rel:g}: a particle changed branch", flush=True)
+                continue
+            fd, h = r
+            print(f"  {name}{z} rel {rel:g}: adjoint {g:.9e} fd {fd:.9e}", flush=True)
+            seen.append(fd)
+            if abs(g - fd) <= 1e-4 * abs(fd) + 1e-14 * A["misfit"] / h:
+                checked += 1
+                break
+        else:
+            assert not seen, (name, z, g, seen)
+    assert checked >= len(adj) - 1
+
+
+def test_forward_part_is_the_advance():
+    """The misfit call advances bit-identically to rwpt_advance, and the
+    gradient leaves the same final state."""
+    P, zones, aL, aT, dm = base()
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    E1, E2 = Engine(Q), Engine(Q)
+    try:
+        for E in (E1, E2):
+            E.load_particles(*release(P))
+        obs = observations(P)
+        out = E1.adjoint(WINDOWS, obs, np.zeros(len(obs)), (1.5, 1.5, 1.0), gradient=True)
+        for t in WINDOWS:
+            E2.advance(t)
+        a, b = E1.particles(), E2.particles()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+        gx, gy, gz = E2.global_xyz()
+        on = b["status"] == 1
+        c = np.zeros(len(obs))
+        for o, (x, y, z) in enumerate(obs):
+            k = (np.exp(-0.5 * ((gx[on] - x) / 1.5) ** 2) / (1.5 * np.sqrt(2 * np.pi)) *
+                 np.exp(-0.5 * ((gy[on] - y) / 1.5) ** 2) / (1.5 * np.sqrt(2 * np.pi)) *
+                 np.exp(-0.5 * ((gz[on] - z) / 1.0) ** 2) / (1.0 * np.sqrt(2 * np.pi)))
+            c[o] = k.sum()
+        np.testing.assert_allclose(out["conc"], c, rtol=1e-12, atol=1e-300)
+        assert out["misfit"] == pytest.approx((c ** 2).sum(), rel=1e-12)
+    finally:
+        E1.close()
+        E2.close()
+
+
+def test_adjoint_rejects_unsupported():
+    P, zones, aL, aT, dm = base()
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    E = Engine(Q, precision=1)
+    try:
+        E.load_particles(*release(P))
+        obs = observations(P)
+        with pytest.raises(RwptError):
+            E.adjoint(WINDOWS, obs, np.zeros(len(obs)), (1.5, 1.5, 1.0), gradient=True)
+    finally:
+        E.close()
