@@ -217,13 +217,17 @@ template <int N, int P0> struct RecD {
 // order and every branch test are those of iterate() in rwpt_walk.hip
 // (TrackSubCell.F90:956-1368); the replay checks the equality bit for bit.
 // ---------------------------------------------------------------------------
+// The visit start is a constant of every iteration (its adjoint is carried
+// separately), so it stays a plain double.
 template <typename T> struct VisitG {
-  T x, y, z, t;      // location and time
-  T ix, iy, iz, t0;  // visit start
+  T x, y, z, t;            // location and time
+  double ix, iy, iz, t0;   // visit start
   int restarts, exitFace;
 };
+// The advective displacement v * dt is formed where it is used: every use
+// follows an update of v or dt exactly as the plain walk's advect() does.
 template <typename T> struct TermsG {
-  T vx, vy, vz, dAx, dAy, dAz, dBx, dBy, dBz, ddx, ddy, ddz;
+  T vx, vy, vz, dBx, dBy, dBz, ddx, ddy, ddz;
 };
 
 template <typename T, typename Rec>
@@ -245,12 +249,6 @@ __device__ __forceinline__ T deriv4g(const Rec &rec, int b, const T &u, const T 
   return (1.0 - w) * v0 + w * v1;
 }
 
-template <typename T>
-__device__ __forceinline__ void advect_g(const T &dt, TermsG<T> &s) {
-  s.dAx = s.vx * dt;
-  s.dAy = s.vy * dt;
-  s.dAz = s.vz * dt;
-}
 
 template <typename T, typename Rec, int NDIM>
 __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
@@ -318,7 +316,6 @@ __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
     s.dBy = d1 == 1 ? u1 : (d2 == 1 ? u2 : zero);
     s.dBz = d1 == 2 ? u1 : (d2 == 2 ? u2 : zero);
   }
-  advect_g(dt, s);
 }
 
 // One axis of pr_DetectExitFaceAndUpdateTimeStepQuadratic.
@@ -435,18 +432,17 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
   }
   displacements_g<T, Rec, NDIM>(rec, W, x, y, z, dt, rng, s);
   T sq = dsqrt(dt);
-  T dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-  T dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-  T dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+  T dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
+  T dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
+  T dzr = s.vz * dt + s.ddz * dt + s.dBz * sq;
   while ((fabs(val(dxr) / val(DX)) > maxRelJump) || (fabs(val(dyr) / val(DY)) > maxRelJump) ||
          (fabs(val(dzr) / val(DZ)) > maxRelJump)) {
     t = t - dt;
     dt = dtRed * dt;
-    advect_g(dt, s);
     sq = dsqrt(dt);
-    dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-    dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-    dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+    dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
+    dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
+    dzr = s.vz * dt + s.ddz * dt + s.dBz * sq;
     t = t + dt;
     atMax = false;
   }
@@ -458,19 +454,18 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
     loops = loops + 1;
     exit_time_g<T>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
     if (loops > 5) {
-      nx = V.ix; ny = V.iy; nz = V.iz;
-      t = V.t0;
+      nx = Cst<T>::of(V.ix); ny = Cst<T>::of(V.iy); nz = Cst<T>::of(V.iz);
+      t = Cst<T>::of(V.t0);
       dt = dtcell;
       V.restarts = V.restarts + 1;
       restart = 1;
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
-    advect_g(dt, s);
     atMax = false;
     if (exitFace == 0) {
-      nx = V.ix; ny = V.iy; nz = V.iz;
-      t = V.t0;
+      nx = Cst<T>::of(V.ix); ny = Cst<T>::of(V.iy); nz = Cst<T>::of(V.iz);
+      t = Cst<T>::of(V.t0);
       dt = dtcell;
       V.restarts = V.restarts + 1;
       restart = 1;
@@ -480,20 +475,20 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
     sq = dsqrt(dt);
     if (exitFace <= 2) {
       nx = Cst<T>::of(exitFace == 1 ? 0.0 : 1.0);
-      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
       ny = y + dyr / DY;
-      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      dzr = s.vz * dt + s.ddz * dt + s.dBz * sq;
       nz = z + dzr / DZ;
     } else if (exitFace <= 4) {
-      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
       nx = x + dxr / DX;
       ny = Cst<T>::of(exitFace == 3 ? 0.0 : 1.0);
-      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+      dzr = s.vz * dt + s.ddz * dt + s.dBz * sq;
       nz = z + dzr / DZ;
     } else {
-      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
+      dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
       nx = x + dxr / DX;
-      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
+      dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
       ny = y + dyr / DY;
       nz = Cst<T>::of(exitFace == 5 ? 0.0 : 1.0);
     }
@@ -505,7 +500,7 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
       if (rebounds > 10) return IT_UNDEFINED;
       const T xi = nx, yi = ny, zi = nz;
       if (val(dt) == 0.0) {
-        nx = V.ix; ny = V.iy; nz = V.iz;
+        nx = Cst<T>::of(V.ix); ny = Cst<T>::of(V.iy); nz = Cst<T>::of(V.iz);
         t = t - dt;
         dt = dtcell;
         exitFace = 0;
@@ -520,11 +515,10 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
         dt = val(rem) > 0.0 ? rem : Cst<T>::of(fmax(val(rem), 0.0));
         t = Cst<T>::of(W.t_stop);
         atMax = true;
-        advect_g(dt, s);
-        sq = dsqrt(dt);
-        dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-        dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-        dzr = s.dAz + s.ddz * dt + s.dBz * sq;
+            sq = dsqrt(dt);
+        dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
+        dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
+        dzr = s.vz * dt + s.ddz * dt + s.dBz * sq;
       }
       if (exitFace <= 2) {
         nx = nx - dxr / DX;
@@ -770,8 +764,7 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
       V.y = dconst<4>(E.y); V.y.d[1] = 1.0;
       V.z = dconst<4>(E.z); V.z.d[2] = 1.0;
       V.t = dconst<4>(E.t); V.t.d[3] = 1.0;
-      V.ix = dconst<4>(E.ix); V.iy = dconst<4>(E.iy);
-      V.iz = dconst<4>(E.iz); V.t0 = dconst<4>(E.t0);
+      V.ix = E.ix; V.iy = E.iy; V.iz = E.iz; V.t0 = E.t0;
       V.restarts = rs;
       V.exitFace = ef;
       Stream rng{E.ra, E.rb};
@@ -791,8 +784,7 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
         VisitG<D> U;
         U.x = dconst<4>(E.x); U.y = dconst<4>(E.y); U.z = dconst<4>(E.z);
         U.t = dconst<4>(E.t);
-        U.ix = dconst<4>(E.ix); U.iy = dconst<4>(E.iy);
-        U.iz = dconst<4>(E.iz); U.t0 = dconst<4>(E.t0);
+        U.ix = E.ix; U.iy = E.iy; U.iz = E.iz; U.t0 = E.t0;
         U.restarts = rs;
         U.exitFace = ef;
         Stream r2{E.ra, E.rb};

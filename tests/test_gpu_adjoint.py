@@ -171,3 +171,37 @@ def test_adjoint_rejects_unsupported():
             E.adjoint(WINDOWS, obs, np.zeros(len(obs)), (1.5, 1.5, 1.0), gradient=True)
     finally:
         E.close()
+
+
+def test_autograd_wrapper():
+    """ZonedMisfit: the forward is the engine's misfit, the backward its
+    zone-summed adjoint gradient (porosity scale and Dm included)."""
+    import torch
+    from rwpt.adjoint import CalibrationProblem, ZonedMisfit
+    P, zones, aL, aT, dm = base()
+    obs = observations(P)
+    prob = CalibrationProblem(P, zones, release(P), WINDOWS, obs, np.zeros(len(obs)),
+                              (1.5, 1.5, 1.0))
+    theta0 = np.concatenate([aL, aT, [1.0, dm]])
+    ref, _ = prob.evaluate(theta0, gradient=False)
+    prob.target = 0.7 * np.sqrt(ref / len(obs)) * np.ones(len(obs))
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    J = ZonedMisfit.apply(theta, prob)
+    J.backward()
+    Jn, g = prob.evaluate(theta0, gradient=True)
+    assert float(J) == Jn
+    np.testing.assert_array_equal(theta.grad.numpy(), g)
+    assert np.all(np.isfinite(g)) and np.any(g != 0)
+    # a change of the porosity scale: d/ds at s = 1.1 is the engine's value / 1.1
+    th2 = theta0.copy()
+    th2[2 * NZ] = 1.1
+    _, g2 = prob.evaluate(th2, gradient=True)
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    Q.porosity = P.porosity * 1.1
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        out = E.adjoint(WINDOWS, obs, prob.target, (1.5, 1.5, 1.0))
+    finally:
+        E.close()
+    assert g2[2 * NZ] == pytest.approx(out["grad_porosity_scale"] / 1.1, rel=1e-12)
