@@ -16,6 +16,7 @@ module RwptGpuModule
   public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
   public :: rwpt_create_ensemble, rwpt_autotune, rwpt_write_timeseries_header
   public :: rwpt_write_timeseries, rwpt_write_endpoints, rwpt_adjoint
+  public :: rwpt_write_observation
 
   type, bind(C) :: rwpt_grid
     integer(c_int32_t) :: ncol, nrow, nlay
@@ -227,6 +228,18 @@ module RwptGpuModule
       integer(c_int32_t), intent(in) :: initial_cell(*), initial_face(*)
       real(c_double), intent(in) :: initial_lx(*), initial_ly(*), initial_lz(*)
       real(c_double), intent(in) :: initial_time(*), mass(*)
+    end function
+    integer(c_int) function rwpt_write_observation(e, path, append, style, format, n_cells, &
+        cells, time_point, time_step, id, group, group_solute, mass, flow_rate, select) &
+        bind(C, name='rwpt_write_observation')
+      import :: c_int, c_ptr, c_char, c_double, c_int32_t
+      type(c_ptr), value :: e
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: append, style, format, n_cells, time_point, time_step
+      integer(c_int32_t), intent(in) :: cells(*), id(*), group(*), group_solute(*)
+      type(c_ptr), value :: mass                    ! c_loc of the masses or c_null_ptr
+      real(c_double), value :: flow_rate
+      type(c_ptr), value :: select                  ! c_loc of integer(c_int8_t) or c_null_ptr
     end function
     integer(c_int) function rwpt_adjoint(e, n_windows, t_windows, n_obs, obs_xyz, target, &
         bandwidth, mass, misfit, conc, grad_al, grad_at, grad_dm, grad_porosity_scale, ms) &

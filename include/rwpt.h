@@ -310,6 +310,28 @@ int rwpt_write_endpoints(rwpt_engine *e, const char *path, int32_t format,
                          const double *initial_time, const int32_t *initial_face,
                          const double *mass);
 
+/* MODPATH-RW's observation records (WriteResidentObsRecord[Binary],
+ * WriteSinkObsRecord[Binary], Observation.f90:114-221, as MPathRW.F90
+ * :1344-1392 and :1449-1530 writes them) for the engine's particles in the
+ * observation's cells (n_cells engine cell numbers): style 1 resident, the
+ * particles with status 1, at a time point after rwpt_advance; style 2
+ * sink, the particles stopped at a weak sink (status 3; weak source, 4, in
+ * backward tracking) with flow_rate the observation's summed sink flow.
+ * format 0 text, (2I8,ES18.9E3,I10,ES18.9E3,2I5,2I10,5ES18.9E3) resident
+ * and (...,ES18.9E3) sink: time point, time step, time, ID, mass, group,
+ * solute, cell, layer, then the cell's retardation, water volume (porosity
+ * DX DY and the saturated thickness; 0 when dry) and the model x, y, global
+ * z, or the flow rate; format 1 the same fields as a stream of int32 and
+ * float64.  id, group (1-based) and mass (NULL: 1) per particle,
+ * group_solute per group; select (may be NULL) limits the particles, e.g.
+ * to those stopped in the last advance.  append 0 replaces the file. */
+int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t append,
+                           int32_t style, int32_t format, int32_t n_cells,
+                           const int32_t *cells, int32_t time_point, int32_t time_step,
+                           const int32_t *id, const int32_t *group,
+                           const int32_t *group_solute, const double *mass,
+                           double flow_rate, const uint8_t *select);
+
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,
  * corner discharge qx qy qz |q| (8 x 4), corner porosity (8), corner phi*D

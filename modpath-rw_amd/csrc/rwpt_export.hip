@@ -416,3 +416,99 @@ extern "C" int rwpt_write_endpoints(
   }
   return write_all(path, false, b);
 }
+
+// Observation records (Observation.f90:114-221) of the particles in one
+// observation's cells, as MPathRW.F90 writes them: resident records at a
+// time point for the active particles (:1449-1530), sink records for the
+// particles stopped at a weak sink (weak source in backward tracking,
+// :1344-1392).  Text: (2I8,es18.9e3,i10,es18.9e3,2i5,2i10,5es18.9e3) and
+// (...,es18.9e3); binary: the same fields as a stream of int32 and float64.
+extern "C" int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t append,
+                                      int32_t style, int32_t format, int32_t n_cells,
+                                      const int32_t *cells, int32_t time_point,
+                                      int32_t time_step, const int32_t *id,
+                                      const int32_t *group, const int32_t *group_solute,
+                                      const double *mass, double flow_rate,
+                                      const uint8_t *select) {
+  if (!e || !path || (style != 1 && style != 2) || (format != 0 && format != 1) ||
+      n_cells < 0 || (n_cells > 0 && !cells) || !id || !group || !group_solute)
+    return RWPT_E_ARG;
+  rwpt::DeviceProblem &P = e->prob;
+  for (int32_t k = 0; k < n_cells; ++k)
+    if (cells[k] < 1 || cells[k] > P.ncell) return RWPT_E_ARG;
+  HostGrid G;
+  HostParticles H;
+  if (!G.load(e) || !H.load(e)) return RWPT_E_DEVICE;
+  // the observation's cells and, for resident records, their retardation
+  // and water volume (GetWaterVolume, ModpathCellData.f90:3133-3157:
+  // porosity DX DY times the saturated thickness of GetDZRW, 0 when dry)
+  std::vector<int32_t> obs_index(P.ncell, -1);
+  std::vector<double> rfac(n_cells), wvol(n_cells);
+  for (int32_t k = 0; k < n_cells; ++k) {
+    double rec[rwpt::CELL_DOUBLES];
+    if (hipMemcpy(rec, P.cells + (int64_t)(cells[k] - 1) * rwpt::CELL_DOUBLES, sizeof rec,
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      return RWPT_E_DEVICE;
+    const int32_t *ints = reinterpret_cast<const int32_t *>(rec);
+    const bool dry = (ints[2 * rwpt::F_FLAGS + 1] & rwpt::CF_DRY) != 0;
+    rfac[k] = rec[rwpt::F_R];
+    wvol[k] = dry ? 0.0 : rec[rwpt::F_PHI] * rec[rwpt::F_DX] * rec[rwpt::F_DY] * rec[rwpt::F_DZ];
+    obs_index[cells[k] - 1] = k;
+  }
+  const int stop_status = e->opt.backward_tracking ? 4 : 3;
+  std::string b;
+  auto i4 = [&](int32_t v) { b.append(reinterpret_cast<const char *>(&v), 4); };
+  auto f8 = [&](double v) { b.append(reinterpret_cast<const char *>(&v), 8); };
+  for (size_t p = 0; p < H.cell.size(); ++p) {
+    const int32_t c = H.cell[p];
+    if (c < 1 || c > P.ncell || obs_index[c - 1] < 0) continue;
+    if (H.status[p] != (style == 1 ? 1 : stop_status)) continue;
+    if (select && !select[p]) continue;
+    const int k = obs_index[c - 1];
+    const double m = mass ? mass[p] : 1.0;
+    const int32_t sol = group_solute[group[p] - 1];
+    double g[3] = {0.0, 0.0, 0.0};
+    if (style == 1) G.xyz(c, H.lx[p], H.ly[p], H.lz[p], g);
+    if (format == 0) {
+      put_i(b, time_point, 8);
+      put_i(b, time_step, 8);
+      put_es18_9e3(b, H.t[p]);
+      put_i(b, id[p], 10);
+      put_es18_9e3(b, m);
+      put_i(b, group[p], 5);
+      put_i(b, sol, 5);
+      put_i(b, G.local_cell(c), 10);
+      put_i(b, G.layer(c), 10);
+      if (style == 1) {
+        put_es18_9e3(b, rfac[k]);
+        put_es18_9e3(b, wvol[k]);
+        put_es18_9e3(b, g[0]);
+        put_es18_9e3(b, g[1]);
+        put_es18_9e3(b, g[2]);
+      } else {
+        put_es18_9e3(b, flow_rate);
+      }
+      b += '\n';
+    } else {
+      i4(time_point);
+      i4(time_step);
+      f8(H.t[p]);
+      i4(id[p]);
+      f8(m);
+      i4(group[p]);
+      i4(sol);
+      i4(G.local_cell(c));
+      i4(G.layer(c));
+      if (style == 1) {
+        f8(rfac[k]);
+        f8(wvol[k]);
+        f8(g[0]);
+        f8(g[1]);
+        f8(g[2]);
+      } else {
+        f8(flow_rate);
+      }
+    }
+  }
+  return write_all(path, append != 0, b);
+}

@@ -95,6 +95,9 @@ EXPORTS = {
                               C.c_double, C.c_double, C.c_double, C.c_int32,
                               C.POINTER(C.c_char_p), _i32p, _i32p, _i32p, _i32p, _i32p,
                               _dp, _dp, _dp, _dp, _i32p, _dp], C.c_int),
+    "rwpt_write_observation": ([C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                C.c_int32, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
+                                _i32p, _dp, C.c_double, C.POINTER(C.c_uint8)], C.c_int),
     "rwpt_kde_write": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                         C.c_double, C.c_int32, C.POINTER(RwptKdeOptions), _dp, _dp,
                         C.c_double], C.c_int),
@@ -322,6 +325,21 @@ class Engine:
             out.update({"grad_al": g[0], "grad_at": g[1], "grad_dm": g[2],
                         "grad_porosity_scale": gs.value})
         return out
+
+    def write_observation(self, path, style, cells, time_point, time_step, ids, group,
+                          group_solute, mass=None, flow_rate=0.0, fmt=0, append=True,
+                          select=None):
+        """Observation records of the particles in `cells` (rwpt_write_observation):
+        style 1 resident (active particles), 2 sink (stopped at a weak sink)."""
+        cl, idv, grp, sol = _i(cells), _i(ids), _i(group), _i(group_solute)
+        m = None if mass is None else _d(mass)
+        sel = None if select is None else np.ascontiguousarray(select, dtype=np.uint8)
+        rc = lib().rwpt_write_observation(
+            self.h, os.fsencode(path), int(bool(append)), int(style), int(fmt), cl.size,
+            _p(cl, _i32p), int(time_point), int(time_step), _p(idv, _i32p), _p(grp, _i32p),
+            _p(sol, _i32p), _p(m, _dp), float(flow_rate),
+            None if sel is None else sel.ctypes.data_as(C.POINTER(C.c_uint8)))
+        self._check(rc, "rwpt_write_observation")
 
     def write_timeseries(self, path, time_point, time_step, time, sequence, ids, group,
                          select=None):

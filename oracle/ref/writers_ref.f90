@@ -6,8 +6,9 @@
 ! on particles read from a binary file, with a RectangularGridDisType built
 ! from arrays, as MPathRW.F90:615-712 and :1454, :1743 use them.
 !
-! usage: writers_ref <input>   (writes ts.out, ep.out, epm.out in the
-!                               working directory)
+! usage: writers_ref <input>   (writes ts.out, ep.out, epm.out and the
+!                               observation records obsres.out/.bin,
+!                               obssink.out/.bin in the working directory)
 !   input (stream): int32 hdr(8) = ncol, nrow, nlay, np, ngroups,
 !         trackingDirection, timePointIndex, timeStep;
 !         real64 referenceTime, originX, originY, rotation, time;
@@ -15,7 +16,13 @@
 !         int32 zones(ncell);
 !         int32 group, sequence, id, status, cell, initialCell,
 !         initialFace (np each); real64 lx, ly, lz, t, ilx, ily, ilz, it,
-!         mass (np each); int32 solute(ngroups); character(20) names(ngroups)
+!         mass (np each); int32 solute(ngroups); character(20) names(ngroups);
+!         int32 isobs(ncell); real64 rfac(ncell), wvol(ncell), flowRate
+!
+! Observation records (WriteResidentObsRecord[Binary], WriteSinkObsRecord
+! [Binary], Observation.f90:114-221) for the particles in cells with
+! isobs /= 0: resident for status 1 (at the time point, as MPathRW.F90:1470),
+! sink for status 3 (MPathRW.F90:1354).
 program writers_ref
   use iso_fortran_env, only: int32, real64
   use ModflowRectangularGridModule, only: ModflowRectangularGridType
@@ -24,6 +31,9 @@ program writers_ref
   use ParticleCoordinateModule, only: ParticleCoordinateType
   use GeoReferenceModule, only: GeoReferenceType
   use ParticleManagerModule
+  use ParticleModule, only: ParticleType
+  use ObservationModule, only: ObservationType, WriteResidentObsRecord, &
+    WriteResidentObsRecordBinary, WriteSinkObsRecord, WriteSinkObsRecordBinary
   implicit none
   integer(int32) :: hdr(8)
   integer :: ncol, nrow, nlay, ncell, np, ng, nlc, i, j, k, n, p, g
@@ -40,6 +50,12 @@ program writers_ref
   type(ModpathSimulationDataType), target :: sim
   type(ParticleCoordinateType) :: pc
   type(GeoReferenceType) :: geo
+  integer(int32), allocatable :: isobs(:)
+  real(real64), allocatable :: rfac(:), wvol(:)
+  real(real64) :: flowRate
+  type(ObservationType) :: obs
+  type(ParticleType) :: q1
+  integer :: c
 
   call get_command_argument(1, fin)
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
@@ -103,6 +119,8 @@ program writers_ref
   read(10) igrp, iseq, iid, ist, icell, icell0, iface0
   read(10) lx, ly, lz, tt, ilx, ily, ilz, it, mass
   read(10) solute, names
+  allocate(isobs(ncell), rfac(ncell), wvol(ncell))
+  read(10) isobs, rfac, wvol, flowRate
   close(10)
 
   ! timeseries: header, then the active particles in order
@@ -170,4 +188,43 @@ program writers_ref
   open(13, file='epm.out', status='replace', form='formatted', access='sequential')
   call WriteEndpointsMassParticles(sim, grid, geo, 13)
   close(13)
+
+  ! observation records, particles in file order
+  open(14, file='obsres.out', status='replace', form='formatted', access='sequential')
+  open(15, file='obsres.bin', status='replace', form='unformatted', access='stream')
+  open(16, file='obssink.out', status='replace', form='formatted', access='sequential')
+  open(17, file='obssink.bin', status='replace', form='unformatted', access='stream')
+  do p = 1, np
+    c = icell(p)
+    if (isobs(c) .eq. 0) cycle
+    q1%ID = iid(p)
+    q1%Mass = mass(p)
+    q1%Group = igrp(p)
+    q1%CellNumber = c
+    q1%Layer = (c - 1) / nlc + 1
+    q1%TrackingTime = tt(p)
+    if (ist(p) .eq. 1) then
+      pc%CellNumber = c
+      pc%Layer = (c - 1) / nlc + 1
+      pc%LocalX = lx(p)
+      pc%LocalY = ly(p)
+      pc%LocalZ = lz(p)
+      pc%TrackingTime = tt(p)
+      call grid%ConvertToModelXYZ(pc%CellNumber, pc%LocalX, pc%LocalY, pc%LocalZ, &
+        pc%GlobalX, pc%GlobalY, pc%GlobalZ)
+      call WriteResidentObsRecord(obs, int(hdr(8)), int(hdr(7)), q1, pc, &
+        int(solute(igrp(p))), rfac(c), wvol(c), 14)
+      call WriteResidentObsRecordBinary(obs, int(hdr(8)), int(hdr(7)), q1, pc, &
+        int(solute(igrp(p))), rfac(c), wvol(c), 15)
+    else if (ist(p) .eq. 3) then
+      call WriteSinkObsRecord(obs, int(hdr(8)), int(hdr(7)), q1, int(solute(igrp(p))), &
+        flowRate, 16)
+      call WriteSinkObsRecordBinary(obs, int(hdr(8)), int(hdr(7)), q1, &
+        int(solute(igrp(p))), flowRate, 17)
+    end if
+  end do
+  close(14)
+  close(15)
+  close(16)
+  close(17)
 end program writers_ref

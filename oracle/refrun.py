@@ -153,12 +153,15 @@ def writers_available() -> bool:
 
 
 def writers(P, state, initial, meta, timeout=120):
-    """Runs the reference's timeseries and endpoint writers (writers_ref) on
-    one grid and particle set; returns the bytes of (timeseries, endpoints,
-    mass endpoints).  state: cell, lx, ly, lz, t, status; initial: cell,
-    lx, ly, lz, t, face; meta: group, sequence, id, mass, solute (per
-    group), names (per group), tracking_direction, time_point, time_step,
-    reference_time, origin, rotation, time, zones (per cell)."""
+    """Runs the reference's timeseries, endpoint and observation writers
+    (writers_ref) on one grid and particle set; returns the bytes of
+    (timeseries, endpoints, mass endpoints, resident records text, binary,
+    sink records text, binary).  state: cell, lx, ly, lz, t, status;
+    initial: cell, lx, ly, lz, t, face; meta: group, sequence, id, mass,
+    solute (per group), names (per group), tracking_direction, time_point,
+    time_step, reference_time, origin, rotation, time, zones (per cell) and
+    optionally obs_cells (1-based), retardation and water_volume (per cell),
+    flow_rate."""
     f8, i4 = np.float64, np.int32
     nb = P.ncell
     hdr = np.array([P.ncol, P.nrow, P.nlay, state["cell"].size, len(meta["names"]),
@@ -175,13 +178,21 @@ def writers(P, state, initial, meta, timeout=120):
               initial["ly"], initial["lz"], initial["t"], meta["mass"]):
         parts.append(np.ascontiguousarray(a, f8))
     parts.append(np.ascontiguousarray(meta["solute"], i4))
+    isobs = np.zeros(nb, i4)
+    isobs[np.asarray(meta.get("obs_cells", []), np.int64) - 1] = 1
+    tail = [isobs, np.asarray(meta.get("retardation", np.ones(nb)), f8).reshape(nb),
+            np.asarray(meta.get("water_volume", np.zeros(nb)), f8).reshape(nb),
+            np.array([meta.get("flow_rate", 0.0)], f8)]
     with tempfile.TemporaryDirectory() as d:
         fin = os.path.join(d, "in.bin")
         with open(fin, "wb") as fh:
             for a in parts:
                 fh.write(np.ascontiguousarray(a).tobytes())
             fh.write(names)
+            for a in tail:
+                fh.write(np.ascontiguousarray(a).tobytes())
         subprocess.run([WRITERS_BINARY, fin], check=True, timeout=timeout,
                        stdout=subprocess.DEVNULL, cwd=d)
         return tuple(open(os.path.join(d, f), "rb").read()
-                     for f in ("ts.out", "ep.out", "epm.out"))
+                     for f in ("ts.out", "ep.out", "epm.out", "obsres.out", "obsres.bin",
+                               "obssink.out", "obssink.bin"))

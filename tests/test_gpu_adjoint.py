@@ -189,8 +189,10 @@ def test_autograd_wrapper():
     J = ZonedMisfit.apply(theta, prob)
     J.backward()
     Jn, g = prob.evaluate(theta0, gradient=True)
-    assert float(J) == Jn
-    np.testing.assert_array_equal(theta.grad.numpy(), g)
+    # per-cell gradients and concentrations are sums of FP64 atomics, so two
+    # evaluations agree to rounding, not bit for bit
+    assert float(J) == pytest.approx(Jn, rel=1e-12)
+    np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-9, atol=0)
     assert np.all(np.isfinite(g)) and np.any(g != 0)
     # a change of the porosity scale: d/ds at s = 1.1 is the engine's value / 1.1
     th2 = theta0.copy()
