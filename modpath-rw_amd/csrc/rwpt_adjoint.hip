@@ -199,13 +199,12 @@ template <> struct RecT<double> {
 // directions.
 template <int N, int P0> struct RecD {
   const double *rec;
-  const double *tan;  // this cell's tangent rows: tan[k * stride + f]
-  int64_t stride;     // ncell * kTan
+  const double *tan;  // this cell's tangents: tan[f * 4 + k]
   __device__ __forceinline__ Dual<N> operator()(int f) const {
     Dual<N> r = dconst<N>(rec[f]);
     if (P0 < N && f < kTan) {
 #pragma unroll
-      for (int k = 0; k < N - P0; ++k) r.d[P0 + k] = tan[k * stride + f];
+      for (int k = 0; k < N - P0; ++k) r.d[P0 + k] = tan[f * 4 + k];
     }
     return r;
   }
@@ -250,10 +249,10 @@ __device__ __forceinline__ T deriv4g(const Rec &rec, int b, const T &u, const T 
 }
 
 
-template <typename T, typename Rec, int NDIM>
+template <typename T, typename Rec, int NDIM, typename Obs>
 __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
                                 const T &y, const T &z, const T &dt, Stream &rng,
-                                TermsG<T> &s) {
+                                TermsG<T> &s, Obs &obs) {
   s.vx = (1.0 - x) * rec(F_VX1) + x * rec(F_VX2);
   s.vy = (1.0 - y) * rec(F_VY1) + y * rec(F_VY2);
   s.vz = (1.0 - z) * rec(F_VZ1) + z * rec(F_VZ2);
@@ -293,6 +292,7 @@ __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
     }
     const double r1 = Normal<double, 0>::draw(rng), r2 = Normal<double, 0>::draw(rng);
     const double r3 = Normal<double, 0>::draw(rng);
+    obs.draws(r1, r2, r3);
     s.dBx = B11 * r1 + B12 * r2 + B13 * r3;
     s.dBy = B21 * r1 + B22 * r2 + B23 * r3;
     s.dBz = B31 * r1 + B32 * r2;
@@ -311,6 +311,7 @@ __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
       B22 = (b1 * st) / vn;
     }
     const double r1 = Normal<double, 0>::draw(rng), r2 = Normal<double, 0>::draw(rng);
+    obs.draws(r1, r2, 0.0);
     const T u1 = B11 * r1 + B12 * r2, u2 = B21 * r1 + B22 * r2;
     s.dBx = d1 == 0 ? u1 : (d2 == 0 ? u2 : zero);
     s.dBy = d1 == 1 ? u1 : (d2 == 1 ? u2 : zero);
@@ -321,7 +322,7 @@ __device__ void displacements_g(const Rec &rec, const WalkParams &W, const T &x,
 // One axis of pr_DetectExitFaceAndUpdateTimeStepQuadratic.
 template <typename T>
 __device__ bool quadratic_axis_g(const T &dI, const T &A, const T &B, const T &dtcur,
-                                 T &dtaxis) {
+                                 T &dtaxis, T &root) {
   const T arg = B * B + 4.0 * dI * A;
   if ((val(arg) >= 0.0) && (val(A) != 0.0)) {
     const T sq = dsqrt(arg);
@@ -331,6 +332,7 @@ __device__ bool quadratic_axis_g(const T &dI, const T &A, const T &B, const T &d
       const T m = (val(z1) > 0.0 && val(z2) > 0.0) ? (val(z2) < val(z1) ? z2 : z1)
                                                    : (val(z1) > 0.0 ? z1 : z2);
       dtaxis = m * m;
+      root = m;
       if (val(dtaxis) == 0.0) return true;
     } else if (val(z1) == 0.0 || val(z2) == 0.0) {
       return true;
@@ -339,6 +341,7 @@ __device__ bool quadratic_axis_g(const T &dI, const T &A, const T &B, const T &d
     if (val(B) != 0.0) {
       const T r = dI / B;
       dtaxis = r * r;
+      root = r;
       if (val(dtaxis) == 0.0) return true;
     }
   }
@@ -346,10 +349,10 @@ __device__ bool quadratic_axis_g(const T &dI, const T &A, const T &B, const T &d
   return false;
 }
 
-template <typename T, typename Rec>
+template <typename T, typename Rec, typename Obs>
 __device__ void exit_time_g(const Rec &rec, const T &x, const T &y, const T &z,
                             const T &nx, const T &ny, const T &nz, const TermsG<T> &s,
-                            T &t, T &dt, int &exitFace) {
+                            T &t, T &dt, int &exitFace, Obs &obs) {
   t = t - dt;
   const T pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
   const T dd[3] = {rec(F_DX), rec(F_DY), rec(F_DZ)};
@@ -357,6 +360,8 @@ __device__ void exit_time_g(const Rec &rec, const T &x, const T &y, const T &z,
   const T dv[3] = {s.ddx, s.ddy, s.ddz};
   const T db[3] = {s.dBx, s.dBy, s.dBz};
   T dtx[3] = {Cst<T>::of(0.0), Cst<T>::of(0.0), Cst<T>::of(0.0)};
+  T mr[3] = {Cst<T>::of(0.0), Cst<T>::of(0.0), Cst<T>::of(0.0)};
+  double dIv[3] = {0.0, 0.0, 0.0};
   int fx[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -372,17 +377,21 @@ __device__ void exit_time_g(const Rec &rec, const T &x, const T &y, const T &z,
       if (val(dI) == 0.0) {
         exitFace = fx[a];
         dt = Cst<T>::of(0.0);
+        obs.zero_exit(a);
         return;
       }
-      if (quadratic_axis_g(dI, v[a] + dv[a], db[a], dt, dtx[a])) {
+      dIv[a] = val(dI);
+      if (quadratic_axis_g(dI, v[a] + dv[a], db[a], dt, dtx[a], mr[a])) {
         exitFace = fx[a];
         dt = Cst<T>::of(0.0);
+        obs.zero_exit(a);
         return;
       }
     }
   }
   if (val(dtx[0]) == 0.0 && val(dtx[1]) == 0.0 && val(dtx[2]) == 0.0) {
     t = t + dt;
+    obs.rare();
     return;
   }
   int im = -1;
@@ -398,7 +407,56 @@ __device__ void exit_time_g(const Rec &rec, const T &x, const T &y, const T &z,
   dt = im == 0 ? dtx[0] : (im == 1 ? dtx[1] : dtx[2]);
   exitFace = im == 0 ? fx[0] : (im == 1 ? fx[1] : fx[2]);
   t = t + dt;
+  if constexpr (Obs::kRecords) {
+    const T &m = im == 0 ? mr[0] : (im == 1 ? mr[1] : mr[2]);
+    obs.exit_axis(im, val(m), val(v[im] + dv[im]), val(db[im]), dIv[im]);
+  }
 }
+
+// Observers of an iteration's branches: the plain and dual iterations record
+// nothing; the hand-written reverse step records the branch taken.
+struct NoObs {
+  static constexpr bool kRecords = false;
+  __device__ void clip() {}
+  __device__ void limit(double) {}
+  __device__ void zero_exit(int) {}
+  __device__ void exit_axis(int, double, double, double, double) {}
+  __device__ void rare() {}
+  __device__ void final_dt(double) {}
+  __device__ void draws(double, double, double) {}
+};
+
+struct StepObs {
+  static constexpr bool kRecords = true;
+  bool clipped = false, exited = false, zero = false, odd = false;
+  double c = 1.0;                       // product of the jump-limiter cuts
+  int im = -1;                          // exit axis of the last exit round
+  double m = 0.0, A = 0.0, B = 0.0, dI = 0.0, dt = 0.0;
+  __device__ void clip() { clipped = true; }
+  __device__ void limit(double f) { c = c * f; }
+  __device__ void zero_exit(int a) {
+    exited = true;
+    zero = true;
+    im = a;
+  }
+  __device__ void exit_axis(int a, double mm, double AA, double BB, double dII) {
+    exited = true;
+    zero = false;
+    im = a;
+    m = mm;
+    A = AA;
+    B = BB;
+    dI = dII;
+  }
+  double r[3] = {0.0, 0.0, 0.0};         // the sub-step's normal draws
+  __device__ void rare() { odd = true; }
+  __device__ void final_dt(double d) { dt = d; }
+  __device__ void draws(double a, double b, double c3) {
+    r[0] = a;
+    r[1] = b;
+    r[2] = c3;
+  }
+};
 
 template <typename T>
 __device__ __forceinline__ bool outside_g(const T &x, const T &y, const T &z) {
@@ -408,9 +466,9 @@ __device__ __forceinline__ bool outside_g(const T &x, const T &y, const T &z) {
 // *restart: 1 when the iteration returned to the visit start (location and
 // time), 2 when an elastic rebound with a zero time step returned it to the
 // visit start location only.
-template <typename T, typename Rec, int NDIM>
+template <typename T, typename Rec, int NDIM, typename Obs = NoObs>
 __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
-                         Stream &rng, bool &zero_dt, int &restart) {
+                         Stream &rng, bool &zero_dt, int &restart, Obs &&obs = Obs{}) {
   const T dtcell = rec(F_DTCELL);
   const double maxRelJump = 0.5, dtRed = f32lit(0.1f);
   const T DX = rec(F_DX), DY = rec(F_DY), DZ = rec(F_DZ);
@@ -429,8 +487,9 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
     dt = val(rem) > 0.0 ? rem : Cst<T>::of(fmax(val(rem), 0.0));
     t = Cst<T>::of(W.t_stop);
     atMax = true;
+    obs.clip();
   }
-  displacements_g<T, Rec, NDIM>(rec, W, x, y, z, dt, rng, s);
+  displacements_g<T, Rec, NDIM>(rec, W, x, y, z, dt, rng, s, obs);
   T sq = dsqrt(dt);
   T dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
   T dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
@@ -439,6 +498,7 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
          (fabs(val(dzr) / val(DZ)) > maxRelJump)) {
     t = t - dt;
     dt = dtRed * dt;
+    obs.limit(dtRed);
     sq = dsqrt(dt);
     dxr = s.vx * dt + s.ddx * dt + s.dBx * sq;
     dyr = s.vy * dt + s.ddy * dt + s.dBy * sq;
@@ -452,13 +512,14 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
   int loops = 0;
   while (outside_g(nx, ny, nz)) {
     loops = loops + 1;
-    exit_time_g<T>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace);
+    exit_time_g<T>(rec, x, y, z, nx, ny, nz, s, t, dt, exitFace, obs);
     if (loops > 5) {
       nx = Cst<T>::of(V.ix); ny = Cst<T>::of(V.iy); nz = Cst<T>::of(V.iz);
       t = Cst<T>::of(V.t0);
       dt = dtcell;
       V.restarts = V.restarts + 1;
       restart = 1;
+      obs.rare();
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
@@ -469,6 +530,7 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
       dt = dtcell;
       V.restarts = V.restarts + 1;
       restart = 1;
+      obs.rare();
       if (V.restarts > 10) return IT_UNDEFINED;
       break;
     }
@@ -497,6 +559,7 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
     while (exitFace > 0) {
       if (mb[exitFace - 1] != 1) break;
       rebounds = rebounds + 1;
+      obs.rare();
       if (rebounds > 10) return IT_UNDEFINED;
       const T xi = nx, yi = ny, zi = nz;
       if (val(dt) == 0.0) {
@@ -557,6 +620,7 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
   }
   V.x = nx; V.y = ny; V.z = nz; V.t = t;
   V.exitFace = exitFace;
+  obs.final_dt(val(dt));
   if (atMax) return IT_MAXTIME;
   if (exitFace > 0) {
     zero_dt = val(dt) == 0.0;
@@ -570,10 +634,14 @@ __device__ int iterate_g(const Rec &rec, const WalkParams &W, VisitG<T> &V,
 // particles that are active at the final time.
 // ---------------------------------------------------------------------------
 struct TapeEntry {
-  double x, y, z, t, ix, iy, iz, t0;
-  uint64_t ra, rb;
+  double x, y, z, t, ix, iy, iz, t0;  // input: location, time, visit start
+  uint64_t ra, rb;                    // input: rng state
+  double dt, c, m, r[3];              // outcome: final step, limiter factor, exit root, draws
   int32_t cell;
-  int32_t flags;  // bit 0 first iteration of a visit, 1-4 restarts, 5-7 exitFace
+  int32_t flags;  // input: bit 0 first iteration of a visit, 1-4 restarts,
+                  // 5-7 exitFace; outcome: 8 exited, 9 zero-step exit,
+                  // 10 cut at the window end, 11 rare branch, 12-13 exit
+                  // axis, 14-16 result code, 17-19 exitFace
 };
 
 // Particle arrays of a checkpoint (or of the engine after the window).
@@ -586,6 +654,350 @@ struct CkPtrs {
 
 __device__ __forceinline__ bool same(double a, double b) {
   return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// ---------------------------------------------------------------------------
+// Hand-written reverse of the common iteration (no restart, no rebound):
+// the displacement terms at the iteration's start point are recomputed with
+// their intermediates, and the output adjoint is carried back through
+//   n_a = x_a + (v_a dt + dd_a dt + dB_a sqrt(dt)) / DX_a
+// (dt the cell step times the jump-limiter cuts, or t_stop - t when the
+// step is cut at the end of the window), or, after an exit on axis a*,
+// through dt = m^2 with A m^2 + B m - dI = 0 (A = v + dd, B = dB along a*,
+// dI = DX (1 - x) or -DX x), whose implicit derivative is
+//   (2 A m + B) dm = d(dI) - m^2 dA - m dB,
+// then through the velocity, drift and dispersion-matrix formulas to the
+// start point and to the record fields, whose parameter tangents give the
+// cell parameters' gradient.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ivn_safe(double v) { return v > 0.0 ? 1.0 / v : 0.0; }
+
+// element i of a 3-array and an update of it without a dynamic index (which
+// would place the array in scratch memory)
+__device__ __forceinline__ double pick(const double a[3], int i) {
+  return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+__device__ __forceinline__ void bump(double a[3], int i, double v) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (q == i) a[q] += v;
+}
+
+struct DispFull {
+  double v[3], V1[3], V2[3], dd[3], dB[3], r[3];
+  double b[3], vn, vxy, sl, st, sh;
+  double B11, B12, B13, B21, B22, B23, B31, B32;
+  double phi, R, aL, aT, dm;
+};
+
+__device__ __forceinline__ double tri8d(const double *c, double x, double y, double z) {
+  const double v00 = (1.0 - x) * c[0] + x * c[1];
+  const double v01 = (1.0 - x) * c[4] + x * c[5];
+  const double v10 = (1.0 - x) * c[2] + x * c[3];
+  const double v11 = (1.0 - x) * c[6] + x * c[7];
+  const double v0 = (1.0 - y) * v00 + y * v10;
+  const double v1 = (1.0 - y) * v01 + y * v11;
+  return (1.0 - z) * v0 + z * v1;
+}
+
+__device__ __forceinline__ double der4d(const double *d, double u, double w) {
+  const double v0 = (1.0 - u) * d[0] + u * d[1];
+  const double v1 = (1.0 - u) * d[2] + u * d[3];
+  return (1.0 - w) * v0 + w * v1;
+}
+
+// Reverse of the trilinear interpolation of the corner values at base f:
+// adds lam * db/d(x, y, z) to lx and lam * weight to the parameters through
+// the corner values' tangents.
+__device__ __forceinline__ void tri8_rev(const double *rec, const double *tan, int f,
+                                         double x, double y, double z, double lam,
+                                         double lx[3], double g[4]) {
+  const double *c = rec + f;
+  const double v00 = (1.0 - x) * c[0] + x * c[1], v10 = (1.0 - x) * c[2] + x * c[3];
+  const double v01 = (1.0 - x) * c[4] + x * c[5], v11 = (1.0 - x) * c[6] + x * c[7];
+  lx[0] += lam * ((1.0 - z) * ((1.0 - y) * (c[1] - c[0]) + y * (c[3] - c[2])) +
+                  z * ((1.0 - y) * (c[5] - c[4]) + y * (c[7] - c[6])));
+  lx[1] += lam * ((1.0 - z) * (v10 - v00) + z * (v11 - v01));
+  lx[2] += lam * (((1.0 - y) * v01 + y * v11) - ((1.0 - y) * v00 + y * v10));
+  const double w[8] = {(1.0 - x) * (1.0 - y) * (1.0 - z), x * (1.0 - y) * (1.0 - z),
+                       (1.0 - x) * y * (1.0 - z),         x * y * (1.0 - z),
+                       (1.0 - x) * (1.0 - y) * z,         x * (1.0 - y) * z,
+                       (1.0 - x) * y * z,                 x * y * z};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double *tq = tan + (f + q) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += w[q] * tq[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) g[k] += lam * acc[k];
+}
+
+// Reverse of der4d at base f over the coordinates (iu, iw).
+__device__ __forceinline__ void der4_rev(const double *rec, const double *tan, int f,
+                                         const double p[3], int iu, int iw, double lam,
+                                         double lx[3], double g[4]) {
+  const double *d = rec + f;
+  const double u = p[iu], w = p[iw];
+  lx[iu] += lam * ((1.0 - w) * (d[1] - d[0]) + w * (d[3] - d[2]));
+  lx[iw] += lam * (((1.0 - u) * d[2] + u * d[3]) - ((1.0 - u) * d[0] + u * d[1]));
+  const double wt[4] = {(1.0 - w) * (1.0 - u), (1.0 - w) * u, w * (1.0 - u), w * u};
+  const double *t0 = tan + f * 4;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    g[k] += lam * (wt[0] * t0[k] + wt[1] * t0[4 + k] + wt[2] * t0[8 + k] + wt[3] * t0[12 + k]);
+}
+
+// lam times the parameter tangents of record field f
+__device__ __forceinline__ void field_rev(const double *tan, int f, double lam, double g[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) g[k] += lam * tan[f * 4 + k];
+}
+
+template <int NDIM>
+__device__ void disp_full(const double *rec, const WalkParams &W, const double p[3],
+                          const double r[3], DispFull &D) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    D.V1[a] = rec[F_VX1 + 2 * a];
+    D.V2[a] = rec[F_VX2 + 2 * a];
+    D.v[a] = (1.0 - p[a]) * D.V1[a] + p[a] * D.V2[a];
+  }
+  D.aL = rec[F_ALH];
+  D.aT = rec[F_ATH];
+  D.dm = rec[F_DMEFF];
+  D.phi = rec[F_PHI];
+  D.R = rec[F_R];
+  const double *dd = rec + F_DD;
+  const double x = p[0], y = p[1], z = p[2];
+  const double t0 = der4d(dd + 0, y, z), t1 = der4d(dd + 4, x, z), t2 = der4d(dd + 8, x, y);
+  const double t3 = der4d(dd + 12, y, z), t4 = der4d(dd + 16, y, z);
+  const double t5 = der4d(dd + 20, x, z), t6 = der4d(dd + 24, x, z);
+  const double t7 = der4d(dd + 28, x, y), t8 = der4d(dd + 32, x, y);
+  D.dd[0] = ((t0 + t5 + t7) / D.phi) / D.R;
+  D.dd[1] = ((t3 + t1 + t8) / D.phi) / D.R;
+  D.dd[2] = ((t4 + t6 + t2) / D.phi) / D.R;
+  D.B11 = D.B12 = D.B13 = D.B21 = D.B22 = D.B23 = D.B31 = D.B32 = 0.0;
+  D.sl = D.st = D.sh = 0.0;
+  if constexpr (NDIM == 3) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) D.b[a] = tri8d(rec + F_VC + 8 * a, x, y, z);
+    D.vn = sqrt(D.b[0] * D.b[0] + D.b[1] * D.b[1] + D.b[2] * D.b[2]);
+    D.vxy = sqrt(D.b[0] * D.b[0] + D.b[1] * D.b[1]);
+    if (D.vn > 0.0) {
+      D.sl = sqrt((2.0 * (D.aL * D.vn + D.dm)) / D.R);
+      D.st = sqrt((2.0 * (D.aT * D.vn + D.dm)) / D.R);
+      D.B11 = (D.b[0] * D.sl) / D.vn;
+      D.B21 = (D.b[1] * D.sl) / D.vn;
+      D.B31 = (D.b[2] * D.sl) / D.vn;
+      D.B32 = (D.vxy * D.st) / D.vn;
+      if (D.vxy > 0.0) {
+        D.sh = sqrt((2.0 * (D.aT * D.vn + D.dm)) / D.R);
+        D.B12 = -((((D.b[0] * D.b[2]) * D.st) / D.vn) / D.vxy);
+        D.B13 = -((D.b[1] * D.sh) / D.vxy);
+        D.B22 = -((((D.b[1] * D.b[2]) * D.st) / D.vn) / D.vxy);
+        D.B23 = (D.b[0] * D.sh) / D.vxy;
+      }
+    }
+    D.r[0] = r[0];
+    D.r[1] = r[1];
+    D.r[2] = r[2];
+    D.dB[0] = D.B11 * D.r[0] + D.B12 * D.r[1] + D.B13 * D.r[2];
+    D.dB[1] = D.B21 * D.r[0] + D.B22 * D.r[1] + D.B23 * D.r[2];
+    D.dB[2] = D.B31 * D.r[0] + D.B32 * D.r[1];
+  } else {
+    const int d1 = W.dims[0] - 1, d2 = W.dims[1] - 1;
+    D.b[0] = tri8d(rec + F_VC + 8 * d1, x, y, z);
+    D.b[1] = tri8d(rec + F_VC + 8 * d2, x, y, z);
+    D.b[2] = 0.0;
+    D.vn = sqrt(D.b[0] * D.b[0] + D.b[1] * D.b[1]);
+    D.vxy = 0.0;
+    if (D.vn > 0.0) {
+      D.sl = sqrt((2.0 * (D.aL * D.vn + D.dm)) / D.R);
+      D.st = sqrt((2.0 * (D.aT * D.vn + D.dm)) / D.R);
+      D.B11 = (D.b[0] * D.sl) / D.vn;
+      D.B21 = (D.b[1] * D.sl) / D.vn;
+      D.B12 = -((D.b[1] * D.st) / D.vn);
+      D.B22 = (D.b[0] * D.st) / D.vn;
+    }
+    D.r[0] = r[0];
+    D.r[1] = r[1];
+    D.r[2] = 0.0;
+    const double u1 = D.B11 * D.r[0] + D.B12 * D.r[1], u2 = D.B21 * D.r[0] + D.B22 * D.r[1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) D.dB[a] = a == d1 ? u1 : (a == d2 ? u2 : 0.0);
+  }
+}
+
+// Returns false (nothing changed) for an iteration on a rare branch; the
+// branch taken and the draws come from the tape.
+template <int NDIM>
+__device__ bool reverse_step(const double *rec, const double *tan, const WalkParams &W,
+                             const TapeEntry &E, const double l[4], double ln[4], double g[4]) {
+  const int32_t fl = E.flags;
+  const int out = (fl >> 14) & 7;
+  if ((fl & (1 << 11)) || out == IT_UNDEFINED) return false;
+  const bool exited = (fl >> 8) & 1, zero = (fl >> 9) & 1, clipped = (fl >> 10) & 1;
+  const int im = (fl >> 12) & 3, exitFace = (fl >> 17) & 7;
+  const double p[3] = {E.x, E.y, E.z};
+  const double DXv[3] = {rec[F_DX], rec[F_DY], rec[F_DZ]};
+  DispFull D;
+  disp_full<NDIM>(rec, W, p, E.r, D);
+  // output adjoint; after an exit the face coordinate is the constant face
+  const int ax = (out == IT_EXIT && exitFace > 0) ? (exitFace - 1) / 2 : -1;
+  const double lo[3] = {ax == 0 ? 0.0 : l[0], ax == 1 ? 0.0 : l[1], ax == 2 ? 0.0 : l[2]};
+  const double lt_out = l[3];
+  double lx[3] = {0.0, 0.0, 0.0}, lv[3] = {0.0, 0.0, 0.0}, ldd[3] = {0.0, 0.0, 0.0};
+  double ldB[3] = {0.0, 0.0, 0.0}, lt = 0.0, ldt = 0.0;
+  const double dt = E.dt, sq = sqrt(dt);
+  const double hs = sq > 0.0 ? 0.5 / sq : 0.0;
+  const int skip = exited ? im : -1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a == skip) continue;
+    const double w = lo[a] / DXv[a];
+    lx[a] += lo[a];
+    lv[a] += w * dt;
+    ldd[a] += w * dt;
+    ldB[a] += w * sq;
+    ldt += w * (D.v[a] + D.dd[a] + D.dB[a] * hs);
+  }
+  if (exited) {
+    // t_out = t + dt_exit
+    lt += lt_out;
+    ldt += lt_out;
+    if (!zero) {
+      // the root of A m^2 + B m - dI = 0 on the exit axis, A = v + dd, B = dB
+      const double A = pick(D.v, im) + pick(D.dd, im), Bq = pick(D.dB, im), m = E.m;
+      const double den = 2.0 * A * m + Bq;
+      if (den != 0.0) {
+        const double q = 2.0 * m * ldt / den;
+        bump(lx, im, -pick(DXv, im) * q);   // d(dI)/dx = -DX on either face
+        bump(lv, im, -m * m * q);
+        bump(ldd, im, -m * m * q);
+        bump(ldB, im, -m * q);
+      }
+    }
+  } else if (!clipped) {
+    // dt = c dt_cell, t_out = t + dt
+    lt += lt_out;
+    field_rev(tan, F_DTCELL, E.c * (ldt + lt_out), g);
+  } else if (E.c == 1.0) {
+    // dt = t_stop - t, t_out = t_stop
+    lt += -ldt;
+  } else {
+    // dt = c (t_stop - t), t_out = t + dt
+    lt += lt_out * (1.0 - E.c) - E.c * ldt;
+  }
+  // velocities
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lx[a] += lv[a] * (D.V2[a] - D.V1[a]);
+    field_rev(tan, F_VX1 + 2 * a, lv[a] * (1.0 - p[a]), g);
+    field_rev(tan, F_VX2 + 2 * a, lv[a] * p[a], g);
+  }
+  // drift: dd_a = S_a / phi / R
+  {
+    const double sphi = ldd[0] / D.phi / D.R, sphj = ldd[1] / D.phi / D.R,
+                 sphk = ldd[2] / D.phi / D.R;
+    field_rev(tan, F_PHI,
+              -(ldd[0] * D.dd[0] + ldd[1] * D.dd[1] + ldd[2] * D.dd[2]) / D.phi, g);
+    // term k at F_DD + 4k over its coordinates: t0 yz, t1 xz, t2 xy, t3 yz,
+    // t4 yz, t5 xz, t6 xz, t7 xy, t8 xy; dd_x = t0+t5+t7, dd_y = t3+t1+t8,
+    // dd_z = t4+t6+t2
+    der4_rev(rec, tan, F_DD + 0, p, 1, 2, sphi, lx, g);
+    der4_rev(rec, tan, F_DD + 20, p, 0, 2, sphi, lx, g);
+    der4_rev(rec, tan, F_DD + 28, p, 0, 1, sphi, lx, g);
+    der4_rev(rec, tan, F_DD + 12, p, 1, 2, sphj, lx, g);
+    der4_rev(rec, tan, F_DD + 4, p, 0, 2, sphj, lx, g);
+    der4_rev(rec, tan, F_DD + 32, p, 0, 1, sphj, lx, g);
+    der4_rev(rec, tan, F_DD + 16, p, 1, 2, sphk, lx, g);
+    der4_rev(rec, tan, F_DD + 24, p, 0, 2, sphk, lx, g);
+    der4_rev(rec, tan, F_DD + 8, p, 0, 1, sphk, lx, g);
+  }
+  // dispersion matrix
+  double lb[3] = {0.0, 0.0, 0.0}, lvn = 0.0, lvxy = 0.0, lsl = 0.0, lst = 0.0, lsh = 0.0;
+  if (D.vn > 0.0) {
+    if constexpr (NDIM == 3) {
+      const double l11 = ldB[0] * D.r[0], l12 = ldB[0] * D.r[1], l13 = ldB[0] * D.r[2];
+      const double l21 = ldB[1] * D.r[0], l22 = ldB[1] * D.r[1], l23 = ldB[1] * D.r[2];
+      const double l31 = ldB[2] * D.r[0], l32 = ldB[2] * D.r[1];
+      const double ivn = 1.0 / D.vn;
+      lb[0] += l11 * D.sl * ivn;
+      lb[1] += l21 * D.sl * ivn;
+      lb[2] += l31 * D.sl * ivn;
+      lsl += (l11 * D.b[0] + l21 * D.b[1] + l31 * D.b[2]) * ivn;
+      lvn -= (l11 * D.B11 + l21 * D.B21 + l31 * D.B31 + l32 * D.B32) * ivn;
+      lvxy += l32 * D.st * ivn;
+      lst += l32 * D.vxy * ivn;
+      if (D.vxy > 0.0) {
+        const double ivxy = 1.0 / D.vxy, k = D.st * ivn * ivxy;
+        lb[0] += -l12 * D.b[2] * k + l23 * D.sh * ivxy;
+        lb[1] += -l22 * D.b[2] * k - l13 * D.sh * ivxy;
+        lb[2] += -(l12 * D.b[0] + l22 * D.b[1]) * k;
+        lst += -(l12 * D.b[0] * D.b[2] + l22 * D.b[1] * D.b[2]) * ivn * ivxy;
+        lsh += (-l13 * D.b[1] + l23 * D.b[0]) * ivxy;
+        lvn -= (l12 * D.B12 + l22 * D.B22) * ivn;
+        lvxy -= (l12 * D.B12 + l22 * D.B22 + l13 * D.B13 + l23 * D.B23) * ivxy;
+      }
+    } else {
+      const double b1 = pick(ldB, W.dims[0] - 1), b2 = pick(ldB, W.dims[1] - 1);
+      const double l11 = b1 * D.r[0], l12 = b1 * D.r[1];
+      const double l21 = b2 * D.r[0], l22 = b2 * D.r[1];
+      const double ivn = 1.0 / D.vn;
+      lb[0] += (l11 * D.sl + l22 * D.st) * ivn;
+      lb[1] += (l21 * D.sl - l12 * D.st) * ivn;
+      lsl += (l11 * D.b[0] + l21 * D.b[1]) * ivn;
+      lst += (-l12 * D.b[1] + l22 * D.b[0]) * ivn;
+      lvn -= (l11 * D.B11 + l21 * D.B21 + l12 * D.B12 + l22 * D.B22) * ivn;
+    }
+    // s = sqrt(2 (a vn + dm) / R): ds = (vn da + a dvn + ddm) / (s R)
+    double laL = 0.0, laT = 0.0, ldm = 0.0;
+    if (D.sl > 0.0) {
+      const double f = lsl / (D.sl * D.R);
+      laL += f * D.vn;
+      lvn += f * D.aL;
+      ldm += f;
+    }
+    if (D.st > 0.0) {
+      const double f = lst / (D.st * D.R);
+      laT += f * D.vn;
+      lvn += f * D.aT;
+      ldm += f;
+    }
+    if (D.sh > 0.0) {
+      const double f = lsh / (D.sh * D.R);
+      laT += f * D.vn;
+      lvn += f * D.aT;
+      ldm += f;
+    }
+    field_rev(tan, F_ALH, laL, g);
+    field_rev(tan, F_ATH, laT, g);
+    field_rev(tan, F_DMEFF, ldm, g);
+    // norms
+    lb[0] += lvn * D.b[0] * ivn_safe(D.vn);
+    lb[1] += lvn * D.b[1] * ivn_safe(D.vn);
+    lb[2] += lvn * D.b[2] * ivn_safe(D.vn);
+    if (D.vxy > 0.0) {
+      lb[0] += lvxy * D.b[0] / D.vxy;
+      lb[1] += lvxy * D.b[1] / D.vxy;
+    }
+    // corner velocities
+    if constexpr (NDIM == 3) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        tri8_rev(rec, tan, F_VC + 8 * a, p[0], p[1], p[2], lb[a], lx, g);
+    } else {
+      tri8_rev(rec, tan, F_VC + 8 * (W.dims[0] - 1), p[0], p[1], p[2], lb[0], lx, g);
+      tri8_rev(rec, tan, F_VC + 8 * (W.dims[1] - 1), p[0], p[1], p[2], lb[1], lx, g);
+    }
+  }
+  ln[0] = lx[0];
+  ln[1] = lx[1];
+  ln[2] = lx[2];
+  ln[3] = lt;
+  return true;
 }
 
 // Replay of one window for the listed particles from checkpoint `in`,
@@ -650,13 +1062,25 @@ __global__ void __launch_bounds__(kBlk) adj_replay(
       E.ix = V.ix; E.iy = V.iy; E.iz = V.iz; E.t0 = V.t0;
       E.ra = rng.a; E.rb = rng.b;
       E.cell = c;
-      E.flags = (first ? 1 : 0) | (V.restarts << 1) | (V.exitFace << 5);
-      tape[base + k] = E;
-      k += 1;
+      int32_t fl = (first ? 1 : 0) | (V.restarts << 1) | (V.exitFace << 5);
       first = false;
       bool zero_dt = false;
       int restart = 0;
-      const int out = iterate_g<double, RecT<double>, NDIM>(RecT<double>{rec}, W, V, rng, zero_dt, restart);
+      StepObs ob;
+      const int out = iterate_g<double, RecT<double>, NDIM, StepObs &>(
+          RecT<double>{rec}, W, V, rng, zero_dt, restart, ob);
+      E.dt = ob.dt;
+      E.c = ob.c;
+      E.m = ob.m;
+      E.r[0] = ob.r[0];
+      E.r[1] = ob.r[1];
+      E.r[2] = ob.r[2];
+      fl |= (ob.exited ? 1 << 8 : 0) | (ob.zero ? 1 << 9 : 0) | (ob.clipped ? 1 << 10 : 0) |
+            (ob.odd || restart != 0 ? 1 << 11 : 0) | ((ob.im < 0 ? 0 : ob.im) << 12) |
+            (out << 14) | (V.exitFace << 17);
+      E.flags = fl;
+      tape[base + k] = E;
+      k += 1;
       if (out == IT_CONTINUE) continue;
       if (out == IT_MAXTIME) {
         status = 1;
@@ -730,25 +1154,25 @@ __device__ __forceinline__ void flush(int c, const double g[4], double *g_al, do
   g_s += g[3];
 }
 
-// Reverse sweep of one window: lam (4 per listed particle: x y z t) holds
-// the adjoint of the state after the window on entry and of the state
-// before it on return.
-template <int NDIM>
-__global__ void __launch_bounds__(kBlk) adj_backward(
-    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
-    int64_t nsel, const WalkParams W, const int64_t *__restrict__ off,
-    const TapeEntry *__restrict__ tape, double *__restrict__ lam, double *g_al,
-    double *g_at, double *g_dm, double *g_s) {
+// Reverse sweep of one particle's tape from entry `from` down to the
+// window's first entry, l and mu the adjoints of the state and of the visit
+// start.  LEAN: the hand-written reverse step only; at an iteration it does
+// not cover the sweep stops and returns that entry's index (l and mu then
+// hold the state to resume from).  Otherwise rare iterations use
+// dual-number Jacobians.  Returns -1 when the window is done.
+template <int NDIM, bool LEAN>
+__device__ int64_t sweep(const double *__restrict__ cells, const double *__restrict__ tan,
+                         int64_t ncell, const WalkParams &W, int64_t first, int64_t from,
+                         const TapeEntry *__restrict__ tape, double l[4], double mu[4],
+                         double *g_al, double *g_at, double *g_dm, double &gs) {
   using D = Dual<4>;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double gs = 0.0;
-  if (i < nsel) {
-    double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
-    double mu[4] = {0.0, 0.0, 0.0, 0.0};
-    double g[4] = {0.0, 0.0, 0.0, 0.0};
-    int cur = 0;
-    const bool any = l[0] != 0.0 || l[1] != 0.0 || l[2] != 0.0 || l[3] != 0.0;
-    for (int64_t k = off[i + 1] - 1; any && k >= off[i]; --k) {
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  int cur = 0;
+  const bool any = l[0] != 0.0 || l[1] != 0.0 || l[2] != 0.0 || l[3] != 0.0 || mu[0] != 0.0 ||
+                   mu[1] != 0.0 || mu[2] != 0.0 || mu[3] != 0.0;
+  int64_t stop = -1;
+  {
+    for (int64_t k = from; any && k >= first; --k) {
       const TapeEntry E = tape[k];
       if (E.cell != cur) {
         flush(cur, g, g_al, g_at, g_dm, gs);
@@ -756,45 +1180,56 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
         cur = E.cell;
       }
       const int64_t c0 = (int64_t)(E.cell - 1);
-      const double *crec = cells + c0 * CELL_DOUBLES, *ctan = tan + c0 * kTan;
-      const int rs = (E.flags >> 1) & 15, ef = (E.flags >> 5) & 7;
-      // pass 1: directions x y z t, the record held fixed
-      VisitG<D> V;
-      V.x = dconst<4>(E.x); V.x.d[0] = 1.0;
-      V.y = dconst<4>(E.y); V.y.d[1] = 1.0;
-      V.z = dconst<4>(E.z); V.z.d[2] = 1.0;
-      V.t = dconst<4>(E.t); V.t.d[3] = 1.0;
-      V.ix = E.ix; V.iy = E.iy; V.iz = E.iz; V.t0 = E.t0;
-      V.restarts = rs;
-      V.exitFace = ef;
-      Stream rng{E.ra, E.rb};
-      bool zero_dt = false;
+      const double *crec = cells + c0 * CELL_DOUBLES, *ctan = tan + c0 * kTan * 4;
+      double ln[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {l[0], l[1], l[2], l[3]};
       int restart = 0;
-      const int out = iterate_g<D, RecD<4, 4>, NDIM>(RecD<4, 4>{crec, ctan, ncell * kTan}, W,
-                                                    V, rng, zero_dt, restart);
-      // the face coordinate after an exit is the constant face value
-      const int ax = (out == IT_EXIT && V.exitFace > 0) ? (V.exitFace - 1) / 2 : -1;
-      double lo[4] = {ax == 0 ? 0.0 : l[0], ax == 1 ? 0.0 : l[1], ax == 2 ? 0.0 : l[2], l[3]};
-      double ln[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        ln[j] = lo[0] * V.x.d[j] + lo[1] * V.y.d[j] + lo[2] * V.z.d[j] + lo[3] * V.t.d[j];
-      // pass 2: directions aLH aTH Dm porosity scale, the state held fixed
-      {
-        VisitG<D> U;
-        U.x = dconst<4>(E.x); U.y = dconst<4>(E.y); U.z = dconst<4>(E.z);
-        U.t = dconst<4>(E.t);
-        U.ix = E.ix; U.iy = E.iy; U.iz = E.iz; U.t0 = E.t0;
-        U.restarts = rs;
-        U.exitFace = ef;
-        Stream r2{E.ra, E.rb};
-        bool z2 = false;
-        int rs2 = 0;
-        (void)iterate_g<D, RecD<4, 0>, NDIM>(RecD<4, 0>{crec, ctan, ncell * kTan}, W, U, r2,
-                                            z2, rs2);
+      if (!reverse_step<NDIM>(crec, ctan, W, E, l, ln, g)) {
+        if constexpr (LEAN) {
+          stop = k;
+          break;
+        } else {
+        // a restart, rebound or unmatched exit: dual-number Jacobians
+        restart = 0;
+        const int rs = (E.flags >> 1) & 15, ef = (E.flags >> 5) & 7;
+        // pass 1: directions x y z t, the record held fixed
+        VisitG<D> V;
+        V.x = dconst<4>(E.x); V.x.d[0] = 1.0;
+        V.y = dconst<4>(E.y); V.y.d[1] = 1.0;
+        V.z = dconst<4>(E.z); V.z.d[2] = 1.0;
+        V.t = dconst<4>(E.t); V.t.d[3] = 1.0;
+        V.ix = E.ix; V.iy = E.iy; V.iz = E.iz; V.t0 = E.t0;
+        V.restarts = rs;
+        V.exitFace = ef;
+        Stream rng{E.ra, E.rb};
+        bool zero_dt = false;
+        const int out = iterate_g<D, RecD<4, 4>, NDIM>(RecD<4, 4>{crec, ctan}, W,
+                                                      V, rng, zero_dt, restart);
+        // the face coordinate after an exit is the constant face value
+        const int ax = (out == IT_EXIT && V.exitFace > 0) ? (V.exitFace - 1) / 2 : -1;
+        lo[0] = ax == 0 ? 0.0 : l[0];
+        lo[1] = ax == 1 ? 0.0 : l[1];
+        lo[2] = ax == 2 ? 0.0 : l[2];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          g[j] += lo[0] * U.x.d[j] + lo[1] * U.y.d[j] + lo[2] * U.z.d[j] + lo[3] * U.t.d[j];
+          ln[j] = lo[0] * V.x.d[j] + lo[1] * V.y.d[j] + lo[2] * V.z.d[j] + lo[3] * V.t.d[j];
+        // pass 2: directions aLH aTH Dm porosity scale, the state held fixed
+        {
+          VisitG<D> U;
+          U.x = dconst<4>(E.x); U.y = dconst<4>(E.y); U.z = dconst<4>(E.z);
+          U.t = dconst<4>(E.t);
+          U.ix = E.ix; U.iy = E.iy; U.iz = E.iz; U.t0 = E.t0;
+          U.restarts = rs;
+          U.exitFace = ef;
+          Stream r2{E.ra, E.rb};
+          bool z2 = false;
+          int rs2 = 0;
+          (void)iterate_g<D, RecD<4, 0>, NDIM>(RecD<4, 0>{crec, ctan}, W, U, r2,
+                                              z2, rs2);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            g[j] += lo[0] * U.x.d[j] + lo[1] * U.y.d[j] + lo[2] * U.z.d[j] + lo[3] * U.t.d[j];
+        }
+        }
       }
       if (restart == 1) {
 #pragma unroll
@@ -814,6 +1249,57 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
       for (int q = 0; q < 4; ++q) l[q] = ln[q];
     }
     flush(cur, g, g_al, g_at, g_dm, gs);
+  }
+  return stop;
+}
+
+// Reverse sweep of one window: lam (4 per listed particle: x y z t) holds
+// the adjoint of the state after the window on entry and of the state
+// before it on return.  The lean pass leaves the particles that reach a
+// rare iteration in `deferred` with their resume index; the full pass
+// finishes them.
+template <int NDIM>
+__global__ void __launch_bounds__(kBlk) adj_backward(
+    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
+    int64_t nsel, const WalkParams W, const int64_t *__restrict__ off,
+    const TapeEntry *__restrict__ tape, double *__restrict__ lam, double *__restrict__ mus,
+    int64_t *__restrict__ resume, int32_t *__restrict__ deferred,
+    unsigned long long *__restrict__ ndeferred, double *g_al, double *g_at, double *g_dm,
+    double *g_s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double gs = 0.0;
+  if (i < nsel) {
+    double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
+    double mu[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t k = sweep<NDIM, true>(cells, tan, ncell, W, off[i], off[i + 1] - 1, tape, l,
+                                        mu, g_al, g_at, g_dm, gs);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      lam[4 * i + q] = l[q];
+      mus[4 * i + q] = mu[q];
+    }
+    resume[i] = k;
+    if (k >= 0) deferred[atomicAdd(ndeferred, 1ull)] = (int32_t)i;
+  }
+  for (int o = 32; o > 0; o >>= 1) gs += __shfl_down(gs, o, 64);
+  if (__lane_id() == 0 && gs != 0.0) unsafeAtomicAdd(g_s, gs);
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(kBlk) adj_backward_full(
+    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
+    const WalkParams W, const int64_t *__restrict__ off, const TapeEntry *__restrict__ tape,
+    double *__restrict__ lam, const double *__restrict__ mus,
+    const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred, int64_t ndef,
+    double *g_al, double *g_at, double *g_dm, double *g_s) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double gs = 0.0;
+  if (j < ndef) {
+    const int64_t i = deferred[j];
+    double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
+    double mu[4] = {mus[4 * i], mus[4 * i + 1], mus[4 * i + 2], mus[4 * i + 3]};
+    (void)sweep<NDIM, false>(cells, tan, ncell, W, off[i], resume[i], tape, l, mu, g_al, g_at,
+                             g_dm, gs);
 #pragma unroll
     for (int q = 0; q < 4; ++q) lam[4 * i + q] = l[q];
   }
@@ -1064,15 +1550,19 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   const int64_t nsel = (int64_t)hsel.size();
   int32_t *sel;
   double *lam, *tan, *resid_d, *g_al, *g_at, *g_dm, *g_s;
-  int64_t *cnt, *off;
-  unsigned long long *bad;
+  int64_t *cnt, *off, *resume;
+  double *mus;
+  int32_t *deferred;
+  unsigned long long *bad, *ndef;
   const int64_t ncell = G.ncell;
   if ((err = B.alloc(&sel, nsel)) != hipSuccess || (err = B.alloc(&lam, 4 * nsel)) != hipSuccess ||
       (err = B.alloc(&tan, 4 * (size_t)ncell * kTan)) != hipSuccess ||
       (err = B.alloc(&resid_d, n_obs)) != hipSuccess || (err = B.alloc(&g_al, ncell)) != hipSuccess ||
       (err = B.alloc(&g_at, ncell)) != hipSuccess || (err = B.alloc(&g_dm, ncell)) != hipSuccess ||
       (err = B.alloc(&g_s, 1)) != hipSuccess || (err = B.alloc(&cnt, nsel + 1)) != hipSuccess ||
-      (err = B.alloc(&off, nsel + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess)
+      (err = B.alloc(&off, nsel + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess ||
+      (err = B.alloc(&resume, nsel)) != hipSuccess || (err = B.alloc(&mus, 4 * nsel)) != hipSuccess ||
+      (err = B.alloc(&deferred, nsel)) != hipSuccess || (err = B.alloc(&ndef, 1)) != hipSuccess)
     return adj_hip(e, err, "adjoint memory");
   if ((nsel > 0 && (err = hipMemcpyAsync(sel, hsel.data(), 4 * nsel, hipMemcpyHostToDevice,
                                          st)) != hipSuccess) ||
@@ -1152,16 +1642,40 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       e->err = buf;
       return RWPT_E_STATE;
     }
+    if ((err = hipMemsetAsync(ndef, 0, 8, st)) != hipSuccess)
+      return adj_hip(e, err, "reverse sweep");
+    double *ga = grad_al ? g_al : nullptr, *gt = grad_at ? g_at : nullptr,
+           *gd = grad_dm ? g_dm : nullptr;
     if (ndim == 3)
       hipLaunchKernelGGL(adj_backward<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
                          (const double *)tan, ncell, nsel, W, (const int64_t *)off,
-                         (const TapeEntry *)tape, lam, grad_al ? g_al : nullptr,
-                         grad_at ? g_at : nullptr, grad_dm ? g_dm : nullptr, g_s);
+                         (const TapeEntry *)tape, lam, mus, resume, deferred, ndef, ga, gt, gd,
+                         g_s);
     else
       hipLaunchKernelGGL(adj_backward<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
                          (const double *)tan, ncell, nsel, W, (const int64_t *)off,
-                         (const TapeEntry *)tape, lam, grad_al ? g_al : nullptr,
-                         grad_at ? g_at : nullptr, grad_dm ? g_dm : nullptr, g_s);
+                         (const TapeEntry *)tape, lam, mus, resume, deferred, ndef, ga, gt, gd,
+                         g_s);
+    unsigned long long nd = 0;
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipMemcpyAsync(&nd, ndef, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (err = hipStreamSynchronize(st)) != hipSuccess)
+      return adj_hip(e, err, "reverse sweep");
+    if (nd > 0) {
+      const dim3 dgrid((unsigned)((nd + kBlk - 1) / kBlk));
+      if (ndim == 3)
+        hipLaunchKernelGGL(adj_backward_full<3>, dgrid, dim3(kBlk), 0, st,
+                           (const double *)G.cells, (const double *)tan, ncell, W,
+                           (const int64_t *)off, (const TapeEntry *)tape, lam,
+                           (const double *)mus, (const int64_t *)resume,
+                           (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
+      else
+        hipLaunchKernelGGL(adj_backward_full<2>, dgrid, dim3(kBlk), 0, st,
+                           (const double *)G.cells, (const double *)tan, ncell, W,
+                           (const int64_t *)off, (const TapeEntry *)tape, lam,
+                           (const double *)mus, (const int64_t *)resume,
+                           (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
+    }
     if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "reverse sweep");
   }
   if ((grad_al && (err = hipMemcpyAsync(grad_al, g_al, 8 * ncell, hipMemcpyDeviceToHost, st)) !=

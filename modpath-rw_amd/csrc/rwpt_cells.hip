@@ -482,17 +482,18 @@ __global__ void fd_steps(int64_t n, const double *__restrict__ a, double rel,
   if (i < n) h[i] = (a && a[i] != 0.0) ? rel * fabs(a[i]) : rel;
 }
 
-// (plus - minus) / (2 h) of the double fields 0..F_CONN-1 of every record;
-// h per cell from the step array (one value when nh == 1)
+// (plus - minus) / (2 h) of the double fields 0..F_CONN-1 of every record
+// for parameter k, stored as out[(c * F_CONN + f) * 4 + k]; h per cell from
+// the step array (one value when nh == 1)
 __global__ void fd_tangent(int64_t ncell, const double *__restrict__ plus,
                            const double *__restrict__ minus,
-                           const double *__restrict__ h, int64_t nh,
+                           const double *__restrict__ h, int64_t nh, int k,
                            double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncell * F_CONN) return;
   const int64_t c = i / F_CONN, f = i % F_CONN;
   const double hc = h[nh == 1 ? 0 : c];
-  out[i] = (plus[c * CELL_DOUBLES + f] - minus[c * CELL_DOUBLES + f]) / (2.0 * hc);
+  out[i * 4 + k] = (plus[c * CELL_DOUBLES + f] - minus[c * CELL_DOUBLES + f]) / (2.0 * hc);
 }
 
 }  // namespace
@@ -555,7 +556,7 @@ hipError_t build_record_tangents(rwpt_engine *e, double *out) {
                          (const double *)nullptr, rel, h);
     hipLaunchKernelGGL(fd_tangent, dim3(tgrid), dim3(block), 0, e->stream, P.ncell,
                        (const double *)plus, (const double *)minus, (const double *)h,
-                       k == 3 ? (int64_t)1 : P.ndisp, out + (size_t)k * P.ncell * F_CONN);
+                       k == 3 ? (int64_t)1 : P.ndisp, k, out);
     if ((err = hipGetLastError()) != hipSuccess) return release(), err;
   }
   err = hipStreamSynchronize(e->stream);

@@ -186,7 +186,7 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop);
 WalkParams walk_params(const rwpt_engine *e, double t_stop);
 // d(record)/d(parameter) of every cell, fields 0..F_CONN-1, for the
 // adjoint: 0 aLH, 1 aTH, 2 Dm (each cell's own value), 3 a common scale s
-// of all porosities (at s = 1); out holds 4 * ncell * F_CONN doubles
+// of all porosities (at s = 1); out[(cell * F_CONN + field) * 4 + param]
 hipError_t build_record_tangents(rwpt_engine *e, double *out);
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
                              double *gz);
