@@ -18,16 +18,19 @@
 // start (checkpoints).  Reverse, window by window from the last: the
 // window's walk is replayed from its checkpoint by a walk kernel that
 // records the input of every time-loop iteration (a tape) and must end
-// bit-identical to the forward state after the window (checked); then one
-// thread per active particle walks its tape backwards.  For each iteration
-// it re-runs the iteration twice in forward-mode dual numbers of four
-// directions, first the particle's x, y, z and t, then the cell's aLH, aTH,
-// Dm and the porosity scale, which gives the iteration's 4 x 8 Jacobian on
-// the branch the forward took, and applies its transpose to the adjoint of
-// the iteration's output.  Restarts to the visit start carry their adjoint to
-// the visit's first iteration.  The record's derivatives with respect to
-// the cell parameters come from central differences of the table build
-// (the records are linear in aLH, aTH, Dm and rational in the porosity).
+// bit-identical to the forward state after the window (checked); the tape
+// also holds each iteration's outcome (final step, jump-limiter factor,
+// exit axis and root, the normal draws).  Then one thread per active
+// particle walks its tape backwards and applies the transpose of each
+// iteration's Jacobian (state x, y, z, t and the cell's aLH, aTH, Dm and
+// porosity scale) to the adjoint of the iteration's output: by a
+// hand-written reverse step for the common iterations, and, for the rare
+// ones (restarts to the visit start, elastic rebounds), in a second pass
+// that re-runs the iteration in forward-mode dual numbers of four
+// directions twice.  Restarts carry their adjoint to the visit's first
+// iteration.  The record's derivatives with respect to the cell parameters
+// come from central differences of the table build (the records are linear
+// in aLH, aTH, Dm and rational in the porosity).
 //
 // Scope: FP64 engines, EULERIAN advection, the linear dispersion model,
 // 2-D and 3-D walks.
@@ -706,12 +709,21 @@ __device__ __forceinline__ double der4d(const double *d, double u, double w) {
   return (1.0 - w) * v0 + w * v1;
 }
 
-// Reverse of the trilinear interpolation of the corner values at base f:
-// adds lam * db/d(x, y, z) to lx and lam * weight to the parameters through
-// the corner values' tangents.
-__device__ __forceinline__ void tri8_rev(const double *rec, const double *tan, int f,
-                                         double x, double y, double z, double lam,
-                                         double lx[3], double g[4]) {
+// Record tangents of the linear dispersion model in closed form, checked
+// against the central differences of the table build by the tests: with
+// porosity scaled by s, the face and corner velocities and the cell
+// porosity scale as 1/s, s and s (d/ds at s = 1: -v, -v, phi); aLH, aTH and
+// Dm are record fields of their own (tangent 1); the drift terms are linear
+// in aLH, aTH and Dm, their Dm part carrying the corner porosity, so
+// d/ds = Dm d/dDm.  Only the drift terms' three tangents are stored
+// (dd3[cell][term][aLH, aTH, Dm]).
+
+// Reverse of the trilinear interpolation of corner values at base f (value
+// bval at the point): lam * db/d(x, y, z) to lx, and to the porosity scale
+// the corner values' tangent -b.
+__device__ __forceinline__ void tri8_rev(const double *rec, int f, double x, double y,
+                                         double z, double bval, double lam, double lx[3],
+                                         double g[4]) {
   const double *c = rec + f;
   const double v00 = (1.0 - x) * c[0] + x * c[1], v10 = (1.0 - x) * c[2] + x * c[3];
   const double v01 = (1.0 - x) * c[4] + x * c[5], v11 = (1.0 - x) * c[6] + x * c[7];
@@ -719,40 +731,29 @@ __device__ __forceinline__ void tri8_rev(const double *rec, const double *tan, i
                   z * ((1.0 - y) * (c[5] - c[4]) + y * (c[7] - c[6])));
   lx[1] += lam * ((1.0 - z) * (v10 - v00) + z * (v11 - v01));
   lx[2] += lam * (((1.0 - y) * v01 + y * v11) - ((1.0 - y) * v00 + y * v10));
-  const double w[8] = {(1.0 - x) * (1.0 - y) * (1.0 - z), x * (1.0 - y) * (1.0 - z),
-                       (1.0 - x) * y * (1.0 - z),         x * y * (1.0 - z),
-                       (1.0 - x) * (1.0 - y) * z,         x * (1.0 - y) * z,
-                       (1.0 - x) * y * z,                 x * y * z};
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double *tq = tan + (f + q) * 4;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] += w[q] * tq[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) g[k] += lam * acc[k];
+  g[3] += -lam * bval;
 }
 
-// Reverse of der4d at base f over the coordinates (iu, iw).
-__device__ __forceinline__ void der4_rev(const double *rec, const double *tan, int f,
-                                         const double p[3], int iu, int iw, double lam,
-                                         double lx[3], double g[4]) {
-  const double *d = rec + f;
+// Reverse of der4d for drift term j (record base F_DD + 4j) over the
+// coordinates (iu, iw); dd3 holds the cell's drift-term tangents.
+__device__ __forceinline__ void der4_rev(const double *rec, const double *dd3, int j,
+                                         const double p[3], int iu, int iw, double dm,
+                                         double lam, double lx[3], double g[4]) {
+  const double *d = rec + F_DD + 4 * j;
   const double u = p[iu], w = p[iw];
   lx[iu] += lam * ((1.0 - w) * (d[1] - d[0]) + w * (d[3] - d[2]));
   lx[iw] += lam * (((1.0 - u) * d[2] + u * d[3]) - ((1.0 - u) * d[0] + u * d[1]));
   const double wt[4] = {(1.0 - w) * (1.0 - u), (1.0 - w) * u, w * (1.0 - u), w * u};
-  const double *t0 = tan + f * 4;
+  const double *t = dd3 + 12 * j;  // [corner pair q][param k] for q < 4, k < 3
+  double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    g[k] += lam * (wt[0] * t0[k] + wt[1] * t0[4 + k] + wt[2] * t0[8 + k] + wt[3] * t0[12 + k]);
-}
-
-// lam times the parameter tangents of record field f
-__device__ __forceinline__ void field_rev(const double *tan, int f, double lam, double g[4]) {
+  for (int q = 0; q < 4; ++q)
 #pragma unroll
-  for (int k = 0; k < 4; ++k) g[k] += lam * tan[f * 4 + k];
+    for (int k = 0; k < 3; ++k) acc[k] += wt[q] * t[3 * q + k];
+  g[0] += lam * acc[0];
+  g[1] += lam * acc[1];
+  g[2] += lam * acc[2];
+  g[3] += lam * dm * acc[2];
 }
 
 template <int NDIM>
@@ -833,8 +834,9 @@ __device__ void disp_full(const double *rec, const WalkParams &W, const double p
 // Returns false (nothing changed) for an iteration on a rare branch; the
 // branch taken and the draws come from the tape.
 template <int NDIM>
-__device__ bool reverse_step(const double *rec, const double *tan, const WalkParams &W,
-                             const TapeEntry &E, const double l[4], double ln[4], double g[4]) {
+__device__ bool reverse_step(const double *rec, const double *tan, const double *dd3,
+                             const WalkParams &W, const TapeEntry &E, const double l[4],
+                             double ln[4], double g[4]) {
   const int32_t fl = E.flags;
   const int out = (fl >> 14) & 7;
   if ((fl & (1 << 11)) || out == IT_UNDEFINED) return false;
@@ -882,7 +884,12 @@ __device__ bool reverse_step(const double *rec, const double *tan, const WalkPar
   } else if (!clipped) {
     // dt = c dt_cell, t_out = t + dt
     lt += lt_out;
-    field_rev(tan, F_DTCELL, E.c * (ldt + lt_out), g);
+    {
+      // the cell step's tangents (zero for FIXED steps) from the full table
+      const double lc = E.c * (ldt + lt_out);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) g[k] += lc * tan[F_DTCELL * 4 + k];
+    }
   } else if (E.c == 1.0) {
     // dt = t_stop - t, t_out = t_stop
     lt += -ldt;
@@ -894,27 +901,25 @@ __device__ bool reverse_step(const double *rec, const double *tan, const WalkPar
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     lx[a] += lv[a] * (D.V2[a] - D.V1[a]);
-    field_rev(tan, F_VX1 + 2 * a, lv[a] * (1.0 - p[a]), g);
-    field_rev(tan, F_VX2 + 2 * a, lv[a] * p[a], g);
+    g[3] += -lv[a] * D.v[a];  // dv/ds = -v
   }
   // drift: dd_a = S_a / phi / R
   {
     const double sphi = ldd[0] / D.phi / D.R, sphj = ldd[1] / D.phi / D.R,
                  sphk = ldd[2] / D.phi / D.R;
-    field_rev(tan, F_PHI,
-              -(ldd[0] * D.dd[0] + ldd[1] * D.dd[1] + ldd[2] * D.dd[2]) / D.phi, g);
+    g[3] += -(ldd[0] * D.dd[0] + ldd[1] * D.dd[1] + ldd[2] * D.dd[2]);  // d(1/phi)/ds
     // term k at F_DD + 4k over its coordinates: t0 yz, t1 xz, t2 xy, t3 yz,
     // t4 yz, t5 xz, t6 xz, t7 xy, t8 xy; dd_x = t0+t5+t7, dd_y = t3+t1+t8,
     // dd_z = t4+t6+t2
-    der4_rev(rec, tan, F_DD + 0, p, 1, 2, sphi, lx, g);
-    der4_rev(rec, tan, F_DD + 20, p, 0, 2, sphi, lx, g);
-    der4_rev(rec, tan, F_DD + 28, p, 0, 1, sphi, lx, g);
-    der4_rev(rec, tan, F_DD + 12, p, 1, 2, sphj, lx, g);
-    der4_rev(rec, tan, F_DD + 4, p, 0, 2, sphj, lx, g);
-    der4_rev(rec, tan, F_DD + 32, p, 0, 1, sphj, lx, g);
-    der4_rev(rec, tan, F_DD + 16, p, 1, 2, sphk, lx, g);
-    der4_rev(rec, tan, F_DD + 24, p, 0, 2, sphk, lx, g);
-    der4_rev(rec, tan, F_DD + 8, p, 0, 1, sphk, lx, g);
+    der4_rev(rec, dd3, 0, p, 1, 2, D.dm, sphi, lx, g);
+    der4_rev(rec, dd3, 5, p, 0, 2, D.dm, sphi, lx, g);
+    der4_rev(rec, dd3, 7, p, 0, 1, D.dm, sphi, lx, g);
+    der4_rev(rec, dd3, 3, p, 1, 2, D.dm, sphj, lx, g);
+    der4_rev(rec, dd3, 1, p, 0, 2, D.dm, sphj, lx, g);
+    der4_rev(rec, dd3, 8, p, 0, 1, D.dm, sphj, lx, g);
+    der4_rev(rec, dd3, 4, p, 1, 2, D.dm, sphk, lx, g);
+    der4_rev(rec, dd3, 6, p, 0, 2, D.dm, sphk, lx, g);
+    der4_rev(rec, dd3, 2, p, 0, 1, D.dm, sphk, lx, g);
   }
   // dispersion matrix
   double lb[3] = {0.0, 0.0, 0.0}, lvn = 0.0, lvxy = 0.0, lsl = 0.0, lst = 0.0, lsh = 0.0;
@@ -972,9 +977,9 @@ __device__ bool reverse_step(const double *rec, const double *tan, const WalkPar
       lvn += f * D.aT;
       ldm += f;
     }
-    field_rev(tan, F_ALH, laL, g);
-    field_rev(tan, F_ATH, laT, g);
-    field_rev(tan, F_DMEFF, ldm, g);
+    g[0] += laL;
+    g[1] += laT;
+    g[2] += ldm;
     // norms
     lb[0] += lvn * D.b[0] * ivn_safe(D.vn);
     lb[1] += lvn * D.b[1] * ivn_safe(D.vn);
@@ -987,10 +992,10 @@ __device__ bool reverse_step(const double *rec, const double *tan, const WalkPar
     if constexpr (NDIM == 3) {
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        tri8_rev(rec, tan, F_VC + 8 * a, p[0], p[1], p[2], lb[a], lx, g);
+        tri8_rev(rec, F_VC + 8 * a, p[0], p[1], p[2], D.b[a], lb[a], lx, g);
     } else {
-      tri8_rev(rec, tan, F_VC + 8 * (W.dims[0] - 1), p[0], p[1], p[2], lb[0], lx, g);
-      tri8_rev(rec, tan, F_VC + 8 * (W.dims[1] - 1), p[0], p[1], p[2], lb[1], lx, g);
+      tri8_rev(rec, F_VC + 8 * (W.dims[0] - 1), p[0], p[1], p[2], D.b[0], lb[0], lx, g);
+      tri8_rev(rec, F_VC + 8 * (W.dims[1] - 1), p[0], p[1], p[2], D.b[1], lb[1], lx, g);
     }
   }
   ln[0] = lx[0];
@@ -1162,7 +1167,7 @@ __device__ __forceinline__ void flush(int c, const double g[4], double *g_al, do
 // dual-number Jacobians.  Returns -1 when the window is done.
 template <int NDIM, bool LEAN>
 __device__ int64_t sweep(const double *__restrict__ cells, const double *__restrict__ tan,
-                         int64_t ncell, const WalkParams &W, int64_t first, int64_t from,
+                         const double *__restrict__ dd3, int64_t ncell, const WalkParams &W, int64_t first, int64_t from,
                          const TapeEntry *__restrict__ tape, double l[4], double mu[4],
                          double *g_al, double *g_at, double *g_dm, double &gs) {
   using D = Dual<4>;
@@ -1183,7 +1188,7 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
       const double *crec = cells + c0 * CELL_DOUBLES, *ctan = tan + c0 * kTan * 4;
       double ln[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {l[0], l[1], l[2], l[3]};
       int restart = 0;
-      if (!reverse_step<NDIM>(crec, ctan, W, E, l, ln, g)) {
+      if (!reverse_step<NDIM>(crec, ctan, dd3 + c0 * 108, W, E, l, ln, g)) {
         if constexpr (LEAN) {
           stop = k;
           break;
@@ -1260,7 +1265,8 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
 // finishes them.
 template <int NDIM>
 __global__ void __launch_bounds__(kBlk) adj_backward(
-    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
+    const double *__restrict__ cells, const double *__restrict__ tan,
+    const double *__restrict__ dd3, int64_t ncell,
     int64_t nsel, const WalkParams W, const int64_t *__restrict__ off,
     const TapeEntry *__restrict__ tape, double *__restrict__ lam, double *__restrict__ mus,
     int64_t *__restrict__ resume, int32_t *__restrict__ deferred,
@@ -1271,7 +1277,7 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
   if (i < nsel) {
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t k = sweep<NDIM, true>(cells, tan, ncell, W, off[i], off[i + 1] - 1, tape, l,
+    const int64_t k = sweep<NDIM, true>(cells, tan, dd3, ncell, W, off[i], off[i + 1] - 1, tape, l,
                                         mu, g_al, g_at, g_dm, gs);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1287,7 +1293,8 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
 
 template <int NDIM>
 __global__ void __launch_bounds__(kBlk) adj_backward_full(
-    const double *__restrict__ cells, const double *__restrict__ tan, int64_t ncell,
+    const double *__restrict__ cells, const double *__restrict__ tan,
+    const double *__restrict__ dd3, int64_t ncell,
     const WalkParams W, const int64_t *__restrict__ off, const TapeEntry *__restrict__ tape,
     double *__restrict__ lam, const double *__restrict__ mus,
     const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred, int64_t ndef,
@@ -1298,7 +1305,7 @@ __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const int64_t i = deferred[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {mus[4 * i], mus[4 * i + 1], mus[4 * i + 2], mus[4 * i + 3]};
-    (void)sweep<NDIM, false>(cells, tan, ncell, W, off[i], resume[i], tape, l, mu, g_al, g_at,
+    (void)sweep<NDIM, false>(cells, tan, dd3, ncell, W, off[i], resume[i], tape, l, mu, g_al, g_at,
                              g_dm, gs);
 #pragma unroll
     for (int q = 0; q < 4; ++q) lam[4 * i + q] = l[q];
@@ -1378,6 +1385,17 @@ __global__ void adj_seed(const int32_t *__restrict__ sel, int64_t nsel,
   lam[4 * i + 1] = gX[1] * wy;
   lam[4 * i + 2] = gX[2] * wz;
   lam[4 * i + 3] = 0.0;
+}
+
+// The drift terms' tangents for aLH, aTH and Dm in the reverse step's
+// layout: dd3[cell][term j][corner pair q][param k] from the full table.
+__global__ void adj_dd3(int64_t ncell, const double *__restrict__ tan,
+                        double *__restrict__ dd3) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncell * 108) return;
+  const int64_t c = i / 108;
+  const int r = (int)(i % 108), j = r / 12, q = (r % 12) / 3, k = r % 3;
+  dd3[i] = tan[(c * kTan + F_DD + 4 * j + q) * 4 + k];
 }
 
 __global__ void copy_ck(int64_t n, const int32_t *__restrict__ cell,
@@ -1575,6 +1593,12 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     return adj_hip(e, err, "adjoint set-up");
   if ((err = build_record_tangents(e, tan)) != hipSuccess)
     return adj_hip(e, err, "record tangents");
+  double *dd3;
+  if ((err = B.alloc(&dd3, (size_t)ncell * 108)) != hipSuccess)
+    return adj_hip(e, err, "adjoint memory");
+  hipLaunchKernelGGL(adj_dd3, dim3((unsigned)((ncell * 108 + blk - 1) / blk)), dim3(blk), 0, st,
+                     ncell, (const double *)tan, dd3);
+  if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "record tangents");
   const dim3 sgrid((unsigned)((std::max<int64_t>(nsel, 1) + kBlk - 1) / kBlk));
   if (nsel > 0) {
     hipLaunchKernelGGL(adj_seed, sgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nsel,
@@ -1648,14 +1672,14 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
            *gd = grad_dm ? g_dm : nullptr;
     if (ndim == 3)
       hipLaunchKernelGGL(adj_backward<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, ncell, nsel, W, (const int64_t *)off,
-                         (const TapeEntry *)tape, lam, mus, resume, deferred, ndef, ga, gt, gd,
-                         g_s);
+                         (const double *)tan, (const double *)dd3, ncell, nsel, W,
+                         (const int64_t *)off, (const TapeEntry *)tape, lam, mus, resume,
+                         deferred, ndef, ga, gt, gd, g_s);
     else
       hipLaunchKernelGGL(adj_backward<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, ncell, nsel, W, (const int64_t *)off,
-                         (const TapeEntry *)tape, lam, mus, resume, deferred, ndef, ga, gt, gd,
-                         g_s);
+                         (const double *)tan, (const double *)dd3, ncell, nsel, W,
+                         (const int64_t *)off, (const TapeEntry *)tape, lam, mus, resume,
+                         deferred, ndef, ga, gt, gd, g_s);
     unsigned long long nd = 0;
     if ((err = hipGetLastError()) != hipSuccess ||
         (err = hipMemcpyAsync(&nd, ndef, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
@@ -1665,13 +1689,15 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       const dim3 dgrid((unsigned)((nd + kBlk - 1) / kBlk));
       if (ndim == 3)
         hipLaunchKernelGGL(adj_backward_full<3>, dgrid, dim3(kBlk), 0, st,
-                           (const double *)G.cells, (const double *)tan, ncell, W,
+                           (const double *)G.cells, (const double *)tan,
+                           (const double *)dd3, ncell, W,
                            (const int64_t *)off, (const TapeEntry *)tape, lam,
                            (const double *)mus, (const int64_t *)resume,
                            (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
       else
         hipLaunchKernelGGL(adj_backward_full<2>, dgrid, dim3(kBlk), 0, st,
-                           (const double *)G.cells, (const double *)tan, ncell, W,
+                           (const double *)G.cells, (const double *)tan,
+                           (const double *)dd3, ncell, W,
                            (const int64_t *)off, (const TapeEntry *)tape, lam,
                            (const double *)mus, (const int64_t *)resume,
                            (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
