@@ -709,8 +709,9 @@ __device__ __forceinline__ double der4d(const double *d, double u, double w) {
   return (1.0 - w) * v0 + w * v1;
 }
 
-// Record tangents of the linear dispersion model in closed form, checked
-// against the central differences of the table build by the tests: with
+// Record tangents of the linear dispersion model in closed form (the
+// gradient they give is checked against central differences of the misfit
+// by the tests): with
 // porosity scaled by s, the face and corner velocities and the cell
 // porosity scale as 1/s, s and s (d/ds at s = 1: -v, -v, phi); aLH, aTH and
 // Dm are record fields of their own (tangent 1); the drift terms are linear
