@@ -647,6 +647,72 @@ struct TapeEntry {
                   // axis, 14-16 result code, 17-19 exitFace
 };
 
+// The tape of one window is laid out by wavefront: the wavefront of
+// threads 64w .. 64w+63 owns a block of len_w entries per lane (len_w its
+// lanes' largest iteration count), entry k of the block stored field by
+// field, each field as 64 consecutive words, one per lane.  A lane's column
+// is block start + lane; its entry k begins k * kTapeWords * 64 words
+// further on.  Every tape store and load of a wavefront then covers
+// consecutive words.
+constexpr int kTapeWords = 17;
+static_assert(sizeof(TapeEntry) == kTapeWords * 8, "tape entry is 17 words");
+
+__device__ __forceinline__ uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ double bitsd(uint64_t v) { return __longlong_as_double((long long)v); }
+
+__device__ __forceinline__ void tape_put(uint64_t *__restrict__ tape, int64_t col, int64_t k,
+                                         const TapeEntry &E) {
+  uint64_t *q = tape + col + k * (int64_t)(kTapeWords * kBlk);
+  q[0 * kBlk] = dbits(E.x);
+  q[1 * kBlk] = dbits(E.y);
+  q[2 * kBlk] = dbits(E.z);
+  q[3 * kBlk] = dbits(E.t);
+  q[4 * kBlk] = dbits(E.ix);
+  q[5 * kBlk] = dbits(E.iy);
+  q[6 * kBlk] = dbits(E.iz);
+  q[7 * kBlk] = dbits(E.t0);
+  q[8 * kBlk] = E.ra;
+  q[9 * kBlk] = E.rb;
+  q[10 * kBlk] = dbits(E.dt);
+  q[11 * kBlk] = dbits(E.c);
+  q[12 * kBlk] = dbits(E.m);
+  q[13 * kBlk] = dbits(E.r[0]);
+  q[14 * kBlk] = dbits(E.r[1]);
+  q[15 * kBlk] = dbits(E.r[2]);
+  q[16 * kBlk] = (uint64_t)(uint32_t)E.cell | ((uint64_t)(uint32_t)E.flags << 32);
+}
+
+__device__ __forceinline__ TapeEntry tape_get(const uint64_t *__restrict__ tape, int64_t col,
+                                              int64_t k) {
+  const uint64_t *q = tape + col + k * (int64_t)(kTapeWords * kBlk);
+  TapeEntry E;
+  E.x = bitsd(q[0 * kBlk]);
+  E.y = bitsd(q[1 * kBlk]);
+  E.z = bitsd(q[2 * kBlk]);
+  E.t = bitsd(q[3 * kBlk]);
+  E.ix = bitsd(q[4 * kBlk]);
+  E.iy = bitsd(q[5 * kBlk]);
+  E.iz = bitsd(q[6 * kBlk]);
+  E.t0 = bitsd(q[7 * kBlk]);
+  E.ra = q[8 * kBlk];
+  E.rb = q[9 * kBlk];
+  E.dt = bitsd(q[10 * kBlk]);
+  E.c = bitsd(q[11 * kBlk]);
+  E.m = bitsd(q[12 * kBlk]);
+  E.r[0] = bitsd(q[13 * kBlk]);
+  E.r[1] = bitsd(q[14 * kBlk]);
+  E.r[2] = bitsd(q[15 * kBlk]);
+  const uint64_t w = q[16 * kBlk];
+  E.cell = (int32_t)(uint32_t)w;
+  E.flags = (int32_t)(uint32_t)(w >> 32);
+  return E;
+}
+
+// Column of thread j of a window's launch, from the block starts (entries).
+__device__ __forceinline__ int64_t tape_col(const int64_t *__restrict__ wstart, int64_t j) {
+  return wstart[j / kBlk] * (int64_t)(kTapeWords * kBlk) + j % kBlk;
+}
+
 // Particle arrays of a checkpoint (or of the engine after the window).
 struct CkPtrs {
   int32_t *cell, *status;
@@ -1010,14 +1076,17 @@ __device__ bool reverse_step(const double *rec, const double *tan, const double 
 // recording the tape; the end state must equal `want` bit for bit (the
 // listed particle count of mismatches goes to *bad).  The start of an
 // advance (prepare_advance) and the visit loop (walk_kernel) without launch
-// budget: one thread walks its particle to the end of the window.
+// budget: one thread walks its particle to the end of the window, thread j
+// taking listed particle ord[j].
 template <int NDIM>
 __global__ void __launch_bounds__(kBlk) adj_replay(
     const double *__restrict__ cells, const int32_t *__restrict__ sel, int64_t nsel,
-    CkPtrs in, CkPtrs want, const WalkParams W, const int64_t *__restrict__ off,
-    TapeEntry *__restrict__ tape, unsigned long long *__restrict__ bad) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nsel) return;
+    const int32_t *__restrict__ ord, CkPtrs in, CkPtrs want, const WalkParams W,
+    const int64_t *__restrict__ cnt, const int64_t *__restrict__ wstart,
+    uint64_t *__restrict__ tape, unsigned long long *__restrict__ bad) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsel) return;
+  const int64_t i = ord[j];
   const int p = sel[i];
   int st = in.status[p];
   int c = in.cell[p];
@@ -1037,7 +1106,7 @@ __global__ void __launch_bounds__(kBlk) adj_replay(
   if (st == 7 && (ints(cells + (int64_t)(c - 1) * CELL_DOUBLES)[Lay<double>::FLAGS + 1] &
                   CF_VERIFY_PDRY))
     st = 1;
-  const int64_t base = off[i], cap = off[i + 1] - off[i];
+  const int64_t col = tape_col(wstart, j), cap = cnt[i];
   int64_t k = 0;
   int status = st;
   bool overflow = false;
@@ -1085,7 +1154,7 @@ __global__ void __launch_bounds__(kBlk) adj_replay(
             (ob.odd || restart != 0 ? 1 << 11 : 0) | ((ob.im < 0 ? 0 : ob.im) << 12) |
             (out << 14) | (V.exitFace << 17);
       E.flags = fl;
-      tape[base + k] = E;
+      tape_put(tape, col, k, E);
       k += 1;
       if (out == IT_CONTINUE) continue;
       if (out == IT_MAXTIME) {
@@ -1140,6 +1209,40 @@ __global__ void __launch_bounds__(kBlk) adj_replay(
   if (!ok) atomicAdd(bad, 1ull);
 }
 
+// Sort keys of the listed particles for one window: the particle's
+// iteration count in the window (clamped to what fits above the cell
+// bits), then the cell it starts the window in.  A thread replays or
+// reverses its particle's whole window, so a wavefront lasts as long as
+// its longest lane: equal counts keep the lanes busy together, and equal
+// cells let them share cell records.
+__global__ void adj_order_keys(const int32_t *__restrict__ sel, int64_t nsel,
+                               const int32_t *__restrict__ cell,
+                               const int64_t *__restrict__ cnt, int cell_bits,
+                               int32_t *__restrict__ key, int32_t *__restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const int64_t cmax = (int64_t(1) << (31 - cell_bits)) - 1;
+  const int64_t c = cnt[i] < cmax ? cnt[i] : cmax;
+  key[i] = (int32_t)((c << cell_bits) | (int64_t)cell[sel[i]]);
+  idx[i] = (int32_t)i;
+}
+
+// Tape block length of each wavefront of a window's launch: the largest
+// iteration count among its lanes (threads 64w .. 64w+63, particle ord[j]);
+// entry nwaves is 0 so that an exclusive scan also gives the total.
+__global__ void adj_wave_len(const int32_t *__restrict__ ord, int64_t nsel,
+                             const int64_t *__restrict__ cnt, int64_t *__restrict__ wlen) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = (nsel + kBlk - 1) / kBlk;
+  if (w > nw) return;
+  int64_t m = 0;
+  if (w < nw) {
+    const int64_t e = (w + 1) * kBlk < nsel ? (w + 1) * kBlk : nsel;
+    for (int64_t j = w * kBlk; j < e; ++j) m = cnt[ord[j]] > m ? cnt[ord[j]] : m;
+  }
+  wlen[w] = m;
+}
+
 // Iterations of each listed particle in the window, from the sub-step
 // counters before and after it.
 __global__ void adj_counts(const int32_t *__restrict__ sel, int64_t nsel,
@@ -1160,16 +1263,17 @@ __device__ __forceinline__ void flush(int c, const double g[4], double *g_al, do
   g_s += g[3];
 }
 
-// Reverse sweep of one particle's tape from entry `from` down to the
-// window's first entry, l and mu the adjoints of the state and of the visit
+// Reverse sweep of the tape column `col` from entry `from` down to the
+// window's first entry (0), l and mu the adjoints of the state and of the visit
 // start.  LEAN: the hand-written reverse step only; at an iteration it does
 // not cover the sweep stops and returns that entry's index (l and mu then
 // hold the state to resume from).  Otherwise rare iterations use
 // dual-number Jacobians.  Returns -1 when the window is done.
 template <int NDIM, bool LEAN>
 __device__ int64_t sweep(const double *__restrict__ cells, const double *__restrict__ tan,
-                         const double *__restrict__ dd3, int64_t ncell, const WalkParams &W, int64_t first, int64_t from,
-                         const TapeEntry *__restrict__ tape, double l[4], double mu[4],
+                         const double *__restrict__ dd3, int64_t ncell, const WalkParams &W,
+                         int64_t col, int64_t from, const uint64_t *__restrict__ tape,
+                         double l[4], double mu[4],
                          double *g_al, double *g_at, double *g_dm, double &gs) {
   using D = Dual<4>;
   double g[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1178,8 +1282,8 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
                    mu[1] != 0.0 || mu[2] != 0.0 || mu[3] != 0.0;
   int64_t stop = -1;
   {
-    for (int64_t k = from; any && k >= first; --k) {
-      const TapeEntry E = tape[k];
+    for (int64_t k = from; any && k >= 0; --k) {
+      const TapeEntry E = tape_get(tape, col, k);
       if (E.cell != cur) {
         flush(cur, g, g_al, g_at, g_dm, gs);
         g[0] = g[1] = g[2] = g[3] = 0.0;
@@ -1262,31 +1366,34 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
 // Reverse sweep of one window: lam (4 per listed particle: x y z t) holds
 // the adjoint of the state after the window on entry and of the state
 // before it on return.  The lean pass leaves the particles that reach a
-// rare iteration in `deferred` with their resume index; the full pass
-// finishes them.
+// rare iteration in `deferred` (as thread indices j) with their resume
+// entry and visit-start adjoint (resume[j], mus[4j..]); the full pass
+// finishes them.  Thread j reverses listed particle ord[j].
 template <int NDIM>
 __global__ void __launch_bounds__(kBlk) adj_backward(
     const double *__restrict__ cells, const double *__restrict__ tan,
     const double *__restrict__ dd3, int64_t ncell,
-    int64_t nsel, const WalkParams W, const int64_t *__restrict__ off,
-    const TapeEntry *__restrict__ tape, double *__restrict__ lam, double *__restrict__ mus,
-    int64_t *__restrict__ resume, int32_t *__restrict__ deferred,
-    unsigned long long *__restrict__ ndeferred, double *g_al, double *g_at, double *g_dm,
-    double *g_s) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nsel, const int32_t *__restrict__ ord, const WalkParams W,
+    const int64_t *__restrict__ cnt, const int64_t *__restrict__ wstart,
+    const uint64_t *__restrict__ tape,
+    double *__restrict__ lam, double *__restrict__ mus, int64_t *__restrict__ resume,
+    int32_t *__restrict__ deferred, unsigned long long *__restrict__ ndeferred, double *g_al,
+    double *g_at, double *g_dm, double *g_s) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double gs = 0.0;
-  if (i < nsel) {
+  if (j < nsel) {
+    const int64_t i = ord[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t k = sweep<NDIM, true>(cells, tan, dd3, ncell, W, off[i], off[i + 1] - 1, tape, l,
-                                        mu, g_al, g_at, g_dm, gs);
+    const int64_t k = sweep<NDIM, true>(cells, tan, dd3, ncell, W, tape_col(wstart, j),
+                                        cnt[i] - 1, tape, l, mu, g_al, g_at, g_dm, gs);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       lam[4 * i + q] = l[q];
-      mus[4 * i + q] = mu[q];
+      mus[4 * j + q] = mu[q];
     }
-    resume[i] = k;
-    if (k >= 0) deferred[atomicAdd(ndeferred, 1ull)] = (int32_t)i;
+    resume[j] = k;
+    if (k >= 0) deferred[atomicAdd(ndeferred, 1ull)] = (int32_t)j;
   }
   for (int o = 32; o > 0; o >>= 1) gs += __shfl_down(gs, o, 64);
   if (__lane_id() == 0 && gs != 0.0) unsafeAtomicAdd(g_s, gs);
@@ -1296,18 +1403,18 @@ template <int NDIM>
 __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const double *__restrict__ cells, const double *__restrict__ tan,
     const double *__restrict__ dd3, int64_t ncell,
-    const WalkParams W, const int64_t *__restrict__ off, const TapeEntry *__restrict__ tape,
-    double *__restrict__ lam, const double *__restrict__ mus,
+    const WalkParams W, const int32_t *__restrict__ ord, const int64_t *__restrict__ wstart,
+    const uint64_t *__restrict__ tape, double *__restrict__ lam, const double *__restrict__ mus,
     const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred, int64_t ndef,
     double *g_al, double *g_at, double *g_dm, double *g_s) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double gs = 0.0;
-  if (j < ndef) {
-    const int64_t i = deferred[j];
+  if (t < ndef) {
+    const int64_t j = deferred[t], i = ord[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
-    double mu[4] = {mus[4 * i], mus[4 * i + 1], mus[4 * i + 2], mus[4 * i + 3]};
-    (void)sweep<NDIM, false>(cells, tan, dd3, ncell, W, off[i], resume[i], tape, l, mu, g_al, g_at,
-                             g_dm, gs);
+    double mu[4] = {mus[4 * j], mus[4 * j + 1], mus[4 * j + 2], mus[4 * j + 3]};
+    (void)sweep<NDIM, false>(cells, tan, dd3, ncell, W, tape_col(wstart, j), resume[j], tape, l,
+                             mu, g_al, g_at, g_dm, gs);
 #pragma unroll
     for (int q = 0; q < 4; ++q) lam[4 * i + q] = l[q];
   }
@@ -1569,7 +1676,8 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   const int64_t nsel = (int64_t)hsel.size();
   int32_t *sel;
   double *lam, *tan, *resid_d, *g_al, *g_at, *g_dm, *g_s;
-  int64_t *cnt, *off, *resume;
+  int64_t *cnt, *wlen, *wstart, *resume;
+  const int64_t nwaves = (nsel + kBlk - 1) / kBlk;
   double *mus;
   int32_t *deferred;
   unsigned long long *bad, *ndef;
@@ -1579,7 +1687,8 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       (err = B.alloc(&resid_d, n_obs)) != hipSuccess || (err = B.alloc(&g_al, ncell)) != hipSuccess ||
       (err = B.alloc(&g_at, ncell)) != hipSuccess || (err = B.alloc(&g_dm, ncell)) != hipSuccess ||
       (err = B.alloc(&g_s, 1)) != hipSuccess || (err = B.alloc(&cnt, nsel + 1)) != hipSuccess ||
-      (err = B.alloc(&off, nsel + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess ||
+      (err = B.alloc(&wlen, nwaves + 1)) != hipSuccess ||
+      (err = B.alloc(&wstart, nwaves + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess ||
       (err = B.alloc(&resume, nsel)) != hipSuccess || (err = B.alloc(&mus, 4 * nsel)) != hipSuccess ||
       (err = B.alloc(&deferred, nsel)) != hipSuccess || (err = B.alloc(&ndef, 1)) != hipSuccess)
     return adj_hip(e, err, "adjoint memory");
@@ -1611,15 +1720,28 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
                        lam);
     if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "seed");
   }
-  TapeEntry *tape = nullptr;
-  size_t tape_cap = 0;
+  uint64_t *tape = nullptr;
+  size_t tape_cap = 0;  // words
   char *scan_tmp = nullptr;
   size_t scan_bytes = 0;
-  if ((err = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, off, (int)(nsel + 1),
-                                              st)) != hipSuccess)
+  if ((err = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, wlen, wstart,
+                                              (int)(nwaves + 1), st)) != hipSuccess)
     return adj_hip(e, err, "scan");
   if ((err = B.alloc(&scan_tmp, scan_bytes)) != hipSuccess)
     return adj_hip(e, err, "scan memory");
+  // per-window order of the listed particles (iteration count, then cell)
+  int32_t *okey, *okey2, *oidx, *ord;
+  char *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  int cell_bits = 1;
+  while (cell_bits < 24 && (int64_t(1) << cell_bits) <= ncell) ++cell_bits;
+  if ((err = B.alloc(&okey, nsel)) != hipSuccess || (err = B.alloc(&okey2, nsel)) != hipSuccess ||
+      (err = B.alloc(&oidx, nsel)) != hipSuccess || (err = B.alloc(&ord, nsel)) != hipSuccess ||
+      (err = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, okey, okey2, oidx,
+                                                          ord, (int)nsel, 0, 31, st)) !=
+          hipSuccess ||
+      (err = B.alloc(&sort_tmp, sort_bytes)) != hipSuccess)
+    return adj_hip(e, err, "order memory");
   const CkPtrs now{P.cell, P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps};
   for (int w = n_windows - 1; w >= 0 && nsel > 0; --w) {
     const CkPtrs &in = ck[w];
@@ -1627,32 +1749,45 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nsel + 1 + blk - 1) / blk)), dim3(blk), 0,
                        st, (const int32_t *)sel, nsel, (const int64_t *)in.substeps,
                        (const int64_t *)want.substeps, cnt);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "tape offsets");
+    hipLaunchKernelGGL(adj_order_keys, sgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nsel,
+                       (const int32_t *)in.cell, (const int64_t *)cnt, cell_bits, okey, oidx);
+    size_t sbytes = sort_bytes;
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sbytes, okey, okey2, oidx,
+                                                            ord, (int)nsel, 0, 31, st)) !=
+            hipSuccess)
+      return adj_hip(e, err, "particle order");
+    hipLaunchKernelGGL(adj_wave_len, dim3((unsigned)((nwaves + 1 + blk - 1) / blk)), dim3(blk),
+                       0, st, (const int32_t *)ord, nsel, (const int64_t *)cnt, wlen);
     size_t bytes = scan_bytes;
     if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, cnt, off, (int)(nsel + 1),
+        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wlen, wstart, (int)(nwaves + 1),
                                                 st)) != hipSuccess)
       return adj_hip(e, err, "tape offsets");
     int64_t total = 0;
-    if ((err = hipMemcpyAsync(&total, off + nsel, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+    if ((err = hipMemcpyAsync(&total, wstart + nwaves, 8, hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
         (err = hipStreamSynchronize(st)) != hipSuccess)
       return adj_hip(e, err, "tape size");
-    if ((size_t)total > tape_cap) {
+    const size_t words = (size_t)total * kTapeWords * kBlk;
+    if (words > tape_cap) {
       if (tape) (void)hipFree(tape);
       B.ptrs.erase(std::remove(B.ptrs.begin(), B.ptrs.end(), (void *)tape), B.ptrs.end());
       tape = nullptr;
-      tape_cap = std::max<size_t>((size_t)total, tape_cap + tape_cap / 4);
+      tape_cap = std::max<size_t>(words, tape_cap + tape_cap / 4);
       if ((err = B.alloc(&tape, tape_cap)) != hipSuccess) return adj_hip(e, err, "tape memory");
     }
     if ((err = hipMemsetAsync(bad, 0, 8, st)) != hipSuccess) return adj_hip(e, err, "replay");
     const WalkParams &W = Ws[w];
     if (ndim == 3)
       hipLaunchKernelGGL(adj_replay<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nsel, in, want, W, (const int64_t *)off, tape,
-                         bad);
+                         (const int32_t *)sel, nsel, (const int32_t *)ord, in, want, W,
+                         (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
     else
       hipLaunchKernelGGL(adj_replay<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nsel, in, want, W, (const int64_t *)off, tape,
-                         bad);
+                         (const int32_t *)sel, nsel, (const int32_t *)ord, in, want, W,
+                         (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
     unsigned long long nbad = 0;
     if ((err = hipGetLastError()) != hipSuccess ||
         (err = hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
@@ -1673,13 +1808,15 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
            *gd = grad_dm ? g_dm : nullptr;
     if (ndim == 3)
       hipLaunchKernelGGL(adj_backward<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nsel, W,
-                         (const int64_t *)off, (const TapeEntry *)tape, lam, mus, resume,
+                         (const double *)tan, (const double *)dd3, ncell, nsel,
+                         (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
+                         (const uint64_t *)tape, lam, mus, resume,
                          deferred, ndef, ga, gt, gd, g_s);
     else
       hipLaunchKernelGGL(adj_backward<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nsel, W,
-                         (const int64_t *)off, (const TapeEntry *)tape, lam, mus, resume,
+                         (const double *)tan, (const double *)dd3, ncell, nsel,
+                         (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
+                         (const uint64_t *)tape, lam, mus, resume,
                          deferred, ndef, ga, gt, gd, g_s);
     unsigned long long nd = 0;
     if ((err = hipGetLastError()) != hipSuccess ||
@@ -1692,14 +1829,16 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
         hipLaunchKernelGGL(adj_backward_full<3>, dgrid, dim3(kBlk), 0, st,
                            (const double *)G.cells, (const double *)tan,
                            (const double *)dd3, ncell, W,
-                           (const int64_t *)off, (const TapeEntry *)tape, lam,
+                           (const int32_t *)ord, (const int64_t *)wstart,
+                           (const uint64_t *)tape, lam,
                            (const double *)mus, (const int64_t *)resume,
                            (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
       else
         hipLaunchKernelGGL(adj_backward_full<2>, dgrid, dim3(kBlk), 0, st,
                            (const double *)G.cells, (const double *)tan,
                            (const double *)dd3, ncell, W,
-                           (const int64_t *)off, (const TapeEntry *)tape, lam,
+                           (const int32_t *)ord, (const int64_t *)wstart,
+                           (const uint64_t *)tape, lam,
                            (const double *)mus, (const int64_t *)resume,
                            (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
     }
