@@ -1227,6 +1227,26 @@ __global__ void adj_order_keys(const int32_t *__restrict__ sel, int64_t nsel,
   idx[i] = (int32_t)i;
 }
 
+// The listed particles whose adjoint is not zero, with their adjoints
+// (order not kept).  An adjoint that is exactly zero at the final time (a
+// particle beyond the reach of every observation kernel, where the
+// Gaussian underflows) stays zero through every window, so such particles
+// need neither replay nor reverse sweep.
+__global__ void adj_keep(const int32_t *__restrict__ sel, const double *__restrict__ lam,
+                         int64_t nsel, int32_t *__restrict__ sel2, double *__restrict__ lam2,
+                         unsigned long long *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const double l0 = lam[4 * i], l1 = lam[4 * i + 1], l2 = lam[4 * i + 2], l3 = lam[4 * i + 3];
+  if (l0 == 0.0 && l1 == 0.0 && l2 == 0.0 && l3 == 0.0) return;
+  const unsigned long long q = atomicAdd(count, 1ull);
+  sel2[q] = sel[i];
+  lam2[4 * q] = l0;
+  lam2[4 * q + 1] = l1;
+  lam2[4 * q + 2] = l2;
+  lam2[4 * q + 3] = l3;
+}
+
 // Tape block length of each wavefront of a window's launch: the largest
 // iteration count among its lanes (threads 64w .. 64w+63, particle ord[j]);
 // entry nwaves is 0 so that an exclusive scan also gives the total.
@@ -1720,6 +1740,28 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
                        lam);
     if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "seed");
   }
+  int64_t nlive = 0;
+  if (nsel > 0) {
+    int32_t *sel2;
+    double *lam2;
+    unsigned long long *nk;
+    unsigned long long hk = 0;
+    if ((err = B.alloc(&sel2, nsel)) != hipSuccess ||
+        (err = B.alloc(&lam2, 4 * nsel)) != hipSuccess || (err = B.alloc(&nk, 1)) != hipSuccess ||
+        (err = hipMemsetAsync(nk, 0, 8, st)) != hipSuccess)
+      return adj_hip(e, err, "adjoint memory");
+    hipLaunchKernelGGL(adj_keep, sgrid, dim3(kBlk), 0, st, (const int32_t *)sel,
+                       (const double *)lam, nsel, sel2, lam2, nk);
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipMemcpyAsync(&hk, nk, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (err = hipStreamSynchronize(st)) != hipSuccess)
+      return adj_hip(e, err, "seed");
+    nlive = (int64_t)hk;
+    sel = sel2;
+    lam = lam2;
+  }
+  const int64_t nwl = (nlive + kBlk - 1) / kBlk;
+  const dim3 lgrid((unsigned)((std::max<int64_t>(nlive, 1) + kBlk - 1) / kBlk));
   uint64_t *tape = nullptr;
   size_t tape_cap = 0;  // words
   char *scan_tmp = nullptr;
@@ -1743,30 +1785,30 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       (err = B.alloc(&sort_tmp, sort_bytes)) != hipSuccess)
     return adj_hip(e, err, "order memory");
   const CkPtrs now{P.cell, P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps};
-  for (int w = n_windows - 1; w >= 0 && nsel > 0; --w) {
+  for (int w = n_windows - 1; w >= 0 && nlive > 0; --w) {
     const CkPtrs &in = ck[w];
     const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
-    hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nsel + 1 + blk - 1) / blk)), dim3(blk), 0,
-                       st, (const int32_t *)sel, nsel, (const int64_t *)in.substeps,
+    hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nlive + 1 + blk - 1) / blk)), dim3(blk), 0,
+                       st, (const int32_t *)sel, nlive, (const int64_t *)in.substeps,
                        (const int64_t *)want.substeps, cnt);
     if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "tape offsets");
-    hipLaunchKernelGGL(adj_order_keys, sgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nsel,
+    hipLaunchKernelGGL(adj_order_keys, lgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nlive,
                        (const int32_t *)in.cell, (const int64_t *)cnt, cell_bits, okey, oidx);
     size_t sbytes = sort_bytes;
     if ((err = hipGetLastError()) != hipSuccess ||
         (err = hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sbytes, okey, okey2, oidx,
-                                                            ord, (int)nsel, 0, 31, st)) !=
+                                                            ord, (int)nlive, 0, 31, st)) !=
             hipSuccess)
       return adj_hip(e, err, "particle order");
-    hipLaunchKernelGGL(adj_wave_len, dim3((unsigned)((nwaves + 1 + blk - 1) / blk)), dim3(blk),
-                       0, st, (const int32_t *)ord, nsel, (const int64_t *)cnt, wlen);
+    hipLaunchKernelGGL(adj_wave_len, dim3((unsigned)((nwl + 1 + blk - 1) / blk)), dim3(blk),
+                       0, st, (const int32_t *)ord, nlive, (const int64_t *)cnt, wlen);
     size_t bytes = scan_bytes;
     if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wlen, wstart, (int)(nwaves + 1),
+        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wlen, wstart, (int)(nwl + 1),
                                                 st)) != hipSuccess)
       return adj_hip(e, err, "tape offsets");
     int64_t total = 0;
-    if ((err = hipMemcpyAsync(&total, wstart + nwaves, 8, hipMemcpyDeviceToHost, st)) !=
+    if ((err = hipMemcpyAsync(&total, wstart + nwl, 8, hipMemcpyDeviceToHost, st)) !=
             hipSuccess ||
         (err = hipStreamSynchronize(st)) != hipSuccess)
       return adj_hip(e, err, "tape size");
@@ -1781,12 +1823,12 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     if ((err = hipMemsetAsync(bad, 0, 8, st)) != hipSuccess) return adj_hip(e, err, "replay");
     const WalkParams &W = Ws[w];
     if (ndim == 3)
-      hipLaunchKernelGGL(adj_replay<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nsel, (const int32_t *)ord, in, want, W,
+      hipLaunchKernelGGL(adj_replay<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const int32_t *)sel, nlive, (const int32_t *)ord, in, want, W,
                          (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
     else
-      hipLaunchKernelGGL(adj_replay<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nsel, (const int32_t *)ord, in, want, W,
+      hipLaunchKernelGGL(adj_replay<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const int32_t *)sel, nlive, (const int32_t *)ord, in, want, W,
                          (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
     unsigned long long nbad = 0;
     if ((err = hipGetLastError()) != hipSuccess ||
@@ -1807,14 +1849,14 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     double *ga = grad_al ? g_al : nullptr, *gt = grad_at ? g_at : nullptr,
            *gd = grad_dm ? g_dm : nullptr;
     if (ndim == 3)
-      hipLaunchKernelGGL(adj_backward<3>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nsel,
+      hipLaunchKernelGGL(adj_backward<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, (const double *)dd3, ncell, nlive,
                          (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
                          (const uint64_t *)tape, lam, mus, resume,
                          deferred, ndef, ga, gt, gd, g_s);
     else
-      hipLaunchKernelGGL(adj_backward<2>, sgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nsel,
+      hipLaunchKernelGGL(adj_backward<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, (const double *)dd3, ncell, nlive,
                          (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
                          (const uint64_t *)tape, lam, mus, resume,
                          deferred, ndef, ga, gt, gd, g_s);
