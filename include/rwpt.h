@@ -335,7 +335,9 @@ int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t append,
 /* Per-cell tables as built on the device (checking aid), ncell * 104
  * doubles per cell: face velocities (6), DX DY DZ, porosity, R, dt_cell,
  * corner discharge qx qy qz |q| (8 x 4), corner porosity (8), corner phi*D
- * Dxx Dyy Dzz Dxy Dxz Dyz (6 x 8), dry flag, sub-cell exit flag, 2 unused. */
+ * Dxx Dyy Dzz Dxy Dxz Dyz (6 x 8), dry flag, sub-cell exit flag, and the
+ * cell's source and sink flows (boundary flows on faces with an active
+ * neighbour included). */
 int rwpt_get_cell_tables(rwpt_engine *e, double *out);
 
 /* Pathwise adjoint of an observation misfit (no counterpart in the

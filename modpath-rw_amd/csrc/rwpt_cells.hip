@@ -389,8 +389,8 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
       for (int ci = 0; ci < 8; ++ci) o[52 + 8 * b6 + ci] = Dc[b6][ci];
     o[100] = (fl[1] & CF_DRY) ? 1.0 : 0.0;
     o[101] = sub_exit ? 1.0 : 0.0;
-    o[102] = 0.0;
-    o[103] = 0.0;
+    o[102] = rec[F_SOURCE];
+    o[103] = rec[F_SINK];
   }
 }
 

@@ -1,0 +1,434 @@
+"""MODFLOW 6 DIS output as engine input.
+
+MPathRW reads three files of a MODFLOW 6 simulation with a structured (DIS)
+grid: the binary grid file, the head file and the cell-by-cell budget file
+(MPathRW.F90:305-330, 394-433).  This module reads them and turns one time
+step into the arrays of rwpt_set_flow and rwpt_set_cell_extras, as the
+reference's MODFLOW 6 path does:
+
+- ``read_grb`` follows ReadGRB (RectangularGridDisMf6.f90:122-330): four
+  50-character header lines (GRID DIS, VERSION 1, NTXT 16, LENTXT 100), 16
+  definition lines of 100 characters whose first word names the variable,
+  then the variables in that order; the top of every cell below the first
+  layer is the bottom of the cell above.
+- ``read_heads`` reads the structured head records HeadReader takes from a
+  MODFLOW 6 DIS head file (kstp, kper, pertim, totim, text, ncol, nrow,
+  ilay and one layer of doubles).
+- ``read_budget`` reads compact budget records as BudgetReader's record
+  header does (BudgetReader.f90:560-720): kstp, kper, text, ndim1..3, and,
+  with ndim3 < 0, method, delt, pertim, totim; method 1 arrays, method 2
+  lists, method 5 and 6 lists with auxiliary values (method 6 with the four
+  16-character model and package names and two ids per item).
+- ``load_time_step`` restates FlowModelData%LoadTimeStep for MODFLOW 6 grids
+  (FlowModelData.f90:217-622): IBoundTS from the dry-cell test of
+  ICELLTYPE 1 cells (single-precision comparison with HDry), then each
+  budget term of the time step into FlowsJA, StorageFlows, BoundaryFlows
+  (IFACE from the IFACE auxiliary variable or the default IFACE labels)
+  or the source and sink flows.
+- ``engine_arrays`` gives the engine's MODFLOW-2005 face flows: on a DIS
+  grid a face has at most one connection, and SetMassTransportDataUnstructured
+  (ModpathCellData.f90:886-950) sets Q1 = F(n, left), Q2 = -F(n, right),
+  Q3 = F(n, front), Q4 = -F(n, back), Q5 = F(n, below), Q6 = -F(n, above)
+  from FLOW-JA-FACE F.  The structured assembly the engine implements
+  (ModpathCellData.f90:1315-1351) forms Q2 = frf(n), Q3 = -fff(n),
+  Q5 = -flf(n) and Q1, Q4, Q6 from the neighbours' values, so
+  frf(n) = -F(n, right), fff(n) = -F(n, front), flf(n) = -F(n, below);
+  MODFLOW 6 writes F antisymmetric (F(m, n) = -F(n, m)), which makes the
+  neighbours' faces agree as well.
+
+Rows grow toward the front (the y face coordinates fall with the row,
+RectangularGridDisMf6.f90:300-306) and layers downward.  Only grids whose
+IDOMAIN is 1 everywhere are read: MODFLOW 6 numbers the cells of a grid
+with removed cells differently in the budget file.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_I4 = np.dtype("<i4")
+_F8 = np.dtype("<f8")
+
+
+class Mf6Error(ValueError):
+    """A file that is not MODFLOW 6 DIS output of the supported form."""
+
+
+class _Stream:
+    def __init__(self, path):
+        with open(path, "rb") as f:
+            self.b = f.read()
+        self.p = 0
+
+    def eof(self):
+        return self.p >= len(self.b)
+
+    def take(self, n):
+        if self.p + n > len(self.b):
+            raise Mf6Error("unexpected end of file")
+        s = self.b[self.p:self.p + n]
+        self.p += n
+        return s
+
+    def i4(self, n=None):
+        k = 1 if n is None else n
+        a = np.frombuffer(self.take(4 * k), _I4)
+        return int(a[0]) if n is None else a.astype(np.int32)
+
+    def f8(self, n=None):
+        k = 1 if n is None else n
+        a = np.frombuffer(self.take(8 * k), _F8)
+        return float(a[0]) if n is None else a.astype(np.float64)
+
+    def text(self, n):
+        return self.take(n).decode("ascii", "replace")
+
+
+@dataclasses.dataclass
+class Grb:
+    ncells: int
+    nlay: int
+    nrow: int
+    ncol: int
+    nja: int
+    xorigin: float
+    yorigin: float
+    angrot: float
+    delr: np.ndarray
+    delc: np.ndarray
+    top: np.ndarray        # per cell (the layer above's bottom below layer 1)
+    botm: np.ndarray
+    ia: np.ndarray         # 0-based offsets, ncells + 1
+    ja: np.ndarray         # 1-based cell numbers
+    idomain: np.ndarray
+    icelltype: np.ndarray
+
+
+def read_grb(path) -> Grb:
+    """The binary grid file of a DIS model (ReadGRB)."""
+    s = _Stream(path)
+    words = s.text(50).split()
+    if words[:2] != ["GRID", "DIS"]:
+        raise Mf6Error("not a DIS binary grid file")
+    words = s.text(50).split()
+    if words[:1] != ["VERSION"] or int(words[1]) != 1:
+        raise Mf6Error("unsupported binary grid file version")
+    words = s.text(50).split()
+    if words[:1] != ["NTXT"] or int(words[1]) != 16:
+        raise Mf6Error("binary grid file NTXT must be 16")
+    words = s.text(50).split()
+    if words[:1] != ["LENTXT"] or int(words[1]) != 100:
+        raise Mf6Error("binary grid file LENTXT must be 100")
+    names = []
+    for _ in range(16):
+        line = s.text(100)
+        names.append("#" if line.startswith("#") else line.split()[0])
+    v: Dict[str, object] = {}
+    for name in names:
+        if name == "#":
+            continue
+        if name in ("NCELLS", "NLAY", "NROW", "NCOL", "NJA"):
+            v[name] = s.i4()
+        elif name in ("XORIGIN", "YORIGIN", "ANGROT"):
+            v[name] = s.f8()
+        elif name == "DELR":
+            v[name] = s.f8(v["NCOL"])
+        elif name == "DELC":
+            v[name] = s.f8(v["NROW"])
+        elif name == "TOP":
+            v[name] = s.f8(v["NROW"] * v["NCOL"])
+        elif name == "BOTM":
+            v[name] = s.f8(v["NCELLS"])
+        elif name == "IA":
+            v[name] = s.i4(v["NCELLS"] + 1) - 1
+        elif name == "JA":
+            v[name] = s.i4(v["NJA"])
+        elif name in ("IDOMAIN", "ICELLTYPE"):
+            v[name] = s.i4(v["NCELLS"])
+        else:
+            raise Mf6Error(f"unknown binary grid variable {name}")
+    need = ("NCELLS", "NLAY", "NROW", "NCOL", "NJA", "DELR", "DELC", "TOP", "BOTM", "IA",
+            "JA", "IDOMAIN", "ICELLTYPE")
+    if any(k not in v for k in need):
+        raise Mf6Error("binary grid file lacks a required variable")
+    n, ncpl = v["NCELLS"], v["NROW"] * v["NCOL"]
+    if n != v["NLAY"] * ncpl:
+        raise Mf6Error("NCELLS differs from NLAY * NROW * NCOL")
+    top = np.empty(n)
+    top[:ncpl] = v["TOP"]
+    top[ncpl:] = v["BOTM"][:n - ncpl]
+    return Grb(n, v["NLAY"], v["NROW"], v["NCOL"], v["NJA"], v.get("XORIGIN", 0.0),
+               v.get("YORIGIN", 0.0), v.get("ANGROT", 0.0), v["DELR"], v["DELC"], top,
+               v["BOTM"], v["IA"], v["JA"], v["IDOMAIN"], v["ICELLTYPE"])
+
+
+@dataclasses.dataclass
+class HeadRecord:
+    kstp: int
+    kper: int
+    pertim: float
+    totim: float
+    text: str
+    ncol: int
+    nrow: int
+    ilay: int
+    data: np.ndarray
+
+
+def read_heads(path) -> List[HeadRecord]:
+    """Structured double-precision head records."""
+    s = _Stream(path)
+    out = []
+    while not s.eof():
+        kstp, kper = s.i4(), s.i4()
+        pertim, totim = s.f8(), s.f8()
+        text = s.text(16)
+        ncol, nrow, ilay = s.i4(), s.i4(), s.i4()
+        if ncol < 1 or nrow < 1 or ilay < 1:
+            raise Mf6Error("not a structured head file")
+        out.append(HeadRecord(kstp, kper, pertim, totim, text, ncol, nrow, ilay,
+                              s.f8(ncol * nrow)))
+    return out
+
+
+@dataclasses.dataclass
+class BudgetRecord:
+    kstp: int
+    kper: int
+    text: str                       # trimmed label
+    ndim1: int
+    ndim2: int
+    ndim3: int
+    method: int
+    delt: float = 0.0
+    pertim: float = 0.0
+    totim: float = 0.0
+    array: Optional[np.ndarray] = None     # methods 0, 1, 3, 4
+    layer: Optional[np.ndarray] = None     # method 3
+    node: Optional[np.ndarray] = None      # methods 2, 5, 6 (id1)
+    node2: Optional[np.ndarray] = None     # method 6 (id2)
+    q: Optional[np.ndarray] = None
+    aux_names: Tuple[str, ...] = ()
+    aux: Optional[np.ndarray] = None       # (nlist, naux)
+
+    @property
+    def array_item_count(self):
+        if self.method in (2, 5, 6):
+            return 0
+        if self.method in (3, 4):
+            return self.ndim1 * self.ndim2
+        return self.ndim1 * self.ndim2 * self.ndim3
+
+
+def read_budget(path) -> List[BudgetRecord]:
+    """Double-precision budget records, standard or compact."""
+    s = _Stream(path)
+    out = []
+    while not s.eof():
+        kstp, kper = s.i4(), s.i4()
+        text = s.text(16).strip()
+        n1, n2, n3 = s.i4(), s.i4(), s.i4()
+        if n3 < 0:
+            n3 = -n3
+            method = s.i4()
+            delt, pertim, totim = s.f8(), s.f8(), s.f8()
+        else:
+            method, delt, pertim, totim = 0, 0.0, 0.0, 0.0
+        if not 0 <= method <= 6:
+            raise Mf6Error(f"budget method {method}")
+        r = BudgetRecord(kstp, kper, text, n1, n2, n3, method, delt, pertim, totim)
+        if method in (0, 1):
+            r.array = s.f8(n1 * n2 * n3)
+        elif method == 3:
+            r.layer = s.i4(n1 * n2)
+            r.array = s.f8(n1 * n2)
+        elif method == 4:
+            r.array = s.f8(n1 * n2)
+        else:
+            naux = 0
+            if method == 6:
+                s.take(64)
+            if method in (5, 6):
+                nval = s.i4()
+                naux = nval - 1
+                r.aux_names = tuple(s.text(16).strip() for _ in range(naux))
+            nlist = s.i4()
+            ids = 2 if method == 6 else 1
+            rec = np.dtype([("id", "<i4", (ids,)), ("v", "<f8", (1 + naux,))])
+            a = np.frombuffer(s.take(rec.itemsize * nlist), rec)
+            r.node = a["id"][:, 0].astype(np.int64)
+            r.node2 = a["id"][:, 1].astype(np.int64) if ids == 2 else None
+            r.q = a["v"][:, 0].astype(np.float64)
+            r.aux = a["v"][:, 1:].astype(np.float64)
+        out.append(r)
+    return out
+
+
+@dataclasses.dataclass
+class TimeStepData:
+    """FlowModelData's arrays after LoadTimeStep (1-based cells as index + 1)."""
+    ibound_ts: np.ndarray
+    heads: np.ndarray
+    flows_ja: np.ndarray
+    boundary_flows: np.ndarray      # 6 per cell, faces 1..6
+    source_flows: np.ndarray
+    sink_flows: np.ndarray
+    storage_flows: np.ndarray
+    steady_state: bool
+
+
+def _dry_tolerance(hdry):
+    # HDryTol = abs(epsilon(HDryTol) * sngl(HDry)) in default REAL
+    return np.float32(abs(np.finfo(np.float32).eps * np.float32(hdry)))
+
+
+def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetRecord],
+                   kper: int, kstp: int, ibound: np.ndarray, hdry: float,
+                   default_iface: Dict[str, int] = None) -> TimeStepData:
+    """FlowModelData%LoadTimeStep(kper, kstp) for a MODFLOW 6 DIS grid."""
+    if np.any(grb.idomain != 1):
+        raise Mf6Error("grids with IDOMAIN other than 1 are not supported")
+    n = grb.ncells
+    default_iface = {k.strip(): int(v) for k, v in (default_iface or {}).items()}
+    recs = [r for r in budget if r.kper == kper and r.kstp == kstp]
+    if not recs:
+        raise Mf6Error(f"no budget records for period {kper} step {kstp}")
+    h = np.zeros(n)
+    ncpl = grb.nrow * grb.ncol
+    for r in heads:
+        if r.kper == kper and r.kstp == kstp:
+            if r.ncol * r.nrow != ncpl or not 1 <= r.ilay <= grb.nlay:
+                raise Mf6Error("head record does not match the grid")
+            h[(r.ilay - 1) * ncpl:r.ilay * ncpl] = r.data
+    ib = np.asarray(ibound, np.int32).copy()
+    # IBoundTS: dry ICELLTYPE 1 cells (FlowModelData.f90:260-282)
+    conv = grb.icelltype == 1
+    diff = np.abs(h.astype(np.float32) - np.float32(hdry))
+    ib[conv & (diff < _dry_tolerance(hdry))] = 0
+    src, snk, sto = np.zeros(n), np.zeros(n), np.zeros(n)
+    bfl = np.zeros(6 * n)
+    fja = np.zeros(grb.nja)
+    steady = True
+
+    def split(cells0, values):
+        # cells0: 0-based, values in record order (sums kept in order)
+        for c, v in zip(cells0, values):
+            if v > 0.0:
+                src[c] += v
+            elif v < 0.0:
+                snk[c] += v
+
+    def boundary(cells0, faces, values):
+        for c, f, v in zip(cells0, faces, values):
+            bfl[6 * c + f - 1] += v
+
+    def listed(r, allow_aux_iface):
+        iface0 = default_iface.get(r.text, 0)
+        k = r.aux_names.index("IFACE") if (allow_aux_iface and "IFACE" in r.aux_names) \
+            else -1
+        for m in range(len(r.q)):
+            c = int(r.node[m]) - 1
+            iface = int(r.aux[m, k]) if k >= 0 else iface0
+            v = float(r.q[m])
+            if iface > 0:
+                bfl[6 * c + iface - 1] += v
+            elif v > 0.0:
+                src[c] += v
+            elif v < 0.0:
+                snk[c] += v
+
+    for r in recs:
+        t = r.text
+        if t in ("CONSTANT HEAD", "CHD"):
+            if r.method == 0:
+                if r.array_item_count == n:
+                    split(range(n), r.array)
+            elif r.method == 2:
+                split(r.node - 1, r.q)
+            elif r.method in (5, 6):
+                listed(r, True)
+        elif t in ("STORAGE", "STO-SS", "STO-SY"):
+            if r.method in (0, 1) and r.array_item_count == n:
+                sto += r.array
+                if np.any(sto != 0.0):
+                    steady = False
+        elif t in ("FLOW JA FACE", "FLOW-JA-FACE"):
+            if r.method in (0, 1):
+                if r.array_item_count == grb.nja:
+                    fja = r.array.copy()
+            elif r.method == 6:
+                split(r.node - 1, r.q)
+        elif t in ("FLOW RIGHT FACE", "FLOW FRONT FACE", "FLOW LOWER FACE"):
+            raise Mf6Error(f"{t} is a MODFLOW-2005 budget term")
+        elif t in ("DATA-SPDIS", "DATA-SAT"):
+            continue
+        else:
+            iface = default_iface.get(t, 0)
+            if r.method in (0, 1):
+                if r.array_item_count == n:
+                    if iface > 0:
+                        boundary(range(n), [iface] * n, r.array)
+                    else:
+                        split(range(n), r.array)
+            elif r.method == 3:
+                cells = (r.layer.astype(np.int64) - 1) * len(r.array) + \
+                    np.arange(len(r.array))
+                if iface > 0:
+                    boundary(cells, [iface] * len(cells), r.array)
+                else:
+                    split(r.layer.astype(np.int64) - 1, r.array)
+            elif r.method == 4:
+                if iface > 0:
+                    boundary(range(len(r.array)), [iface] * len(r.array), r.array)
+                else:
+                    split(range(len(r.array)), r.array)
+            elif r.method == 2:
+                if iface > 0:
+                    boundary(r.node - 1, [iface] * len(r.q), r.q)
+                else:
+                    split(r.node - 1, r.q)
+            elif r.method in (5, 6):
+                listed(r, True)
+    return TimeStepData(ib, h, fja, bfl, src, snk, sto, steady)
+
+
+def _neighbour_index(grb: Grb):
+    """Index into FLOW-JA-FACE of each cell's right, front and lower
+    connection (-1 where the grid has no such neighbour)."""
+    n, ncol, ncpl = grb.ncells, grb.ncol, grb.nrow * grb.ncol
+    idx = -np.ones((3, n), np.int64)
+    cell = np.arange(n)
+    col, row, lay = cell % ncol, (cell % ncpl) // ncol, cell // ncpl
+    want = [(col + 1 < grb.ncol, cell + 1), (row + 1 < grb.nrow, cell + ncol),
+            (lay + 1 < grb.nlay, cell + ncpl)]
+    for c in range(n):
+        a, b = grb.ia[c], grb.ia[c + 1]
+        conn = grb.ja[a:b] - 1
+        if b - a < 1 or conn[0] != c:
+            raise Mf6Error("JA must list each cell first in its own row")
+        for f, (has, other) in enumerate(want):
+            if has[c]:
+                hit = np.nonzero(conn == other[c])[0]
+                if len(hit) != 1:
+                    raise Mf6Error("a DIS neighbour is missing from JA")
+                idx[f, c] = a + hit[0]
+    return idx
+
+
+def engine_arrays(grb: Grb, ts: TimeStepData):
+    """The engine's inputs of one time step: MODFLOW-2005 face flows frf,
+    fff, flf, the 6 boundary flows per cell, IBoundTS, heads, layer types
+    (ICELLTYPE) and the source and sink flows."""
+    idx = _neighbour_index(grb)
+    face = []
+    for f in range(3):
+        q = np.zeros(grb.ncells)
+        on = idx[f] >= 0
+        q[on] = -ts.flows_ja[idx[f][on]]
+        face.append(q)
+    return dict(frf=face[0], fff=face[1], flf=face[2], bflow=ts.boundary_flows.copy(),
+                ibound=ts.ibound_ts.copy(), head=ts.heads.copy(),
+                layer_type=grb.icelltype.astype(np.int32), source_flow=ts.source_flows.copy(),
+                sink_flow=ts.sink_flows.copy())

@@ -196,3 +196,61 @@ def writers(P, state, initial, meta, timeout=120):
         return tuple(open(os.path.join(d, f), "rb").read()
                      for f in ("ts.out", "ep.out", "epm.out", "obsres.out", "obsres.bin",
                                "obssink.out", "obssink.bin"))
+
+
+MF6_BINARY = os.path.join(HERE, "_ref", "mf6_ref")
+
+
+def mf6_available() -> bool:
+    return os.access(MF6_BINARY, os.X_OK)
+
+
+def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e30,
+        hdry=-1e30, default_iface=(), timeout=120):
+    """Reads MODFLOW 6 DIS grid, head and budget files with the reference's
+    readers and FlowModelData%LoadTimeStep (mf6_ref), and assembles every
+    cell as ModpathCellData%SetMassTransportDataUnstructured does.  Returns
+    FlowModelData's arrays and per cell the first sub-face flows Q1..Q6,
+    the cell's source and sink flows and its sub-cell velocities and sizes."""
+    ibound = np.ascontiguousarray(ibound, np.int32)
+    n = ibound.size
+    labels = [k for k, _ in default_iface]
+    values = np.array([v for _, v in default_iface], np.int32)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(np.array([kper, kstp], np.int32).tobytes())
+            fh.write(np.array([hnoflow, hdry], np.float64).tobytes())
+            fh.write(np.array([len(labels)], np.int32).tobytes())
+            fh.write(ibound.tobytes())
+            fh.write(np.ascontiguousarray(porosity, np.float64).tobytes())
+            fh.write(np.ascontiguousarray(retardation, np.float64).tobytes())
+            for lab in labels:
+                fh.write(lab.encode().ljust(16)[:16])
+            fh.write(values.tobytes())
+        subprocess.run([MF6_BINARY, str(grb), str(hds), str(cbc), fin, fout], check=True,
+                       timeout=timeout, stdout=subprocess.DEVNULL, cwd=d)
+        b = open(fout, "rb").read()
+    ncell, nja, steady = np.frombuffer(b[:12], np.int32)
+    assert ncell == n
+    p = 12
+    out = {"steady_state": bool(steady)}
+    out["ibound_ts"] = np.frombuffer(b[p:p + 4 * n], np.int32).copy()
+    p += 4 * n
+    for k in ("heads", "source_flows", "sink_flows", "storage_flows"):
+        out[k] = np.frombuffer(b[p:p + 8 * n], np.float64).copy()
+        p += 8 * n
+    out["boundary_flows"] = np.frombuffer(b[p:p + 48 * n], np.float64).copy()
+    p += 48 * n
+    out["flows_ja"] = np.frombuffer(b[p:p + 8 * nja], np.float64).copy()
+    p += 8 * nja
+    for k in ("top", "bottom"):
+        out[k] = np.frombuffer(b[p:p + 8 * n], np.float64).copy()
+        p += 8 * n
+    cells = np.frombuffer(b[p:p + 8 * 17 * n], np.float64).reshape(n, 17)
+    out["q"] = cells[:, :6].copy()
+    out["cell_source"] = cells[:, 6].copy()
+    out["cell_sink"] = cells[:, 7].copy()
+    out["velocity"] = cells[:, 8:14].copy()
+    out["size"] = cells[:, 14:17].copy()
+    return out

@@ -1517,7 +1517,8 @@ int orc_advance(const orc_problem *P, long n, int *cell, double *lx,
  *  [10] retardation  [11] dtcell  [12..43] corner q (8 corners x 4)
  *  [44..51] corner porosity  [52..99] Dxxx Dyyy Dzzz Dxyx Dxzx Dyzy
  *  (6 blocks of 8; Dxyy=Dxyx, Dxzz=Dxzx, Dyzz=Dyzy)  [100] dry
- *  [101] sub-cell has exit face  [102..103] unused */
+ *  [101] sub-cell has exit face  [102] source flow  [103] sink flow (after
+ *  boundary flows on faces with an active neighbour were added) */
 int orc_cell_tables(const orc_problem *P, double *out) {
   int nc = ncells(P);
   int nd = P->dim_mask[0] + P->dim_mask[1] + P->dim_mask[2];
@@ -1556,8 +1557,8 @@ int orc_cell_tables(const orc_problem *P, double *out) {
       }
       o[100] = s->dry;
       o[101] = subcell_has_exit(s);
-      o[102] = 0.0;
-      o[103] = 0.0;
+      o[102] = center.source;
+      o[103] = center.sink;
     }
     free(nb);
   }

@@ -1,0 +1,89 @@
+"""MODFLOW 6 DIS input through the engine on the GPU.
+
+The engine is built from rwpt.mf6's reading of a small MODFLOW 6 DIS
+simulation's grid, head and budget files (the scenario of
+tests/test_mf6.py).  Its cell records must equal, bit for bit, the
+reference's MODFLOW 6 assembly of the same files (face pore velocities,
+DX DY DZ, source and sink flows; oracle/_ref/mf6_ref) and the C oracle's
+records, and a walk over them must equal the oracle's walk bit for bit
+(EULERIAN: IEEE add, multiply, divide and square root in the reference's
+operand order on both sides)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import refrun
+from rwpt import Engine, mf6, problems
+
+from test_mf6 import HDRY, IFACE, engine_problem, oracle_problem, scenario
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("cell", "status", "lx", "ly", "lz", "t", "substeps", "rng")
+
+
+def _read(tmp):
+    paths, P0, ibound = scenario(tmp)
+    por = np.full(P0.ncell, 0.25)
+    por[::5] = 0.3
+    ret = np.ones(P0.ncell)
+    ret[::7] = 1.5
+    g = mf6.read_grb(paths[0])
+    ts = mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
+                            ibound, HDRY, dict(IFACE))
+    P, extras = engine_problem(P0, g, ts, por, ret)
+    P.ts_params = (0.1, 0.0)
+    return paths, P, extras, ibound, por, ret
+
+
+def _engine(P, extras):
+    E = Engine(P)
+    E.set_cell_extras(**extras)
+    return E
+
+
+def test_engine_cells_match_reference_mf6(tmp_path):
+    paths, P, extras, ibound, por, ret = _read(str(tmp_path))
+    print("mf6 cells", P.ncol, P.nrow, P.nlay, flush=True)
+    E = _engine(P, extras)
+    try:
+        got = E.cell_tables()
+    finally:
+        E.close()
+    want = O.cell_tables(oracle_problem(P, extras))
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    if not refrun.mf6_available():
+        pytest.skip("oracle/_ref/mf6_ref is not built")
+    ref = refrun.mf6(*paths, ibound, por, ret, hdry=HDRY, default_iface=IFACE)
+    act = P.ibound != 0
+    np.testing.assert_array_equal(got[act, 0:6], ref["velocity"][act])
+    np.testing.assert_array_equal(got[act, 6:9], ref["size"][act])
+    np.testing.assert_array_equal(got[act, 102], ref["cell_source"][act])
+    np.testing.assert_array_equal(got[act, 103], ref["cell_sink"][act])
+
+
+def test_advance_from_mf6_bitwise(tmp_path):
+    _, P, extras, _, _, _ = _read(str(tmp_path))
+    n = 600
+    cell, lx, ly, lz = problems.release_in_block(P, n, (2, 7), (2, 5), (1, 3), seed=4)
+    print("mf6 advance", n, flush=True)
+    E = _engine(P, extras)
+    try:
+        E.load_particles(cell, lx, ly, lz)
+        st = O.new_state(cell, lx, ly, lz)
+        OP = oracle_problem(P, extras)
+        t = 0.0
+        for k in range(1, 4):
+            t1 = 0.5 * k
+            E.advance(t1)
+            assert O.advance(OP, st, t, t1) == 0
+            t = t1
+        g = E.particles()
+    finally:
+        E.close()
+    for f in FIELDS:
+        nan = f in ("lx", "ly", "lz", "t")
+        assert np.array_equal(g[f], st[f], equal_nan=nan), (f, int((g[f] != st[f]).sum()))
+    assert st["substeps"].sum() > n
+    # the walk reached the statuses the boundary, sink and dry cells give
+    assert len(set(st["status"].tolist())) >= 2
