@@ -312,43 +312,43 @@ def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[Budge
     fja = np.zeros(grb.nja)
     steady = True
 
+    # np.add.at applies repeated indices one after another in item order,
+    # so every sum is formed in the reference's order
     def split(cells0, values):
-        # cells0: 0-based, values in record order (sums kept in order)
-        for c, v in zip(cells0, values):
-            if v > 0.0:
-                src[c] += v
-            elif v < 0.0:
-                snk[c] += v
+        c = np.asarray(cells0, np.int64)
+        v = np.asarray(values, np.float64)
+        pos, neg = v > 0.0, v < 0.0
+        np.add.at(src, c[pos], v[pos])
+        np.add.at(snk, c[neg], v[neg])
 
-    def boundary(cells0, faces, values):
-        for c, f, v in zip(cells0, faces, values):
-            bfl[6 * c + f - 1] += v
+    def boundary(cells0, iface, values):
+        c = np.asarray(cells0, np.int64)
+        np.add.at(bfl, 6 * c + np.asarray(iface, np.int64) - 1,
+                  np.asarray(values, np.float64))
 
-    def listed(r, allow_aux_iface):
-        iface0 = default_iface.get(r.text, 0)
-        k = r.aux_names.index("IFACE") if (allow_aux_iface and "IFACE" in r.aux_names) \
-            else -1
-        for m in range(len(r.q)):
-            c = int(r.node[m]) - 1
-            iface = int(r.aux[m, k]) if k >= 0 else iface0
-            v = float(r.q[m])
-            if iface > 0:
-                bfl[6 * c + iface - 1] += v
-            elif v > 0.0:
-                src[c] += v
-            elif v < 0.0:
-                snk[c] += v
+    def listed(r):
+        c = r.node - 1
+        if "IFACE" in r.aux_names:
+            # int() of the auxiliary value: truncation toward zero
+            iface = np.trunc(r.aux[:, r.aux_names.index("IFACE")]).astype(np.int64)
+        else:
+            iface = np.full(len(c), default_iface.get(r.text, 0), np.int64)
+        on = iface > 0
+        # one cell's items go to a face or to source/sink in item order, and
+        # the two targets are separate arrays, so the split keeps each order
+        boundary(c[on], iface[on], r.q[on])
+        split(c[~on], r.q[~on])
 
     for r in recs:
         t = r.text
         if t in ("CONSTANT HEAD", "CHD"):
             if r.method == 0:
                 if r.array_item_count == n:
-                    split(range(n), r.array)
+                    split(np.arange(n), r.array)
             elif r.method == 2:
                 split(r.node - 1, r.q)
             elif r.method in (5, 6):
-                listed(r, True)
+                listed(r)
         elif t in ("STORAGE", "STO-SS", "STO-SY"):
             if r.method in (0, 1) and r.array_item_count == n:
                 sto += r.array
@@ -369,28 +369,28 @@ def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[Budge
             if r.method in (0, 1):
                 if r.array_item_count == n:
                     if iface > 0:
-                        boundary(range(n), [iface] * n, r.array)
+                        boundary(np.arange(n), iface, r.array)
                     else:
-                        split(range(n), r.array)
+                        split(np.arange(n), r.array)
             elif r.method == 3:
                 cells = (r.layer.astype(np.int64) - 1) * len(r.array) + \
                     np.arange(len(r.array))
                 if iface > 0:
-                    boundary(cells, [iface] * len(cells), r.array)
+                    boundary(cells, iface, r.array)
                 else:
                     split(r.layer.astype(np.int64) - 1, r.array)
             elif r.method == 4:
                 if iface > 0:
-                    boundary(range(len(r.array)), [iface] * len(r.array), r.array)
+                    boundary(np.arange(len(r.array)), iface, r.array)
                 else:
-                    split(range(len(r.array)), r.array)
+                    split(np.arange(len(r.array)), r.array)
             elif r.method == 2:
                 if iface > 0:
-                    boundary(r.node - 1, [iface] * len(r.q), r.q)
+                    boundary(r.node - 1, iface, r.q)
                 else:
                     split(r.node - 1, r.q)
             elif r.method in (5, 6):
-                listed(r, True)
+                listed(r)
     return TimeStepData(ib, h, fja, bfl, src, snk, sto, steady)
 
 
@@ -398,22 +398,25 @@ def _neighbour_index(grb: Grb):
     """Index into FLOW-JA-FACE of each cell's right, front and lower
     connection (-1 where the grid has no such neighbour)."""
     n, ncol, ncpl = grb.ncells, grb.ncol, grb.nrow * grb.ncol
+    cnt = np.diff(grb.ia)
+    if np.any(cnt < 1) or np.any(grb.ja[grb.ia[:-1]] - 1 != np.arange(n)):
+        raise Mf6Error("JA must list each cell first in its own row")
+    # (row cell, connected cell) pairs as one sortable key
+    row = np.repeat(np.arange(n, dtype=np.int64), cnt)
+    key = row * (n + 1) + (grb.ja.astype(np.int64) - 1)
+    order = np.argsort(key, kind="stable")
+    skey = key[order]
+    cell = np.arange(n, dtype=np.int64)
+    col, rrow, lay = cell % ncol, (cell % ncpl) // ncol, cell // ncpl
     idx = -np.ones((3, n), np.int64)
-    cell = np.arange(n)
-    col, row, lay = cell % ncol, (cell % ncpl) // ncol, cell // ncpl
-    want = [(col + 1 < grb.ncol, cell + 1), (row + 1 < grb.nrow, cell + ncol),
-            (lay + 1 < grb.nlay, cell + ncpl)]
-    for c in range(n):
-        a, b = grb.ia[c], grb.ia[c + 1]
-        conn = grb.ja[a:b] - 1
-        if b - a < 1 or conn[0] != c:
-            raise Mf6Error("JA must list each cell first in its own row")
-        for f, (has, other) in enumerate(want):
-            if has[c]:
-                hit = np.nonzero(conn == other[c])[0]
-                if len(hit) != 1:
-                    raise Mf6Error("a DIS neighbour is missing from JA")
-                idx[f, c] = a + hit[0]
+    for f, (has, other) in enumerate(((col + 1 < grb.ncol, cell + 1),
+                                      (rrow + 1 < grb.nrow, cell + ncol),
+                                      (lay + 1 < grb.nlay, cell + ncpl))):
+        q = cell[has] * (n + 1) + other[has]
+        pos = np.minimum(np.searchsorted(skey, q), len(skey) - 1)
+        if np.any(skey[pos] != q):
+            raise Mf6Error("a DIS neighbour is missing from JA")
+        idx[f, has] = order[pos]
     return idx
 
 
