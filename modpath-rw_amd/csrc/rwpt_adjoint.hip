@@ -1425,11 +1425,12 @@ __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const double *__restrict__ dd3, int64_t ncell,
     const WalkParams W, const int32_t *__restrict__ ord, const int64_t *__restrict__ wstart,
     const uint64_t *__restrict__ tape, double *__restrict__ lam, const double *__restrict__ mus,
-    const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred, int64_t ndef,
-    double *g_al, double *g_at, double *g_dm, double *g_s) {
+    const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred,
+    const unsigned long long *__restrict__ ndef, double *g_al, double *g_at, double *g_dm,
+    double *g_s) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double gs = 0.0;
-  if (t < ndef) {
+  if (t < (int64_t)*ndef) {
     const int64_t j = deferred[t], i = ord[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {mus[4 * j], mus[4 * j + 1], mus[4 * j + 2], mus[4 * j + 3]};
@@ -1696,21 +1697,20 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   const int64_t nsel = (int64_t)hsel.size();
   int32_t *sel;
   double *lam, *tan, *resid_d, *g_al, *g_at, *g_dm, *g_s;
-  int64_t *cnt, *wlen, *wstart, *resume;
+  int64_t *wlen, *wstart, *resume;
   const int64_t nwaves = (nsel + kBlk - 1) / kBlk;
   double *mus;
   int32_t *deferred;
-  unsigned long long *bad, *ndef;
   const int64_t ncell = G.ncell;
   if ((err = B.alloc(&sel, nsel)) != hipSuccess || (err = B.alloc(&lam, 4 * nsel)) != hipSuccess ||
       (err = B.alloc(&tan, 4 * (size_t)ncell * kTan)) != hipSuccess ||
       (err = B.alloc(&resid_d, n_obs)) != hipSuccess || (err = B.alloc(&g_al, ncell)) != hipSuccess ||
       (err = B.alloc(&g_at, ncell)) != hipSuccess || (err = B.alloc(&g_dm, ncell)) != hipSuccess ||
-      (err = B.alloc(&g_s, 1)) != hipSuccess || (err = B.alloc(&cnt, nsel + 1)) != hipSuccess ||
+      (err = B.alloc(&g_s, 1)) != hipSuccess ||
       (err = B.alloc(&wlen, nwaves + 1)) != hipSuccess ||
-      (err = B.alloc(&wstart, nwaves + 1)) != hipSuccess || (err = B.alloc(&bad, 1)) != hipSuccess ||
+      (err = B.alloc(&wstart, nwaves + 1)) != hipSuccess ||
       (err = B.alloc(&resume, nsel)) != hipSuccess || (err = B.alloc(&mus, 4 * nsel)) != hipSuccess ||
-      (err = B.alloc(&deferred, nsel)) != hipSuccess || (err = B.alloc(&ndef, 1)) != hipSuccess)
+      (err = B.alloc(&deferred, nsel)) != hipSuccess)
     return adj_hip(e, err, "adjoint memory");
   if ((nsel > 0 && (err = hipMemcpyAsync(sel, hsel.data(), 4 * nsel, hipMemcpyHostToDevice,
                                          st)) != hipSuccess) ||
@@ -1762,130 +1762,128 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   }
   const int64_t nwl = (nlive + kBlk - 1) / kBlk;
   const dim3 lgrid((unsigned)((std::max<int64_t>(nlive, 1) + kBlk - 1) / kBlk));
-  uint64_t *tape = nullptr;
-  size_t tape_cap = 0;  // words
-  char *scan_tmp = nullptr;
-  size_t scan_bytes = 0;
-  if ((err = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, wlen, wstart,
-                                              (int)(nwaves + 1), st)) != hipSuccess)
-    return adj_hip(e, err, "scan");
-  if ((err = B.alloc(&scan_tmp, scan_bytes)) != hipSuccess)
-    return adj_hip(e, err, "scan memory");
-  // per-window order of the listed particles (iteration count, then cell)
-  int32_t *okey, *okey2, *oidx, *ord;
-  char *sort_tmp = nullptr;
-  size_t sort_bytes = 0;
+  // Per window, before the reverse pass: the listed particles' iteration
+  // counts, their order (iteration count, then starting cell) and the tape
+  // blocks of the wavefronts.  The windows' replays and sweeps are then
+  // queued back to back with no host round trip: one tape of the largest
+  // window's size, replay mismatches counted per window and checked at the
+  // end, and the full pass reads its particle count on the device.
   int cell_bits = 1;
   while (cell_bits < 24 && (int64_t(1) << cell_bits) <= ncell) ++cell_bits;
-  if ((err = B.alloc(&okey, nsel)) != hipSuccess || (err = B.alloc(&okey2, nsel)) != hipSuccess ||
-      (err = B.alloc(&oidx, nsel)) != hipSuccess || (err = B.alloc(&ord, nsel)) != hipSuccess ||
+  std::vector<int32_t *> ords(n_windows, nullptr);
+  std::vector<int64_t *> cnts(n_windows, nullptr), wstarts(n_windows, nullptr);
+  int32_t *okey, *okey2, *oidx;
+  int64_t *totals;
+  unsigned long long *bads, *ndefs;
+  char *sort_tmp = nullptr, *scan_tmp = nullptr;
+  size_t sort_bytes = 0, scan_bytes = 0;
+  if ((err = B.alloc(&okey, nlive)) != hipSuccess || (err = B.alloc(&okey2, nlive)) != hipSuccess ||
+      (err = B.alloc(&oidx, nlive)) != hipSuccess ||
+      (err = B.alloc(&totals, n_windows)) != hipSuccess ||
+      (err = B.alloc(&bads, n_windows)) != hipSuccess ||
+      (err = B.alloc(&ndefs, n_windows)) != hipSuccess ||
       (err = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, okey, okey2, oidx,
-                                                          ord, (int)nsel, 0, 31, st)) !=
+                                                          oidx, (int)nlive, 0, 31, st)) !=
           hipSuccess ||
-      (err = B.alloc(&sort_tmp, sort_bytes)) != hipSuccess)
+      (err = B.alloc(&sort_tmp, sort_bytes)) != hipSuccess ||
+      (err = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, wlen, wstart, (int)(nwl + 1),
+                                              st)) != hipSuccess ||
+      (err = B.alloc(&scan_tmp, scan_bytes)) != hipSuccess ||
+      (err = hipMemsetAsync(totals, 0, 8 * (size_t)n_windows, st)) != hipSuccess ||
+      (err = hipMemsetAsync(bads, 0, 8 * (size_t)n_windows, st)) != hipSuccess ||
+      (err = hipMemsetAsync(ndefs, 0, 8 * (size_t)n_windows, st)) != hipSuccess)
     return adj_hip(e, err, "order memory");
   const CkPtrs now{P.cell, P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps};
+  for (int w = 0; w < n_windows && nlive > 0; ++w) {
+    const CkPtrs &in = ck[w];
+    const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
+    if ((err = B.alloc(&ords[w], nlive)) != hipSuccess ||
+        (err = B.alloc(&cnts[w], nlive + 1)) != hipSuccess ||
+        (err = B.alloc(&wstarts[w], nwl + 1)) != hipSuccess)
+      return adj_hip(e, err, "order memory");
+    hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nlive + 1 + blk - 1) / blk)), dim3(blk), 0,
+                       st, (const int32_t *)sel, nlive, (const int64_t *)in.substeps,
+                       (const int64_t *)want.substeps, cnts[w]);
+    hipLaunchKernelGGL(adj_order_keys, lgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nlive,
+                       (const int32_t *)in.cell, (const int64_t *)cnts[w], cell_bits, okey,
+                       oidx);
+    size_t sb = sort_bytes, cb = scan_bytes;
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sb, okey, okey2, oidx,
+                                                            ords[w], (int)nlive, 0, 31, st)) !=
+            hipSuccess)
+      return adj_hip(e, err, "particle order");
+    hipLaunchKernelGGL(adj_wave_len, dim3((unsigned)((nwl + 1 + blk - 1) / blk)), dim3(blk), 0,
+                       st, (const int32_t *)ords[w], nlive, (const int64_t *)cnts[w], wlen);
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, cb, wlen, wstarts[w], (int)(nwl + 1),
+                                                st)) != hipSuccess ||
+        (err = hipMemcpyAsync(totals + w, wstarts[w] + nwl, 8, hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess)
+      return adj_hip(e, err, "tape offsets");
+  }
+  std::vector<int64_t> htotal(n_windows, 0);
+  if ((err = hipMemcpyAsync(htotal.data(), totals, 8 * (size_t)n_windows,
+                            hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return adj_hip(e, err, "tape size");
+  const int64_t most = *std::max_element(htotal.begin(), htotal.end());
+  uint64_t *tape = nullptr;
+  if ((err = B.alloc(&tape, (size_t)most * kTapeWords * kBlk)) != hipSuccess)
+    return adj_hip(e, err, "tape memory");
+  double *ga = grad_al ? g_al : nullptr, *gt = grad_at ? g_at : nullptr,
+         *gd = grad_dm ? g_dm : nullptr;
   for (int w = n_windows - 1; w >= 0 && nlive > 0; --w) {
     const CkPtrs &in = ck[w];
     const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
-    hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nlive + 1 + blk - 1) / blk)), dim3(blk), 0,
-                       st, (const int32_t *)sel, nlive, (const int64_t *)in.substeps,
-                       (const int64_t *)want.substeps, cnt);
-    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "tape offsets");
-    hipLaunchKernelGGL(adj_order_keys, lgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nlive,
-                       (const int32_t *)in.cell, (const int64_t *)cnt, cell_bits, okey, oidx);
-    size_t sbytes = sort_bytes;
-    if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, sbytes, okey, okey2, oidx,
-                                                            ord, (int)nlive, 0, 31, st)) !=
-            hipSuccess)
-      return adj_hip(e, err, "particle order");
-    hipLaunchKernelGGL(adj_wave_len, dim3((unsigned)((nwl + 1 + blk - 1) / blk)), dim3(blk),
-                       0, st, (const int32_t *)ord, nlive, (const int64_t *)cnt, wlen);
-    size_t bytes = scan_bytes;
-    if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wlen, wstart, (int)(nwl + 1),
-                                                st)) != hipSuccess)
-      return adj_hip(e, err, "tape offsets");
-    int64_t total = 0;
-    if ((err = hipMemcpyAsync(&total, wstart + nwl, 8, hipMemcpyDeviceToHost, st)) !=
-            hipSuccess ||
-        (err = hipStreamSynchronize(st)) != hipSuccess)
-      return adj_hip(e, err, "tape size");
-    const size_t words = (size_t)total * kTapeWords * kBlk;
-    if (words > tape_cap) {
-      if (tape) (void)hipFree(tape);
-      B.ptrs.erase(std::remove(B.ptrs.begin(), B.ptrs.end(), (void *)tape), B.ptrs.end());
-      tape = nullptr;
-      tape_cap = std::max<size_t>(words, tape_cap + tape_cap / 4);
-      if ((err = B.alloc(&tape, tape_cap)) != hipSuccess) return adj_hip(e, err, "tape memory");
-    }
-    if ((err = hipMemsetAsync(bad, 0, 8, st)) != hipSuccess) return adj_hip(e, err, "replay");
     const WalkParams &W = Ws[w];
+    const int32_t *ord = ords[w];
+    const int64_t *cw = cnts[w], *ws = wstarts[w];
     if (ndim == 3)
       hipLaunchKernelGGL(adj_replay<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nlive, (const int32_t *)ord, in, want, W,
-                         (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
+                         (const int32_t *)sel, nlive, ord, in, want, W, cw, ws, tape, bads + w);
     else
       hipLaunchKernelGGL(adj_replay<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const int32_t *)sel, nlive, (const int32_t *)ord, in, want, W,
-                         (const int64_t *)cnt, (const int64_t *)wstart, tape, bad);
-    unsigned long long nbad = 0;
-    if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-        (err = hipStreamSynchronize(st)) != hipSuccess)
-      return adj_hip(e, err, "replay");
-    if (nbad != 0) {
+                         (const int32_t *)sel, nlive, ord, in, want, W, cw, ws, tape, bads + w);
+    if (ndim == 3)
+      hipLaunchKernelGGL(adj_backward<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,
+                         (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt,
+                         gd, g_s);
+    else
+      hipLaunchKernelGGL(adj_backward<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
+                         (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,
+                         (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt,
+                         gd, g_s);
+    if (ndim == 3)
+      hipLaunchKernelGGL(adj_backward_full<3>, lgrid, dim3(kBlk), 0, st,
+                         (const double *)G.cells, (const double *)tan, (const double *)dd3,
+                         ncell, W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,
+                         (const int64_t *)resume, (const int32_t *)deferred,
+                         (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s);
+    else
+      hipLaunchKernelGGL(adj_backward_full<2>, lgrid, dim3(kBlk), 0, st,
+                         (const double *)G.cells, (const double *)tan, (const double *)dd3,
+                         ncell, W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,
+                         (const int64_t *)resume, (const int32_t *)deferred,
+                         (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "reverse sweep");
+  }
+  std::vector<unsigned long long> hbad(n_windows, 0);
+  if ((err = hipMemcpyAsync(hbad.data(), bads, 8 * (size_t)n_windows, hipMemcpyDeviceToHost,
+                            st)) != hipSuccess ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return adj_hip(e, err, "replay");
+  for (int w = 0; w < n_windows; ++w)
+    if (hbad[w] != 0) {
       char buf[160];
       snprintf(buf, sizeof buf,
                "adjoint: the replay of window %d differs from the forward walk for %llu "
                "particles",
-               w, nbad);
+               w, hbad[w]);
       e->err = buf;
       return RWPT_E_STATE;
     }
-    if ((err = hipMemsetAsync(ndef, 0, 8, st)) != hipSuccess)
-      return adj_hip(e, err, "reverse sweep");
-    double *ga = grad_al ? g_al : nullptr, *gt = grad_at ? g_at : nullptr,
-           *gd = grad_dm ? g_dm : nullptr;
-    if (ndim == 3)
-      hipLaunchKernelGGL(adj_backward<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nlive,
-                         (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
-                         (const uint64_t *)tape, lam, mus, resume,
-                         deferred, ndef, ga, gt, gd, g_s);
-    else
-      hipLaunchKernelGGL(adj_backward<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nlive,
-                         (const int32_t *)ord, W, (const int64_t *)cnt, (const int64_t *)wstart,
-                         (const uint64_t *)tape, lam, mus, resume,
-                         deferred, ndef, ga, gt, gd, g_s);
-    unsigned long long nd = 0;
-    if ((err = hipGetLastError()) != hipSuccess ||
-        (err = hipMemcpyAsync(&nd, ndef, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-        (err = hipStreamSynchronize(st)) != hipSuccess)
-      return adj_hip(e, err, "reverse sweep");
-    if (nd > 0) {
-      const dim3 dgrid((unsigned)((nd + kBlk - 1) / kBlk));
-      if (ndim == 3)
-        hipLaunchKernelGGL(adj_backward_full<3>, dgrid, dim3(kBlk), 0, st,
-                           (const double *)G.cells, (const double *)tan,
-                           (const double *)dd3, ncell, W,
-                           (const int32_t *)ord, (const int64_t *)wstart,
-                           (const uint64_t *)tape, lam,
-                           (const double *)mus, (const int64_t *)resume,
-                           (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
-      else
-        hipLaunchKernelGGL(adj_backward_full<2>, dgrid, dim3(kBlk), 0, st,
-                           (const double *)G.cells, (const double *)tan,
-                           (const double *)dd3, ncell, W,
-                           (const int32_t *)ord, (const int64_t *)wstart,
-                           (const uint64_t *)tape, lam,
-                           (const double *)mus, (const int64_t *)resume,
-                           (const int32_t *)deferred, (int64_t)nd, ga, gt, gd, g_s);
-    }
-    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "reverse sweep");
-  }
   if ((grad_al && (err = hipMemcpyAsync(grad_al, g_al, 8 * ncell, hipMemcpyDeviceToHost, st)) !=
                       hipSuccess) ||
       (grad_at && (err = hipMemcpyAsync(grad_at, g_at, 8 * ncell, hipMemcpyDeviceToHost, st)) !=
