@@ -435,3 +435,22 @@ def engine_arrays(grb: Grb, ts: TimeStepData):
                 ibound=ts.ibound_ts.copy(), head=ts.heads.copy(),
                 layer_type=grb.icelltype.astype(np.int32), source_flow=ts.source_flows.copy(),
                 sink_flow=ts.sink_flows.copy())
+
+
+def problem(grb: Grb, ts: TimeStepData, porosity, retardation=None, **options):
+    """The engine's Problem and cell extras (rwpt_set_cell_extras keyword
+    arguments) for one read time step.  options: the Problem's dispersion,
+    advection, time-step and tracking fields (aL, aT, dmeff, ts_params ...)."""
+    from .problems import Problem
+    a = engine_arrays(grb, ts)
+    n = grb.ncells
+    P = Problem(ncol=grb.ncol, nrow=grb.nrow, nlay=grb.nlay, delr=grb.delr.copy(),
+                delc=grb.delc.copy(), top=grb.top.copy(), bot=grb.botm.copy(),
+                frf=a["frf"], fff=a["fff"], flf=a["flf"], bflow=a["bflow"],
+                porosity=np.broadcast_to(np.asarray(porosity, np.float64), (n,)).copy(),
+                retardation=None if retardation is None else
+                np.broadcast_to(np.asarray(retardation, np.float64), (n,)).copy(),
+                ibound=a["ibound"], **options)
+    extras = dict(head=a["head"], layer_type=a["layer_type"], source_flow=a["source_flow"],
+                  sink_flow=a["sink_flow"])
+    return P, extras

@@ -31,8 +31,11 @@ def _read(tmp):
     g = mf6.read_grb(paths[0])
     ts = mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
                             ibound, HDRY, dict(IFACE))
-    P, extras = engine_problem(P0, g, ts, por, ret)
-    P.ts_params = (0.1, 0.0)
+    P, extras = mf6.problem(g, ts, por, ret, aL=P0.aL, aT=P0.aT, dmeff=P0.dmeff,
+                            ts_params=(0.1, 0.0))
+    Q, _ = engine_problem(P0, g, ts, por, ret)
+    for k in ("frf", "fff", "flf", "bflow", "ibound", "top", "bot", "delr", "delc"):
+        assert np.array_equal(getattr(P, k), getattr(Q, k)), k
     return paths, P, extras, ibound, por, ret
 
 
