@@ -57,20 +57,35 @@ class Mf6Error(ValueError):
 
 
 class _Stream:
+    """Sequential reads of a binary file, with skips that do not read."""
+
     def __init__(self, path):
-        with open(path, "rb") as f:
-            self.b = f.read()
-        self.p = 0
+        import os
+        self.f = open(path, "rb")
+        self.size = os.fstat(self.f.fileno()).st_size
+
+    def close(self):
+        self.f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def eof(self):
-        return self.p >= len(self.b)
+        return self.f.tell() >= self.size
 
     def take(self, n):
-        if self.p + n > len(self.b):
+        b = self.f.read(n)
+        if len(b) != n:
             raise Mf6Error("unexpected end of file")
-        s = self.b[self.p:self.p + n]
-        self.p += n
-        return s
+        return b
+
+    def skip(self, n):
+        if self.f.tell() + n > self.size:
+            raise Mf6Error("unexpected end of file")
+        self.f.seek(n, 1)
 
     def i4(self, n=None):
         k = 1 if n is None else n
@@ -108,7 +123,11 @@ class Grb:
 
 def read_grb(path) -> Grb:
     """The binary grid file of a DIS model (ReadGRB)."""
-    s = _Stream(path)
+    with _Stream(path) as s:
+        return _read_grb(s)
+
+
+def _read_grb(s) -> Grb:
     words = s.text(50).split()
     if words[:2] != ["GRID", "DIS"]:
         raise Mf6Error("not a DIS binary grid file")
@@ -177,19 +196,23 @@ class HeadRecord:
     data: np.ndarray
 
 
-def read_heads(path) -> List[HeadRecord]:
-    """Structured double-precision head records."""
-    s = _Stream(path)
+def read_heads(path, kper=None, kstp=None) -> List[HeadRecord]:
+    """Structured double-precision head records (of one time step when kper
+    and kstp are given; the others are skipped unread)."""
     out = []
-    while not s.eof():
-        kstp, kper = s.i4(), s.i4()
-        pertim, totim = s.f8(), s.f8()
-        text = s.text(16)
-        ncol, nrow, ilay = s.i4(), s.i4(), s.i4()
-        if ncol < 1 or nrow < 1 or ilay < 1:
-            raise Mf6Error("not a structured head file")
-        out.append(HeadRecord(kstp, kper, pertim, totim, text, ncol, nrow, ilay,
-                              s.f8(ncol * nrow)))
+    with _Stream(path) as s:
+        while not s.eof():
+            kstp_, kper_ = s.i4(), s.i4()
+            pertim, totim = s.f8(), s.f8()
+            text = s.text(16)
+            ncol, nrow, ilay = s.i4(), s.i4(), s.i4()
+            if ncol < 1 or nrow < 1 or ilay < 1:
+                raise Mf6Error("not a structured head file")
+            if (kper is not None and kper_ != kper) or (kstp is not None and kstp_ != kstp):
+                s.skip(8 * ncol * nrow)
+                continue
+            out.append(HeadRecord(kstp_, kper_, pertim, totim, text, ncol, nrow, ilay,
+                                  s.f8(ncol * nrow)))
     return out
 
 
@@ -222,9 +245,15 @@ class BudgetRecord:
         return self.ndim1 * self.ndim2 * self.ndim3
 
 
-def read_budget(path) -> List[BudgetRecord]:
-    """Double-precision budget records, standard or compact."""
-    s = _Stream(path)
+def read_budget(path, kper=None, kstp=None) -> List[BudgetRecord]:
+    """Double-precision budget records, standard or compact (of one time
+    step when kper and kstp are given; the others' data are skipped
+    unread)."""
+    with _Stream(path) as s:
+        return _read_budget(s, kper, kstp)
+
+
+def _read_budget(s, want_kper, want_kstp) -> List[BudgetRecord]:
     out = []
     while not s.eof():
         kstp, kper = s.i4(), s.i4()
@@ -239,6 +268,12 @@ def read_budget(path) -> List[BudgetRecord]:
         if not 0 <= method <= 6:
             raise Mf6Error(f"budget method {method}")
         r = BudgetRecord(kstp, kper, text, n1, n2, n3, method, delt, pertim, totim)
+        keep = (want_kper is None or kper == want_kper) and \
+            (want_kstp is None or kstp == want_kstp)
+        if not keep and method in (0, 1, 3, 4):
+            s.skip({0: 8 * n1 * n2 * n3, 1: 8 * n1 * n2 * n3, 3: 12 * n1 * n2,
+                    4: 8 * n1 * n2}[method])
+            continue
         if method in (0, 1):
             r.array = s.f8(n1 * n2 * n3)
         elif method == 3:
@@ -257,6 +292,9 @@ def read_budget(path) -> List[BudgetRecord]:
             nlist = s.i4()
             ids = 2 if method == 6 else 1
             rec = np.dtype([("id", "<i4", (ids,)), ("v", "<f8", (1 + naux,))])
+            if not keep:
+                s.skip(rec.itemsize * nlist)
+                continue
             a = np.frombuffer(s.take(rec.itemsize * nlist), rec)
             r.node = a["id"][:, 0].astype(np.int64)
             r.node2 = a["id"][:, 1].astype(np.int64) if ids == 2 else None

@@ -69,16 +69,18 @@ def write_grb(path, nlay, nrow, ncol, delr, delc, top, botm, icelltype, idomain=
 
 
 def write_heads(path, nlay, nrow, ncol, head, steps=((1, 1, 1.0),)):
-    """One record per layer and time step (kper, kstp, totim)."""
+    """One record per layer and time step (kper, kstp, totim); head is one
+    array for every step or a list of one array per step."""
     ncpl = nrow * ncol
     with open(path, "wb") as f:
-        for kper, kstp, totim in steps:
+        for si, (kper, kstp, totim) in enumerate(steps):
+            h = head[si] if isinstance(head, (list, tuple)) else head
             for k in range(nlay):
                 f.write(np.array([kstp, kper], "<i4").tobytes())
                 f.write(np.array([totim, totim], "<f8").tobytes())
                 f.write(_txt("HEAD", 16))
                 f.write(np.array([ncol, nrow, k + 1], "<i4").tobytes())
-                f.write(np.asarray(head[k * ncpl:(k + 1) * ncpl], "<f8").tobytes())
+                f.write(np.asarray(h[k * ncpl:(k + 1) * ncpl], "<f8").tobytes())
 
 
 def flow_ja(ia, ja, frf, fff, flf, nrow, ncol):
@@ -113,11 +115,12 @@ def _head(f, kper, kstp, text, n1, n2, n3, method, totim):
     f.write(np.array([1.0, totim, totim], "<f8").tobytes())
 
 
-def write_budget(path, nlay, nrow, ncol, fja, records, kper=1, kstp=1, totim=1.0):
+def write_budget(path, nlay, nrow, ncol, fja, records, kper=1, kstp=1, totim=1.0,
+                 append=False):
     """FLOW-JA-FACE, then each record: ("array", text, values) written as a
     method 1 array over the grid, or ("list", text, package, nodes, q,
     aux_names, aux) written as a method 6 list."""
-    with open(path, "wb") as f:
+    with open(path, "ab" if append else "wb") as f:
         _head(f, kper, kstp, "FLOW-JA-FACE", len(fja), 1, 1, 1, totim)
         f.write(np.asarray(fja, "<f8").tobytes())
         for r in records:

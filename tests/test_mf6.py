@@ -187,3 +187,37 @@ def test_rejects_other_files(tmp_path):
     with pytest.raises(mf6.Mf6Error):
         mf6.load_time_step(mf6.read_grb(paths[0]), mf6.read_heads(paths[1]),
                            mf6.read_budget(paths[2]), 2, 1, ibound, HDRY)
+
+
+def test_second_time_step(tmp_path):
+    """Files with two time steps: the reader keeps only the requested one
+    (the other is skipped unread) and equals the reference's
+    LoadTimeStep(1, 2)."""
+    R = _refrun()
+    if not R.mf6_available():
+        pytest.skip("oracle/_ref/mf6_ref is not built (needs the reference sources)")
+    paths, P, ibound = scenario(str(tmp_path))
+    h1 = np.concatenate([r.data for r in mf6.read_heads(paths[1])])
+    h2 = np.where(h1 == HDRY, HDRY, h1 + 0.125)
+    S.write_heads(paths[1], P.nlay, P.nrow, P.ncol, [h1, h2], steps=((1, 1, 1.0), (1, 2, 2.0)))
+    g = mf6.read_grb(paths[0])
+    ia, ja = g.ia + 1, g.ja
+    fja2 = 2.0 * S.flow_ja(ia, ja, P.frf, P.fff, P.flf, P.nrow, P.ncol)
+    wel = [int(P.cell_number(2, 3, 4)), int(P.cell_number(1, 2, 2))]
+    S.write_budget(paths[2], P.nlay, P.nrow, P.ncol, fja2,
+                   [("list", "WEL", "WEL-1", wel, [-4.0, 1.0], (), None),
+                    ("list", "RCH", "RCH-1", [1, 2, 3], [0.5, 0.25, 0.125], (), None)],
+                   kper=1, kstp=2, totim=2.0, append=True)
+    por, ret = np.full(P.ncell, 0.25), np.ones(P.ncell)
+    ref = R.mf6(*paths, ibound, por, ret, kper=1, kstp=2, hdry=HDRY, default_iface=IFACE)
+    heads = mf6.read_heads(paths[1], kper=1, kstp=2)
+    budget = mf6.read_budget(paths[2], kper=1, kstp=2)
+    assert {(r.kper, r.kstp) for r in heads + budget} == {(1, 2)}
+    ts = mf6.load_time_step(g, heads, budget, 1, 2, ibound, HDRY, dict(IFACE))
+    np.testing.assert_array_equal(ts.ibound_ts, ref["ibound_ts"])
+    for k in ("heads", "flows_ja", "boundary_flows", "source_flows", "sink_flows",
+              "storage_flows"):
+        np.testing.assert_array_equal(getattr(ts, k), ref[k], err_msg=k)
+    assert ts.steady_state == ref["steady_state"] and ts.steady_state
+    np.testing.assert_array_equal(mf6.engine_arrays(g, ts)["frf"], 2.0 * P.frf)
+
