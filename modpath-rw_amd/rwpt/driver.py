@@ -15,7 +15,9 @@ the time reaches the stop time (:1730).
 
 ``Driver`` runs an Engine over such a schedule, writing the timeseries
 records, resident observation records and gpkde density rows at the
-points the reference writes them.  All tracking, reconstruction and file
+points the reference writes them.  With a flow source it loads each MODFLOW
+time step's flow before the step's first interval, as TIME_STEP_LOOP calls
+FlowModelData%LoadTimeStep (MPathRW.F90:850-870).  All tracking, reconstruction and file
 formatting runs in librwpt; this module only orders the calls.
 """
 from __future__ import annotations
@@ -103,8 +105,14 @@ class Driver:
     def __init__(self, engine, intervals: Sequence[Interval], *, timeseries_path=None,
                  sequence=None, ids=None, group=None, group_solute=None, mass=None,
                  observations=(), kde_options=None, kde_path=None, kde_format=0,
-                 kde_column_format=0):
+                 kde_column_format=0, flow_for_step=None):
+        """flow_for_step: None (the engine's flow holds for every step) or a
+        callable of the 1-based cumulative time step returning a Problem
+        (its face flows, boundary flows, porosity, retardation and ibound
+        are set) and the keyword arguments of Engine.set_cell_extras (or
+        None)."""
         self.E = engine
+        self.flow_for_step = flow_for_step
         self.intervals = list(intervals)
         self.timeseries_path = timeseries_path
         n = engine.n
@@ -125,7 +133,14 @@ class Driver:
         sub-steps) after each and stops early when nothing is left to track."""
         from .engine import kde_write
         log = []
+        loaded = None
         for iv in self.intervals:
+            if self.flow_for_step is not None and iv.ktime != loaded:
+                P, extras = self.flow_for_step(iv.ktime)
+                self.E.set_flow(P)
+                if extras:
+                    self.E.set_cell_extras(**extras)
+                loaded = iv.ktime
             active, steps = self.E.advance(iv.time)
             st = self.E.particles()["status"]
             pending = int((st == 0).sum())

@@ -492,3 +492,28 @@ def problem(grb: Grb, ts: TimeStepData, porosity, retardation=None, **options):
     extras = dict(head=a["head"], layer_type=a["layer_type"], source_flow=a["source_flow"],
                   sink_flow=a["sink_flow"])
     return P, extras
+
+
+class FlowSeries:
+    """The flow of successive MODFLOW time steps for Driver(flow_for_step=):
+    steps lists (kper, kstp) of the cumulative time steps 1, 2, ...; each
+    call reads that step's head and budget records and returns
+    problem(...)'s Problem and cell extras."""
+
+    def __init__(self, grb_path, heads_path, budget_path, steps, ibound, hdry, porosity,
+                 retardation=None, default_iface=None, **options):
+        self.grb = read_grb(grb_path)
+        self.heads_path, self.budget_path = heads_path, budget_path
+        self.steps = list(steps)
+        self.ibound, self.hdry = ibound, hdry
+        self.porosity, self.retardation = porosity, retardation
+        self.default_iface = default_iface
+        self.options = options
+
+    def __call__(self, ktime):
+        kper, kstp = self.steps[ktime - 1]
+        ts = load_time_step(self.grb, read_heads(self.heads_path, kper, kstp),
+                            read_budget(self.budget_path, kper, kstp), kper, kstp,
+                            self.ibound, self.hdry, self.default_iface)
+        return problem(self.grb, ts, self.porosity, self.retardation, **self.options)
+

@@ -7,7 +7,8 @@ reference's MODFLOW 6 assembly of the same files (face pore velocities,
 DX DY DZ, source and sink flows; oracle/_ref/mf6_ref) and the C oracle's
 records, and a walk over them must equal the oracle's walk bit for bit
 (EULERIAN: IEEE add, multiply, divide and square root in the reference's
-operand order on both sides)."""
+operand order on both sides).  A driver over a two-step MODFLOW 6 run
+loads each step's flow before the step's first interval."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,8 @@ from oracle import oracle as O
 from oracle import refrun
 from rwpt import Engine, mf6, problems
 
-from test_mf6 import HDRY, IFACE, engine_problem, oracle_problem, scenario
+from test_mf6 import (HDRY, IFACE, engine_problem, oracle_problem, scenario,
+                      two_step_scenario)
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +92,46 @@ def test_advance_from_mf6_bitwise(tmp_path):
     assert st["substeps"].sum() > n
     # the walk reached the statuses the boundary, sink and dry cells give
     assert len(set(st["status"].tolist())) >= 2
+
+
+def test_driver_loads_each_time_step(tmp_path):
+    """Two MODFLOW time steps with different flows: Driver with an
+    mf6.FlowSeries gives the particles of the same set_flow,
+    set_cell_extras and advance calls made by hand, and they differ from a
+    run that keeps the first step's flow."""
+    from rwpt.driver import Driver, schedule
+    paths, P0, ibound = two_step_scenario(str(tmp_path))
+    por, ret = np.full(P0.ncell, 0.25), np.ones(P0.ncell)
+    fs = mf6.FlowSeries(*paths, steps=[(1, 1), (1, 2)], ibound=ibound, hdry=HDRY,
+                        porosity=por, retardation=ret, default_iface=dict(IFACE),
+                        aL=P0.aL, aT=P0.aT, dmeff=P0.dmeff, ts_params=(0.1, 0.0))
+    ivs = schedule([1.0, 2.0], 2.0, time_points=[0.5, 1.5])
+    assert [iv.ktime for iv in ivs] == [1, 1, 2, 2]
+    rel = problems.release_in_block(P0, 400, (2, 7), (2, 5), (1, 3), seed=6)
+    print("mf6 driver", len(ivs), flush=True)
+
+    def run(mode):
+        P1, x1 = fs(1)
+        E = Engine(P1)
+        try:
+            E.set_cell_extras(**x1)
+            E.load_particles(*rel)
+            if mode == "driver":
+                Driver(E, ivs, flow_for_step=fs).run()
+            else:
+                for iv in ivs:
+                    if mode == "manual" and iv.ktime == 2 and iv.time == 1.5:
+                        P2, x2 = fs(2)
+                        E.set_flow(P2)
+                        E.set_cell_extras(**x2)
+                    E.advance(iv.time)
+            return E.particles()
+        finally:
+            E.close()
+
+    a, b, c = run("driver"), run("manual"), run("steady")
+    for f in FIELDS:
+        nan = f in ("lx", "ly", "lz", "t")
+        assert np.array_equal(a[f], b[f], equal_nan=nan), f
+    assert not np.array_equal(a["lx"], c["lx"], equal_nan=True)
+

@@ -189,6 +189,24 @@ def test_rejects_other_files(tmp_path):
                            mf6.read_budget(paths[2]), 2, 1, ibound, HDRY)
 
 
+def two_step_scenario(tmp):
+    """scenario() with a second time step (kper 1, kstp 2): heads raised by
+    0.125 outside the dry cells, face flows doubled, other wells and
+    recharge, no storage."""
+    paths, P, ibound = scenario(tmp)
+    h1 = np.concatenate([r.data for r in mf6.read_heads(paths[1])])
+    h2 = np.where(h1 == HDRY, HDRY, h1 + 0.125)
+    S.write_heads(paths[1], P.nlay, P.nrow, P.ncol, [h1, h2], steps=((1, 1, 1.0), (1, 2, 2.0)))
+    g = mf6.read_grb(paths[0])
+    fja2 = 2.0 * S.flow_ja(g.ia + 1, g.ja, P.frf, P.fff, P.flf, P.nrow, P.ncol)
+    wel = [int(P.cell_number(2, 3, 4)), int(P.cell_number(1, 2, 2))]
+    S.write_budget(paths[2], P.nlay, P.nrow, P.ncol, fja2,
+                   [("list", "WEL", "WEL-1", wel, [-4.0, 1.0], (), None),
+                    ("list", "RCH", "RCH-1", [1, 2, 3], [0.5, 0.25, 0.125], (), None)],
+                   kper=1, kstp=2, totim=2.0, append=True)
+    return paths, P, ibound
+
+
 def test_second_time_step(tmp_path):
     """Files with two time steps: the reader keeps only the requested one
     (the other is skipped unread) and equals the reference's
@@ -196,18 +214,8 @@ def test_second_time_step(tmp_path):
     R = _refrun()
     if not R.mf6_available():
         pytest.skip("oracle/_ref/mf6_ref is not built (needs the reference sources)")
-    paths, P, ibound = scenario(str(tmp_path))
-    h1 = np.concatenate([r.data for r in mf6.read_heads(paths[1])])
-    h2 = np.where(h1 == HDRY, HDRY, h1 + 0.125)
-    S.write_heads(paths[1], P.nlay, P.nrow, P.ncol, [h1, h2], steps=((1, 1, 1.0), (1, 2, 2.0)))
+    paths, P, ibound = two_step_scenario(str(tmp_path))
     g = mf6.read_grb(paths[0])
-    ia, ja = g.ia + 1, g.ja
-    fja2 = 2.0 * S.flow_ja(ia, ja, P.frf, P.fff, P.flf, P.nrow, P.ncol)
-    wel = [int(P.cell_number(2, 3, 4)), int(P.cell_number(1, 2, 2))]
-    S.write_budget(paths[2], P.nlay, P.nrow, P.ncol, fja2,
-                   [("list", "WEL", "WEL-1", wel, [-4.0, 1.0], (), None),
-                    ("list", "RCH", "RCH-1", [1, 2, 3], [0.5, 0.25, 0.125], (), None)],
-                   kper=1, kstp=2, totim=2.0, append=True)
     por, ret = np.full(P.ncell, 0.25), np.ones(P.ncell)
     ref = R.mf6(*paths, ibound, por, ret, kper=1, kstp=2, hdry=HDRY, default_iface=IFACE)
     heads = mf6.read_heads(paths[1], kper=1, kstp=2)
