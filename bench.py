@@ -22,8 +22,10 @@ Prints one JSON line: particle-steps/s (accepted RW sub-steps, the unit the
 reference's TrackSubCell loop iterates, over the walk's wall time), the
 roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
 and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
-and the CPU baseline (the oracle, OpenMP, on this host's cores, on a bounded
-sample of the same workload: the first interval of all particles).  The
+and the CPU baseline on this host's cores over the first interval of the
+same workload: the reference's own engine compiled from its sources
+(OpenMP, one rng stream per thread) on 180,000 particles when it is built,
+beside the oracle (OpenMP) on all particles.  The
 precision workload (config 2: 1e7 particles, 1000 steps, FP32-local and
 FP64, errors on moments and on the concentration at 64 observation cells)
 is reported as config2 (--no-config2 skips it).  The calibration workload
@@ -72,23 +74,53 @@ def workload(n_particles):
     return P, rel
 
 
-def cpu_baseline(P, rel, n_sample, seconds_budget=20.0):
-    """The oracle (plain C, OpenMP over particles) on this host's cores: the
-    first tracking interval of the same workload on n_sample particles."""
+def cpu_baseline(P, rel, n_sample, n_reference=180000):
+    """The host cores on the first tracking interval of the same workload:
+    the reference's own tracking engine compiled from its sources
+    (oracle/_ref/rwpt_ref in timing mode: OpenMP over particles with one rng
+    stream per thread, as MPathRW runs it) on n_reference particles when it
+    is built, and the oracle (plain C, OpenMP over particles) on n_sample
+    particles.  The reference reports no sub-step count, so its rate is the
+    oracle's sub-step count on the same particles over the reference's
+    tracking time (the two walks draw different numbers, so the counts agree
+    statistically, not exactly)."""
     from oracle import oracle as O
     threads = os.cpu_count() or 1
-    cell, lx, ly, lz = (a[:n_sample] for a in rel)
-    st = O.new_state(cell, lx, ly, lz)
-    OP = O.OracleProblem(P)
-    t0 = time.perf_counter()
-    O.advance(OP, st, 0.0, INTERVAL_STEPS * DT, nthreads=threads)
-    el = time.perf_counter() - t0
-    steps = int(st["substeps"].sum())
-    return {"value": steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n_sample} particles x {INTERVAL_STEPS} RW steps "
-                      f"({steps} particle-steps) of the same workload, "
-                      f"{el:.1f} s on {threads} OpenMP threads"}
+
+    def port(n):
+        cell, lx, ly, lz = (a[:n] for a in rel)
+        st = O.new_state(cell, lx, ly, lz)
+        t0 = time.perf_counter()
+        O.advance(O.OracleProblem(P), st, 0.0, INTERVAL_STEPS * DT, nthreads=threads)
+        return int(st["substeps"].sum()), time.perf_counter() - t0
+
+    steps, el = port(n_sample)
+    out = {"value": steps / el, "unit": "particle-steps/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{n_sample} particles x {INTERVAL_STEPS} RW steps "
+                     f"({steps} particle-steps) of the same workload, "
+                     f"{el:.1f} s on {threads} OpenMP threads"}
+    try:
+        from oracle import refrun
+        if not refrun.available():
+            return out
+        n = min(n_reference, n_sample)
+        cell, lx, ly, lz = (a[:n] for a in rel)
+        tm = {}
+        refrun.run(P, cell, lx, ly, lz, [INTERVAL_STEPS * DT], threads=threads, timing=tm,
+                   timeout=300)
+        rsteps, _ = port(n)
+        port_out = dict(out)
+        out = {"value": rsteps / tm["seconds"], "unit": "particle-steps/s",
+               "cores": threads, "kind": "reference",
+               "sample": f"{n} particles x {INTERVAL_STEPS} RW steps of the same workload, "
+                         f"{tm['seconds']:.1f} s of tracking on {threads} OpenMP threads; "
+                         f"particle-steps = the oracle's {rsteps} sub-steps on the same "
+                         "particles (the reference counts none)",
+               "port": port_out}
+    except Exception as exc:  # the port's figure stands without the reference
+        out["reference_error"] = str(exc)
+    return out
 
 
 def adaptive_kde(E, origin, nbins, loops):

@@ -44,8 +44,10 @@ program rwpt_ref
   real(real64), allocatable :: lx(:), ly(:), lz(:), tt(:)
   integer(int32), allocatable :: cell(:), status(:), ibuf(:)
   real(real64) :: time, maxTime, tPoint(1)
-  character(len=1024) :: fin, fout
+  character(len=1024) :: fin, fout, targ
   logical :: teamOk
+  integer :: nteam
+  real(real64) :: wall0
 
   class(ModflowRectangularGridType), pointer :: grid
   type(RectangularGridDisType), pointer :: dis
@@ -63,6 +65,15 @@ program rwpt_ref
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  ! optional third argument: timing mode with that many OpenMP threads,
+  ! one rng stream per thread as MPathRW runs it (draws differ from the
+  ! per-particle streams of the default mode); the elapsed tracking time
+  ! is printed as "seconds <s>"
+  nteam = 0
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, targ)
+    read(targ, *) nteam
+  end if
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
   read(10) hdr
   ncol = hdr(1); nrow = hdr(2); nlay = hdr(3); np = hdr(4); nout = hdr(5)
@@ -177,21 +188,23 @@ program rwpt_ref
 
   ! one rng stream per particle: omp_get_max_threads() streams are made by
   ! TrackSubCell%InitializeRandomWalk during Initialize
+  if (nteam < 1) nteam = np
   call omp_set_dynamic(.false.)
-  call omp_set_num_threads(np)
+  call omp_set_num_threads(nteam)
   call engine%Initialize(grid, options, flow, transport)
   call engine%UpdateDispersionFunction(dispersion)
 
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
   time = 0d0
   teamOk = .true.
+  wall0 = omp_get_wtime()
   do kt = 1, nout
     maxTime = tout(kt)
     tPoint(1) = maxTime
-    !$omp parallel do num_threads(np) schedule(static, 1) default(shared) &
+    !$omp parallel do num_threads(nteam) schedule(static, 1) default(shared) &
     !$omp private(p, loc, result, last, plCount, st, cbuf) firstprivate(engine)
     do p = 1, np
-      if (omp_get_num_threads() .ne. np) teamOk = .false.
+      if (omp_get_num_threads() .ne. nteam) teamOk = .false.
       if (status(p) .eq. 0) then
         if (tt(p) .lt. time) then
           status(p) = 8
@@ -247,4 +260,5 @@ program rwpt_ref
     time = maxTime
   end do
   close(11)
+  write(*, '(A, ES16.8)') 'seconds ', omp_get_wtime() - wall0
 end program rwpt_ref

@@ -29,9 +29,13 @@ def _filled(a, n, value, dtype):
     return np.ascontiguousarray(a, dtype)
 
 
-def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
+def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600, threads=None,
+        timing=None):
     """Tracks the particles from time 0 through the output times t_out.
-    Returns a list (one per output time) of dicts cell/status/lx/ly/lz/t."""
+    Returns a list (one per output time) of dicts cell/status/lx/ly/lz/t.
+    threads: timing mode, the reference's own one rng stream per OpenMP
+    thread on that many threads (the draws then differ from the
+    per-particle streams); the tracking seconds go to timing["seconds"]."""
     n = int(np.asarray(cell).size)
     nc = P.ncell
     nd = int(np.asarray(P.aL).size)
@@ -68,10 +72,16 @@ def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
         with open(fin, "wb") as fh:
             for a in parts:
                 fh.write(np.ascontiguousarray(a).tobytes())
-        env = dict(os.environ, OMP_STACKSIZE="2M", OMP_THREAD_LIMIT=str(max(n, 1)),
+        team = max(n, 1) if threads is None else int(threads)
+        env = dict(os.environ, OMP_STACKSIZE="2M", OMP_THREAD_LIMIT=str(team),
                    OMP_DYNAMIC="false")
-        subprocess.run([BINARY, fin, fout], check=True, timeout=timeout, env=env,
-                       stdout=subprocess.DEVNULL)
+        args = [BINARY, fin, fout] + ([] if threads is None else [str(team)])
+        res = subprocess.run(args, check=True, timeout=timeout, env=env,
+                             stdout=subprocess.PIPE, text=True)
+        if timing is not None:
+            for line in res.stdout.splitlines():
+                if line.startswith("seconds"):
+                    timing["seconds"] = float(line.split()[1])
         raw = open(fout, "rb").read()
     out, off = [], 0
     for _ in range(t_out.size):
