@@ -1,44 +1,42 @@
 """Benchmark of the random-walk hot path on one MI355X.
 
-Workload (one "step" = one tracking interval = one rwpt_advance call plus
-one gpkde reconstruction): the heterogeneous 200x200x20 lognormal field (1 m
-cells, divergence-free face flows, porosity U[0.20,0.30], aL=0.5 m,
-aT=0.05 m, Dm=1e-4 m2/d), 1e6 particles released uniformly in the
-10x50x20-cell inlet block, FIXED dt = 0.5 d, timeseries every 50 steps -- so
-one bench step advances every particle by 50 RW time steps (25 d) and
---steps 10 is the whole 500-step run.  Particle state is resident in HBM
-before the timed region.  Each interval ends with the gpkde density of the
-active particles on the 200x200x20 flow grid (reported as KDE cells/s; the
-wall time includes the host copies of the C ABI).
+Headline workload (the precision workload): the heterogeneous 200x200x20
+lognormal field (1 m cells, divergence-free face flows, porosity
+U[0.20,0.30], aL=0.5 m, aT=0.05 m, Dm=1e-4 m2/d) with 1e7 particles
+released uniformly in the 10x50x20-cell inlet block, FIXED dt = 0.5 d.  One
+bench step is one tracking interval: one rwpt_advance of every particle by
+50 RW time steps (25 d), then one gpkde reconstruction (one density pass) of
+the active particles on the 200x200x20 grid, as MPathRW does at a gpkde
+time point.  The run is 20 intervals (1000 RW steps); --steps beyond 20
+starts the run again from the release (the reload from host memory is then
+inside the timed region).  Particle state is resident in HBM before the
+timed region; --warmup intervals run first and the state is then reloaded.
 
-The headline walk is the FP32-local form (--precision 1: FP32 cell-local
-positions and sub-step arithmetic, FP64 times, the FP64 walk's random
-draws); the plain FP64 walk runs the same workload for the speedup, and an
-untimed pass reports the FP32 errors against FP64 (plume mean and variance,
-same-cell fraction, KDE density L1).  --precision 0 makes plain FP64 the
-headline.
+value = accepted RW sub-steps (the iterations of the reference's
+TrackSubCell loop) per second of the timed region (walk and
+reconstructions), FP32-local walk (--precision 1: FP32 cell-local positions
+and sub-step arithmetic, FP64 times, the FP64 walk's random draws).  The
+plain FP64 walk runs the same timed workload for the speed-up; an untimed
+pass of both walks gives the FP32 errors on the plume mean and variance and
+on the concentration at 64 observation cells.  When a moment error exceeds
+0.5 % the value is the FP64 rate instead (error_gate).
 
-Prints one JSON line: particle-steps/s (accepted RW sub-steps, the unit the
-reference's TrackSubCell loop iterates, over the walk's wall time), the
-roofline of the walk kernel (algorithmic bytes = the SoA state a step reads
-and writes: 96 B per particle-step in FP64, 72 B in FP32; HBM bound, 8 TB/s)
-and the CPU baseline on this host's cores over the first interval of the
-same workload: the reference's own engine compiled from its sources
-(OpenMP, one rng stream per thread) on 180,000 particles when it is built,
-beside the oracle (OpenMP) on all particles.  The
-precision workload (config 2: 1e7 particles, 1000 steps, FP32-local and
-FP64, errors on moments and on the concentration at 64 observation cells)
-is reported as config2 (--no-config2 skips it).  The calibration workload
-(config 4: 18 zoned parameters, the pathwise adjoint of an observation
-misfit) is reported as adjoint with its cost against the forward pass and
-its check against central differences (--no-adjoint skips it).  The batched workload
-(256 ensemble members, about 6.3e7 particles, one walk over
-all members) is timed for --ensemble-intervals intervals and reported as
-ensemble.  Before
-the timed region the plume after the first interval (25 d, still inside the
-grid) is reconstructed with the gpkde bandwidth optimisation (--kde-loops,
-default 10, convergence tests on) and reported as kde_adaptive, beside one
-loop of the compiled reference on the same points when it is built.
+The roofline is that of the walk kernel over the timed region (HIP events
+on the engine stream): algorithmic bytes = the SoA state one particle-step
+reads and writes (96 B FP64, 72 B FP32-local) times the launch's
+particle-steps, HBM bound, 8 TB/s; traffic = the measured HBM bytes per
+launch of the committed rocprofv3 summary of this workload.  cpu_baseline =
+the reference's own tracking engine compiled from its sources (OpenMP, one
+rng stream per thread) on the first interval of 180,000 of the particles,
+beside the oracle (OpenMP) on 1e6.
+
+Further legs of the same line: config1 (1e6 particles, 10 intervals, FP32
+and FP64 walk rates), kde_adaptive (gpkde bandwidth optimisation of the
+plume after one interval, beside one loop of the compiled reference),
+ensemble (256 members, about 6.3e7 particles, one walk over all members)
+and adjoint (the calibration workload: 18 zoned parameters, the pathwise
+gradient of an observation misfit, its cost and its check against central
+differences).  --quick leaves them out.
 """
 from __future__ import annotations
 
@@ -59,15 +57,22 @@ import numpy as np  # noqa: E402
 BYTES_PER_STEP = {0: 2 * (24 + 8 + 16), 1: 2 * (12 + 8 + 16), 2: 2 * (12 + 8 + 16)}
 HBM_PEAK_GBS = 8000.0
 INTERVAL_STEPS = 50
+RUN_INTERVALS = 20          # 1000 RW steps
 DT = 0.5
+HEADLINE_PARTICLES = 10_000_000
+ERROR_BOUND = 0.005
 PRECISION_NAME = {0: "plain FP64", 1: "FP32 cell-local, FP64 time, the FP64 walk's draws",
                   2: "FP32 cell-local, FP64 time, 4-draw normals"}
-TRAFFIC_FILE = {0: "walk_kernel_config1_traffic.txt",
-                1: "walk_kernel_fp32_config1_traffic.txt",
-                2: "walk_kernel_fp32n4_config1_traffic.txt"}
+TRAFFIC_FILE = {0: "walk_kernel_fp64_config2_traffic.txt",
+                1: "walk_kernel_fp32_config2_traffic.txt",
+                2: "walk_kernel_fp32n4_config2_traffic.txt"}
+TUNED_NAME = {0: "config2_fp64", 1: "config2_fp32", 2: "config2_fp32n4"}
+TUNED_NAME_1 = {0: "config1_fp64", 1: "config1_fp32", 2: "config1_fp32n4"}
 
 
 def workload(n_particles):
+    """The config-1 field and n_particles released uniformly in the inlet
+    block (columns 1-10, rows 76-125, all 20 layers)."""
     from rwpt import problems
     P = problems.config1()
     rel = problems.release_in_block(P, n_particles, (1, 10), (76, 125), (1, 20))
@@ -164,7 +169,6 @@ def adaptive_kde(E, origin, nbins, loops):
     return out
 
 
-TUNED_NAME = {0: "config1_fp64", 1: "config1_fp32", 2: "config1_fp32n4"}
 
 
 def observation_cells(P):
@@ -176,93 +180,19 @@ def observation_cells(P):
     return np.array([c + P.ncol * (r + P.nrow * k) for r in rows for c in cols])
 
 
-def config2_run(precision, report_steps=(50, 100, 200, 400, 1000)):
-    """The precision workload: the config-1 field with 1e7 particles and
-    1000 RW steps, run in FP32-local and in plain FP64.  Reports both walks'
-    throughput and the FP32 errors against FP64 on the plume mean and
-    variance (moments summed in FP64 over the active particles) and on the
-    KDE concentration at 64 observation cells, after the steps listed."""
-    from rwpt import Engine, problems
-    P = problems.config1()
-    rel = problems.release_in_block(P, 10_000_000, (1, 10), (76, 125), (1, 20))
-    nbins = (P.ncol, P.nrow, P.nlay)
-    origin = (0.0, 0.0, float(P.bot.min()))
-    obs = observation_cells(P)
-    out, snaps = {}, {}
-    for prec in (0, precision):
-        E = Engine(P, precision=prec)
-        tune = apply_tuning(E, TUNED_NAME[prec])
-        try:
-            E.load_particles(*rel)
-            walk = 0.0
-            steps = 0
-            res = []
-            done = 0
-            for target in report_steps:
-                t0 = time.perf_counter()
-                steps += E.advance(target * DT)[1]
-                walk += time.perf_counter() - t0
-                g = E.particles()
-                on = g["status"] == 1
-                gx, gy, gz = E.global_xyz()
-                x = np.stack([gx[on], gy[on], gz[on]])
-                dens, hist = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
-                res.append({"steps": target, "active": int(on.sum()),
-                            "mean": x.mean(axis=1) if on.any() else None,
-                            "var": x.var(axis=1) if on.any() else None,
-                            "obs": dens[obs], "obs_counts": hist[obs].astype(float)})
-                done = target
-            kms, launches = E.kernel_time(0)
-        finally:
-            E.close()
-        snaps[prec] = res
-        out["fp64" if prec == 0 else "fp32"] = {
-            "value": steps / walk, "particle_steps": steps, "walk_s": walk,
-            "walk_launches": launches, "walk_kernel_avg_launch_ms": kms / max(launches, 1),
-            "roofline_frac": BYTES_PER_STEP[prec] * steps / (kms / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "tuning": tune, "steps_run": done}
-    errs = []
-    for a, b in zip(snaps[0], snaps[precision]):
-        if a["active"] < 0.01 * rel[0].size:
-            continue  # the plume has left the grid: nothing left to compare
-        o = np.abs(b["obs"] - a["obs"])
-        ca, cb = a["obs_counts"], b["obs_counts"]
-        e = {"steps": a["steps"], "active_fp64": a["active"], "active_fp32": b["active"],
-             "mean_rel_err": float(np.max(np.abs(b["mean"] - a["mean"]) / np.abs(a["mean"]))),
-             "variance_rel_err": float(np.max(np.abs(b["var"] - a["var"]) / a["var"])),
-             "obs_particles_fp64": int(ca.sum())}
-        if ca.sum() >= 100:  # else the plume has passed the observation cells
-            e.update({"obs_conc_rel_err_mean": float(o.sum() / a["obs"].sum()),
-                      "obs_conc_rel_err_max": float(o.max() / a["obs"].max()),
-                      "obs_count_rel_l1": float(np.abs(cb - ca).sum() / ca.sum()),
-                      "obs_count_noise_l1": float(np.sqrt(2.0 / np.pi) *
-                                                  np.sqrt(ca + cb).sum() / ca.sum())})
-        errs.append(e)
-    out["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
-    out["errors"] = errs
-    out["workload"] = ("config-1 field, 1e7 particles in the inlet block, 1000 RW steps of "
-                       "0.5 d; errors of the FP32-local walk against the FP64 walk while "
-                       "at least 1 % of the particles are in the grid (observation "
-                       "cells while they hold at least 100 particles; "
-                       "concentrations: relative L1 over the 64 cells and largest cell "
-                       "difference over the largest FP64 value; particle counts in the "
-                       "64 cells: relative L1 difference beside the expected relative L1 "
-                       "difference of two independent walks, sqrt(2/pi) sum "
-                       "sqrt(n64 + n32) / sum n64, the Monte Carlo noise floor)")
-    return out
-
-
-def apply_tuning(E, name):
+def apply_tuning(E, name, fallback=None):
     """The walk's launch shape for this workload as the autotuner
-    (python -m rwpt.autotune, rwpt_autotune) stored it; defaults when the
-    workload has no entry."""
+    (python -m rwpt.autotune, rwpt_autotune) stored it; the fallback
+    workload's shape, else the defaults, when it has no entry."""
     from rwpt import autotune
-    t = autotune.tuned(name)
-    if t is None:
-        return {"source": "defaults"}
-    E.set_tuning(budget=t["budget"], block=t["block"], sort=bool(t["sort"]),
-                 min_waves=t["min_waves"])
-    return dict(t, source="modpath-rw_amd/rwpt/tuning.json (rwpt_autotune)")
+    for key in (name, fallback):
+        t = autotune.tuned(key) if key else None
+        if t is not None:
+            E.set_tuning(budget=t["budget"], block=t["block"], sort=bool(t["sort"]),
+                         min_waves=t["min_waves"])
+            return dict(t, source="modpath-rw_amd/rwpt/tuning.json (rwpt_autotune)",
+                        workload=key)
+    return {"source": "defaults"}
 
 
 def ensemble_run(intervals, warmup):
@@ -329,7 +259,7 @@ def adjoint_run(n=1_000_000):
         Q.porosity = P.porosity * s
         E = Engine(Q)
         try:
-            apply_tuning(E, TUNED_NAME[0])
+            apply_tuning(E, TUNED_NAME_1[0])
             E.load_particles(*rel)
             out = E.adjoint(windows, obs, np.zeros(len(obs)) if target is None else target,
                             bw, gradient=grad)
@@ -398,50 +328,56 @@ def adjoint_run(n=1_000_000):
                                            "practice over this horizon"}}
 
 
-def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
-    """warmup intervals, then `steps` timed intervals from the initial state,
-    each followed by the gpkde reconstruction on the flow grid; with
-    adaptive_loops > 0, one adaptive reconstruction of the plume after the
-    first interval is timed before the timed region."""
+
+
+def timed_run(P, rel, precision, steps, warmup, tuning_names, kde=True,
+              adaptive_loops=0):
+    """`warmup` intervals, then `steps` timed intervals from the release,
+    each one rwpt_advance by 50 RW steps followed (kde) by one gpkde density
+    pass of the active particles on the flow grid.  After RUN_INTERVALS
+    intervals the run starts again from the release.  With adaptive_loops
+    > 0 the gpkde optimisation is measured on the plume after the first
+    interval before the timed region."""
     from rwpt import Engine
     E = Engine(P, precision=precision)
-    tune = apply_tuning(E, TUNED_NAME[precision])
-    cell, lx, ly, lz = rel
+    tune = apply_tuning(E, tuning_names[precision], TUNED_NAME_1[precision])
     nbins = (P.ncol, P.nrow, P.nlay)
     origin = (0.0, 0.0, float(P.bot.min()))
-    try:
-        E.load_particles(cell, lx, ly, lz)
+
+    def restart():
+        E.load_particles(*rel)
         E.set_time(0.0)
-        warm_steps = 0
-        t_now = 0.0
+
+    try:
+        restart()
         for w in range(warmup):
-            t_now = (w + 1) * INTERVAL_STEPS * DT
-            warm_steps += E.advance(t_now)[1]
+            j = w % RUN_INTERVALS
+            if j == 0 and w > 0:
+                restart()
+            E.advance((j + 1) * INTERVAL_STEPS * DT)
         adaptive = None
         if adaptive_loops > 0:
             # the plume after the first tracking interval, still inside the grid
-            E.load_particles(cell, lx, ly, lz)
-            E.set_time(0.0)
-            t_now = INTERVAL_STEPS * DT
-            warm_steps += E.advance(t_now)[1]
+            restart()
+            E.advance(INTERVAL_STEPS * DT)
             adaptive = adaptive_kde(E, origin, nbins, adaptive_loops)
-        warm_ms, warm_launches = E.kernel_time(0)
-        E.load_particles(cell, lx, ly, lz)
-        E.set_time(0.0)
+        restart()
         E.reset_kernel_time()
         total_steps = 0
         walk_s = kde_s = 0.0
-        dens = []
         t0 = time.perf_counter()
         for k in range(steps):
+            j = k % RUN_INTERVALS
+            if j == 0 and k > 0:
+                restart()
             ta = time.perf_counter()
-            _, ns = E.advance((k + 1) * INTERVAL_STEPS * DT)
+            _, ns = E.advance((j + 1) * INTERVAL_STEPS * DT)
             tb = time.perf_counter()
-            d, _ = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
+            if kde:
+                E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
             kde_s += time.perf_counter() - tb
             walk_s += tb - ta
             total_steps += ns
-            dens.append(d)
         el = time.perf_counter() - t0
         kms, launches = E.kernel_time(0)
         hms, _ = E.kernel_time(1)
@@ -453,116 +389,169 @@ def timed_run(P, rel, precision, steps, warmup, adaptive_loops=0):
     return {"adaptive": adaptive, "tuning": tune,
             "steps": total_steps, "walk_s": walk_s, "kde_s": kde_s, "elapsed": el,
             "kernel_ms": kms, "launches": launches, "hist_ms": hms, "dens_ms": dms,
-            "bandwidth": bw, "dens": dens, "active": active,
-            "rate": total_steps / walk_s,
-            "all": [warm_ms + kms, warm_launches + launches, warm_steps + total_steps]}
+            "bandwidth": bw, "active": active,
+            "rate": total_steps / el, "walk_rate": total_steps / walk_s}
 
 
-def fp32_errors(P, rel, precision, runs, intervals=2):
-    """Errors of the FP32-local walk against the FP64 walk (untimed): plume
-    mean and variance of the active particles after each of the first
-    `intervals` intervals, the fraction of particles in the same cell with
-    the same status, and the relative L1 difference of the KDE densities of
-    the timed runs over the intervals with at least half the plume active."""
-    from oracle import oracle as O
+def precision_errors(P, rel, precision, report_steps=(50, 100, 200, 400)):
+    """Errors of the FP32-local walk against the plain FP64 walk on the
+    same particles and draws (untimed): plume mean and variance of the
+    active particles (moments summed in FP64) and the gpkde concentration
+    at 64 observation cells after each of report_steps RW steps while at
+    least 1 % of the particles are in the grid.  The same quantities
+    between two FP64 walks with independent random streams give the
+    Monte Carlo floor of the concentration differences: FP32 and FP64
+    trajectories follow the same draws but part ways at cell crossings."""
     from rwpt import Engine
+    nbins = (P.ncol, P.nrow, P.nlay)
+    origin = (0.0, 0.0, float(P.bot.min()))
+    obs = observation_cells(P)
+    n = rel[0].size
     snaps = {}
-    for prec in (0, precision):
+    for key, prec, offset in (("fp64", 0, 0), ("fp32", precision, 0), ("fp64_indep", 0, n)):
         E = Engine(P, precision=prec)
-        apply_tuning(E, TUNED_NAME[prec])
+        apply_tuning(E, TUNED_NAME[prec], TUNED_NAME_1[prec])
         try:
-            E.load_particles(*rel)
-            snaps[prec] = []
-            steps = 0
-            for k in range(intervals):
-                steps += E.advance((k + 1) * INTERVAL_STEPS * DT)[1]
-                snaps[prec].append(E.particles())
-            kms, launches = E.kernel_time(0)
-            runs[prec]["all"] = [runs[prec]["all"][0] + kms, runs[prec]["all"][1] + launches,
-                                 runs[prec]["all"][2] + steps]
+            E.load_particles(*rel, stream_offset=offset)
+            res = []
+            for target in report_steps:
+                E.advance(target * DT)
+                g = E.particles()
+                on = g["status"] == 1
+                gx, gy, gz = E.global_xyz()
+                x = np.stack([gx[on], gy[on], gz[on]])
+                dens, hist = E.kde(origin, (1.0, 1.0, 1.0), nbins, mode=1)
+                res.append({"steps": target, "active": int(on.sum()),
+                            "mean": x.mean(axis=1) if on.any() else None,
+                            "var": x.var(axis=1) if on.any() else None,
+                            "obs": dens[obs], "obs_counts": hist[obs].astype(float)})
         finally:
             E.close()
-    em = ev = 0.0
-    same = []
-    for k in range(intervals):
-        mom = []
-        for prec in (0, precision):
-            g = snaps[prec][k]
-            on = g["status"] == 1
-            x = np.stack(O.global_xyz(P, g["cell"][on], g["lx"][on], g["ly"][on], g["lz"][on]))
-            mom.append((x.mean(axis=1), x.var(axis=1)))
-        em = max(em, float(np.max(np.abs(mom[1][0] - mom[0][0]) / np.abs(mom[0][0]))))
-        ev = max(ev, float(np.max(np.abs(mom[1][1] - mom[0][1]) / mom[0][1])))
-        g0, g1 = snaps[0][k], snaps[precision][k]
-        same.append(float(((g0["cell"] == g1["cell"]) & (g0["status"] == g1["status"])).mean()))
-    n = rel[0].size
-    l1 = [float(np.abs(d1 - d0).sum() / d0.sum())
-          for d0, d1 in zip(runs[0]["dens"], runs[precision]["dens"])
-          if d0.sum() >= 0.5 * n]
-    return {"moment_intervals": intervals, "mean_rel_err_max": em,
-            "variance_rel_err_max": ev, "same_cell_status_fraction": same,
-            "kde_rel_l1_max": max(l1) if l1 else None,
-            "kde_intervals_compared": len(l1)}
+        snaps[key] = res
+    errs = []
+    for a, b, c in zip(snaps["fp64"], snaps["fp32"], snaps["fp64_indep"]):
+        if a["active"] < 0.01 * n:
+            continue  # the plume has left the grid: nothing left to compare
+        e = {"steps": a["steps"], "active_fp64": a["active"], "active_fp32": b["active"],
+             "mean_rel_err": float(np.max(np.abs(b["mean"] - a["mean"]) / np.abs(a["mean"]))),
+             "variance_rel_err": float(np.max(np.abs(b["var"] - a["var"]) / a["var"])),
+             "mean_rel_diff_indep": float(np.max(np.abs(c["mean"] - a["mean"]) /
+                                                 np.abs(a["mean"]))),
+             "variance_rel_diff_indep": float(np.max(np.abs(c["var"] - a["var"]) / a["var"])),
+             "obs_particles_fp64": int(a["obs_counts"].sum())}
+        if a["obs_counts"].sum() >= 100:  # else the plume has passed the observation cells
+            e.update({
+                "obs_conc_rel_l1": float(np.abs(b["obs"] - a["obs"]).sum() / a["obs"].sum()),
+                "obs_conc_rel_l1_indep": float(np.abs(c["obs"] - a["obs"]).sum() /
+                                               a["obs"].sum()),
+                "obs_conc_rel_max": float(np.abs(b["obs"] - a["obs"]).max() / a["obs"].max()),
+                "obs_count_rel_l1": float(np.abs(b["obs_counts"] - a["obs_counts"]).sum() /
+                                          a["obs_counts"].sum()),
+                "obs_count_rel_l1_indep": float(np.abs(c["obs_counts"] - a["obs_counts"]).sum()
+                                                / a["obs_counts"].sum())})
+        errs.append(e)
+    worst = max((max(e["mean_rel_err"], e["variance_rel_err"]) for e in errs), default=None)
+    gate = {"bound": ERROR_BOUND, "quantities": "plume mean and variance (concentration "
+                                                "moments) of the FP32-local walk vs FP64",
+            "worst": worst, "passed": worst is not None and worst <= ERROR_BOUND}
+    return {"errors": errs, "error_gate": gate,
+            "how": "FP32-local vs FP64 walk on the same particles and draws after 50, 100, "
+                   "200, 400 RW steps (while >= 1 % of the particles are in the grid); "
+                   "*_indep: FP64 vs FP64 with independent random streams (stream offset "
+                   "1e7), the Monte Carlo floor of each difference; concentrations: one "
+                   "gpkde density pass at 64 observation cells (8 x 8, middle layer), "
+                   "relative L1 over the 64 cells while they hold >= 100 particles"}
 
 
-def main():
+def config1_run(steps=10):
+    """The config-1 workload (1e6 particles, 10 intervals): walk rates of
+    the FP32-local and the FP64 walk."""
+    P, rel = workload(1_000_000)
+    out = {}
+    for prec in (1, 0):
+        r = timed_run(P, rel, prec, steps, 1, TUNED_NAME_1, kde=False)
+        out["fp32" if prec else "fp64"] = {
+            "value": r["rate"], "particle_steps": r["steps"], "elapsed_s": r["elapsed"],
+            "walk_kernel_avg_launch_ms": r["kernel_ms"] / max(r["launches"], 1),
+            "launches": r["launches"],
+            "roofline_frac": BYTES_PER_STEP[prec] * r["steps"] / (r["kernel_ms"] / 1e3)
+            / 1e9 / HBM_PEAK_GBS}
+    out["speedup_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["workload"] = "config-1 field, 1e6 particles in the inlet block, 10 intervals of 50 RW steps"
+    return out
+
+
+def read_traffic(prec):
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE[prec])
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":")[1]), TRAFFIC_FILE[prec]
+    return None, "not profiled for this precision"
+
+
+def parse_args(argv=None):
+    """The command line; exits with status 2 on a request this build cannot
+    meet (more than one GPU, no timed step)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--steps", type=int, default=RUN_INTERVALS,
+                    help="timed tracking intervals of 50 RW steps (20 = the 1000-step "
+                         "run; more restart from the release)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=int, default=HEADLINE_PARTICLES)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
-                    help="particles of the CPU baseline's first interval")
+                    help="particles of the oracle's CPU-baseline interval")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", type=int, default=1, choices=(0, 1, 2),
                     help="walk arithmetic: 0 plain FP64, 1 FP32-local with the FP64 "
                          "walk's draws (default), 2 FP32-local with 4-draw normals")
     ap.add_argument("--no-fp64", action="store_true",
-                    help="skip the plain FP64 comparison run")
+                    help="skip the plain FP64 comparison run and the error pass")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline only: no config1, ensemble, adjoint or adaptive "
+                         "gpkde legs")
     ap.add_argument("--ensemble-intervals", type=int, default=1,
-                    help="tracking intervals of the 256-member ensemble timed "
-                         "(0: skip)")
-    ap.add_argument("--no-config2", action="store_true",
-                    help="skip the 1e7-particle, 1000-step precision workload")
+                    help="tracking intervals of the 256-member ensemble timed (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint (calibration) workload")
     ap.add_argument("--kde-loops", type=int, default=10,
-                    help="optimisation loops of the adaptive gpkde measured on the "
-                         "final plume (0: skip)")
-    args = ap.parse_args()
+                    help="optimisation loops of the adaptive gpkde (0: skip)")
+    args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.steps < 1 or args.steps > 10:
-        print("--steps must be 1..10 (the workload has 10 tracking intervals)",
-              file=sys.stderr)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
         sys.exit(2)
+    if args.quick:
+        args.ensemble_intervals, args.no_adjoint, args.kde_loops = 0, True, 0
+    return args
 
+
+def main(argv=None):
+    args = parse_args(argv)
     P, rel = workload(args.particles)
     prec = args.precision
-    runs = {prec: timed_run(P, rel, prec, args.steps, args.warmup,
-                            adaptive_loops=args.kde_loops)}
+    print(f"[bench] headline: {args.particles} particles, precision {prec}, "
+          f"{args.warmup} warm-up + {args.steps} timed intervals", file=sys.stderr, flush=True)
+    r = timed_run(P, rel, prec, args.steps, args.warmup, TUNED_NAME,
+                  adaptive_loops=args.kde_loops)
+    r0 = errors = None
     if prec != 0 and not args.no_fp64:
-        runs[0] = timed_run(P, rel, 0, args.steps, args.warmup)
-    r = runs[prec]
-    value = r["rate"]
-    errors = None
-    if 0 in runs and prec != 0:
-        errors = fp32_errors(P, rel, prec, runs)
-    # the roofline averages over every launch of this walk in the process
-    # (warm-up, timed and error passes), as a profiler's per-kernel
-    # statistics of the same command do
-    all_ms, all_launches, all_steps = r["all"]
-    avg_launch_s = (all_ms / 1e3) / max(all_launches, 1)
-    bytes_per_launch = BYTES_PER_STEP[prec] * all_steps / max(all_launches, 1)
+        print("[bench] FP64 comparison run", file=sys.stderr, flush=True)
+        r0 = timed_run(P, rel, 0, args.steps, args.warmup, TUNED_NAME)
+        print("[bench] FP32 vs FP64 error pass", file=sys.stderr, flush=True)
+        errors = precision_errors(P, rel, prec)
+    value, value_from = r["rate"], PRECISION_NAME[prec]
+    if errors is not None and not errors["error_gate"]["passed"]:
+        value, value_from = r0["rate"], PRECISION_NAME[0] + " (FP32 error gate failed)"
+    # roofline of the walk kernel over the timed region's launches
+    avg_launch_s = (r["kernel_ms"] / 1e3) / max(r["launches"], 1)
+    bytes_per_launch = BYTES_PER_STEP[prec] * r["steps"] / max(r["launches"], 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE[prec])
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("hbm_bytes_per_launch:"):
-                traffic = float(line.split(":")[1])
+    traffic, traffic_src = read_traffic(prec)
     nbins = (P.ncol, P.nrow, P.nlay)
     kde_cells = int(np.prod(nbins))
     out = {
@@ -580,23 +569,24 @@ def main():
         "dtype": "f64" if prec == 0 else "f32",
         "data": "synthetic (seeded lognormal field, curl-of-potential fluxes)",
         "config": {
-            "workload": "heterogeneous 200x200x20 lognormal field, 1e6 particles, "
-                        "FIXED dt 0.5 d, 50 RW steps per tracking interval, gpkde "
-                        "reconstruction per interval",
+            "workload": "config 2: heterogeneous 200x200x20 lognormal field, %d particles "
+                        "in the 10x50x20-cell inlet block, FIXED dt 0.5 d, one step = one "
+                        "tracking interval of 50 RW steps + one gpkde density pass, 20 "
+                        "intervals = 1000 RW steps" % args.particles,
             "grid": [200, 200, 20], "particles": args.particles,
             "rw_steps_per_interval": INTERVAL_STEPS, "dt_days": DT,
             "precision": PRECISION_NAME[prec], "advection": "EULERIAN",
         },
+        "value_from": value_from,
         "particle_steps": r["steps"],
+        "walk_only_rate": r["walk_rate"],
         "active_after": r["active"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "walk_kernel",
-                     "launches": all_launches,
+                     "launches": r["launches"],
                      "avg_launch_ms": avg_launch_s * 1e3,
-                     "timed_launches": r["launches"],
-                     "timed_avg_launch_ms": r["kernel_ms"] / max(r["launches"], 1),
                      "algorithmic_bytes_per_step": BYTES_PER_STEP[prec],
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "tuning": r["tuning"],
@@ -612,21 +602,27 @@ def main():
     }
     if r["adaptive"] is not None:
         out["kde_adaptive"] = r["adaptive"]
-    if 0 in runs and prec != 0:
-        b0 = runs[0]
-        out["fp64"] = {"value": b0["rate"], "ms_per_step": b0["elapsed"] * 1e3 / args.steps,
-                       "particle_steps": b0["steps"],
-                       "walk_kernel_avg_launch_ms": b0["kernel_ms"] / max(b0["launches"], 1),
-                       "launches": b0["launches"]}
-        out["speedup_vs_fp64"] = value / b0["rate"]
+    if r0 is not None:
+        out["fp64"] = {"value": r0["rate"], "walk_only_rate": r0["walk_rate"],
+                       "ms_per_step": r0["elapsed"] * 1e3 / args.steps,
+                       "particle_steps": r0["steps"],
+                       "walk_kernel_avg_launch_ms": r0["kernel_ms"] / max(r0["launches"], 1),
+                       "launches": r0["launches"],
+                       "roofline_frac": BYTES_PER_STEP[0] * r0["steps"] /
+                       (r0["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+        out["speedup_vs_fp64"] = r["rate"] / r0["rate"]
         out["error_vs_fp64"] = errors
-    if not args.no_config2 and prec != 0:
-        out["config2"] = config2_run(prec)
+    if not args.quick:
+        print("[bench] config1 leg", file=sys.stderr, flush=True)
+        out["config1"] = config1_run()
     if args.ensemble_intervals > 0:
+        print("[bench] ensemble leg", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
     if not args.no_adjoint:
+        print("[bench] adjoint leg", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_run()
     if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)
     print(json.dumps(out), flush=True)
 

@@ -22,10 +22,10 @@ INTERVAL_STEPS = 50
 DT = 0.5
 
 
-def _config1(precision):
+def _config1(precision, n=1_000_000):
     from . import problems
     P = problems.config1()
-    rel = problems.release_in_block(P, 1_000_000, (1, 10), (76, 125), (1, 20))
+    rel = problems.release_in_block(P, n, (1, 10), (76, 125), (1, 20))
     return P, rel, precision, INTERVAL_STEPS
 
 
@@ -39,6 +39,8 @@ def _ensemble(precision):
 WORKLOADS = {
     "config1_fp64": lambda: _config1(0),
     "config1_fp32": lambda: _config1(1),
+    "config2_fp64": lambda: _config1(0, 10_000_000),
+    "config2_fp32": lambda: _config1(1, 10_000_000),
     "ensemble256_fp32": lambda: _ensemble(1),
 }
 
