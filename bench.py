@@ -517,6 +517,9 @@ def parse_args(argv=None):
                     help="skip the adjoint (calibration) workload")
     ap.add_argument("--kde-loops", type=int, default=10,
                     help="optimisation loops of the adaptive gpkde (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device 3 times (the known-size "
+                         "workload that calibrates rocprofv3's FETCH_SIZE and WRITE_SIZE)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -530,8 +533,36 @@ def parse_args(argv=None):
     return args
 
 
+def copy_probe(gib=1, reps=3):
+    """A plain device-to-device copy of `gib` GiB, `reps` times (HIP's own
+    copy kernel): the known-size workload that scales the FETCH_SIZE and
+    WRITE_SIZE counters of a profile to HBM bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(gib << 30)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, n) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, n, 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    print(json.dumps({"copy_probe": f"{reps} x {gib} GiB device-to-device"}), flush=True)
+
+
 def main(argv=None):
     args = parse_args(argv)
+    if args.copy_probe:
+        copy_probe()
+        return
     P, rel = workload(args.particles)
     prec = args.precision
     print(f"[bench] headline: {args.particles} particles, precision {prec}, "

@@ -11,7 +11,7 @@ module RwptGpuModule
   public :: rwpt_default_options, rwpt_create, rwpt_destroy, rwpt_last_error
   public :: rwpt_set_flow, rwpt_set_cell_extras, rwpt_set_dispersion
   public :: rwpt_load_particles, rwpt_get_particles, rwpt_particle_count
-  public :: rwpt_advance, rwpt_set_time, rwpt_global_xyz, rwpt_kde, rwpt_kde_info
+  public :: rwpt_advance, rwpt_set_time, rwpt_set_steady_state, rwpt_global_xyz, rwpt_kde, rwpt_kde_info
   public :: rwpt_kde_write, rwpt_get_substeps, rwpt_set_tuning, rwpt_last_launches
   public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
   public :: rwpt_create_ensemble, rwpt_autotune, rwpt_write_timeseries_header
@@ -110,6 +110,12 @@ module RwptGpuModule
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: e
       real(c_double), value :: t
+    end function
+    integer(c_int) function rwpt_set_steady_state(e, steady_state) &
+        bind(C, name='rwpt_set_steady_state')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: steady_state
     end function
     integer(c_int) function rwpt_advance(e, t_stop, n_active, n_substeps) &
         bind(C, name='rwpt_advance')

@@ -201,6 +201,15 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
  * `time` of MPathRW.F90:1256-1262 used for the release test). */
 int rwpt_set_time(rwpt_engine *e, double t);
 
+/* FlowModelData%SteadyState of the time step whose flow is loaded: the
+ * reference copies it into TrackCell%SteadyState at every time step
+ * (ParticleTrackingEngine.f90:893, 1091), after FlowModelData%LoadTimeStep
+ * has set it from the step's storage flows.  It decides, with stop at weak
+ * sinks (forward) or weak sources (backward), whether a cell without an
+ * exit face stops its particles (TrackCell.f90:455-497).  rwpt_create
+ * takes the first value from rwpt_options.steady_state. */
+int rwpt_set_steady_state(rwpt_engine *e, int32_t steady_state);
+
 /* Launch shape of the walk (no counterpart in the reference: the
  * reference walks each particle to t_stop in one call).  budget = RW
  * iterations per particle per launch (the loop state is saved between

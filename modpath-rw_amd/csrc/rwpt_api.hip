@@ -650,6 +650,15 @@ int rwpt_set_time(rwpt_engine *e, double t) {
   return RWPT_OK;
 }
 
+int rwpt_set_steady_state(rwpt_engine *e, int32_t steady_state) {
+  if (!e) return RWPT_E_ARG;
+  if (steady_state != 0 && steady_state != 1)
+    return fail(e, RWPT_E_ARG, "steady_state must be 0 or 1");
+  // read by the walk's entry checks only: the cell tables do not change
+  e->opt.steady_state = steady_state;
+  return RWPT_OK;
+}
+
 int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz) {
   if (!e || !gx || !gy || !gz) return RWPT_E_ARG;
   const int64_t n = e->part.n;

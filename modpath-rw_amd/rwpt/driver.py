@@ -108,9 +108,9 @@ class Driver:
                  kde_column_format=0, flow_for_step=None):
         """flow_for_step: None (the engine's flow holds for every step) or a
         callable of the 1-based cumulative time step returning a Problem
-        (its face flows, boundary flows, porosity, retardation and ibound
-        are set) and the keyword arguments of Engine.set_cell_extras (or
-        None)."""
+        (its face flows, boundary flows, porosity, retardation, ibound and
+        steady-state flag are set) and the keyword arguments of
+        Engine.set_cell_extras (or None)."""
         self.E = engine
         self.flow_for_step = flow_for_step
         self.intervals = list(intervals)
@@ -140,6 +140,9 @@ class Driver:
                 self.E.set_flow(P)
                 if extras:
                     self.E.set_cell_extras(**extras)
+                # TrackCell%SteadyState follows the loaded step
+                # (ParticleTrackingEngine.f90:893, 1091)
+                self.E.set_steady_state(getattr(P, "steady_state", 1))
                 loaded = iv.ktime
             active, steps = self.E.advance(iv.time)
             st = self.E.particles()["status"]

@@ -78,6 +78,7 @@ EXPORTS = {
     "rwpt_particle_count": ([C.c_void_p], C.c_int64),
     "rwpt_advance": ([C.c_void_p, C.c_double, _i64p, _i64p], C.c_int),
     "rwpt_set_time": ([C.c_void_p, C.c_double], C.c_int),
+    "rwpt_set_steady_state": ([C.c_void_p, C.c_int32], C.c_int),
     "rwpt_set_tuning": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
                         C.c_int),
     "rwpt_last_launches": ([C.c_void_p], C.c_int),
@@ -205,12 +206,17 @@ class Engine:
         arrs = [_d(P.delr), _d(P.delc), _d(P.top)[:ncm].copy(), _d(P.bot)[:ncm].copy()]
         self._keep += arrs
         g.delr, g.delc, g.top, g.bot = [_p(a, _dp) for a in arrs]
+        stops = {k: int(getattr(P, a, d)) for k, a, d in (
+            ("stop_zone", "stop_zone", 0), ("stop_at_weak_sinks", "stop_weak_sinks", 0),
+            ("stop_at_weak_sources", "stop_weak_sources", 0),
+            ("steady_state", "steady_state", 1))}
+        stops.update(opt)
         o = default_options(advection_kind=P.advection_kind,
                             time_step_kind=P.time_step_kind,
                             time_step_params=P.ts_params,
                             dimension_mask=P.dim_mask,
                             backward_tracking=P.backward, precision=precision,
-                            **opt)
+                            **stops)
         h = C.c_void_p()
         rc = lib().rwpt_create_ensemble(C.byref(g), C.byref(o), int(self.members),
                                         C.byref(h))
@@ -221,6 +227,8 @@ class Engine:
         self.ncell_total = ncm * self.members
         self.set_flow(P)
         self.set_dispersion(P)
+        if getattr(P, "has_extras", False):
+            self.set_cell_extras(P.head, P.layer_type, P.zone, P.source_flow, P.sink_flow)
 
     # -- error handling -------------------------------------------------------
     def _check(self, rc, what):
@@ -281,6 +289,12 @@ class Engine:
     # -- stepping -------------------------------------------------------------
     def set_time(self, t):
         self._check(lib().rwpt_set_time(self.h, float(t)), "rwpt_set_time")
+
+    def set_steady_state(self, steady):
+        """rwpt_set_steady_state: FlowModelData%SteadyState of the loaded
+        time step."""
+        self._check(lib().rwpt_set_steady_state(self.h, int(bool(steady))),
+                    "rwpt_set_steady_state")
 
     def set_tuning(self, budget=64, block=256, sort=True, min_waves=1):
         self._check(lib().rwpt_set_tuning(self.h, int(budget), int(block), int(sort),

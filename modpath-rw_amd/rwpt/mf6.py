@@ -488,7 +488,7 @@ def problem(grb: Grb, ts: TimeStepData, porosity, retardation=None, **options):
                 porosity=np.broadcast_to(np.asarray(porosity, np.float64), (n,)).copy(),
                 retardation=None if retardation is None else
                 np.broadcast_to(np.asarray(retardation, np.float64), (n,)).copy(),
-                ibound=a["ibound"], **options)
+                ibound=a["ibound"], steady_state=int(bool(ts.steady_state)), **options)
     extras = dict(head=a["head"], layer_type=a["layer_type"], source_flow=a["source_flow"],
                   sink_flow=a["sink_flow"])
     return P, extras

@@ -49,6 +49,24 @@ class Problem:
     ts_params: tuple = (0.25, 0.0)
     dim_mask: tuple = (1, 1, 1)
     backward: int = 0
+    # cell extras (rwpt_set_cell_extras; None: not given): heads and layer
+    # types of convertible layers, zones, weak source and sink flows
+    head: Optional[np.ndarray] = None
+    layer_type: Optional[np.ndarray] = None
+    zone: Optional[np.ndarray] = None
+    source_flow: Optional[np.ndarray] = None
+    sink_flow: Optional[np.ndarray] = None
+    # stopping options (ParticleTrackingOptions.f90:20-31) and
+    # FlowModelData%SteadyState
+    stop_zone: int = 0
+    stop_weak_sinks: int = 0
+    stop_weak_sources: int = 0
+    steady_state: int = 1
+
+    @property
+    def has_extras(self) -> bool:
+        return any(a is not None for a in (self.head, self.layer_type, self.zone,
+                                           self.source_flow, self.sink_flow))
 
     @property
     def ncell(self) -> int:

@@ -116,11 +116,11 @@ class OracleProblem:
         s.bflow = dd("bflow", P.bflow)
         s.porosity = dd("porosity", P.porosity)
         s.retardation = dd("retardation", P.retardation)
-        s.head = None
-        s.layer_type = None
-        s.source_flow = None
-        s.sink_flow = None
-        s.zone = None
+        s.head = dd("head", getattr(P, "head", None))
+        s.layer_type = ii("layer_type", getattr(P, "layer_type", None))
+        s.source_flow = dd("source_flow", getattr(P, "source_flow", None))
+        s.sink_flow = dd("sink_flow", getattr(P, "sink_flow", None))
+        s.zone = ii("zone", getattr(P, "zone", None))
         s.icbound = ii("icbound", P.icbound)
         s.default_icbound = P.default_icbound
         s.disp_model = P.disp_model
@@ -137,10 +137,10 @@ class OracleProblem:
             s.dim_mask[d] = P.dim_mask[d]
         s.uniform_porosity = P.uniform_porosity
         s.backward = P.backward
-        s.stop_weak_sinks = 0
-        s.stop_weak_sources = 0
-        s.stop_zone = 0
-        s.steady_state = 1
+        s.stop_weak_sinks = int(getattr(P, "stop_weak_sinks", 0))
+        s.stop_weak_sources = int(getattr(P, "stop_weak_sources", 0))
+        s.stop_zone = int(getattr(P, "stop_zone", 0))
+        s.steady_state = int(getattr(P, "steady_state", 1))
         self.s = s
 
 

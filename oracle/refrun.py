@@ -49,7 +49,7 @@ def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600, threads=No
     hdr[15] = getattr(P, "stop_weak_sinks", 0)
     hdr[16] = getattr(P, "stop_weak_sources", 0)
     hdr[17] = getattr(P, "stop_zone", 0)
-    hdr[19] = 1
+    hdr[19] = int(getattr(P, "steady_state", 1))
     f8 = np.float64
     parts = [hdr, np.asarray(P.ts_params, f8),
              _filled(P.delr, P.ncol, 0, f8), _filled(P.delc, P.nrow, 0, f8),
@@ -57,10 +57,14 @@ def run(P, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600, threads=No
              _filled(P.frf, nc, 0, f8), _filled(P.fff, nc, 0, f8),
              _filled(P.flf, nc, 0, f8), _filled(P.bflow, 6 * nc, 0, f8),
              _filled(P.porosity, nc, 0, f8), _filled(P.retardation, nc, 1.0, f8),
-             _filled(P.top, nc, 0, f8),                     # heads at the cell top
-             np.zeros(nc, f8), np.zeros(nc, f8),            # source, sink flows
+             # heads at the cell top: the structured fill's layer type is an
+             # uninitialised local (ParticleTrackingEngine.f90:801-817), so
+             # heads must not matter here
+             _filled(P.top, nc, 0, f8),
+             _filled(getattr(P, "source_flow", None), nc, 0, f8),
+             _filled(getattr(P, "sink_flow", None), nc, 0, f8),
              _filled(P.ibound, nc, 1, np.int32), _filled(P.icbound, nc, 0, np.int32),
-             np.zeros(nc, np.int32),                        # zones
+             _filled(getattr(P, "zone", None), nc, 1, np.int32),
              _filled(P.aL, nd, 0, f8), _filled(P.aT, nd, 0, f8),
              _filled(P.aLV, nd, 0, f8), _filled(P.aTV, nd, 0, f8),
              _filled(P.dmeff, nd, 0, f8),

@@ -15,6 +15,56 @@ def hetero(**kw):
     return P
 
 
+def closed_cell(P, k, i, j, q=0.5, inward=True):
+    """Makes the face flows of cell (k, i, j) (1-based, interior) all point
+    into the cell (inward: no exit face in forward tracking, a sink) or all
+    out of it (no exit face in backward tracking, a source), with the
+    MODFLOW-2005 signs: frf = flow from column j to j+1, fff = row i to
+    i+1, flf = layer k to k+1."""
+    sg = 1.0 if inward else -1.0
+    c = P.cell_number(k, i, j) - 1
+    P.frf[c - 1] = sg * q               # left neighbour -> cell
+    P.frf[c] = -sg * q                  # right neighbour -> cell
+    P.fff[c - P.ncol] = sg * q          # row above -> cell
+    P.fff[c] = -sg * q                  # row below -> cell
+    P.flf[c - P.ncol * P.nrow] = sg * q  # layer above -> cell
+    P.flf[c] = -sg * q                  # layer below -> cell
+    return c
+
+
+def stopping(kind, **kw):
+    """The stopping branches of TrackCell.f90:433-497 on the heterogeneous
+    field: stop zones, weak sinks (forward) and weak sources (backward),
+    steady or transient flow, and cells with no exit face whose sink or
+    source flow is zero or not."""
+    P = hetero(**kw)
+    P.frf, P.fff, P.flf = P.frf.copy(), P.fff.copy(), P.flf.copy()
+    g = np.random.default_rng(11)
+    P.zone = np.ones(P.ncell, np.int32)
+    P.sink_flow = np.zeros(P.ncell)
+    P.source_flow = np.zeros(P.ncell)
+    if kind == "zone":
+        col = (np.arange(P.ncell) % P.ncol) + 1
+        P.zone[(col >= 11) & (col <= 12)] = 3
+        P.zone[g.random(P.ncell) < 0.03] = 2
+        P.stop_zone = 3
+        return P
+    backward = bool(P.backward)
+    flows = P.source_flow if backward else P.sink_flow
+    weak = g.random(P.ncell) < 0.04
+    flows[weak] = 0.02 if backward else -0.02
+    # cells without an exit face: every other one with a weak flow
+    cols = (4, 5, 6) if backward else (9, 10, 11, 12)
+    m = 0
+    for j in cols:
+        for i in range(7, 15, 2):
+            for k in (2, 4, 5):
+                c = closed_cell(P, k, i, j, inward=not backward)
+                flows[c] = (0.05 if backward else -0.05) if m % 2 else 0.0
+                m += 1
+    return P
+
+
 def _scenarios():
     s = {}
     s["cfg0"] = (problems.config0(), (1, 25, 10), 0.25, 12)
@@ -63,6 +113,16 @@ def _scenarios():
     P.top = np.repeat(tops, P.nrow * P.ncol)
     P.bot = np.repeat(tops - dz, P.nrow * P.ncol)
     s["nonuniform_spacing"] = (P, None, 0.5, 6)
+    # stopping conditions of TrackCell (statuses 3, 4, 5, 6)
+    s["stop_zone"] = (stopping("zone"), None, 0.5, 8)
+    s["weak_sinks"] = (stopping("sink", stop_weak_sinks=1), None, 0.5, 8)
+    s["weak_sources_backward"] = (stopping("source", backward=1, stop_weak_sources=1),
+                                  None, 0.5, 6)
+    s["transient_weak_sinks"] = (stopping("sink", stop_weak_sinks=1, steady_state=0),
+                                 None, 0.5, 8)
+    s["transient_no_exit"] = (stopping("sink", steady_state=0), None, 0.5, 8)
+    s["transient_weak_sources_backward"] = (
+        stopping("source", backward=1, stop_weak_sources=1, steady_state=0), None, 0.5, 6)
     return s
 
 

@@ -7,6 +7,8 @@ call exp/pow, whose last bit differs between the device and host math
 libraries, so they are compared with a tolerance on positions (1e-10
 relative) and on the fraction of particles whose path is unchanged.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -187,3 +189,37 @@ def test_walk_launch_shape_does_not_change_results(budget, block, sort, minw):
         E.close()
     for f in FIELDS:
         assert np.array_equal(g[f], st[f]), f
+
+
+def test_steady_state_follows_the_loaded_step():
+    """rwpt_set_steady_state: an engine created for steady flow and switched
+    to a transient step walks as the oracle does with steady_state 0 (cells
+    without an exit face keep their particles when stop-at-weak-sinks is on
+    and the sink flow is zero, TrackCell.f90:464-497), and differently from
+    the steady walk."""
+    P, where, dt_out, nout = SCEN["transient_weak_sinks"]
+    assert P.steady_state == 0
+    n = 1500
+    cell, lx, ly, lz = _release(P, where, n)
+    E = Engine(dataclasses.replace(P, steady_state=1))
+    try:
+        E.set_steady_state(0)
+        E.load_particles(cell, lx, ly, lz)
+        for k in range(1, nout + 1):
+            E.advance(k * dt_out)
+        g = E.particles()
+    finally:
+        E.close()
+    runs = {}
+    for steady in (0, 1):
+        st = O.new_state(cell, lx, ly, lz)
+        OP = O.OracleProblem(dataclasses.replace(P, steady_state=steady))
+        t = 0.0
+        for k in range(1, nout + 1):
+            assert O.advance(OP, st, t, k * dt_out) == 0
+            t = k * dt_out
+        runs[steady] = st
+    for f in FIELDS:
+        assert np.array_equal(g[f], runs[0][f], equal_nan=f in ("lx", "ly", "lz", "t")), f
+    assert (runs[1]["status"] == 5).sum() > 0 and (runs[0]["status"] == 5).sum() == 0
+

@@ -118,6 +118,13 @@ def test_time_step_arrays_bitwise(case):
     assert not ts.steady_state
 
 
+def test_problem_carries_the_steps_steady_state(case):
+    """FlowModelData%SteadyState of the read step goes to the engine's
+    Problem (the Driver hands it to rwpt_set_steady_state)."""
+    P, _ = mf6.problem(case["grb"], case["ts"], case["por"], case["ret"])
+    assert P.steady_state == int(case["ref"]["steady_state"]) == 0
+
+
 def test_face_flows_invert_flow_ja(case):
     """The MODFLOW-2005 face flows of the reader are the ones the files were
     written from."""

@@ -20,6 +20,7 @@ import pytest
 
 from oracle import oracle as O
 from rwpt import problems
+from scenarios import SCEN, release
 
 
 def _run(P, n, t_end, steps=1, seed=3, lz=0.5, cellkij=(1, 25, 10), rng=None):
@@ -301,3 +302,23 @@ def test_heterogeneous_field_is_divergence_free():
     Q = _assembled_faces(P)
     div = Q[..., 1] - Q[..., 0] + Q[..., 3] - Q[..., 2] + Q[..., 5] - Q[..., 4]
     assert np.abs(div).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["stop_zone", "weak_sinks", "weak_sources_backward",
+                                  "transient_weak_sinks", "transient_no_exit",
+                                  "transient_weak_sources_backward"])
+def test_stopping_scenarios_reach_their_statuses(name):
+    """The stopping scenarios stop particles with the status they exercise
+    (checked on the oracle, which the bit-exact tests tie to the device)."""
+    P, where, dt_out, nout = SCEN[name]
+    cell, lx, ly, lz = release(P, where, 1500)
+    st = O.new_state(cell, lx, ly, lz)
+    OP = O.OracleProblem(P)
+    t = 0.0
+    for k in range(1, nout + 1):
+        O.advance(OP, st, t, k * dt_out)
+        t = k * dt_out
+    want = {"stop_zone": 6, "weak_sinks": 3, "weak_sources_backward": 4,
+            "transient_weak_sinks": 3, "transient_no_exit": 5,
+            "transient_weak_sources_backward": 4}[name]
+    assert (st["status"] == want).sum() > 50
