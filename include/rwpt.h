@@ -256,6 +256,28 @@ int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz);
 int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
              double *density, double *histogram);
 
+/* Largest grid (bins) and 1-D kernel storage (weights over the active
+ * dimensions, sum of ceiling(3h/bin) + 1) of rwpt_kde_device. */
+#define RWPT_KDE_DEVICE_MAX_CELLS 4194304
+#define RWPT_KDE_DEVICE_MAX_TAPS 8192
+
+/* rwpt_kde without any device allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * one-pass reconstruction of rwpt_kde (n_opt_loops 0, mode 0 or 1) over the
+ * engine's particles with unit masses, written into caller-owned DEVICE
+ * buffers density_dev and histogram_dev of nbins[0]*nbins[1]*nbins[2]
+ * doubles (x fastest).  Results equal rwpt_kde's bit for bit.  Its scratch
+ * memory is a fixed device array of about 32 MiB per GPU, held until the
+ * process ends and shared by every engine on that GPU: calls that overlap
+ * on one GPU race.  Grids above RWPT_KDE_DEVICE_MAX_CELLS bins and kernels
+ * above RWPT_KDE_DEVICE_MAX_TAPS weights return RWPT_E_UNSUPPORTED;
+ * weighted particles (opt->weighted) and n_opt_loops > 0 return
+ * RWPT_E_UNSUPPORTED (their sorts and per-bin arrays need allocations).
+ * rwpt_kde_info reports the call like rwpt_kde. */
+int rwpt_kde_device(rwpt_engine *e, const rwpt_kde_options *opt, double *density_dev,
+                    double *histogram_dev);
+
 /* Quantities of the last rwpt_kde call: the Silverman bandwidth used, the
  * number of active particles passed, the histogram's nPoints, the
  * effective mass (Histogram.F90:332-341) and the optimisation loops run. */

@@ -155,6 +155,8 @@ int rwpt_create_ensemble(const rwpt_grid *grid, const rwpt_options *opt,
   rwpt_engine *e = new rwpt_engine();
   e->opt = *opt;
   hipError_t err = hipGetDevice(&e->device);
+  // rwpt_kde_device's scratch array on this device
+  if (err == hipSuccess) err = kde_device_scratch_init(e->device);
   if (err == hipSuccess) err = hipStreamCreate(&e->stream);
   if (err != hipSuccess) {
     int rc = hip_fail(e, err, "create");
@@ -709,9 +711,9 @@ int rwpt_reset_kernel_time(rwpt_engine *e) {
   return RWPT_OK;
 }
 
-int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
-             double *density, double *histogram) {
-  if (!e || !opt) return RWPT_E_ARG;
+namespace {
+// rwpt_kde's option checks; the bin count goes to *ncells_out
+int check_kde_options(rwpt_engine *e, const rwpt_kde_options *opt, int64_t *ncells_out) {
   if (opt->mode != 0 && opt->mode != 1)
     return fail(e, RWPT_E_ARG, "kde mode must be 0 or 1");
   if (opt->n_opt_loops < 0) return fail(e, RWPT_E_ARG, "n_opt_loops < 0");
@@ -745,6 +747,26 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
   }
   if (ncells >= 0x7fffffffLL) return fail(e, RWPT_E_ARG, "kde grid too large");
   if (!e->have_flow) return fail(e, RWPT_E_STATE, "flow not set");
+  *ncells_out = ncells;
+  return RWPT_OK;
+}
+
+void add_kde_times(rwpt_engine *e) {
+  for (int k = 1; k <= 2; ++k) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e->timers[k].start, e->timers[k].stop);
+    e->timers[k].total_ms += ms;
+    e->timers[k].launches += 1;
+  }
+}
+}  // namespace
+
+int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
+             double *density, double *histogram) {
+  if (!e || !opt) return RWPT_E_ARG;
+  int64_t ncells = 0;
+  int rc = check_kde_options(e, opt, &ncells);
+  if (rc != RWPT_OK) return rc;
   const int64_t n = e->part.n;
   double *dens = nullptr, *hist = nullptr, *m = nullptr;
   hipError_t err = dev_alloc(&dens, ncells);
@@ -768,18 +790,39 @@ int rwpt_kde(rwpt_engine *e, const rwpt_kde_options *opt, const double *mass,
   if (err == hipSuccess && histogram)
     err = hipMemcpyAsync(histogram, hist, ncells * 8, hipMemcpyDeviceToHost, e->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
-  if (err == hipSuccess) {
-    for (int k = 1; k <= 2; ++k) {
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e->timers[k].start, e->timers[k].stop);
-      e->timers[k].total_ms += ms;
-      e->timers[k].launches += 1;
-    }
-  }
+  if (err == hipSuccess) add_kde_times(e);
   dev_free(dens);
   dev_free(hist);
   dev_free(m);
   if (err != hipSuccess) return hip_fail(e, err, "kde");
+  return RWPT_OK;
+}
+
+int rwpt_kde_device(rwpt_engine *e, const rwpt_kde_options *opt, double *density_dev,
+                    double *histogram_dev) {
+  if (!e || !opt) return RWPT_E_ARG;
+  if (!density_dev || !histogram_dev)
+    return fail(e, RWPT_E_ARG, "rwpt_kde_device needs both device output buffers");
+  int64_t ncells = 0;
+  int rc = check_kde_options(e, opt, &ncells);
+  if (rc != RWPT_OK) return rc;
+  if (opt->n_opt_loops != 0)
+    return fail(e, RWPT_E_UNSUPPORTED, "rwpt_kde_device: one pass only (n_opt_loops 0)");
+  if (opt->weighted)
+    return fail(e, RWPT_E_UNSUPPORTED, "rwpt_kde_device: unit masses only");
+  if (ncells > RWPT_KDE_DEVICE_MAX_CELLS)
+    return fail(e, RWPT_E_UNSUPPORTED,
+                "rwpt_kde_device: grid above RWPT_KDE_DEVICE_MAX_CELLS bins");
+  if (e->part.n > 0 && !e->part.keys[0])
+    return fail(e, RWPT_E_STATE, "particles not loaded");
+  hipError_t err = launch_kde_device(e, opt, density_dev, histogram_dev, e->kde_last);
+  if (err == hipErrorInvalidConfiguration)
+    return fail(e, RWPT_E_UNSUPPORTED,
+                "rwpt_kde_device: kernel above RWPT_KDE_DEVICE_MAX_TAPS weights");
+  if (err == hipErrorNotInitialized)
+    return fail(e, RWPT_E_STATE, "rwpt_kde_device: scratch not set up on this device");
+  if (err != hipSuccess) return hip_fail(e, err, "kde_device");
+  add_kde_times(e);
   return RWPT_OK;
 }
 

@@ -194,6 +194,13 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                       const double *mass_dev, double *density_dev,
                       double *hist_dev, double *stats_host);
 
+// rwpt_kde_device: the device address of the file-scope scratch array on
+// `device` (looked up once per device, with that device current), and the
+// one-pass reconstruction that uses it instead of allocating
+hipError_t kde_device_scratch_init(int device);
+hipError_t launch_kde_device(rwpt_engine *e, const rwpt_kde_options *opt,
+                             double *density_dev, double *hist_dev, double *stats_host);
+
 hipError_t ensure_sort_storage(rwpt_engine *e);
 hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
                             const double *counts_dev, double hinit,

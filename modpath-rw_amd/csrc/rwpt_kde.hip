@@ -16,6 +16,7 @@
 // run sequentially, the order the reference's particle loop adds weights in.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -38,22 +39,39 @@ __device__ __forceinline__ bool kde_takes(int32_t status, int32_t cell, const Kd
   return status == 1 && (G.member < 0 || (int64_t)(cell - 1) / G.ncell_member == G.member);
 }
 
+// What the global coordinates of a particle are computed from: the DIS
+// face coordinates and cell sizes, and the particle's local coordinates.
+struct KdeGeom {
+  int ncol, nrow;
+  const double *xface, *yface, *delr, *delc, *top, *bot;
+  const double *lx, *ly, *lz;
+};
+
+// ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549) of particle p in
+// 0-based cell c0, clamped to the cell.
+__device__ __forceinline__ void kde_point(const KdeGeom &Q, int c0, int64_t p,
+                                          double g[3]) {
+  const int lc = Q.nrow * Q.ncol, r = c0 % lc, i = r / Q.ncol, j = r % Q.ncol;
+  const double cx = (Q.xface[j] + Q.xface[j + 1]) / 2.0;
+  const double cy = (Q.yface[i] + Q.yface[i + 1]) / 2.0;
+  const double left = cx - (Q.delr[j] / 2.0), right = cx + (Q.delr[j] / 2.0);
+  const double front = cy - (Q.delc[i] / 2.0), back = cy + (Q.delc[i] / 2.0);
+  const double b = Q.bot[c0], t = Q.top[c0];
+  const double x = Q.lx[p], y = Q.ly[p], z = Q.lz[p];
+  g[0] = fmin(fmax(((1.0 - x) * left) + (x * right), left), right);
+  g[1] = fmin(fmax(((1.0 - y) * front) + (y * back), front), back);
+  g[2] = fmin(fmax(((1.0 - z) * b) + (z * t), b), t);
+}
+
 // Global coordinates (ConvertToModelXYZ) and bin key of every particle;
-// inactive particles and points outside the grid get the key ncells.
+// inactive particles and points outside the grid get the key ncells.  gx
+// may be null: the coordinates are then not kept (the statistics kernel
+// recomputes them).
 __global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
-                         const int32_t *__restrict__ cellv,
-                         const double *__restrict__ lxv,
-                         const double *__restrict__ lyv,
-                         const double *__restrict__ lzv, int ncol, int nrow,
-                         const double *__restrict__ xface,
-                         const double *__restrict__ yface,
-                         const double *__restrict__ delr,
-                         const double *__restrict__ delc,
-                         const double *__restrict__ top,
-                         const double *__restrict__ bot, const KdeGrid G,
-                         double *__restrict__ gx, double *__restrict__ gy,
-                         double *__restrict__ gz, int32_t *__restrict__ keys,
-                         int32_t *__restrict__ vals) {
+                         const int32_t *__restrict__ cellv, const KdeGeom Q,
+                         const KdeGrid G, double *__restrict__ gx,
+                         double *__restrict__ gy, double *__restrict__ gz,
+                         int32_t *__restrict__ keys, int32_t *__restrict__ vals) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
@@ -62,20 +80,13 @@ __global__ void kde_keys(int64_t n, const int32_t *__restrict__ status,
     keys[p] = (int32_t)ncells;
     return;
   }
-  const int c0 = cellv[p] - 1;
-  const int lc = nrow * ncol, r = c0 % lc, i = r / ncol, j = r % ncol;
-  const double cx = (xface[j] + xface[j + 1]) / 2.0;
-  const double cy = (yface[i] + yface[i + 1]) / 2.0;
-  const double left = cx - (delr[j] / 2.0), right = cx + (delr[j] / 2.0);
-  const double front = cy - (delc[i] / 2.0), back = cy + (delc[i] / 2.0);
-  const double b = bot[c0], t = top[c0];
-  const double x = lxv[p], y = lyv[p], z = lzv[p];
-  const double g[3] = {fmin(fmax(((1.0 - x) * left) + (x * right), left), right),
-                       fmin(fmax(((1.0 - y) * front) + (y * back), front), back),
-                       fmin(fmax(((1.0 - z) * b) + (z * t), b), t)};
-  gx[p] = g[0];
-  gy[p] = g[1];
-  gz[p] = g[2];
+  double g[3];
+  kde_point(Q, cellv[p] - 1, p, g);
+  if (gx) {
+    gx[p] = g[0];
+    gy[p] = g[1];
+    gz[p] = g[2];
+  }
   // Histogram.F90:276-350: floor((x - origin)/bin) + 1 in every dimension
   // with a bin size (a single-bin dimension still drops points outside it)
   int64_t key = 0, stride = 1;
@@ -133,7 +144,7 @@ __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
                           const int32_t *__restrict__ keys, int64_t ncells,
                           const double *__restrict__ gx,
                           const double *__restrict__ gy,
-                          const double *__restrict__ gz,
+                          const double *__restrict__ gz, const KdeGeom Q,
                           const double *__restrict__ mass, int mode,
                           const double *__restrict__ mean,
                           double *__restrict__ partial) {
@@ -144,18 +155,25 @@ __global__ void kde_stats(int64_t n, const int32_t *__restrict__ status,
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   for (int64_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
     if (!kde_takes(status[p], cellv[p], G)) continue;
+    double g[3];
+    if (gx) {
+      g[0] = gx[p];
+      g[1] = gy[p];
+      g[2] = gz[p];
+    } else {
+      kde_point(Q, cellv[p] - 1, p, g);
+    }
     if (mode == 0) {
       const double w = mass ? mass[p] : 1.0;
-      acc[0] += gx[p];
-      acc[1] += gy[p];
-      acc[2] += gz[p];
+      acc[0] += g[0];
+      acc[1] += g[1];
+      acc[2] += g[2];
       acc[3] += w;
       acc[4] += w * w;
       acc[5] += 1.0;
       acc[6] += keys[p] < ncells ? 1.0 : 0.0;
     } else {
-      const double ex = gx[p] - mean[0], ey = gy[p] - mean[1],
-                   ez = gz[p] - mean[2];
+      const double ex = g[0] - mean[0], ey = g[1] - mean[1], ez = g[2] - mean[2];
       acc[0] += ex * ex;
       acc[1] += ey * ey;
       acc[2] += ez * ez;
@@ -201,6 +219,13 @@ __global__ void kde_count_to_hist(int64_t ncells, const uint32_t *__restrict__ c
                                   double *__restrict__ hist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ncells) hist[i] = (double)cnt[i];
+}
+
+// zeroes n words (a kernel, not a memset node, so that a captured graph
+// replays it correctly)
+__global__ void kde_zero_u32(int64_t n, uint32_t *__restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0u;
 }
 
 __global__ void kde_scale(int64_t n, const double *__restrict__ in,
@@ -267,14 +292,11 @@ static void kernel_1d(double h, double bin, std::vector<double> &w,
   for (int i = 0; i < (int)w.size(); ++i) w[i] = e[i] / sum;
 }
 
-hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
-                      const double *mass_dev, double *density_dev,
-                      double *hist_dev, double *stats_host) {
-  DeviceProblem &P = e->prob;
-  Particles &S = e->part;
-  const int64_t n = S.n;
+namespace {
+
+KdeGrid make_grid(const rwpt_engine *e, const rwpt_kde_options *opt, int &nd) {
   KdeGrid G{};
-  int nd = 0;
+  nd = 0;
   for (int d = 0; d < 3; ++d) {
     G.origin[d] = opt->origin[d];
     G.bin[d] = opt->bin_size[d];
@@ -282,8 +304,195 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     G.active[d] = opt->nbins[d] > 1;
     nd += G.active[d];
   }
-  G.member = P.members > 1 ? opt->member : -1;
-  G.ncell_member = P.ncell_member;
+  G.member = e->prob.members > 1 ? opt->member : -1;
+  G.ncell_member = e->prob.ncell_member;
+  return G;
+}
+
+KdeGeom make_geom(const rwpt_engine *e) {
+  const DeviceProblem &P = e->prob;
+  const Particles &S = e->part;
+  return KdeGeom{P.ncol,  P.nrow, e->xface, e->yface, P.delr, P.delc,
+                 P.top,   P.bot,  S.lx,     S.ly,     S.lz};
+}
+
+// The particle statistics of the reconstruction: hs = sums of x, y, z, w,
+// w^2, the active count and the count inside the grid; dev2 = the sums of
+// squared deviations from the mean.  Two deterministic fixed-tree passes
+// and two host read-backs.
+hipError_t kde_moments(rwpt_engine *e, const KdeGrid &G, const KdeGeom &Q,
+                       const double *gx, const double *gy, const double *gz,
+                       const double *mass_dev, int64_t ncells, double *partial,
+                       double *sums, double hs[kStats], double dev2[kStats]) {
+  const Particles &S = e->part;
+  const int64_t n = S.n;
+  hipStream_t st = e->stream;
+  hipError_t err = hipSuccess;
+  for (int q = 0; q < kStats; ++q) hs[q] = dev2[q] = 0.0;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status, S.cell,
+                     G, S.keys[0], ncells, gx, gy, gz, Q, mass_dev, 0,
+                     (const double *)nullptr, partial);
+  hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
+  if ((err = hipMemcpyAsync(hs, sums, kStats * sizeof(double), hipMemcpyDeviceToHost,
+                            st)) != hipSuccess)
+    return err;
+  if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+  const double nact = hs[5];
+  if (nact <= 0) return hipSuccess;
+  double mean[3];
+  for (int d = 0; d < 3; ++d) mean[d] = hs[d] / nact;
+  if ((err = hipMemcpyAsync(sums + kStats, mean, sizeof mean, hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+    return err;
+  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status, S.cell,
+                     G, S.keys[0], ncells, gx, gy, gz, Q, mass_dev, 1,
+                     (const double *)(sums + kStats), partial);
+  hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
+  if ((err = hipMemcpyAsync(dev2, sums, kStats * sizeof(double), hipMemcpyDeviceToHost,
+                            st)) != hipSuccess)
+    return err;
+  return hipStreamSynchronize(st);
+}
+
+struct KdeBandwidth {
+  double h = 0.0, effMass = 1.0, nPoints = 0.0, binVolume = 1.0;
+};
+
+// Bin volume, effective mass, point count and initial smoothing of
+// ComputeDensity for the statistics hs and dev2.
+KdeBandwidth kde_bandwidth(const rwpt_kde_options *opt, const KdeGrid &G, int nd,
+                           bool weighted, const double hs[kStats],
+                           const double dev2[kStats]) {
+  KdeBandwidth B;
+  // bin volume and bin distance over the dimensions with a bin size
+  // (Histogram.F90:121, 154-155)
+  int nbd = 0;
+  for (int d = 0; d < 3; ++d)
+    if (G.bin[d] > 0.0) {
+      B.binVolume *= G.bin[d];
+      ++nbd;
+    }
+  const double binDistance = std::pow(B.binVolume, 1.0 / (nbd > 0 ? nbd : 1));
+  // effective mass (Histogram.F90:326-343; formats 2 and 3 keep 1) and
+  // point count: the reference divides the weighted counts by the
+  // effective mass and multiplies the density back by it; both cancel in
+  // a single pass, so the bins hold the plain sums of the masses
+  const int wfmt = opt->effective_weight_format;
+  const double nact = hs[5];
+  B.nPoints = nact;
+  if (weighted) {
+    if (wfmt == 0) {
+      const double neff = hs[3] * hs[3] / hs[4];
+      B.effMass = hs[3] / neff;
+    } else if (wfmt == 1) {
+      B.effMass = hs[3] / nact;
+    }
+  } else {
+    // unit masses: nPoints counts the points inside the grid
+    B.nPoints = hs[6];
+  }
+  // initial smoothing: Silverman (1986) over the standard deviations
+  // (GridProjectedKDE.F90:4034-4054, 4109-4117), or a factor of the bin
+  // distance (:643-648)
+  const double fac = opt->initial_smoothing_factor;
+  if (opt->initial_smoothing == 1) {
+    if (nact > 0 && nd > 0) B.h = fac * binDistance;
+  } else if (nact > 0 && nd > 0) {
+    double sscale = 1.0;
+    for (int d = 0; d < 3; ++d)
+      if (G.active[d]) sscale *= std::sqrt(dev2[d] / nact);
+    sscale = std::pow(sscale, 1.0 / nd);
+    B.h = sscale * std::pow(4.0 / ((nd + 2.0) * B.nPoints), 1.0 / (nd + 4.0));
+    if (sscale == 0.0) B.h = 2.0 * binDistance;
+    else if (fac > 0.0 && fac != 1.0) B.h = B.h * fac;
+  }
+  return B;
+}
+
+// The 1-D kernels of the one-pass density (the same bandwidth in every
+// active dimension, or the database entry it looks up), concatenated:
+// dimension d has m[d] + 1 weights from off[d].
+void kde_weights(const rwpt_kde_options *opt, const KdeGrid &G, double h,
+                 std::vector<double> &all, int off[3], int m[3]) {
+  double hv[3] = {0, 0, 0};
+  int collapse = -1;
+  for (int d = 0; d < 3; ++d) hv[d] = G.active[d] ? h : 0.0;
+  if (opt->kernel_database) {
+    KernelDb db;
+    db.build(opt->min_h_over_delta, opt->delta_h_over_delta, opt->max_h_over_delta,
+             opt->bin_size, opt->nbins);
+    const double hin[3] = {hv[0], hv[1], hv[2]};
+    collapse = db.w(hin, hv);
+  }
+  all.clear();
+  for (int d = 0; d < 3; ++d) {
+    off[d] = m[d] = 0;
+    if (!G.active[d]) continue;
+    std::vector<double> w;
+    kernel_1d(hv[d], G.bin[d], w, d == collapse);
+    m[d] = (int)w.size() - 1;
+    off[d] = (int)all.size();
+    all.insert(all.end(), w.begin(), w.end());
+  }
+}
+
+// The separable passes from the histogram to the density.  With tmpB null
+// the passes alternate between the density and tmpA so that the last one
+// ends in the density.
+hipError_t kde_passes(hipStream_t st, const KdeGrid &G, const rwpt_kde_options *opt,
+                      double binVolume, const double *hist, double *density,
+                      double *tmpA, double *tmpB, const double *weights,
+                      const int off[3], const int m[3]) {
+  const int blk = 256;
+  const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+  int last = -1, passes = 0;
+  for (int d = 0; d < 3; ++d)
+    if (G.active[d]) {
+      last = d;
+      ++passes;
+    }
+  const double *src = hist;
+  int ping = 0, k = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (!G.active[d]) continue;
+    double *dst;
+    if (d == last) dst = density;
+    else if (tmpB) dst = ping ? tmpB : tmpA;
+    else dst = ((passes - 1 - k) % 2 == 0) ? density : tmpA;
+    const double sc = d != last ? 1.0
+                      : (opt->unit_volume ? opt->scaling_factor
+                                          : opt->scaling_factor / binVolume);
+    hipLaunchKernelGGL(kde_pass, dim3((unsigned)((ncells + blk - 1) / blk)), dim3(blk),
+                       0, st, src, dst, G.nb[0], G.nb[1], G.nb[2], d, weights + off[d],
+                       m[d], sc);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    src = dst;
+    ping ^= 1;
+    ++k;
+  }
+  return hipSuccess;
+}
+
+void record_stats(double *stats_host, const KdeBandwidth &B, double nact) {
+  if (!stats_host) return;
+  stats_host[0] = B.h;
+  stats_host[1] = nact;
+  stats_host[2] = B.nPoints;
+  stats_host[3] = B.effMass;
+}
+
+}  // namespace
+
+hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
+                      const double *mass_dev, double *density_dev,
+                      double *hist_dev, double *stats_host) {
+  Particles &S = e->part;
+  const int64_t n = S.n;
+  int nd = 0;
+  const KdeGrid G = make_grid(e, opt, nd);
+  const KdeGeom Q = make_geom(e);
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
   hipStream_t st = e->stream;
   // scratch: gx gy gz (3n), keys/vals reuse the walk's work lists, two grid
@@ -305,10 +514,8 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   (void)hipEventRecord(TH.start, st);
   const int blk = 256;
   if (n > 0) {
-    hipLaunchKernelGGL(kde_keys, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk),
-                       0, st, n, S.status, S.cell, S.lx, S.ly, S.lz, P.ncol, P.nrow,
-                       e->xface, e->yface, P.delr, P.delc, P.top, P.bot, G, gx, gy,
-                       gz, S.keys[0], S.vals[0]);
+    hipLaunchKernelGGL(kde_keys, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0, st,
+                       n, S.status, S.cell, Q, G, gx, gy, gz, S.keys[0], S.vals[0]);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   }
   if (n > 0 && !mass_dev) {
@@ -350,80 +557,14 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   }
   (void)hipEventRecord(TH.stop, st);
   // statistics for the bandwidth and the effective mass
-  double hs[kStats] = {0, 0, 0, 0, 0, 0, 0, 0}, dev2[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (n > 0) {
-    hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       S.cell, G, S.keys[0], ncells, gx, gy, gz, mass_dev, 0,
-                       (const double *)nullptr, partial);
-    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
-    if ((err = hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
-      return err;
-    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
-  }
-  const double nact = hs[5];
-  double mean[3] = {0, 0, 0};
-  if (nact > 0)
-    for (int d = 0; d < 3; ++d) mean[d] = hs[d] / nact;
-  if (n > 0 && nact > 0) {
-    if ((err = hipMemcpyAsync(sums + kStats, mean, sizeof mean, hipMemcpyHostToDevice, st)) != hipSuccess)
-      return err;
-    hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status,
-                       S.cell, G, S.keys[0], ncells, gx, gy, gz, mass_dev, 1,
-                       (const double *)(sums + kStats), partial);
-    hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
-    if ((err = hipMemcpyAsync(dev2, sums, sizeof dev2, hipMemcpyDeviceToHost, st)) != hipSuccess)
-      return err;
-    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
-  }
-  // bin volume and bin distance over the dimensions with a bin size
-  // (Histogram.F90:121, 154-155)
-  double binVolume = 1.0;
-  int nbd = 0;
-  for (int d = 0; d < 3; ++d)
-    if (G.bin[d] > 0.0) {
-      binVolume *= G.bin[d];
-      ++nbd;
-    }
-  const double binDistance = std::pow(binVolume, 1.0 / (nbd > 0 ? nbd : 1));
-  // effective mass (Histogram.F90:326-343; formats 2 and 3 keep 1) and
-  // point count: the reference divides the weighted counts by the
-  // effective mass and multiplies the density back by it; both cancel in
-  // a single pass, so the bins hold the plain sums of the masses
+  double hs[kStats], dev2[kStats];
+  if ((err = kde_moments(e, G, Q, gx, gy, gz, mass_dev, ncells, partial, sums, hs,
+                         dev2)) != hipSuccess)
+    return err;
+  const KdeBandwidth B = kde_bandwidth(opt, G, nd, mass_dev != nullptr, hs, dev2);
+  const double h = B.h, effMass = B.effMass, nPoints = B.nPoints;
+  record_stats(stats_host, B, hs[5]);
   const int wfmt = opt->effective_weight_format;
-  double effMass = 1.0, nPoints = nact;
-  if (mass_dev) {
-    if (wfmt == 0) {
-      const double neff = hs[3] * hs[3] / hs[4];
-      effMass = hs[3] / neff;
-    } else if (wfmt == 1) {
-      effMass = hs[3] / nact;
-    }
-  } else {
-    // unit masses: nPoints counts the points inside the grid
-    nPoints = hs[6];
-  }
-  // initial smoothing: Silverman (1986) over the standard deviations
-  // (GridProjectedKDE.F90:4034-4054, 4109-4117), or a factor of the bin
-  // distance (:643-648)
-  double h = 0.0;
-  const double fac = opt->initial_smoothing_factor;
-  if (opt->initial_smoothing == 1) {
-    if (nact > 0 && nd > 0) h = fac * binDistance;
-  } else if (nact > 0 && nd > 0) {
-    double sscale = 1.0;
-    for (int d = 0; d < 3; ++d)
-      if (G.active[d]) sscale *= std::sqrt(dev2[d] / nact);
-    sscale = std::pow(sscale, 1.0 / nd);
-    h = sscale * std::pow(4.0 / ((nd + 2.0) * nPoints), 1.0 / (nd + 4.0));
-    if (sscale == 0.0) h = 2.0 * binDistance;
-    else if (fac > 0.0 && fac != 1.0) h = h * fac;
-  }
-  if (stats_host) {
-    stats_host[0] = h;
-    stats_host[1] = nact;
-    stats_host[2] = nPoints;
-    stats_host[3] = effMass;
-  }
   (void)hipEventRecord(TD.start, st);
   e->kde_loops = 0;
   if (opt->mode == 1 && opt->n_opt_loops > 0 && h > 0.0) {
@@ -460,37 +601,18 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     e->kde_loops = loops;
     hipLaunchKernelGGL(kde_finish, dim3((unsigned)((ncells + blk - 1) / blk)),
                        dim3(blk), 0, st, ncells, density_dev, mass_dev ? effMass : 1.0,
-                       opt->unit_volume ? binVolume : 1.0, opt->scaling_factor);
+                       opt->unit_volume ? B.binVolume : 1.0, opt->scaling_factor);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   } else if (opt->mode == 0 || h <= 0.0) {
     // histogram only: the raw density counts / binVolume
     hipLaunchKernelGGL(kde_scale, dim3((unsigned)((ncells + blk - 1) / blk)),
                        dim3(blk), 0, st, ncells, hist_dev, density_dev,
-                       opt->scaling_factor / binVolume);
+                       opt->scaling_factor / B.binVolume);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   } else {
-    // the same bandwidth in every active dimension, or the database entry
-    // it looks up
-    double hv[3] = {0, 0, 0};
-    int collapse = -1;
-    for (int d = 0; d < 3; ++d) hv[d] = G.active[d] ? h : 0.0;
-    if (opt->kernel_database) {
-      KernelDb db;
-      db.build(opt->min_h_over_delta, opt->delta_h_over_delta, opt->max_h_over_delta,
-               opt->bin_size, opt->nbins);
-      const double hin[3] = {hv[0], hv[1], hv[2]};
-      collapse = db.w(hin, hv);
-    }
     std::vector<double> all;
-    int off[3] = {0, 0, 0}, m[3] = {0, 0, 0};
-    for (int d = 0; d < 3; ++d) {
-      if (!G.active[d]) continue;
-      std::vector<double> w;
-      kernel_1d(hv[d], G.bin[d], w, d == collapse);
-      m[d] = (int)w.size() - 1;
-      off[d] = (int)all.size();
-      all.insert(all.end(), w.begin(), w.end());
-    }
+    int off[3], m[3];
+    kde_weights(opt, G, h, all, off, m);
     if ((int64_t)all.size() > e->kde_weights_size) {
       if (e->kde_weights) (void)hipFree(e->kde_weights);
       e->kde_weights = nullptr;
@@ -502,26 +624,112 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
     if ((err = hipMemcpyAsync(e->kde_weights, all.data(), all.size() * sizeof(double),
                               hipMemcpyHostToDevice, st)) != hipSuccess)
       return err;
-    const double *src = hist_dev;
-    int last = -1;
-    for (int d = 0; d < 3; ++d)
-      if (G.active[d]) last = d;
-    int ping = 0;
-    for (int d = 0; d < 3; ++d) {
-      if (!G.active[d]) continue;
-      double *dst = d == last ? density_dev : (ping ? tmpB : tmpA);
-      const double sc = d != last ? 1.0
-                        : (opt->unit_volume ? opt->scaling_factor
-                                            : opt->scaling_factor / binVolume);
-      hipLaunchKernelGGL(kde_pass, dim3((unsigned)((ncells + blk - 1) / blk)),
-                         dim3(blk), 0, st, src, dst, G.nb[0], G.nb[1], G.nb[2], d,
-                         e->kde_weights + off[d], m[d], sc);
-      if ((err = hipGetLastError()) != hipSuccess) return err;
-      src = dst;
-      ping ^= 1;
-    }
+    if ((err = kde_passes(st, G, opt, B.binVolume, hist_dev, density_dev, tmpA, tmpB,
+                          e->kde_weights, off, m)) != hipSuccess)
+      return err;
   }
   (void)hipEventRecord(TD.stop, st);
+  return hipStreamSynchronize(st);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free one-pass reconstruction (rwpt_kde_device).  Its scratch
+// memory -- the count grid that becomes the ping-pong grid of the
+// separable passes, the statistics partials and the 1-D kernel weights --
+// is one file-scope device array, of which each GPU holds its own copy for
+// the life of the process.  Host code never uses the array's name as an
+// address: hipGetSymbolAddress gives each device's address once, at engine
+// creation with that device current, and that address is what the kernels
+// receive.
+constexpr int64_t kNoAllocGrid = RWPT_KDE_DEVICE_MAX_CELLS;
+constexpr int64_t kNoAllocTaps = RWPT_KDE_DEVICE_MAX_TAPS;
+constexpr int64_t kNoAllocDoubles =
+    kNoAllocGrid + kStats * kStatBlocks + 2 * kStats + kNoAllocTaps;
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t kde_device_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (err != hipSuccess) return err;
+  g_scratch_ptr[device] = static_cast<double *>(ptr);
+  return hipSuccess;
+}
+
+hipError_t launch_kde_device(rwpt_engine *e, const rwpt_kde_options *opt,
+                             double *density_dev, double *hist_dev, double *stats_host) {
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (e->device >= 0 && e->device < kMaxDevices) base = g_scratch_ptr[e->device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  Particles &S = e->part;
+  const int64_t n = S.n;
+  int nd = 0;
+  const KdeGrid G = make_grid(e, opt, nd);
+  const KdeGeom Q = make_geom(e);
+  const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+  if (ncells > kNoAllocGrid) return hipErrorInvalidValue;
+  double *gridA = base;
+  double *partial = gridA + kNoAllocGrid, *sums = partial + kStats * kStatBlocks;
+  double *weights = sums + 2 * kStats;
+  hipStream_t st = e->stream;
+  hipError_t err = hipSuccess;
+  KernelTimer &TH = e->timers[1], &TD = e->timers[2];
+  (void)hipEventRecord(TH.start, st);
+  const int blk = 256;
+  const unsigned gcells = (unsigned)((ncells + blk - 1) / blk);
+  // unit masses: exact integer counts in the scratch grid (cleared first:
+  // it holds the previous call's values)
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(gridA);
+  hipLaunchKernelGGL(kde_zero_u32, dim3(gcells), dim3(blk), 0, st, ncells, cnt);
+  if (n > 0) {
+    hipLaunchKernelGGL(kde_keys, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0, st,
+                       n, S.status, S.cell, Q, G, (double *)nullptr, (double *)nullptr,
+                       (double *)nullptr, S.keys[0], S.vals[0]);
+    hipLaunchKernelGGL(kde_count, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
+                       st, n, S.keys[0], ncells, cnt);
+  }
+  hipLaunchKernelGGL(kde_count_to_hist, dim3(gcells), dim3(blk), 0, st, ncells, cnt,
+                     hist_dev);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  (void)hipEventRecord(TH.stop, st);
+  double hs[kStats], dev2[kStats];
+  if ((err = kde_moments(e, G, Q, nullptr, nullptr, nullptr, nullptr, ncells, partial,
+                         sums, hs, dev2)) != hipSuccess)
+    return err;
+  const KdeBandwidth B = kde_bandwidth(opt, G, nd, false, hs, dev2);
+  record_stats(stats_host, B, hs[5]);
+  (void)hipEventRecord(TD.start, st);
+  e->kde_loops = 0;
+  if (opt->mode == 0 || B.h <= 0.0) {
+    hipLaunchKernelGGL(kde_scale, dim3(gcells), dim3(blk), 0, st, ncells, hist_dev,
+                       density_dev, opt->scaling_factor / B.binVolume);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  } else {
+    std::vector<double> all;
+    int off[3], m[3];
+    kde_weights(opt, G, B.h, all, off, m);
+    if ((int64_t)all.size() > kNoAllocTaps) return hipErrorInvalidConfiguration;
+    if ((err = hipMemcpyAsync(weights, all.data(), all.size() * sizeof(double),
+                              hipMemcpyHostToDevice, st)) != hipSuccess)
+      return err;
+    if ((err = kde_passes(st, G, opt, B.binVolume, hist_dev, density_dev, gridA, nullptr,
+                          weights, off, m)) != hipSuccess)
+      return err;
+  }
+  (void)hipEventRecord(TD.stop, st);
+  // the host vector of weights is read by the copy above before it goes
   return hipStreamSynchronize(st);
 }
 

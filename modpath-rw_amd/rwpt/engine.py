@@ -87,6 +87,8 @@ EXPORTS = {
                        _i64p], C.c_int),
     "rwpt_global_xyz": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "rwpt_kde": ([C.c_void_p, C.POINTER(RwptKdeOptions), _dp, _dp, _dp], C.c_int),
+    "rwpt_kde_device": ([C.c_void_p, C.POINTER(RwptKdeOptions), C.c_void_p, C.c_void_p],
+                        C.c_int),
     "rwpt_kde_info": ([C.c_void_p, _dp, _dp, _dp, _dp, _i32p], C.c_int),
     "rwpt_write_timeseries_header": ([C.c_char_p, C.c_int32, C.c_double, C.c_double,
                                       C.c_double, C.c_double], C.c_int),
@@ -432,23 +434,22 @@ class Engine:
     def reset_kernel_time(self):
         self._check(lib().rwpt_reset_kernel_time(self.h), "rwpt_reset_kernel_time")
 
-    def kde(self, origin, bin_size, nbins, mode=0, mass=None, scaling_factor=1.0,
-            unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
-            rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
-            bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
-            effective_weight_format=0, kernel_database=False, delta_h_over_delta=0.0,
-            member=-1):
-        """gpkde reconstruction of the active particles; the keywords are
-        the options of the gpkde input file (ReadGPKDEData,
-        ModpathSimulationData.f90:816-1273).  Returns (density, histogram),
-        x fastest."""
+    @staticmethod
+    def kde_options(origin, bin_size, nbins, mode=0, weighted=False, scaling_factor=1.0,
+                    unit_volume=False, n_opt_loops=0, isotropic=False, skip_error=False,
+                    rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
+                    bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
+                    effective_weight_format=0, kernel_database=False,
+                    delta_h_over_delta=0.0, member=-1):
+        """rwpt_kde_options from the options of the gpkde input file
+        (ReadGPKDEData, ModpathSimulationData.f90:816-1273)."""
         o = RwptKdeOptions()
         for d in range(3):
             o.origin[d] = origin[d]
             o.bin_size[d] = bin_size[d]
             o.nbins[d] = nbins[d]
         o.mode = mode
-        o.weighted = int(mass is not None)
+        o.weighted = int(bool(weighted))
         o.scaling_factor = scaling_factor
         o.unit_volume = int(bool(unit_volume))
         o.n_opt_loops = n_opt_loops
@@ -464,6 +465,13 @@ class Engine:
         o.kernel_database = int(bool(kernel_database))
         o.delta_h_over_delta = delta_h_over_delta
         o.member = member
+        return o
+
+    def kde(self, origin, bin_size, nbins, mode=0, mass=None, **kw):
+        """gpkde reconstruction of the active particles; the keywords are
+        those of kde_options.  Returns (density, histogram), x fastest."""
+        o = self.kde_options(origin, bin_size, nbins, mode=mode, weighted=mass is not None,
+                             **kw)
         nb = int(np.prod(nbins))
         dens = np.zeros(nb)
         hist = np.zeros(nb)
@@ -471,6 +479,14 @@ class Engine:
         rc = lib().rwpt_kde(self.h, C.byref(o), _p(m, _dp), _p(dens, _dp), _p(hist, _dp))
         self._check(rc, "rwpt_kde")
         return dens, hist
+
+    def kde_device(self, density_dev, histogram_dev, origin, bin_size, nbins, mode=1, **kw):
+        """rwpt_kde_device: the one-pass reconstruction into device buffers
+        (addresses as integers), allocating nothing."""
+        o = self.kde_options(origin, bin_size, nbins, mode=mode, **kw)
+        rc = lib().rwpt_kde_device(self.h, C.byref(o), C.c_void_p(density_dev),
+                                   C.c_void_p(histogram_dev))
+        self._check(rc, "rwpt_kde_device")
 
     def kde_info(self):
         v = [C.c_double(0) for _ in range(4)]
