@@ -404,6 +404,35 @@ __global__ void float_records(int64_t ncell, const double *__restrict__ cells,
   const double *rec = cells + c * CELL_DOUBLES;
   float *o = out + c * CELL_FLOATS;
   for (int f = 0; f < F_CONN; ++f) o[f] = (float)rec[f];
+  // face velocity slopes
+  for (int a = 0; a < 3; ++a)
+    o[G_DV + a] = (float)(rec[F_VX1 + 2 * a + 1] - rec[F_VX1 + 2 * a]);
+  // dispersion factors 2 alpha / R, 2 Dm / R
+  for (int k = 0; k < 5; ++k) o[G_S2ALH + k] = (float)(2.0 * rec[F_ALH + k] / rec[F_R]);
+  // trilinear coefficients of the corner velocities (corner i + 2j + 4k
+  // at x = i, y = j, z = k)
+  for (int comp = 0; comp < 3; ++comp) {
+    const double *v = rec + F_VC + 8 * comp;
+    float *cc = o + G_VC + 8 * comp;
+    cc[0] = (float)v[0];
+    cc[1] = (float)(v[1] - v[0]);
+    cc[2] = (float)(v[2] - v[0]);
+    cc[3] = (float)(v[4] - v[0]);
+    cc[4] = (float)((v[3] - v[2]) - (v[1] - v[0]));
+    cc[5] = (float)((v[5] - v[4]) - (v[1] - v[0]));
+    cc[6] = (float)((v[6] - v[4]) - (v[2] - v[0]));
+    cc[7] = (float)(((v[7] - v[6]) - (v[5] - v[4])) - ((v[3] - v[2]) - (v[1] - v[0])));
+  }
+  // bilinear coefficients of the drift terms over phi R
+  const double s = 1.0 / (rec[F_PHI] * rec[F_R]);
+  for (int term = 0; term < 9; ++term) {
+    const double *d = rec + F_DD + 4 * term;
+    float *e = o + G_DD + 4 * term;
+    e[0] = (float)(d[0] * s);
+    e[1] = (float)((d[1] - d[0]) * s);
+    e[2] = (float)((d[2] - d[0]) * s);
+    e[3] = (float)(((d[3] - d[2]) - (d[1] - d[0])) * s);
+  }
   const int32_t *ri = reinterpret_cast<const int32_t *>(rec);
   int32_t *oi = reinterpret_cast<int32_t *>(o);
   for (int k = 0; k < 6; ++k) {

@@ -46,10 +46,25 @@ enum : int32_t {
   CF_VERIFY_PDRY = 8,     // VerifyDryCell partiallyDry (release of status 7)
 };
 
-// FP32-local cell record (precision 1 and 2): the float fields 0..79 at the
-// same indices as in the FP64 record, then the integer fields one per slot,
-// then full reciprocals of the divisors the sub-step uses.
+// FP32-local cell record (precision 1 and 2), in the forms the FP32 walk
+// evaluates with fused multiply-adds:
+//   0..11   as in the FP64 record (face velocities, DX DY DZ, phi, R, dt_cell);
+//   12..14  G_DV: VX2 - VX1, VY2 - VY1, VZ2 - VZ1 (the face velocity slope;
+//           the exponential gradient is G_DV times 1/DX);
+//   15..19  G_S2ALH..G_S2DM: 2 aLH / R, 2 aTH / R, 2 aLV / R, 2 aTV / R,
+//           2 Dm / R (the dispersion factors under the square roots);
+//   20..43  per velocity component, the 8 coefficients c0..c7 of the
+//           trilinear corner interpolant c0 + c1 x + c2 y + c3 z + c4 xy +
+//           c5 xz + c6 yz + c7 xyz;
+//   44..79  per drift term, the 4 coefficients e0..e3 of the bilinear
+//           interpolant e0 + e1 u + e2 w + e3 uw of its edge differences,
+//           divided by phi R (so that the drift is their plain sum);
+// then the integer fields one per slot and full reciprocals.
 enum CellField32 : int {
+  G_DV = 12,
+  G_S2ALH = 15, G_S2ATH, G_S2ALV, G_S2ATV, G_S2DM,  // 15..19
+  G_VC = 20,        // 20..43: trilinear coefficients [component][8]
+  G_DD = 44,        // 44..79: bilinear coefficients [term][4]
   G_CONN = 80,      // 80..85: 6 x int32 connections
   G_MASSBND = 86,   // 86..91: 6 x int32 mass-boundary flags
   G_FLAGS = 92,     // int32 ibound, int32 flags
@@ -202,6 +217,16 @@ hipError_t launch_kde_device(rwpt_engine *e, const rwpt_kde_options *opt,
                              double *density_dev, double *hist_dev, double *stats_host);
 
 hipError_t ensure_sort_storage(rwpt_engine *e);
+// walk launches by arithmetic type: FP64 (rwpt_walk.hip) and FP32-local
+// (rwpt_walk32.hip, a translation unit of its own built with contraction)
+hipError_t walk_launch64(rwpt_engine *e, const WalkParams &W, dim3 grid, int block,
+                         const int32_t *work, int64_t cnt, int budget, int32_t *nk,
+                         int32_t *nv);
+hipError_t walk_launch32(rwpt_engine *e, const WalkParams &W, dim3 grid, int block,
+                         const int32_t *work, int64_t cnt, int budget, int32_t *nk,
+                         int32_t *nv);
+hipError_t walk_prepare32(rwpt_engine *e, const WalkParams &W);
+hipError_t walk_terminate32(rwpt_engine *e, const int32_t *work, int64_t cnt);
 hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
                             const double *counts_dev, double hinit,
                             double nPoints, const double *final_weights_dev,

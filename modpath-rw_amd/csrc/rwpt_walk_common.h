@@ -16,15 +16,24 @@ struct Stream {
   uint64_t a, b;
 };
 
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) {
-  return (x << k) | (x >> (64 - k));
-}
-
+// One xoroshiro128+ step on 32-bit halves: out = a + b; with s2 = b ^ a,
+// a' = rotl(a, 24) ^ s2 ^ (s2 << 16) and b' = rotl(s2, 37).  Each 64-bit
+// rotation or shift is two funnel shifts (v_alignbit_b32) and each
+// three-way xor one v_bitop3_b32, the same bits as the 64-bit expressions.
 __device__ __forceinline__ uint64_t draw_bits(Stream &s) {
-  const uint64_t s1 = s.a, s2 = s.b ^ s.a;
+  const uint32_t alo = (uint32_t)s.a, ahi = (uint32_t)(s.a >> 32);
+  const uint32_t xlo = (uint32_t)s.b ^ alo, xhi = (uint32_t)(s.b >> 32) ^ ahi;
   const uint64_t out = s.a + s.b;
-  s.a = (rotl64(s1, 24) ^ s2) ^ (s2 << 16);
-  s.b = rotl64(s2, 37);
+  const uint32_t rhi = __builtin_amdgcn_alignbit(ahi, alo, 8);   // rotl(a, 24)
+  const uint32_t rlo = __builtin_amdgcn_alignbit(alo, ahi, 8);
+  const uint32_t shi = __builtin_amdgcn_alignbit(xhi, xlo, 16);  // s2 << 16
+  const uint32_t slo = xlo << 16;
+  const uint32_t ahi2 = __builtin_amdgcn_bitop3_b32(rhi, xhi, shi, 0x96);  // x ^ y ^ z
+  const uint32_t alo2 = __builtin_amdgcn_bitop3_b32(rlo, xlo, slo, 0x96);
+  const uint32_t bhi2 = __builtin_amdgcn_alignbit(xlo, xhi, 27);  // rotl(s2, 37)
+  const uint32_t blo2 = __builtin_amdgcn_alignbit(xhi, xlo, 27);
+  s.a = ((uint64_t)ahi2 << 32) | alo2;
+  s.b = ((uint64_t)bhi2 << 32) | blo2;
   return out;
 }
 
