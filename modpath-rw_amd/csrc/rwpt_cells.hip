@@ -127,8 +127,12 @@ __global__ void assemble_cells(const GridDims g, int64_t ncell,
                    (Q[3] > 0.0) || (Q[4] < 0.0) || (Q[5] > 0.0);
   int32_t *fl = reinterpret_cast<int32_t *>(rec + F_FLAGS);
   fl[0] = ibound ? ibound[c0] : 1;
+  int32_t nbi = 0;
+  for (int f = 0; f < 6; ++f)
+    if (conn[f] > 0 && ibound && ibound[conn[f] - 1] == 0) nbi |= CF_NB_INACTIVE << f;
   fl[1] = (dry ? CF_DRY : 0) | (cell_exit ? CF_CELL_EXIT : 0) |
-          (vpdry ? CF_VERIFY_PDRY : 0);
+          (vpdry ? CF_VERIFY_PDRY : 0) | (src != 0.0 ? CF_SOURCE : 0) |
+          (snk != 0.0 ? CF_SINK : 0) | nbi;
   int32_t *zn = reinterpret_cast<int32_t *>(rec + F_ZONE);
   zn[0] = zone ? zone[c0] : 0;
   zn[1] = 0;
@@ -371,7 +375,8 @@ __global__ void corner_tables(const GridDims g, int64_t ncell,
   const bool sub_exit = (VX1 < 0.0) || (VX2 > 0.0) || (VY1 < 0.0) ||
                         (VY2 > 0.0) || (VZ1 < 0.0) || (VZ2 > 0.0);
   int32_t *fl = reinterpret_cast<int32_t *>(rec + F_FLAGS);
-  fl[1] = (fl[1] & ~CF_SUBCELL_EXIT) | (sub_exit ? CF_SUBCELL_EXIT : 0);
+  fl[1] = (fl[1] & ~(CF_SUBCELL_EXIT | CF_DT_NONPOS)) | (sub_exit ? CF_SUBCELL_EXIT : 0) |
+          (dtc <= 0.0 ? CF_DT_NONPOS : 0);
   rec[F_RDX] = pow2_recip(DX);
   rec[F_RDY] = pow2_recip(DY);
   rec[F_RDZ] = pow2_recip(DZ);

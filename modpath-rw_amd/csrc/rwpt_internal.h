@@ -44,6 +44,11 @@ enum : int32_t {
   CF_SUBCELL_EXIT = 2,    // sub-cell HasExitFace
   CF_CELL_EXIT = 4,       // cell HasExitFace
   CF_VERIFY_PDRY = 8,     // VerifyDryCell partiallyDry (release of status 7)
+  CF_SOURCE = 16,         // source flow != 0 (weak source)
+  CF_SINK = 32,           // sink flow != 0 (weak sink)
+  CF_DT_NONPOS = 64,      // cell time step <= 0
+  CF_NB_INACTIVE = 256,   // << face-1 (6 bits): the neighbour across that
+                          // face is a cell with IBOUND 0
 };
 
 // FP32-local cell record (precision 1 and 2), in the forms the FP32 walk

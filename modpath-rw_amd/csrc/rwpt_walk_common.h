@@ -115,27 +115,25 @@ __device__ __forceinline__ int entry_checks(const R *__restrict__ rec,
   if (outside(x, y, z)) return 9;  // InvalidLocation -> Undefined
   const int32_t *fl = ints(rec) + Lay<R>::FLAGS;
   if (fl[0] == 0) return 7;
-  const int32_t zone = ints(rec)[Lay<R>::ZONE];
-  if (W.stop_zone > 0 && zone == W.stop_zone) return 6;
+  if (W.stop_zone > 0 && ints(rec)[Lay<R>::ZONE] == W.stop_zone) return 6;
+  const int32_t b = fl[1];
+  const bool src = (b & CF_SOURCE) != 0, snk = (b & CF_SINK) != 0;
   bool stopIfNoExit = true;
-  const R src = rec[Lay<R>::SOURCE], snk = rec[Lay<R>::SINK];
   if (W.backward) {
     if (W.stop_weak_sources) {
-      if (src != R(0)) return 4;
+      if (src) return 4;
       if (!W.steady_state) stopIfNoExit = false;
     }
   } else if (W.stop_weak_sinks) {
-    if (snk != R(0)) return 3;
+    if (snk) return 3;
     if (!W.steady_state) stopIfNoExit = false;
   }
-  if (!(fl[1] & CF_CELL_EXIT)) {
-    if (W.steady_state || (W.backward && src != R(0)) ||
-        (!W.backward && snk != R(0)))
-      return 5;
+  if (!(b & CF_CELL_EXIT)) {
+    if (W.steady_state || (W.backward && src) || (!W.backward && snk)) return 5;
   }
-  if (stopIfNoExit && !(fl[1] & CF_SUBCELL_EXIT)) return 5;
-  if (fl[1] & CF_DRY) return 7;
-  if (rec[F_DTCELL] <= R(0)) return 9;
+  if (stopIfNoExit && !(b & CF_SUBCELL_EXIT)) return 5;
+  if (b & CF_DRY) return 7;
+  if (b & CF_DT_NONPOS) return 9;
   return 0;
 }
 

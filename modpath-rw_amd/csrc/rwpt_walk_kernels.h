@@ -939,22 +939,25 @@ __global__ void __launch_bounds__(256, MINW)
     Stream rng{S.rng[2 * p], S.rng[2 * p + 1]};
     int status = 1;
     bool done = false;
-    while (!done) {
-      const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
-      if (!A.phase) {  // entering a cell: TrackCell checks, new visit
-        const int st = entry_checks(rec, W, V.x, V.y, V.z);
-        if (st != 0) {
-          status = st;
-          if (st == 9)  // InvalidLocation: last recorded point
-            last_point(cells, A.entryFace, c, V.x, V.y, V.z);
-          done = true;
-          break;
-        }
-        A.phase = 1;
-        V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
-        V.restarts = 0;
-        V.exitFace = 0;
+    const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
+    // entering a cell (a released particle): TrackCell checks, new visit;
+    // after a transfer the same checks run at once, in the exit branch
+    auto enter = [&]() -> bool {
+      const int st = entry_checks(rec, W, V.x, V.y, V.z);
+      if (st != 0) {
+        status = st;
+        if (st == 9)  // InvalidLocation: last recorded point
+          last_point(cells, A.entryFace, c, V.x, V.y, V.z);
+        return false;
       }
+      A.phase = 1;
+      V.ix = V.x; V.iy = V.y; V.iz = V.z; V.t0 = V.t;
+      V.restarts = 0;
+      V.exitFace = 0;
+      return true;
+    };
+    if (!A.phase) done = !enter();
+    while (!done) {
       if ((int)steps >= budget) {
         more = true;
         break;
@@ -980,12 +983,9 @@ __global__ void __launch_bounds__(256, MINW)
           break;
         }
         const int next = ints(rec)[Lay<R>::CONN + face - 1];
-        bool inactive_next = next <= 0;
-        if (!inactive_next) {
-          const R *nrec = cells + (int64_t)(next - 1) * Lay<R>::STRIDE;
-          inactive_next = ints(nrec)[Lay<R>::FLAGS] == 0;
-        }
-        if (inactive_next) {
+        // no neighbour, or one with IBOUND 0 (a flag of this cell's record)
+        if (next <= 0 ||
+            (ints(rec)[Lay<R>::FLAGS + 1] & (CF_NB_INACTIVE << (face - 1)))) {
           status = 2;
           done = true;
           break;
@@ -1007,9 +1007,11 @@ __global__ void __launch_bounds__(256, MINW)
           break;
         }
         c = next;
+        rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
         V.x = fx; V.y = fy; V.z = fz;
         A.phase = 0;
         A.entryFace = face <= 2 ? 3 - face : (face <= 4 ? 7 - face : 11 - face);
+        done = !enter();
       }
     }
     // statuses 3-7 come from the entry checks and report the entry point,
