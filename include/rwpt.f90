@@ -11,7 +11,8 @@ module RwptGpuModule
   public :: rwpt_default_options, rwpt_create, rwpt_destroy, rwpt_last_error
   public :: rwpt_set_flow, rwpt_set_cell_extras, rwpt_set_dispersion
   public :: rwpt_load_particles, rwpt_get_particles, rwpt_particle_count
-  public :: rwpt_advance, rwpt_set_time, rwpt_set_steady_state, rwpt_global_xyz, rwpt_kde, rwpt_kde_info
+  public :: rwpt_advance, rwpt_set_time, rwpt_set_steady_state, rwpt_global_xyz
+  public :: rwpt_kde, rwpt_kde_device, rwpt_kde_info
   public :: rwpt_kde_write, rwpt_get_substeps, rwpt_set_tuning, rwpt_last_launches
   public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
   public :: rwpt_create_ensemble, rwpt_autotune, rwpt_write_timeseries_header
@@ -42,6 +43,7 @@ module RwptGpuModule
     real(c_double)     :: min_h_over_delta, max_h_over_delta
     integer(c_int32_t) :: effective_weight_format, kernel_database
     real(c_double)     :: delta_h_over_delta
+    integer(c_int32_t) :: member
   end type
 
   interface
@@ -133,8 +135,15 @@ module RwptGpuModule
       import :: c_int, c_ptr, c_double, rwpt_kde_options
       type(c_ptr), value :: e
       type(rwpt_kde_options), intent(in) :: opt
-      real(c_double), intent(in) :: mass(*)
+      type(c_ptr), value :: mass                 ! c_null_ptr: unit masses
       real(c_double), intent(out) :: density(*), histogram(*)
+    end function
+    integer(c_int) function rwpt_kde_device(e, opt, density_dev, histogram_dev) &
+        bind(C, name='rwpt_kde_device')
+      import :: c_int, c_ptr, rwpt_kde_options
+      type(c_ptr), value :: e
+      type(rwpt_kde_options), intent(in) :: opt
+      type(c_ptr), value :: density_dev, histogram_dev   ! device addresses
     end function
     integer(c_int) function rwpt_kde_info(e, bandwidth, points, npoints, eff_mass, &
         loops) bind(C, name='rwpt_kde_info')

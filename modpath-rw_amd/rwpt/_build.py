@@ -59,6 +59,27 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+FLANG = "/opt/rocm/lib/llvm/bin/amdflang"
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_engine")
+
+
+def build_fortran_driver():
+    """Compiles the Fortran test driver of the engine (include/rwpt.f90 +
+    tests/fortran/drive_engine.f90, linked against librwpt.so) where
+    amdflang is present; the library is found next to the driver's tree."""
+    if not os.path.exists(FLANG):
+        return False
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod:
+        tmp = FORTRAN_DRIVER + ".tmp"
+        _run([FLANG, "-O1", "-J", mod, "-o", tmp, os.path.join(INCLUDE, "rwpt.f90"),
+              os.path.join(ROOT, "tests", "fortran", "drive_engine.f90"),
+              "-L", os.path.dirname(LIB), "-lrwpt",
+              "-Wl,-rpath,$ORIGIN/../../modpath-rw_amd"])
+        os.replace(tmp, FORTRAN_DRIVER)
+    return True
+
+
 REFERENCE_SRC = "/root/reference/src"
 
 
