@@ -235,6 +235,8 @@ int rwpt_destroy(rwpt_engine *e) {
   dev_free(e->kde_weights);
   dev_free(e->xface);
   dev_free(e->yface);
+  for (void *m : e->gk_mem)
+    if (m) (void)hipFree(m);
   for (auto &T : e->timers) {
     if (T.start) (void)hipEventDestroy(T.start);
     if (T.stop) (void)hipEventDestroy(T.stop);

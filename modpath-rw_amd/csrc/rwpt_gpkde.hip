@@ -19,9 +19,9 @@
 // factors), and walks the footprint with the outer product.  The per-bin
 // formulas between the passes run on the device too, one thread per active
 // bin in FP64, so the loop keeps every per-bin array in HBM and the host
-// only reads back the widest kernel before each pass and three sums per
-// loop.  Scatters add with atomics, so the summation order (and the last
-// bits of the density) can vary from run to run.
+// only reads back the widest kernel and the tile-list length before each
+// pass and three sums per loop.  Scatters sum per grid tile in a fixed bin
+// order (no atomics), so the density is the same from run to run.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -220,18 +220,33 @@ __device__ void build_factors(double *lds, int cap, const GkGrid &G, const int c
   __syncthreads();
 }
 
-// density or curvature scatter: grid += w(bin) * K(bin)
-__global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
+// Scatters (density and curvature grids): grid += w(bin) * K(bin) over every
+// active bin, output-stationary.  gk_factors builds each bin's three folded
+// 1-D factor arrays once (one workgroup per bin) and stores them with the
+// bin's span and amplitude; every grid tile then lists the bins whose span
+// reaches it (sorted by tile, ascending bin within a tile) and one
+// workgroup per tile sums those bins' contributions for its cells in
+// registers and writes each cell once -- no atomics, and a summation order
+// fixed by the bin order.
+constexpr int kSpan = 9;  // per bin: lo[3], hi[3] (1-based, inclusive), base[3]
+
+__global__ void gk_factors(const int32_t *__restrict__ bins, int nact,
                            const double *__restrict__ hv, int hstride,
-                           const double *__restrict__ w, const GkGrid G,
-                           int kind, int der, int range, int cap,
-                           const uint8_t *__restrict__ skip,
-                           const int8_t *__restrict__ collapse,
-                           double *__restrict__ grid) {
+                           const double *__restrict__ w, const GkGrid G, int kind,
+                           int der, int range, int cap, const uint8_t *__restrict__ skip,
+                           const int8_t *__restrict__ collapse, double *__restrict__ fac,
+                           int32_t *__restrict__ span, double *__restrict__ amp) {
   extern __shared__ double lds[];
   const int n = blockIdx.x;
   if (n >= nact) return;
-  if (skip && skip[n]) return;
+  int32_t *sp = span + (int64_t)n * kSpan;
+  if (skip && skip[n]) {
+    if (threadIdx.x == 0) {
+      sp[0] = 1; sp[3] = 0;  // empty span
+      amp[n] = 0.0;
+    }
+    return;
+  }
   const int id = bins[n];
   const int c[3] = {id % G.nb[0] + 1, (id / G.nb[0]) % G.nb[1] + 1,
                     id / (G.nb[0] * G.nb[1]) + 1};
@@ -241,32 +256,171 @@ __global__ void gk_scatter(const int32_t *__restrict__ bins, int nact,
     for (int d = 0; d < 3; ++d) h[d] = G.active[d] ? h[d] : 0.0;
   Factors F;
   build_factors(lds, cap, G, c, h, kind, der, range, collapse ? collapse[n] : -1, F);
-  const double *ax = lds, *ay = lds + cap, *az = lds + 2 * cap;
-  const int sx = F.hi[0] - F.lo[0] + 1, sy = F.hi[1] - F.lo[1] + 1,
-            sz = F.hi[2] - F.lo[2] + 1;
-  const double a = w[n] * F.scale;
-  const int64_t tot = (int64_t)sx * sy * sz;
-  for (int64_t t = threadIdx.x; t < tot; t += blockDim.x) {
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((int64_t)sx * sy));
-    const int gi = F.lo[0] + i, gj = F.lo[1] + j, gk = F.lo[2] + k;
-    const double v = ax[gi - c[0] + F.m[0]] * ay[gj - c[1] + F.m[1]] *
-                     az[gk - c[2] + F.m[2]];
-    const int64_t g = (int64_t)(gi - 1) + (int64_t)G.nb[0] * ((gj - 1) + (int64_t)G.nb[1] * (gk - 1));
-    unsafeAtomicAdd(&grid[g], a * v);
+  double *out = fac + (int64_t)n * 3 * cap;
+  for (int d = 0; d < 3; ++d)
+    for (int i = threadIdx.x; i <= 2 * F.m[d]; i += blockDim.x)
+      out[d * cap + i] = lds[d * cap + i];
+  if (threadIdx.x == 0) {
+    const double a = w[n] * F.scale;
+    amp[n] = a;
+    for (int d = 0; d < 3; ++d) {
+      sp[d] = F.lo[d];
+      sp[3 + d] = F.hi[d];
+      sp[6 + d] = c[d] - F.m[d];
+    }
+    if (a == 0.0) sp[3] = sp[0] - 1;  // adds nothing: no tiles
+  }
+}
+
+struct Tiles {
+  int td[3];  // tile extent per dimension
+  int nt[3];  // tiles per dimension
+};
+
+__device__ __forceinline__ bool span_tiles(const int32_t *sp, const Tiles &T, int lo[3],
+                                           int hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    if (sp[3 + d] < sp[d]) return false;
+    lo[d] = (sp[d] - 1) / T.td[d];
+    hi[d] = (sp[3 + d] - 1) / T.td[d];
+  }
+  return true;
+}
+
+__global__ void gk_tile_count(int nact, const int32_t *__restrict__ span, const Tiles T,
+                              int32_t *__restrict__ count) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nact) return;
+  int lo[3], hi[3];
+  count[n] = span_tiles(span + (int64_t)n * kSpan, T, lo, hi)
+                 ? (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+                 : 0;
+}
+
+__global__ void gk_tile_pairs(int nact, const int32_t *__restrict__ span, const Tiles T,
+                              const int32_t *__restrict__ offs, int32_t *__restrict__ keys,
+                              int32_t *__restrict__ vals) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nact) return;
+  int lo[3], hi[3];
+  if (!span_tiles(span + (int64_t)n * kSpan, T, lo, hi)) return;
+  int o = offs[n];
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        keys[o] = i + T.nt[0] * (j + T.nt[1] * k);
+        vals[o] = n;
+        ++o;
+      }
+}
+
+__global__ void gk_fill_int(int64_t n, int32_t *__restrict__ a, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// [start, end) of every tile's run in the sorted pair list
+__global__ void gk_tile_bounds(int npairs, const int32_t *__restrict__ keys,
+                               int32_t *__restrict__ start, int32_t *__restrict__ end) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  const int32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) start[k] = i;
+  if (i == npairs - 1 || keys[i + 1] != k) end[k] = i + 1;
+}
+
+// One workgroup per tile, one cell per thread: the contributions of the
+// tile's bins in ascending bin order, then one read-add-write of the cell.
+// The bins are taken in chunks: each bin's amplitude and the parts of its
+// three factors over the tile (zero outside the bin's span) are staged in
+// LDS, so the inner loop is three LDS reads and three multiplies and adds.
+constexpr int kChunk = 32;
+constexpr int kTileSpan = 16;  // largest tile extent in one dimension
+
+__global__ void __launch_bounds__(kBlock)
+    gk_tile_accumulate(const Tiles T, const GkGrid G, const int32_t *__restrict__ start,
+                       const int32_t *__restrict__ end, const int32_t *__restrict__ vals,
+                       const int32_t *__restrict__ span, const double *__restrict__ amp,
+                       const double *__restrict__ fac, int cap, double *__restrict__ grid) {
+  __shared__ double s_f[kChunk][3][kTileSpan];
+  __shared__ double s_a[kChunk];
+  const int t = blockIdx.x;
+  const int q0 = start[t], q1 = end[t];
+  if (q1 <= q0) return;
+  const int tx = t % T.nt[0], ty = (t / T.nt[0]) % T.nt[1], tz = t / (T.nt[0] * T.nt[1]);
+  const int g0[3] = {tx * T.td[0] + 1, ty * T.td[1] + 1, tz * T.td[2] + 1};
+  const int tid = threadIdx.x;
+  const int li = tid % T.td[0], lj = (tid / T.td[0]) % T.td[1], lk = tid / (T.td[0] * T.td[1]);
+  const int gi = g0[0] + li, gj = g0[1] + lj, gk = g0[2] + lk;
+  // staging: the chunk's (bin, factor entry) pairs, bin-major, dealt to the
+  // threads in turn
+  const int tsum = T.td[0] + T.td[1] + T.td[2];
+  const int b_start = tid / tsum, e_start = tid - (tid / tsum) * tsum;
+  double acc = 0.0;
+  for (int q = q0; q < q1; q += kChunk) {
+    const int nb = min(kChunk, q1 - q);
+    for (int b = b_start, e = e_start; b < nb;) {
+      int o = e, d = 0;
+      while (d < 2 && o >= T.td[d]) {
+        o -= T.td[d];
+        ++d;
+      }
+      const int n = vals[q + b];
+      const int32_t *sp = span + (int64_t)n * kSpan;
+      const int g = g0[d] + o;
+      double f = 0.0;
+      if (g >= sp[d] && g <= sp[3 + d]) f = fac[((int64_t)n * 3 + d) * cap + g - sp[6 + d]];
+      s_f[b][d][o] = f;
+      if (e == 0) s_a[b] = amp[n];
+      e += kBlock;
+      while (e >= tsum) {
+        e -= tsum;
+        ++b;
+      }
+    }
+    __syncthreads();
+    for (int b = 0; b < nb; ++b)
+      acc += s_a[b] * (s_f[b][0][li] * s_f[b][1][lj] * s_f[b][2][lk]);
+    __syncthreads();
+  }
+  if (gi > G.nb[0] || gj > G.nb[1] || gk > G.nb[2]) return;
+  const int64_t g = (int64_t)(gi - 1) + (int64_t)G.nb[0] * ((gj - 1) + (int64_t)G.nb[1] * (gk - 1));
+  grid[g] += acc;
+}
+
+// The roughness gathers read the curvature grids transformed once per cell:
+// k_d = curv_d / binVol / delta_d^2 (3-D: squared, 2-D: as is), the same
+// expressions the gather would otherwise form at every footprint cell.
+__global__ void gk_curv_transform(int64_t ncells, int squared, double binVol, double cs0,
+                                  double cs1, double cs2, double *__restrict__ g0,
+                                  double *__restrict__ g1, double *__restrict__ g2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  double k0 = g0[i] / binVol, k1 = g1[i] / binVol;
+  k0 = k0 / cs0;
+  k1 = k1 / cs1;
+  if (squared) {
+    double k2 = g2[i] / binVol;
+    k2 = k2 / cs2;
+    g0[i] = k0 * k0;
+    g1[i] = k1 * k1;
+    g2[i] = k2 * k2;
+  } else {
+    g0[i] = k0;
+    g1[i] = k1;
   }
 }
 
 // sigma-kernel gathers.  mode 0: sum density*K; mode 1 (3-D roughness):
-// with k_d = curv_d / binVol / delta_d^2 the sums of kx^2, ky^2, kz^2,
-// kx^2 ky^2, kx^2 kz^2, ky^2 kz^2 (cross terms after squaring,
-// GridProjectedKDE.F90:2464-2469); mode 2 (2-D): k1^2, k2^2, k1 k2.
+// over the squared curvatures k_d^2 (gk_curv_transform) the sums of kx^2,
+// ky^2, kz^2, kx^2 ky^2, kx^2 kz^2, ky^2 kz^2 (cross terms after squaring,
+// GridProjectedKDE.F90:2464-2469); mode 2 (2-D): over k1, k2 the sums of
+// k1^2, k2^2, k1 k2.  Threads walk the footprint x fastest with the row
+// and layer carried incrementally (no division per cell).
 __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
                          const double *__restrict__ hv, int hstride, int kind,
-                         const GkGrid G,
-                         int mode, int cap, const double *__restrict__ g0,
-                         const double *__restrict__ g1,
-                         const double *__restrict__ g2, double binVol,
-                         double cs0, double cs1, double cs2,
+                         const GkGrid G, int mode, int cap, const double *__restrict__ g0,
+                         const double *__restrict__ g1, const double *__restrict__ g2,
                          double *__restrict__ out) {
   extern __shared__ double lds[];
   __shared__ double red[6][kBlock];
@@ -280,27 +434,29 @@ __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
   for (int d = 0; d < 3; ++d) h[d] = hv[(int64_t)n * hstride + (hstride == 1 ? 0 : d)];
   Factors F;
   build_factors(lds, cap, G, c, h, kind, 0, kRangeW, -1, F);
-  const double *ax = lds, *ay = lds + cap, *az = lds + 2 * cap;
+  const double *ax = lds + F.lo[0] - c[0] + F.m[0];
+  const double *ay = lds + cap + F.lo[1] - c[1] + F.m[1];
+  const double *az = lds + 2 * cap + F.lo[2] - c[2] + F.m[2];
   const int sx = F.hi[0] - F.lo[0] + 1, sy = F.hi[1] - F.lo[1] + 1,
             sz = F.hi[2] - F.lo[2] + 1;
+  const int64_t nx = G.nb[0], nxy = (int64_t)G.nb[0] * G.nb[1];
+  const int64_t g00 = (int64_t)(F.lo[0] - 1) + nx * (F.lo[1] - 1) + nxy * (F.lo[2] - 1);
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  const int64_t tot = (int64_t)sx * sy * sz;
-  for (int64_t t = threadIdx.x; t < tot; t += blockDim.x) {
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((int64_t)sx * sy));
-    const int gi = F.lo[0] + i, gj = F.lo[1] + j, gk = F.lo[2] + k;
-    const double kv = ax[gi - c[0] + F.m[0]] * ay[gj - c[1] + F.m[1]] *
-                      az[gk - c[2] + F.m[2]] * F.scale;
-    const int64_t g = (int64_t)(gi - 1) + (int64_t)G.nb[0] * ((gj - 1) + (int64_t)G.nb[1] * (gk - 1));
+  // element t = i + sx (j + sy k); thread t0 = threadIdx.x, stride B
+  const int B = blockDim.x;
+  const int di = B % sx, dr = B / sx;
+  int i = threadIdx.x % sx, j = threadIdx.x / sx, k = 0;
+  while (j >= sy && k < sz) {
+    j -= sy;
+    ++k;
+  }
+  while (k < sz) {
+    const int64_t g = g00 + i + nx * j + nxy * k;
+    const double kv = ax[i] * ay[j] * az[k] * F.scale;
     if (mode == 0) {
       acc[0] += g0[g] * kv;
     } else if (mode == 1) {
-      double kx = g0[g] / binVol, ky = g1[g] / binVol, kz = g2[g] / binVol;
-      kx = kx / cs0;
-      ky = ky / cs1;
-      kz = kz / cs2;
-      kx = kx * kx;
-      ky = ky * ky;
-      kz = kz * kz;
+      const double kx = g0[g], ky = g1[g], kz = g2[g];
       acc[0] += kx * kv;
       acc[1] += ky * kv;
       acc[2] += kz * kv;
@@ -308,12 +464,20 @@ __global__ void gk_gather(const int32_t *__restrict__ bins, int nact,
       acc[4] += (kx * kz) * kv;
       acc[5] += (ky * kz) * kv;
     } else {
-      double k1 = g0[g] / binVol, k2 = g1[g] / binVol;
-      k1 = k1 / cs0;
-      k2 = k2 / cs1;
+      const double k1 = g0[g], k2 = g1[g];
       acc[0] += (k1 * k1) * kv;
       acc[1] += (k2 * k2) * kv;
       acc[2] += (k1 * k2) * kv;
+    }
+    i += di;
+    j += dr;
+    if (i >= sx) {
+      i -= sx;
+      ++j;
+    }
+    while (j >= sy && k < sz) {
+      j -= sy;
+      ++k;
     }
   }
 #pragma unroll
@@ -356,17 +520,38 @@ __global__ void gk_flag_active(int64_t ncells, const double *__restrict__ counts
 }
 
 
+// Grow-only device buffer `slot` of the engine, at least `bytes` large.
+hipError_t gk_reserve(rwpt_engine *e, int slot, size_t bytes, void **out) {
+  bytes = std::max(bytes, (size_t)256);
+  if (e->gk_bytes[slot] < bytes) {
+    if (e->gk_mem[slot]) (void)hipFree(e->gk_mem[slot]);
+    e->gk_mem[slot] = nullptr;
+    e->gk_bytes[slot] = 0;
+    hipError_t err = hipMalloc(&e->gk_mem[slot], bytes);
+    if (err != hipSuccess) return err;
+    e->gk_bytes[slot] = bytes;
+  }
+  *out = e->gk_mem[slot];
+  return hipSuccess;
+}
+
+enum GkSlot : int {
+  kSlotBins = 0, kSlotFlags, kSlotSelect, kSlotPool, kSlotGrids, kSlotSmall,
+  kSlotFactors, kSlotSpans, kSlotPairs, kSlotSort, kSlotTiles
+};
+
 // indices of the non-zero histogram bins, ascending, and their number
-hipError_t compact_nonzero(const double *counts, int64_t ncells, int32_t *bins,
-                           int32_t *dcount, int *count, hipStream_t st) {
+hipError_t compact_nonzero(rwpt_engine *e, const double *counts, int64_t ncells,
+                           int32_t *bins, int32_t *dcount, int *count, hipStream_t st) {
   int32_t *flags = nullptr;
   void *tmp = nullptr;
   size_t bytes = 0;
   const hipcub::CountingInputIterator<int32_t> ids(0);
   hipError_t err = hipcub::DeviceSelect::Flagged(nullptr, bytes, ids, (const int32_t *)nullptr,
                                                  bins, dcount, (int)ncells, st);
-  if (err == hipSuccess) err = hipMalloc(&flags, (size_t)ncells * 4);
-  if (err == hipSuccess) err = hipMalloc(&tmp, std::max(bytes, (size_t)16));
+  if (err == hipSuccess)
+    err = gk_reserve(e, kSlotFlags, (size_t)ncells * 4, reinterpret_cast<void **>(&flags));
+  if (err == hipSuccess) err = gk_reserve(e, kSlotSelect, bytes, &tmp);
   if (err == hipSuccess) {
     hipLaunchKernelGGL(gk_flag_active, dim3((int)((ncells + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, st, ncells, counts, flags);
@@ -378,10 +563,93 @@ hipError_t compact_nonzero(const double *counts, int64_t ncells, int32_t *bins,
   int32_t c = 0;
   if (err == hipSuccess) err = hipMemcpyAsync(&c, dcount, 4, hipMemcpyDeviceToHost, st);
   if (err == hipSuccess) err = hipStreamSynchronize(st);
-  if (flags) (void)hipFree(flags);
-  if (tmp) (void)hipFree(tmp);
   *count = c;
   return err;
+}
+
+// Tile shape: 8 x 8 x 4 cells in 3-D, 16 x 16 over the two active
+// dimensions of a 2-D grid (one cell per thread of a tile's workgroup).
+Tiles make_tiles(const GkGrid &G) {
+  Tiles T{};
+  for (int d = 0; d < 3; ++d)
+    T.td[d] = G.ndim == 3 ? (d < 2 ? 8 : 4) : (G.active[d] ? 16 : 1);
+  for (int d = 0; d < 3; ++d) T.nt[d] = (G.nb[d] + T.td[d] - 1) / T.td[d];
+  return T;
+}
+
+// One scatter pass (grid += w K over the active bins) as the tiled,
+// output-stationary sum: per-bin factors, the (tile, bin) list sorted by
+// tile, one workgroup per tile.
+hipError_t scatter_pass(rwpt_engine *e, hipStream_t st, const GkGrid &G,
+                        const int32_t *bins, int N, const double *hv, int hstride,
+                        const double *w, int kind, int der, int range, int cap,
+                        const uint8_t *skip, const int8_t *collapse, double *grid) {
+  hipError_t err;
+  double *fac = nullptr, *amp = nullptr;
+  int32_t *span = nullptr, *cnt = nullptr, *offs = nullptr;
+  const size_t nB = (size_t)N;
+  if ((err = gk_reserve(e, kSlotFactors, nB * 3 * (size_t)cap * 8,
+                        reinterpret_cast<void **>(&fac))) != hipSuccess)
+    return err;
+  void *spans = nullptr;
+  if ((err = gk_reserve(e, kSlotSpans, nB * (kSpan * 4 + 8 + 8), &spans)) != hipSuccess)
+    return err;
+  amp = reinterpret_cast<double *>(spans);
+  span = reinterpret_cast<int32_t *>(amp + nB);
+  cnt = span + nB * kSpan;
+  offs = cnt + nB;
+  // one wavefront per bin: a bin's factors are at most 3 x cap values
+  hipLaunchKernelGGL(gk_factors, dim3(N), dim3(64), cap * 3 * 8, st, bins, N, hv,
+                     hstride, w, G, kind, der, range, cap, skip, collapse, fac, span, amp);
+  const Tiles T = make_tiles(G);
+  const unsigned bgrid = (unsigned)((N + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(gk_tile_count, dim3(bgrid), dim3(kBlock), 0, st, N, span, T, cnt);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  size_t bytes = 0;
+  if ((err = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, offs, N, st)) != hipSuccess)
+    return err;
+  void *tmp = nullptr;
+  if ((err = gk_reserve(e, kSlotSort, bytes, &tmp)) != hipSuccess) return err;
+  if ((err = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, offs, N, st)) != hipSuccess)
+    return err;
+  int32_t last[2] = {0, 0};
+  if ((err = hipMemcpyAsync(&last[0], offs + N - 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (err = hipMemcpyAsync(&last[1], cnt + N - 1, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return err;
+  const int P = last[0] + last[1];
+  if (P == 0) return hipSuccess;
+  int32_t *pairs = nullptr;
+  if ((err = gk_reserve(e, kSlotPairs, (size_t)P * 16, reinterpret_cast<void **>(&pairs))) !=
+      hipSuccess)
+    return err;
+  int32_t *keys = pairs, *vals = pairs + P, *skeys = pairs + 2 * (size_t)P,
+          *svals = pairs + 3 * (size_t)P;
+  hipLaunchKernelGGL(gk_tile_pairs, dim3(bgrid), dim3(kBlock), 0, st, N, span, T, offs, keys,
+                     vals);
+  const int ntiles = T.nt[0] * T.nt[1] * T.nt[2];
+  int bits = 1;
+  while ((1ll << bits) <= ntiles) ++bits;
+  bytes = 0;
+  if ((err = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, skeys, vals, svals, P, 0,
+                                                bits, st)) != hipSuccess)
+    return err;
+  if ((err = gk_reserve(e, kSlotSort, bytes, &tmp)) != hipSuccess) return err;
+  if ((err = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, skeys, vals, svals, P, 0,
+                                                bits, st)) != hipSuccess)
+    return err;
+  int32_t *tiles = nullptr;
+  if ((err = gk_reserve(e, kSlotTiles, (size_t)ntiles * 8,
+                        reinterpret_cast<void **>(&tiles))) != hipSuccess)
+    return err;
+  int32_t *tstart = tiles, *tend = tiles + ntiles;
+  hipLaunchKernelGGL(gk_fill_int, dim3((unsigned)((2 * (int64_t)ntiles + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, st, 2 * (int64_t)ntiles, tiles, 0);
+  hipLaunchKernelGGL(gk_tile_bounds, dim3((unsigned)((P + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, st, P, skeys, tstart, tend);
+  hipLaunchKernelGGL(gk_tile_accumulate, dim3(ntiles), dim3(kBlock), 0, st, T, G, tstart, tend,
+                     svals, span, amp, fac, cap, grid);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -947,18 +1215,15 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
 
   // active bins (ascending linear index = the reference's column-major order)
   int32_t *dbins = nullptr, *dcount = nullptr;
-  hipError_t err = hipMalloc(&dcount, 4);
+  hipError_t err = gk_reserve(e, kSlotBins, (size_t)ncells * 4 + 256,
+                              reinterpret_cast<void **>(&dbins));
   if (err != hipSuccess) return err;
-  err = hipMalloc(&dbins, (size_t)ncells * 4);
-  if (err != hipSuccess) return (void)hipFree(dcount), err;
+  dcount = dbins + ncells;
   int N = 0;
-  err = compact_nonzero(counts_dev, ncells, dbins, dcount, &N, st);
-  (void)hipFree(dcount);
+  err = compact_nonzero(e, counts_dev, ncells, dbins, dcount, &N, st);
   if (loops_run) *loops_run = 0;
-  if (err != hipSuccess || N == 0) {
-    (void)hipFree(dbins);
+  if (err != hipSuccess || N == 0)
     return err != hipSuccess ? err : hipMemsetAsync(density_dev, 0, ncells * 8, st);
-  }
 
   KernelDb db;
   K.useDb = opt->kernel_database != 0;
@@ -979,16 +1244,17 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   unsigned long long *du = nullptr;
   int8_t *dcol = nullptr;
   uint8_t *dskip = nullptr;
-  auto cleanup = [&]() {
-    for (void *p : {(void *)dbins, (void *)pool, (void *)grids, (void *)du, (void *)dcol,
-                    (void *)dskip})
-      if (p) (void)hipFree(p);
-  };
-  if ((err = hipMalloc(&pool, nDoubles * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&grids, 3 * (size_t)ncells * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&du, 8 * 8)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dcol, nB)) != hipSuccess) return cleanup(), err;
-  if ((err = hipMalloc(&dskip, nB)) != hipSuccess) return cleanup(), err;
+  // buffers belong to the engine and are reused by later reconstructions
+  void *small = nullptr;
+  if ((err = gk_reserve(e, kSlotPool, nDoubles * 8, reinterpret_cast<void **>(&pool))) !=
+          hipSuccess ||
+      (err = gk_reserve(e, kSlotGrids, 3 * (size_t)ncells * 8,
+                        reinterpret_cast<void **>(&grids))) != hipSuccess ||
+      (err = gk_reserve(e, kSlotSmall, 64 + 2 * nB, &small)) != hipSuccess)
+    return err;
+  du = reinterpret_cast<unsigned long long *>(small);
+  dcol = reinterpret_cast<int8_t *>(du + 8);
+  dskip = reinterpret_cast<uint8_t *>(dcol + nB);
   GkA A{};
   double *p = pool;
   auto take = [&](size_t k) {
@@ -1022,7 +1288,7 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   if (K.useDb &&
       (err = hipMemcpyAsync(dhod, db.hod.data(), db.n * 8, hipMemcpyHostToDevice, st)) !=
           hipSuccess)
-    return cleanup(), err;
+    return err;
   double *cg[3] = {grids, grids + ncells, grids + 2 * ncells};
 
   const dim3 binGrid((N + kBlock - 1) / kBlock), blk(kBlock);
@@ -1050,10 +1316,11 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     int cap = 0;
     if ((r = read_cap(&cap)) != hipSuccess) return r;
     hipLaunchKernelGGL(gk_fill, dim3(cellBlocks), blk, 0, st, ncells, density_dev, 0.0);
-    hipLaunchKernelGGL(gk_scatter, dim3(N), blk, cap * 3 * 8, st, dbins, N,
-                       (const double *)A.hw, 3, w, G, 0, 0, kRangeW, cap,
-                       (const uint8_t *)dskip,
-                       K.useDb ? (const int8_t *)dcol : (const int8_t *)nullptr, density_dev);
+    if ((r = scatter_pass(e, st, G, dbins, N, (const double *)A.hw, 3, w, 0, 0, kRangeW,
+                          cap, (const uint8_t *)dskip,
+                          K.useDb ? (const int8_t *)dcol : (const int8_t *)nullptr,
+                          density_dev)) != hipSuccess)
+      return r;
     hipLaunchKernelGGL(gk_div, dim3(cellBlocks), blk, 0, st, ncells, density_dev, binVol);
     hipLaunchKernelGGL(gk_extract, binGrid, blk, 0, st, dbins, N, density_dev, A.dens);
     return hipGetLastError();
@@ -1069,12 +1336,12 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     hipLaunchKernelGGL(gk_gather, dim3(N), blk, sigCap * 3 * 8, st, dbins, N,
                        (const double *)A.hs, 3, sigKind, G, 0, sigCap,
                        (const double *)density_dev, (const double *)nullptr,
-                       (const double *)nullptr, 1.0, 1.0, 1.0, 1.0, A.s6);
+                       (const double *)nullptr, A.s6);
     return hipGetLastError();
   };
 
   hipLaunchKernelGGL(gk_init, binGrid, blk, 0, st, N, K, A, hinit);
-  if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
+  if ((err = density_pass(wbins)) != hipSuccess) return err;
   hipLaunchKernelGGL(gk_keep_density, binGrid, blk, 0, st, N, A);
   // nEstimate and the net roughness start at zero, so the first ALMISE is 0
   double almiseOld = 0.0;
@@ -1087,47 +1354,50 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
     loops = m;
     hipLaunchKernelGGL(gk_reset, dim3(1), dim3(64), 0, st, du);
     // support sizes and nEstimate (:4816-4858)
-    if ((err = nest_pass()) != hipSuccess) return cleanup(), err;
+    if ((err = nest_pass()) != hipSuccess) return err;
     hipLaunchKernelGGL(gk_support, binGrid, blk, 0, st, N, K, A);
-    if ((err = nest_pass()) != hipSuccess) return cleanup(), err;
+    if ((err = nest_pass()) != hipSuccess) return err;
     // curvature bandwidths and kernels (:5300-5400), curvature grids
     // (:1905-2136)
-    if ((err = reset_cap()) != hipSuccess) return cleanup(), err;
+    if ((err = reset_cap()) != hipSuccess) return err;
     hipLaunchKernelGGL(gk_curv, binGrid, blk, 0, st, N, K, A);
     int sdCap = 0;
-    if ((err = read_cap(&sdCap)) != hipSuccess) return cleanup(), err;
+    if ((err = read_cap(&sdCap)) != hipSuccess) return err;
     for (int g = 0; g < 3; ++g)
       hipLaunchKernelGGL(gk_fill, dim3(cellBlocks), blk, 0, st, ncells, cg[g], 0.0);
     const int nder = nd == 3 ? 3 : 2;
     for (int q = 0; q < nder; ++q) {
       const int der = nd == 3 ? q : K.dims[q];
-      hipLaunchKernelGGL(gk_scatter, dim3(N), blk, sdCap * 3 * 8, st, dbins, N,
-                         (const double *)(A.hsd + 3 * der), 9, (const double *)wbins, G, 2,
-                         der, kRangeSD, sdCap, (const uint8_t *)dskip,
-                         (const int8_t *)nullptr, cg[q]);
+      if ((err = scatter_pass(e, st, G, dbins, N, (const double *)(A.hsd + 3 * der), 9,
+                              (const double *)wbins, 2, der, kRangeSD, sdCap,
+                              (const uint8_t *)dskip, (const int8_t *)nullptr, cg[q])) !=
+          hipSuccess)
+        return err;
     }
     // roughness: sigma gathers of the squared curvatures (:2137-2709)
+    hipLaunchKernelGGL(gk_curv_transform, dim3(cellBlocks), blk, 0, st, ncells,
+                       nd == 3 ? 1 : 0, binVol, c0, c1, c2, cg[0], cg[1], cg[2]);
     hipLaunchKernelGGL(gk_gather, dim3(N), blk, sigCap * 3 * 8, st, dbins, N,
                        (const double *)A.hs, 3, sigKind, G, nd == 3 ? 1 : 2, sigCap,
                        (const double *)cg[0], (const double *)cg[1], (const double *)cg[2],
-                       binVol, c0, c1, c2, A.s6);
+                       A.s6);
     hipLaunchKernelGGL(gk_rough, binGrid, blk, 0, st, N, K, A);
     // optimal smoothing and shape (:5404-5724)
     hipLaunchKernelGGL(gk_scale, binGrid, blk, 0, st, N, K, A);
     hipLaunchKernelGGL(gk_shape, binGrid, blk, 0, st, N, K, A);
     hipLaunchKernelGGL(gk_rescale, binGrid, blk, 0, st, N, K, A);
-    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
+    if ((err = hipGetLastError()) != hipSuccess) return err;
     // new density (:4930-4960)
-    if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
+    if ((err = density_pass(wbins)) != hipSuccess) return err;
     // convergence (:4962-5038)
     hipLaunchKernelGGL(gk_max_density, binGrid, blk, 0, st, N, A);
     hipLaunchKernelGGL(gk_conv, dim3(kSumBlocks), blk, 0, st, N, K, A);
     hipLaunchKernelGGL(gk_conv_total, dim3(1), dim3(kSumBlocks), 0, st, A, dsums);
     double sums[3];
-    if ((err = hipGetLastError()) != hipSuccess) return cleanup(), err;
+    if ((err = hipGetLastError()) != hipSuccess) return err;
     if ((err = hipMemcpyAsync(sums, dsums, 24, hipMemcpyDeviceToHost, st)) != hipSuccess)
-      return cleanup(), err;
-    if ((err = hipStreamSynchronize(st)) != hipSuccess) return cleanup(), err;
+      return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
     const double errDensity = std::sqrt(sums[0] / (double)N);
     const double errSmoothing = std::sqrt(sums[1] / (double)N);
     const double alm = std::sqrt(sums[2] / (double)N);
@@ -1149,12 +1419,10 @@ hipError_t launch_gpkde_opt(rwpt_engine *e, const rwpt_kde_options *opt,
   if (final_weights_dev) {
     // last reconstruction over the mass sums (:4730-4758)
     hipLaunchKernelGGL(gk_extract, binGrid, blk, 0, st, dbins, N, final_weights_dev, wbins);
-    if ((err = density_pass(wbins)) != hipSuccess) return cleanup(), err;
+    if ((err = density_pass(wbins)) != hipSuccess) return err;
   }
   if (loops_run) *loops_run = loops;
-  err = hipStreamSynchronize(st);
-  cleanup();
-  return err;
+  return hipStreamSynchronize(st);
 }
 
 void KernelDb::build(double min_hod, double delta_hod, double max_hod,

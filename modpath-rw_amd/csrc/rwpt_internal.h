@@ -189,6 +189,13 @@ struct rwpt_engine {
   double kde_last[4] = {0, 0, 0, 0};  // bandwidth, points, nPoints, effMass
   int32_t kde_loops = 0;              // optimisation loops of the last kde
   double *xface = nullptr, *yface = nullptr;  // DIS face coordinates X, Y
+  // grow-only device buffers of the gpkde optimisation loop (active bins,
+  // per-bin arrays, curvature grids, kernel factors, tile lists), kept for
+  // the engine's lifetime so that a reconstruction allocates nothing once
+  // they are large enough
+  static constexpr int kGkBuffers = 12;
+  void *gk_mem[kGkBuffers] = {};
+  size_t gk_bytes[kGkBuffers] = {};
   rwpt::KernelTimer timers[3];
   rwpt::Tuning tune;
   std::vector<hipEvent_t> walk_events;  // start/stop pairs per launch
