@@ -585,10 +585,75 @@ __device__ __noinline__ double newton_axis(double dt, double v, double g, double
 // exactly as the reference (including leaving exitFace untouched when no
 // axis yields a time).  The Newton solve of the exponential form runs in
 // FP64 for either arithmetic type.
+// FP32 quadratic exit time without branches: the time for one axis (0:
+// none) and, in hit, whether the particle sits on the interface (the cases
+// of the branched form above, evaluated with selects; the values a case
+// does not use may be infinite or NaN and are never selected).
+__device__ __forceinline__ float quadratic_axis_flat(float dI, float A, float B,
+                                                     float dtcur, bool &hit) {
+  const float arg = B * B + 4 * dI * A;
+  const bool caseA = (arg >= 0.0f) && (A != 0.0f);
+  const bool caseB = (A == 0.0f) && (B != 0.0f);
+  const float sq = __builtin_amdgcn_sqrtf(fmaxf(arg, 0.0f));
+  const float q = -0.5f * (B + copysignf(sq, B));
+  const float z1 = q * __builtin_amdgcn_rcpf(A);
+  const float z2 = q != 0.0f ? -dI * __builtin_amdgcn_rcpf(q) : 0.0f;
+  const bool p1 = z1 > 0.0f, p2 = z2 > 0.0f, pos = p1 || p2;
+  const float m = (p1 && p2) ? fminf(z1, z2) : (p1 ? z1 : z2);
+  const float dA = m * m;
+  const float r = dI * __builtin_amdgcn_rcpf(B);
+  const float dB = r * r;
+  const bool h = caseA ? (pos ? dA == 0.0f : (z1 == 0.0f || z2 == 0.0f))
+                       : (caseB && dB == 0.0f);
+  float dta = caseA ? (pos ? dA : 0.0f) : (caseB ? dB : 0.0f);
+  if (!h && dta > dtcur) dta = 0.0f;
+  hit = h;
+  return dta;
+}
+
 template <typename R, int ADV>
 __device__ void exit_time(const R *__restrict__ rec, R x, R y, R z, R nx, R ny,
                           R nz, const StepTerms<R> &s, double &t, R &dt,
                           int &exitFace) {
+  if constexpr (sizeof(R) == 4 && ADV == 2) {
+    // every axis evaluated, the reference's first early return (the
+    // particle on an interface) chosen afterwards in axis order
+    t = t - dt;
+    const float pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
+    const float dd[3] = {rec[F_DX], rec[F_DY], rec[F_DZ]};
+    const float A[3] = {s.vx + s.ddx, s.vy + s.ddy, s.vz + s.ddz};
+    const float db[3] = {s.dBx, s.dBy, s.dBz};
+    float dtx[3];
+    int fx[3], ret = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool up = pn[a] > 1.0f, out = up || (pn[a] < 0.0f);
+      const float dI = up ? dd[a] * (1.0f - pp[a]) : -dd[a] * pp[a];
+      fx[a] = up ? 2 * a + 2 : 2 * a + 1;
+      bool hit;
+      const float d = quadratic_axis_flat(dI, A[a], db[a], dt, hit);
+      const bool early = out && (dI == 0.0f || hit);
+      if (ret == 0 && early) ret = fx[a];
+      dtx[a] = (out && !early) ? d : 0.0f;
+    }
+    if (ret != 0) {
+      exitFace = ret;
+      dt = 0.0f;
+      return;
+    }
+    if (dtx[0] == 0.0f && dtx[1] == 0.0f && dtx[2] == 0.0f) {
+      t = t + dt;
+      return;
+    }
+    int im = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (dtx[a] > 0.0f && (im < 0 || dtx[a] < dtx[im])) im = a;
+    dt = im == 0 ? dtx[0] : (im == 1 ? dtx[1] : dtx[2]);
+    exitFace = im == 0 ? fx[0] : (im == 1 ? fx[1] : fx[2]);
+    t = t + dt;
+    return;
+  } else {
   t = t - dt;
   const R pn[3] = {nx, ny, nz}, pp[3] = {x, y, z};
   const R dd[3] = {rec[F_DX], rec[F_DY], rec[F_DZ]};
@@ -638,6 +703,7 @@ __device__ void exit_time(const R *__restrict__ rec, R x, R y, R z, R nx, R ny,
   dt = dtx[im];
   exitFace = fx[im];
   t = t + dt;
+  }
 }
 
 
@@ -726,7 +792,20 @@ __device__ int iterate(const R *__restrict__ rec, const WalkParams &W,
       break;
     }
     sq = Mx::sq(dt);
-    if (exitFace <= 2) {
+    if constexpr (sizeof(R) == 4) {
+      // the point at the exit time, the exit axis on its face (selects)
+      const R ex = s.dAx + s.ddx * dt + s.dBx * sq;
+      const R ey = s.dAy + s.ddy * dt + s.dBy * sq;
+      const R ez = s.dAz + s.ddz * dt + s.dBz * sq;
+      const int ax = (exitFace - 1) >> 1;
+      const R side = (exitFace & 1) ? R(0) : R(1);
+      if (ax != 0) dxr = ex;
+      if (ax != 1) dyr = ey;
+      if (ax != 2) dzr = ez;
+      nx = ax == 0 ? side : x + Mx::divr(ex, DX, rX);
+      ny = ax == 1 ? side : y + Mx::divr(ey, DY, rY);
+      nz = ax == 2 ? side : z + Mx::divr(ez, DZ, rZ);
+    } else if (exitFace <= 2) {
       nx = exitFace == 1 ? R(0) : R(1);
       dyr = s.dAy + s.ddy * dt + s.dBy * sq;
       ny = y + Mx::divr(dyr, DY, rY);
@@ -990,17 +1069,14 @@ __global__ void __launch_bounds__(256, MINW)
           done = true;
           break;
         }
-        // ConvertFromNeighbor: full-face connection, face coordinate flips
-        bool bad;
-        R fx = V.x, fy = V.y, fz = V.z;
-        switch (face) {
-          case 1: bad = fx != R(0); fx = R(1); break;
-          case 2: bad = fx != R(1); fx = R(0); break;
-          case 3: bad = fy != R(0); fy = R(1); break;
-          case 4: bad = fy != R(1); fy = R(0); break;
-          case 5: bad = fz != R(0); fz = R(1); break;
-          default: bad = fz != R(1); fz = R(0); break;
-        }
+        // ConvertFromNeighbor: full-face connection, the face coordinate
+        // (0 on odd faces, 1 on even ones) flips
+        const int ax = (face - 1) >> 1;
+        const R side = (face & 1) ? R(0) : R(1);
+        const bool bad = (ax == 0 ? V.x : (ax == 1 ? V.y : V.z)) != side;
+        const R fx = ax == 0 ? R(1) - side : V.x;
+        const R fy = ax == 1 ? R(1) - side : V.y;
+        const R fz = ax == 2 ? R(1) - side : V.z;
         if (bad) {  // no defined continuation in the reference
           status = 9;
           done = true;

@@ -351,6 +351,39 @@ def config3(members=256, seed=20240602, n_min=10_000, n_max=1_000_000):
     return E, counts
 
 
+def member(E: Ensemble, m: int) -> Problem:
+    """Member m of an ensemble as a problem of its own (the same member run
+    alone, with stream_offset = its first particle's index, reproduces its
+    particles in the ensemble)."""
+    n = E.ncell_member
+    cells = slice(m * n, (m + 1) * n)
+
+    def cut(a, per=1):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        return a[per * m * n:per * (m + 1) * n].copy()
+
+    def disp(a):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        if a.size == E.members and E._per_member:
+            return a[m:m + 1].copy()
+        return a[cells].copy()
+
+    return Problem(ncol=E.ncol, nrow=E.nrow, nlay=E.nlay, delr=E.delr, delc=E.delc,
+                   top=np.asarray(E.top)[:n], bot=np.asarray(E.bot)[:n],
+                   frf=cut(E.frf), fff=cut(E.fff), flf=cut(E.flf), porosity=cut(E.porosity),
+                   bflow=cut(E.bflow, 6), retardation=cut(E.retardation),
+                   ibound=cut(E.ibound), icbound=cut(E.icbound),
+                   default_icbound=E.default_icbound, disp_model=E.disp_model,
+                   aL=disp(E.aL), aT=disp(E.aT), aLV=disp(E.aLV), aTV=disp(E.aTV),
+                   dmeff=disp(E.dmeff), advection_kind=E.advection_kind,
+                   time_step_kind=E.time_step_kind, ts_params=E.ts_params,
+                   dim_mask=E.dim_mask, backward=E.backward)
+
+
 def release_ensemble(E, counts, cols, rows, lays, seed=1):
     """counts[m] particles of member m uniformly inside the block of cells,
     member-major (the order of the members' rng stream offsets)."""
