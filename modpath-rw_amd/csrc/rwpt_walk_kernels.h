@@ -792,38 +792,18 @@ __device__ int iterate(const R *__restrict__ rec, const WalkParams &W,
       break;
     }
     sq = Mx::sq(dt);
-    if constexpr (sizeof(R) == 4) {
-      // the point at the exit time, the exit axis on its face (selects)
-      const R ex = s.dAx + s.ddx * dt + s.dBx * sq;
-      const R ey = s.dAy + s.ddy * dt + s.dBy * sq;
-      const R ez = s.dAz + s.ddz * dt + s.dBz * sq;
-      const int ax = (exitFace - 1) >> 1;
-      const R side = (exitFace & 1) ? R(0) : R(1);
-      if (ax != 0) dxr = ex;
-      if (ax != 1) dyr = ey;
-      if (ax != 2) dzr = ez;
-      nx = ax == 0 ? side : x + Mx::divr(ex, DX, rX);
-      ny = ax == 1 ? side : y + Mx::divr(ey, DY, rY);
-      nz = ax == 2 ? side : z + Mx::divr(ez, DZ, rZ);
-    } else if (exitFace <= 2) {
-      nx = exitFace == 1 ? R(0) : R(1);
-      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + Mx::divr(dyr, DY, rY);
-      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + Mx::divr(dzr, DZ, rZ);
-    } else if (exitFace <= 4) {
-      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + Mx::divr(dxr, DX, rX);
-      ny = exitFace == 3 ? R(0) : R(1);
-      dzr = s.dAz + s.ddz * dt + s.dBz * sq;
-      nz = z + Mx::divr(dzr, DZ, rZ);
-    } else {
-      dxr = s.dAx + s.ddx * dt + s.dBx * sq;
-      nx = x + Mx::divr(dxr, DX, rX);
-      dyr = s.dAy + s.ddy * dt + s.dBy * sq;
-      ny = y + Mx::divr(dyr, DY, rY);
-      nz = exitFace == 5 ? R(0) : R(1);
-    }
+    // the point at the exit time, the exit axis on its face
+    const R ex = s.dAx + s.ddx * dt + s.dBx * sq;
+    const R ey = s.dAy + s.ddy * dt + s.dBy * sq;
+    const R ez = s.dAz + s.ddz * dt + s.dBz * sq;
+    const int ax = (exitFace - 1) >> 1;
+    const R side = (exitFace & 1) ? R(0) : R(1);
+    if (ax != 0) dxr = ex;
+    if (ax != 1) dyr = ey;
+    if (ax != 2) dzr = ez;
+    nx = ax == 0 ? side : x + Mx::divr(ex, DX, rX);
+    ny = ax == 1 ? side : y + Mx::divr(ey, DY, rY);
+    nz = ax == 2 ? side : z + Mx::divr(ez, DZ, rZ);
     if (outside(nx, ny, nz)) continue;
     // elastic rebound on faces whose mass boundary is 1
     int rebounds = 0;
