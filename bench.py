@@ -235,37 +235,54 @@ def ensemble_run(intervals, warmup):
             "active_after": active, "tuning": tune}
 
 
-def adjoint_run(n=1_000_000):
+def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
     """The calibration workload: the config-1 field with 8 x-bands of 25
     columns (own aL, aT each), a common Dm and porosity scale (18
     parameters), 1e6 particles, 300 RW steps of 0.25 d in 12 checkpoint
     windows of 25 steps, misfit of the Gaussian-smoothed (h = 2, 2, 1 m)
     concentrations at 32 observation points over the plume against 0.8 x
     the model's own.
-    Times the forward (rwpt_adjoint without gradient) and the adjoint, then
-    checks the pathwise gradient against central differences with common
-    random numbers: on a 4-step horizon, where no particle changes branch
-    at a relative step of 1e-8, and on the full horizon, where the tangent
-    of the walk grows multiplicatively (position-dependent dispersion) and
-    steps down to 1e-10 still move most particles' end points."""
+
+    Times the forward (rwpt_adjoint without gradient) and the adjoint, and
+    checks the gradient with respect to aL of zone 1 and to the porosity
+    scale:
+    - on a 4-step horizon against central differences at a relative step
+      of 1e-8 (no particle changes branch: the two must agree to rounding);
+    - on the full horizon against central differences at the relative
+      steps `rel_steps` with common random numbers within each +- pair,
+      averaged over `seeds` base seeds, beside the gradient estimators of
+      the same seeds: the exact pathwise gradient, the adjoint restarted
+      straight-through at every window start (tangent_windows 1), the
+      gradient without the random displacements' derivative with respect
+      to the sub-step duration (frozen_noise), and both together.
+      Reports each estimator's seed mean, standard error and bias against
+      the central differences at each step (paired by seed, with the
+      standard error of the difference);
+    - with the sweep's diagnostics (seed 0, pathwise): the adjoint growth
+      per iteration kind (sub-step, jump-limiter cut, exit, grazing exit,
+      rare), the exits by their distance from a double root, and each
+      particle's growth over the horizon."""
     from rwpt import Engine, problems
+    from rwpt.engine import default_options
     dt, every, total = 0.25, 25, 300
     P, zones, (aL, aT, _, dm) = problems.config4(ts_params=(dt, 0.0))
     rel = problems.release_in_block(P, n, (1, 10), (76, 125), (1, 20))
     bw = (2.0, 2.0, 1.0)
+    seed0 = tuple(int(x) for x in default_options().base_seed)
 
-    def run(windows, obs, a_l, s=1.0, grad=False, target=None):
+    def run(windows, obs, a_l=aL, por=1.0, grad=False, target=None, seed=0, opts=None):
         Q = problems.zoned_dispersion(P, zones, a_l, aT, dm)
-        Q.porosity = P.porosity * s
-        E = Engine(Q)
+        Q.porosity = P.porosity * por
+        E = Engine(Q, base_seed=((seed0[0] + 7919 * seed) % 2**64, seed0[1]))
         try:
             apply_tuning(E, TUNED_NAME_1[0])
+            if opts:
+                E.adjoint_options(**opts)
             E.load_particles(*rel)
             out = E.adjoint(windows, obs, np.zeros(len(obs)) if target is None else target,
                             bw, gradient=grad)
-            g = E.particles()
-            out["pos"] = np.stack([g["lx"], g["ly"], g["lz"]])
-            out["cell"] = g["cell"].copy()
+            if opts and opts.get("diagnostics"):
+                out["stats"] = E.adjoint_stats()
         finally:
             E.close()
         return out
@@ -289,26 +306,74 @@ def adjoint_run(n=1_000_000):
         return np.array([(x, y, zmid) for y in np.linspace(my - 1.5 * sy, my + 1.5 * sy, 4)
                          for x in np.linspace(mx - 1.5 * sx, mx + 1.5 * sx, 8)])
 
-    def fd_check(windows, rel_step):
-        obs = plume_points(windows)
-        ref = run(windows, obs, aL)
-        tgt = 0.8 * ref["conc"]
-        fwd = run(windows, obs, aL, target=tgt)
-        A = run(windows, obs, aL, grad=True, target=tgt)
-        g = float(A["grad_al"][zones == 0].sum())
+    def central(windows, obs, tgt, param, h, seed):
         vals = []
         for sg in (1, -1):
-            a2 = aL.copy()
-            a2[0] *= 1 + sg * rel_step
-            vals.append(run(windows, obs, a2, target=tgt))
-        fd = (vals[0]["misfit"] - vals[1]["misfit"]) / (2 * rel_step * aL[0])
-        moved = int(((np.abs(vals[0]["pos"] - vals[1]["pos"]).max(axis=0) > 1e2 * rel_step) |
-                     (vals[0]["cell"] != vals[1]["cell"])).sum())
-        return fwd, A, g, fd, moved
+            if param == "aL_zone1":
+                a2 = aL.copy()
+                a2[0] *= 1 + sg * h
+                vals.append(run(windows, obs, a_l=a2, target=tgt, seed=seed)["misfit"])
+            else:
+                vals.append(run(windows, obs, por=1 + sg * h, target=tgt, seed=seed)["misfit"])
+        return (vals[0] - vals[1]) / (2 * h * (aL[0] if param == "aL_zone1" else 1.0))
 
+    def pick(out):
+        return {"aL_zone1": float(out["grad_al"][zones == 0].sum()),
+                "porosity_scale": float(out["grad_porosity_scale"])}
+
+    params = ("aL_zone1", "porosity_scale")
+    # 4-step check: no branch changes at 1e-8
+    short = [dt * 4]
+    obs_s = plume_points(short)
+    tgt_s = 0.8 * run(short, obs_s)["conc"]
+    g_short = pick(run(short, obs_s, grad=True, target=tgt_s))
+    fd_short = {p: central(short, obs_s, tgt_s, p, 1e-8, 0) for p in params}
+
+    # full horizon: cost, diagnostics, seed-averaged central differences
     full = [dt * every * (k + 1) for k in range(total // every)]
-    fwd, A, g_full, fd_full, moved_full = fd_check(full, 1e-10)
-    _, _, g_short, fd_short, moved_short = fd_check([dt * 4], 1e-8)
+    obs = plume_points(full)
+    tgt = 0.8 * run(full, obs)["conc"]
+    fwd = run(full, obs, target=tgt)
+    A = run(full, obs, grad=True, target=tgt)
+    diag = run(full, obs, grad=True, target=tgt, opts={"diagnostics": True})["stats"]
+    estimators = {"pathwise": None,
+                  "restart_every_window": {"tangent_windows": 1},
+                  "frozen_noise": {"frozen_noise": True},
+                  "frozen_noise_restart_every_window": {"frozen_noise": True,
+                                                        "tangent_windows": 1},
+                  "frozen_noise_graze_cut_0.1_restart_every_window": {
+                      "frozen_noise": True, "tangent_windows": 1, "graze_cut": 0.1}}
+    g = {k: {p: [] for p in params} for k in estimators}
+    fd = {h: {p: [] for p in params} for h in rel_steps}
+    for s in range(seeds):
+        for h in rel_steps:
+            for p in params:
+                fd[h][p].append(central(full, obs, tgt, p, h, s))
+        for k, o in estimators.items():
+            v = pick(run(full, obs, grad=True, target=tgt, seed=s, opts=o))
+            for p in params:
+                g[k][p].append(v[p])
+        print(f"[bench] adjoint seed {s}: " + "; ".join(
+            f"{p}: central " + ", ".join(f"{fd[h][p][-1]:.4e}" for h in rel_steps) + ", "
+            + ", ".join(f"{k} {g[k][p][-1]:.4e}" for k in estimators) for p in params),
+            file=sys.stderr, flush=True)
+
+    def stats(x, refs=None):
+        x = np.asarray(x)
+        out = {"mean": float(x.mean()), "std_error": float(x.std(ddof=1) / np.sqrt(len(x))),
+               "per_seed": x.tolist()}
+        for h, ref in (refs or {}).items():
+            d = x - np.asarray(ref)
+            out[f"bias_vs_central_{h:g}"] = {
+                "mean": float(d.mean()), "std_error": float(d.std(ddof=1) / np.sqrt(len(d)))}
+        return out
+
+    checks = {}
+    for p in params:
+        refs = {h: fd[h][p] for h in rel_steps}
+        checks[p] = {"central_difference": {f"rel_step_{h:g}": stats(fd[h][p])
+                                            for h in rel_steps},
+                     "estimators": {k: stats(g[k][p], refs) for k in estimators}}
     return {"workload": "config-1 field, 8 x-bands of 25 columns with own aL, aT; Dm and "
                         "porosity scale; 1e6 particles, 300 RW steps of 0.25 d, 12 "
                         "checkpoint windows; misfit at 32 observation points",
@@ -316,16 +381,11 @@ def adjoint_run(n=1_000_000):
             "forward_ms": fwd["ms"], "adjoint_ms": A["ms"],
             "adjoint_over_forward": A["ms"] / fwd["ms"],
             "misfit": A["misfit"],
-            "fd_check_4_steps": {"grad_aL_zone1": g_short, "central_difference": fd_short,
-                                 "rel_step": 1e-8, "ratio": g_short / fd_short,
-                                 "particles_moved": moved_short},
-            "fd_check_300_steps": {"grad_aL_zone1": g_full, "central_difference": fd_full,
-                                   "rel_step": 1e-10, "ratio": g_full / fd_full,
-                                   "particles_moved": moved_full,
-                                   "note": "particles whose end point moved by more than "
-                                           "100 x the relative step (cells): the "
-                                           "pathwise map is not differentiable in "
-                                           "practice over this horizon"}}
+            "fd_check_4_steps": {p: {"adjoint": g_short[p], "central_difference": fd_short[p],
+                                     "rel_step": 1e-8, "ratio": g_short[p] / fd_short[p]}
+                                 for p in params},
+            "fd_check_300_steps": {"seeds": seeds, **checks},
+            "sweep_diagnostics": diag}
 
 
 

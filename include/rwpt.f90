@@ -17,8 +17,13 @@ module RwptGpuModule
   public :: rwpt_kernel_time, rwpt_reset_kernel_time, rwpt_get_cell_tables
   public :: rwpt_create_ensemble, rwpt_autotune, rwpt_write_timeseries_header
   public :: rwpt_write_timeseries, rwpt_write_endpoints, rwpt_adjoint
-  public :: rwpt_write_observation
+  public :: rwpt_write_observation, rwpt_adjoint_options, rwpt_adjoint_stats
+  public :: rwpt_adjoint_opts
 
+  type, bind(C) :: rwpt_adjoint_opts
+    real(c_double)     :: graze_cut
+    integer(c_int32_t) :: reverse_windows, tangent_windows, frozen_noise, diagnostics
+  end type
   type, bind(C) :: rwpt_grid
     integer(c_int32_t) :: ncol, nrow, nlay
     type(c_ptr) :: delr, delc, top, bot          ! c_loc of real(c_double) arrays
@@ -266,6 +271,16 @@ module RwptGpuModule
       type(c_ptr), value :: mass                    ! c_loc of the masses or c_null_ptr
       real(c_double), intent(out) :: misfit, conc(*)
       type(c_ptr), value :: grad_al, grad_at, grad_dm, grad_porosity_scale, ms
+    end function
+    integer(c_int) function rwpt_adjoint_options(e, opt) bind(C, name='rwpt_adjoint_options')
+      import :: c_int, c_ptr, rwpt_adjoint_opts
+      type(c_ptr), value :: e
+      type(rwpt_adjoint_opts), intent(in) :: opt
+    end function
+    integer(c_int) function rwpt_adjoint_stats(e, out) bind(C, name='rwpt_adjoint_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: out(33)
     end function
   end interface
 end module RwptGpuModule

@@ -395,6 +395,65 @@ int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_windows,
                  double *conc, double *grad_al, double *grad_at, double *grad_dm,
                  double *grad_porosity_scale, double *ms);
 
+/* Options of rwpt_adjoint's reverse sweep (no counterpart in the
+ * reference).  The exact pathwise gradient of the walk grows along a path
+ * by a random factor at every cell exit: a sub-step that ends at a face at
+ * time tau is followed by one over the rest of the time step with fresh
+ * draws, so the final position depends on tau through both random
+ * displacements, B xi1 sqrt(tau) and B xi2 sqrt(dt - tau), and the
+ * Jacobian of one crossing is (v + B xi2 / (2 sqrt(dt - tau))) /
+ * (v + B xi1 / (2 sqrt(tau))) in one dimension (1 for the continuous
+ * process).  These options trade that variance for a bias; all zero (the
+ * default) is the exact pathwise gradient. */
+typedef struct {
+  /* exits with kappa = |2 A m + B| / (|2 A m| + |B|) < graze_cut lose the
+   * derivative of their exit time (the exit root m = sqrt(dt) of
+   * A m^2 + B m - dI = 0 on the exit axis; kappa near 0 is a grazing,
+   * double root); 0 keeps every term */
+  double graze_cut;
+  /* K > 0 sweeps only the last K windows: the gradient holds the
+   * sensitivities of those windows, the state at the start of window
+   * n_windows - K counting as fixed */
+  int32_t reverse_windows;
+  /* L > 0 restarts the adjoint at the start of every L-th window counted
+   * from the last (straight-through): each particle's adjoint becomes its
+   * final-time adjoint of the global position, expressed in the cell it
+   * starts that window in, with a zero time adjoint, as if the transport
+   * from there to the final time moved every displacement unchanged */
+  int32_t tangent_windows;
+  /* 1 drops the derivative of every random displacement B xi sqrt(dt) with
+   * respect to the sub-step duration dt (exit roots keep theirs), which
+   * makes the crossing Jacobian above v / (v + B xi1 / (2 sqrt(tau))) */
+  int32_t frozen_noise;
+  /* 1 makes the next gradients tally the sweep's statistics for
+   * rwpt_adjoint_stats (a slower sweep) */
+  int32_t diagnostics;
+} rwpt_adjoint_opts;
+
+/* Sets the options of the engine's following rwpt_adjoint calls. */
+int rwpt_adjoint_options(rwpt_engine *e, const rwpt_adjoint_opts *opt);
+
+/* Statistics of the last rwpt_adjoint gradient taken with diagnostics on,
+ * RWPT_ADJ_NSTATS doubles.  Every reverse iteration is one of five kinds
+ * k: 0 a sub-step inside the cell, 1 a sub-step cut by the jump limiter,
+ * 2 a cell exit with kappa >= 1e-3, 3 a cell exit with kappa < 1e-3,
+ * 4 a rare iteration (restart, rebound; dual-number Jacobians).  Its
+ * growth is |adjoint of its input| / |adjoint of its output| (L1 norm of
+ * the x, y, z, t adjoints).
+ *   out[k]       iterations of kind k
+ *   out[5 + k]   sum of log10(growth) over those with growth > 1
+ *   out[10 + k]  largest log10(growth) (0 when none grows)
+ *   out[15 + k]  iterations with growth > 10
+ *   out[20 .. 26] exits by kappa: < 1e-10, [1e-10, 1e-8), [1e-8, 1e-6),
+ *                [1e-6, 1e-4), [1e-4, 1e-2), [1e-2, 1e-1), >= 1e-1
+ *   out[27]      exits whose exit-time derivative graze_cut dropped
+ *   out[28]      particles swept; out[29] the largest log10 of a
+ *                particle's adjoint growth from the final time to the
+ *                start of the first window swept; out[30 .. 32] particles
+ *                whose growth exceeds 1e3, 1e6, 1e9 */
+#define RWPT_ADJ_NSTATS 33
+int rwpt_adjoint_stats(rwpt_engine *e, double *out);
+
 /* Device timing of the last rwpt_advance/rwpt_kde calls, measured with HIP
  * events on the engine stream: which = 0 advance kernel, 1 histogram,
  * 2 density.  Returns the summed milliseconds and the launch count. */

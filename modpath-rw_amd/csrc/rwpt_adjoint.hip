@@ -898,12 +898,21 @@ __device__ void disp_full(const double *rec, const WalkParams &W, const double p
   }
 }
 
+// Options of the reverse sweep (rwpt_adjoint_options).
+struct SweepOpts {
+  double graze_cut;  // exits with kappa below it lose their exit-time derivative
+  int frozen_noise;  // 1: the random displacement's derivative with respect
+                     // to the sub-step duration is dropped
+};
+
 // Returns false (nothing changed) for an iteration on a rare branch; the
-// branch taken and the draws come from the tape.
+// branch taken and the draws come from the tape.  At a cell exit, kappa
+// receives |2 A m + B| / (|2 A m| + |B|) of the exit root (1 otherwise), and
+// a root with kappa < O.graze_cut contributes no exit-time derivative.
 template <int NDIM>
 __device__ bool reverse_step(const double *rec, const double *tan, const double *dd3,
                              const WalkParams &W, const TapeEntry &E, const double l[4],
-                             double ln[4], double g[4]) {
+                             double ln[4], double g[4], const SweepOpts &O, double &kappa) {
   const int32_t fl = E.flags;
   const int out = (fl >> 14) & 7;
   if ((fl & (1 << 11)) || out == IT_UNDEFINED) return false;
@@ -930,7 +939,7 @@ __device__ bool reverse_step(const double *rec, const double *tan, const double 
     lv[a] += w * dt;
     ldd[a] += w * dt;
     ldB[a] += w * sq;
-    ldt += w * (D.v[a] + D.dd[a] + D.dB[a] * hs);
+    ldt += w * (D.v[a] + D.dd[a] + (O.frozen_noise ? 0.0 : D.dB[a] * hs));
   }
   if (exited) {
     // t_out = t + dt_exit
@@ -940,7 +949,9 @@ __device__ bool reverse_step(const double *rec, const double *tan, const double 
       // the root of A m^2 + B m - dI = 0 on the exit axis, A = v + dd, B = dB
       const double A = pick(D.v, im) + pick(D.dd, im), Bq = pick(D.dB, im), m = E.m;
       const double den = 2.0 * A * m + Bq;
-      if (den != 0.0) {
+      const double scale = fabs(2.0 * A * m) + fabs(Bq);
+      kappa = scale > 0.0 ? fabs(den) / scale : 0.0;
+      if (den != 0.0 && !(kappa < O.graze_cut)) {
         const double q = 2.0 * m * ldt / den;
         bump(lx, im, -pick(DXv, im) * q);   // d(dI)/dx = -DX on either face
         bump(lv, im, -m * m * q);
@@ -1283,18 +1294,81 @@ __device__ __forceinline__ void flush(int c, const double g[4], double *g_al, do
   g_s += g[3];
 }
 
+// Tallies of the reverse sweep for rwpt_adjoint_stats, per thread; the
+// layout of the reduced array is the one rwpt_adjoint_stats documents.
+struct SweepStats {
+  int n[5] = {0, 0, 0, 0, 0};            // iterations per kind
+  float lg[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // sum of log10(growth) > 0
+  float mx[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // largest log10(growth)
+  int big[5] = {0, 0, 0, 0, 0};          // growth > 10
+  int kap[7] = {0, 0, 0, 0, 0, 0, 0};    // exits by kappa decade
+  int cut = 0;                           // exit-time derivatives dropped
+
+  __device__ void add(int kind, const double lo[4], const double ln[4]) {
+    const double a = fabs(lo[0]) + fabs(lo[1]) + fabs(lo[2]) + fabs(lo[3]);
+    const double b = fabs(ln[0]) + fabs(ln[1]) + fabs(ln[2]) + fabs(ln[3]);
+    n[kind] += 1;
+    if (a > 0.0 && b > a) {
+      const float g = (float)log10(b / a);
+      lg[kind] += g;
+      mx[kind] = fmaxf(mx[kind], g);
+      big[kind] += g > 1.f ? 1 : 0;
+    }
+  }
+  __device__ void exit_kappa(double k, double cut_at) {
+    const int b = k < 1e-10 ? 0 : k < 1e-8 ? 1 : k < 1e-6 ? 2 : k < 1e-4 ? 3
+                : k < 1e-2 ? 4 : k < 1e-1 ? 5 : 6;
+    kap[b] += 1;
+    cut += k < cut_at ? 1 : 0;
+  }
+};
+
+// Wave sums (maxima for the log growth maxima) of the tallies, added to
+// out[0 .. 27] by lane 0.  Every lane of the wave takes part.
+__device__ void flush_stats(const SweepStats &S, double *out) {
+  double v[28];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    v[k] = S.n[k];
+    v[5 + k] = S.lg[k];
+    v[10 + k] = S.mx[k];
+    v[15 + k] = S.big[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) v[20 + k] = S.kap[k];
+  v[27] = S.cut;
+#pragma unroll
+  for (int k = 0; k < 28; ++k) {
+    const bool is_max = k >= 10 && k < 15;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double u = __shfl_down(v[k], o, 64);
+      v[k] = is_max ? fmax(v[k], u) : v[k] + u;
+    }
+  }
+  if (__lane_id() == 0)
+    for (int k = 0; k < 28; ++k) {
+      if (k >= 10 && k < 15) {
+        // non-negative doubles order as their bit patterns
+        atomicMax((unsigned long long *)&out[k], (unsigned long long)__double_as_longlong(v[k]));
+      } else if (v[k] != 0.0) {
+        unsafeAtomicAdd(&out[k], v[k]);
+      }
+    }
+}
+
 // Reverse sweep of the tape column `col` from entry `from` down to the
 // window's first entry (0), l and mu the adjoints of the state and of the visit
 // start.  LEAN: the hand-written reverse step only; at an iteration it does
 // not cover the sweep stops and returns that entry's index (l and mu then
 // hold the state to resume from).  Otherwise rare iterations use
 // dual-number Jacobians.  Returns -1 when the window is done.
-template <int NDIM, bool LEAN>
+template <int NDIM, bool LEAN, bool STATS>
 __device__ int64_t sweep(const double *__restrict__ cells, const double *__restrict__ tan,
                          const double *__restrict__ dd3, int64_t ncell, const WalkParams &W,
                          int64_t col, int64_t from, const uint64_t *__restrict__ tape,
                          double l[4], double mu[4],
-                         double *g_al, double *g_at, double *g_dm, double &gs) {
+                         double *g_al, double *g_at, double *g_dm, double &gs,
+                         const SweepOpts &O, SweepStats &S) {
   using D = Dual<4>;
   double g[4] = {0.0, 0.0, 0.0, 0.0};
   int cur = 0;
@@ -1313,7 +1387,14 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
       const double *crec = cells + c0 * CELL_DOUBLES, *ctan = tan + c0 * kTan * 4;
       double ln[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {l[0], l[1], l[2], l[3]};
       int restart = 0;
-      if (!reverse_step<NDIM>(crec, ctan, dd3 + c0 * 108, W, E, l, ln, g)) {
+      double kappa = 1.0;
+      if (reverse_step<NDIM>(crec, ctan, dd3 + c0 * 108, W, E, l, ln, g, O, kappa)) {
+        if constexpr (STATS) {
+          const bool exited = (E.flags >> 8) & 1, clipped = (E.flags >> 10) & 1;
+          S.add(exited ? (kappa < 1e-3 ? 3 : 2) : (clipped ? 1 : 0), lo, ln);
+          if (exited) S.exit_kappa(kappa, O.graze_cut);
+        }
+      } else {
         if constexpr (LEAN) {
           stop = k;
           break;
@@ -1359,6 +1440,7 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
           for (int j = 0; j < 4; ++j)
             g[j] += lo[0] * U.x.d[j] + lo[1] * U.y.d[j] + lo[2] * U.z.d[j] + lo[3] * U.t.d[j];
         }
+        if constexpr (STATS) S.add(4, l, ln);
         }
       }
       if (restart == 1) {
@@ -1389,7 +1471,7 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
 // rare iteration in `deferred` (as thread indices j) with their resume
 // entry and visit-start adjoint (resume[j], mus[4j..]); the full pass
 // finishes them.  Thread j reverses listed particle ord[j].
-template <int NDIM>
+template <int NDIM, bool STATS>
 __global__ void __launch_bounds__(kBlk) adj_backward(
     const double *__restrict__ cells, const double *__restrict__ tan,
     const double *__restrict__ dd3, int64_t ncell,
@@ -1398,15 +1480,17 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
     const uint64_t *__restrict__ tape,
     double *__restrict__ lam, double *__restrict__ mus, int64_t *__restrict__ resume,
     int32_t *__restrict__ deferred, unsigned long long *__restrict__ ndeferred, double *g_al,
-    double *g_at, double *g_dm, double *g_s) {
+    double *g_at, double *g_dm, double *g_s, const SweepOpts O, double *stats) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double gs = 0.0;
+  SweepStats S;
   if (j < nsel) {
     const int64_t i = ord[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t k = sweep<NDIM, true>(cells, tan, dd3, ncell, W, tape_col(wstart, j),
-                                        cnt[i] - 1, tape, l, mu, g_al, g_at, g_dm, gs);
+    const int64_t k =
+        sweep<NDIM, true, STATS>(cells, tan, dd3, ncell, W, tape_col(wstart, j), cnt[i] - 1,
+                                 tape, l, mu, g_al, g_at, g_dm, gs, O, S);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       lam[4 * i + q] = l[q];
@@ -1417,9 +1501,10 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
   }
   for (int o = 32; o > 0; o >>= 1) gs += __shfl_down(gs, o, 64);
   if (__lane_id() == 0 && gs != 0.0) unsafeAtomicAdd(g_s, gs);
+  if constexpr (STATS) flush_stats(S, stats);
 }
 
-template <int NDIM>
+template <int NDIM, bool STATS>
 __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const double *__restrict__ cells, const double *__restrict__ tan,
     const double *__restrict__ dd3, int64_t ncell,
@@ -1427,20 +1512,64 @@ __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const uint64_t *__restrict__ tape, double *__restrict__ lam, const double *__restrict__ mus,
     const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred,
     const unsigned long long *__restrict__ ndef, double *g_al, double *g_at, double *g_dm,
-    double *g_s) {
+    double *g_s, const SweepOpts O, double *stats) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double gs = 0.0;
+  SweepStats S;
   if (t < (int64_t)*ndef) {
     const int64_t j = deferred[t], i = ord[j];
     double l[4] = {lam[4 * i], lam[4 * i + 1], lam[4 * i + 2], lam[4 * i + 3]};
     double mu[4] = {mus[4 * j], mus[4 * j + 1], mus[4 * j + 2], mus[4 * j + 3]};
-    (void)sweep<NDIM, false>(cells, tan, dd3, ncell, W, tape_col(wstart, j), resume[j], tape, l,
-                             mu, g_al, g_at, g_dm, gs);
+    (void)sweep<NDIM, false, STATS>(cells, tan, dd3, ncell, W, tape_col(wstart, j), resume[j],
+                                    tape, l, mu, g_al, g_at, g_dm, gs, O, S);
 #pragma unroll
     for (int q = 0; q < 4; ++q) lam[4 * i + q] = l[q];
   }
   for (int o = 32; o > 0; o >>= 1) gs += __shfl_down(gs, o, 64);
   if (__lane_id() == 0 && gs != 0.0) unsafeAtomicAdd(g_s, gs);
+  if constexpr (STATS) flush_stats(S, stats);
+}
+
+// Whole-horizon adjoint growth of the listed particles: nrm holds the L1
+// norm of each particle's adjoint at the final time; lam the adjoint at
+// the first window's start.  out[28] particles, out[29] largest log10
+// growth (max with 0), out[30 .. 32] particles growing beyond 1e3, 1e6, 1e9.
+__global__ void adj_growth(int64_t nsel, const double *__restrict__ nrm,
+                           const double *__restrict__ lam, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < nsel) {
+    const double b = fabs(lam[4 * i]) + fabs(lam[4 * i + 1]) + fabs(lam[4 * i + 2]) +
+                     fabs(lam[4 * i + 3]);
+    v[0] = 1.0;
+    if (nrm[i] > 0.0 && b > nrm[i]) {
+      const double g = log10(b / nrm[i]);
+      v[1] = g;
+      v[2] = g > 3.0 ? 1.0 : 0.0;
+      v[3] = g > 6.0 ? 1.0 : 0.0;
+      v[4] = g > 9.0 ? 1.0 : 0.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    for (int o = 32; o > 0; o >>= 1) {
+      const double u = __shfl_down(v[k], o, 64);
+      v[k] = k == 1 ? fmax(v[k], u) : v[k] + u;
+    }
+  if (__lane_id() == 0) {
+    if (v[0] != 0.0) unsafeAtomicAdd(&out[28], v[0]);
+    atomicMax((unsigned long long *)&out[29], (unsigned long long)__double_as_longlong(v[1]));
+    for (int k = 2; k < 5; ++k)
+      if (v[k] != 0.0) unsafeAtomicAdd(&out[28 + k], v[k]);
+  }
+}
+
+// L1 norms of the listed particles' adjoints.
+__global__ void adj_norms(int64_t nsel, const double *__restrict__ lam, double *__restrict__ nrm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsel)
+    nrm[i] = fabs(lam[4 * i]) + fabs(lam[4 * i + 1]) + fabs(lam[4 * i + 2]) +
+             fabs(lam[4 * i + 3]);
 }
 
 // Concentrations at the observation points from the active particles'
@@ -1473,6 +1602,19 @@ __global__ void adj_conc(int64_t n, const int32_t *__restrict__ status,
 // Adjoint of the listed particles' local coordinates from the misfit:
 // dJ/dX_d = sum_o 2 (c_o - target_o) m K_o (-(X_d - x_od) / h_d^2), times
 // dX/dlx = right - left of the cell (ConvertToModelXYZ), and 0 for t.
+// Right - left, back - front and top - bottom of 0-based cell c0 as
+// global_xyz_kernel forms them.
+__device__ __forceinline__ void cell_widths(int c0, int ncol, int nrow,
+                                            const double *__restrict__ delr,
+                                            const double *__restrict__ delc,
+                                            const double *__restrict__ top,
+                                            const double *__restrict__ bot, double w[3]) {
+  const int lc = nrow * ncol, rr = c0 % lc, ii = rr / ncol, jj = rr % ncol;
+  w[0] = (delr[jj] / 2.0) + (delr[jj] / 2.0);
+  w[1] = (delc[ii] / 2.0) + (delc[ii] / 2.0);
+  w[2] = top[c0] - bot[c0];
+}
+
 __global__ void adj_seed(const int32_t *__restrict__ sel, int64_t nsel,
                          const int32_t *__restrict__ cellv, const double *__restrict__ gx,
                          const double *__restrict__ gy, const double *__restrict__ gz,
@@ -1503,16 +1645,33 @@ __global__ void adj_seed(const int32_t *__restrict__ sel, int64_t nsel,
     for (int d = 0; d < 3; ++d)
       if (h[d] > 0.0) gX[d] += r * (-u[d] / h[d]);
   }
-  const int c0 = cellv[p] - 1;
-  const int lc = nrow * ncol, rr = c0 % lc, ii = rr / ncol, jj = rr % ncol;
-  // right - left, back - front and top - bottom of the cell as
-  // global_xyz_kernel forms them
-  const double wx = (delr[jj] / 2.0) + (delr[jj] / 2.0);
-  const double wy = (delc[ii] / 2.0) + (delc[ii] / 2.0);
-  const double wz = top[c0] - bot[c0];
-  lam[4 * i + 0] = gX[0] * wx;
-  lam[4 * i + 1] = gX[1] * wy;
-  lam[4 * i + 2] = gX[2] * wz;
+  double w[3];
+  cell_widths(cellv[p] - 1, ncol, nrow, delr, delc, top, bot, w);
+  lam[4 * i + 0] = gX[0] * w[0];
+  lam[4 * i + 1] = gX[1] * w[1];
+  lam[4 * i + 2] = gX[2] * w[2];
+  lam[4 * i + 3] = 0.0;
+}
+
+// Straight-through restart of the adjoint at a window start (the option
+// tangent_windows): the final-time adjoint of the particle's global
+// position (lamT, local to its final cell) carried to the local
+// coordinates of the cell it starts the window in, the transport from there
+// to the final time taken as the identity; time adjoint 0.
+__global__ void adj_restart(const int32_t *__restrict__ sel, int64_t nsel,
+                            const int32_t *__restrict__ cell_w, const int32_t *__restrict__ cell_T,
+                            const double *__restrict__ lamT, int ncol, int nrow,
+                            const double *__restrict__ delr, const double *__restrict__ delc,
+                            const double *__restrict__ top, const double *__restrict__ bot,
+                            double *__restrict__ lam) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsel) return;
+  const int p = sel[i];
+  double wT[3], ww[3];
+  cell_widths(cell_T[p] - 1, ncol, nrow, delr, delc, top, bot, wT);
+  cell_widths(cell_w[p] - 1, ncol, nrow, delr, delc, top, bot, ww);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) lam[4 * i + a] = lamT[4 * i + a] / wT[a] * ww[a];
   lam[4 * i + 3] = 0.0;
 }
 
@@ -1580,6 +1739,25 @@ struct AdjBuffers {
 
 }  // namespace
 
+extern "C" int rwpt_adjoint_options(rwpt_engine *e, const rwpt_adjoint_opts *o) {
+  if (!e || !o) return RWPT_E_ARG;
+  if (!(o->graze_cut >= 0.0 && o->graze_cut <= 1.0))
+    return adj_fail(e, RWPT_E_ARG, "graze_cut must lie in [0, 1]");
+  if (o->reverse_windows < 0) return adj_fail(e, RWPT_E_ARG, "reverse_windows must be >= 0");
+  if (o->tangent_windows < 0) return adj_fail(e, RWPT_E_ARG, "tangent_windows must be >= 0");
+  if ((o->frozen_noise != 0 && o->frozen_noise != 1) ||
+      (o->diagnostics != 0 && o->diagnostics != 1))
+    return adj_fail(e, RWPT_E_ARG, "frozen_noise and diagnostics must be 0 or 1");
+  e->adj_opt = *o;
+  return RWPT_OK;
+}
+
+extern "C" int rwpt_adjoint_stats(rwpt_engine *e, double *out) {
+  if (!e || !out) return RWPT_E_ARG;
+  std::memcpy(out, e->adj_stats, sizeof e->adj_stats);
+  return RWPT_OK;
+}
+
 extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_windows,
                             int32_t n_obs, const double *obs_xyz, const double *target,
                             const double *bandwidth, const double *mass, double *misfit,
@@ -1622,14 +1800,25 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   // checkpoints at every window start
   std::vector<CkPtrs> ck(want_grad ? n_windows : 0);
   std::vector<WalkParams> Ws(n_windows);
-  for (size_t w = 0; w < ck.size(); ++w) {
-    CkPtrs &C = ck[w];
-    if ((err = B.alloc(&C.cell, n)) != hipSuccess || (err = B.alloc(&C.status, n)) != hipSuccess ||
-        (err = B.alloc(&C.lx, n)) != hipSuccess || (err = B.alloc(&C.ly, n)) != hipSuccess ||
-        (err = B.alloc(&C.lz, n)) != hipSuccess || (err = B.alloc(&C.t, n)) != hipSuccess ||
-        (err = B.alloc(&C.rng, 2 * n)) != hipSuccess ||
-        (err = B.alloc(&C.substeps, n)) != hipSuccess)
+  {
+    // one allocation for all checkpoints: per window the 8-byte fields
+    // (lx ly lz t, rng, substeps), then cell and status, 256-B aligned
+    const size_t stride = ((size_t)std::max<int64_t>(n, 1) * 64 + 255) / 256 * 256;
+    char *ckmem = nullptr;
+    if (!ck.empty() && (err = B.alloc(&ckmem, stride * ck.size())) != hipSuccess)
       return adj_hip(e, err, "checkpoint memory");
+    for (size_t w = 0; w < ck.size(); ++w) {
+      char *q = ckmem + w * stride;
+      CkPtrs &C = ck[w];
+      C.lx = reinterpret_cast<double *>(q);
+      C.ly = C.lx + n;
+      C.lz = C.ly + n;
+      C.t = C.lz + n;
+      C.rng = reinterpret_cast<uint64_t *>(C.t + n);
+      C.substeps = reinterpret_cast<int64_t *>(C.rng + 2 * n);
+      C.cell = reinterpret_cast<int32_t *>(C.substeps + n);
+      C.status = C.cell + n;
+    }
   }
   // forward: the engine's own advances
   for (int w = 0; w < n_windows; ++w) {
@@ -1760,6 +1949,12 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     sel = sel2;
     lam = lam2;
   }
+  double *lamT = nullptr;  // the final-time adjoints, for adj_restart
+  if (e->adj_opt.tangent_windows > 0 &&
+      ((err = B.alloc(&lamT, 4 * std::max<int64_t>(nlive, 1))) != hipSuccess ||
+       (nlive > 0 && (err = hipMemcpyAsync(lamT, lam, 32 * nlive, hipMemcpyDeviceToDevice,
+                                           st)) != hipSuccess)))
+    return adj_hip(e, err, "adjoint memory");
   const int64_t nwl = (nlive + kBlk - 1) / kBlk;
   const dim3 lgrid((unsigned)((std::max<int64_t>(nlive, 1) + kBlk - 1) / kBlk));
   // Per window, before the reverse pass: the listed particles' iteration
@@ -1794,13 +1989,22 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       (err = hipMemsetAsync(ndefs, 0, 8 * (size_t)n_windows, st)) != hipSuccess)
     return adj_hip(e, err, "order memory");
   const CkPtrs now{P.cell, P.status, P.lx, P.ly, P.lz, P.t, P.rng, P.substeps};
-  for (int w = 0; w < n_windows && nlive > 0; ++w) {
+  int32_t *ord_mem;
+  int64_t *cnt_mem, *ws_mem;
+  if ((err = B.alloc(&ord_mem, (size_t)n_windows * nlive)) != hipSuccess ||
+      (err = B.alloc(&cnt_mem, (size_t)n_windows * (nlive + 1))) != hipSuccess ||
+      (err = B.alloc(&ws_mem, (size_t)n_windows * (nwl + 1))) != hipSuccess)
+    return adj_hip(e, err, "order memory");
+  // the sweep covers the windows from w_first on (all with reverse_windows 0)
+  const int w_first =
+      e->adj_opt.reverse_windows > 0 ? std::max(0, n_windows - e->adj_opt.reverse_windows)
+                                    : 0;
+  for (int w = w_first; w < n_windows && nlive > 0; ++w) {
     const CkPtrs &in = ck[w];
     const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
-    if ((err = B.alloc(&ords[w], nlive)) != hipSuccess ||
-        (err = B.alloc(&cnts[w], nlive + 1)) != hipSuccess ||
-        (err = B.alloc(&wstarts[w], nwl + 1)) != hipSuccess)
-      return adj_hip(e, err, "order memory");
+    ords[w] = ord_mem + (size_t)w * nlive;
+    cnts[w] = cnt_mem + (size_t)w * (nlive + 1);
+    wstarts[w] = ws_mem + (size_t)w * (nwl + 1);
     hipLaunchKernelGGL(adj_counts, dim3((unsigned)((nlive + 1 + blk - 1) / blk)), dim3(blk), 0,
                        st, (const int32_t *)sel, nlive, (const int64_t *)in.substeps,
                        (const int64_t *)want.substeps, cnts[w]);
@@ -1833,7 +2037,17 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     return adj_hip(e, err, "tape memory");
   double *ga = grad_al ? g_al : nullptr, *gt = grad_at ? g_at : nullptr,
          *gd = grad_dm ? g_dm : nullptr;
-  for (int w = n_windows - 1; w >= 0 && nlive > 0; --w) {
+  // diagnostics: the sweep's tallies and the adjoint norms at the final time
+  double *dstats = nullptr, *nrm0 = nullptr;
+  if (e->adj_opt.diagnostics) {
+    if ((err = B.alloc(&dstats, RWPT_ADJ_NSTATS)) != hipSuccess ||
+        (err = B.alloc(&nrm0, std::max<int64_t>(nlive, 1))) != hipSuccess ||
+        (err = hipMemsetAsync(dstats, 0, 8 * RWPT_ADJ_NSTATS, st)) != hipSuccess)
+      return adj_hip(e, err, "diagnostics memory");
+    if (nlive > 0) hipLaunchKernelGGL(adj_norms, lgrid, dim3(kBlk), 0, st, nlive,
+                                      (const double *)lam, nrm0);
+  }
+  for (int w = n_windows - 1; w >= w_first && nlive > 0; --w) {
     const CkPtrs &in = ck[w];
     const CkPtrs &want = w + 1 < n_windows ? ck[w + 1] : now;
     const WalkParams &W = Ws[w];
@@ -1845,29 +2059,44 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     else
       hipLaunchKernelGGL(adj_replay<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
                          (const int32_t *)sel, nlive, ord, in, want, W, cw, ws, tape, bads + w);
-    if (ndim == 3)
-      hipLaunchKernelGGL(adj_backward<3>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,
-                         (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt,
-                         gd, g_s);
-    else
-      hipLaunchKernelGGL(adj_backward<2>, lgrid, dim3(kBlk), 0, st, (const double *)G.cells,
-                         (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,
-                         (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt,
-                         gd, g_s);
-    if (ndim == 3)
-      hipLaunchKernelGGL(adj_backward_full<3>, lgrid, dim3(kBlk), 0, st,
-                         (const double *)G.cells, (const double *)tan, (const double *)dd3,
-                         ncell, W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,
-                         (const int64_t *)resume, (const int32_t *)deferred,
-                         (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s);
-    else
-      hipLaunchKernelGGL(adj_backward_full<2>, lgrid, dim3(kBlk), 0, st,
-                         (const double *)G.cells, (const double *)tan, (const double *)dd3,
-                         ncell, W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,
-                         (const int64_t *)resume, (const int32_t *)deferred,
-                         (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s);
+    const bool diag = e->adj_opt.diagnostics != 0;
+    const SweepOpts sopt{e->adj_opt.graze_cut, e->adj_opt.frozen_noise};
+#define RWPT_ADJ_SWEEP(ND, ST)                                                                   \
+  hipLaunchKernelGGL((adj_backward<ND, ST>), lgrid, dim3(kBlk), 0, st, (const double *)G.cells,  \
+                     (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,     \
+                     (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt, gd,  \
+                     g_s, sopt, dstats);                                                         \
+  hipLaunchKernelGGL((adj_backward_full<ND, ST>), lgrid, dim3(kBlk), 0, st,                      \
+                     (const double *)G.cells, (const double *)tan, (const double *)dd3, ncell,   \
+                     W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,               \
+                     (const int64_t *)resume, (const int32_t *)deferred,                         \
+                     (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s, sopt, dstats)
+    if (ndim == 3 && diag) {
+      RWPT_ADJ_SWEEP(3, true);
+    } else if (ndim == 3) {
+      RWPT_ADJ_SWEEP(3, false);
+    } else if (diag) {
+      RWPT_ADJ_SWEEP(2, true);
+    } else {
+      RWPT_ADJ_SWEEP(2, false);
+    }
+#undef RWPT_ADJ_SWEEP
+    const int tw = e->adj_opt.tangent_windows;
+    if (tw > 0 && w > w_first && (n_windows - w) % tw == 0)
+      hipLaunchKernelGGL(adj_restart, lgrid, dim3(kBlk), 0, st, (const int32_t *)sel, nlive,
+                         (const int32_t *)in.cell, (const int32_t *)now.cell,
+                         (const double *)lamT, G.ncol, G.nrow, (const double *)G.delr,
+                         (const double *)G.delc, (const double *)G.top, (const double *)G.bot,
+                         lam);
     if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "reverse sweep");
+  }
+  if (e->adj_opt.diagnostics) {
+    if (nlive > 0) hipLaunchKernelGGL(adj_growth, lgrid, dim3(kBlk), 0, st, nlive,
+                                      (const double *)nrm0, (const double *)lam, dstats);
+    if ((err = hipGetLastError()) != hipSuccess ||
+        (err = hipMemcpyAsync(e->adj_stats, dstats, 8 * RWPT_ADJ_NSTATS, hipMemcpyDeviceToHost,
+                              st)) != hipSuccess)
+      return adj_hip(e, err, "diagnostics");
   }
   std::vector<unsigned long long> hbad(n_windows, 0);
   if ((err = hipMemcpyAsync(hbad.data(), bads, 8 * (size_t)n_windows, hipMemcpyDeviceToHost,

@@ -188,6 +188,10 @@ struct rwpt_engine {
   int64_t kde_weights_size = 0;
   double kde_last[4] = {0, 0, 0, 0};  // bandwidth, points, nPoints, effMass
   int32_t kde_loops = 0;              // optimisation loops of the last kde
+  // options of rwpt_adjoint's reverse sweep (rwpt_adjoint_options) and the
+  // statistics of its last gradient taken with diagnostics on
+  rwpt_adjoint_opts adj_opt{0.0, 0, 0, 0, 0};
+  double adj_stats[RWPT_ADJ_NSTATS] = {};
   double *xface = nullptr, *yface = nullptr;  // DIS face coordinates X, Y
   // grow-only device buffers of the gpkde optimisation loop (active bins,
   // per-bin arrays, curvature grids, kernel factors, tile lists), kept for

@@ -58,6 +58,12 @@ class RwptKdeOptions(C.Structure):
                 ("member", C.c_int32)]
 
 
+class RwptAdjointOpts(C.Structure):
+    _fields_ = [("graze_cut", C.c_double), ("reverse_windows", C.c_int32),
+                ("tangent_windows", C.c_int32), ("frozen_noise", C.c_int32),
+                ("diagnostics", C.c_int32)]
+
+
 EXPORTS = {
     "rwpt_default_options": ([C.POINTER(RwptOptions)], None),
     "rwpt_create": ([C.POINTER(RwptGrid), C.POINTER(RwptOptions),
@@ -107,6 +113,8 @@ EXPORTS = {
     "rwpt_get_cell_tables": ([C.c_void_p, _dp], C.c_int),
     "rwpt_adjoint": ([C.c_void_p, C.c_int32, _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp,
                       _dp, _dp, _dp, _dp, _dp], C.c_int),
+    "rwpt_adjoint_options": ([C.c_void_p, C.POINTER(RwptAdjointOpts)], C.c_int),
+    "rwpt_adjoint_stats": ([C.c_void_p, _dp], C.c_int),
     "rwpt_kernel_time": ([C.c_void_p, C.c_int32, _dp, _i64p], C.c_int),
     "rwpt_reset_kernel_time": ([C.c_void_p], C.c_int),
 }
@@ -341,6 +349,40 @@ class Engine:
             out.update({"grad_al": g[0], "grad_at": g[1], "grad_dm": g[2],
                         "grad_porosity_scale": gs.value})
         return out
+
+    def adjoint_options(self, graze_cut=0.0, reverse_windows=0, tangent_windows=0,
+                        frozen_noise=False, diagnostics=False):
+        """rwpt_adjoint_options: exits with kappa < graze_cut lose their
+        exit-time derivative; reverse_windows > 0 sweeps only the last
+        windows; tangent_windows > 0 restarts the adjoint from the final
+        time's every that many windows (straight-through); frozen_noise
+        drops the random displacements' derivative with respect to the
+        sub-step duration; diagnostics tallies the reverse sweep."""
+        o = RwptAdjointOpts(float(graze_cut), int(reverse_windows), int(tangent_windows),
+                            int(bool(frozen_noise)), int(bool(diagnostics)))
+        self._check(lib().rwpt_adjoint_options(self.h, C.byref(o)), "rwpt_adjoint_options")
+
+    ADJ_KINDS = ("substep", "clipped", "exit", "grazing_exit", "rare")
+    ADJ_KAPPA_BINS = ("<1e-10", "1e-10..1e-8", "1e-8..1e-6", "1e-6..1e-4", "1e-4..1e-2",
+                      "1e-2..1e-1", ">=1e-1")
+
+    def adjoint_stats(self):
+        """rwpt_adjoint_stats of the last gradient taken with diagnostics on,
+        by iteration kind (substep, clipped, exit, grazing_exit with
+        kappa < 1e-3, rare): iterations, sum and maximum of log10 adjoint
+        growth, iterations growing by more than 10; exits by kappa decade;
+        whole-horizon growth of the particles."""
+        v = np.zeros(33)
+        self._check(lib().rwpt_adjoint_stats(self.h, _p(v, _dp)), "rwpt_adjoint_stats")
+        kinds = {k: {"iterations": int(v[i]), "log10_growth_sum": float(v[5 + i]),
+                     "log10_growth_max": float(v[10 + i]), "growth_over_10": int(v[15 + i])}
+                 for i, k in enumerate(self.ADJ_KINDS)}
+        return {"kinds": kinds,
+                "exit_kappa": dict(zip(self.ADJ_KAPPA_BINS, (int(x) for x in v[20:27]))),
+                "exits_cut": int(v[27]), "particles": int(v[28]),
+                "particle_log10_growth_max": float(v[29]),
+                "particles_growth_over": {"1e3": int(v[30]), "1e6": int(v[31]),
+                                          "1e9": int(v[32])}}
 
     def write_observation(self, path, style, cells, time_point, time_step, ids, group,
                           group_solute, mass=None, flow_rate=0.0, fmt=0, append=True,

@@ -207,3 +207,187 @@ def test_autograd_wrapper():
     finally:
         E.close()
     assert g2[2 * NZ] == pytest.approx(out["grad_porosity_scale"] / 1.1, rel=1e-12)
+
+
+def _gradient(P, zones, aL, aT, dm, target, opts=None, state=None, windows=WINDOWS):
+    """Gradient of the misfit with the sweep options `opts`; `state` (a
+    particles() dict at time t0) starts the run from that state instead."""
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    E = Engine(Q)
+    try:
+        if opts:
+            E.adjoint_options(**opts)
+        if state is None:
+            E.load_particles(*release(P))
+        else:
+            t0, g = state
+            E.load_particles(g["cell"], g["lx"], g["ly"], g["lz"], t=g["t"],
+                             status=g["status"], rng=g["rng"])
+            E.set_time(t0)
+        obs = observations(P)
+        out = E.adjoint(windows, obs, target, (1.5, 1.5, 1.0), gradient=True)
+        if opts and opts.get("diagnostics"):
+            out["stats"] = E.adjoint_stats()
+    finally:
+        E.close()
+    return out
+
+
+def _same_gradient(a, b, rtol=1e-10):
+    """Per-cell gradients are sums of FP64 atomics: equal to rounding."""
+    for k in ("grad_al", "grad_at", "grad_dm"):
+        scale = np.abs(a[k]).max() + 1e-300
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=rtol * scale, err_msg=k)
+    assert a["grad_porosity_scale"] == pytest.approx(b["grad_porosity_scale"], rel=rtol, abs=1e-300)
+
+
+def test_sweep_diagnostics_and_graze_cut():
+    """The diagnostics change no gradient and classify every exit by its
+    kappa; graze_cut drops exactly the exits below it, and a cut below
+    every exit's kappa leaves the gradient as it is; frozen_noise keeps the
+    misfit and changes the gradient of a walk with exits."""
+    P, zones, aL, aT, dm = base()
+    target = 0.7 * run(P, zones, aL, aT, dm)["conc"][::-1]
+    plain = _gradient(P, zones, aL, aT, dm, target)
+    d = _gradient(P, zones, aL, aT, dm, target, {"diagnostics": True})
+    _same_gradient(plain, d)
+    S = d["stats"]
+    print("sweep stats:", S, flush=True)
+    kinds = S["kinds"]
+    exits = kinds["exit"]["iterations"] + kinds["grazing_exit"]["iterations"]
+    bins = list(S["exit_kappa"].values())
+    assert exits > 0 and sum(bins) == exits
+    assert kinds["substep"]["iterations"] > 0
+    assert S["exits_cut"] == 0
+    assert 0 < S["particles"] <= 400
+    for k in kinds.values():
+        assert k["log10_growth_max"] >= 0 and k["growth_over_10"] <= k["iterations"]
+    # a cut at 1e-2 drops the exits of the five lowest kappa bins
+    c = _gradient(P, zones, aL, aT, dm, target, {"graze_cut": 1e-2, "diagnostics": True})
+    assert c["stats"]["exits_cut"] == sum(bins[:5])
+    if sum(bins[:5]) == 0:
+        _same_gradient(plain, c)
+    # a cut at 1 drops every exit whose root terms cancel at all (kappa < 1);
+    # kappa is exactly 1 where 2 A m and B have the same sign
+    full = _gradient(P, zones, aL, aT, dm, target, {"graze_cut": 1.0, "diagnostics": True})
+    assert sum(bins[:6]) <= full["stats"]["exits_cut"] <= exits
+    assert np.isfinite(full["grad_al"]).all()
+    if full["stats"]["exits_cut"] > 0:
+        assert not np.allclose(full["grad_al"], plain["grad_al"], rtol=1e-9, atol=0)
+    # frozen noise: the sub-steps' random displacements lose their duration
+    # derivative, so a walk with cell exits gets another (finite) gradient
+    fz = _gradient(P, zones, aL, aT, dm, target, {"frozen_noise": True})
+    assert fz["misfit"] == pytest.approx(plain["misfit"], rel=1e-12)
+    assert np.isfinite(fz["grad_al"]).all() and np.isfinite(fz["grad_porosity_scale"])
+    assert not np.allclose(fz["grad_al"], plain["grad_al"], rtol=1e-6, atol=0)
+    # the lowest non-empty bin's upper edge leaves those exits uncut
+    edges = (1e-10, 1e-8, 1e-6, 1e-4, 1e-2, 1e-1)
+    low = next(i for i, b in enumerate(bins) if b > 0)
+    if low > 0:
+        below = _gradient(P, zones, aL, aT, dm, target, {"graze_cut": edges[low - 1]})
+        _same_gradient(plain, below)
+
+
+def test_sweep_truncated_to_last_window():
+    """reverse_windows = 1 is the gradient of the last window alone: that
+    of a run started from the state at the last window's start (the
+    checkpoint, random streams included)."""
+    P, zones, aL, aT, dm = base()
+    target = 0.7 * run(P, zones, aL, aT, dm)["conc"][::-1]
+    last = _gradient(P, zones, aL, aT, dm, target, {"reverse_windows": 1})
+    every = _gradient(P, zones, aL, aT, dm, target, {"reverse_windows": len(WINDOWS)})
+    _same_gradient(_gradient(P, zones, aL, aT, dm, target), every)
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        for t in WINDOWS[:-1]:
+            E.advance(t)
+        state = (WINDOWS[-2], E.particles())
+    finally:
+        E.close()
+    alone = _gradient(P, zones, aL, aT, dm, target, state=state, windows=WINDOWS[-1:])
+    assert alone["misfit"] == pytest.approx(last["misfit"], rel=1e-12)
+    _same_gradient(alone, last)
+    assert not np.allclose(last["grad_al"], every["grad_al"])
+
+
+def test_tangent_restart_is_the_window_local_sensitivity():
+    """tangent_windows = 1: window w contributes lambda_T . dX(t_w+1)/dtheta
+    with the parameters perturbed during window w only, lambda_T the
+    final-time derivative of the misfit with respect to each particle's
+    global position.  Window 2 of 3 (the difference of the sweeps of the
+    last two and the last window) against central differences of the
+    particles' positions at its end, common random numbers, the
+    parameters changed at its start; the final-time gradient from the
+    misfit formula evaluated here."""
+    P, zones, aL, aT, dm = base()
+    obs = observations(P)
+    bw = np.array([1.5, 1.5, 1.0])
+    target = 0.7 * run(P, zones, aL, aT, dm)["conc"][::-1]
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        for t in WINDOWS:
+            E.advance(t)
+        X = np.stack(E.global_xyz(), axis=1)
+        on = E.particles()["status"] == 1
+    finally:
+        E.close()
+    u = (X[on, None, :] - obs[None, :, :]) / bw            # (p, o, d)
+    K = np.prod(np.exp(-0.5 * u ** 2) / (bw * np.sqrt(2 * np.pi)), axis=2)
+    resid = K.sum(axis=0) - target
+    gX = np.zeros_like(X)
+    gX[on] = np.einsum("o,po,pod->pd", 2 * resid, K, -u / bw)
+
+    adj = {}
+    for rw in (1, 2):
+        adj[rw] = _gradient(P, zones, aL, aT, dm, target,
+                            {"tangent_windows": 1, "reverse_windows": rw})
+    assert adj[1]["misfit"] == pytest.approx((resid ** 2).sum(), rel=1e-10)
+    last = _gradient(P, zones, aL, aT, dm, target, {"reverse_windows": 1})
+    _same_gradient(adj[1], last)
+
+    def window2_positions(aL2, aT2, dm2):
+        E = Engine(Q)
+        try:
+            E.load_particles(*release(P))
+            E.advance(WINDOWS[0])
+            E.set_dispersion(problems.zoned_dispersion(P, zones, aL2, aT2, dm2))
+            E.advance(WINDOWS[1])
+            g = E.particles()
+            return np.stack(E.global_xyz(), axis=1), (g["cell"], g["status"], g["substeps"])
+        finally:
+            E.close()
+
+    checked = 0
+    params = [("aL", z) for z in range(NZ)] + [("aT", 0), ("dm", 0)]
+    for name, z in params:
+        key = {"aL": "grad_al", "aT": "grad_at", "dm": "grad_dm"}[name]
+        cells = zones == z if name != "dm" else np.ones(len(zones), bool)
+        g = adj[2][key][cells].sum() - adj[1][key][cells].sum()
+        for rel in (1e-6, 1e-7, 1e-8):
+            runs = []
+            for sg in (1.0, -1.0):
+                a2, t2, d2 = aL.copy(), aT.copy(), dm
+                if name == "aL":
+                    h = rel * aL[z]
+                    a2[z] += sg * h
+                elif name == "aT":
+                    h = rel * aT[z]
+                    t2[z] += sg * h
+                else:
+                    h = rel * dm
+                    d2 = dm + sg * h
+                runs.append(window2_positions(a2, t2, d2))
+            if not all(np.array_equal(a, b) for a, b in zip(runs[0][1], runs[1][1])):
+                continue
+            fd = float((gX * (runs[0][0] - runs[1][0])).sum() / (2 * h))
+            print(f"  window 2 {name}{z} rel {rel:g}: adjoint {g:.9e} fd {fd:.9e}", flush=True)
+            if abs(g - fd) <= 1e-4 * abs(fd) + 1e-12 * np.abs(gX).sum() / h:
+                checked += 1
+                break
+    assert checked >= len(params) - 1
+    every = _gradient(P, zones, aL, aT, dm, target, {"tangent_windows": len(WINDOWS)})
+    _same_gradient(_gradient(P, zones, aL, aT, dm, target), every)
