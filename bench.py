@@ -577,6 +577,10 @@ def parse_args(argv=None):
                     help="skip the adjoint (calibration) workload")
     ap.add_argument("--kde-loops", type=int, default=10,
                     help="optimisation loops of the adaptive gpkde (0: skip)")
+    ap.add_argument("--adjoint-seeds", type=int, default=8,
+                    help="base seeds of the adjoint leg's central-difference check (>= 2)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint leg (its JSON object on one line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 3 times (the known-size "
                          "workload that calibrates rocprofv3's FETCH_SIZE and WRITE_SIZE)")
@@ -587,6 +591,9 @@ def parse_args(argv=None):
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
+    if args.adjoint_seeds < 2:
+        print("--adjoint-seeds must be >= 2", file=sys.stderr)
         sys.exit(2)
     if args.quick:
         args.ensemble_intervals, args.no_adjoint, args.kde_loops = 0, True, 0
@@ -622,6 +629,9 @@ def main(argv=None):
     args = parse_args(argv)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_run(seeds=args.adjoint_seeds)), flush=True)
         return
     P, rel = workload(args.particles)
     prec = args.precision
@@ -711,7 +721,7 @@ def main(argv=None):
         out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
     if not args.no_adjoint:
         print("[bench] adjoint leg", file=sys.stderr, flush=True)
-        out["adjoint"] = adjoint_run()
+        out["adjoint"] = adjoint_run(seeds=args.adjoint_seeds)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, rel, args.cpu_sample)

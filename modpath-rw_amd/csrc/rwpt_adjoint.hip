@@ -783,7 +783,11 @@ __device__ __forceinline__ double der4d(const double *d, double u, double w) {
 // Dm are record fields of their own (tangent 1); the drift terms are linear
 // in aLH, aTH and Dm, their Dm part carrying the corner porosity, so
 // d/ds = Dm d/dDm.  Only the drift terms' three tangents are stored
-// (dd3[cell][term][aLH, aTH, Dm]).
+// (dd3[cell][term][aLH, aTH, Dm]), rounded to FP32: they are the largest
+// load of a reverse iteration (108 values), and in FP32 the sweep needs
+// half the registers and load round trips for them (11 % of the gradient's
+// time at 1e6 particles) while the gradient changes by less than 1e-7
+// relative.
 
 // Reverse of the trilinear interpolation of corner values at base f (value
 // bval at the point): lam * db/d(x, y, z) to lx, and to the porosity scale
@@ -803,7 +807,7 @@ __device__ __forceinline__ void tri8_rev(const double *rec, int f, double x, dou
 
 // Reverse of der4d for drift term j (record base F_DD + 4j) over the
 // coordinates (iu, iw); dd3 holds the cell's drift-term tangents.
-__device__ __forceinline__ void der4_rev(const double *rec, const double *dd3, int j,
+__device__ __forceinline__ void der4_rev(const double *rec, const float *dd3, int j,
                                          const double p[3], int iu, int iw, double dm,
                                          double lam, double lx[3], double g[4]) {
   const double *d = rec + F_DD + 4 * j;
@@ -811,12 +815,12 @@ __device__ __forceinline__ void der4_rev(const double *rec, const double *dd3, i
   lx[iu] += lam * ((1.0 - w) * (d[1] - d[0]) + w * (d[3] - d[2]));
   lx[iw] += lam * (((1.0 - u) * d[2] + u * d[3]) - ((1.0 - u) * d[0] + u * d[1]));
   const double wt[4] = {(1.0 - w) * (1.0 - u), (1.0 - w) * u, w * (1.0 - u), w * u};
-  const double *t = dd3 + 12 * j;  // [corner pair q][param k] for q < 4, k < 3
+  const float *t = dd3 + 12 * j;  // [corner pair q][param k] for q < 4, k < 3
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc[k] += wt[q] * t[3 * q + k];
+    for (int k = 0; k < 3; ++k) acc[k] += wt[q] * (double)t[3 * q + k];
   g[0] += lam * acc[0];
   g[1] += lam * acc[1];
   g[2] += lam * acc[2];
@@ -910,7 +914,7 @@ struct SweepOpts {
 // receives |2 A m + B| / (|2 A m| + |B|) of the exit root (1 otherwise), and
 // a root with kappa < O.graze_cut contributes no exit-time derivative.
 template <int NDIM>
-__device__ bool reverse_step(const double *rec, const double *tan, const double *dd3,
+__device__ bool reverse_step(const double *rec, const double *tan, const float *dd3,
                              const WalkParams &W, const TapeEntry &E, const double l[4],
                              double ln[4], double g[4], const SweepOpts &O, double &kappa) {
   const int32_t fl = E.flags;
@@ -1238,6 +1242,22 @@ __global__ void adj_order_keys(const int32_t *__restrict__ sel, int64_t nsel,
   idx[i] = (int32_t)i;
 }
 
+// The particles with status 1 (active at the final time) in sel, order not
+// kept: one atomic per wavefront reserves its lanes' slots.
+__global__ void adj_select(int64_t n, const int32_t *__restrict__ status,
+                           int32_t *__restrict__ sel, unsigned long long *__restrict__ count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = p < n && status[p] == 1;
+  const unsigned long long mask = __ballot(on);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int first = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == first) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, first, 64);
+  if (on) sel[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)p;
+}
+
 // The listed particles whose adjoint is not zero, with their adjoints
 // (order not kept).  An adjoint that is exactly zero at the final time (a
 // particle beyond the reach of every observation kernel, where the
@@ -1364,7 +1384,7 @@ __device__ void flush_stats(const SweepStats &S, double *out) {
 // dual-number Jacobians.  Returns -1 when the window is done.
 template <int NDIM, bool LEAN, bool STATS>
 __device__ int64_t sweep(const double *__restrict__ cells, const double *__restrict__ tan,
-                         const double *__restrict__ dd3, int64_t ncell, const WalkParams &W,
+                         const float *__restrict__ dd3, int64_t ncell, const WalkParams &W,
                          int64_t col, int64_t from, const uint64_t *__restrict__ tape,
                          double l[4], double mu[4],
                          double *g_al, double *g_at, double *g_dm, double &gs,
@@ -1474,7 +1494,7 @@ __device__ int64_t sweep(const double *__restrict__ cells, const double *__restr
 template <int NDIM, bool STATS>
 __global__ void __launch_bounds__(kBlk) adj_backward(
     const double *__restrict__ cells, const double *__restrict__ tan,
-    const double *__restrict__ dd3, int64_t ncell,
+    const float *__restrict__ dd3, int64_t ncell,
     int64_t nsel, const int32_t *__restrict__ ord, const WalkParams W,
     const int64_t *__restrict__ cnt, const int64_t *__restrict__ wstart,
     const uint64_t *__restrict__ tape,
@@ -1507,7 +1527,7 @@ __global__ void __launch_bounds__(kBlk) adj_backward(
 template <int NDIM, bool STATS>
 __global__ void __launch_bounds__(kBlk) adj_backward_full(
     const double *__restrict__ cells, const double *__restrict__ tan,
-    const double *__restrict__ dd3, int64_t ncell,
+    const float *__restrict__ dd3, int64_t ncell,
     const WalkParams W, const int32_t *__restrict__ ord, const int64_t *__restrict__ wstart,
     const uint64_t *__restrict__ tape, double *__restrict__ lam, const double *__restrict__ mus,
     const int64_t *__restrict__ resume, const int32_t *__restrict__ deferred,
@@ -1676,14 +1696,15 @@ __global__ void adj_restart(const int32_t *__restrict__ sel, int64_t nsel,
 }
 
 // The drift terms' tangents for aLH, aTH and Dm in the reverse step's
-// layout: dd3[cell][term j][corner pair q][param k] from the full table.
+// layout: dd3[cell][term j][corner pair q][param k] from the full table,
+// in FP32.
 __global__ void adj_dd3(int64_t ncell, const double *__restrict__ tan,
-                        double *__restrict__ dd3) {
+                        float *__restrict__ dd3) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncell * 108) return;
   const int64_t c = i / 108;
   const int r = (int)(i % 108), j = r / 12, q = (r % 12) / 3, k = r % 3;
-  dd3[i] = tan[(c * kTan + F_DD + 4 * j + q) * 4 + k];
+  dd3[i] = (float)tan[(c * kTan + F_DD + 4 * j + q) * 4 + k];
 }
 
 __global__ void copy_ck(int64_t n, const int32_t *__restrict__ cell,
@@ -1875,23 +1896,27 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
   }
 
   // reverse: the particles active at the final time carry the adjoint
-  std::vector<int32_t> hs(n);
-  if (n > 0 &&
-      ((err = hipMemcpyAsync(hs.data(), P.status, 4 * n, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-       (err = hipStreamSynchronize(st)) != hipSuccess))
-    return adj_hip(e, err, "statuses");
-  std::vector<int32_t> hsel;
-  for (int64_t p = 0; p < n; ++p)
-    if (hs[p] == 1) hsel.push_back((int32_t)p);
-  const int64_t nsel = (int64_t)hsel.size();
   int32_t *sel;
+  unsigned long long *nsel_d, nsel_h = 0;
+  if ((err = B.alloc(&sel, n)) != hipSuccess || (err = B.alloc(&nsel_d, 1)) != hipSuccess ||
+      (err = hipMemsetAsync(nsel_d, 0, 8, st)) != hipSuccess)
+    return adj_hip(e, err, "adjoint memory");
+  if (n > 0) {
+    hipLaunchKernelGGL(adj_select, pgrid, dim3(blk), 0, st, n, (const int32_t *)P.status, sel,
+                       nsel_d);
+    if ((err = hipGetLastError()) != hipSuccess) return adj_hip(e, err, "statuses");
+  }
+  if ((err = hipMemcpyAsync(&nsel_h, nsel_d, 8, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (err = hipStreamSynchronize(st)) != hipSuccess)
+    return adj_hip(e, err, "statuses");
+  const int64_t nsel = (int64_t)nsel_h;
   double *lam, *tan, *resid_d, *g_al, *g_at, *g_dm, *g_s;
   int64_t *wlen, *wstart, *resume;
   const int64_t nwaves = (nsel + kBlk - 1) / kBlk;
   double *mus;
   int32_t *deferred;
   const int64_t ncell = G.ncell;
-  if ((err = B.alloc(&sel, nsel)) != hipSuccess || (err = B.alloc(&lam, 4 * nsel)) != hipSuccess ||
+  if ((err = B.alloc(&lam, 4 * nsel)) != hipSuccess ||
       (err = B.alloc(&tan, 4 * (size_t)ncell * kTan)) != hipSuccess ||
       (err = B.alloc(&resid_d, n_obs)) != hipSuccess || (err = B.alloc(&g_al, ncell)) != hipSuccess ||
       (err = B.alloc(&g_at, ncell)) != hipSuccess || (err = B.alloc(&g_dm, ncell)) != hipSuccess ||
@@ -1901,9 +1926,7 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
       (err = B.alloc(&resume, nsel)) != hipSuccess || (err = B.alloc(&mus, 4 * nsel)) != hipSuccess ||
       (err = B.alloc(&deferred, nsel)) != hipSuccess)
     return adj_hip(e, err, "adjoint memory");
-  if ((nsel > 0 && (err = hipMemcpyAsync(sel, hsel.data(), 4 * nsel, hipMemcpyHostToDevice,
-                                         st)) != hipSuccess) ||
-      (err = hipMemcpyAsync(resid_d, resid.data(), 8 * (size_t)n_obs, hipMemcpyHostToDevice,
+  if ((err = hipMemcpyAsync(resid_d, resid.data(), 8 * (size_t)n_obs, hipMemcpyHostToDevice,
                             st)) != hipSuccess ||
       (err = hipMemsetAsync(g_al, 0, 8 * ncell, st)) != hipSuccess ||
       (err = hipMemsetAsync(g_at, 0, 8 * ncell, st)) != hipSuccess ||
@@ -1912,7 +1935,7 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     return adj_hip(e, err, "adjoint set-up");
   if ((err = build_record_tangents(e, tan)) != hipSuccess)
     return adj_hip(e, err, "record tangents");
-  double *dd3;
+  float *dd3;
   if ((err = B.alloc(&dd3, (size_t)ncell * 108)) != hipSuccess)
     return adj_hip(e, err, "adjoint memory");
   hipLaunchKernelGGL(adj_dd3, dim3((unsigned)((ncell * 108 + blk - 1) / blk)), dim3(blk), 0, st,
@@ -2063,11 +2086,11 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
     const SweepOpts sopt{e->adj_opt.graze_cut, e->adj_opt.frozen_noise};
 #define RWPT_ADJ_SWEEP(ND, ST)                                                                   \
   hipLaunchKernelGGL((adj_backward<ND, ST>), lgrid, dim3(kBlk), 0, st, (const double *)G.cells,  \
-                     (const double *)tan, (const double *)dd3, ncell, nlive, ord, W, cw, ws,     \
+                     (const double *)tan, (const float *)dd3, ncell, nlive, ord, W, cw, ws,     \
                      (const uint64_t *)tape, lam, mus, resume, deferred, ndefs + w, ga, gt, gd,  \
                      g_s, sopt, dstats);                                                         \
   hipLaunchKernelGGL((adj_backward_full<ND, ST>), lgrid, dim3(kBlk), 0, st,                      \
-                     (const double *)G.cells, (const double *)tan, (const double *)dd3, ncell,   \
+                     (const double *)G.cells, (const double *)tan, (const float *)dd3, ncell,   \
                      W, ord, ws, (const uint64_t *)tape, lam, (const double *)mus,               \
                      (const int64_t *)resume, (const int32_t *)deferred,                         \
                      (const unsigned long long *)(ndefs + w), ga, gt, gd, g_s, sopt, dstats)

@@ -516,18 +516,19 @@ __global__ void fd_steps(int64_t n, const double *__restrict__ a, double rel,
   if (i < n) h[i] = (a && a[i] != 0.0) ? rel * fabs(a[i]) : rel;
 }
 
-// (plus - minus) / (2 h) of the double fields 0..F_CONN-1 of every record
-// for parameter k, stored as out[(c * F_CONN + f) * 4 + k]; h per cell from
-// the step array (one value when nh == 1)
+// (plus - minus) / (span h) of the double fields 0..F_CONN-1 of every
+// record for parameter k (span 2 for a central difference, 1 for a forward
+// one), stored as out[(c * F_CONN + f) * 4 + k]; h per cell from the step
+// array (one value when nh == 1)
 __global__ void fd_tangent(int64_t ncell, const double *__restrict__ plus,
                            const double *__restrict__ minus,
-                           const double *__restrict__ h, int64_t nh, int k,
+                           const double *__restrict__ h, int64_t nh, int k, double span,
                            double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncell * F_CONN) return;
   const int64_t c = i / F_CONN, f = i % F_CONN;
   const double hc = h[nh == 1 ? 0 : c];
-  out[i * 4 + k] = (plus[c * CELL_DOUBLES + f] - minus[c * CELL_DOUBLES + f]) / (2.0 * hc);
+  out[i * 4 + k] = (plus[c * CELL_DOUBLES + f] - minus[c * CELL_DOUBLES + f]) / (span * hc);
 }
 
 }  // namespace
@@ -541,8 +542,12 @@ hipError_t build_record_tangents(rwpt_engine *e, double *out) {
                        e->opt.time_step_params[0], e->opt.time_step_params[1]};
   // the records are rational in the porosity scale and linear in aLH, aTH
   // and Dm (for FIXED time steps): a relative step of 1e-5 leaves the
-  // truncation error near 1e-10 and the rounding error near 1e-11 relative
+  // truncation error near 1e-10 and the rounding error near 1e-11 relative.
+  // Where a record is linear in the parameter, a forward difference from
+  // the engine's own records has no truncation error either and needs one
+  // table build instead of two.
   const double rel = 1.0e-5;
+  const bool linear = e->opt.time_step_kind == 4;
   const size_t recBytes = (size_t)P.ncell * CELL_DOUBLES * sizeof(double);
   const int64_t nd = 5 * P.ndisp;
   double *plus = nullptr, *minus = nullptr, *por = nullptr, *disp = nullptr, *h = nullptr;
@@ -565,7 +570,8 @@ hipError_t build_record_tangents(rwpt_engine *e, double *out) {
     if (k < 3)
       hipLaunchKernelGGL(fd_steps, dim3((unsigned)((P.ndisp + block - 1) / block)),
                          dim3(block), 0, e->stream, P.ndisp, P.disp + lo, rel, h);
-    for (int sgn = 0; sgn < 2; ++sgn) {
+    const bool forward = linear && k < 3;
+    for (int sgn = 0; sgn < (forward ? 1 : 2); ++sgn) {
       double *dst = sgn == 0 ? plus : minus;
       const double sign = sgn == 0 ? 1.0 : -1.0;
       err = hipMemcpyAsync(dst, P.cells, recBytes, hipMemcpyDeviceToDevice, e->stream);
@@ -589,8 +595,9 @@ hipError_t build_record_tangents(rwpt_engine *e, double *out) {
       hipLaunchKernelGGL(fd_steps, dim3(1), dim3(1), 0, e->stream, (int64_t)1,
                          (const double *)nullptr, rel, h);
     hipLaunchKernelGGL(fd_tangent, dim3(tgrid), dim3(block), 0, e->stream, P.ncell,
-                       (const double *)plus, (const double *)minus, (const double *)h,
-                       k == 3 ? (int64_t)1 : P.ndisp, k, out);
+                       (const double *)plus, forward ? (const double *)P.cells : minus,
+                       (const double *)h, k == 3 ? (int64_t)1 : P.ndisp, k,
+                       forward ? 1.0 : 2.0, out);
     if ((err = hipGetLastError()) != hipSuccess) return release(), err;
   }
   err = hipStreamSynchronize(e->stream);

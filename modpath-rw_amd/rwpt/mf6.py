@@ -37,9 +37,15 @@ reference's MODFLOW 6 path does:
   neighbours' faces agree as well.
 
 Rows grow toward the front (the y face coordinates fall with the row,
-RectangularGridDisMf6.f90:300-306) and layers downward.  Only grids whose
-IDOMAIN is 1 everywhere are read: MODFLOW 6 numbers the cells of a grid
-with removed cells differently in the budget file.
+RectangularGridDisMf6.f90:300-306) and layers downward.
+
+Cells with IDOMAIN < 1 are removed from the model: the binary grid file's
+IA and JA use the user cell numbers with an empty row for each removed cell
+and no connection to it, FLOW-JA-FACE follows JA position by position, list
+budget records name user cells and array records cover every user cell.
+MODPATH sets IBOUND = 0 at such cells (ModpathBasicData.f90:177-185,
+ibound_from_idomain); the engine's face flow across a missing connection
+is 0, as the unstructured assembly leaves a face without a connection.
 """
 from __future__ import annotations
 
@@ -325,9 +331,8 @@ def _dry_tolerance(hdry):
 def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetRecord],
                    kper: int, kstp: int, ibound: np.ndarray, hdry: float,
                    default_iface: Dict[str, int] = None) -> TimeStepData:
-    """FlowModelData%LoadTimeStep(kper, kstp) for a MODFLOW 6 DIS grid."""
-    if np.any(grb.idomain != 1):
-        raise Mf6Error("grids with IDOMAIN other than 1 are not supported")
+    """FlowModelData%LoadTimeStep(kper, kstp) for a MODFLOW 6 DIS grid;
+    ibound None takes MODPATH's IBOUND from IDOMAIN (ibound_from_idomain)."""
     n = grb.ncells
     default_iface = {k.strip(): int(v) for k, v in (default_iface or {}).items()}
     recs = [r for r in budget if r.kper == kper and r.kstp == kstp]
@@ -340,7 +345,7 @@ def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[Budge
             if r.ncol * r.nrow != ncpl or not 1 <= r.ilay <= grb.nlay:
                 raise Mf6Error("head record does not match the grid")
             h[(r.ilay - 1) * ncpl:r.ilay * ncpl] = r.data
-    ib = np.asarray(ibound, np.int32).copy()
+    ib = np.asarray(ibound_from_idomain(grb) if ibound is None else ibound, np.int32).copy()
     # IBoundTS: dry ICELLTYPE 1 cells (FlowModelData.f90:260-282)
     conv = grb.icelltype == 1
     diff = np.abs(h.astype(np.float32) - np.float32(hdry))
@@ -432,13 +437,23 @@ def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[Budge
     return TimeStepData(ib, h, fja, bfl, src, snk, sto, steady)
 
 
+def ibound_from_idomain(grb: Grb) -> np.ndarray:
+    """MODPATH's IBOUND of a MODFLOW 6 grid: 0 where IDOMAIN < 1, else 1
+    (ModpathBasicData.f90:177-185)."""
+    return np.where(grb.idomain < 1, 0, 1).astype(np.int32)
+
+
 def _neighbour_index(grb: Grb):
     """Index into FLOW-JA-FACE of each cell's right, front and lower
-    connection (-1 where the grid has no such neighbour)."""
+    connection (-1 where the grid has no such neighbour or either cell is
+    removed, IDOMAIN < 1)."""
     n, ncol, ncpl = grb.ncells, grb.ncol, grb.nrow * grb.ncol
     cnt = np.diff(grb.ia)
-    if np.any(cnt < 1) or np.any(grb.ja[grb.ia[:-1]] - 1 != np.arange(n)):
-        raise Mf6Error("JA must list each cell first in its own row")
+    live = grb.idomain >= 1
+    if np.any(cnt[live] < 1) or np.any(cnt[~live] != 0) or \
+            np.any(grb.ja[grb.ia[:-1][live]] - 1 != np.flatnonzero(live)):
+        raise Mf6Error("JA must list each cell first in its own row "
+                       "(and nothing for a removed cell)")
     # (row cell, connected cell) pairs as one sortable key
     row = np.repeat(np.arange(n, dtype=np.int64), cnt)
     key = row * (n + 1) + (grb.ja.astype(np.int64) - 1)
@@ -450,6 +465,8 @@ def _neighbour_index(grb: Grb):
     for f, (has, other) in enumerate(((col + 1 < grb.ncol, cell + 1),
                                       (rrow + 1 < grb.nrow, cell + ncol),
                                       (lay + 1 < grb.nlay, cell + ncpl))):
+        has = has.copy()
+        has[has] &= live[has] & live[other[has]]
         q = cell[has] * (n + 1) + other[has]
         pos = np.minimum(np.searchsorted(skey, q), len(skey) - 1)
         if np.any(skey[pos] != q):

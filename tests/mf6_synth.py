@@ -29,29 +29,36 @@ def _txt(s, n):
     return b + b" " * (n - len(b))
 
 
-def connections(nlay, nrow, ncol):
-    """IA (1-based) and JA of a DIS grid: each cell first, then its
-    neighbours in increasing cell number."""
+def connections(nlay, nrow, ncol, idomain=None):
+    """IA (1-based) and JA of a DIS grid in MODFLOW 6's user numbering: each
+    cell first, then its neighbours in increasing cell number.  A cell with
+    IDOMAIN < 1 is removed: its row is empty and no row lists it (the
+    binary grid file of a DIS model with removed cells maps the reduced
+    connections to user cell numbers this way, so FLOW-JA-FACE keeps JA's
+    positions)."""
     ncpl = nrow * ncol
+    n = nlay * nrow * ncol
+    on = np.ones(n, bool) if idomain is None else np.asarray(idomain) >= 1
     ia, ja = [1], []
-    for c in range(nlay * nrow * ncol):
-        k, r, j = c // ncpl, (c % ncpl) // ncol, c % ncol
-        nb = []
-        if k > 0: nb.append(c - ncpl)
-        if r > 0: nb.append(c - ncol)
-        if j > 0: nb.append(c - 1)
-        if j + 1 < ncol: nb.append(c + 1)
-        if r + 1 < nrow: nb.append(c + ncol)
-        if k + 1 < nlay: nb.append(c + ncpl)
-        ja += [c + 1] + [m + 1 for m in sorted(nb)]
+    for c in range(n):
+        if on[c]:
+            k, r, j = c // ncpl, (c % ncpl) // ncol, c % ncol
+            nb = []
+            if k > 0: nb.append(c - ncpl)
+            if r > 0: nb.append(c - ncol)
+            if j > 0: nb.append(c - 1)
+            if j + 1 < ncol: nb.append(c + 1)
+            if r + 1 < nrow: nb.append(c + ncol)
+            if k + 1 < nlay: nb.append(c + ncpl)
+            ja += [c + 1] + [m + 1 for m in sorted(nb) if on[m]]
         ia.append(len(ja) + 1)
     return np.array(ia, np.int32), np.array(ja, np.int32)
 
 
 def write_grb(path, nlay, nrow, ncol, delr, delc, top, botm, icelltype, idomain=None):
-    ia, ja = connections(nlay, nrow, ncol)
     n = nlay * nrow * ncol
     idomain = np.ones(n, np.int32) if idomain is None else idomain
+    ia, ja = connections(nlay, nrow, ncol, idomain)
     with open(path, "wb") as f:
         for line in ("GRID DIS", "VERSION 1", "NTXT 16", "LENTXT 100"):
             f.write(_txt(line, 50))

@@ -26,7 +26,8 @@ def test_defaults_are_the_full_run():
     assert bench.RUN_INTERVALS * bench.INTERVAL_STEPS == 1000
 
 
-@pytest.mark.parametrize("argv", [["--gpus", "2"], ["--gpus", "8"], ["--steps", "0"]])
+@pytest.mark.parametrize("argv", [["--gpus", "2"], ["--gpus", "8"], ["--steps", "0"],
+                                  ["--adjoint-seeds", "1"]])
 def test_unmeetable_requests_exit_2(argv):
     with pytest.raises(SystemExit) as ex:
         bench.parse_args(argv)

@@ -16,16 +16,18 @@ from oracle import oracle as O
 from oracle import refrun
 from rwpt import Engine, mf6, problems
 
-from test_mf6 import (HDRY, IFACE, engine_problem, oracle_problem, scenario,
+from test_mf6 import (HDRY, IFACE, REMOVED, engine_problem, oracle_problem, scenario,
                       two_step_scenario)
 
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("cell", "status", "lx", "ly", "lz", "t", "substeps", "rng")
+# the scenario as written, and with cells removed by IDOMAIN 0
+GRIDS = pytest.mark.parametrize("removed", [(), tuple(REMOVED)], ids=["full", "idomain"])
 
 
-def _read(tmp):
-    paths, P0, ibound = scenario(tmp)
+def _read(tmp, removed=()):
+    paths, P0, ibound = scenario(tmp, removed=removed)
     por = np.full(P0.ncell, 0.25)
     por[::5] = 0.3
     ret = np.ones(P0.ncell)
@@ -47,8 +49,9 @@ def _engine(P, extras):
     return E
 
 
-def test_engine_cells_match_reference_mf6(tmp_path):
-    paths, P, extras, ibound, por, ret = _read(str(tmp_path))
+@GRIDS
+def test_engine_cells_match_reference_mf6(tmp_path, removed):
+    paths, P, extras, ibound, por, ret = _read(str(tmp_path), removed)
     print("mf6 cells", P.ncol, P.nrow, P.nlay, flush=True)
     E = _engine(P, extras)
     try:
@@ -67,11 +70,12 @@ def test_engine_cells_match_reference_mf6(tmp_path):
     np.testing.assert_array_equal(got[act, 103], ref["cell_sink"][act])
 
 
-def test_advance_from_mf6_bitwise(tmp_path):
-    _, P, extras, _, _, _ = _read(str(tmp_path))
+@GRIDS
+def test_advance_from_mf6_bitwise(tmp_path, removed):
+    _, P, extras, _, _, _ = _read(str(tmp_path), removed)
     n = 600
     cell, lx, ly, lz = problems.release_in_block(P, n, (2, 7), (2, 5), (1, 3), seed=4)
-    print("mf6 advance", n, flush=True)
+    print("mf6 advance", n, "removed cells", len(removed), flush=True)
     E = _engine(P, extras)
     try:
         E.load_particles(cell, lx, ly, lz)

@@ -17,6 +17,9 @@ RCH with a default IFACE of 6 (top face), CHD and RIV with an IFACE
 auxiliary variable (boundary faces, and 0 = source/sink), STO-SS storage
 and a DATA-SPDIS record that must be skipped.  Layer 1 is convertible
 (ICELLTYPE 1) with some dry cells (head = HDRY) and a head below a bottom.
+Each test runs on that grid and on the same grid with six cells removed
+(IDOMAIN 0: empty rows in IA/JA, no flow across their faces, head
+HNOFLOW), whose IBOUND then comes from IDOMAIN as in ModpathBasicData.
 """
 import os
 
@@ -39,20 +42,44 @@ HDRY = -1e30
 IFACE = (("RCH", 6), ("EVT", 6))
 
 
-def scenario(tmp, seed=3, nlay=3, nrow=6, ncol=8):
-    """Writes the three files; returns (paths, problem, ibound, heads)."""
+# cells removed with IDOMAIN 0 (k, i, j, 1-based) in the "idomain" case: a
+# block through two layers, a column end and a single cell, none of them
+# holding a boundary package
+REMOVED = [(2, 3, 2), (2, 3, 3), (3, 3, 2), (3, 3, 3), (1, 5, 8), (3, 1, 5)]
+
+
+def scenario(tmp, seed=3, nlay=3, nrow=6, ncol=8, removed=()):
+    """Writes the three files; returns (paths, problem, ibound).  Cells in
+    `removed` get IDOMAIN 0: no connections in the grid file, head HNOFLOW
+    (1e30), no flow across their faces (the problem's face flows there are
+    set to 0 to match) and IBOUND 0."""
     rng = np.random.default_rng(seed)
     P = problems.heterogeneous_flow(ncol, nrow, nlay, corr_len=2.0, seed=seed)
     n, ncpl = P.ncell, nrow * ncol
+    idomain = np.ones(n, np.int32)
+    for k, i, j in removed:
+        c = int(P.cell_number(k, i, j)) - 1
+        idomain[c] = 0
+        # faces of c: right/left (frf), front/back (fff), lower/upper (flf)
+        P.frf[c] = 0.0
+        if c % ncol > 0:
+            P.frf[c - 1] = 0.0
+        P.fff[c] = 0.0
+        if (c % ncpl) // ncol > 0:
+            P.fff[c - ncol] = 0.0
+        P.flf[c] = 0.0
+        if c >= ncpl:
+            P.flf[c - ncpl] = 0.0
     icelltype = np.zeros(n, np.int32)
     icelltype[:ncpl] = 1
     head = P.bot + (P.top - P.bot) * rng.uniform(0.3, 1.4, n)
     dry = rng.choice(ncpl, 4, replace=False)
     head[dry] = HDRY
     head[dry[0] + 1 if dry[0] + 1 not in dry else dry[0]] = P.bot[0] - 1.0
+    head[idomain == 0] = 1e30
     grb, hds, cbc = (os.path.join(tmp, f) for f in ("m.dis.grb", "m.hds", "m.cbc"))
     ia, ja = S.write_grb(grb, nlay, nrow, ncol, P.delr, P.delc, P.top[:ncpl], P.bot,
-                         icelltype)
+                         icelltype, idomain)
     S.write_heads(hds, nlay, nrow, ncol, head)
     fja = S.flow_ja(ia, ja, P.frf, P.fff, P.flf, nrow, ncol)
     cells = lambda k, i, j: int(P.cell_number(k, i, j))
@@ -67,23 +94,29 @@ def scenario(tmp, seed=3, nlay=3, nrow=6, ncol=8):
          [[1.0 if m % 3 else 0.0] for m in range(len(chd))]),
         ("list", "RIV", "RIV-1", riv, [-1.5, 0.75, -0.25], ("IFACE", "COND"),
          [[6.0, 10.0], [0.0, 3.0], [2.0, 1.0]]),
-        ("array", "STO-SS", np.where(rng.uniform(size=n) < 0.2, rng.normal(size=n), 0.0)),
+        ("array", "STO-SS", np.where((rng.uniform(size=n) < 0.2) & (idomain == 1),
+                                     rng.normal(size=n), 0.0)),
         ("list", "DATA-SPDIS", "MODEL", list(range(1, n + 1)), [0.0] * n,
          ("QX", "QY", "QZ"), rng.normal(size=(n, 3)).tolist()),
     ]
     S.write_budget(cbc, nlay, nrow, ncol, fja, recs)
-    ibound = np.ones(n, np.int32)
-    ibound[cells(3, 6, 8) - 1] = 0
+    if removed:
+        # MODPATH's IBOUND of a MODFLOW 6 grid comes from IDOMAIN
+        ibound = mf6.ibound_from_idomain(mf6.read_grb(grb))
+    else:
+        ibound = np.ones(n, np.int32)
+        ibound[cells(3, 6, 8) - 1] = 0
     return (grb, hds, cbc), P, ibound
 
 
-@pytest.fixture(scope="module")
-def case(tmp_path_factory):
+@pytest.fixture(scope="module", params=["full", "idomain"])
+def case(request, tmp_path_factory):
     R = _refrun()
     if not R.mf6_available():
         pytest.skip("oracle/_ref/mf6_ref is not built (needs the reference sources)")
     tmp = str(tmp_path_factory.mktemp("mf6"))
-    paths, P, ibound = scenario(tmp)
+    removed = REMOVED if request.param == "idomain" else ()
+    paths, P, ibound = scenario(tmp, removed=removed)
     por = np.full(P.ncell, 0.25)
     por[::5] = 0.3
     ret = np.ones(P.ncell)
@@ -92,7 +125,8 @@ def case(tmp_path_factory):
     g = mf6.read_grb(paths[0])
     ts = mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
                             ibound, HDRY, dict(IFACE))
-    return dict(paths=paths, P=P, ibound=ibound, por=por, ret=ret, ref=ref, grb=g, ts=ts)
+    return dict(paths=paths, P=P, ibound=ibound, por=por, ret=ret, ref=ref, grb=g, ts=ts,
+                removed=removed)
 
 
 def test_grid_file(case):
@@ -102,6 +136,8 @@ def test_grid_file(case):
     np.testing.assert_array_equal(g.botm, ref["bottom"])
     np.testing.assert_array_equal(g.top, P.top)
     assert g.nja == len(ref["flows_ja"])
+    assert (g.idomain == 0).sum() == len(case["removed"])
+    np.testing.assert_array_equal(mf6.ibound_from_idomain(g) == 0, g.idomain < 1)
 
 
 def test_time_step_arrays_bitwise(case):
@@ -188,9 +224,11 @@ def test_rejects_other_files(tmp_path):
     paths, P, ibound = scenario(str(tmp_path))
     g = mf6.read_grb(paths[0])
     g.idomain[0] = 0
+    ts = mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
+                            ibound, HDRY)
     with pytest.raises(mf6.Mf6Error):
-        mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
-                           ibound, HDRY)
+        # a removed cell whose row in JA is not empty
+        mf6.engine_arrays(g, ts)
     with pytest.raises(mf6.Mf6Error):
         mf6.load_time_step(mf6.read_grb(paths[0]), mf6.read_heads(paths[1]),
                            mf6.read_budget(paths[2]), 2, 1, ibound, HDRY)
