@@ -246,8 +246,9 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
     Times the forward (rwpt_adjoint without gradient) and the adjoint, and
     checks the gradient with respect to aL of zone 1 and to the porosity
     scale:
-    - on a 4-step horizon against central differences at a relative step
-      of 1e-8 (no particle changes branch: the two must agree to rounding);
+    - on a 4-step horizon against central differences at the largest
+      relative step of 1e-8, 1e-9, 1e-10 at which no particle changes
+      branch between the +- runs (the two must then agree to rounding);
     - on the full horizon against central differences at the relative
       steps `rel_steps` with common random numbers within each +- pair,
       averaged over `seeds` base seeds, beside the gradient estimators of
@@ -270,7 +271,8 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
     bw = (2.0, 2.0, 1.0)
     seed0 = tuple(int(x) for x in default_options().base_seed)
 
-    def run(windows, obs, a_l=aL, por=1.0, grad=False, target=None, seed=0, opts=None):
+    def run(windows, obs, a_l=aL, por=1.0, grad=False, target=None, seed=0, opts=None,
+            state=False):
         Q = problems.zoned_dispersion(P, zones, a_l, aT, dm)
         Q.porosity = P.porosity * por
         E = Engine(Q, base_seed=((seed0[0] + 7919 * seed) % 2**64, seed0[1]))
@@ -283,6 +285,9 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
                             bw, gradient=grad)
             if opts and opts.get("diagnostics"):
                 out["stats"] = E.adjoint_stats()
+            if state:
+                g = E.particles()
+                out["state"] = (g["cell"].copy(), g["status"].copy(), g["substeps"].copy())
         finally:
             E.close()
         return out
@@ -306,16 +311,22 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
         return np.array([(x, y, zmid) for y in np.linspace(my - 1.5 * sy, my + 1.5 * sy, 4)
                          for x in np.linspace(mx - 1.5 * sx, mx + 1.5 * sx, 8)])
 
-    def central(windows, obs, tgt, param, h, seed):
+    def central(windows, obs, tgt, param, h, seed, changes=False):
         vals = []
         for sg in (1, -1):
             if param == "aL_zone1":
                 a2 = aL.copy()
                 a2[0] *= 1 + sg * h
-                vals.append(run(windows, obs, a_l=a2, target=tgt, seed=seed)["misfit"])
+                vals.append(run(windows, obs, a_l=a2, target=tgt, seed=seed, state=changes))
             else:
-                vals.append(run(windows, obs, por=1 + sg * h, target=tgt, seed=seed)["misfit"])
-        return (vals[0] - vals[1]) / (2 * h * (aL[0] if param == "aL_zone1" else 1.0))
+                vals.append(run(windows, obs, por=1 + sg * h, target=tgt, seed=seed,
+                                state=changes))
+        fd = (vals[0]["misfit"] - vals[1]["misfit"]) / (
+            2 * h * (aL[0] if param == "aL_zone1" else 1.0))
+        if not changes:
+            return fd
+        moved = sum(int((a != b).sum()) for a, b in zip(vals[0]["state"], vals[1]["state"]))
+        return fd, moved
 
     def pick(out):
         return {"aL_zone1": float(out["grad_al"][zones == 0].sum()),
@@ -327,7 +338,19 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
     obs_s = plume_points(short)
     tgt_s = 0.8 * run(short, obs_s)["conc"]
     g_short = pick(run(short, obs_s, grad=True, target=tgt_s))
-    fd_short = {p: central(short, obs_s, tgt_s, p, 1e-8, 0) for p in params}
+    # the largest of these steps at which no particle's cell, status or
+    # sub-step count differs between the +- runs (a branch change moves a
+    # particle by a fraction of a cell, not by O(h))
+    fd_short = {}
+    for p in params:
+        tried = []
+        for h in (1e-8, 1e-9, 1e-10):
+            fd, moved = central(short, obs_s, tgt_s, p, h, 0, changes=True)
+            tried.append({"rel_step": h, "central_difference": fd,
+                          "state_entries_changed": moved})
+            if moved == 0:
+                break
+        fd_short[p] = tried
 
     # full horizon: cost, diagnostics, seed-averaged central differences
     full = [dt * every * (k + 1) for k in range(total // every)]
@@ -381,8 +404,11 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
             "forward_ms": fwd["ms"], "adjoint_ms": A["ms"],
             "adjoint_over_forward": A["ms"] / fwd["ms"],
             "misfit": A["misfit"],
-            "fd_check_4_steps": {p: {"adjoint": g_short[p], "central_difference": fd_short[p],
-                                     "rel_step": 1e-8, "ratio": g_short[p] / fd_short[p]}
+            "fd_check_4_steps": {p: {"adjoint": g_short[p],
+                                     "central_difference": fd_short[p][-1]["central_difference"],
+                                     "rel_step": fd_short[p][-1]["rel_step"],
+                                     "ratio": g_short[p] / fd_short[p][-1]["central_difference"],
+                                     "steps_tried": fd_short[p]}
                                  for p in params},
             "fd_check_300_steps": {"seeds": seeds, **checks},
             "sweep_diagnostics": diag}
