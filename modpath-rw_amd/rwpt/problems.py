@@ -432,3 +432,74 @@ def observation_points(P: Problem, n_x=8, n_y=4, cols=(20, 160), rows=(85, 115))
     js = np.linspace(cols[0], cols[1], n_x).astype(int) - 1
     is_ = np.linspace(rows[0], rows[1], n_y).astype(int) - 1
     return np.array([(xc[j], yc[i], zmid) for i in is_ for j in js])
+
+
+@dataclasses.dataclass
+class UsgProblem:
+    """An unstructured grid of rectangular cells (MODFLOW-USG, or a MODFLOW
+    6 DISV smoothed quadtree) with its flows, as ModflowRectangularGrid and
+    FlowModelData hold them (RectangularGridDisvMf6.f90:47-274,
+    FlowModelData.f90:217-622):
+
+    * ``ia`` (ncell+1) and ``ja`` are MODFLOW's 1-based CSR connectivity,
+      each row led by the cell itself; ``layer`` is each cell's 1-based
+      layer; ``cell_x, cell_y, del_x, del_y`` its centre and sizes;
+    * ``flowja`` is FLOW-JA-FACE: entry q of row n is the flow into n from
+      cell ja[q]; ``bflow`` (6 per cell), ``storage``, ``source``, ``sink``
+      the boundary, storage and source/sink flows;
+    * the per-cell properties and the options are those of ``Problem``;
+      ``layer_type`` is the grid's ICELLTYPE (GetDZRW).
+    """
+    ncell: int
+    nlay: int
+    ia: np.ndarray
+    ja: np.ndarray
+    layer: np.ndarray
+    cell_x: np.ndarray
+    cell_y: np.ndarray
+    del_x: np.ndarray
+    del_y: np.ndarray
+    top: np.ndarray
+    bot: np.ndarray
+    flowja: np.ndarray
+    porosity: np.ndarray
+    bflow: Optional[np.ndarray] = None
+    storage: Optional[np.ndarray] = None
+    source: Optional[np.ndarray] = None
+    sink: Optional[np.ndarray] = None
+    retardation: Optional[np.ndarray] = None
+    head: Optional[np.ndarray] = None
+    layer_type: Optional[np.ndarray] = None
+    zone: Optional[np.ndarray] = None
+    ibound: Optional[np.ndarray] = None
+    icbound: Optional[np.ndarray] = None
+    default_icbound: int = 0
+    disp_model: int = 0
+    aL: np.ndarray = dataclasses.field(default_factory=lambda: np.array([0.1]))
+    aT: np.ndarray = dataclasses.field(default_factory=lambda: np.array([0.01]))
+    aLV: Optional[np.ndarray] = None
+    aTV: Optional[np.ndarray] = None
+    dmeff: np.ndarray = dataclasses.field(default_factory=lambda: np.array([1e-4]))
+    advection_kind: int = 2
+    time_step_kind: int = 4
+    ts_params: tuple = (0.25, 0.0)
+    dim_mask: tuple = (1, 1, 1)
+    backward: int = 0
+    stop_zone: int = 0
+    stop_weak_sinks: int = 0
+    stop_weak_sources: int = 0
+    steady_state: int = 1
+
+    @property
+    def nja(self) -> int:
+        return int(np.asarray(self.ja).size)
+
+    @property
+    def disp_uniform(self) -> int:
+        return int(np.asarray(self.aL).size == 1)
+
+    @property
+    def uniform_porosity(self) -> int:
+        """isUniformPorosity, ModpathBasicData.f90:208."""
+        p = np.asarray(self.porosity)
+        return int(bool(np.all(p == p[0])))

@@ -224,3 +224,129 @@ def kde_density(x, y, z, origin, binsize, nbins, w=None, selection=0, factor=1.0
                           C.c_int(selection), C.c_double(factor), _ptr(dens, _dp),
                           _ptr(counts, _dp), _ptr(h, _dp))
     return dens, counts, h
+
+
+# --------------------------------------------------------------------------
+# Unstructured grids (rwpt_oracle_usg.c)
+# --------------------------------------------------------------------------
+
+class OrcUsgProblem(C.Structure):
+    _fields_ = [
+        ("base", OrcProblem), ("ncell", C.c_int), ("nlay", C.c_int),
+        ("nja", C.c_long), ("ia", _ip), ("ja", _ip), ("layer", _ip),
+        ("cell_x", _dp), ("cell_y", _dp), ("del_x", _dp), ("del_y", _dp),
+        ("flowja", _dp), ("storage", _dp),
+    ]
+
+
+USG_FIELDS = 126
+
+
+def _usg_lib():
+    L = lib()
+    if not hasattr(L, "_usg_ready"):
+        L.orc_usg_tables.argtypes = [C.POINTER(OrcUsgProblem), _dp]
+        L.orc_usg_tables.restype = C.c_long
+        L.orc_usg_advance.argtypes = [C.POINTER(OrcUsgProblem), C.c_long, _ip, _dp, _dp,
+                                      _dp, _dp, _ip, _u64p, C.c_double, C.c_double,
+                                      C.POINTER(C.c_longlong), C.c_int]
+        L.orc_usg_advance.restype = C.c_int
+        L.orc_usg_global_xyz.argtypes = [C.POINTER(OrcUsgProblem), C.c_long, _ip, _dp,
+                                         _dp, _dp, _dp, _dp, _dp]
+        L._usg_ready = True
+    return L
+
+
+class OracleUsgProblem:
+    """The C view of an unstructured problem U (attributes as
+    rwpt.problems.UsgProblem); keeps the arrays alive."""
+
+    def __init__(self, U):
+        self._keep = {}
+        s = OrcUsgProblem()
+        b = s.base
+
+        def dd(name, arr):
+            a = _d(arr)
+            self._keep[name] = a
+            return _ptr(a, _dp)
+
+        def ii(name, arr):
+            a = _i(arr)
+            self._keep[name] = a
+            return _ptr(a, _ip)
+
+        b.top = dd("top", U.top)
+        b.bot = dd("bot", U.bot)
+        b.ibound = ii("ibound", U.ibound)
+        b.bflow = dd("bflow", U.bflow)
+        b.porosity = dd("porosity", U.porosity)
+        b.retardation = dd("retardation", U.retardation)
+        b.head = dd("head", U.head)
+        b.layer_type = ii("layer_type", U.layer_type)
+        b.source_flow = dd("source_flow", U.source)
+        b.sink_flow = dd("sink_flow", U.sink)
+        b.zone = ii("zone", U.zone)
+        b.icbound = ii("icbound", U.icbound)
+        b.default_icbound = U.default_icbound
+        b.disp_model = U.disp_model
+        b.disp_uniform = 1 if np.asarray(U.aL).size == 1 else 0
+        b.aL = dd("aL", U.aL)
+        b.aT = dd("aT", U.aT)
+        b.aLV = dd("aLV", U.aLV)
+        b.aTV = dd("aTV", U.aTV)
+        b.dmeff = dd("dmeff", U.dmeff)
+        b.advection_kind = U.advection_kind
+        b.time_step_kind = U.time_step_kind
+        b.ts_params[0], b.ts_params[1] = U.ts_params
+        for d in range(3):
+            b.dim_mask[d] = U.dim_mask[d]
+        b.uniform_porosity = U.uniform_porosity
+        b.backward = U.backward
+        b.stop_weak_sinks = int(U.stop_weak_sinks)
+        b.stop_weak_sources = int(U.stop_weak_sources)
+        b.stop_zone = int(U.stop_zone)
+        b.steady_state = int(U.steady_state)
+        s.ncell, s.nlay, s.nja = U.ncell, U.nlay, len(U.ja)
+        s.ia = ii("ia", U.ia)
+        s.ja = ii("ja", U.ja)
+        s.layer = ii("layer", U.layer)
+        s.cell_x = dd("cell_x", U.cell_x)
+        s.cell_y = dd("cell_y", U.cell_y)
+        s.del_x = dd("del_x", U.del_x)
+        s.del_y = dd("del_y", U.del_y)
+        s.flowja = dd("flowja", U.flowja)
+        s.storage = dd("storage", U.storage)
+        self.s = s
+
+
+def usg_tables(U):
+    """(n_subcells, USG_FIELDS) sub-cell tables (layout in rwpt_oracle_usg.c)."""
+    OP = U if isinstance(U, OracleUsgProblem) else OracleUsgProblem(U)
+    L = _usg_lib()
+    k = L.orc_usg_tables(C.byref(OP.s), None)
+    if k < 0:
+        raise RuntimeError(f"orc_usg_tables: error {k}")
+    out = np.zeros((k, USG_FIELDS))
+    L.orc_usg_tables(C.byref(OP.s), out.ctypes.data_as(_dp))
+    return out
+
+
+def usg_advance(U, state, t_engine, t_stop, nthreads=0):
+    OP = U if isinstance(U, OracleUsgProblem) else OracleUsgProblem(U)
+    n = state["cell"].size
+    return _usg_lib().orc_usg_advance(
+        C.byref(OP.s), n, _ptr(state["cell"], _ip), _ptr(state["lx"], _dp),
+        _ptr(state["ly"], _dp), _ptr(state["lz"], _dp), _ptr(state["t"], _dp),
+        _ptr(state["status"], _ip), _ptr(state["rng"], _u64p), t_engine, t_stop,
+        state["substeps"].ctypes.data_as(C.POINTER(C.c_longlong)), nthreads)
+
+
+def usg_global_xyz(U, cell, lx, ly, lz):
+    OP = U if isinstance(U, OracleUsgProblem) else OracleUsgProblem(U)
+    n = cell.size
+    gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
+    _usg_lib().orc_usg_global_xyz(C.byref(OP.s), n, _ptr(_i(cell), _ip),
+                                  _ptr(_d(lx), _dp), _ptr(_d(ly), _dp), _ptr(_d(lz), _dp),
+                                  _ptr(gx, _dp), _ptr(gy, _dp), _ptr(gz, _dp))
+    return gx, gy, gz

@@ -268,3 +268,106 @@ def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e
     out["velocity"] = cells[:, 8:14].copy()
     out["size"] = cells[:, 14:17].copy()
     return out
+
+
+USG_BINARY = os.path.join(HERE, "_ref", "usg_ref")
+
+
+def usg_available() -> bool:
+    return os.access(USG_BINARY, os.X_OK)
+
+
+def _usg_input(U, cell, lx, ly, lz, t_out, t, status):
+    n = int(np.asarray(cell).size)
+    nc = U.ncell
+    nd = int(np.asarray(U.aL).size)
+    t_out = np.ascontiguousarray(t_out, np.float64)
+    hdr = np.zeros(20, np.int32)
+    hdr[0], hdr[1] = n, t_out.size
+    hdr[2], hdr[3] = U.advection_kind, U.time_step_kind
+    hdr[4:7] = U.dim_mask
+    hdr[7], hdr[8], hdr[9] = U.disp_model, nd, U.default_icbound
+    hdr[10], hdr[11] = U.backward, U.uniform_porosity
+    hdr[12] = U.stop_weak_sinks
+    hdr[13] = U.stop_weak_sources
+    hdr[14] = U.stop_zone
+    hdr[15] = int(U.steady_state)
+    f8 = np.float64
+    return [hdr, np.asarray(U.ts_params, f8), np.asarray(U.flowja, f8),
+            _filled(U.bflow, 6 * nc, 0, f8), _filled(U.storage, nc, 0, f8),
+            _filled(U.source, nc, 0, f8), _filled(U.sink, nc, 0, f8),
+            _filled(U.porosity, nc, 0, f8), _filled(U.retardation, nc, 1.0, f8),
+            _filled(U.head, nc, 0, f8) if U.head is not None else _filled(U.top, nc, 0, f8),
+            _filled(U.ibound, nc, 1, np.int32), _filled(U.icbound, nc, 0, np.int32),
+            _filled(U.zone, nc, 1, np.int32),
+            _filled(U.aL, nd, 0, f8), _filled(U.aT, nd, 0, f8),
+            _filled(U.aLV, nd, 0, f8), _filled(U.aTV, nd, 0, f8),
+            _filled(U.dmeff, nd, 0, f8),
+            np.ascontiguousarray(cell, np.int32), np.ascontiguousarray(lx, f8),
+            np.ascontiguousarray(ly, f8), np.ascontiguousarray(lz, f8),
+            _filled(t, n, 0.0, f8), _filled(status, n, 1, np.int32), t_out]
+
+
+def _usg_call(grb, parts, mode, team, timeout):
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as fh:
+            for a in parts:
+                fh.write(np.ascontiguousarray(a).tobytes())
+        env = dict(os.environ, OMP_STACKSIZE="2M", OMP_THREAD_LIMIT=str(team),
+                   OMP_DYNAMIC="false")
+        subprocess.run([USG_BINARY, grb, fin, fout, mode], check=True, timeout=timeout,
+                       env=env, stdout=subprocess.PIPE, text=True)
+        return open(fout, "rb").read()
+
+
+def usg_walk(grb, U, cell, lx, ly, lz, t_out, t=None, status=None, timeout=600):
+    """Tracks particles on the DISV grid of the binary grid file grb through
+    the output times (one OpenMP thread and rng stream per particle).
+    Returns a list of dicts cell/status/lx/ly/lz/t, one per output time."""
+    n = int(np.asarray(cell).size)
+    raw = _usg_call(grb, _usg_input(U, cell, lx, ly, lz, t_out, t, status), "walk",
+                    max(n, 1), timeout)
+    out, off = [], 0
+    for _ in range(np.asarray(t_out).size):
+        rec = {}
+        for name, dt in (("cell", np.int32), ("status", np.int32), ("lx", np.float64),
+                         ("ly", np.float64), ("lz", np.float64), ("t", np.float64)):
+            sz = n * np.dtype(dt).itemsize
+            rec[name] = np.frombuffer(raw[off:off + sz], dt).copy()
+            off += sz
+        out.append(rec)
+    return out
+
+
+# fields of one sub-cell record of usg_ref's tables mode
+USG_INTS = 18      # row col conn(6) massbnd(6) hasexit cellhasexit dry partiallydry
+USG_DOUBLES = 18 + 32 + 8 + 48
+
+
+def usg_tables(grb, U, timeout=600):
+    """Sub-cell tables of every cell: a list (per cell) of lists (per
+    sub-cell, row-major) of (ints[18], doubles[106]) as usg_ref writes them:
+    ints row col conn(6) massbnd(6) hasexit cellhasexit dry partiallydry;
+    doubles VX1..VZ2 DX DY DZ offx(2) offy(2) porosity retardation dtcell
+    source sink, qcorner(4 x 8 corners), porosity corners(8), Dxxx Dyyy
+    Dzzz Dxyx Dxzx Dyzy (8 each)."""
+    z = np.zeros(0)
+    raw = _usg_call(grb, _usg_input(U, np.zeros(0, np.int32), z, z, z, np.zeros(1), None,
+                                    None), "tables", 1, timeout)
+    off = 0
+    ncell = int(np.frombuffer(raw[off:off + 4], np.int32)[0])
+    off += 4
+    cells = []
+    for _ in range(ncell):
+        nsub = int(np.frombuffer(raw[off:off + 4], np.int32)[0])
+        off += 4
+        subs = []
+        for _ in range(nsub):
+            ints = np.frombuffer(raw[off:off + 4 * USG_INTS], np.int32).copy()
+            off += 4 * USG_INTS
+            dbl = np.frombuffer(raw[off:off + 8 * USG_DOUBLES], np.float64).copy()
+            off += 8 * USG_DOUBLES
+            subs.append((ints, dbl))
+        cells.append(subs)
+    return cells

@@ -364,6 +364,11 @@ static const int cp_idx[8][6] = {{1, 2, 7, 8, 13, 14},   {4, 5, 7, 8, 13, 15},
                                  {1, 2, 7, 9, 16, 17},   {4, 5, 7, 9, 16, 18},
                                  {1, 3, 10, 12, 16, 17}, {4, 6, 10, 11, 13, 15}};
 
+static void corners_from_stencil(const orc_problem *P, const scbuf *s,
+                                 const double cq[6], double nflow[18][6],
+                                 double narea[18][3], const double nvol[18],
+                                 const double nphi[18], corners *C);
+
 static void compute_corners(const orc_problem *P, cbuf *center, cbuf nb[18],
                             const scbuf *s, corners *C) {
   double nflow[18][6], narea[18][3], nvol[18], nphi[18];
@@ -383,8 +388,17 @@ static void compute_corners(const orc_problem *P, cbuf *center, cbuf nb[18],
     nvol[n] = dx * dy * dz2;
     nphi[n] = nb[n].porosity;
   }
+  corners_from_stencil(P, s, center->Q, nflow, narea, nvol, nphi, C);
+}
+
+/* Corner variables of a sub-cell from its face flows cq and the face
+ * flows, face areas, volumes and porosities of its 18 stencil cells
+ * (pr_ComputeCornerVariables after pr_FillNeighborSubCellVariables). */
+static void corners_from_stencil(const orc_problem *P, const scbuf *s,
+                                 const double cq[6], double nflow[18][6],
+                                 double narea[18][3], const double nvol[18],
+                                 const double nphi[18], corners *C) {
   /* pr_ComputeCornerDischarge, :4016-4070 + GetInterpolatedCornerDischarge */
-  const double *cq = center->Q;
   for (int dir = 0; dir < 3; ++dir) {
     for (int ci = 0; ci < 8; ++ci) {
       const int *ix = dir == 0 ? cx_idx[ci] : dir == 1 ? cy_idx[ci] : cz_idx[ci];
@@ -474,7 +488,6 @@ static void compute_corners(const orc_problem *P, cbuf *center, cbuf nb[18],
       C->Dyzz[ci] = C->Dyzy[ci];
     }
   }
-  (void)center;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1875,3 +1888,8 @@ int orc_kde_density_sel(long n, const double *x, const double *y,
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Unstructured grids (shares the static sub-cell walk and corner formulas) */
+/* ------------------------------------------------------------------------ */
+#include "rwpt_oracle_usg.c"

@@ -114,6 +114,38 @@ double orc_newton_axis(double dt, double v, double v1, double v2, double dx,
 double orc_quadratic_axis(double dt, double n, double p, double d, double v,
                           double divD, double dB, int *exit_face);
 
+/* Unstructured grids of rectangular cells (MODFLOW-USG, MODFLOW 6 DISV
+ * smoothed quadtrees): the grid as CSR connectivity plus cell geometry,
+ * FLOW-JA-FACE, and the options and per-cell arrays of `base` (top, bot,
+ * ibound, bflow, porosity, retardation, head, layer_type, source_flow,
+ * sink_flow, zone, icbound, dispersion and tracking options; its ncol,
+ * nrow, nlay, delr, delc, frf, fff, flf are not used). */
+typedef struct {
+  orc_problem base;
+  int ncell, nlay;
+  long nja;
+  const int *ia;         /* ncell+1, 1-based (MODFLOW IA)                   */
+  const int *ja;         /* nja, 1-based, each row led by the cell itself    */
+  const int *layer;      /* ncell, 1-based layer of each cell                */
+  const double *cell_x, *cell_y, *del_x, *del_y; /* ncell                     */
+  const double *flowja;  /* nja: flow into the row's cell from JA's cell     */
+  const double *storage; /* ncell or NULL                                    */
+} orc_usg_problem;
+
+/* Sub-cell tables, ORC_USG_FIELDS doubles per sub-cell (rwpt_oracle_usg.c);
+ * returns the record count (out NULL: count only) or a negative error. */
+#define ORC_USG_FIELDS 126
+long orc_usg_tables(const orc_usg_problem *U, double *out);
+/* orc_advance on an unstructured grid; locations are those of the parent
+ * cell (what pr_RWPTrackPath reports). */
+int orc_usg_advance(const orc_usg_problem *U, long n, int *cell, double *lx,
+                    double *ly, double *lz, double *t, int *status,
+                    uint64_t *rng, double t_engine, double t_stop,
+                    long long *substeps, int nthreads);
+void orc_usg_global_xyz(const orc_usg_problem *U, long n, const int *cell,
+                        const double *lx, const double *ly, const double *lz,
+                        double *gx, double *gy, double *gz);
+
 #ifdef __cplusplus
 }
 #endif
