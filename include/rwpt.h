@@ -54,6 +54,26 @@ typedef struct {
   const double *bot;  /* ncell cell bottom elevations                   */
 } rwpt_grid;
 
+/* Unstructured grid of rectangular cells: MODFLOW-USG or a MODFLOW 6 DISV
+ * smoothed quadtree (RectangularGridDisvMf6.f90:47-574,
+ * RectangularGridDisuMfusg.f90): MODFLOW's 1-based CSR connectivity, each
+ * row led by the cell itself, and every cell's centre, sizes, layer and
+ * elevations, as ModflowRectangularGridType holds them after ReadGRB /
+ * CheckRectangular.  A horizontal face meets one cell of its size, one
+ * twice its size or two of half its size; cells are then tracked in 2 x 2
+ * sub-cells as the reference does. */
+typedef struct {
+  int32_t ncell, nlay;
+  int64_t nja;
+  const int32_t *ia;        /* ncell+1, 1-based (IA)                      */
+  const int32_t *ja;        /* nja, 1-based (JA)                          */
+  const int32_t *layer;     /* ncell, 1-based layer of each cell          */
+  const double *cell_x, *cell_y;  /* ncell cell centres (CellX, CellY)    */
+  const double *del_x, *del_y;    /* ncell cell sizes (DelX, DelY)        */
+  const double *top, *bot;        /* ncell elevations                      */
+  const int32_t *cell_type; /* ncell ICELLTYPE (GetDZRW's layer type) or NULL */
+} rwpt_usg_grid;
+
 /* Tracking options: the random-walk fields of ParticleTrackingOptionsType
  * (ParticleTrackingOptions.f90:20-31) read from RWOPTS
  * (ModpathSimulationData.f90:1770-2021). */
@@ -151,6 +171,37 @@ int rwpt_create(const rwpt_grid *grid, const rwpt_options *opt,
  * rwpt_create is rwpt_create_ensemble with one member. */
 int rwpt_create_ensemble(const rwpt_grid *grid, const rwpt_options *opt,
                          int32_t members, rwpt_engine **out);
+/* ParticleTrackingEngine%Initialize on an unstructured grid
+ * (ParticleTrackingEngine.f90:393-492, grid types 2 and 4: the unstructured
+ * cell fill pr_FillTransportCellBufferUnstructured :711-748 and neighbour
+ * fill pr_FillNeighborCellDataUnstructured :1257-2005).  Needs precision 0
+ * (FP64).  The flow comes from rwpt_set_flow_usg; rwpt_set_cell_extras,
+ * rwpt_set_dispersion (uniform 0 or 1, per cell), the particle calls,
+ * rwpt_advance, rwpt_global_xyz, rwpt_kde and rwpt_kde_write work as on a
+ * structured grid (cells numbered as the grid's).  Particle locations are
+ * local coordinates of the cell, as pr_RWPTrackPath reports them. */
+int rwpt_create_usg(const rwpt_usg_grid *grid, const rwpt_options *opt,
+                    rwpt_engine **out);
+/* FlowModelData's arrays on an unstructured grid (FlowModelData.f90:217-622):
+ * flowja (nja) is FLOW-JA-FACE, entry q of row n the flow into n from
+ * ja[q]; bflow (6*ncell, IFACE boundary flows) and storage (ncell) may be
+ * NULL; the others as in rwpt_set_flow. */
+int rwpt_set_flow_usg(rwpt_engine *e, const double *flowja, const double *bflow,
+                      const double *storage, const double *porosity,
+                      const double *retardation, const int32_t *ibound,
+                      const int32_t *icbound, int32_t default_icbound);
+/* Tracking units of an unstructured engine (one per unrefined cell, four
+ * per refined cell), -1 for a structured engine. */
+int64_t rwpt_usg_unit_count(const rwpt_engine *e);
+/* Checking copy of the unit records (no counterpart in the reference):
+ * RWPT_USG_TABLE_FIELDS doubles per unit: cell, sub-row, sub-column,
+ * sub-cells of the cell, connection and mass boundary of faces 1-6,
+ * sub-cell has exit, cell has exit, dry, partially dry, VX1..VZ2, DX DY DZ,
+ * offsets x0 x1 y0 y1, porosity, retardation, time step, source, sink,
+ * corner discharge (8 corners x 4), corner porosity (8), Dxxx Dyyy Dzzz
+ * Dxyx Dxzx Dyzy (8 each). */
+#define RWPT_USG_TABLE_FIELDS 126
+int rwpt_get_usg_tables(rwpt_engine *e, double *out);
 int rwpt_destroy(rwpt_engine *e);
 const char *rwpt_last_error(const rwpt_engine *e);
 

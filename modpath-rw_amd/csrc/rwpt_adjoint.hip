@@ -1785,6 +1785,7 @@ extern "C" int rwpt_adjoint(rwpt_engine *e, int32_t n_windows, const double *t_w
                             double *conc, double *grad_al, double *grad_at, double *grad_dm,
                             double *grad_porosity_scale, double *ms) {
   if (!e) return RWPT_E_ARG;
+  if (e->usg) return adj_fail(e, RWPT_E_UNSUPPORTED, "the adjoint needs a structured grid");
   if (n_windows < 1 || !t_windows) return adj_fail(e, RWPT_E_ARG, "n_windows >= 1 times");
   for (int w = 0; w < n_windows; ++w)
     if (!std::isfinite(t_windows[w])) return adj_fail(e, RWPT_E_ARG, "window time not finite");

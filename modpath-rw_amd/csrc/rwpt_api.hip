@@ -82,12 +82,30 @@ void free_particles(Particles &P) {
   P.n = P.cap = 0;
 }
 
+void free_usg_particles(UsgData &U) {
+  dev_free(U.unit); dev_free(U.pcell);
+  dev_free(U.ex); dev_free(U.ey); dev_free(U.ez); dev_free(U.et);
+  dev_free(U.px); dev_free(U.py); dev_free(U.pz);
+  U.pcap = 0;
+}
+
+void free_usg(UsgData *&U) {
+  if (!U) return;
+  free_usg_particles(*U);
+  dev_free(U->topo); dev_free(U->cell_unit); dev_free(U->side); dev_free(U->slots);
+  dev_free(U->layer); dev_free(U->cell_type); dev_free(U->off);
+  dev_free(U->cx); dev_free(U->cy); dev_free(U->dx); dev_free(U->dy);
+  dev_free(U->flowja); dev_free(U->storage); dev_free(U->casm);
+  delete U;
+  U = nullptr;
+}
+
 int ensure_tables(rwpt_engine *e) {
   if (!e->have_flow) return fail(e, RWPT_E_STATE, "flow not set");
   if (!e->have_disp) return fail(e, RWPT_E_STATE, "dispersion not set");
   hipError_t err = hipSuccess;
   if (e->tables_dirty) {
-    err = build_cell_tables(e, nullptr);
+    err = e->usg ? build_usg_tables(e, nullptr) : build_cell_tables(e, nullptr);
     if (err != hipSuccess) return hip_fail(e, err, "cell tables");
     e->tables_dirty = false;
     e->tables32_dirty = true;
@@ -229,6 +247,7 @@ int rwpt_destroy(rwpt_engine *e) {
   dev_free(P.q6); dev_free(P.dzrw); dev_free(P.cells); dev_free(P.cells32);
   dev_free(P.uniform_member);
   free_particles(e->part);
+  free_usg(e->usg);
   dev_free(e->jump_mats);
   dev_free(e->counters);
   dev_free(e->kde_scratch);
@@ -257,6 +276,7 @@ int rwpt_set_flow(rwpt_engine *e, const double *frf, const double *fff,
                   const int32_t *ibound, const int32_t *icbound,
                   int32_t default_icbound) {
   if (!e) return RWPT_E_ARG;
+  if (e->usg) return fail(e, RWPT_E_STATE, "unstructured grid: use rwpt_set_flow_usg");
   if (!frf || !fff || !flf || !porosity)
     return fail(e, RWPT_E_ARG, "frf, fff, flf and porosity are required");
   DeviceProblem &P = e->prob;
@@ -392,6 +412,15 @@ int rwpt_load_particles(rwpt_engine *e, int64_t n, const int32_t *cell,
     for (int k = 0; k < 2 && err == hipSuccess; ++k) {
       err = dev_alloc(&P.keys[k], n);
       if (err == hipSuccess) err = dev_alloc(&P.vals[k], n);
+    }
+    if (e->usg && err == hipSuccess) {
+      UsgData &U = *e->usg;
+      free_usg_particles(U);
+      err = dev_alloc(&U.unit, n);
+      if (err == hipSuccess) err = dev_alloc(&U.pcell, n);
+      for (double **a : {&U.ex, &U.ey, &U.ez, &U.et, &U.px, &U.py, &U.pz})
+        if (err == hipSuccess) err = dev_alloc(a, n);
+      U.pcap = n;
     }
     P.cap = n;
     if (err == hipSuccess) err = ensure_sort_storage(e);
@@ -546,6 +575,7 @@ int rwpt_autotune(rwpt_engine *e, double t_probe, int32_t *budget, int32_t *bloc
                   int32_t *sort, int32_t *min_waves, double *ms_best,
                   int64_t *trial_substeps) {
   if (!e) return RWPT_E_ARG;
+  if (e->usg) return fail(e, RWPT_E_UNSUPPORTED, "the autotuner needs a structured grid");
   if (!std::isfinite(t_probe) || t_probe == e->t_engine)
     return fail(e, RWPT_E_ARG, "t_probe must differ from the engine time");
   int rc = ensure_tables(e);
@@ -681,6 +711,7 @@ int rwpt_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz) {
 
 int rwpt_get_cell_tables(rwpt_engine *e, double *out) {
   if (!e || !out) return RWPT_E_ARG;
+  if (e->usg) return fail(e, RWPT_E_STATE, "unstructured grid: use rwpt_get_usg_tables");
   if (!e->have_flow || !e->have_disp)
     return fail(e, RWPT_E_STATE, "flow and dispersion must be set");
   double *d = nullptr;
@@ -693,6 +724,157 @@ int rwpt_get_cell_tables(rwpt_engine *e, double *out) {
   if (err != hipSuccess) return hip_fail(e, err, "cell tables");
   e->tables_dirty = false;
   e->tables32_dirty = true;
+  return RWPT_OK;
+}
+
+int rwpt_create_usg(const rwpt_usg_grid *grid, const rwpt_options *opt,
+                    rwpt_engine **out) {
+  if (!out) return RWPT_E_ARG;
+  *out = nullptr;
+  if (!grid || !opt) return RWPT_E_ARG;
+  const rwpt_usg_grid &g = *grid;
+  if (g.ncell <= 0 || g.nlay <= 0 || g.nja < g.ncell || g.nja > 0x7fffffffLL || !g.ia ||
+      !g.ja || !g.layer || !g.cell_x || !g.cell_y || !g.del_x || !g.del_y || !g.top ||
+      !g.bot)
+    return RWPT_E_ARG;
+  if (opt->precision != 0) return RWPT_E_UNSUPPORTED;
+  if (opt->advection_kind != 1 && opt->advection_kind != 2) return RWPT_E_ARG;
+  if (opt->time_step_kind < 1 || opt->time_step_kind > 4) return RWPT_E_ARG;
+  int nd = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (opt->dimension_mask[d] != 0 && opt->dimension_mask[d] != 1) return RWPT_E_ARG;
+    nd += opt->dimension_mask[d];
+  }
+  if (nd == 0) return RWPT_E_ARG;
+  // the connectivity must be a CSR of the cells, each row led by its cell
+  if (g.ia[0] != 1 || g.ia[g.ncell] != g.nja + 1) return RWPT_E_ARG;
+  for (int c = 1; c <= g.ncell; ++c) {
+    const int32_t a = g.ia[c - 1], b = g.ia[c];
+    if (b <= a || g.ja[a - 1] != c) return RWPT_E_ARG;
+    if (g.layer[c - 1] < 1 || g.layer[c - 1] > g.nlay) return RWPT_E_ARG;
+    if (!(g.del_x[c - 1] > 0.0) || !(g.del_y[c - 1] > 0.0)) return RWPT_E_ARG;
+    for (int32_t q = a; q < b - 1; ++q)
+      if (g.ja[q] < 1 || g.ja[q] > g.ncell) return RWPT_E_ARG;
+  }
+  rwpt_engine *e = new rwpt_engine();
+  e->opt = *opt;
+  UsgHost H;
+  const std::string terr = usg_build_topology(g, H);
+  if (!terr.empty()) {
+    fprintf(stderr, "rwpt_create_usg: %s\n", terr.c_str());
+    delete e;
+    return RWPT_E_UNSUPPORTED;
+  }
+  hipError_t err = hipGetDevice(&e->device);
+  if (err == hipSuccess) err = kde_device_scratch_init(e->device);
+  if (err == hipSuccess) err = hipStreamCreate(&e->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(e, err, "create_usg");
+    fprintf(stderr, "rwpt_create_usg: %s\n", e->err.c_str());
+    delete e;
+    return rc;
+  }
+  e->usg = new UsgData();
+  UsgData &U = *e->usg;
+  U.ncell = g.ncell;
+  U.nunit = H.nunit;
+  U.nlay = g.nlay;
+  U.nja = g.nja;
+  DeviceProblem &P = e->prob;
+  P.ncol = 0;
+  P.nrow = 0;
+  P.nlay = g.nlay;
+  P.nlay_member = g.nlay;
+  P.members = 1;
+  P.ncell = g.ncell;
+  P.ncell_member = g.ncell;
+  const int64_t n = g.ncell;
+  err = upload(e, &P.top, g.top, n);
+  if (err == hipSuccess) err = upload(e, &P.bot, g.bot, n);
+  if (err == hipSuccess) err = upload(e, &U.cx, g.cell_x, n);
+  if (err == hipSuccess) err = upload(e, &U.cy, g.cell_y, n);
+  if (err == hipSuccess) err = upload(e, &U.dx, g.del_x, n);
+  if (err == hipSuccess) err = upload(e, &U.dy, g.del_y, n);
+  if (err == hipSuccess) err = upload(e, &U.layer, g.layer, n);
+  if (err == hipSuccess) err = upload(e, &U.cell_type, g.cell_type, n);
+  if (err == hipSuccess) err = upload(e, &U.topo, H.topo.data(), H.topo.size());
+  if (err == hipSuccess) err = upload(e, &U.cell_unit, H.cell_unit.data(), H.cell_unit.size());
+  if (err == hipSuccess) err = upload(e, &U.side, H.side.data(), H.side.size());
+  if (err == hipSuccess) err = upload(e, &U.slots, H.slots.data(), H.slots.size());
+  if (err == hipSuccess) err = upload(e, &U.off, H.off.data(), H.off.size());
+  if (err == hipSuccess) err = dev_alloc(&U.casm, 28 * (size_t)n);
+  if (err == hipSuccess) err = dev_alloc(&P.uniform_member, 1);
+  if (err == hipSuccess) err = dev_alloc(&P.cells, (int64_t)U.nunit * CELL_DOUBLES);
+  if (err == hipSuccess) err = dev_alloc(&e->counters, 4);
+  std::vector<uint64_t> mats;
+  jump_power_matrices(64, mats);
+  if (err == hipSuccess) err = upload(e, &e->jump_mats, mats.data(), mats.size());
+  for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+    err = hipEventCreate(&e->timers[k].start);
+    if (err == hipSuccess) err = hipEventCreate(&e->timers[k].stop);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(e, err, "create_usg");
+    rwpt_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return RWPT_OK;
+}
+
+int rwpt_set_flow_usg(rwpt_engine *e, const double *flowja, const double *bflow,
+                      const double *storage, const double *porosity,
+                      const double *retardation, const int32_t *ibound,
+                      const int32_t *icbound, int32_t default_icbound) {
+  if (!e) return RWPT_E_ARG;
+  if (!e->usg) return fail(e, RWPT_E_STATE, "not an unstructured grid: use rwpt_set_flow");
+  if (!flowja || !porosity) return fail(e, RWPT_E_ARG, "flowja and porosity are required");
+  UsgData &U = *e->usg;
+  DeviceProblem &P = e->prob;
+  const int64_t n = U.ncell;
+  int32_t uni = 1;
+  for (int64_t c = 0; c < n; ++c) {
+    if (!(porosity[c] > 0.0)) return fail(e, RWPT_E_ARG, "porosity must be > 0");
+    if (retardation && !(retardation[c] > 0.0))
+      return fail(e, RWPT_E_ARG, "retardation must be > 0");
+    if (porosity[c] != porosity[0]) uni = 0;
+  }
+  hipError_t err = hipMemcpyAsync(P.uniform_member, &uni, 4, hipMemcpyHostToDevice, e->stream);
+  if (err == hipSuccess) err = upload(e, &U.flowja, flowja, U.nja);
+  if (err == hipSuccess) err = upload(e, &P.bflow, bflow, 6 * n);
+  if (err == hipSuccess) err = upload(e, &U.storage, storage, n);
+  if (err == hipSuccess) err = upload(e, &P.porosity, porosity, n);
+  if (err == hipSuccess) err = upload(e, &P.retardation, retardation, n);
+  if (err == hipSuccess) err = upload(e, &P.ibound, ibound, n);
+  if (err == hipSuccess) err = upload(e, &P.icbound, icbound, n);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "set_flow_usg");
+  P.default_icbound = default_icbound;
+  P.uniform_porosity = uni;
+  e->have_flow = true;
+  e->tables_dirty = true;
+  return RWPT_OK;
+}
+
+int64_t rwpt_usg_unit_count(const rwpt_engine *e) {
+  return (e && e->usg) ? e->usg->nunit : -1;
+}
+
+int rwpt_get_usg_tables(rwpt_engine *e, double *out) {
+  if (!e || !out) return RWPT_E_ARG;
+  if (!e->usg) return fail(e, RWPT_E_STATE, "not an unstructured grid");
+  if (!e->have_flow || !e->have_disp)
+    return fail(e, RWPT_E_STATE, "flow and dispersion must be set");
+  double *d = nullptr;
+  const int64_t n = (int64_t)e->usg->nunit * USG_DEBUG_FIELDS;
+  hipError_t err = dev_alloc(&d, n);
+  if (err == hipSuccess) err = build_usg_tables(e, d);
+  if (err == hipSuccess) err = hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  dev_free(d);
+  if (err != hipSuccess) return hip_fail(e, err, "usg tables");
+  e->tables_dirty = false;
   return RWPT_OK;
 }
 

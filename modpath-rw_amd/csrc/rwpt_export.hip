@@ -164,8 +164,31 @@ struct HostGrid {
   int64_t ncm = 0;
   std::vector<double> xf, yf, delr, delc, top, bot;
   std::vector<int32_t> zone;
+  // unstructured grids: per-cell centres, sizes and layers
+  std::vector<double> ucx, ucy, udx, udy;
+  std::vector<int32_t> ulayer;
   bool load(rwpt_engine *e) {
     rwpt::DeviceProblem &P = e->prob;
+    if (e->usg) {
+      const rwpt::UsgData &U = *e->usg;
+      const size_t n = (size_t)U.ncell;
+      ncm = n;
+      ucx.resize(n); ucy.resize(n); udx.resize(n); udy.resize(n); ulayer.resize(n);
+      top.resize(n);
+      bot.resize(n);
+      hipError_t err = hipMemcpy(ucx.data(), U.cx, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(ucy.data(), U.cy, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(udx.data(), U.dx, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(udy.data(), U.dy, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(ulayer.data(), U.layer, n * 4, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(top.data(), P.top, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(bot.data(), P.bot, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess && P.zone) {
+        zone.resize(n);
+        err = hipMemcpy(zone.data(), P.zone, n * 4, hipMemcpyDeviceToHost);
+      }
+      return err == hipSuccess;
+    }
     ncol = P.ncol;
     nrow = P.nrow;
     ncm = P.ncell_member;
@@ -190,11 +213,21 @@ struct HostGrid {
   // ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549); c is 1-based
   void xyz(int32_t c, double x, double y, double z, double out[3]) const {
     const int64_t c0 = c - 1;
-    const int64_t lc = (int64_t)nrow * ncol, r = c0 % lc;
-    const int i = (int)(r / ncol), j = (int)(r % ncol);
-    const double cx = (xf[j] + xf[j + 1]) / 2.0, cy = (yf[i] + yf[i + 1]) / 2.0;
-    const double left = cx - (delr[j] / 2.0), right = cx + (delr[j] / 2.0);
-    const double front = cy - (delc[i] / 2.0), back = cy + (delc[i] / 2.0);
+    double left, right, front, back;
+    if (!ucx.empty()) {
+      left = ucx[c0] - (udx[c0] / 2.0);
+      right = ucx[c0] + (udx[c0] / 2.0);
+      front = ucy[c0] - (udy[c0] / 2.0);
+      back = ucy[c0] + (udy[c0] / 2.0);
+    } else {
+      const int64_t lc = (int64_t)nrow * ncol, r = c0 % lc;
+      const int i = (int)(r / ncol), j = (int)(r % ncol);
+      const double cx = (xf[j] + xf[j + 1]) / 2.0, cy = (yf[i] + yf[i + 1]) / 2.0;
+      left = cx - (delr[j] / 2.0);
+      right = cx + (delr[j] / 2.0);
+      front = cy - (delc[i] / 2.0);
+      back = cy + (delc[i] / 2.0);
+    }
     const double b = bot[c0], t = top[c0];
     out[0] = std::fmin(std::fmax(((1.0 - x) * left) + (x * right), left), right);
     out[1] = std::fmin(std::fmax(((1.0 - y) * front) + (y * back), front), back);
@@ -202,6 +235,7 @@ struct HostGrid {
   }
   int32_t local_cell(int32_t c) const { return (int32_t)((c - 1) % ncm + 1); }
   int32_t layer(int32_t c) const {
+    if (!ulayer.empty()) return ulayer[c - 1];
     return (int32_t)(((c - 1) % ncm) / ((int64_t)nrow * ncol) + 1);
   }
   int32_t zone_of(int32_t c) const { return zone.empty() ? 1 : zone[c - 1]; }

@@ -169,6 +169,49 @@ struct WalkParams {
   double t_engine, t_stop;
 };
 
+// Unstructured grids (rwpt_usg.hip).  A tracking unit is an unrefined
+// cell or one of the 2 x 2 sub-cells of a refined one; units are numbered
+// cell by cell (sub-cells row-major) and each has a CellRec.
+//  per cell, USG_TOPO int32: potential connection count per face, sub-face
+//    count per face (1 without connections), sub-face connections [6][4]
+//    and their FLOW-JA-FACE positions [6][4] (-1 none);
+//  per unit, USG_SIDE int32: cell (1-based), sub-row, sub-column, sub-cells
+//    of the cell, then per face: the sibling unit across an internal face,
+//    or for a face with a neighbour cell the face of that neighbour which
+//    meets this cell plus 8 when ConvertFromNeighbor's full-face copy
+//    applies (one connection on both sides), else 0;
+//  per unit, 4 doubles: the sub-cell's offsets x0 x1 y0 y1 in its cell;
+//  per unit, 18 x 2 int32: how each stencil entry is formed.
+enum : int {
+  UT_PCOUNT = 0, UT_SFC = 6, UT_CONN = 12, UT_QIDX = 36, USG_TOPO = 60,
+  US_CELL = 0, US_ROW = 1, US_COL = 2, US_NSUB = 3, US_FACE = 4, USG_SIDE = 10,
+  USG_DEBUG_FIELDS = 126,
+};
+
+struct UsgHost {
+  int32_t nunit = 0;
+  std::vector<int32_t> cell_unit, topo, side, slots;
+  std::vector<double> off;
+};
+
+struct UsgData {
+  int32_t ncell = 0, nunit = 0, nlay = 0;
+  int64_t nja = 0;
+  int32_t *topo = nullptr, *cell_unit = nullptr, *side = nullptr, *slots = nullptr;
+  int32_t *layer = nullptr, *cell_type = nullptr;
+  double *off = nullptr, *cx = nullptr, *cy = nullptr, *dx = nullptr, *dy = nullptr;
+  double *flowja = nullptr, *storage = nullptr, *casm = nullptr;
+  // particle state of the unstructured walk: the unit of a visit in
+  // progress, the cell-entry point (parent-cell coordinates and time) and
+  // the last pathline point before the entry (cell and coordinates)
+  int32_t *unit = nullptr, *pcell = nullptr;
+  double *ex = nullptr, *ey = nullptr, *ez = nullptr, *et = nullptr;
+  double *px = nullptr, *py = nullptr, *pz = nullptr;
+  int64_t pcap = 0;
+};
+
+std::string usg_build_topology(const rwpt_usg_grid &g, UsgHost &H);
+
 }  // namespace rwpt
 
 struct rwpt_engine {
@@ -177,6 +220,7 @@ struct rwpt_engine {
   rwpt_options opt{};
   rwpt::DeviceProblem prob;
   rwpt::Particles part;
+  rwpt::UsgData *usg = nullptr;  // unstructured grid (rwpt_create_usg), else null
   bool have_flow = false, have_disp = false, tables_dirty = true;
   bool tables32_dirty = true;  // FP32-local records behind the FP64 ones
   double t_engine = 0.0;
@@ -221,6 +265,10 @@ WalkParams walk_params(const rwpt_engine *e, double t_stop);
 hipError_t build_record_tangents(rwpt_engine *e, double *out);
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
                              double *gz);
+// unstructured grids: unit records (debug: USG_DEBUG_FIELDS per unit,
+// optional) and global coordinates
+hipError_t build_usg_tables(rwpt_engine *e, double *debug_out);
+hipError_t launch_usg_global_xyz(rwpt_engine *e, double *gx, double *gy, double *gz);
 hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
                       const double *mass_dev, double *density_dev,
                       double *hist_dev, double *stats_host);

@@ -45,17 +45,29 @@ struct KdeGeom {
   int ncol, nrow;
   const double *xface, *yface, *delr, *delc, *top, *bot;
   const double *lx, *ly, *lz;
+  // unstructured grids: per-cell centres and sizes (null on DIS grids)
+  const double *ucx, *ucy, *udx, *udy;
 };
 
 // ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549) of particle p in
 // 0-based cell c0, clamped to the cell.
 __device__ __forceinline__ void kde_point(const KdeGeom &Q, int c0, int64_t p,
                                           double g[3]) {
-  const int lc = Q.nrow * Q.ncol, r = c0 % lc, i = r / Q.ncol, j = r % Q.ncol;
-  const double cx = (Q.xface[j] + Q.xface[j + 1]) / 2.0;
-  const double cy = (Q.yface[i] + Q.yface[i + 1]) / 2.0;
-  const double left = cx - (Q.delr[j] / 2.0), right = cx + (Q.delr[j] / 2.0);
-  const double front = cy - (Q.delc[i] / 2.0), back = cy + (Q.delc[i] / 2.0);
+  double left, right, front, back;
+  if (Q.ucx) {
+    left = Q.ucx[c0] - (Q.udx[c0] / 2.0);
+    right = Q.ucx[c0] + (Q.udx[c0] / 2.0);
+    front = Q.ucy[c0] - (Q.udy[c0] / 2.0);
+    back = Q.ucy[c0] + (Q.udy[c0] / 2.0);
+  } else {
+    const int lc = Q.nrow * Q.ncol, r = c0 % lc, i = r / Q.ncol, j = r % Q.ncol;
+    const double cx = (Q.xface[j] + Q.xface[j + 1]) / 2.0;
+    const double cy = (Q.yface[i] + Q.yface[i + 1]) / 2.0;
+    left = cx - (Q.delr[j] / 2.0);
+    right = cx + (Q.delr[j] / 2.0);
+    front = cy - (Q.delc[i] / 2.0);
+    back = cy + (Q.delc[i] / 2.0);
+  }
   const double b = Q.bot[c0], t = Q.top[c0];
   const double x = Q.lx[p], y = Q.ly[p], z = Q.lz[p];
   g[0] = fmin(fmax(((1.0 - x) * left) + (x * right), left), right);
@@ -312,8 +324,10 @@ KdeGrid make_grid(const rwpt_engine *e, const rwpt_kde_options *opt, int &nd) {
 KdeGeom make_geom(const rwpt_engine *e) {
   const DeviceProblem &P = e->prob;
   const Particles &S = e->part;
-  return KdeGeom{P.ncol,  P.nrow, e->xface, e->yface, P.delr, P.delc,
-                 P.top,   P.bot,  S.lx,     S.ly,     S.lz};
+  const UsgData *U = e->usg;
+  return KdeGeom{P.ncol, P.nrow, e->xface, e->yface, P.delr, P.delc, P.top, P.bot,
+                 S.lx, S.ly, S.lz, U ? U->cx : nullptr, U ? U->cy : nullptr,
+                 U ? U->dx : nullptr, U ? U->dy : nullptr};
 }
 
 // The particle statistics of the reconstruction: hs = sums of x, y, z, w,

@@ -10,7 +10,7 @@
 
 #include <hipcub/hipcub.hpp>
 
-#include "rwpt_walk_kernels.h"
+#include "rwpt_walk_usg.h"
 
 namespace rwpt {
 namespace {
@@ -137,7 +137,9 @@ hipError_t ensure_sort_storage(rwpt_engine *e) {
   size_t bytes = 0;
   hipError_t err = hipcub::DeviceRadixSort::SortPairs(
       nullptr, bytes, P.keys[0], P.keys[1], P.vals[0], P.vals[1],
-      (int)std::max<int64_t>(P.cap, 1), 0, radix_bits(e->prob.ncell), e->stream);
+      (int)std::max<int64_t>(P.cap, 1), 0,
+      radix_bits(e->usg ? std::max<int64_t>(e->prob.ncell, e->usg->nunit) : e->prob.ncell),
+      e->stream);
   if (err != hipSuccess) return err;
   if (bytes > P.sort_tmp_bytes) {
     if (P.sort_tmp) (void)hipFree(P.sort_tmp);
@@ -148,6 +150,35 @@ hipError_t ensure_sort_storage(rwpt_engine *e) {
     P.sort_tmp_bytes = bytes;
   }
   return hipSuccess;
+}
+
+UsgWalk usg_walk_view(rwpt_engine *e) {
+  UsgData &U = *e->usg;
+  return UsgWalk{U.cell_unit, U.side, U.off, U.cx, U.cy, U.dx, U.dy, e->prob.ibound,
+                 U.unit, U.pcell, U.ex, U.ey, U.ez, U.et, U.px, U.py, U.pz};
+}
+
+// The unstructured walk for the engine's dimensions, model and advection.
+hipError_t walk_launch_usg(rwpt_engine *e, const WalkParams &W, dim3 grid, int block,
+                           const int32_t *work, int64_t cnt, int budget, int32_t *nk,
+                           int32_t *nv) {
+  const UsgWalk G = usg_walk_view(e);
+  const ParticlePtrs<double> S = particle_ptrs<double>(e->part);
+  const int mode = (W.ndim - 1) * 4 + W.disp_model * 2 + (W.advection_kind - 1);
+#define RWPT_WALK_USG(ND, MO, AD)                                                     \
+  case (ND - 1) * 4 + MO * 2 + (AD - 1):                                              \
+    hipLaunchKernelGGL((walk_kernel_usg<ND, MO, AD>), grid, dim3(block), 0, e->stream, \
+                       e->prob.cells, G, work, cnt, S, W, budget, nk, nv, e->counters); \
+    break;
+  switch (mode) {
+    RWPT_WALK_USG(1, 0, 1) RWPT_WALK_USG(1, 0, 2) RWPT_WALK_USG(1, 1, 1)
+    RWPT_WALK_USG(1, 1, 2) RWPT_WALK_USG(2, 0, 1) RWPT_WALK_USG(2, 0, 2)
+    RWPT_WALK_USG(2, 1, 1) RWPT_WALK_USG(2, 1, 2) RWPT_WALK_USG(3, 0, 1)
+    RWPT_WALK_USG(3, 0, 2) RWPT_WALK_USG(3, 1, 1) RWPT_WALK_USG(3, 1, 2)
+    default: return hipErrorInvalidValue;
+  }
+#undef RWPT_WALK_USG
+  return hipGetLastError();
 }
 
 hipError_t walk_launch64(rwpt_engine *e, const WalkParams &W, dim3 grid, int block,
@@ -197,7 +228,9 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   const int budget = std::max(1, e->tune.budget);
   const int block = e->tune.block;
   const int64_t max_launches = (kMaxSteps + budget - 1) / budget;
-  const int bits = radix_bits(e->prob.ncell);
+  const bool usg = e->usg != nullptr;
+  const int bits = radix_bits(usg ? std::max<int64_t>(e->prob.ncell, e->usg->nunit)
+                                  : e->prob.ncell);
   int64_t launches = 0;
   while (cnt > 0) {
     const int32_t *work = P.vals[0];
@@ -210,8 +243,15 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
       work = P.vals[1];
     }
     if (launches >= max_launches) {  // step guard reached
-      err = fp32 ? walk_terminate32(e, work, (int64_t)cnt)
-                 : launch_terminate_typed<double>(e, work, (int64_t)cnt);
+      if (usg) {
+        hipLaunchKernelGGL(terminate_walk_usg, dim3((unsigned)((cnt + 255) / 256)),
+                           dim3(256), 0, e->stream, usg_walk_view(e), work, (int64_t)cnt,
+                           particle_ptrs<double>(P));
+        err = hipGetLastError();
+      } else {
+        err = fp32 ? walk_terminate32(e, work, (int64_t)cnt)
+                   : launch_terminate_typed<double>(e, work, (int64_t)cnt);
+      }
       if (err != hipSuccess) return err;
       break;
     }
@@ -230,8 +270,9 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
     }
     (void)hipEventRecord(e->walk_events[2 * launches], e->stream);
     const dim3 grid((unsigned)((cnt + block - 1) / block));
-    err = fp32 ? walk_launch32(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv)
-               : walk_launch64(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv);
+    err = usg    ? walk_launch_usg(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv)
+          : fp32 ? walk_launch32(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv)
+                 : walk_launch64(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv);
     if (err != hipSuccess) return err;
     (void)hipEventRecord(e->walk_events[2 * launches + 1], e->stream);
     launches += 1;
@@ -250,6 +291,7 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
 
 hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
                              double *gz) {
+  if (e->usg) return launch_usg_global_xyz(e, gx, gy, gz);
   DeviceProblem &P = e->prob;
   const int block = 256;
   const int64_t n = e->part.n;

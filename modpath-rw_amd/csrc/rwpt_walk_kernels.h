@@ -935,7 +935,8 @@ template <typename R> struct ParticlePtrs {
 // Start of an advance: release pending particles (MPathRW.F90:1256-1302),
 // reset the per-call loop state and list the active particles.
 template <typename R>
-__global__ void prepare_advance(const R *__restrict__ cells, int64_t n,
+__global__ void prepare_advance(const R *__restrict__ cells,
+                                const int32_t *__restrict__ cell_unit, int64_t n,
                                 ParticlePtrs<R> S, const WalkParams W,
                                 int32_t *__restrict__ keys,
                                 int32_t *__restrict__ vals,
@@ -946,17 +947,17 @@ __global__ void prepare_advance(const R *__restrict__ cells, int64_t n,
   if (p < n) {
     int st = S.status[p];
     c = S.cell[p];
+    // the cell's record (its first unit's on an unstructured grid)
+    const R *rec = cells + (int64_t)(cell_unit ? cell_unit[c - 1] : c - 1) * Lay<R>::STRIDE;
     if (st == 0) {
       const double t = S.t[p];
       if (t < W.t_engine) st = 8;
       else if (t <= W.t_stop) {
         st = 1;
-        const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
         if (ints(rec)[Lay<R>::FLAGS] == 0) st = 7;
       }
     }
     if (st == 7) {
-      const R *rec = cells + (int64_t)(c - 1) * Lay<R>::STRIDE;
       if (ints(rec)[Lay<R>::FLAGS + 1] & CF_VERIFY_PDRY) st = 1;
     }
     S.status[p] = st;
@@ -1181,7 +1182,9 @@ hipError_t launch_prepare_typed(rwpt_engine *e, const WalkParams &W) {
   Particles &P = e->part;
   const int pb = 256;
   hipLaunchKernelGGL(prepare_advance<R>, dim3((unsigned)((P.n + pb - 1) / pb)), dim3(pb),
-                     0, e->stream, cell_table<R>(e), P.n, particle_ptrs<R>(P), W,
+                     0, e->stream, cell_table<R>(e),
+                     (const int32_t *)(e->usg ? e->usg->cell_unit : nullptr), P.n,
+                     particle_ptrs<R>(P), W,
                      P.keys[0], P.vals[0], e->counters);
   return hipGetLastError();
 }

@@ -12,7 +12,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(os.path.dirname(PKG), "librwpt.so")
 
 SOURCES = ["rwpt_api.hip", "rwpt_cells.hip", "rwpt_walk.hip", "rwpt_walk32.hip",
-           "rwpt_kde.hip", "rwpt_gpkde.hip", "rwpt_export.hip", "rwpt_adjoint.hip"]
+           "rwpt_kde.hip", "rwpt_gpkde.hip", "rwpt_export.hip", "rwpt_adjoint.hip",
+           "rwpt_usg.hip"]
 
 # The FP32-local walk has no bit-exactness contract: its translation unit
 # contracts a * b + c into fused multiply-add; every other one keeps the

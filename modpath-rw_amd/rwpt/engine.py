@@ -31,6 +31,13 @@ class RwptGrid(C.Structure):
                 ("delr", _dp), ("delc", _dp), ("top", _dp), ("bot", _dp)]
 
 
+class RwptUsgGrid(C.Structure):
+    _fields_ = [("ncell", C.c_int32), ("nlay", C.c_int32), ("nja", C.c_int64),
+                ("ia", _i32p), ("ja", _i32p), ("layer", _i32p), ("cell_x", _dp),
+                ("cell_y", _dp), ("del_x", _dp), ("del_y", _dp), ("top", _dp),
+                ("bot", _dp), ("cell_type", _i32p)]
+
+
 class RwptOptions(C.Structure):
     _fields_ = [("advection_kind", C.c_int32), ("time_step_kind", C.c_int32),
                 ("time_step_params", C.c_double * 2),
@@ -70,6 +77,12 @@ EXPORTS = {
                      C.POINTER(C.c_void_p)], C.c_int),
     "rwpt_create_ensemble": ([C.POINTER(RwptGrid), C.POINTER(RwptOptions), C.c_int32,
                               C.POINTER(C.c_void_p)], C.c_int),
+    "rwpt_create_usg": ([C.POINTER(RwptUsgGrid), C.POINTER(RwptOptions),
+                         C.POINTER(C.c_void_p)], C.c_int),
+    "rwpt_set_flow_usg": ([C.c_void_p, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p, C.c_int32],
+                          C.c_int),
+    "rwpt_usg_unit_count": ([C.c_void_p], C.c_int64),
+    "rwpt_get_usg_tables": ([C.c_void_p, _dp], C.c_int),
     "rwpt_destroy": ([C.c_void_p], C.c_int),
     "rwpt_last_error": ([C.c_void_p], C.c_char_p),
     "rwpt_set_flow": ([C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p,
@@ -204,18 +217,31 @@ class Engine:
     """One particle-tracking engine resident on the current HIP device."""
 
     def __init__(self, problem, precision: int = 0, **opt):
-        """problem: a Problem, or an ensemble from problems.stack (its
-        members share the first member's grid)."""
+        """problem: a Problem, an ensemble from problems.stack (its members
+        share the first member's grid), or a UsgProblem (an unstructured
+        grid of rectangular cells)."""
         P = problem
         self.problem = P
         self.members = getattr(P, "members", 1)
+        self.unstructured = hasattr(P, "ja")
         self._keep = []
-        g = RwptGrid()
-        g.ncol, g.nrow, g.nlay = P.ncol, P.nrow, P.nlay
-        ncm = P.ncol * P.nrow * P.nlay
-        arrs = [_d(P.delr), _d(P.delc), _d(P.top)[:ncm].copy(), _d(P.bot)[:ncm].copy()]
-        self._keep += arrs
-        g.delr, g.delc, g.top, g.bot = [_p(a, _dp) for a in arrs]
+        if self.unstructured:
+            g = RwptUsgGrid()
+            g.ncell, g.nlay, g.nja = int(P.ncell), int(P.nlay), int(np.asarray(P.ja).size)
+            ints = [_i(P.ia), _i(P.ja), _i(P.layer), _i(P.layer_type)]
+            dbls = [_d(P.cell_x), _d(P.cell_y), _d(P.del_x), _d(P.del_y), _d(P.top),
+                    _d(P.bot)]
+            self._keep += ints + dbls
+            g.ia, g.ja, g.layer, g.cell_type = [_p(a, _i32p) for a in ints]
+            g.cell_x, g.cell_y, g.del_x, g.del_y, g.top, g.bot = [_p(a, _dp) for a in dbls]
+            ncm = int(P.ncell)
+        else:
+            g = RwptGrid()
+            g.ncol, g.nrow, g.nlay = P.ncol, P.nrow, P.nlay
+            ncm = P.ncol * P.nrow * P.nlay
+            arrs = [_d(P.delr), _d(P.delc), _d(P.top)[:ncm].copy(), _d(P.bot)[:ncm].copy()]
+            self._keep += arrs
+            g.delr, g.delc, g.top, g.bot = [_p(a, _dp) for a in arrs]
         stops = {k: int(getattr(P, a, d)) for k, a, d in (
             ("stop_zone", "stop_zone", 0), ("stop_at_weak_sinks", "stop_weak_sinks", 0),
             ("stop_at_weak_sources", "stop_weak_sources", 0),
@@ -228,16 +254,24 @@ class Engine:
                             backward_tracking=P.backward, precision=precision,
                             **stops)
         h = C.c_void_p()
-        rc = lib().rwpt_create_ensemble(C.byref(g), C.byref(o), int(self.members),
-                                        C.byref(h))
+        if self.unstructured:
+            rc = lib().rwpt_create_usg(C.byref(g), C.byref(o), C.byref(h))
+            what = "rwpt_create_usg"
+        else:
+            rc = lib().rwpt_create_ensemble(C.byref(g), C.byref(o), int(self.members),
+                                            C.byref(h))
+            what = "rwpt_create_ensemble"
         if rc != 0:
-            raise RwptError(f"rwpt_create_ensemble failed ({rc})")
+            raise RwptError(f"{what} failed ({rc})")
         self.h = h
         self.n = 0
         self.ncell_total = ncm * self.members
         self.set_flow(P)
         self.set_dispersion(P)
-        if getattr(P, "has_extras", False):
+        if self.unstructured:
+            if any(a is not None for a in (P.head, P.zone, P.source, P.sink)):
+                self.set_cell_extras(P.head, None, P.zone, P.source, P.sink)
+        elif getattr(P, "has_extras", False):
             self.set_cell_extras(P.head, P.layer_type, P.zone, P.source_flow, P.sink_flow)
 
     # -- error handling -------------------------------------------------------
@@ -259,6 +293,14 @@ class Engine:
 
     # -- inputs ---------------------------------------------------------------
     def set_flow(self, P):
+        if self.unstructured:
+            a = [_d(P.flowja), _d(P.bflow), _d(P.storage), _d(P.porosity),
+                 _d(P.retardation)]
+            ib, ic = _i(P.ibound), _i(P.icbound)
+            rc = lib().rwpt_set_flow_usg(self.h, *[_p(x, _dp) for x in a], _p(ib, _i32p),
+                                         _p(ic, _i32p), int(P.default_icbound))
+            self._check(rc, "rwpt_set_flow_usg")
+            return
         a = [_d(P.frf), _d(P.fff), _d(P.flf), _d(P.bflow), _d(P.porosity),
              _d(P.retardation)]
         ib, ic = _i(P.ibound), _i(P.icbound)
@@ -281,6 +323,18 @@ class Engine:
                                         _p(a[2], _i32p), _p(a[3], _dp), _p(a[4], _dp))
         self._check(rc, "rwpt_set_cell_extras")
         self._keep_extras = a
+
+    def unit_count(self) -> int:
+        """Tracking units of an unstructured engine (sub-cells of refined
+        cells count one each)."""
+        return int(lib().rwpt_usg_unit_count(self.h))
+
+    def usg_tables(self) -> np.ndarray:
+        """rwpt_get_usg_tables: (units, 126) checking copy of the unit
+        records."""
+        out = np.zeros((self.unit_count(), 126))
+        self._check(lib().rwpt_get_usg_tables(self.h, _p(out, _dp)), "rwpt_get_usg_tables")
+        return out
 
     def load_particles(self, cell, lx, ly, lz, t=None, status=None, rng=None,
                        stream_offset=0):
