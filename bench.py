@@ -156,7 +156,8 @@ def adaptive_kde(E, origin, nbins, loops):
             for L in (0, 1):
                 t = time.perf_counter()
                 refrun.gpkde(gx[act], gy[act], gz[act], origin, (1.0, 1.0, 1.0), nbins,
-                             n_opt_loops=L, unit_volume=False, skip_error=True)
+                             n_opt_loops=L, unit_volume=False, skip_error=True,
+                             threads=os.cpu_count() or 1)
                 ts.append(time.perf_counter() - t)
             out["cpu_reference"] = {
                 "ms_per_loop": (ts[1] - ts[0]) * 1e3, "ms_without_loops": ts[0] * 1e3,

@@ -19,6 +19,8 @@ module RwptGpuModule
   public :: rwpt_write_timeseries, rwpt_write_endpoints, rwpt_adjoint
   public :: rwpt_write_observation, rwpt_adjoint_options, rwpt_adjoint_stats
   public :: rwpt_adjoint_opts
+  public :: rwpt_usg_grid, rwpt_create_usg, rwpt_set_flow_usg, rwpt_usg_unit_count
+  public :: rwpt_get_usg_tables
 
   type, bind(C) :: rwpt_adjoint_opts
     real(c_double)     :: graze_cut
@@ -27,6 +29,13 @@ module RwptGpuModule
   type, bind(C) :: rwpt_grid
     integer(c_int32_t) :: ncol, nrow, nlay
     type(c_ptr) :: delr, delc, top, bot          ! c_loc of real(c_double) arrays
+  end type
+  type, bind(C) :: rwpt_usg_grid
+    integer(c_int32_t) :: ncell, nlay
+    integer(c_int64_t) :: nja
+    type(c_ptr) :: ia, ja, layer                 ! c_loc of integer(c_int32_t) arrays
+    type(c_ptr) :: cell_x, cell_y, del_x, del_y, top, bot  ! real(c_double) arrays
+    type(c_ptr) :: cell_type                     ! integer(c_int32_t) array or c_null_ptr
   end type
   type, bind(C) :: rwpt_options
     integer(c_int32_t) :: advection_kind, time_step_kind
@@ -61,6 +70,30 @@ module RwptGpuModule
       type(rwpt_grid), intent(in) :: grid
       type(rwpt_options), intent(in) :: opt
       type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function rwpt_create_usg(grid, opt, e) bind(C, name='rwpt_create_usg')
+      import :: c_int, c_ptr, rwpt_usg_grid, rwpt_options
+      type(rwpt_usg_grid), intent(in) :: grid
+      type(rwpt_options), intent(in) :: opt
+      type(c_ptr), intent(out) :: e
+    end function
+    ! bflow, storage, retardation, ibound and icbound may be c_null_ptr
+    integer(c_int) function rwpt_set_flow_usg(e, flowja, bflow, storage, porosity, &
+        retardation, ibound, icbound, default_icbound) bind(C, name='rwpt_set_flow_usg')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: e
+      real(c_double), intent(in) :: flowja(*), porosity(*)
+      type(c_ptr), value :: bflow, storage, retardation, ibound, icbound
+      integer(c_int32_t), value :: default_icbound
+    end function
+    integer(c_int64_t) function rwpt_usg_unit_count(e) bind(C, name='rwpt_usg_unit_count')
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function rwpt_get_usg_tables(e, out) bind(C, name='rwpt_get_usg_tables')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function rwpt_destroy(e) bind(C, name='rwpt_destroy')
       import :: c_int, c_ptr

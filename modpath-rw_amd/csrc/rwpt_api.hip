@@ -750,7 +750,8 @@ int rwpt_create_usg(const rwpt_usg_grid *grid, const rwpt_options *opt,
   if (g.ia[0] != 1 || g.ia[g.ncell] != g.nja + 1) return RWPT_E_ARG;
   for (int c = 1; c <= g.ncell; ++c) {
     const int32_t a = g.ia[c - 1], b = g.ia[c];
-    if (b <= a || g.ja[a - 1] != c) return RWPT_E_ARG;
+    // a removed cell (IDOMAIN < 1) has an empty row
+    if (b < a || (b > a && g.ja[a - 1] != c)) return RWPT_E_ARG;
     if (g.layer[c - 1] < 1 || g.layer[c - 1] > g.nlay) return RWPT_E_ARG;
     if (!(g.del_x[c - 1] > 0.0) || !(g.del_y[c - 1] > 0.0)) return RWPT_E_ARG;
     for (int32_t q = a; q < b - 1; ++q)

@@ -268,10 +268,7 @@ class Engine:
         self.ncell_total = ncm * self.members
         self.set_flow(P)
         self.set_dispersion(P)
-        if self.unstructured:
-            if any(a is not None for a in (P.head, P.zone, P.source, P.sink)):
-                self.set_cell_extras(P.head, None, P.zone, P.source, P.sink)
-        elif getattr(P, "has_extras", False):
+        if not self.unstructured and getattr(P, "has_extras", False):
             self.set_cell_extras(P.head, P.layer_type, P.zone, P.source_flow, P.sink_flow)
 
     # -- error handling -------------------------------------------------------
@@ -300,6 +297,8 @@ class Engine:
             rc = lib().rwpt_set_flow_usg(self.h, *[_p(x, _dp) for x in a], _p(ib, _i32p),
                                          _p(ic, _i32p), int(P.default_icbound))
             self._check(rc, "rwpt_set_flow_usg")
+            # the problem brings its heads, zones and source and sink flows
+            self.set_cell_extras(P.head, None, P.zone, P.source, P.sink)
             return
         a = [_d(P.frf), _d(P.fff), _d(P.flf), _d(P.bflow), _d(P.porosity),
              _d(P.retardation)]

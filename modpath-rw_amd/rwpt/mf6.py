@@ -127,16 +127,176 @@ class Grb:
     icelltype: np.ndarray
 
 
-def read_grb(path) -> Grb:
-    """The binary grid file of a DIS model (ReadGRB)."""
+@dataclasses.dataclass
+class GrbDisv:
+    """A DISV binary grid of rectangular cells as RectangularGridDisvMf6
+    holds it after ReadGRB and CheckRectangular: per cell (all layers) the
+    centre and sizes from the vertices, tops, bottoms, IA (0-based offsets)
+    and JA, IDOMAIN and ICELLTYPE."""
+    ncells: int
+    nlay: int
+    ncpl: int
+    nja: int
+    xorigin: float
+    yorigin: float
+    angrot: float
+    top: np.ndarray
+    botm: np.ndarray
+    ia: np.ndarray
+    ja: np.ndarray
+    idomain: np.ndarray
+    icelltype: np.ndarray
+    cell_x: np.ndarray
+    cell_y: np.ndarray
+    del_x: np.ndarray
+    del_y: np.ndarray
+
+    @property
+    def layer(self) -> np.ndarray:
+        """1-based layer of every cell (GetLayer over LayerOffsets)."""
+        return (np.arange(self.ncells) // self.ncpl + 1).astype(np.int32)
+
+
+def _ncpl(grb) -> int:
+    return grb.ncpl if isinstance(grb, GrbDisv) else grb.nrow * grb.ncol
+
+
+def read_grb(path):
+    """The binary grid file of a DIS model (ReadGRB,
+    RectangularGridDisMf6.f90:122-330) as a Grb, or of a DISV model of
+    rectangular cells (RectangularGridDisvMf6.f90:47-274) as a GrbDisv."""
     with _Stream(path) as s:
+        words = s.text(50).split()
+        if words[:2] == ["GRID", "DISV"]:
+            return _read_grb_disv(s)
+        if words[:2] != ["GRID", "DIS"]:
+            raise Mf6Error("not a DIS or DISV binary grid file")
         return _read_grb(s)
 
 
-def _read_grb(s) -> Grb:
+def _colinear(x1, y1, x2, y2, x3, y3, tol):
+    # IsColinear (RectangularGridDisvMf6.f90:309-323)
+    d1 = np.sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1))
+    d2 = np.sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2))
+    d3 = np.sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1))
+    return abs((d3 - d1 - d2) / d3) <= tol
+
+
+# the four clockwise side directions of a rectangle starting at each corner
+# (CheckRectangular's ix, iy tables)
+_SIDES = [((1, 0), (0, -1), (-1, 0), (0, 1)), ((0, -1), (-1, 0), (0, 1), (1, 0)),
+          ((-1, 0), (0, 1), (1, 0), (0, -1)), ((0, 1), (1, 0), (0, -1), (-1, 0))]
+
+
+def _check_rectangular(ncpl, iavert, javert, vx, vy):
+    """CheckRectangular (RectangularGridDisvMf6.f90:326-570): each cell's
+    corners (vertices where the outline turns), at most one mid-face vertex
+    per side, four axis-aligned sides; the corners reordered from the
+    upper-left one; DelX, DelY, CellX, CellY of layer 1 from them."""
+    tol = float(np.float32(0.00001))
+    smoothed = True
+    dx, dy, cx, cy = (np.zeros(ncpl) for _ in range(4))
+    for n in range(ncpl):
+        off, cnt = iavert[n], iavert[n + 1] - iavert[n]
+        corners = [javert[off]]
+        pts, mid = 1, 0
+        for i in range(1, cnt - 1):
+            v1, v2, v3 = javert[off + i - 1], javert[off + i], javert[off + i + 1]
+            if not _colinear(vx[v1 - 1], vy[v1 - 1], vx[v2 - 1], vy[v2 - 1], vx[v3 - 1],
+                             vy[v3 - 1], tol):
+                pts += 1
+                if pts <= 4:
+                    corners.append(v2)
+                if mid > 1:
+                    smoothed = False
+                mid = 0
+            else:
+                mid += 1
+        pts += 1
+        if pts != 5:
+            raise Mf6Error("a DISV cell is not a quadrilateral")
+        dirs = []
+        for a in range(4):
+            p, q = corners[a] - 1, corners[(a + 1) % 4] - 1
+            ddx, ddy = vx[q] - vx[p], vy[q] - vy[p]
+            dirs.append((1 if ddx > tol else (-1 if ddx < -tol else 0),
+                         1 if ddy > tol else (-1 if ddy < -tol else 0)))
+        k = next((m for m in range(4) if tuple(dirs) == _SIDES[m]), -1)
+        if k < 0:
+            raise Mf6Error("a DISV cell is not a rectangle aligned with the axes")
+        corners = corners[k:] + corners[:k]
+        x1, x2 = vx[corners[0] - 1], vx[corners[1] - 1]
+        y1, y2 = vy[corners[3] - 1], vy[corners[0] - 1]
+        dx[n] = x2 - x1
+        cx[n] = (x1 + x2) / 2.0
+        dy[n] = y2 - y1
+        cy[n] = (y1 + y2) / 2.0
+    if not smoothed:
+        raise Mf6Error("a DISV cell side meets more than two cells")
+    return dx, dy, cx, cy
+
+
+def _read_grb_disv(s) -> GrbDisv:
     words = s.text(50).split()
-    if words[:2] != ["GRID", "DIS"]:
-        raise Mf6Error("not a DIS binary grid file")
+    if words[:1] != ["VERSION"] or int(words[1]) != 1:
+        raise Mf6Error("unsupported binary grid file version")
+    words = s.text(50).split()
+    if words[:1] != ["NTXT"]:
+        raise Mf6Error("binary grid file lacks NTXT")
+    ntxt = int(words[1])
+    words = s.text(50).split()
+    if words[:1] != ["LENTXT"] or int(words[1]) != 100:
+        raise Mf6Error("binary grid file LENTXT must be 100")
+    names = []
+    for _ in range(ntxt):
+        line = s.text(100)
+        names.append("#" if line.startswith("#") else line.split()[0])
+    v: Dict[str, object] = {}
+    for name in names:
+        if name == "#":
+            continue
+        if name in ("NCELLS", "NLAY", "NCPL", "NVERT", "NJAVERT", "NJA"):
+            v[name] = s.i4()
+        elif name in ("XORIGIN", "YORIGIN", "ANGROT"):
+            v[name] = s.f8()
+        elif name in ("TOP", "CELLX", "CELLY"):
+            v[name] = s.f8(v["NCPL"])
+        elif name == "BOTM":
+            v[name] = s.f8(v["NCELLS"])
+        elif name == "VERTICES":
+            v[name] = s.f8(2 * v["NVERT"]).reshape(-1, 2)
+        elif name == "IAVERT":
+            v[name] = s.i4(v["NCPL"] + 1) - 1
+        elif name == "JAVERT":
+            v[name] = s.i4(v["NJAVERT"])
+        elif name == "IA":
+            v[name] = s.i4(v["NCELLS"] + 1) - 1
+        elif name == "JA":
+            v[name] = s.i4(v["NJA"])
+        elif name in ("IDOMAIN", "ICELLTYPE"):
+            v[name] = s.i4(v["NCELLS"])
+        else:
+            raise Mf6Error(f"unknown binary grid variable {name}")
+    need = ("NCELLS", "NLAY", "NCPL", "NVERT", "NJAVERT", "NJA", "TOP", "BOTM",
+            "VERTICES", "IAVERT", "JAVERT", "IA", "JA", "IDOMAIN", "ICELLTYPE")
+    if any(k not in v for k in need):
+        raise Mf6Error("binary grid file lacks a required variable")
+    n, ncpl, nlay = v["NCELLS"], v["NCPL"], v["NLAY"]
+    if n != nlay * ncpl:
+        raise Mf6Error("NCELLS differs from NLAY * NCPL")
+    verts = v["VERTICES"]
+    dx, dy, cx, cy = _check_rectangular(ncpl, v["IAVERT"], v["JAVERT"], verts[:, 0],
+                                        verts[:, 1])
+    top = np.empty(n)
+    top[:ncpl] = v["TOP"]
+    top[ncpl:] = v["BOTM"][:n - ncpl]
+    rep = lambda a: np.tile(a, nlay)
+    return GrbDisv(n, nlay, ncpl, v["NJA"], v.get("XORIGIN", 0.0), v.get("YORIGIN", 0.0),
+                   v.get("ANGROT", 0.0), top, v["BOTM"], v["IA"], v["JA"], v["IDOMAIN"],
+                   v["ICELLTYPE"], rep(cx), rep(cy), rep(dx), rep(dy))
+
+
+def _read_grb(s) -> Grb:
     words = s.text(50).split()
     if words[:1] != ["VERSION"] or int(words[1]) != 1:
         raise Mf6Error("unsupported binary grid file version")
@@ -328,10 +488,11 @@ def _dry_tolerance(hdry):
     return np.float32(abs(np.finfo(np.float32).eps * np.float32(hdry)))
 
 
-def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetRecord],
+def load_time_step(grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetRecord],
                    kper: int, kstp: int, ibound: np.ndarray, hdry: float,
                    default_iface: Dict[str, int] = None) -> TimeStepData:
-    """FlowModelData%LoadTimeStep(kper, kstp) for a MODFLOW 6 DIS grid;
+    """FlowModelData%LoadTimeStep(kper, kstp) for a MODFLOW 6 DIS or DISV
+    grid (a DISV head record holds one layer as NCPL columns of one row);
     ibound None takes MODPATH's IBOUND from IDOMAIN (ibound_from_idomain)."""
     n = grb.ncells
     default_iface = {k.strip(): int(v) for k, v in (default_iface or {}).items()}
@@ -339,7 +500,7 @@ def load_time_step(grb: Grb, heads: Sequence[HeadRecord], budget: Sequence[Budge
     if not recs:
         raise Mf6Error(f"no budget records for period {kper} step {kstp}")
     h = np.zeros(n)
-    ncpl = grb.nrow * grb.ncol
+    ncpl = _ncpl(grb)
     for r in heads:
         if r.kper == kper and r.kstp == kstp:
             if r.ncol * r.nrow != ncpl or not 1 <= r.ilay <= grb.nlay:
@@ -511,6 +672,29 @@ def problem(grb: Grb, ts: TimeStepData, porosity, retardation=None, **options):
     return P, extras
 
 
+def usg_problem(grb: GrbDisv, ts: TimeStepData, porosity, retardation=None, **options):
+    """The engine's UsgProblem for one read time step of a DISV model: the
+    grid of ReadGRB/CheckRectangular, FlowModelData's FLOW-JA-FACE,
+    boundary, storage, source and sink flows, IBoundTS and heads, ICELLTYPE
+    as the layer type (pr_FillTransportCellBufferUnstructured,
+    ParticleTrackingEngine.f90:711-748).  options: dispersion, advection,
+    time-step and tracking fields (aL, aT, dmeff, ts_params ...)."""
+    from .problems import UsgProblem
+    n = grb.ncells
+    return UsgProblem(
+        ncell=n, nlay=grb.nlay, ia=(grb.ia + 1).astype(np.int32),
+        ja=grb.ja.astype(np.int32), layer=grb.layer, cell_x=grb.cell_x.copy(),
+        cell_y=grb.cell_y.copy(), del_x=grb.del_x.copy(), del_y=grb.del_y.copy(),
+        top=grb.top.copy(), bot=grb.botm.copy(), flowja=ts.flows_ja.copy(),
+        porosity=np.broadcast_to(np.asarray(porosity, np.float64), (n,)).copy(),
+        bflow=ts.boundary_flows.copy(), storage=ts.storage_flows.copy(),
+        source=ts.source_flows.copy(), sink=ts.sink_flows.copy(),
+        retardation=None if retardation is None else
+        np.broadcast_to(np.asarray(retardation, np.float64), (n,)).copy(),
+        head=ts.heads.copy(), layer_type=grb.icelltype.astype(np.int32),
+        ibound=ts.ibound_ts.copy(), steady_state=int(bool(ts.steady_state)), **options)
+
+
 class FlowSeries:
     """The flow of successive MODFLOW time steps for Driver(flow_for_step=):
     steps lists (kper, kstp) of the cumulative time steps 1, 2, ...; each
@@ -532,5 +716,9 @@ class FlowSeries:
         ts = load_time_step(self.grb, read_heads(self.heads_path, kper, kstp),
                             read_budget(self.budget_path, kper, kstp), kper, kstp,
                             self.ibound, self.hdry, self.default_iface)
+        if isinstance(self.grb, GrbDisv):
+            # an unstructured problem carries its cell extras
+            return usg_problem(self.grb, ts, self.porosity, self.retardation,
+                               **self.options), None
         return problem(self.grb, ts, self.porosity, self.retardation, **self.options)
 

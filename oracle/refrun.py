@@ -110,7 +110,7 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
           database=False, isotropic=False, unit_volume=True, scaling_factor=1.0,
           smoothing_selection=0, smoothing_factor=1.0, skip_error=False,
           export_vars=False, bound=0, h_over_delta=(0.0, 0.0, 0.0),
-          weight_format=0, export=None, timeout=600):
+          weight_format=0, export=None, timeout=600, threads=1):
     """Runs the reference's GridProjectedKDE on the points; returns
     (density, histogram counts, initial smoothing) with x fastest, and with
     export_vars also the per-loop optimisation variables (a list of arrays,
@@ -120,7 +120,9 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
     delta, max) h/delta passed to Initialize when min > 0; weight_format:
     effectiveWeightFormat; export = (format, column format, time id, time,
     group id) makes the reference write its density rows
-    (ExportDensityUnit[Binary]) and appends their bytes to the result."""
+    (ExportDensityUnit[Binary]) and appends their bytes to the result.
+    threads: OpenMP threads of the reference's loops (1, the default, gives
+    its sequential results; timing runs take all host cores)."""
     n = int(np.asarray(x).size)
     ex = export if export is not None else (-1, 0, 0, 0.0, 0)
     hdr = np.array([n, *nbins, n_opt_loops, int(database), int(isotropic),
@@ -139,8 +141,9 @@ def gpkde(x, y, z, origin, bin_size, nbins, weights=None, n_opt_loops=0,
         with open(fin, "wb") as fh:
             for a in parts:
                 fh.write(np.ascontiguousarray(a).tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS=str(max(int(threads), 1)))
         subprocess.run([GPKDE_BINARY, fin, fout], check=True, timeout=timeout,
-                       stdout=subprocess.DEVNULL, cwd=d)
+                       stdout=subprocess.DEVNULL, cwd=d, env=env)
         raw = np.fromfile(fout, f8)
         loops = []
         if export_vars:
@@ -220,7 +223,7 @@ def mf6_available() -> bool:
 
 
 def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e30,
-        hdry=-1e30, default_iface=(), timeout=120):
+        hdry=-1e30, default_iface=(), timeout=120, disv=False):
     """Reads MODFLOW 6 DIS grid, head and budget files with the reference's
     readers and FlowModelData%LoadTimeStep (mf6_ref), and assembles every
     cell as ModpathCellData%SetMassTransportDataUnstructured does.  Returns
@@ -242,7 +245,8 @@ def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e
             for lab in labels:
                 fh.write(lab.encode().ljust(16)[:16])
             fh.write(values.tobytes())
-        subprocess.run([MF6_BINARY, str(grb), str(hds), str(cbc), fin, fout], check=True,
+        subprocess.run([MF6_BINARY, str(grb), str(hds), str(cbc), fin, fout] +
+                       (["disv"] if disv else []), check=True,
                        timeout=timeout, stdout=subprocess.DEVNULL, cwd=d)
         b = open(fout, "rb").read()
     ncell, nja, steady = np.frombuffer(b[:12], np.int32)
@@ -267,6 +271,11 @@ def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e
     out["cell_sink"] = cells[:, 7].copy()
     out["velocity"] = cells[:, 8:14].copy()
     out["size"] = cells[:, 14:17].copy()
+    p += 8 * 17 * n
+    if disv:
+        for k in ("cell_x", "cell_y", "del_x", "del_y"):
+            out[k] = np.frombuffer(b[p:p + 8 * n], np.float64).copy()
+            p += 8 * n
     return out
 
 

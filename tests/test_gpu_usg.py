@@ -166,3 +166,25 @@ def test_usg_plain_grid_equals_structured_engine():
             E.close()
     for f in FIELDS:
         assert _same(res[0][f], res[1][f]).all(), f
+
+
+def test_usg_from_mf6_disv_files(tmp_path):
+    """A DISV model read from its binary grid, head and budget files
+    (rwpt.mf6, pinned against the reference's readers by test_mf6_disv.py):
+    the engine's walk over mf6.usg_problem equals the oracle's."""
+    from rwpt import mf6
+    from test_mf6_disv import HDRY, IFACE, scenario
+    Q, paths = scenario(str(tmp_path))
+    g = mf6.read_grb(paths[0])
+    ts = mf6.load_time_step(g, mf6.read_heads(paths[1]), mf6.read_budget(paths[2]), 1, 1,
+                            np.ones(g.ncells, np.int32), HDRY, dict(IFACE))
+    U = mf6.usg_problem(g, ts, 0.3, aL=np.array([0.3]), aT=np.array([0.03]),
+                        ts_params=(0.1, 0.0))
+    out = _run_both(U, 600, t_out=(0.5, 1.0, 2.0), seed=4)
+    for k, (gp, st, act, ns, ns_orc) in enumerate(out):
+        assert ns == ns_orc
+        for f in FIELDS:
+            ok = _same(gp[f], st[f])
+            assert ok.all(), (k, f, np.nonzero(~ok)[0][:8])
+    # dry cells (IBoundTS 0) stop the particles released in them
+    assert (out[-1][1]["status"] == 7).any()
