@@ -221,6 +221,15 @@ int rwpt_set_cell_extras(rwpt_engine *e, const double *head,
                          const int32_t *layer_type, const int32_t *zone,
                          const double *source_flow, const double *sink_flow);
 
+/* Grid%SaturatedTop of the loaded time step (FlowModelData.f90:255-300:
+ * the head in a wet convertible cell whose head lies between its bottom
+ * and top, the bottom of a dry MODFLOW 6 cell whose head is below it, the
+ * top otherwise), ncell values, or NULL for the cell tops.  It is the top
+ * of ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549): the global z
+ * of rwpt_global_xyz, the KDE's coordinates and the written records.  The
+ * tracking itself uses the heads of rwpt_set_cell_extras. */
+int rwpt_set_saturated_top(rwpt_engine *e, const double *saturated_top);
+
 /* Dispersion (DSP file, TransportModelData.f90:583-1040): model 0 linear
  * isotropic, 1 axisymmetric.  uniform=1 => each array holds one value,
  * uniform=2 => one value per ensemble member, uniform=0 => one value per

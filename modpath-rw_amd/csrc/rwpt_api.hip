@@ -244,6 +244,7 @@ int rwpt_destroy(rwpt_engine *e) {
   dev_free(P.porosity); dev_free(P.retardation); dev_free(P.ibound);
   dev_free(P.icbound); dev_free(P.head); dev_free(P.layer_type);
   dev_free(P.zone); dev_free(P.source); dev_free(P.sink); dev_free(P.disp);
+  dev_free(P.sat_top);
   dev_free(P.q6); dev_free(P.dzrw); dev_free(P.cells); dev_free(P.cells32);
   dev_free(P.uniform_member);
   free_particles(e->part);
@@ -332,6 +333,15 @@ int rwpt_set_cell_extras(rwpt_engine *e, const double *head,
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return hip_fail(e, err, "set_cell_extras");
   e->tables_dirty = true;
+  return RWPT_OK;
+}
+
+int rwpt_set_saturated_top(rwpt_engine *e, const double *saturated_top) {
+  if (!e) return RWPT_E_ARG;
+  DeviceProblem &P = e->prob;
+  hipError_t err = upload(e, &P.sat_top, saturated_top, P.ncell);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return hip_fail(e, err, "set_saturated_top");
   return RWPT_OK;
 }
 

@@ -181,7 +181,7 @@ struct HostGrid {
       if (err == hipSuccess) err = hipMemcpy(udx.data(), U.dx, n * 8, hipMemcpyDeviceToHost);
       if (err == hipSuccess) err = hipMemcpy(udy.data(), U.dy, n * 8, hipMemcpyDeviceToHost);
       if (err == hipSuccess) err = hipMemcpy(ulayer.data(), U.layer, n * 4, hipMemcpyDeviceToHost);
-      if (err == hipSuccess) err = hipMemcpy(top.data(), P.top, n * 8, hipMemcpyDeviceToHost);
+      if (err == hipSuccess) err = hipMemcpy(top.data(), model_top(P), n * 8, hipMemcpyDeviceToHost);
       if (err == hipSuccess) err = hipMemcpy(bot.data(), P.bot, n * 8, hipMemcpyDeviceToHost);
       if (err == hipSuccess && P.zone) {
         zone.resize(n);
@@ -202,7 +202,7 @@ struct HostGrid {
     if (err == hipSuccess) err = hipMemcpy(yf.data(), e->yface, yf.size() * 8, hipMemcpyDeviceToHost);
     if (err == hipSuccess) err = hipMemcpy(delr.data(), P.delr, ncol * 8, hipMemcpyDeviceToHost);
     if (err == hipSuccess) err = hipMemcpy(delc.data(), P.delc, nrow * 8, hipMemcpyDeviceToHost);
-    if (err == hipSuccess) err = hipMemcpy(top.data(), P.top, P.ncell * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(top.data(), model_top(P), P.ncell * 8, hipMemcpyDeviceToHost);
     if (err == hipSuccess) err = hipMemcpy(bot.data(), P.bot, P.ncell * 8, hipMemcpyDeviceToHost);
     if (err == hipSuccess && P.zone) {
       zone.resize(P.ncell);
@@ -479,14 +479,27 @@ extern "C" int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t 
   std::vector<int32_t> obs_index(P.ncell, -1);
   std::vector<double> rfac(n_cells), wvol(n_cells);
   for (int32_t k = 0; k < n_cells; ++k) {
+    // the record of the cell, or on an unstructured grid of its first unit,
+    // whose porosity, retardation, thickness and dry flag are the cell's;
+    // DX and DY are then the whole cell's (CellData of TrackCell)
+    int64_t rec_index = cells[k] - 1;
+    if (e->usg) {
+      int32_t u = 0;
+      if (hipMemcpy(&u, e->usg->cell_unit + (cells[k] - 1), sizeof u,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return RWPT_E_DEVICE;
+      rec_index = u;
+    }
     double rec[rwpt::CELL_DOUBLES];
-    if (hipMemcpy(rec, P.cells + (int64_t)(cells[k] - 1) * rwpt::CELL_DOUBLES, sizeof rec,
+    if (hipMemcpy(rec, P.cells + rec_index * rwpt::CELL_DOUBLES, sizeof rec,
                   hipMemcpyDeviceToHost) != hipSuccess)
       return RWPT_E_DEVICE;
     const int32_t *ints = reinterpret_cast<const int32_t *>(rec);
     const bool dry = (ints[2 * rwpt::F_FLAGS + 1] & rwpt::CF_DRY) != 0;
+    const double dx = e->usg ? G.udx[cells[k] - 1] : rec[rwpt::F_DX];
+    const double dy = e->usg ? G.udy[cells[k] - 1] : rec[rwpt::F_DY];
     rfac[k] = rec[rwpt::F_R];
-    wvol[k] = dry ? 0.0 : rec[rwpt::F_PHI] * rec[rwpt::F_DX] * rec[rwpt::F_DY] * rec[rwpt::F_DZ];
+    wvol[k] = dry ? 0.0 : rec[rwpt::F_PHI] * dx * dy * rec[rwpt::F_DZ];
     obs_index[cells[k] - 1] = k;
   }
   const int stop_status = e->opt.backward_tracking ? 4 : 3;

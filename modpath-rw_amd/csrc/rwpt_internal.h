@@ -122,6 +122,7 @@ struct DeviceProblem {
   int32_t *layer_type = nullptr, *zone = nullptr;
   double *source = nullptr, *sink = nullptr;
   double *disp = nullptr;  // 5 arrays of ndisp: aL aT aLV aTV dmeff
+  double *sat_top = nullptr;  // SaturatedTop of the time step, null: top
   int64_t ndisp = 0;
   int32_t default_icbound = 0;
   int32_t disp_model = 0;
@@ -135,6 +136,11 @@ struct DeviceProblem {
   float *cells32 = nullptr; // FP32-local tables (ncell * CELL_FLOATS)
   bool pow2_cells = false;  // every cell's DX, DY, DZ, R a power of two
 };
+
+// the top of ConvertToModelXYZ: SaturatedTop when it was set, else Top
+inline const double *model_top(const DeviceProblem &P) {
+  return P.sat_top ? P.sat_top : P.top;
+}
 
 struct Particles {
   int64_t n = 0, cap = 0;

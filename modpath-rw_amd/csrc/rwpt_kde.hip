@@ -325,7 +325,7 @@ KdeGeom make_geom(const rwpt_engine *e) {
   const DeviceProblem &P = e->prob;
   const Particles &S = e->part;
   const UsgData *U = e->usg;
-  return KdeGeom{P.ncol, P.nrow, e->xface, e->yface, P.delr, P.delc, P.top, P.bot,
+  return KdeGeom{P.ncol, P.nrow, e->xface, e->yface, P.delr, P.delc, model_top(P), P.bot,
                  S.lx, S.ly, S.lz, U ? U->cx : nullptr, U ? U->cy : nullptr,
                  U ? U->dx : nullptr, U ? U->dy : nullptr};
 }

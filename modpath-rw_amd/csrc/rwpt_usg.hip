@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(64) usg_units(const UsgView V, const TablePara
 }
 
 // ConvertToModelXYZ (ModflowRectangularGrid.f90:524-549) with the cell
-// bounds of GetLeft/Right/Front/Back (:275-309), SaturatedTop = Top
+// bounds of GetLeft/Right/Front/Back (:275-309); top is SaturatedTop
 __global__ void usg_global_xyz(int64_t n, const int32_t *__restrict__ cellv,
                                const double *__restrict__ lxv, const double *__restrict__ lyv,
                                const double *__restrict__ lzv, const double *__restrict__ cx,
@@ -1105,7 +1105,7 @@ hipError_t launch_usg_global_xyz(rwpt_engine *e, double *gx, double *gy, double 
   if (n > 0)
     hipLaunchKernelGGL(usg_global_xyz, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        e->stream, n, e->part.cell, e->part.lx, e->part.ly, e->part.lz, U.cx,
-                       U.cy, U.dx, U.dy, e->prob.top, e->prob.bot, gx, gy, gz);
+                       U.cy, U.dx, U.dy, model_top(e->prob), e->prob.bot, gx, gy, gz);
   return hipGetLastError();
 }
 

@@ -299,7 +299,7 @@ hipError_t launch_global_xyz(rwpt_engine *e, double *gx, double *gy,
     hipLaunchKernelGGL(global_xyz_kernel, dim3((unsigned)((n + block - 1) / block)),
                        dim3(block), 0, e->stream, n, e->part.cell, e->part.lx,
                        e->part.ly, e->part.lz, P.ncol, P.nrow, e->xface, e->yface,
-                       P.delr, P.delc, P.top, P.bot, gx, gy, gz);
+                       P.delr, P.delc, model_top(P), P.bot, gx, gy, gz);
   return hipGetLastError();
 }
 }  // namespace rwpt

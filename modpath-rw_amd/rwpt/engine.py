@@ -88,6 +88,7 @@ EXPORTS = {
     "rwpt_set_flow": ([C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _i32p,
                        C.c_int32], C.c_int),
     "rwpt_set_cell_extras": ([C.c_void_p, _dp, _i32p, _i32p, _dp, _dp], C.c_int),
+    "rwpt_set_saturated_top": ([C.c_void_p, _dp], C.c_int),
     "rwpt_set_dispersion": ([C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp,
                              _dp, _dp], C.c_int),
     "rwpt_load_particles": ([C.c_void_p, C.c_int64, _i32p, _dp, _dp, _dp, _dp,
@@ -299,13 +300,21 @@ class Engine:
             self._check(rc, "rwpt_set_flow_usg")
             # the problem brings its heads, zones and source and sink flows
             self.set_cell_extras(P.head, None, P.zone, P.source, P.sink)
-            return
-        a = [_d(P.frf), _d(P.fff), _d(P.flf), _d(P.bflow), _d(P.porosity),
-             _d(P.retardation)]
-        ib, ic = _i(P.ibound), _i(P.icbound)
-        rc = lib().rwpt_set_flow(self.h, *[_p(x, _dp) for x in a], _p(ib, _i32p),
-                                 _p(ic, _i32p), int(P.default_icbound))
-        self._check(rc, "rwpt_set_flow")
+        else:
+            a = [_d(P.frf), _d(P.fff), _d(P.flf), _d(P.bflow), _d(P.porosity),
+                 _d(P.retardation)]
+            ib, ic = _i(P.ibound), _i(P.icbound)
+            rc = lib().rwpt_set_flow(self.h, *[_p(x, _dp) for x in a], _p(ib, _i32p),
+                                     _p(ic, _i32p), int(P.default_icbound))
+            self._check(rc, "rwpt_set_flow")
+        self.set_saturated_top(getattr(P, "saturated_top", None))
+
+    def set_saturated_top(self, saturated_top=None):
+        """rwpt_set_saturated_top: Grid%SaturatedTop of the time step (one
+        value per cell), the top of the global z; None: the cell tops."""
+        a = _d(saturated_top)
+        self._check(lib().rwpt_set_saturated_top(self.h, _p(a, _dp)),
+                    "rwpt_set_saturated_top")
 
     def set_dispersion(self, P):
         a = [_d(P.aL), _d(P.aT), _d(P.aLV), _d(P.aTV), _d(P.dmeff)]

@@ -481,6 +481,7 @@ class TimeStepData:
     sink_flows: np.ndarray
     storage_flows: np.ndarray
     steady_state: bool
+    saturated_top: np.ndarray = None   # Grid%SaturatedTop
 
 
 def _dry_tolerance(hdry):
@@ -510,7 +511,18 @@ def load_time_step(grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetReco
     # IBoundTS: dry ICELLTYPE 1 cells (FlowModelData.f90:260-282)
     conv = grb.icelltype == 1
     diff = np.abs(h.astype(np.float32) - np.float32(hdry))
-    ib[conv & (diff < _dry_tolerance(hdry))] = 0
+    dry = conv & (diff < _dry_tolerance(hdry))
+    ib[dry] = 0
+    # SaturatedTop (FlowModelData.f90:259-280, MODFLOW 6 grids): the bottom
+    # of a dry cell whose head lies below it, the head of a wet convertible
+    # cell whose head lies between its bottom and top, else the top
+    top = np.asarray(grb.top, np.float64)
+    bot = np.asarray(grb.botm, np.float64)
+    sat = top.copy()
+    low = dry & (h < bot)
+    sat[low] = bot[low]
+    wet = conv & (ib != 0) & (h <= top) & (h >= bot)
+    sat[wet] = h[wet]
     src, snk, sto = np.zeros(n), np.zeros(n), np.zeros(n)
     bfl = np.zeros(6 * n)
     fja = np.zeros(grb.nja)
@@ -595,7 +607,7 @@ def load_time_step(grb, heads: Sequence[HeadRecord], budget: Sequence[BudgetReco
                     split(r.node - 1, r.q)
             elif r.method in (5, 6):
                 listed(r)
-    return TimeStepData(ib, h, fja, bfl, src, snk, sto, steady)
+    return TimeStepData(ib, h, fja, bfl, src, snk, sto, steady, sat)
 
 
 def ibound_from_idomain(grb: Grb) -> np.ndarray:
@@ -666,7 +678,9 @@ def problem(grb: Grb, ts: TimeStepData, porosity, retardation=None, **options):
                 porosity=np.broadcast_to(np.asarray(porosity, np.float64), (n,)).copy(),
                 retardation=None if retardation is None else
                 np.broadcast_to(np.asarray(retardation, np.float64), (n,)).copy(),
-                ibound=a["ibound"], steady_state=int(bool(ts.steady_state)), **options)
+                ibound=a["ibound"], steady_state=int(bool(ts.steady_state)),
+                saturated_top=None if ts.saturated_top is None else ts.saturated_top.copy(),
+                **options)
     extras = dict(head=a["head"], layer_type=a["layer_type"], source_flow=a["source_flow"],
                   sink_flow=a["sink_flow"])
     return P, extras
@@ -692,7 +706,9 @@ def usg_problem(grb: GrbDisv, ts: TimeStepData, porosity, retardation=None, **op
         retardation=None if retardation is None else
         np.broadcast_to(np.asarray(retardation, np.float64), (n,)).copy(),
         head=ts.heads.copy(), layer_type=grb.icelltype.astype(np.int32),
-        ibound=ts.ibound_ts.copy(), steady_state=int(bool(ts.steady_state)), **options)
+        ibound=ts.ibound_ts.copy(), steady_state=int(bool(ts.steady_state)),
+        saturated_top=None if ts.saturated_top is None else ts.saturated_top.copy(),
+        **options)
 
 
 class FlowSeries:

@@ -62,6 +62,9 @@ class Problem:
     stop_weak_sinks: int = 0
     stop_weak_sources: int = 0
     steady_state: int = 1
+    # Grid%SaturatedTop of the time step (the top of ConvertToModelXYZ),
+    # None: the cell tops
+    saturated_top: Optional[np.ndarray] = None
 
     @property
     def has_extras(self) -> bool:
@@ -324,7 +327,10 @@ def stack(problems):
         aTV=disp("aTV", 0.0) if p0.disp_model == 1 else None,
         dmeff=disp("dmeff", 0.0), advection_kind=p0.advection_kind,
         time_step_kind=p0.time_step_kind, ts_params=p0.ts_params,
-        dim_mask=p0.dim_mask, backward=p0.backward, members=len(problems))
+        dim_mask=p0.dim_mask, backward=p0.backward, members=len(problems),
+        saturated_top=None if all(q.saturated_top is None for q in problems) else
+        np.concatenate([q.top if q.saturated_top is None else q.saturated_top
+                        for q in problems]))
     E._per_member = per_member
     return E
 
@@ -489,6 +495,7 @@ class UsgProblem:
     stop_weak_sinks: int = 0
     stop_weak_sources: int = 0
     steady_state: int = 1
+    saturated_top: Optional[np.ndarray] = None   # as Problem.saturated_top
 
     @property
     def nja(self) -> int:

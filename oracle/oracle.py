@@ -6,6 +6,7 @@ The oracle restates the reference algorithm on the CPU; see rwpt_oracle.c.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 
 import numpy as np
@@ -176,8 +177,15 @@ def cell_tables(P):
     return out
 
 
+def _with_saturated_top(P):
+    # ConvertToModelXYZ's top is Grid%SaturatedTop (ModflowRectangularGrid.f90:
+    # 524-549); a problem without one uses the cell tops
+    sat = getattr(P, "saturated_top", None)
+    return P if sat is None else dataclasses.replace(P, top=np.asarray(sat, np.float64))
+
+
 def global_xyz(P, cell, lx, ly, lz):
-    OP = P if isinstance(P, OracleProblem) else OracleProblem(P)
+    OP = P if isinstance(P, OracleProblem) else OracleProblem(_with_saturated_top(P))
     n = cell.size
     gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
     lib().orc_global_xyz(C.byref(OP.s), n, _ptr(_i(cell), _ip), _ptr(_d(lx), _dp),
@@ -343,7 +351,7 @@ def usg_advance(U, state, t_engine, t_stop, nthreads=0):
 
 
 def usg_global_xyz(U, cell, lx, ly, lz):
-    OP = U if isinstance(U, OracleUsgProblem) else OracleUsgProblem(U)
+    OP = U if isinstance(U, OracleUsgProblem) else OracleUsgProblem(_with_saturated_top(U))
     n = cell.size
     gx, gy, gz = np.zeros(n), np.zeros(n), np.zeros(n)
     _usg_lib().orc_usg_global_xyz(C.byref(OP.s), n, _ptr(_i(cell), _ip),

@@ -18,7 +18,8 @@
 !   SourceFlow and SinkFlow of the assembled cell, and the face pore
 !   velocities VX1 VX2 VY1 VY2 VZ1 VZ2 and DX DY DZ of its sub-cell buffer
 !   (FillMassSubCellDataBuffer of sub-cell (1,1), forward tracking) (f8);
-!   with disv, then CellX, CellY, DelX, DelY (f8, ncell each).
+!   with disv, then CellX, CellY, DelX, DelY (f8, ncell each); last the
+!   grid's SaturatedTop after LoadTimeStep (f8, ncell).
 program mf6_ref
   use RectangularGridDisMf6Module, only: RectangularGridDisMf6Type
   use RectangularGridDisvMf6Module, only: RectangularGridDisvMf6Type
@@ -114,5 +115,6 @@ program mf6_ref
          sc%DX, sc%DY, sc%DZ
   end do
   if (trim(kind) .eq. 'disv') write(30) gp%CellX, gp%CellY, gp%DelX, gp%DelY
+  write(30) gp%SaturatedTop
   close(30)
 end program mf6_ref

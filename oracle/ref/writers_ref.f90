@@ -17,7 +17,9 @@
 !         int32 group, sequence, id, status, cell, initialCell,
 !         initialFace (np each); real64 lx, ly, lz, t, ilx, ily, ilz, it,
 !         mass (np each); int32 solute(ngroups); character(20) names(ngroups);
-!         int32 isobs(ncell); real64 rfac(ncell), wvol(ncell), flowRate
+!         int32 isobs(ncell); real64 rfac(ncell), wvol(ncell), flowRate;
+!         optionally real64 saturatedTop(ncell) (Grid%SaturatedTop, which
+!         ConvertToModelXYZ uses as the cell top; the tops when absent)
 !
 ! Observation records (WriteResidentObsRecord[Binary], WriteSinkObsRecord
 ! [Binary], Observation.f90:114-221) for the particles in cells with
@@ -53,6 +55,8 @@ program writers_ref
   integer(int32), allocatable :: isobs(:)
   real(real64), allocatable :: rfac(:), wvol(:)
   real(real64) :: flowRate
+  real(real64), allocatable :: satTop(:)
+  integer :: ios
   type(ObservationType) :: obs
   type(ParticleType) :: q1
   integer :: c
@@ -121,6 +125,9 @@ program writers_ref
   read(10) solute, names
   allocate(isobs(ncell), rfac(ncell), wvol(ncell))
   read(10) isobs, rfac, wvol, flowRate
+  allocate(satTop(ncell))
+  read(10, iostat=ios) satTop
+  if (ios == 0) dis%SaturatedTop = satTop
   close(10)
 
   ! timeseries: header, then the active particles in order

@@ -178,7 +178,7 @@ def writers(P, state, initial, meta, timeout=120):
     solute (per group), names (per group), tracking_direction, time_point,
     time_step, reference_time, origin, rotation, time, zones (per cell) and
     optionally obs_cells (1-based), retardation and water_volume (per cell),
-    flow_rate."""
+    flow_rate and saturated_top (per cell: the top of ConvertToModelXYZ)."""
     f8, i4 = np.float64, np.int32
     nb = P.ncell
     hdr = np.array([P.ncol, P.nrow, P.nlay, state["cell"].size, len(meta["names"]),
@@ -200,6 +200,8 @@ def writers(P, state, initial, meta, timeout=120):
     tail = [isobs, np.asarray(meta.get("retardation", np.ones(nb)), f8).reshape(nb),
             np.asarray(meta.get("water_volume", np.zeros(nb)), f8).reshape(nb),
             np.array([meta.get("flow_rate", 0.0)], f8)]
+    if meta.get("saturated_top") is not None:
+        tail.append(np.asarray(meta["saturated_top"], f8).reshape(nb))
     with tempfile.TemporaryDirectory() as d:
         fin = os.path.join(d, "in.bin")
         with open(fin, "wb") as fh:
@@ -276,6 +278,7 @@ def mf6(grb, hds, cbc, ibound, porosity, retardation, kper=1, kstp=1, hnoflow=1e
         for k in ("cell_x", "cell_y", "del_x", "del_y"):
             out[k] = np.frombuffer(b[p:p + 8 * n], np.float64).copy()
             p += 8 * n
+    out["saturated_top"] = np.frombuffer(b[p:p + 8 * n], np.float64).copy()
     return out
 
 

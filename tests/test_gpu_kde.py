@@ -70,6 +70,39 @@ def test_kde_matches_oracle(case):
     assert abs(dens.sum() * vol - hist.sum()) <= 1e-9 * hist.sum()
 
 
+def test_global_z_uses_the_saturated_top():
+    """A convertible top layer whose saturated top lies below the cell top
+    (rwpt_set_saturated_top): the global z and the histogram follow
+    ConvertToModelXYZ with Grid%SaturatedTop."""
+    P = problems.heterogeneous_flow(24, 20, 6, corr_len=4.0, seed=7)
+    P.aL, P.aT, P.dmeff = np.array([0.5]), np.array([0.05]), np.array([1e-4])
+    P.ts_params = (0.5, 0.0)
+    sat = P.top.copy()
+    top_layer = np.arange(P.ncol * P.nrow)
+    sat[top_layer] = P.bot[top_layer] + (P.top - P.bot)[top_layer] * \
+        np.random.default_rng(2).uniform(0.2, 0.9, top_layer.size)
+    P.saturated_top = sat
+    E = _walked(P, None, 4000, 4.0, 2)
+    try:
+        g = E.particles()
+        gx, gy, gz = E.global_xyz()
+        origin, binsize, nbins = _grid(P)
+        _, hist = E.kde(origin, binsize, nbins, mode=0)
+        E.set_saturated_top(None)
+        gz_top = E.global_xyz()[2]
+    finally:
+        E.close()
+    ox, oy, oz = O.global_xyz(P, g["cell"], g["lx"], g["ly"], g["lz"])
+    assert np.array_equal(gx, ox) and np.array_equal(gy, oy) and np.array_equal(gz, oz)
+    in_top = g["cell"] <= P.ncol * P.nrow
+    assert in_top.sum() > 100
+    assert (gz[in_top] < gz_top[in_top]).mean() > 0.9
+    assert np.array_equal(gz[~in_top], gz_top[~in_top])
+    act = g["status"] == 1
+    assert np.array_equal(hist, O.histogram(ox[act], oy[act], oz[act], origin, binsize,
+                                            nbins))
+
+
 def test_kde_full_size_mass_conservation():
     P = problems.config1()
     n = 1_000_000

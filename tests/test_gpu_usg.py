@@ -124,6 +124,9 @@ def test_usg_larger_grid_bitwise():
 
 def test_usg_global_xyz_and_histogram():
     Q, U = build("block")
+    # a saturated top below the cell top in the upper layer
+    U.saturated_top = U.top.copy()
+    U.saturated_top[:Q.ncpl] = U.bot[:Q.ncpl] + 0.6 * (U.top - U.bot)[:Q.ncpl]
     cell, lx, ly, lz = release(U, 2000, seed=8)
     E = Engine(U)
     try:
@@ -142,9 +145,11 @@ def test_usg_global_xyz_and_histogram():
     assert np.array_equal(hist, want)
 
 
-def test_usg_plain_grid_equals_structured_engine():
+def test_usg_plain_grid_equals_structured_engine(tmp_path):
     """A DISV grid without refined cells is a structured grid: the
-    unstructured walk must give the structured walk's results."""
+    unstructured walk must give the structured walk's results, and the
+    timeseries and observation records written from both engines (global
+    coordinates with a saturated top, water volumes) the same bytes."""
     from test_oracle_usg_vs_reference import _structured_twin
     Q = Quadtree(**GRIDS["plain"])
     A = Q.cell_arrays()
@@ -153,19 +158,33 @@ def test_usg_plain_grid_equals_structured_engine():
                             aL=np.array([0.3]), aT=np.array([0.03]), ts_params=(0.1, 0.0),
                             **A)
     P = _structured_twin(Q, U)
+    sat = U.top.copy()
+    sat[:Q.ncpl] -= 0.3 * (U.top - U.bot)[:Q.ncpl]
+    U.saturated_top = P.saturated_top = sat
     cell, lx, ly, lz = release(U, 1000, seed=9)
-    res = []
-    for prob in (U, P):
+    n = cell.size
+    ids, group, solute = np.arange(1, n + 1, dtype=np.int32), np.ones(n, np.int32), \
+        np.ones(1, np.int32)
+    obs_cells = np.arange(1, Q.ncell + 1, dtype=np.int32)
+    res, files = [], []
+    for k, prob in enumerate((U, P)):
         E = Engine(prob)
         try:
             E.load_particles(cell, lx, ly, lz)
             for t1 in (0.4, 1.0, 2.5):
                 E.advance(t1)
             res.append(E.particles())
+            ts, ob = tmp_path / f"ts{k}", tmp_path / f"obs{k}"
+            E.write_timeseries(str(ts), 3, 1, 2.5, ids, ids, group)
+            E.write_observation(str(ob), 1, obs_cells, 3, 1, ids, group, solute,
+                                fmt=0, append=False)
+            files.append((ts.read_bytes(), ob.read_bytes()))
         finally:
             E.close()
     for f in FIELDS:
         assert _same(res[0][f], res[1][f]).all(), f
+    assert files[0][0] == files[1][0] and len(files[0][0]) > 0
+    assert files[0][1] == files[1][1] and len(files[0][1]) > 0
 
 
 def test_usg_from_mf6_disv_files(tmp_path):

@@ -5,7 +5,9 @@ WriteResidentObsRecord[Binary] and WriteSinkObsRecord[Binary] of
 Observation.f90, compiled here into oracle/_ref/writers_ref) on the same
 particles: byte-identical.  The water volume of a resident record is the
 porosity times DX DY and the thickness of the (confined, wet) cell, given to
-the reference driver from the problem arrays."""
+the reference driver from the problem arrays.  The top layer's saturated top
+(Grid%SaturatedTop) lies below its cell top, so the global z of its
+particles follows ConvertToModelXYZ with the saturated top."""
 import numpy as np
 import pytest
 
@@ -48,10 +50,14 @@ def test_files_match_reference(tmp_path):
     k, i, j = np.meshgrid(np.arange(2, 6), np.arange(5, 13), np.arange(4, 12), indexing="ij")
     obs_cells = np.sort(P.cell_number(k.ravel(), i.ravel(), j.ravel())).astype(np.int32)
     flow_rate = float(sink[obs_cells - 1].sum())
+    # the saturated top of a convertible top layer (the top of the global z)
+    sat = P.top.copy()
+    sat[:P.ncol * P.nrow] -= 0.4 * (P.top - P.bot)[:P.ncol * P.nrow]
     print("writers: particles", n, flush=True)
     E = Engine(P, stop_at_weak_sinks=1)
     try:
         E.set_cell_extras(zone=zones, sink_flow=sink)
+        E.set_saturated_top(sat)
         E.load_particles(cell, lx, ly, lz, t=t0, status=status)
         E.advance(8.0)
         st = E.particles()
@@ -81,7 +87,7 @@ def test_files_match_reference(tmp_path):
             "names": names, "tracking_direction": 1, "time_point": 3, "time_step": 1,
             "reference_time": 0.5, "origin": (100.25, -3.5), "rotation": 12.0,
             "time": 8.0, "zones": zones, "obs_cells": obs_cells,
-            "retardation": np.ones(P.ncell), "flow_rate": flow_rate,
+            "retardation": np.ones(P.ncell), "flow_rate": flow_rate, "saturated_top": sat,
             "water_volume": (P.porosity * P.delr[np.arange(P.ncell) % P.ncol] *
                              P.delc[(np.arange(P.ncell) // P.ncol) % P.nrow] *
                              (P.top - P.bot))}

@@ -147,6 +147,11 @@ def test_time_step_arrays_bitwise(case):
               "storage_flows"):
         np.testing.assert_array_equal(getattr(ts, k), ref[k], err_msg=k)
     assert ts.steady_state == ref["steady_state"]
+    # SaturatedTop: heads of wet convertible cells, bottoms of dry ones
+    np.testing.assert_array_equal(ts.saturated_top, ref["saturated_top"])
+    g = case["grb"]
+    assert (ts.saturated_top == ts.heads).sum() >= 3
+    assert ((ts.saturated_top == g.botm) & (ts.ibound_ts == 0)).sum() >= 2
     # the scenario exercises what it claims
     assert (ts.ibound_ts == 0).sum() >= 4
     assert np.count_nonzero(ts.boundary_flows) >= 5

@@ -101,6 +101,9 @@ def test_disv_time_step_arrays_bitwise(case):
               "storage_flows"):
         np.testing.assert_array_equal(getattr(ts, k), ref[k], err_msg=k)
     assert ts.steady_state == ref["steady_state"]
+    np.testing.assert_array_equal(ts.saturated_top, ref["saturated_top"])
+    assert (ts.saturated_top == ts.heads).sum() >= 3
+    assert ((ts.saturated_top == case["grb"].botm) & (ts.ibound_ts == 0)).sum() >= 3
     assert (ts.ibound_ts == 0).sum() >= 3
     assert np.count_nonzero(ts.boundary_flows) >= 5
 
