@@ -57,7 +57,7 @@ module RwptGpuModule
     real(c_double)     :: min_h_over_delta, max_h_over_delta
     integer(c_int32_t) :: effective_weight_format, kernel_database
     real(c_double)     :: delta_h_over_delta
-    integer(c_int32_t) :: member
+    integer(c_int32_t) :: member, initial
   end type
 
   interface

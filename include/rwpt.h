@@ -148,6 +148,12 @@ typedef struct {
                                      particles are reconstructed (0-based),
                                      -1 all members together; ignored with
                                      one member                          */
+  int32_t initial;                /* 1: the particles pending release
+                                     (status 0) whose release time is 0, at
+                                     their loaded locations: MPathRW's
+                                     initial-condition reconstruction
+                                     (MPathRW.F90:1043-1100); 0: the active
+                                     particles (status 1)                */
 } rwpt_kde_options;
 
 /* Fills *opt with the reference defaults (EULERIAN, FIXED, 3-D, forward,

@@ -925,6 +925,8 @@ int check_kde_options(rwpt_engine *e, const rwpt_kde_options *opt, int64_t *ncel
                                "max_h_over_delta");
   if (opt->effective_weight_format < 0 || opt->effective_weight_format > 3)
     return fail(e, RWPT_E_ARG, "effective_weight_format must be 0..3");
+  if (opt->initial != 0 && opt->initial != 1)
+    return fail(e, RWPT_E_ARG, "initial must be 0 or 1");
   // InitializeKernelDatabaseFlat's parameter checks (GridProjectedKDE.F90:693-740)
   if (opt->kernel_database &&
       !(opt->min_h_over_delta > 0.0 && opt->max_h_over_delta > opt->min_h_over_delta &&
@@ -1005,6 +1007,8 @@ int rwpt_kde_device(rwpt_engine *e, const rwpt_kde_options *opt, double *density
     return fail(e, RWPT_E_UNSUPPORTED, "rwpt_kde_device: one pass only (n_opt_loops 0)");
   if (opt->weighted)
     return fail(e, RWPT_E_UNSUPPORTED, "rwpt_kde_device: unit masses only");
+  if (opt->initial)
+    return fail(e, RWPT_E_UNSUPPORTED, "rwpt_kde_device: active particles only");
   if (ncells > RWPT_KDE_DEVICE_MAX_CELLS)
     return fail(e, RWPT_E_UNSUPPORTED,
                 "rwpt_kde_device: grid above RWPT_KDE_DEVICE_MAX_CELLS bins");

@@ -75,6 +75,17 @@ __device__ __forceinline__ void kde_point(const KdeGeom &Q, int c0, int64_t p,
   g[2] = fmin(fmax(((1.0 - z) * b) + (z * t), b), t);
 }
 
+// The initial-condition reconstruction of MPathRW.F90:1043-1100 takes the
+// particles pending release (status 0) whose release time is 0: sel holds 1
+// for them (the status value the reconstruction kernels take) and 0 else.
+__global__ void kde_select_initial(int64_t n, const int32_t *__restrict__ status,
+                                   const double *__restrict__ t,
+                                   int32_t *__restrict__ sel) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  sel[p] = (status[p] == 0 && t[p] == 0.0) ? 1 : 0;
+}
+
 // Global coordinates (ConvertToModelXYZ) and bin key of every particle;
 // inactive particles and points outside the grid get the key ncells.  gx
 // may be null: the coordinates are then not kept (the statistics kernel
@@ -258,9 +269,16 @@ __global__ void kde_finish(int64_t n, double *__restrict__ d, double eff,
   d[i] = v * sf;
 }
 
-// One 1-D pass of the separable kernel along `axis` with half-sample
-// symmetric padding (d c b a | a b c d | d c b a); weights w[0..m] are the
-// normalised 1-D kernel at offsets 0..m.
+// One 1-D pass of the separable kernel along `axis` with the reflection of
+// KernelMultiGaussian.f90:303-390: a bin's kernel entries below the grid
+// fold once onto the grid (target -1-q), then the entries above it fold
+// once (target 2n-1-q), and what either fold leaves outside is dropped.
+// As a gather, output p takes the source bins p+o (direct), o-1-p (folded
+// at the lower face), 2n-1-p+o (at the upper face) and p-2n+o (at the lower
+// then the upper face) with weight w[|o|].  For m <= n only single folds
+// occur, half-sample symmetric padding (d c b a | a b c d | d c b a); wider
+// kernels lose the mass folded beyond the opposite face, as the reference
+// does.  Weights w[0..m] are the normalised 1-D kernel at offsets 0..m.
 __global__ void kde_pass(const double *__restrict__ in, double *__restrict__ out,
                          int nx, int ny, int nz, int axis,
                          const double *__restrict__ w, int m, double scale) {
@@ -276,9 +294,12 @@ __global__ void kde_pass(const double *__restrict__ in, double *__restrict__ out
   const int64_t base = idx - (int64_t)pos * stride;
   double s = 0.0;
   for (int o = -m; o <= m; ++o) {
-    int q = pos + o;
-    while (q < 0 || q >= n) q = q < 0 ? -1 - q : 2 * n - 1 - q;
-    s += w[o < 0 ? -o : o] * in[base + (int64_t)q * stride];
+    const double wo = w[o < 0 ? -o : o];
+    const int a = pos + o, b = o - 1 - pos, c = 2 * n - 1 - pos + o, d = pos - 2 * n + o;
+    if (a >= 0 && a < n) s += wo * in[base + (int64_t)a * stride];
+    if (b >= 0 && b < n) s += wo * in[base + (int64_t)b * stride];
+    if (c >= 0 && c < n) s += wo * in[base + (int64_t)c * stride];
+    if (d >= 0 && d < n) s += wo * in[base + (int64_t)d * stride];
   }
   out[idx] = s * scale;
 }
@@ -335,16 +356,17 @@ KdeGeom make_geom(const rwpt_engine *e) {
 // squared deviations from the mean.  Two deterministic fixed-tree passes
 // and two host read-backs.
 hipError_t kde_moments(rwpt_engine *e, const KdeGrid &G, const KdeGeom &Q,
-                       const double *gx, const double *gy, const double *gz,
-                       const double *mass_dev, int64_t ncells, double *partial,
-                       double *sums, double hs[kStats], double dev2[kStats]) {
+                       const int32_t *sel, const double *gx, const double *gy,
+                       const double *gz, const double *mass_dev, int64_t ncells,
+                       double *partial, double *sums, double hs[kStats],
+                       double dev2[kStats]) {
   const Particles &S = e->part;
   const int64_t n = S.n;
   hipStream_t st = e->stream;
   hipError_t err = hipSuccess;
   for (int q = 0; q < kStats; ++q) hs[q] = dev2[q] = 0.0;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status, S.cell,
+  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, sel, S.cell,
                      G, S.keys[0], ncells, gx, gy, gz, Q, mass_dev, 0,
                      (const double *)nullptr, partial);
   hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
@@ -359,7 +381,7 @@ hipError_t kde_moments(rwpt_engine *e, const KdeGrid &G, const KdeGeom &Q,
   if ((err = hipMemcpyAsync(sums + kStats, mean, sizeof mean, hipMemcpyHostToDevice,
                             st)) != hipSuccess)
     return err;
-  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, S.status, S.cell,
+  hipLaunchKernelGGL(kde_stats, dim3(kStatBlocks), dim3(256), 0, st, n, sel, S.cell,
                      G, S.keys[0], ncells, gx, gy, gz, Q, mass_dev, 1,
                      (const double *)(sums + kStats), partial);
   hipLaunchKernelGGL(kde_stats_final, dim3(1), dim3(kStatBlocks), 0, st, partial, sums);
@@ -510,8 +532,10 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   const int64_t ncells = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
   hipStream_t st = e->stream;
   // scratch: gx gy gz (3n), keys/vals reuse the walk's work lists, two grid
-  // buffers, stats partials
-  const int64_t need = 3 * n + 2 * ncells + kStats * kStatBlocks + 2 * kStats;
+  // buffers, stats partials, the selection of an initial-condition pass
+  const bool initial = opt->initial != 0;
+  const int64_t need = 3 * n + 2 * ncells + kStats * kStatBlocks + 2 * kStats +
+                       (initial ? (n + 1) / 2 : 0);
   hipError_t err = hipSuccess;
   if (need > e->kde_scratch_size) {
     if (e->kde_scratch) (void)hipFree(e->kde_scratch);
@@ -524,12 +548,20 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   double *gx = e->kde_scratch, *gy = gx + n, *gz = gy + n;
   double *tmpA = gz + n, *tmpB = tmpA + ncells;
   double *partial = tmpB + ncells, *sums = partial + kStats * kStatBlocks;
+  // the particles taken: status 1, or the selection of the initial condition
+  const int32_t *sel = S.status;
   KernelTimer &TH = e->timers[1], &TD = e->timers[2];
   (void)hipEventRecord(TH.start, st);
   const int blk = 256;
+  if (n > 0 && initial) {
+    int32_t *s = reinterpret_cast<int32_t *>(sums + 2 * kStats);
+    hipLaunchKernelGGL(kde_select_initial, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk),
+                       0, st, n, S.status, S.t, s);
+    sel = s;
+  }
   if (n > 0) {
     hipLaunchKernelGGL(kde_keys, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0, st,
-                       n, S.status, S.cell, Q, G, gx, gy, gz, S.keys[0], S.vals[0]);
+                       n, sel, S.cell, Q, G, gx, gy, gz, S.keys[0], S.vals[0]);
     if ((err = hipGetLastError()) != hipSuccess) return err;
   }
   if (n > 0 && !mass_dev) {
@@ -572,7 +604,7 @@ hipError_t launch_kde(rwpt_engine *e, const rwpt_kde_options *opt,
   (void)hipEventRecord(TH.stop, st);
   // statistics for the bandwidth and the effective mass
   double hs[kStats], dev2[kStats];
-  if ((err = kde_moments(e, G, Q, gx, gy, gz, mass_dev, ncells, partial, sums, hs,
+  if ((err = kde_moments(e, G, Q, sel, gx, gy, gz, mass_dev, ncells, partial, sums, hs,
                          dev2)) != hipSuccess)
     return err;
   const KdeBandwidth B = kde_bandwidth(opt, G, nd, mass_dev != nullptr, hs, dev2);
@@ -719,8 +751,8 @@ hipError_t launch_kde_device(rwpt_engine *e, const rwpt_kde_options *opt,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   (void)hipEventRecord(TH.stop, st);
   double hs[kStats], dev2[kStats];
-  if ((err = kde_moments(e, G, Q, nullptr, nullptr, nullptr, nullptr, ncells, partial,
-                         sums, hs, dev2)) != hipSuccess)
+  if ((err = kde_moments(e, G, Q, S.status, nullptr, nullptr, nullptr, nullptr, ncells,
+                         partial, sums, hs, dev2)) != hipSuccess)
     return err;
   const KdeBandwidth B = kde_bandwidth(opt, G, nd, false, hs, dev2);
   record_stats(stats_host, B, hs[5]);

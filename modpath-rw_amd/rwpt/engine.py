@@ -62,7 +62,7 @@ class RwptKdeOptions(C.Structure):
                 ("max_h_over_delta", C.c_double),
                 ("effective_weight_format", C.c_int32),
                 ("kernel_database", C.c_int32), ("delta_h_over_delta", C.c_double),
-                ("member", C.c_int32)]
+                ("member", C.c_int32), ("initial", C.c_int32)]
 
 
 class RwptAdjointOpts(C.Structure):
@@ -544,7 +544,7 @@ class Engine:
                     rel_error=0.02, initial_smoothing=0, smoothing_factor=1.0,
                     bound_kernel_size=0, min_h_over_delta=0.0, max_h_over_delta=0.0,
                     effective_weight_format=0, kernel_database=False,
-                    delta_h_over_delta=0.0, member=-1):
+                    delta_h_over_delta=0.0, member=-1, initial=False):
         """rwpt_kde_options from the options of the gpkde input file
         (ReadGPKDEData, ModpathSimulationData.f90:816-1273)."""
         o = RwptKdeOptions()
@@ -569,6 +569,7 @@ class Engine:
         o.kernel_database = int(bool(kernel_database))
         o.delta_h_over_delta = delta_h_over_delta
         o.member = member
+        o.initial = int(bool(initial))
         return o
 
     def kde(self, origin, bin_size, nbins, mode=0, mass=None, **kw):

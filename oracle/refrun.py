@@ -383,3 +383,24 @@ def usg_tables(grb, U, timeout=600):
             subs.append((ints, dbl))
         cells.append(subs)
     return cells
+
+
+MPATHRW_BINARY = os.path.join(HERE, "_ref", "mpathrw")
+
+
+def mpathrw_available() -> bool:
+    return os.access(MPATHRW_BINARY, os.X_OK)
+
+
+def mpathrw(sim_path, timeout=120):
+    """Runs MODPATH-RW itself (MPathRW.F90 compiled from the reference's
+    sources) on a simulation file, serially, in the file's folder; returns
+    its console output.  Raises unless it ends with "Normal termination"."""
+    d, name = os.path.split(os.path.abspath(sim_path))
+    r = subprocess.run([MPATHRW_BINARY, "-nl", name], cwd=d, stdin=subprocess.DEVNULL,
+                       capture_output=True, text=True, timeout=timeout,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    if r.returncode != 0 or "Normal termination" not in r.stdout:
+        raise RuntimeError(f"mpathrw failed ({r.returncode}): {r.stdout[-2000:]}"
+                           f"{r.stderr[-2000:]}")
+    return r.stdout

@@ -90,6 +90,7 @@ program drive_engine
   ko%kernel_database = 0
   ko%delta_h_over_delta = 0d0
   ko%member = -1
+  ko%initial = 0
   if (rc == 0) rc = rwpt_kde(e, ko, c_null_ptr, dens, hist)
   if (rc /= 0) print '(a,i4)', 'engine error ', rc
   if (c_associated(e)) then

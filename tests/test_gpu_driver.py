@@ -28,7 +28,7 @@ def test_driver_outputs(tmp_path):
         E.load_particles(*rel)
         engine.write_timeseries_header(str(ts), 1, 0.0, (0.0, 0.0), 0.0)
         log = Driver(E, iv, timeseries_path=str(ts), kde_options=kde,
-                     kde_path=str(kd)).run()
+                     kde_path=str(kd), initial_point=False).run()
     finally:
         E.close()
     assert [x[0].time for x in log] == [1.0, 1.5, 2.0, 5.0, 6.0]
@@ -45,7 +45,7 @@ def test_driver_outputs(tmp_path):
             if tp is not None:
                 E.write_timeseries(str(ts2), tp, 1, t, seq, seq, np.ones(2000, np.int32))
             if gp is not None:
-                d, h = E.kde(**kde)
+                d, h = E.kde(mass=np.ones(2000), **kde)
                 engine.kde_write(str(kd2), kde["origin"], kde["bin_size"], kde["nbins"], d, h,
                                  time_id=gp, time=t, append=not first)
                 first = False

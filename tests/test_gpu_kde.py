@@ -70,6 +70,37 @@ def test_kde_matches_oracle(case):
     assert abs(dens.sum() * vol - hist.sum()) <= 1e-9 * hist.sum()
 
 
+@pytest.mark.parametrize("n,spread", [(1, 0.0), (3, 4.5), (8, 4.0)])
+def test_kernel_wider_than_the_grid(n, spread):
+    """Kernels whose half-width exceeds a grid dimension (3 layers, h up to
+    2 bins): the reference folds the part below the grid once and then the
+    part above it once, and drops what lands outside after that (its density
+    loses mass); a single particle takes the zero-variance bandwidth."""
+    P = problems.heterogeneous_flow(14, 10, 3, corr_len=3.0, seed=11)
+    g = np.random.default_rng(n)
+    s = int(spread)
+    cell = P.cell_number(g.integers(1, 4, n), g.integers(5 - s, 6 + s, n),
+                         g.integers(7 - s, 8 + s, n))
+    if n == 1:
+        cell = P.cell_number(np.array([3]), np.array([10]), np.array([10]))
+    lx, ly, lz = g.uniform(0.0, 1.0, n), g.uniform(0.0, 1.0, n), g.uniform(0.0, 0.05, n)
+    origin, binsize, nbins = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (14, 10, 3)
+    E = Engine(P)
+    try:
+        E.load_particles(cell, lx, ly, lz)
+        dens, hist = E.kde(origin, binsize, nbins, mode=1, mass=np.ones(n))
+        h = E.kde_info()["bandwidth"]
+    finally:
+        E.close()
+    gx, gy, gz = O.global_xyz(P, cell.astype(np.int32), lx, ly, lz)
+    od, oc, oh = O.kde_density(gx, gy, gz, origin, binsize, nbins, np.ones(n))
+    assert np.array_equal(hist, oc)
+    assert abs(h - oh[0]) <= 1e-12 * oh[0] and 3 * oh[0] > 3
+    np.testing.assert_allclose(dens, od, rtol=0, atol=1e-12 * od.max())
+    # the reference's folds drop mass when the kernel is wider than the grid
+    assert dens.sum() < 0.9999 * n
+
+
 def test_global_z_uses_the_saturated_top():
     """A convertible top layer whose saturated top lies below the cell top
     (rwpt_set_saturated_top): the global z and the histogram follow
