@@ -10,6 +10,7 @@ module RwptGpuModule
   public :: rwpt_grid, rwpt_options, rwpt_kde_options
   public :: rwpt_default_options, rwpt_create, rwpt_destroy, rwpt_last_error
   public :: rwpt_set_flow, rwpt_set_cell_extras, rwpt_set_dispersion, rwpt_set_saturated_top
+  public :: rwpt_set_graph
   public :: rwpt_load_particles, rwpt_get_particles, rwpt_particle_count
   public :: rwpt_advance, rwpt_set_time, rwpt_set_steady_state, rwpt_global_xyz
   public :: rwpt_kde, rwpt_kde_device, rwpt_kde_info
@@ -117,6 +118,11 @@ module RwptGpuModule
       type(c_ptr), value :: e
       real(c_double), intent(in) :: head(*), source_flow(*), sink_flow(*)
       integer(c_int32_t), intent(in) :: layer_type(*), zone(*)
+    end function
+    integer(c_int) function rwpt_set_graph(e, launches) bind(C, name='rwpt_set_graph')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: launches
     end function
     ! saturated_top: c_loc(Grid%SaturatedTop), or c_null_ptr for the cell tops
     integer(c_int) function rwpt_set_saturated_top(e, saturated_top) &

@@ -286,6 +286,15 @@ int rwpt_set_steady_state(rwpt_engine *e, int32_t steady_state);
  * Defaults 64, 256, 1, 1. */
 int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
                     int32_t sort, int32_t min_waves);
+/* Graph mode of rwpt_advance (no counterpart in the reference): with
+ * launches = K > 0 (even, at most 256) the walk of an interval runs as
+ * replays of a HIP graph of K walk launches captured on the engine stream,
+ * each launch reading its work-list length from device memory, so that the
+ * host waits once per K launches instead of once per launch; the work
+ * lists are not sorted by cell between launches.  Results are bit-identical
+ * to the eager walk (they depend on no launch shape).  0 (the default)
+ * turns it off. */
+int rwpt_set_graph(rwpt_engine *e, int32_t launches);
 /* Launch-shape autotuner (no counterpart in the reference): times trial
  * advances from the current state to t_probe over workgroup size and
  * occupancy variant, per-launch iteration budget and work-list sort (a

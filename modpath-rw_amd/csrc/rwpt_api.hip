@@ -262,6 +262,7 @@ int rwpt_destroy(rwpt_engine *e) {
     if (T.stop) (void)hipEventDestroy(T.stop);
   }
   for (auto ev : e->walk_events) (void)hipEventDestroy(ev);
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return RWPT_OK;
@@ -506,11 +507,11 @@ int rwpt_advance(rwpt_engine *e, double t_stop, int64_t *n_active,
   err = hipMemcpyAsync(h, e->counters, sizeof h, hipMemcpyDeviceToHost, e->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return hip_fail(e, err, "advance");
-  for (int64_t k = 0; k < e->last_launches; ++k) {
+  for (int64_t k = 0; k < e->last_timed; ++k) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e->walk_events[2 * k], e->walk_events[2 * k + 1]);
     e->timers[0].total_ms += ms;
-    e->timers[0].launches += 1;
+    e->timers[0].launches += e->walk_event_launches[k];
   }
   e->t_engine = t_stop;
   if (n_active) *n_active = (int64_t)h[0];
@@ -530,6 +531,18 @@ int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
   e->tune.budget = budget;
   e->tune.block = block;
   e->tune.sort = sort ? 1 : 0;
+  return RWPT_OK;
+}
+
+int rwpt_set_graph(rwpt_engine *e, int32_t launches) {
+  if (!e) return RWPT_E_ARG;
+  if (launches < 0 || launches > 256 || (launches & 1))
+    return fail(e, RWPT_E_ARG, "graph launches must be 0 or an even number up to 256");
+  if (launches != e->graph_launches && e->graph_exec) {
+    (void)hipGraphExecDestroy(e->graph_exec);
+    e->graph_exec = nullptr;
+  }
+  e->graph_launches = launches;
   return RWPT_OK;
 }
 
@@ -615,11 +628,11 @@ int rwpt_autotune(rwpt_engine *e, double t_probe, int32_t *budget, int32_t *bloc
     if (r == hipSuccess)
       r = hipMemcpy(h, e->counters, sizeof h, hipMemcpyDeviceToHost);
     steps += (int64_t)h[1];
-    for (int64_t k = 0; r == hipSuccess && k < e->last_launches; ++k) {
+    for (int64_t k = 0; r == hipSuccess && k < e->last_timed; ++k) {
       float km = 0.f;
       (void)hipEventElapsedTime(&km, e->walk_events[2 * k], e->walk_events[2 * k + 1]);
       e->timers[0].total_ms += km;
-      e->timers[0].launches += 1;
+      e->timers[0].launches += e->walk_event_launches[k];
     }
     return r;
   };

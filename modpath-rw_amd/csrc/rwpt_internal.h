@@ -254,6 +254,15 @@ struct rwpt_engine {
   rwpt::Tuning tune;
   std::vector<hipEvent_t> walk_events;  // start/stop pairs per launch
   int64_t last_launches = 0;
+  // timed event pairs of the last advance and the walk launches each spans
+  // (one per eager launch, a graph's launches per replay)
+  int64_t last_timed = 0;
+  std::vector<int32_t> walk_event_launches;
+  // graph mode (rwpt_set_graph): walk launches per captured graph, its
+  // executable, and the device list length captured launches read
+  int32_t graph_launches = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  const unsigned long long *nwork_dev = nullptr;
   std::string err;
 };
 

@@ -168,7 +168,8 @@ hipError_t walk_launch_usg(rwpt_engine *e, const WalkParams &W, dim3 grid, int b
 #define RWPT_WALK_USG(ND, MO, AD)                                                     \
   case (ND - 1) * 4 + MO * 2 + (AD - 1):                                              \
     hipLaunchKernelGGL((walk_kernel_usg<ND, MO, AD>), grid, dim3(block), 0, e->stream, \
-                       e->prob.cells, G, work, cnt, S, W, budget, nk, nv, e->counters); \
+                       e->prob.cells, G, work, cnt, S, W, budget, nk, nv, e->counters,  \
+                       e->nwork_dev);                                                  \
     break;
   switch (mode) {
     RWPT_WALK_USG(1, 0, 1) RWPT_WALK_USG(1, 0, 2) RWPT_WALK_USG(1, 1, 1)
@@ -210,10 +211,114 @@ WalkParams walk_params(const rwpt_engine *e, double t_stop) {
   return W;
 }
 
+// Graph mode: the list written by the last launch becomes the list the
+// next one reads (counters[2] -> counters[3]), by a kernel, so that a
+// captured graph holds no memset node.
+__global__ void rotate_list(unsigned long long *counters) {
+  counters[3] = counters[2];
+  counters[2] = 0;
+}
+
+// Records the start or end of a timed span of walk launches.
+hipError_t timed_event(rwpt_engine *e, int64_t pair, int end, int32_t launches) {
+  if ((size_t)(2 * pair + 2) > e->walk_events.size()) {
+    hipEvent_t ev[2];
+    hipError_t err;
+    if ((err = hipEventCreate(&ev[0])) != hipSuccess) return err;
+    if ((err = hipEventCreate(&ev[1])) != hipSuccess) return err;
+    e->walk_events.push_back(ev[0]);
+    e->walk_events.push_back(ev[1]);
+  }
+  if ((size_t)(pair + 1) > e->walk_event_launches.size())
+    e->walk_event_launches.resize(pair + 1);
+  e->walk_event_launches[pair] = launches;
+  return hipEventRecord(e->walk_events[2 * pair + end], e->stream);
+}
+
+// The walk of one interval as replays of a captured graph of K launches
+// (K even): launch k reads list k % 2 with the length in counters[3] and
+// writes list (k + 1) % 2, its grid covering every particle.  The lists are
+// not sorted between launches (the results do not depend on it); the host
+// reads the list length once per replay instead of once per launch.  The
+// graph is captured for every advance (the kernels take t_stop by value)
+// and updated into the engine's executable when its shape allows.
+hipError_t advance_graph(rwpt_engine *e, const WalkParams &W, unsigned long long cnt,
+                         int64_t max_launches) {
+  Particles &P = e->part;
+  const int K = e->graph_launches;
+  const int budget = std::max(1, e->tune.budget);
+  const int block = e->tune.block;
+  const bool usg = e->usg != nullptr;
+  const bool fp32 = e->opt.precision != 0;
+  const dim3 grid((unsigned)((P.n + block - 1) / block));
+  hipStream_t st = e->stream;
+  hipError_t err = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) return err;
+  e->nwork_dev = e->counters + 3;
+  for (int k = 0; k < K && err == hipSuccess; ++k) {
+    hipLaunchKernelGGL(rotate_list, dim3(1), dim3(1), 0, st, e->counters);
+    const int a = k & 1, b = a ^ 1;
+    const int32_t *work = P.vals[a];
+    err = usg    ? walk_launch_usg(e, W, grid, block, work, P.n, budget, P.keys[b], P.vals[b])
+          : fp32 ? walk_launch32(e, W, grid, block, work, P.n, budget, P.keys[b], P.vals[b])
+                 : walk_launch64(e, W, grid, block, work, P.n, budget, P.keys[b], P.vals[b]);
+  }
+  e->nwork_dev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipError_t end = hipStreamEndCapture(st, &graph);
+  if (err == hipSuccess) err = end;
+  if (err != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return err;
+  }
+  if (e->graph_exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(e->graph_exec, graph, &bad, &res) != hipSuccess ||
+        res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(e->graph_exec);
+      e->graph_exec = nullptr;
+    }
+  }
+  if (!e->graph_exec) err = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (err != hipSuccess) return err;
+  int64_t launches = 0, timed = 0;
+  while (cnt > 0) {
+    if (launches >= max_launches) {  // step guard reached
+      const int32_t *work = P.vals[0];
+      if (usg) {
+        hipLaunchKernelGGL(terminate_walk_usg, dim3((unsigned)((cnt + 255) / 256)),
+                           dim3(256), 0, st, usg_walk_view(e), work, (int64_t)cnt,
+                           particle_ptrs<double>(P));
+        err = hipGetLastError();
+      } else {
+        err = fp32 ? walk_terminate32(e, work, (int64_t)cnt)
+                   : launch_terminate_typed<double>(e, work, (int64_t)cnt);
+      }
+      if (err != hipSuccess) return err;
+      break;
+    }
+    if ((err = timed_event(e, timed, 0, K)) != hipSuccess) return err;
+    if ((err = hipGraphLaunch(e->graph_exec, st)) != hipSuccess) return err;
+    if ((err = timed_event(e, timed, 1, K)) != hipSuccess) return err;
+    timed += 1;
+    launches += K;
+    err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return err;
+  }
+  e->last_launches = launches;
+  e->last_timed = timed;
+  return hipSuccess;
+}
+
 hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   Particles &P = e->part;
   const WalkParams W = walk_params(e, t_stop);
   e->last_launches = 0;
+  e->last_timed = 0;
   hipError_t err = hipMemsetAsync(e->counters, 0, 4 * sizeof(unsigned long long),
                                   e->stream);
   if (err != hipSuccess || P.n == 0) return err;
@@ -228,6 +333,7 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   const int budget = std::max(1, e->tune.budget);
   const int block = e->tune.block;
   const int64_t max_launches = (kMaxSteps + budget - 1) / budget;
+  if (e->graph_launches > 0) return advance_graph(e, W, cnt, max_launches);
   const bool usg = e->usg != nullptr;
   const int bits = radix_bits(usg ? std::max<int64_t>(e->prob.ncell, e->usg->nunit)
                                   : e->prob.ncell);
@@ -261,20 +367,13 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
     int32_t *nv = e->tune.sort ? P.vals[0] : P.vals[1];
     err = hipMemsetAsync(e->counters + 2, 0, sizeof(unsigned long long), e->stream);
     if (err != hipSuccess) return err;
-    if ((size_t)(2 * launches + 2) > e->walk_events.size()) {
-      hipEvent_t ev[2];
-      if ((err = hipEventCreate(&ev[0])) != hipSuccess) return err;
-      if ((err = hipEventCreate(&ev[1])) != hipSuccess) return err;
-      e->walk_events.push_back(ev[0]);
-      e->walk_events.push_back(ev[1]);
-    }
-    (void)hipEventRecord(e->walk_events[2 * launches], e->stream);
+    if ((err = timed_event(e, launches, 0, 1)) != hipSuccess) return err;
     const dim3 grid((unsigned)((cnt + block - 1) / block));
     err = usg    ? walk_launch_usg(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv)
           : fp32 ? walk_launch32(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv)
                  : walk_launch64(e, W, grid, block, work, (int64_t)cnt, budget, nk, nv);
     if (err != hipSuccess) return err;
-    (void)hipEventRecord(e->walk_events[2 * launches + 1], e->stream);
+    if ((err = timed_event(e, launches, 1, 1)) != hipSuccess) return err;
     launches += 1;
     err = hipMemcpyAsync(&cnt, e->counters + 2, sizeof cnt, hipMemcpyDeviceToHost,
                          e->stream);
@@ -286,6 +385,7 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
     }
   }
   e->last_launches = launches;
+  e->last_timed = launches;
   return hipSuccess;
 }
 

@@ -971,14 +971,20 @@ __global__ void prepare_advance(const R *__restrict__ cells,
 
 // One launch of the walk: every listed particle runs up to `budget`
 // time-loop iterations; particles not done are appended to the next list.
+// A launch captured in a graph reads the list length from nwork_dev (its
+// grid covers every particle; the blocks beyond the list exit at once).
 template <typename R, int NRM, int P2, int NDIM, int MODEL, int ADV, int MINW>
 __global__ void __launch_bounds__(256, MINW)
     walk_kernel(const R *__restrict__ cells,
                 const int32_t *__restrict__ work, int64_t nwork,
                 ParticlePtrs<R> S, const WalkParams W, int budget,
                 int32_t *__restrict__ nkeys, int32_t *__restrict__ nvals,
-                unsigned long long *__restrict__ counters) {
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+                unsigned long long *__restrict__ counters,
+                const unsigned long long *__restrict__ nwork_dev) {
+  if (nwork_dev) nwork = (int64_t)*nwork_dev;
+  const int64_t nblk = (nwork + blockDim.x - 1) / blockDim.x;
+  if ((int64_t)blockIdx.x >= nblk) return;
+  const int64_t blk = xcd_block(blockIdx.x, nblk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   bool more = false;
   int c = 0, p = 0;
@@ -1141,18 +1147,19 @@ void launch_walk_variant(rwpt_engine *e, const WalkParams &W, dim3 grid, int blo
     if (mw == 3) {
       hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 3>), grid, dim3(block), 0,
                          e->stream, cells, work, cnt, S, W, budget, nk, nv,
-                         e->counters);
+                         e->counters, e->nwork_dev);
       return;
     }
     if (mw == 4) {
       hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 4>), grid, dim3(block), 0,
                          e->stream, cells, work, cnt, S, W, budget, nk, nv,
-                         e->counters);
+                         e->counters, e->nwork_dev);
       return;
     }
   }
   hipLaunchKernelGGL((walk_kernel<R, NRM, P2, ND, MO, AD, 1>), grid, dim3(block), 0,
-                     e->stream, cells, work, cnt, S, W, budget, nk, nv, e->counters);
+                     e->stream, cells, work, cnt, S, W, budget, nk, nv, e->counters,
+                     e->nwork_dev);
 }
 
 // The walk variant for the engine's dimensions, dispersion model and

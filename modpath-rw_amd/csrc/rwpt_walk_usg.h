@@ -82,8 +82,13 @@ __global__ void __launch_bounds__(256)
                     const int32_t *__restrict__ work, int64_t nwork,
                     ParticlePtrs<double> S, const WalkParams W, int budget,
                     int32_t *__restrict__ nkeys, int32_t *__restrict__ nvals,
-                    unsigned long long *__restrict__ counters) {
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+                    unsigned long long *__restrict__ counters,
+                    const unsigned long long *__restrict__ nwork_dev) {
+  // a launch captured in a graph reads the list length from the device
+  if (nwork_dev) nwork = (int64_t)*nwork_dev;
+  const int64_t nblk = (nwork + blockDim.x - 1) / blockDim.x;
+  if ((int64_t)blockIdx.x >= nblk) return;
+  const int64_t blk = xcd_block(blockIdx.x, nblk);
   const int64_t i = blk * blockDim.x + threadIdx.x;
   bool more = false;
   int c = 0, p = 0, u = 0;
