@@ -418,16 +418,20 @@ def adjoint_run(n=1_000_000, seeds=8, rel_steps=(1e-4, 1e-2)):
 
 
 def timed_run(P, rel, precision, steps, warmup, tuning_names, kde=True,
-              adaptive_loops=0):
+              adaptive_loops=0, graph=0, graph_max_list=0):
     """`warmup` intervals, then `steps` timed intervals from the release,
     each one rwpt_advance by 50 RW steps followed (kde) by one gpkde density
     pass of the active particles on the flow grid.  After RUN_INTERVALS
     intervals the run starts again from the release.  With adaptive_loops
     > 0 the gpkde optimisation is measured on the plume after the first
-    interval before the timed region."""
+    interval before the timed region.  graph > 0 runs the walk as replays of
+    captured graphs of that many launches (rwpt_set_graph) once at most
+    graph_max_list particles are unfinished."""
     from rwpt import Engine
     E = Engine(P, precision=precision)
     tune = apply_tuning(E, tuning_names[precision], TUNED_NAME_1[precision])
+    if graph:
+        E.set_graph(graph, graph_max_list)
     nbins = (P.ncol, P.nrow, P.nlay)
     origin = (0.0, 0.0, float(P.bot.min()))
 
@@ -548,6 +552,26 @@ def precision_errors(P, rel, precision, report_steps=(50, 100, 200, 400)):
                    "1e7), the Monte Carlo floor of each difference; concentrations: one "
                    "gpkde density pass at 64 observation cells (8 x 8, middle layer), "
                    "relative L1 over the 64 cells while they hold >= 100 particles"}
+
+
+def graph_run(P, rel, prec, steps, warmup, small=200_000, launches=8, max_list=200_000):
+    """The walk with graph replays (rwpt_set_graph) against eager launches,
+    walk only: the headline workload with the tail of each interval (at most
+    max_list unfinished particles) replayed as graphs of `launches`
+    launches, and the first `small` particles of it (a launch-bound run)
+    replayed from the first launch."""
+    out = {"launches_per_graph": launches}
+    for key, sub, ml in (("config2_tail", None, max_list), ("small_full", small, 0)):
+        r_rel = rel if sub is None else tuple(a[:sub] for a in rel)
+        e = timed_run(P, r_rel, prec, steps, warmup, TUNED_NAME, kde=False)
+        g = timed_run(P, r_rel, prec, steps, warmup, TUNED_NAME, kde=False, graph=launches,
+                      graph_max_list=ml)
+        assert g["steps"] == e["steps"]
+        out[key] = {"particles": int(r_rel[0].size), "max_list": ml,
+                    "eager_ms_per_step": e["elapsed"] * 1e3 / steps,
+                    "graph_ms_per_step": g["elapsed"] * 1e3 / steps,
+                    "speedup": e["elapsed"] / g["elapsed"]}
+    return out
 
 
 def config1_run(steps=10):
@@ -741,6 +765,8 @@ def main(argv=None):
         out["speedup_vs_fp64"] = r["rate"] / r0["rate"]
         out["error_vs_fp64"] = errors
     if not args.quick:
+        print("[bench] graph replay leg", file=sys.stderr, flush=True)
+        out["graph"] = graph_run(P, rel, prec, args.steps, args.warmup)
         print("[bench] config1 leg", file=sys.stderr, flush=True)
         out["config1"] = config1_run()
     if args.ensemble_intervals > 0:

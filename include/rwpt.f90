@@ -119,10 +119,12 @@ module RwptGpuModule
       real(c_double), intent(in) :: head(*), source_flow(*), sink_flow(*)
       integer(c_int32_t), intent(in) :: layer_type(*), zone(*)
     end function
-    integer(c_int) function rwpt_set_graph(e, launches) bind(C, name='rwpt_set_graph')
-      import :: c_int, c_ptr, c_int32_t
+    integer(c_int) function rwpt_set_graph(e, launches, max_list) &
+        bind(C, name='rwpt_set_graph')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: e
       integer(c_int32_t), value :: launches
+      integer(c_int64_t), value :: max_list
     end function
     ! saturated_top: c_loc(Grid%SaturatedTop), or c_null_ptr for the cell tops
     integer(c_int) function rwpt_set_saturated_top(e, saturated_top) &

@@ -287,14 +287,16 @@ int rwpt_set_steady_state(rwpt_engine *e, int32_t steady_state);
 int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
                     int32_t sort, int32_t min_waves);
 /* Graph mode of rwpt_advance (no counterpart in the reference): with
- * launches = K > 0 (even, at most 256) the walk of an interval runs as
- * replays of a HIP graph of K walk launches captured on the engine stream,
- * each launch reading its work-list length from device memory, so that the
- * host waits once per K launches instead of once per launch; the work
- * lists are not sorted by cell between launches.  Results are bit-identical
- * to the eager walk (they depend on no launch shape).  0 (the default)
- * turns it off. */
-int rwpt_set_graph(rwpt_engine *e, int32_t launches);
+ * launches = K > 0 (even, at most 256) the walk of an interval continues,
+ * once its list of unfinished particles holds at most max_list of them
+ * (max_list <= 0: from the first launch), as replays of a HIP graph of K
+ * walk launches captured on the engine stream, each launch reading its
+ * work-list length from device memory, so that the host waits once per K
+ * launches instead of once per launch; those lists are not sorted by cell.
+ * The launches before stay eager and cell-sorted (large lists walk faster
+ * sorted).  Results are bit-identical to the eager walk (they depend on no
+ * launch shape).  launches 0 (the default) turns it off. */
+int rwpt_set_graph(rwpt_engine *e, int32_t launches, int64_t max_list);
 /* Launch-shape autotuner (no counterpart in the reference): times trial
  * advances from the current state to t_probe over workgroup size and
  * occupancy variant, per-launch iteration budget and work-list sort (a

@@ -534,7 +534,7 @@ int rwpt_set_tuning(rwpt_engine *e, int32_t budget, int32_t block,
   return RWPT_OK;
 }
 
-int rwpt_set_graph(rwpt_engine *e, int32_t launches) {
+int rwpt_set_graph(rwpt_engine *e, int32_t launches, int64_t max_list) {
   if (!e) return RWPT_E_ARG;
   if (launches < 0 || launches > 256 || (launches & 1))
     return fail(e, RWPT_E_ARG, "graph launches must be 0 or an even number up to 256");
@@ -543,6 +543,7 @@ int rwpt_set_graph(rwpt_engine *e, int32_t launches) {
     e->graph_exec = nullptr;
   }
   e->graph_launches = launches;
+  e->graph_max_list = max_list;
   return RWPT_OK;
 }
 

@@ -261,6 +261,7 @@ struct rwpt_engine {
   // graph mode (rwpt_set_graph): walk launches per captured graph, its
   // executable, and the device list length captured launches read
   int32_t graph_launches = 0;
+  int64_t graph_max_list = 0;  // replays take over at this list length (<= 0: at once)
   hipGraphExec_t graph_exec = nullptr;
   const unsigned long long *nwork_dev = nullptr;
   std::string err;

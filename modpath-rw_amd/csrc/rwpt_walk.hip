@@ -235,15 +235,16 @@ hipError_t timed_event(rwpt_engine *e, int64_t pair, int end, int32_t launches) 
   return hipEventRecord(e->walk_events[2 * pair + end], e->stream);
 }
 
-// The walk of one interval as replays of a captured graph of K launches
-// (K even): launch k reads list k % 2 with the length in counters[3] and
+// The rest of an interval's walk as replays of a captured graph of K
+// launches (K even), from the list in keys[0]/vals[0] whose length is in
+// counters[2]: launch k reads list k % 2 with the length in counters[3] and
 // writes list (k + 1) % 2, its grid covering every particle.  The lists are
 // not sorted between launches (the results do not depend on it); the host
 // reads the list length once per replay instead of once per launch.  The
 // graph is captured for every advance (the kernels take t_stop by value)
 // and updated into the engine's executable when its shape allows.
 hipError_t advance_graph(rwpt_engine *e, const WalkParams &W, unsigned long long cnt,
-                         int64_t max_launches) {
+                         int64_t max_launches, int64_t launches, int64_t timed) {
   Particles &P = e->part;
   const int K = e->graph_launches;
   const int budget = std::max(1, e->tune.budget);
@@ -284,7 +285,6 @@ hipError_t advance_graph(rwpt_engine *e, const WalkParams &W, unsigned long long
   if (!e->graph_exec) err = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (err != hipSuccess) return err;
-  int64_t launches = 0, timed = 0;
   while (cnt > 0) {
     if (launches >= max_launches) {  // step guard reached
       const int32_t *work = P.vals[0];
@@ -333,12 +333,15 @@ hipError_t launch_advance(rwpt_engine *e, double t_stop) {
   const int budget = std::max(1, e->tune.budget);
   const int block = e->tune.block;
   const int64_t max_launches = (kMaxSteps + budget - 1) / budget;
-  if (e->graph_launches > 0) return advance_graph(e, W, cnt, max_launches);
   const bool usg = e->usg != nullptr;
   const int bits = radix_bits(usg ? std::max<int64_t>(e->prob.ncell, e->usg->nunit)
                                   : e->prob.ncell);
   int64_t launches = 0;
   while (cnt > 0) {
+    // graph mode: the rest of the walk once the list is short enough
+    if (e->graph_launches > 0 && (e->graph_max_list <= 0 ||
+                                  cnt <= (unsigned long long)e->graph_max_list))
+      return advance_graph(e, W, cnt, max_launches, launches, launches);
     const int32_t *work = P.vals[0];
     if (e->tune.sort) {
       size_t bytes = P.sort_tmp_bytes;

@@ -89,7 +89,7 @@ EXPORTS = {
                        C.c_int32], C.c_int),
     "rwpt_set_cell_extras": ([C.c_void_p, _dp, _i32p, _i32p, _dp, _dp], C.c_int),
     "rwpt_set_saturated_top": ([C.c_void_p, _dp], C.c_int),
-    "rwpt_set_graph": ([C.c_void_p, C.c_int32], C.c_int),
+    "rwpt_set_graph": ([C.c_void_p, C.c_int32, C.c_int64], C.c_int),
     "rwpt_set_dispersion": ([C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp,
                              _dp, _dp], C.c_int),
     "rwpt_load_particles": ([C.c_void_p, C.c_int64, _i32p, _dp, _dp, _dp, _dp,
@@ -310,10 +310,12 @@ class Engine:
             self._check(rc, "rwpt_set_flow")
         self.set_saturated_top(getattr(P, "saturated_top", None))
 
-    def set_graph(self, launches: int):
-        """rwpt_set_graph: run each interval's walk as replays of a captured
-        graph of `launches` walk launches (0: eager launches)."""
-        self._check(lib().rwpt_set_graph(self.h, int(launches)), "rwpt_set_graph")
+    def set_graph(self, launches: int, max_list: int = 0):
+        """rwpt_set_graph: continue each interval's walk, once at most
+        max_list particles are unfinished (0: from the start), as replays of
+        a captured graph of `launches` walk launches (0: eager launches)."""
+        self._check(lib().rwpt_set_graph(self.h, int(launches), int(max_list)),
+                    "rwpt_set_graph")
 
     def set_saturated_top(self, saturated_top=None):
         """rwpt_set_saturated_top: Grid%SaturatedTop of the time step (one

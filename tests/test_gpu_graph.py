@@ -18,12 +18,12 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("cell", "status", "lx", "ly", "lz", "t", "substeps", "rng")
 
 
-def _advance_all(P, rel, times, graph, precision=0, kde=None, budget=16):
+def _advance_all(P, rel, times, graph, precision=0, kde=None, budget=16, max_list=0):
     E = Engine(P, precision=precision)
     out, launches = [], []
     try:
         E.set_tuning(budget=budget, block=256, sort=True)
-        E.set_graph(graph)
+        E.set_graph(graph, max_list)
         E.load_particles(*rel)
         for t in times:
             E.advance(t)
@@ -37,8 +37,8 @@ def _advance_all(P, rel, times, graph, precision=0, kde=None, budget=16):
     return out, launches, nl
 
 
-@pytest.mark.parametrize("graph", [2, 8])
-def test_graph_equals_eager_and_oracle(graph):
+@pytest.mark.parametrize("graph,max_list", [(2, 0), (8, 0), (4, 300)])
+def test_graph_equals_eager_and_oracle(graph, max_list):
     P, where, dt_out, nout = SCEN["hetero_ibound_rebound_R"]
     rel = _release(P, where, 1500)
     times = [k * dt_out for k in range(1, 4)]
@@ -47,7 +47,7 @@ def test_graph_equals_eager_and_oracle(graph):
     print("graph: eager", flush=True)
     ref, l0, _ = _advance_all(P, rel, times, 0, kde=kde)
     print("graph: replays", graph, flush=True)
-    got, l1, nl = _advance_all(P, rel, times, graph, kde=kde)
+    got, l1, nl = _advance_all(P, rel, times, graph, kde=kde, max_list=max_list)
     st = O.new_state(*rel)
     OP = O.OracleProblem(P)
     t0 = 0.0
@@ -57,10 +57,14 @@ def test_graph_equals_eager_and_oracle(graph):
         for f in FIELDS:
             assert np.array_equal(a[f], b[f]), (k, f)
             assert np.array_equal(b[f], st[f]), (k, f)
+    assert nl == sum(l1)
+    if max_list:
+        # eager cell-sorted launches first, then whole graphs
+        assert all(n > graph for n in l1)
+        return
     # whole graphs of K launches (with K = 2 several replays an interval)
     assert all(n % graph == 0 and n >= graph for n in l1)
     assert graph > 2 or max(l1) > graph
-    assert nl == sum(l1)
 
 
 def test_graph_fp32_and_ensemble():
