@@ -46,7 +46,7 @@ double orc_uniform(uint64_t s[2]) {
   uint64_t x = orc_xoroshiro_next(s);
   uint64_t b = 0x3FF0000000000000ull | (x >> 12);
   double d;
-  memcpy(&d, &b, sizeof d);
+  memcpy(&d, &b, sizeof b);  /* the value's 8 bytes (a dual build's value) */
   return d - 1.0;
 }
 
@@ -1892,4 +1892,6 @@ int orc_kde_density_sel(long n, const double *x, const double *y,
 /* ------------------------------------------------------------------------ */
 /* Unstructured grids (shares the static sub-cell walk and corner formulas) */
 /* ------------------------------------------------------------------------ */
+#ifndef ORC_DUAL_BUILD
 #include "rwpt_oracle_usg.c"
+#endif

@@ -391,3 +391,60 @@ def test_tangent_restart_is_the_window_local_sensitivity():
     assert checked >= len(params) - 1
     every = _gradient(P, zones, aL, aT, dm, target, {"tangent_windows": len(WINDOWS)})
     _same_gradient(_gradient(P, zones, aL, aT, dm, target), every)
+
+
+def _dual_misfit(Q, rel, obs, tgt, bw, tangents):
+    """The misfit and its derivatives along the tangent directions from the
+    dual-number oracle: final global positions with tangents, then the
+    Gaussian-smoothed concentrations of rwpt_adjoint differentiated by hand
+    (d N(z)/dz = -z N(z))."""
+    from oracle import oracle_dual as D
+    DP = D.DualProblem(Q, tangents)
+    st = D.new_state(*rel)
+    t0 = 0.0
+    for t1 in WINDOWS:
+        assert D.advance(DP, st, t0, t1) == 0
+        t0 = t1
+    act = st["status"] == 1
+    g = D.global_xyz(DP, st["cell"][act], st["lx"][act], st["ly"][act], st["lz"][act])
+    X = np.stack([a[:, 0] for a in g], axis=1)                  # (n, 3)
+    dX = np.stack([a[:, 1:] for a in g], axis=1)                # (n, 3, ND)
+    h = np.asarray(bw, float)
+    J, dJ = 0.0, np.zeros(D.ND)
+    for o in range(len(obs)):
+        z = (X - obs[o]) / h
+        K = np.prod(np.exp(-0.5 * z * z) / (np.sqrt(2.0 * np.pi) * h), axis=1)
+        dK = K[:, None] * np.einsum("nd,ndk->nk", -z / h, dX)
+        c, dc = K.sum(), dK.sum(axis=0)
+        J += (c - tgt[o]) ** 2
+        dJ += 2.0 * (c - tgt[o]) * dc
+    return J, dJ
+
+
+def test_gradient_equals_forward_mode_duals():
+    """The FP64 reference gradient: forward-mode dual numbers through the
+    oracle's cell records and walk (oracle/rwpt_oracle_dual.cpp, pinned
+    against central differences of the oracle by test_oracle_dual.py) give
+    the misfit's derivatives along random per-cell directions of aL, aT and
+    Dm and along the porosity scale.  The device's reverse sweep must give
+    the same numbers as the dot products of its per-cell gradients with the
+    directions (its drift-term tangents are FP32: 1e-6 relative)."""
+    P, zones, aL, aT, dm = base()
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    obs = observations(P)
+    bw = (1.5, 1.5, 1.0)
+    first = run(P, zones, aL, aT, dm)
+    tgt = 0.5 * first["conc"]
+    out = run(P, zones, aL, aT, dm, gradient=True, target=tgt)
+    g = np.random.default_rng(3)
+    v = [g.uniform(-1.0, 1.0, P.ncell) for _ in range(3)]
+    tang = {k: np.zeros((P.ncell, 4)) for k in ("aL", "aT", "dmeff", "porosity")}
+    tang["aL"][:, 0], tang["aT"][:, 1], tang["dmeff"][:, 2] = v
+    tang["porosity"][:, 3] = Q.porosity
+    J, dJ = _dual_misfit(Q, release(P), obs, tgt, bw, tang)
+    assert J == pytest.approx(out["misfit"], rel=1e-12)
+    want = np.array([np.dot(out["grad_al"], v[0]), np.dot(out["grad_at"], v[1]),
+                     np.dot(out["grad_dm"], v[2]), out["grad_porosity_scale"]])
+    print("adjoint vs duals:", want, dJ, flush=True)
+    np.testing.assert_allclose(want, dJ, rtol=1e-6, atol=0)
+    assert np.all(np.abs(dJ) > 0)
