@@ -393,7 +393,7 @@ def test_tangent_restart_is_the_window_local_sensitivity():
     _same_gradient(_gradient(P, zones, aL, aT, dm, target), every)
 
 
-def _dual_misfit(Q, rel, obs, tgt, bw, tangents):
+def _dual_misfit(Q, rel, obs, tgt, bw, tangents, windows=WINDOWS):
     """The misfit and its derivatives along the tangent directions from the
     dual-number oracle: final global positions with tangents, then the
     Gaussian-smoothed concentrations of rwpt_adjoint differentiated by hand
@@ -402,7 +402,7 @@ def _dual_misfit(Q, rel, obs, tgt, bw, tangents):
     DP = D.DualProblem(Q, tangents)
     st = D.new_state(*rel)
     t0 = 0.0
-    for t1 in WINDOWS:
+    for t1 in windows:
         assert D.advance(DP, st, t0, t1) == 0
         t0 = t1
     act = st["status"] == 1
@@ -448,3 +448,37 @@ def test_gradient_equals_forward_mode_duals():
     print("adjoint vs duals:", want, dJ, flush=True)
     np.testing.assert_allclose(want, dJ, rtol=1e-6, atol=0)
     assert np.all(np.abs(dJ) > 0)
+
+
+def test_long_horizon_gradient_equals_duals():
+    """Over 30 days (60 RW steps, 6 checkpoint windows, many cell exits per
+    particle) the reverse sweep across window boundaries still gives the
+    forward-mode derivatives."""
+    P, zones, aL, aT, dm = base()
+    Q = problems.zoned_dispersion(P, zones, aL, aT, dm)
+    windows = (5.0, 10.0, 15.0, 20.0, 25.0, 30.0)
+    obs = problems.observation_points(P, 6, 3, (8, 22), (7, 14))
+    bw = (1.5, 1.5, 1.0)
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        tgt = 0.5 * E.adjoint(windows, obs, np.zeros(len(obs)), bw, gradient=False)["conc"]
+    finally:
+        E.close()
+    E = Engine(Q)
+    try:
+        E.load_particles(*release(P))
+        out = E.adjoint(windows, obs, tgt, bw, gradient=True)
+    finally:
+        E.close()
+    g = np.random.default_rng(4)
+    v = [g.uniform(-1.0, 1.0, P.ncell) for _ in range(3)]
+    tang = {k: np.zeros((P.ncell, 4)) for k in ("aL", "aT", "dmeff", "porosity")}
+    tang["aL"][:, 0], tang["aT"][:, 1], tang["dmeff"][:, 2] = v
+    tang["porosity"][:, 3] = Q.porosity
+    J, dJ = _dual_misfit(Q, release(P), obs, tgt, bw, tang, windows=windows)
+    assert J == pytest.approx(out["misfit"], rel=1e-12)
+    want = np.array([np.dot(out["grad_al"], v[0]), np.dot(out["grad_at"], v[1]),
+                     np.dot(out["grad_dm"], v[2]), out["grad_porosity_scale"]])
+    print("long horizon, adjoint vs duals:", want, dJ, flush=True)
+    np.testing.assert_allclose(want, dJ, rtol=1e-5, atol=0)
