@@ -431,8 +431,11 @@ int rwpt_write_endpoints(rwpt_engine *e, const char *path, int32_t format,
  * DX DY and the saturated thickness; 0 when dry) and the model x, y, global
  * z, or the flow rate; format 1 the same fields as a stream of int32 and
  * float64.  id, group (1-based) and mass (NULL: 1) per particle,
- * group_solute per group; select (may be NULL) limits the particles, e.g.
- * to those stopped in the last advance.  append 0 replaces the file. */
+ * group_solute per group; select (may be NULL): style 1 the particles
+ * written instead of those with status 1 (MPathRW's TimeseriesOutputOption
+ * 1 writes every particle in the cells), style 2 a limit on the stopped
+ * particles, e.g. to those stopped in the last advance.  append 0 replaces
+ * the file. */
 int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t append,
                            int32_t style, int32_t format, int32_t n_cells,
                            const int32_t *cells, int32_t time_point, int32_t time_step,

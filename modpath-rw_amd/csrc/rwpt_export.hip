@@ -509,8 +509,13 @@ extern "C" int rwpt_write_observation(rwpt_engine *e, const char *path, int32_t 
   for (size_t p = 0; p < H.cell.size(); ++p) {
     const int32_t c = H.cell[p];
     if (c < 1 || c > P.ncell || obs_index[c - 1] < 0) continue;
-    if (H.status[p] != (style == 1 ? 1 : stop_status)) continue;
-    if (select && !select[p]) continue;
+    if (style == 1 && select) {
+      // the records MPathRW writes at a time point: the particles chosen
+      if (!select[p]) continue;
+    } else {
+      if (H.status[p] != (style == 1 ? 1 : stop_status)) continue;
+      if (select && !select[p]) continue;
+    }
     const int k = obs_index[c - 1];
     const double m = mass ? mass[p] : 1.0;
     const int32_t sol = group_solute[group[p] - 1];

@@ -281,14 +281,25 @@ class Driver:
             st = self.E.particles()["status"]
             pending = int((st == 0).sum())
             if iv.time_point is not None:
+                # a record for every active particle whose tracking in the
+                # interval started before the point: one released at the
+                # point itself does not move and gets none (pr_FindTimeIndex
+                # takes points after the current time,
+                # ParticleTrackingEngine.f90:366-390, 1167-1176); option 1
+                # writes every particle (MPathRW.F90:1492-1530), resident
+                # observation records included
+                if self.timeseries_option == 1:
+                    sel = np.ones(st.size, bool)
+                else:
+                    sel = (st == 1) & (initial["t"] < iv.time)
                 if self.timeseries_path is not None and self.timeseries_option != 2:
-                    sel = np.ones(st.size, bool) if self.timeseries_option == 1 else None
                     self.E.write_timeseries(self.timeseries_path, iv.time_point, iv.ktime,
                                             iv.time, self.sequence, self.ids, self.group,
                                             select=sel)
                 for path, cells in self.observations:
                     self.E.write_observation(path, 1, cells, iv.time_point, iv.ktime, self.ids,
-                                             self.group, self.group_solute, mass=self.mass)
+                                             self.group, self.group_solute, mass=self.mass,
+                                             select=sel)
             if iv.gpkde_point is not None and self.kde_options is not None and active > 0:
                 self._reconstruct(iv.gpkde_point, iv.time, self.kde_options)
             log.append((iv, active, steps))

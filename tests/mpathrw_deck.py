@@ -4,13 +4,17 @@ simulation, RW options, dispersion and gpkde files over the binary grid,
 head and budget files of tests/mf6_synth.py and a TDIS file) and as the
 arguments of rwpt.driver for the same run.  Test infrastructure.
 
-The model: a 14 x 10 x 3 heterogeneous divergence-free flow field
+The DIS model: a 14 x 10 x 3 heterogeneous divergence-free flow field
 (problems.heterogeneous_flow) entering through the west face of column 1
 and leaving through the east face of column 14 (CHD with an IFACE
 auxiliary), a pumping well (a weak sink), a convertible top layer with its
 water table inside the layer, three MODFLOW time steps in two stress
-periods whose flow rates differ.  Dispersion is zero, so every path is
-fixed by the flow and neither run's random draws matter.
+periods whose flow rates differ.  The DISV model (disv_run): a smoothed
+quadtree of 12 x 8 base cells with a refined block and a refined pair in 3
+layers (tests/usg_synth.py), its stream-function flow entering through the
+east face and leaving through the west face, the same well, layer types
+and time steps.  Dispersion is zero, so every path is fixed by the flow and
+neither run's random draws matter.
 """
 from __future__ import annotations
 
@@ -58,6 +62,8 @@ class Run:
     gpkde_points: Optional[Sequence[float]] = (5.0, 10.0, 15.0)  # None: follow
     gpkde_skip_initial: bool = False
     groups: List[Group] = dataclasses.field(default_factory=list)
+    grid: str = "dis"                            # or "disv"
+    refine: tuple = ()                           # refined base cells (DISV)
 
     def problem(self):
         P = problems.heterogeneous_flow(self.ncol, self.nrow, self.nlay, corr_len=3.0,
@@ -77,6 +83,43 @@ class Run:
     @property
     def steps(self):
         return [(p + 1, s + 1) for p, n in enumerate(self.nstp) for s in range(n)]
+
+
+def quadtree(R: Run):
+    from usg_synth import Quadtree
+    return Quadtree(R.nrow, R.ncol, set(R.refine), nlay=R.nlay)
+
+
+def disv_run(**kw) -> Run:
+    """The DISV run: particles enter with the flow near the east edge."""
+    kw.setdefault("nrow", 8)
+    kw.setdefault("ncol", 12)
+    kw.setdefault("refine", ((2, 4), (2, 5), (3, 4), (3, 5), (5, 8), (6, 8)))
+    R = Run(grid="disv", **kw)
+    Q = quadtree(R)
+    g = np.random.default_rng(R.seed + 1)
+    east = np.nonzero(Q.x0 >= (R.ncol - 3) * 1.0)[0]
+    layer_of = lambda p, k: k * Q.ncpl + p + 1
+    n1 = 30
+    cells = np.array([layer_of(int(p), int(k)) for p, k in
+                      zip(g.choice(east, n1), g.integers(0, R.nlay, n1))], np.int32)
+    off = np.zeros(n1)
+    off[-6:] = 2.0
+    R.groups.append(Group("east", [0.0], cells, np.round(g.uniform(0.05, 0.95, n1), 3),
+                          np.round(g.uniform(0.05, 0.95, n1), 3),
+                          np.round(g.uniform(0.05, 0.95, n1), 3), off))
+    mid = np.nonzero((Q.x0 >= 4.0) & (Q.x1 <= 9.0))[0]
+    n2 = 10
+    cells = np.array([layer_of(int(p), int(k)) for p, k in
+                      zip(g.choice(mid, n2), g.integers(0, R.nlay, n2))], np.int32)
+    lx = np.round(g.uniform(0.05, 0.95, n2), 3)
+    lx[0] = 1.0                         # on face 2
+    R.groups.append(Group("twice", [3.0, 9.0], cells, lx, np.full(n2, 0.5),
+                          np.full(n2, 0.25), np.zeros(n2),
+                          ids=np.arange(201, 201 + n2, dtype=np.int32)))
+    R.groups.append(Group("late", [30.0], cells[:2].copy(), np.full(2, 0.5),
+                          np.full(2, 0.5), np.full(2, 0.5), np.zeros(2)))
+    return R
 
 
 def default_run(**kw) -> Run:
@@ -140,28 +183,66 @@ def flows(R: Run, P):
     return icelltype, head, out
 
 
+def disv_flows(R: Run, Q):
+    """flows() for the DISV model: the quadtree's FLOW-JA-FACE scaled per
+    step, CHD with IFACE on the west and east edges, the well."""
+    n, ncpl = Q.ncell, Q.ncpl
+    icelltype = np.zeros(n, np.int32)
+    icelltype[:ncpl] = 1
+    g = np.random.default_rng(R.seed + 2)
+    top = Q.top
+    head = top + 1.0
+    head[:ncpl] = Q.botm[:ncpl] + (top - Q.botm)[:ncpl] * g.uniform(0.55, 0.95, ncpl)
+    fja = Q.flows(seed=R.seed)
+    bnd = Q.boundary_flows()
+    well = (WELL[0] - 1) * ncpl + int(np.nonzero((Q.x0 <= 5.5) & (Q.x1 > 5.5) &
+                                                 (Q.y0 <= 4.5) & (Q.y1 > 4.5))[0][0]) + 1
+    out = []
+    for s in R.rate:
+        recs = [("list", "CHD", "CHD-1", [c for c, _, _ in bnd], [q * s for _, _, q in bnd],
+                 ("IFACE",), [[float(f)] for _, f, _ in bnd]),
+                ("list", "WEL", "WEL-1", [well], [-0.05 * s], (), None)]
+        out.append((s, fja * s, recs))
+    return icelltype, head, out
+
+
 def write(R: Run, tmp: str, name: str = "m") -> dict:
     """Writes the MODFLOW 6 files and MODPATH-RW's deck into tmp; returns
     the paths (the simulation file first)."""
-    P = R.problem()
-    nlay, nrow, ncol = R.nlay, R.nrow, R.ncol
-    ncpl = nrow * ncol
-    icelltype, head, steps = flows(R, P)
+    disv = R.grid == "disv"
     p = {k: os.path.join(tmp, f"{name}.{k}") for k in
          ("grb", "hds", "bud", "tdis", "mpnam", "mpbas", "mpsim", "rwopts", "dsp", "gpkde",
           "imp")}
     out = {k: os.path.join(tmp, f"{name}.{k}") for k in ("mpend", "timeseries", "gpkde.out",
                                                           "mplst")}
-    ia, ja = S.write_grb(p["grb"], nlay, nrow, ncol, P.delr, P.delc, P.top[:ncpl], P.bot,
-                         icelltype)
     totim = R.step_ends
-    S.write_heads(p["hds"], nlay, nrow, ncol, head,
+    nlay = R.nlay
+    if disv:
+        Q = quadtree(R)
+        ncpl = Q.ncpl
+        icelltype, head, steps = disv_flows(R, Q)
+        Q.write_grb(p["grb"], icelltype=icelltype)
+        hrow, hcol = 1, ncpl
+        bot_min, xsize, ysize = float(Q.botm.min()), R.ncol * 1.0, R.nrow * 1.0
+        zsize = float(Q.top.max() - Q.botm.min())
+    else:
+        P = R.problem()
+        nrow, ncol = R.nrow, R.ncol
+        ncpl = nrow * ncol
+        icelltype, head, recs_of = flows(R, P)
+        ia, ja = S.write_grb(p["grb"], nlay, nrow, ncol, P.delr, P.delc, P.top[:ncpl], P.bot,
+                             icelltype)
+        steps = [(s, S.flow_ja(ia, ja, P.frf * s, P.fff * s, P.flf * s, nrow, ncol), recs)
+                 for s, recs in recs_of]
+        hrow, hcol = nrow, ncol
+        bot_min, xsize, ysize = float(P.bot.min()), float(P.delr.sum()), float(P.delc.sum())
+        zsize = float(P.top.max() - P.bot.min())
+    S.write_heads(p["hds"], nlay, hrow, hcol, head,
                   steps=[(kp, ks, t) for (kp, ks), t in zip(R.steps, totim)])
     with open(p["bud"], "wb") as f:
-        for (kp, ks), t, (scale, recs) in zip(R.steps, totim, steps):
-            fja = S.flow_ja(ia, ja, P.frf * scale, P.fff * scale, P.flf * scale, nrow, ncol)
+        for (kp, ks), t, (scale, fja, recs) in zip(R.steps, totim, steps):
             tmpf = p["bud"] + ".part"
-            S.write_budget(tmpf, nlay, nrow, ncol, fja, recs, kper=kp, kstp=ks, totim=t)
+            S.write_budget(tmpf, nlay, hrow, hcol, fja, recs, kper=kp, kstp=ks, totim=t)
             with open(tmpf, "rb") as g:
                 f.write(g.read())
             os.remove(tmpf)
@@ -175,7 +256,8 @@ def write(R: Run, tmp: str, name: str = "m") -> dict:
     b = os.path.basename
     with open(p["mpnam"], "w") as f:
         f.write("# MODPATH-RW name file\n")
-        for key, k in (("MPBAS", "mpbas"), ("GRBDIS", "grb"), ("TDIS", "tdis"),
+        for key, k in (("MPBAS", "mpbas"), ("GRBDISV" if disv else "GRBDIS", "grb"),
+                       ("TDIS", "tdis"),
                        ("HEAD", "hds"), ("BUDGET", "bud"), ("DSP", "dsp"),
                        ("RWOPTS", "rwopts"), ("GPKDE", "gpkde"), ("IMP", "imp")):
             f.write(f"{key:<11}{b(p[k])}\n")
@@ -194,9 +276,8 @@ def write(R: Run, tmp: str, name: str = "m") -> dict:
         f.write("1\nDSP1\n0 0\n0.0\n0.0\n0.0\n")
     with open(p["gpkde"], "w") as f:
         f.write(f"{b(out['gpkde.out'])} 0 0\n")
-        f.write(f"0.0 0.0 {float(P.bot.min())}\n")
-        f.write(f"{float(P.delr.sum())} {float(P.delc.sum())} "
-                f"{float(P.top.max() - P.bot.min())}\n")
+        f.write(f"0.0 0.0 {bot_min}\n")
+        f.write(f"{xsize} {ysize} {zsize}\n")
         f.write("1.0 1.0 1.0\n0\n0\n0 0.02\n0 0 0 2\n0\n0\n0\n")
         skip = 1 if R.gpkde_skip_initial else 0
         if R.gpkde_points is None:
@@ -225,12 +306,12 @@ def write(R: Run, tmp: str, name: str = "m") -> dict:
                 f.write(f"3\n{len(G.release)}\n")
                 f.write(" ".join(repr(float(t)) for t in G.release) + "\n")
             f.write("INTERNAL\n1\n")
-            f.write("1\n" if G.by_layer_row_column else "2\n")
+            f.write("1\n" if G.by_layer_row_column and not disv else "2\n")
             n = len(G.cells)
             f.write(f"{n} {0 if G.ids is None else 1}\n")
             for m in range(n):
                 c = int(G.cells[m])
-                if G.by_layer_row_column:
+                if G.by_layer_row_column and not disv:
                     k0, r = divmod(c - 1, ncpl)
                     loc = f"{k0 + 1} {r // ncol + 1} {r % ncol + 1}"
                 else:

@@ -5,7 +5,8 @@ runs a synthetic MODFLOW 6 model (tests/mpathrw_deck.py: three time steps
 in two stress periods with different flow rates, a convertible top layer,
 a weak sink, three particle groups with offsets, a double release and
 releases after the stop time, gpkde reconstructions at their own times or
-at the timeseries points).  rwpt.driver runs the same run on the device
+at the timeseries points), on a DIS grid and on a DISV smoothed quadtree
+(the unstructured engine, rwpt_create_usg).  rwpt.driver runs the same run on the device
 from rwpt.mf6's reading of the same files: the timeseries and endpoint
 files must be byte-identical to MPathRW's (the schedule with its dropped
 timeseries points, the records at time 0, the release, stop and weak-sink
@@ -32,7 +33,6 @@ pytestmark = [pytest.mark.gpu,
 def _engine_run(R, paths, tmp):
     f = paths["files"]
     g = mf6.read_grb(f["grb"])
-    P = R.problem()
     opts = dict(aL=np.array([0.0]), aT=np.array([0.0]), dmeff=np.array([0.0]),
                 ts_params=(R.dt, 0.0), time_step_kind=4, advection_kind=2,
                 stop_weak_sinks=1, default_icbound=0)
@@ -46,7 +46,7 @@ def _engine_run(R, paths, tmp):
                          gpkde_points=R.gpkde_points or (), gpkde_follows_timeseries=not own,
                          stopping_option=3, reconstruct=True)
     bin_size = (1.0, 1.0, 1.0)
-    origin = (0.0, 0.0, float(P.bot.min()))
+    origin = (0.0, 0.0, float(np.min(g.botm)))
     nbins = (R.ncol, R.nrow, R.nlay)
     scale = 1.0 / np.prod(bin_size)
     kde = dict(origin=origin, bin_size=bin_size, nbins=nbins, mode=1, unit_volume=True,
@@ -79,16 +79,19 @@ def _rows(path):
     return [ln.split() for ln in open(path).read().splitlines() if ln.strip()]
 
 
-@pytest.mark.parametrize("ts_option,own", [(0, True), (1, True), (0, False)],
+@pytest.mark.parametrize("grid,ts_option,own",
+                         [("dis", 0, True), ("dis", 1, True), ("dis", 0, False),
+                          ("disv", 0, True), ("disv", 1, False)],
                          ids=["active_own_gpkde_points", "all_own_gpkde_points",
-                              "active_gpkde_follows_timeseries"])
-def test_driver_files_equal_mpathrw(tmp_path, ts_option, own):
-    R = D.default_run(timeseries_option=ts_option,
-                      gpkde_points=(5.0, 10.0, 15.0) if own else None)
+                              "active_gpkde_follows_timeseries", "disv_active_own_gpkde_points",
+                              "disv_all_gpkde_follows_timeseries"])
+def test_driver_files_equal_mpathrw(tmp_path, grid, ts_option, own):
+    make = D.disv_run if grid == "disv" else D.default_run
+    R = make(timeseries_option=ts_option, gpkde_points=(5.0, 10.0, 15.0) if own else None)
     ref_dir = tmp_path / "ref"
     ref_dir.mkdir()
     paths = D.write(R, str(ref_dir))
-    print("mpathrw: run", ts_option, own, flush=True)
+    print("mpathrw: run", grid, ts_option, own, flush=True)
     refrun.mpathrw(paths["sim"])
     print("mpathrw: engine", flush=True)
     out, log, st = _engine_run(R, paths, tmp_path)

@@ -133,15 +133,36 @@ class Quadtree:
             loops.append(loop)
         return np.array(verts, float), loops
 
+    def psi(self, x, y, q0=0.5, amp=0.15):
+        """The stream function of flows(): a uniform flow towards -x plus a
+        cellular perturbation."""
+        Lx = self.ncol * float(self.del_x.max())
+        Ly = self.nrow * float(self.del_y.max())
+        return -q0 * y + amp * np.sin(2 * np.pi * x / Lx) * np.sin(np.pi * y / Ly) * Ly
+
+    def boundary_flows(self, q0=0.5, amp=0.15):
+        """The flows of flows()' field across the west and east model faces:
+        (cells, face, flow into the cell) of the cells on x = 0 (face 1) and
+        on the east edge (face 2)."""
+        Lx = self.ncol * float(self.del_x.max())
+        thick = self.top - self.botm
+        out = []
+        for n in range(self.ncell):
+            p = n % self.ncpl
+            for x, face, sign in ((0.0, 1, 1.0), (Lx, 2, -1.0)):
+                if (face == 1 and self.x0[p] == 0.0) or (face == 2 and self.x1[p] == Lx):
+                    fx = (self.psi(x, self.y1[p], q0, amp) -
+                          self.psi(x, self.y0[p], q0, amp)) * thick[n]
+                    out.append((n + 1, face, sign * fx))
+        return out
+
     def flows(self, seed=0, q0=0.5, amp=0.15, wscale=0.02):
         """FLOW-JA-FACE of a stream-function horizontal field plus seeded
         vertical flows."""
         rng = np.random.default_rng(seed)
-        Lx = self.ncol * float(self.del_x.max())
-        Ly = self.nrow * float(self.del_y.max())
 
         def psi(x, y):
-            return -q0 * y + amp * np.sin(2 * np.pi * x / Lx) * np.sin(np.pi * y / Ly) * Ly
+            return self.psi(x, y, q0, amp)
 
         thick = self.top - self.botm
         w = rng.normal(0.0, wscale, size=(max(self.nlay - 1, 0), self.ncpl))
