@@ -574,6 +574,52 @@ def graph_run(P, rel, prec, steps, warmup, small=200_000, launches=8, max_list=2
     return out
 
 
+def usg_run(particles=1_000_000, intervals=10, dt=0.5):
+    """The unstructured walk (rwpt_create_usg, FP64) on a smoothed quadtree
+    of 60 x 60 base cells with a refined 10 x 10 block (2 x 2 sub-cells), 10
+    layers, 39,000 cells, stream-function flow, 1e6 particles over the grid,
+    `intervals` intervals of 50 RW steps; beside it the same base grid
+    unrefined through the unstructured engine and through the structured
+    one (the same cells and flows)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from rwpt import Engine, problems
+    from usg_synth import Quadtree
+    from usg_scenarios import release
+    from test_oracle_usg_vs_reference import _structured_twin
+    out = {"workload": "DISV quadtree 60x60 base cells, refined 10x10 block, 10 layers, "
+                       "%d particles, %d intervals of 50 RW steps, FP64" % (particles,
+                                                                             intervals)}
+    for key, refine in (("refined", {(i, j) for i in range(25, 35) for j in range(20, 30)}),
+                        ("unrefined", set())):
+        Q = Quadtree(60, 60, refine, nlay=10)
+        U = problems.UsgProblem(ncell=Q.ncell, nlay=Q.nlay, ia=Q.ia, ja=Q.ja,
+                                flowja=Q.flows(seed=3), porosity=np.full(Q.ncell, 0.3),
+                                aL=np.array([0.3]), aT=np.array([0.03]),
+                                dmeff=np.array([1e-4]), ts_params=(dt, 0.0),
+                                **Q.cell_arrays())
+        rel = release(U, particles, seed=5)
+        probs = [("usg", U)] + ([("structured", _structured_twin(Q, U))] if not refine else [])
+        for name, prob in probs:
+            E = Engine(prob)
+            try:
+                E.load_particles(*rel)
+                E.advance(INTERVAL_STEPS * dt)          # warm-up interval
+                E.reset_kernel_time()
+                steps = 0
+                t0 = time.perf_counter()
+                for k in range(2, intervals + 2):
+                    steps += E.advance(k * INTERVAL_STEPS * dt)[1]
+                el = time.perf_counter() - t0
+                kms, launches = E.kernel_time(0)
+            finally:
+                E.close()
+            out[f"{key}_{name}"] = {"value": steps / el, "unit": "particle-steps/s",
+                                    "cells": int(Q.ncell), "steps": int(steps),
+                                    "ms_per_interval": el * 1e3 / intervals,
+                                    "walk_kernel_avg_launch_ms": kms / max(launches, 1)}
+    return out
+
+
 def config1_run(steps=10):
     """The config-1 workload (1e6 particles, 10 intervals): walk rates of
     the FP32-local and the FP64 walk."""
@@ -769,6 +815,8 @@ def main(argv=None):
         out["graph"] = graph_run(P, rel, prec, args.steps, args.warmup)
         print("[bench] config1 leg", file=sys.stderr, flush=True)
         out["config1"] = config1_run()
+        print("[bench] unstructured-grid leg", file=sys.stderr, flush=True)
+        out["usg"] = usg_run()
     if args.ensemble_intervals > 0:
         print("[bench] ensemble leg", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_run(args.ensemble_intervals, 1)
