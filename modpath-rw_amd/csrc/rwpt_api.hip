@@ -564,6 +564,13 @@ struct StateCopy {
               {S.t0, 8 * n}};
     if (S.flx)
       for (float *f : {S.flx, S.fly, S.flz, S.fix, S.fiy, S.fiz}) arrays.push_back({f, 4 * n});
+    if (e->usg) {
+      // the unstructured walk's visit state
+      UsgData &U = *e->usg;
+      arrays.push_back({U.unit, 4 * n});
+      arrays.push_back({U.pcell, 4 * n});
+      for (double *d : {U.ex, U.ey, U.ez, U.et, U.px, U.py, U.pz}) arrays.push_back({d, 8 * n});
+    }
     bytes = 0;
     for (auto &a : arrays)
       if (a.first) bytes += a.second;
@@ -599,7 +606,6 @@ int rwpt_autotune(rwpt_engine *e, double t_probe, int32_t *budget, int32_t *bloc
                   int32_t *sort, int32_t *min_waves, double *ms_best,
                   int64_t *trial_substeps) {
   if (!e) return RWPT_E_ARG;
-  if (e->usg) return fail(e, RWPT_E_UNSUPPORTED, "the autotuner needs a structured grid");
   if (!std::isfinite(t_probe) || t_probe == e->t_engine)
     return fail(e, RWPT_E_ARG, "t_probe must differ from the engine time");
   int rc = ensure_tables(e);

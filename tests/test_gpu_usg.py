@@ -207,3 +207,27 @@ def test_usg_from_mf6_disv_files(tmp_path):
             assert ok.all(), (k, f, np.nonzero(~ok)[0][:8])
     # dry cells (IBoundTS 0) stop the particles released in them
     assert (out[-1][1]["status"] == 7).any()
+
+
+def test_usg_autotune_keeps_results():
+    """rwpt_autotune on an unstructured engine: its trial advances restore the
+    particles (the unstructured visit state included), so the advances after
+    tuning equal those of an untuned engine bit for bit."""
+    Q, U = build("checker")
+    cell, lx, ly, lz = release(U, 600, seed=3)
+    res = []
+    for tune in (False, True):
+        E = Engine(U)
+        try:
+            E.load_particles(cell, lx, ly, lz)
+            E.advance(0.3)
+            if tune:
+                best = E.autotune(0.7)
+                assert best["block"] in (64, 128, 256)
+            E.advance(0.7)
+            E.advance(1.5)
+            res.append(E.particles())
+        finally:
+            E.close()
+    for f in FIELDS:
+        assert _same(res[0][f], res[1][f]).all(), f
